@@ -1,0 +1,580 @@
+// CPU oracle of the FasterGS rasterizer hot path. TEST INFRASTRUCTURE ONLY:
+// it is the checker that the gfx950 kernels are compared against and the
+// timed CPU baseline; no product code path may call it.
+//
+// It restates the reference pipeline (FGS = gs_submodules/diff-gaussian-
+// rasterization_faster in the FasterGS repository) on the CPU:
+//   preprocess + exact tile test + key emission ... FGS/csrc/cuda_rasterizer/preprocess.cu:414-746
+//   stable sort of (tile<<32 | depth bits)       ... FGS/csrc/cuda_rasterizer/sort.cu:28-44
+//   identifyTileRanges                           ... FGS/csrc/cuda_rasterizer/render.cu:54-79
+//   per-tile bucket count / offsets / ranges     ... render.cu:13-49, :418-434
+//   renderBucket (bucket from T=1)               ... render.cu:130-275
+//   mergeResults (bucket merge, T<1e-4 cutoff)   ... render.cu:286-391
+//   bucket backward                              ... FGS/csrc/cuda_rasterizer/backward.cu:695-878
+//   preprocess backward                          ... backward.cu:250-539, :922-954
+//   densification statistics                     ... backward.cu:892-918
+// The per-Gaussian and per-pixel arithmetic is shared with the kernels via
+// fastergs_amd/csrc/fgs_math.h (see the header comment there). Differences
+// from the reference that are decisions of this project:
+//   * pair order is canonical: (tile, depth bits, Gaussian index); the
+//     reference breaks depth ties by atomic emission order;
+//   * exp2/log2 are the project's own polynomials instead of ex2.approx /
+//     lg2.approx, and sqrt/div are IEEE-correct instead of sqrt.approx;
+//   * the radius formula is evaluated in float instead of double.
+// Mode 0 ("bucketed") restates the reference compositing exactly; mode 1
+// ("sequential") is the plain path: one running (T,C) per pixel over the
+// whole list, with the same T<1e-4 test at bucket boundaries, and a
+// per-pixel reverse backward starting from T_final.
+//
+// Per-Gaussian gradient sums are formed per (sorted pair) in double and
+// reduced in sorted-pair order, so results do not depend on thread count.
+#include <parallel/algorithm>
+#include <omp.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../fastergs_amd/csrc/fgs_math.h"
+
+namespace {
+
+using fgs::CamRef;
+using fgs::GaussRef;
+
+template <typename R>
+struct State {
+  int P = 0, W = 0, H = 0, gx = 0, gy = 0, T = 0, B = 32, mode = 0, deg = 0, M = 0;
+  R tanfovx, tanfovy, focal_x, focal_y, scale_mod, mult;
+  std::vector<R> view, proj, campos, bg;
+  std::vector<R> means, opac, scales, rots, dc, shs, colors;
+  std::vector<uint8_t> cull;
+  bool has_sh = false, has_colors = false, has_cull = false;
+
+  // forward products
+  std::vector<fgs::Proj<R>> pj;
+  std::vector<int32_t> radii;
+  std::vector<uint32_t> pair_count;   // per Gaussian
+  std::vector<uint64_t> keys;         // sorted
+  std::vector<uint32_t> vals;         // sorted
+  std::vector<uint32_t> ranges;       // T x 2
+  std::vector<uint32_t> bucket_off;   // T, inclusive scan of bucket counts
+  std::vector<uint32_t> bucket_ranges;  // NB x 2
+  std::vector<uint32_t> bucket_tile;    // NB
+  std::vector<R> out, tfinal, pixc, acc_t, acc_c;
+  std::vector<int32_t> contrib;
+  std::vector<uint32_t> max_contrib;
+  // per splat render data: x, y, ca, cb, cc, w, r, g, b
+  std::vector<R> splat;
+
+  // backward products
+  std::vector<double> raw;  // P x RAW_N
+  std::vector<R> dmean2d, dconic, dopac, dcolor, dmean3d, dcov3d, ddc, dsh, dscale, drot;
+
+  long long R_pairs() const { return (long long)vals.size(); }
+  long long NB() const { return (long long)bucket_tile.size(); }
+
+  CamRef<R> cam() const {
+    CamRef<R> c;
+    c.view = view.data();
+    c.proj = proj.data();
+    c.campos = campos.data();
+    c.tanfovx = tanfovx;
+    c.tanfovy = tanfovy;
+    c.focal_x = focal_x;
+    c.focal_y = focal_y;
+    c.scale_mod = scale_mod;
+    c.mult = mult;
+    c.W = W;
+    c.H = H;
+    c.grid_x = gx;
+    c.grid_y = gy;
+    c.deg = deg;
+    c.max_coeffs = M;
+    return c;
+  }
+  GaussRef<R> gauss() const {
+    GaussRef<R> g;
+    g.means = means.data();
+    g.opac = opac.data();
+    g.scales = scales.data();
+    g.rots = rots.data();
+    g.dc = has_sh ? dc.data() : nullptr;
+    g.shs = has_sh ? shs.data() : nullptr;
+    g.colors = has_colors ? colors.data() : nullptr;
+    g.culling = has_cull ? cull.data() : nullptr;
+    return g;
+  }
+};
+
+template <typename R, typename S>
+void copy_in(std::vector<R>& dst, const S* src, size_t n) {
+  dst.resize(n);
+  for (size_t k = 0; k < n; ++k) dst[k] = (R)src[k];
+}
+
+template <typename R>
+void preprocess(State<R>& st) {
+  const auto cam = st.cam();
+  const auto G = st.gauss();
+  st.pj.assign(st.P, fgs::Proj<R>());
+  st.radii.assign(st.P, 0);
+  st.pair_count.assign(st.P, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < st.P; ++i) {
+    fgs::Proj<R>& o = st.pj[i];
+    fgs::preprocess_gauss(i, G, cam, o);
+    if (!o.valid) continue;
+    st.radii[i] = o.radius;
+    uint32_t n = 0;
+    for (int ty = o.rect[1]; ty < o.rect[3]; ++ty)
+      for (int tx = o.rect[0]; tx < o.rect[2]; ++tx)
+        n += fgs::tile_hit(tx, ty, o.x, o.y, o.conic, o.power) ? 1u : 0u;
+    st.pair_count[i] = n;
+  }
+}
+
+struct Pair {
+  uint64_t key;
+  uint32_t val;
+};
+
+template <typename R>
+void emit_and_sort(State<R>& st) {
+  std::vector<uint64_t> off(st.P + 1, 0);
+  for (int i = 0; i < st.P; ++i) off[i + 1] = off[i] + st.pair_count[i];
+  const size_t Rn = off[st.P];
+  std::vector<Pair> pairs(Rn);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < st.P; ++i) {
+    const fgs::Proj<R>& o = st.pj[i];
+    if (!o.valid || st.pair_count[i] == 0) continue;
+    size_t k = off[i];
+    const float dz = (float)o.depth;
+    for (int ty = o.rect[1]; ty < o.rect[3]; ++ty)
+      for (int tx = o.rect[0]; tx < o.rect[2]; ++tx)
+        if (fgs::tile_hit(tx, ty, o.x, o.y, o.conic, o.power)) {
+          const uint64_t tile = (uint64_t)(ty * st.gx + tx);
+          pairs[k].key = (tile << 32) | fgs::depth_bits(dz);
+          pairs[k].val = (uint32_t)i;
+          ++k;
+        }
+  }
+  // Canonical order (key, index): equals a stable sort by key of the
+  // index-ordered emission.
+  __gnu_parallel::sort(pairs.begin(), pairs.end(), [](const Pair& a, const Pair& b) {
+    return a.key < b.key || (a.key == b.key && a.val < b.val);
+  });
+  st.keys.resize(Rn);
+  st.vals.resize(Rn);
+  for (size_t k = 0; k < Rn; ++k) {
+    st.keys[k] = pairs[k].key;
+    st.vals[k] = pairs[k].val;
+  }
+}
+
+template <typename R>
+void ranges_and_buckets(State<R>& st) {
+  st.ranges.assign(2 * (size_t)st.T, 0);
+  const size_t Rn = st.vals.size();
+  for (size_t k = 0; k < Rn; ++k) {  // identifyTileRanges (render.cu:54-79)
+    const uint32_t t = (uint32_t)(st.keys[k] >> 32);
+    if (k == 0 || (uint32_t)(st.keys[k - 1] >> 32) != t) st.ranges[2 * t] = (uint32_t)k;
+    if (k + 1 == Rn || (uint32_t)(st.keys[k + 1] >> 32) != t) st.ranges[2 * t + 1] = (uint32_t)(k + 1);
+  }
+  st.bucket_off.assign(st.T, 0);
+  uint32_t acc = 0;
+  for (int t = 0; t < st.T; ++t) {
+    const uint32_t len = st.ranges[2 * t + 1] - st.ranges[2 * t];
+    acc += (len + st.B - 1) / st.B;
+    st.bucket_off[t] = acc;
+  }
+  st.bucket_ranges.assign(2 * (size_t)acc, 0);
+  st.bucket_tile.assign(acc, 0);
+  for (int t = 0; t < st.T; ++t) {
+    uint32_t b = t == 0 ? 0 : st.bucket_off[t - 1];
+    uint32_t s = st.ranges[2 * t];
+    const uint32_t e = st.ranges[2 * t + 1];
+    while (s < e) {
+      const uint32_t n = std::min<uint32_t>(st.B, e - s);
+      st.bucket_ranges[2 * b] = s;
+      st.bucket_ranges[2 * b + 1] = s + n;
+      st.bucket_tile[b] = (uint32_t)t;
+      s += n;
+      ++b;
+    }
+  }
+}
+
+template <typename R>
+void build_splats(State<R>& st) {
+  st.splat.assign((size_t)st.P * 9, R(0));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < st.P; ++i) {
+    const fgs::Proj<R>& o = st.pj[i];
+    if (!o.valid) continue;
+    R* s = &st.splat[9 * (size_t)i];
+    s[0] = o.x;
+    s[1] = o.y;
+    fgs::log2_conic(o.conic, o.log2op, s[2], s[3], s[4], s[5]);
+    s[6] = o.rgb[0];
+    s[7] = o.rgb[1];
+    s[8] = o.rgb[2];
+  }
+}
+
+template <typename R>
+void composite(State<R>& st) {
+  const size_t HW = (size_t)st.W * st.H;
+  st.out.assign(3 * HW, R(0));
+  st.tfinal.assign(HW, R(0));
+  st.pixc.assign(3 * HW, R(0));
+  st.contrib.assign(HW, 0);
+  st.max_contrib.assign(st.T, 0);
+  const size_t NB = st.bucket_tile.size();
+  if (st.mode == 0) {
+    st.acc_t.assign(NB * fgs::TILE_PIX, R(0));
+    st.acc_c.assign(NB * fgs::TILE_PIX * 3, R(0));
+  } else {
+    st.acc_t.clear();
+    st.acc_c.clear();
+  }
+  const R bg0 = st.bg[0], bg1 = st.bg[1], bg2 = st.bg[2];
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < st.T; ++t) {
+    const int tx = t % st.gx, ty = t / st.gx;
+    const uint32_t b0 = t == 0 ? 0 : st.bucket_off[t - 1];
+    const uint32_t b1 = st.bucket_off[t];
+    uint32_t mc = 0;
+    for (int ly = 0; ly < fgs::TILE_H; ++ly)
+      for (int lx = 0; lx < fgs::TILE_W; ++lx) {
+        const int px = tx * fgs::TILE_W + lx, py = ty * fgs::TILE_H + ly;
+        if (px >= st.W || py >= st.H) continue;
+        const int lp = ly * fgs::TILE_W + lx;
+        const R fx = (R)px, fy = (R)py;
+        R T = R(1), C0 = R(0), C1 = R(0), C2 = R(0);
+        int32_t cnt = 0;
+        for (uint32_t b = b0; b < b1; ++b) {
+          const uint32_t s = st.bucket_ranges[2 * b], e = st.bucket_ranges[2 * b + 1];
+          if (st.mode == 0) {
+            R Tb = R(1), D0 = R(0), D1 = R(0), D2 = R(0);
+            for (uint32_t k = s; k < e; ++k) {
+              const R* sp = &st.splat[9 * (size_t)st.vals[k]];
+              const R a = fgs::splat_alpha(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], fx, fy);
+              fgs::blend(a, sp[6], sp[7], sp[8], Tb, D0, D1, D2);
+            }
+            fgs::merge(Tb, D0, D1, D2, T, C0, C1, C2);
+            const size_t q = (size_t)b * fgs::TILE_PIX + lp;
+            st.acc_t[q] = T;
+            st.acc_c[3 * q] = C0;
+            st.acc_c[3 * q + 1] = C1;
+            st.acc_c[3 * q + 2] = C2;
+          } else {
+            for (uint32_t k = s; k < e; ++k) {
+              const R* sp = &st.splat[9 * (size_t)st.vals[k]];
+              const R a = fgs::splat_alpha(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], fx, fy);
+              fgs::blend(a, sp[6], sp[7], sp[8], T, C0, C1, C2);
+            }
+          }
+          cnt += (int32_t)(e - s);
+          if (T < R(fgs::T_CUT_F)) break;
+        }
+        const size_t g = (size_t)py * st.W + px;
+        st.contrib[g] = cnt;
+        st.tfinal[g] = T;
+        st.out[g] = fgs::fma_(T, bg0, C0);
+        st.out[HW + g] = fgs::fma_(T, bg1, C1);
+        st.out[2 * HW + g] = fgs::fma_(T, bg2, C2);
+        st.pixc[3 * g] = C0;
+        st.pixc[3 * g + 1] = C1;
+        st.pixc[3 * g + 2] = C2;
+        mc = std::max(mc, (uint32_t)cnt);
+      }
+    st.max_contrib[t] = mc;
+  }
+}
+
+template <typename R>
+void forward(State<R>& st) {
+  preprocess(st);
+  emit_and_sort(st);
+  ranges_and_buckets(st);
+  build_splats(st);
+  composite(st);
+}
+
+template <typename R>
+void backward(State<R>& st, const R* dimg) {
+  const size_t HW = (size_t)st.W * st.H;
+  const size_t Rn = st.vals.size();
+  std::vector<double> rows(Rn * fgs::RAW_N, 0.0);
+  const R bg0 = st.bg[0], bg1 = st.bg[1], bg2 = st.bg[2];
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < st.T; ++t) {
+    const int tx = t % st.gx, ty = t / st.gx;
+    const uint32_t b0 = t == 0 ? 0 : st.bucket_off[t - 1];
+    const uint32_t b1 = st.bucket_off[t];
+    const uint32_t tile_start = st.ranges[2 * t];
+    for (int ly = 0; ly < fgs::TILE_H; ++ly)
+      for (int lx = 0; lx < fgs::TILE_W; ++lx) {
+        const int px = tx * fgs::TILE_W + lx, py = ty * fgs::TILE_H + ly;
+        if (px >= st.W || py >= st.H) continue;
+        const int lp = ly * fgs::TILE_W + lx;
+        const size_t g = (size_t)py * st.W + px;
+        const R fx = (R)px, fy = (R)py;
+        const R dp0 = dimg[g], dp1 = dimg[HW + g], dp2 = dimg[2 * HW + g];
+        const R bgdot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
+        const R Tf = st.tfinal[g];
+        const uint32_t cnt = (uint32_t)st.contrib[g];
+        auto step = [&](uint32_t k, R& T, R& r0, R& r1, R& r2) {
+          const R* sp = &st.splat[9 * (size_t)st.vals[k]];
+          R acc[fgs::RAW_N];
+          for (int f = 0; f < fgs::RAW_N; ++f) acc[f] = R(0);
+          fgs::pix_bwd(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], sp[6], sp[7], sp[8], fx, fy, dp0,
+                       dp1, dp2, bgdot, Tf, T, r0, r1, r2, acc);
+          double* row = &rows[k * fgs::RAW_N];
+          for (int f = 0; f < fgs::RAW_N; ++f) row[f] += (double)acc[f];
+        };
+        if (st.mode == 0) {
+          for (uint32_t b = b0; b < b1; ++b) {
+            const uint32_t s = st.bucket_ranges[2 * b], e = st.bucket_ranges[2 * b + 1];
+            if (s - tile_start >= cnt) break;  // backward.cu:663-664
+            const size_t q = (size_t)b * fgs::TILE_PIX + lp;
+            R T = st.acc_t[q];
+            R r0 = st.pixc[3 * g] - st.acc_c[3 * q];
+            R r1 = st.pixc[3 * g + 1] - st.acc_c[3 * q + 1];
+            R r2 = st.pixc[3 * g + 2] - st.acc_c[3 * q + 2];
+            for (uint32_t k = e; k-- > s;) step(k, T, r0, r1, r2);
+          }
+        } else {
+          R T = Tf, r0 = R(0), r1 = R(0), r2 = R(0);
+          for (uint32_t k = tile_start + cnt; k-- > tile_start;) step(k, T, r0, r1, r2);
+        }
+      }
+  }
+  st.raw.assign((size_t)st.P * fgs::RAW_N, 0.0);
+  for (size_t k = 0; k < Rn; ++k) {
+    double* dst = &st.raw[(size_t)st.vals[k] * fgs::RAW_N];
+    const double* src = &rows[k * fgs::RAW_N];
+    for (int f = 0; f < fgs::RAW_N; ++f) dst[f] += src[f];
+  }
+
+  const int P = st.P, M = st.M;
+  st.dmean2d.assign((size_t)P * 4, R(0));
+  st.dconic.assign((size_t)P * 4, R(0));
+  st.dopac.assign(P, R(0));
+  st.dcolor.assign((size_t)P * 3, R(0));
+  st.dmean3d.assign((size_t)P * 3, R(0));
+  st.dcov3d.assign((size_t)P * 6, R(0));
+  st.ddc.assign((size_t)P * 3, R(0));
+  st.dsh.assign((size_t)P * M * 3, R(0));
+  st.dscale.assign((size_t)P * 3, R(0));
+  st.drot.assign((size_t)P * 4, R(0));
+  const auto cam = st.cam();
+  const auto G = st.gauss();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < P; ++i) {
+    R rawr[fgs::RAW_STRIDE];
+    bool any = false;
+    for (int f = 0; f < fgs::RAW_N; ++f) {
+      rawr[f] = (R)st.raw[(size_t)i * fgs::RAW_N + f];
+      any = any || rawr[f] != R(0);
+    }
+    if (!any) continue;
+    fgs::Grad2D<R> g2;
+    fgs::finalize_raw(rawr, st.opac[i], st.W, st.H, g2);
+    if (rawr[fgs::RAW_POW] == R(0)) g2.opacity = R(0);
+    for (int k = 0; k < 4; ++k) st.dmean2d[4 * (size_t)i + k] = g2.mean2d[k];
+    for (int k = 0; k < 4; ++k) st.dconic[4 * (size_t)i + k] = g2.conic[k];
+    st.dopac[i] = g2.opacity;
+    for (int k = 0; k < 3; ++k) st.dcolor[3 * (size_t)i + k] = g2.color[k];
+    if (!(st.radii[i] > 0)) continue;  // backward.cu:257, :501
+    fgs::GaussGrad<R> out;
+    out.dmean3d = &st.dmean3d[3 * (size_t)i];
+    out.dcov3d = &st.dcov3d[6 * (size_t)i];
+    out.dscale = &st.dscale[3 * (size_t)i];
+    out.drot = &st.drot[4 * (size_t)i];
+    out.ddc = &st.ddc[3 * (size_t)i];
+    out.dsh = M > 0 ? &st.dsh[(size_t)i * M * 3] : nullptr;
+    fgs::gauss_bwd(i, G, cam, st.pj[i].clamp, g2, out);
+  }
+}
+
+template <typename R>
+State<R>* create(int P, const R* means, const R* opac, const R* scales, const R* rots, const R* dc,
+                 const R* shs, int M, const R* colors, const uint8_t* cull, const R* view,
+                 const R* proj, const R* campos, const R* bg, double tanfovx, double tanfovy,
+                 double scale_mod, double mult, int W, int H, int deg, int bucket, int mode) {
+  auto* st = new State<R>();
+  st->P = P;
+  st->W = W;
+  st->H = H;
+  st->gx = (W + fgs::TILE_W - 1) / fgs::TILE_W;
+  st->gy = (H + fgs::TILE_H - 1) / fgs::TILE_H;
+  st->T = st->gx * st->gy;
+  st->B = bucket;
+  st->mode = mode;
+  st->deg = deg;
+  st->M = M;
+  // The camera scalars are float in the API (pybind.cpp:26-35); focal is
+  // computed in float like preprocess.cu:941-942.
+  const float tfx = (float)tanfovx, tfy = (float)tanfovy;
+  st->tanfovx = (R)tfx;
+  st->tanfovy = (R)tfy;
+  st->focal_x = (R)((float)W / (2.0f * tfx));
+  st->focal_y = (R)((float)H / (2.0f * tfy));
+  st->scale_mod = (R)(float)scale_mod;
+  st->mult = (R)(float)mult;
+  copy_in(st->view, view, 16);
+  copy_in(st->proj, proj, 16);
+  copy_in(st->campos, campos, 3);
+  copy_in(st->bg, bg, 3);
+  copy_in(st->means, means, 3 * (size_t)P);
+  copy_in(st->opac, opac, (size_t)P);
+  copy_in(st->scales, scales, 3 * (size_t)P);
+  copy_in(st->rots, rots, 4 * (size_t)P);
+  st->has_colors = colors != nullptr;
+  st->has_sh = !st->has_colors && shs != nullptr && dc != nullptr;
+  if (st->has_sh) {
+    copy_in(st->dc, dc, 3 * (size_t)P);
+    copy_in(st->shs, shs, (size_t)P * M * 3);
+  } else {
+    st->M = 0;
+  }
+  if (st->has_colors) copy_in(st->colors, colors, 3 * (size_t)P);
+  st->has_cull = cull != nullptr;
+  if (st->has_cull) st->cull.assign(cull, cull + P);
+  return st;
+}
+
+template <typename T>
+long long vec_get(const std::vector<T>& v, void* dst) {
+  if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+  return (long long)v.size();
+}
+
+template <typename R>
+long long get_named(State<R>* st, const char* name, void* dst) {
+  const std::string n(name);
+  if (n == "radii") return vec_get(st->radii, dst);
+  if (n == "pair_count") return vec_get(st->pair_count, dst);
+  if (n == "keys") return vec_get(st->keys, dst);
+  if (n == "vals") return vec_get(st->vals, dst);
+  if (n == "ranges") return vec_get(st->ranges, dst);
+  if (n == "bucket_off") return vec_get(st->bucket_off, dst);
+  if (n == "bucket_ranges") return vec_get(st->bucket_ranges, dst);
+  if (n == "bucket_tile") return vec_get(st->bucket_tile, dst);
+  if (n == "out") return vec_get(st->out, dst);
+  if (n == "tfinal") return vec_get(st->tfinal, dst);
+  if (n == "pixc") return vec_get(st->pixc, dst);
+  if (n == "contrib") return vec_get(st->contrib, dst);
+  if (n == "max_contrib") return vec_get(st->max_contrib, dst);
+  if (n == "acc_t") return vec_get(st->acc_t, dst);
+  if (n == "acc_c") return vec_get(st->acc_c, dst);
+  if (n == "splat") return vec_get(st->splat, dst);
+  if (n == "raw") return vec_get(st->raw, dst);
+  if (n == "dmean2d") return vec_get(st->dmean2d, dst);
+  if (n == "dconic") return vec_get(st->dconic, dst);
+  if (n == "dopac") return vec_get(st->dopac, dst);
+  if (n == "dcolor") return vec_get(st->dcolor, dst);
+  if (n == "dmean3d") return vec_get(st->dmean3d, dst);
+  if (n == "dcov3d") return vec_get(st->dcov3d, dst);
+  if (n == "ddc") return vec_get(st->ddc, dst);
+  if (n == "dsh") return vec_get(st->dsh, dst);
+  if (n == "dscale") return vec_get(st->dscale, dst);
+  if (n == "drot") return vec_get(st->drot, dst);
+  if (n == "proj") {  // per Gaussian: valid, radius, rect[4], clamp as float bits + x,y,depth,conic3,power,log2op,rgb3
+    const size_t P = st->pj.size();
+    if (dst) {
+      R* d = (R*)dst;
+      for (size_t i = 0; i < P; ++i) {
+        const auto& o = st->pj[i];
+        R* r = d + 16 * i;
+        r[0] = (R)o.valid;
+        r[1] = (R)o.rect[0];
+        r[2] = (R)o.rect[1];
+        r[3] = (R)o.rect[2];
+        r[4] = (R)o.rect[3];
+        r[5] = (R)o.clamp;
+        r[6] = o.valid ? o.x : R(0);
+        r[7] = o.valid ? o.y : R(0);
+        r[8] = o.valid ? o.depth : R(0);
+        r[9] = o.valid ? o.conic[0] : R(0);
+        r[10] = o.valid ? o.conic[1] : R(0);
+        r[11] = o.valid ? o.conic[2] : R(0);
+        r[12] = o.valid ? o.power : R(0);
+        r[13] = o.valid ? o.rgb[0] : R(0);
+        r[14] = o.valid ? o.rgb[1] : R(0);
+        r[15] = o.valid ? o.rgb[2] : R(0);
+      }
+    }
+    return (long long)(16 * P);
+  }
+  return -1;
+}
+
+template <typename R>
+void add_stats(State<R>* st, R* accum, R* accum_abs, R* max_radii, R* denom) {
+  for (int i = 0; i < st->P; ++i) {
+    if (!(st->radii[i] > 0)) continue;
+    const R* g = &st->dmean2d[4 * (size_t)i];
+    if (denom) denom[i] += R(1);
+    if (max_radii) max_radii[i] = std::max(max_radii[i], (R)st->radii[i]);
+    if (accum) accum[i] += fgs::sqrt_(g[0] * g[0] + g[1] * g[1]);
+    if (accum_abs) accum_abs[i] += fgs::sqrt_(g[2] * g[2] + g[3] * g[3]);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- float32 oracle
+void* orc_create_f32(int P, const float* means, const float* opac, const float* scales,
+                     const float* rots, const float* dc, const float* shs, int M,
+                     const float* colors, const uint8_t* cull, const float* view,
+                     const float* proj, const float* campos, const float* bg, double tanfovx,
+                     double tanfovy, double scale_mod, double mult, int W, int H, int deg,
+                     int bucket, int mode) {
+  return create<float>(P, means, opac, scales, rots, dc, shs, M, colors, cull, view, proj, campos,
+                       bg, tanfovx, tanfovy, scale_mod, mult, W, H, deg, bucket, mode);
+}
+void orc_forward_f32(void* h) { forward(*(State<float>*)h); }
+void orc_backward_f32(void* h, const float* dimg) { backward(*(State<float>*)h, dimg); }
+long long orc_get_f32(void* h, const char* name, void* dst) {
+  return get_named((State<float>*)h, name, dst);
+}
+void orc_stats_f32(void* h, float* accum, float* accum_abs, float* max_radii, float* denom) {
+  add_stats((State<float>*)h, accum, accum_abs, max_radii, denom);
+}
+void orc_destroy_f32(void* h) { delete (State<float>*)h; }
+
+// ---- float64 oracle (finite-difference checks)
+void* orc_create_f64(int P, const double* means, const double* opac, const double* scales,
+                     const double* rots, const double* dc, const double* shs, int M,
+                     const double* colors, const uint8_t* cull, const double* view,
+                     const double* proj, const double* campos, const double* bg, double tanfovx,
+                     double tanfovy, double scale_mod, double mult, int W, int H, int deg,
+                     int bucket, int mode) {
+  return create<double>(P, means, opac, scales, rots, dc, shs, M, colors, cull, view, proj,
+                        campos, bg, tanfovx, tanfovy, scale_mod, mult, W, H, deg, bucket, mode);
+}
+void orc_forward_f64(void* h) { forward(*(State<double>*)h); }
+void orc_backward_f64(void* h, const double* dimg) { backward(*(State<double>*)h, dimg); }
+long long orc_get_f64(void* h, const char* name, void* dst) {
+  return get_named((State<double>*)h, name, dst);
+}
+void orc_destroy_f64(void* h) { delete (State<double>*)h; }
+
+// scalar helpers exposed for unit tests of the shared arithmetic
+float orc_exp2f(float x) { return fgs::exp2_(x); }
+float orc_log2f(float x) { return fgs::log2_(x); }
+int orc_tile_hit_f32(int tx, int ty, float x, float y, float ca, float cb, float cc, float power) {
+  const float conic[3] = {ca, cb, cc};
+  return fgs::tile_hit(tx, ty, x, y, conic, power) ? 1 : 0;
+}
+int orc_num_threads() { return omp_get_max_threads(); }
+
+}  // extern "C"
