@@ -1,0 +1,161 @@
+"""Benchmark of the rasterizer hot path on one MI355X.
+
+One step = one training iteration of the drop-in API on the synthetic
+"heavy" workload (1M Gaussians, 64-cluster mixture with 2% large splats,
+SH degree 3, one 1280x840 camera): GaussianRasterizer forward, L2 loss to a
+fixed random target, backward (rasterizer adjoint, preprocess adjoint and
+the densification statistics that train_faster.py requests). Inputs are
+resident in HBM before timing. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--schedule fast|plain]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="heavy")
+    ap.add_argument("--schedule", default="plain", choices=["plain", "fast"])
+    ap.add_argument("--bucket", type=int, default=32)
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--compare-plain", type=int, default=1)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the image-space sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from fastergs_amd import scenes
+    from fastergs_amd._lib import SCHEDULE_FAST, SCHEDULE_PLAIN
+    from fastergs_amd.rasterizer import (GaussianRasterizationSettings, GaussianRasterizer,
+                                         RasterOptions, set_default_options)
+
+    dev = torch.device("cuda:0")
+    scene, cam = scenes.workload(args.workload, seed=0)
+    P = scene["means3D"].shape[0]
+    params = {k: torch.from_numpy(v).to(dev).requires_grad_(True) for k, v in scene.items()}
+    settings = GaussianRasterizationSettings(
+        image_height=cam.height, image_width=cam.width, tanfovx=cam.tanfovx,
+        tanfovy=cam.tanfovy, bg=torch.zeros(3, device=dev), scale_modifier=1.0,
+        viewmatrix=torch.from_numpy(cam.viewmatrix).to(dev),
+        projmatrix=torch.from_numpy(cam.projmatrix).to(dev), sh_degree=3,
+        campos=torch.from_numpy(cam.campos).to(dev), mult=1.0, prefiltered=False, debug=False)
+    target = torch.from_numpy(scenes.target_image(cam.width, cam.height)).to(dev)
+    accum = torch.zeros(P, 1, device=dev)
+    max_radii = torch.zeros(P, device=dev)
+    denom = torch.zeros(P, 1, device=dev)
+    rast = GaussianRasterizer(settings)
+
+    def step():
+        for t in params.values():
+            t.grad = None
+        img, radii = rast(means3D=params["means3D"], opacities=params["opacities"],
+                          dc=params["dc"], shs=params["shs"], scales=params["scales"],
+                          rotations=params["rotations"], xyz_gradient_accum=accum,
+                          max_radii=max_radii, denom=denom)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        return img
+
+    def timed(schedule: int, steps: int, warmup: int, fwd_only=False):
+        set_default_options(schedule=schedule, bucket_size=args.bucket)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(steps):
+            if fwd_only:
+                with torch.no_grad():
+                    rast(means3D=params["means3D"], opacities=params["opacities"],
+                         dc=params["dc"], shs=params["shs"], scales=params["scales"],
+                         rotations=params["rotations"])
+            else:
+                step()
+        e1.record()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        return e0.elapsed_time(e1) / steps, wall
+
+    sched = SCHEDULE_FAST if args.schedule == "fast" else SCHEDULE_PLAIN
+    ms, wall = timed(sched, args.steps, args.warmup)
+    fwd_ms, _ = timed(sched, args.steps, 2, fwd_only=True)
+    plain_ms = None
+    if args.compare_plain and sched != SCHEDULE_PLAIN:
+        plain_ms, _ = timed(SCHEDULE_PLAIN, max(5, args.steps // 2), 2)
+
+    cpu = None
+    if args.cpu_baseline:
+        cpu = cpu_baseline(scene, cam)
+
+    line = {
+        "metric": "fwd+bwd ms/iter, 1M Gaussians @1280x840",
+        "value": round(ms, 4),
+        "unit": "ms/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 64-cluster scene, 2% large splats, SH degree 3)",
+        "config": {"workload": f"{args.workload}: {P} Gaussians, {cam.width}x{cam.height}, "
+                               f"SH3, bucket {args.bucket}, schedule {args.schedule}",
+                   "parallelism": "single GPU"},
+        "render_fps": round(1000.0 / fwd_ms, 2),
+        "fwd_ms": round(fwd_ms, 4),
+        "wall_ms_per_step": round(wall, 4),
+        "plain_ms_per_step": None if plain_ms is None else round(plain_ms, 4),
+        "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms, 3),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(scene, cam):
+    """CPU oracle (C++, OpenMP) fwd+bwd on the same workload, a few iterations."""
+    from oracle.pyoracle import Oracle, num_threads
+    from fastergs_amd import scenes
+    tgt = scenes.target_image(cam.width, cam.height)
+    times = []
+    t_end = time.perf_counter() + 20.0
+    while len(times) < 3 and (not times or time.perf_counter() < t_end):
+        t0 = time.perf_counter()
+        o = Oracle(scene, cam, bucket=32, mode=0).forward()
+        img = o.image()
+        o.backward((2.0 * (img - tgt) / img.size).astype(np.float32))
+        times.append((time.perf_counter() - t0) * 1e3)
+        del o
+    return {"value": round(float(np.median(times)), 1), "unit": "ms/iter",
+            "cores": num_threads(), "kind": "port",
+            "sample": f"{len(times)} full fwd+bwd iterations of the same workload (median)"}
+
+
+if __name__ == "__main__":
+    main()

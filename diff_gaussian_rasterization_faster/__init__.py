@@ -1,0 +1,8 @@
+"""Import name used by the FasterGS training/rendering scripts
+(``from diff_gaussian_rasterization_faster import GaussianRasterizationSettings,
+GaussianRasterizer``); forwards to fastergs_amd."""
+from fastergs_amd.rasterizer import (  # noqa: F401
+    GaussianRasterizationSettings,
+    GaussianRasterizer,
+    rasterize_gaussians,
+)

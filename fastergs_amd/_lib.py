@@ -1,0 +1,114 @@
+"""ctypes binding of libfastergs.so (declarations in include/fastergs.h).
+
+The library must be built in-tree (``python -m fastergs_amd.build``);
+importing the rasterizer without it raises immediately -- there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfastergs.so")
+
+SCHEDULE_PLAIN = 0
+SCHEDULE_FAST = 1
+
+vp = ctypes.c_void_p
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("tanfovx", ctypes.c_float), ("tanfovy", ctypes.c_float),
+                ("scale_modifier", ctypes.c_float), ("mult", ctypes.c_float),
+                ("sh_degree", ctypes.c_int),
+                ("viewmatrix", vp), ("projmatrix", vp), ("campos", vp), ("bg", vp)]
+
+
+class Gaussians(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_int), ("max_coeffs", ctypes.c_int),
+                ("means3D", vp), ("opacities", vp), ("scales", vp), ("rotations", vp),
+                ("dc", vp), ("shs", vp), ("colors_precomp", vp), ("culling", vp)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("schedule", ctypes.c_int), ("bucket_size", ctypes.c_int),
+                ("split_tiles", ctypes.c_int), ("abs_grad", ctypes.c_int)]
+
+
+class FrameView(ctypes.Structure):
+    _fields_ = [(n, vp) for n in (
+        "geom", "splat", "pair_count", "tile_count", "ranges", "bucket_off", "max_contrib",
+        "counts", "sorted_keys", "sorted_vals", "bucket_ranges", "bucket_tile", "acc_t", "acc_c",
+        "pixel_colors")]
+
+
+class Grads(ctypes.Structure):
+    _fields_ = [(n, vp) for n in (
+        "dmean2D", "dconic", "dopacity", "dcolor", "dmean3D", "dcov3D", "ddc", "dsh", "dscale",
+        "drot")]
+
+
+# name -> (restype, argtypes); the list is the ABI surface checked by tests.
+SIGNATURES = {
+    "fgs_gauss_ws_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(Options)]),
+    "fgs_pair_ws_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.POINTER(Options)]),
+    "fgs_grad_ws_size": (ctypes.c_size_t, [ctypes.c_int]),
+    "fgs_preprocess": (ctypes.c_int, [ctypes.POINTER(Camera), ctypes.POINTER(Gaussians),
+                                      ctypes.POINTER(Options), vp, ctypes.c_size_t, vp, vp]),
+    "fgs_read_counts": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_int64),
+                                       vp]),
+    "fgs_render": (ctypes.c_int, [ctypes.POINTER(Camera), ctypes.POINTER(Gaussians),
+                                  ctypes.POINTER(Options), vp, ctypes.c_size_t, vp,
+                                  ctypes.c_size_t, ctypes.c_int64, ctypes.c_int64, vp, vp, vp,
+                                  vp]),
+    "fgs_backward": (ctypes.c_int, [ctypes.POINTER(Camera), ctypes.POINTER(Gaussians),
+                                    ctypes.POINTER(Options), vp, vp, ctypes.c_int64,
+                                    ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.POINTER(Grads),
+                                    vp]),
+    "fgs_densify_stats": (ctypes.c_int, [ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]),
+    "fgs_frame_view_get": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.POINTER(Options), ctypes.POINTER(FrameView)]),
+    "fgs_sort_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
+    "fgs_sort_pairs": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int,
+                                      ctypes.c_int, vp, ctypes.c_size_t, vp]),
+    "fgs_scan_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
+    "fgs_scan_u32": (ctypes.c_int, [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_size_t, vp]),
+    "fgs_last_error": (ctypes.c_char_p, []),
+    "fgs_build_info": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class FasterGSError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfastergs.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m fastergs_amd.build` "
+                "(the rasterizer has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().fgs_last_error().decode(errors="replace")
+        raise FasterGSError(f"{what} failed ({rc}): {msg}")

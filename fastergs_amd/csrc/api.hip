@@ -1,0 +1,377 @@
+// C ABI entry points (include/fastergs.h): argument checks, workspace
+// carving and the launch sequence of each schedule.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/fastergs.h"
+#include "fgs_kernels.h"
+#include "fgs_layout.h"
+
+using namespace fgs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* where) {
+  if (e == hipSuccess) return FGS_OK;
+  return fail(FGS_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define FGS_TRY(expr, where)                    \
+  do {                                          \
+    int rc__ = hip_check((expr), (where));      \
+    if (rc__ != FGS_OK) return rc__;            \
+  } while (0)
+
+int grid_x_of(int W) { return (W + TILE_W - 1) / TILE_W; }
+int grid_y_of(int H) { return (H + TILE_H - 1) / TILE_H; }
+int tiles_of(int W, int H) { return grid_x_of(W) * grid_y_of(H); }
+
+// Number of bits needed for tile ids (sort.cu:9-24 computes the same).
+int tile_bits(int T) {
+  int b = 0;
+  while (b < 31 && (T >> b) != 0) ++b;
+  return b;
+}
+
+// The plain schedule sorts 64-bit keys on [0, 32 + tile_bits); an odd
+// number of 8-bit passes leaves the result in the alternate buffers.
+bool plain_sorted_in_alt(int T) {
+  const int passes = (32 + tile_bits(T) + 7) / 8;
+  return (passes & 1) != 0;
+}
+
+int check_opts(const fgs_options* o) {
+  if (!o) return fail(FGS_ERR_ARG, "options is NULL");
+  if (o->schedule != FGS_SCHEDULE_PLAIN && o->schedule != FGS_SCHEDULE_FAST)
+    return fail(FGS_ERR_ARG, "unknown schedule");
+  const int b = o->bucket_size;
+  if (!(b == 16 || b == 32 || b == 64 || b == 128 || b == 256))
+    return fail(FGS_ERR_ARG, "bucket_size must be 16, 32, 64, 128 or 256");
+  return FGS_OK;
+}
+
+int check_cam(const fgs_camera* c) {
+  if (!c) return fail(FGS_ERR_ARG, "camera is NULL");
+  if (c->width <= 0 || c->height <= 0) return fail(FGS_ERR_ARG, "image size must be positive");
+  if (grid_x_of(c->width) > 65535 || grid_y_of(c->height) > 65535)
+    return fail(FGS_ERR_ARG, "image too large");
+  if (!c->viewmatrix || !c->projmatrix || !c->campos || !c->bg)
+    return fail(FGS_ERR_ARG, "camera matrices must be device pointers");
+  if (c->sh_degree < 0 || c->sh_degree > 3) return fail(FGS_ERR_ARG, "sh_degree must be 0..3");
+  return FGS_OK;
+}
+
+int check_gauss(const fgs_gaussians* g, const fgs_camera* c) {
+  if (!g) return fail(FGS_ERR_ARG, "gaussians is NULL");
+  if (g->P < 0) return fail(FGS_ERR_ARG, "P must be >= 0");
+  if (g->P == 0) return FGS_OK;
+  if (!g->means3D || !g->opacities || !g->scales || !g->rotations)
+    return fail(FGS_ERR_ARG, "means3D, opacities, scales and rotations are required");
+  if (!g->colors_precomp) {
+    if (!g->dc) return fail(FGS_ERR_ARG, "dc is required when colours come from SH");
+    const int need = sh_count(c->sh_degree) - 1;
+    if (need > 0 && (!g->shs || g->max_coeffs < need))
+      return fail(FGS_ERR_ARG, "shs has fewer coefficients than sh_degree needs");
+  }
+  return FGS_OK;
+}
+
+CamRef<float> make_cam(const fgs_camera* c, const fgs_gaussians* g) {
+  CamRef<float> r;
+  r.view = c->viewmatrix;
+  r.proj = c->projmatrix;
+  r.campos = c->campos;
+  r.tanfovx = c->tanfovx;
+  r.tanfovy = c->tanfovy;
+  r.focal_x = (float)c->width / (2.0f * c->tanfovx);   // preprocess.cu:941-942
+  r.focal_y = (float)c->height / (2.0f * c->tanfovy);
+  r.scale_mod = c->scale_modifier;
+  r.mult = c->mult;
+  r.W = c->width;
+  r.H = c->height;
+  r.grid_x = grid_x_of(c->width);
+  r.grid_y = grid_y_of(c->height);
+  r.deg = c->sh_degree;
+  r.max_coeffs = (g->colors_precomp || !g->shs) ? 0 : g->max_coeffs;
+  return r;
+}
+
+GaussRef<float> make_gauss(const fgs_gaussians* g) {
+  GaussRef<float> r;
+  r.means = g->means3D;
+  r.opac = g->opacities;
+  r.scales = g->scales;
+  r.rots = g->rotations;
+  r.colors = g->colors_precomp;
+  r.dc = g->colors_precomp ? nullptr : g->dc;
+  r.shs = g->colors_precomp ? nullptr : g->shs;
+  r.culling = g->culling;
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fgs_last_error(void) { return g_err.c_str(); }
+
+const char* fgs_build_info(void) {
+  return "fastergs gfx950: wave64, -ffp-contract=off, own exp2/log2, IEEE div/sqrt";
+}
+
+size_t fgs_gauss_ws_size(int P, int width, int height, const fgs_options* opt) {
+  (void)opt;
+  return carve_gauss(nullptr, P, tiles_of(width, height)).total;
+}
+
+size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
+                        const fgs_options* opt) {
+  (void)P;
+  const bool ck = opt && opt->schedule == FGS_SCHEDULE_FAST;
+  return carve_pair(nullptr, R, NB, width * height, ck).total;
+}
+
+size_t fgs_grad_ws_size(int P) { return align_up((size_t)(P > 0 ? P : 1) * RAW_STRIDE * 4); }
+
+size_t fgs_sort_ws_size(int64_t n) { return sort_ws_bytes(n); }
+size_t fgs_scan_ws_size(int64_t n) { return scan_ws_bytes(n); }
+
+int fgs_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, void* ws,
+                 size_t ws_bytes, void* stream) {
+  if (n < 0 || (n > 0 && (!in || !out))) return fail(FGS_ERR_ARG, "bad scan arguments");
+  if (ws_bytes < scan_ws_bytes(n)) return fail(FGS_ERR_CAPACITY, "scan workspace too small");
+  FGS_TRY(scan_u32(in, out, n, total_dev, ws, (hipStream_t)stream), "scan");
+  return FGS_OK;
+}
+
+int fgs_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                   int64_t n, int begin_bit, int end_bit, void* ws, size_t ws_bytes,
+                   void* stream) {
+  if (n < 0 || begin_bit < 0 || end_bit > 64 || (n > 0 && (!keys || !keys_alt)))
+    return fail(FGS_ERR_ARG, "bad sort arguments");
+  if ((vals == nullptr) != (vals_alt == nullptr)) return fail(FGS_ERR_ARG, "vals/vals_alt mismatch");
+  if (n >= (int64_t)0xffffffffll) return fail(FGS_ERR_ARG, "sort supports < 2^32 elements");
+  if (ws_bytes < sort_ws_bytes(n)) return fail(FGS_ERR_CAPACITY, "sort workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  bool alt = false;
+  FGS_TRY(sort_pairs(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, alt), "sort");
+  if (alt) {
+    FGS_TRY(hipMemcpyAsync(keys, keys_alt, n * 8, hipMemcpyDeviceToDevice, s), "sort copy");
+    if (vals) FGS_TRY(hipMemcpyAsync(vals, vals_alt, n * 4, hipMemcpyDeviceToDevice, s), "sort copy");
+  }
+  return FGS_OK;
+}
+
+int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+                   void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream) {
+  int rc;
+  if ((rc = check_opts(opt)) || (rc = check_cam(cam)) || (rc = check_gauss(g, cam))) return rc;
+  const int T = tiles_of(cam->width, cam->height);
+  const GaussWs w = carve_gauss(gauss_ws, g->P, T);
+  if (!gauss_ws || gauss_ws_bytes < w.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
+  if (g->P > 0 && !radii) return fail(FGS_ERR_ARG, "radii is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  FGS_TRY(fill_u32(w.counts, 0u, 16, s), "fill counts");
+  if (g->P == 0) return FGS_OK;
+  PreArgs a;
+  a.G = make_gauss(g);
+  a.cam = make_cam(cam, g);
+  a.geom = w.geom;
+  a.splat = w.splat;
+  a.dkey = w.dkey;
+  a.pcount = w.pcount;
+  a.tcount = w.tcount;
+  a.radii = radii;
+  a.P = g->P;
+  a.split_tiles = opt->split_tiles;
+  a.tasks = w.tasks;
+  a.counts = w.counts;
+  if (opt->schedule == FGS_SCHEDULE_PLAIN) {
+    FGS_TRY(launch_preprocess_plain(a, s), "preprocess");
+    FGS_TRY(scan_u32(w.pcount, w.poff, g->P, w.counts + 0, w.scratch, s), "pair scan");
+  } else {
+    return fail(FGS_ERR_ARG, "fast schedule not built");
+  }
+  return FGS_OK;
+}
+
+int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fgs_options* opt,
+                    int64_t counts_host[2], void* stream) {
+  (void)opt;
+  if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
+  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height));
+  uint32_t c[2] = {0, 0};
+  hipStream_t s = (hipStream_t)stream;
+  FGS_TRY(hipMemcpyAsync(c, w.counts, sizeof(c), hipMemcpyDeviceToHost, s), "read counts");
+  FGS_TRY(hipStreamSynchronize(s), "read counts sync");
+  counts_host[0] = c[0];
+  counts_host[1] = c[1];
+  return FGS_OK;
+}
+
+int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+               void* gauss_ws, size_t gauss_ws_bytes, void* pair_ws, size_t pair_ws_bytes,
+               int64_t R, int64_t NB, float* out_color, float* T_final,
+               int32_t* last_contributor, void* stream) {
+  int rc;
+  if ((rc = check_opts(opt)) || (rc = check_cam(cam)) || (rc = check_gauss(g, cam))) return rc;
+  if (!out_color || !T_final || !last_contributor) return fail(FGS_ERR_ARG, "NULL output");
+  if (R < 0 || NB < 0 || R >= (int64_t)0xffffffffll) return fail(FGS_ERR_ARG, "bad R / NB");
+  const int W = cam->width, H = cam->height, T = tiles_of(W, H);
+  const GaussWs gw = carve_gauss(gauss_ws, g->P, T);
+  if (!gauss_ws || gauss_ws_bytes < gw.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
+  const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
+  const PairWs pw = carve_pair(pair_ws, R, NB, W * H, fast);
+  if (!pair_ws || pair_ws_bytes < pw.total) return fail(FGS_ERR_CAPACITY, "pair workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  const int gx = grid_x_of(W);
+  if (fast) return fail(FGS_ERR_ARG, "fast schedule not built");
+
+  // PLAIN: duplicate with keys, 64-bit radix sort, ranges, composite.
+  FGS_TRY(launch_duplicate_plain(g->P, gw.geom, gw.dkey, gw.pcount, gw.poff, gx, pw.k0, pw.v0, s),
+          "duplicate");
+  bool alt = false;
+  FGS_TRY(sort_pairs(pw.k0, pw.v0, pw.k1, pw.v1, R, 0, 32 + tile_bits(T), pw.scratch, s, alt),
+          "sort");
+  const uint64_t* keys = alt ? pw.k1 : pw.k0;
+  const uint32_t* vals = alt ? pw.v1 : pw.v0;
+  FGS_TRY(launch_identify_ranges(R, keys, gw.ranges, T, s), "ranges");
+  CompArgs c;
+  c.W = W;
+  c.H = H;
+  c.grid_x = gx;
+  c.T = T;
+  c.bucket = opt->bucket_size;
+  c.ranges = gw.ranges;
+  c.vals = vals;
+  c.splat = gw.splat;
+  c.bg = cam->bg;
+  c.boff = gw.boff;
+  c.out = out_color;
+  c.tfinal = T_final;
+  c.contrib = last_contributor;
+  c.pixc = pw.pixc;
+  c.maxc = gw.maxc;
+  c.acc_t = nullptr;
+  c.acc_c = nullptr;
+  FGS_TRY(launch_composite_plain(c, s), "composite");
+  return FGS_OK;
+}
+
+int fgs_backward(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+                 void* gauss_ws, void* pair_ws, int64_t R, int64_t NB, const int32_t* radii,
+                 const float* T_final, const int32_t* last_contributor, const float* dL_dimage,
+                 void* grad_ws, const fgs_grads* out, void* stream) {
+  int rc;
+  if ((rc = check_opts(opt)) || (rc = check_cam(cam)) || (rc = check_gauss(g, cam))) return rc;
+  if (!out || !grad_ws || !dL_dimage || !T_final || !last_contributor)
+    return fail(FGS_ERR_ARG, "NULL argument");
+  if (g->P == 0) return FGS_OK;
+  const int W = cam->width, H = cam->height, T = tiles_of(W, H);
+  const GaussWs gw = carve_gauss(gauss_ws, g->P, T);
+  const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
+  const PairWs pw = carve_pair(pair_ws, R, NB, W * H, fast);
+  hipStream_t s = (hipStream_t)stream;
+  float* raw = (float*)grad_ws;
+  FGS_TRY(fill_f32(raw, 0.0f, (int64_t)g->P * RAW_STRIDE, s), "zero raw");
+  BwdArgs b;
+  b.W = W;
+  b.H = H;
+  b.grid_x = grid_x_of(W);
+  b.T = T;
+  b.bucket = opt->bucket_size;
+  b.abs_grad = opt->abs_grad;
+  b.NB = NB;
+  b.ranges = gw.ranges;
+  b.splat = gw.splat;
+  b.bg = cam->bg;
+  b.boff = gw.boff;
+  b.maxc = gw.maxc;
+  b.bucket_ranges = pw.bucket_ranges;
+  b.bucket_tile = pw.bucket_tile;
+  b.tfinal = T_final;
+  b.contrib = last_contributor;
+  b.pixc = pw.pixc;
+  b.acc_t = pw.acc_t;
+  b.acc_c = pw.acc_c;
+  b.dimg = dL_dimage;
+  b.raw = raw;
+  if (fast) return fail(FGS_ERR_ARG, "fast schedule not built");
+  b.vals = plain_sorted_in_alt(T) ? pw.v1 : pw.v0;
+  FGS_TRY(launch_backward_plain(b, s), "raster backward");
+  GaussBwdArgs q;
+  q.G = make_gauss(g);
+  q.cam = make_cam(cam, g);
+  q.P = g->P;
+  q.raw = raw;
+  q.radii = radii;
+  q.splat = gw.splat;
+  q.dmean2D = out->dmean2D;
+  q.dconic = out->dconic;
+  q.dopacity = out->dopacity;
+  q.dcolor = out->dcolor;
+  q.dmean3D = out->dmean3D;
+  q.dcov3D = out->dcov3D;
+  q.ddc = out->ddc;
+  q.dsh = q.cam.max_coeffs > 0 ? out->dsh : nullptr;
+  q.dscale = out->dscale;
+  q.drot = out->drot;
+  FGS_TRY(launch_gauss_bwd(q, s), "preprocess backward");
+  return FGS_OK;
+}
+
+int fgs_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* xyz_grad_accum,
+                      float* xyz_grad_accum_abs, float* max_radii, float* denom, void* stream) {
+  if (P < 0 || (P > 0 && (!radii || !dmean2D))) return fail(FGS_ERR_ARG, "bad stats arguments");
+  FGS_TRY(launch_densify_stats(P, radii, dmean2D, xyz_grad_accum, xyz_grad_accum_abs, max_radii,
+                               denom, (hipStream_t)stream),
+          "densify stats");
+  return FGS_OK;
+}
+
+int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                       int64_t NB, const fgs_options* opt, fgs_frame_view* v) {
+  int rc;
+  if ((rc = check_opts(opt))) return rc;
+  if (!v) return fail(FGS_ERR_ARG, "view is NULL");
+  const int T = tiles_of(width, height);
+  const GaussWs gw = carve_gauss(gauss_ws, P, T);
+  const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
+  const PairWs pw = carve_pair(pair_ws, R, NB, width * height, fast);
+  *v = fgs_frame_view{};
+  v->geom = gw.geom;
+  v->splat = gw.splat;
+  v->pair_count = gw.pcount;
+  v->tile_count = fast ? gw.tcount : nullptr;
+  v->ranges = gw.ranges;
+  v->bucket_off = fast ? gw.boff : nullptr;
+  v->max_contrib = gw.maxc;
+  v->counts = gw.counts;
+  if (pair_ws) {
+    if (fast) {
+      v->sorted_keys = nullptr;
+      v->sorted_vals = pw.v0;
+      v->bucket_ranges = pw.bucket_ranges;
+      v->bucket_tile = pw.bucket_tile;
+      v->acc_t = pw.acc_t;
+      v->acc_c = pw.acc_c;
+    } else {
+      const bool alt = plain_sorted_in_alt(T);
+      v->sorted_keys = alt ? pw.k1 : pw.k0;
+      v->sorted_vals = alt ? pw.v1 : pw.v0;
+    }
+    v->pixel_colors = pw.pixc;
+  }
+  return FGS_OK;
+}
+
+}  // extern "C"

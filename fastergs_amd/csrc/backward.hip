@@ -1,0 +1,170 @@
+// Backward kernels: rasterizer adjoint, per-Gaussian preprocess adjoint and
+// densification statistics.
+//
+// Per-pixel reverse step: FGS/csrc/cuda_rasterizer/backward.cu:549-632.
+// Per-Gaussian reduction then atomics: backward.cu:634-693 (the reference
+// reduces 11 values over a 32-lane warp; here DPP reductions over wave64).
+// Preprocess adjoint: backward.cu:250-539. Statistics: backward.cu:892-918.
+//
+// Raw per-Gaussian sums land in a P x 12 float buffer (fields in
+// fgs_math.h RawField order); the per-Gaussian kernel applies the constant
+// factors and produces the reference's gradient tensors.
+#include <hip/hip_runtime.h>
+
+#include "fgs_device.h"
+#include "fgs_kernels.h"
+#include "fgs_layout.h"
+
+namespace fgs {
+
+// Reduce NF raw fields over the wave and add them to raw[12*id + f] with one
+// atomic instruction (lane f carries field f).
+template <int NF>
+__device__ __forceinline__ void wave_flush(const float* acc, float* raw, uint32_t id) {
+  float s[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) s[f] = wave_sum(acc[f]);
+  const int l = lane_id();
+  float v = s[0];
+#pragma unroll
+  for (int f = 1; f < NF; ++f) v = (l == f) ? s[f] : v;
+  if (l < NF) atomic_add_f(raw + (size_t)RAW_STRIDE * id + l, v);
+}
+
+// ---------------------------------------------------------- PLAIN backward
+// One workgroup per tile, one pixel per thread, reverse over the tile list
+// from the tile's furthest contributor; each pixel starts at T_final with an
+// empty residual and recovers T by division (no checkpoints).
+template <bool ABS>
+__global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
+  constexpr int NF = ABS ? RAW_N : RAW_POW + 1;
+  __shared__ float4 sA[256];
+  __shared__ float4 sB[256];
+  __shared__ float sC[256];
+  __shared__ uint32_t sId[256];
+  const int tile = blockIdx.x;
+  const int tx = tile % a.grid_x, ty = tile / a.grid_x;
+  const int t = threadIdx.x;
+  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
+  const bool inside = px < a.W && py < a.H;
+  const uint32_t start = a.ranges[2 * tile];
+  const int mc = (int)a.maxc[tile];
+  const size_t HW = (size_t)a.W * a.H;
+  const size_t g = inside ? (size_t)py * a.W + px : 0;
+  const int cnt = inside ? a.contrib[g] : 0;
+  float T = inside ? a.tfinal[g] : 0.0f;
+  const float Tf = T;
+  const float dp0 = inside ? a.dimg[g] : 0.0f;
+  const float dp1 = inside ? a.dimg[HW + g] : 0.0f;
+  const float dp2 = inside ? a.dimg[2 * HW + g] : 0.0f;
+  const float bgdot = a.bg[0] * dp0 + a.bg[1] * dp1 + a.bg[2] * dp2;
+  const float fx = (float)px, fy = (float)py;
+  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+  for (int top = mc; top > 0; top -= 256) {
+    const int lo = top > 256 ? top - 256 : 0;
+    const int n = top - lo;
+    __syncthreads();
+    if (t < n) {
+      const uint32_t id = a.vals[start + lo + t];
+      const float4* s = (const float4*)(a.splat + 12 * (size_t)id);
+      sA[t] = s[0];
+      sB[t] = s[1];
+      sC[t] = s[2].x;
+      sId[t] = id;
+    }
+    __syncthreads();
+    for (int j = n - 1; j >= 0; --j) {
+      const bool act = lo + j < cnt;
+      if (__ballot(act) == 0) continue;
+      float acc[RAW_N];
+#pragma unroll
+      for (int f = 0; f < RAW_N; ++f) acc[f] = 0.0f;
+      if (act) {
+        const float4 A = sA[j], Bv = sB[j];
+        pix_bwd(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, sC[j], fx, fy, dp0, dp1, dp2, bgdot, Tf,
+                T, r0, r1, r2, acc);
+      }
+      wave_flush<NF>(acc, a.raw, sId[j]);
+    }
+  }
+}
+
+hipError_t launch_backward_plain(const BwdArgs& a, hipStream_t s) {
+  if (a.T <= 0) return hipSuccess;
+  if (a.abs_grad)
+    k_backward_plain<true><<<a.T, 256, 0, s>>>(a);
+  else
+    k_backward_plain<false><<<a.T, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------- per-Gaussian adjoint
+__global__ __launch_bounds__(256) void k_gauss_bwd(GaussBwdArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.P) return;
+  const float4* r4 = (const float4*)(a.raw + (size_t)RAW_STRIDE * i);
+  float raw[RAW_STRIDE];
+  const float4 q0 = r4[0], q1 = r4[1], q2 = r4[2];
+  raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
+  raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
+  raw[8] = q2.x; raw[9] = q2.y; raw[10] = q2.z; raw[11] = q2.w;
+  Grad2D<float> g2;
+  finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
+  if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
+  ((float4*)a.dmean2D)[i] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
+  ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
+  a.dopacity[i] = g2.opacity;
+  a.dcolor[3 * (size_t)i] = g2.color[0];
+  a.dcolor[3 * (size_t)i + 1] = g2.color[1];
+  a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+  const int M = a.cam.max_coeffs;
+  float* ddc = a.ddc + 3 * (size_t)i;
+  float* dsh = a.dsh ? a.dsh + (size_t)i * M * 3 : nullptr;
+  for (int k = 0; k < 3; ++k) ddc[k] = 0.0f;
+  if (dsh)
+    for (int k = 0; k < 3 * M; ++k) dsh[k] = 0.0f;
+  GaussGrad<float> out;
+  out.dmean3d = a.dmean3D + 3 * (size_t)i;
+  out.dcov3d = a.dcov3D + 6 * (size_t)i;
+  out.dscale = a.dscale + 3 * (size_t)i;
+  out.drot = a.drot + 4 * (size_t)i;
+  out.ddc = ddc;
+  out.dsh = dsh;
+  if (a.radii[i] > 0) {
+    const int clamp = (int)f2u(a.splat[12 * (size_t)i + 10]);
+    gauss_bwd(i, a.G, a.cam, clamp, g2, out);
+  } else {
+    for (int k = 0; k < 3; ++k) out.dmean3d[k] = 0.0f;
+    for (int k = 0; k < 6; ++k) out.dcov3d[k] = 0.0f;
+    for (int k = 0; k < 3; ++k) out.dscale[k] = 0.0f;
+    for (int k = 0; k < 4; ++k) out.drot[k] = 0.0f;
+  }
+}
+
+hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
+  if (a.P > 0) k_gauss_bwd<<<(a.P + 255) / 256, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_densify_stats(int P, const int32_t* __restrict__ radii,
+                                                       const float* __restrict__ dmean2D,
+                                                       float* accum, float* accum_abs,
+                                                       float* max_radii, float* denom) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P || !(radii[i] > 0)) return;
+  const float4 g = ((const float4*)dmean2D)[i];
+  if (denom) denom[i] += 1.0f;
+  if (max_radii) max_radii[i] = fmaxf(max_radii[i], (float)radii[i]);
+  if (accum) accum[i] += sqrtf(g.x * g.x + g.y * g.y);
+  if (accum_abs) accum_abs[i] += sqrtf(g.z * g.z + g.w * g.w);
+}
+
+hipError_t launch_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* accum,
+                                float* accum_abs, float* max_radii, float* denom, hipStream_t s) {
+  if (P > 0)
+    k_densify_stats<<<(P + 255) / 256, 256, 0, s>>>(P, radii, dmean2D, accum, accum_abs, max_radii,
+                                                    denom);
+  return hipGetLastError();
+}
+
+}  // namespace fgs

@@ -1,0 +1,92 @@
+// Device-side helpers for gfx950 (wave64): DPP reductions, lane masks,
+// block scans. Only included by .hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fgs_math.h"
+
+namespace fgs {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Number of set bits of `mask` in lanes below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                               ROW_MASK, BANK_MASK, false));
+}
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, BANK_MASK, false);
+}
+
+// Sum over the 64 lanes with DPP (quad xor 1, quad xor 2, half-row mirror,
+// row mirror, then row_bcast 15 / 31 across rows). The full sum lands in
+// lane 63; wave_sum() broadcasts it as a wave-uniform value.
+__device__ __forceinline__ float wave_sum_to_63(float v) {
+  v += dpp_f<0xb1>(v);        // quad_perm [1,0,3,2]
+  v += dpp_f<0x4e>(v);        // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);       // row_half_mirror
+  v += dpp_f<0x140>(v);       // row_mirror
+  v += dpp_f<0x142, 0xa>(v);  // row_bcast:15 -> rows 1,3
+  v += dpp_f<0x143, 0xc>(v);  // row_bcast:31 -> rows 2,3
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_readlane(__builtin_bit_cast(int, wave_sum_to_63(v)), 63));
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+  v = max(v, dpp_i<0xb1>(v));
+  v = max(v, dpp_i<0x4e>(v));
+  v = max(v, dpp_i<0x141>(v));
+  v = max(v, dpp_i<0x140>(v));
+  v = max(v, dpp_i<0x142, 0xa>(v));
+  v = max(v, dpp_i<0x143, 0xc>(v));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Inclusive prefix sum over the wave (shuffle based; used by scans only).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const uint32_t u = __shfl_up(v, d, WAVE);
+    if (l >= d) v += u;
+  }
+  return v;
+}
+
+// Exclusive scan over a block of NT threads (NT a multiple of 64, <= 1024).
+// Returns the exclusive prefix of `v`; `total` receives the block sum.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
+  __shared__ uint32_t wsum[NT / WAVE];
+  const int w = threadIdx.x / WAVE;
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane_id() == WAVE - 1) wsum[w] = inc;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < NT / WAVE; ++k) {
+    const uint32_t s = wsum[k];
+    before += (k < w) ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return before + inc - v;
+}
+
+__device__ __forceinline__ void atomic_add_f(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+}  // namespace fgs

@@ -1,0 +1,105 @@
+// Host-side launchers shared between the .hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fgs_math.h"
+
+namespace fgs {
+
+// scan_sort.hip
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
+                    hipStream_t s);
+hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                      int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt);
+
+// fill.hip-style helpers (render.hip)
+hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+hipError_t fill_f32(float* p, float v, int64_t n, hipStream_t s);
+
+// preprocess.hip
+struct PreArgs {
+  GaussRef<float> G;
+  CamRef<float> cam;
+  float* geom;
+  float* splat;
+  uint32_t* dkey;
+  uint32_t* pcount;
+  uint32_t* tcount;  // per-tile histogram (fast) or null
+  int32_t* radii;
+  int P;
+  int split_tiles;
+  uint32_t* tasks;
+  uint32_t* counts;
+};
+hipError_t launch_preprocess_plain(const PreArgs& a, hipStream_t s);
+hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
+hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
+                                  const uint32_t* pcount, const uint32_t* poff, int grid_x,
+                                  uint64_t* keys, uint32_t* vals, hipStream_t s);
+hipError_t launch_emit_fast(int P, const float* geom, const uint32_t* order, const uint32_t* pcount,
+                            const uint32_t* poff_by_rank, int grid_x, int split_tiles,
+                            const uint32_t* tasks, const uint32_t* counts, uint64_t* packed,
+                            hipStream_t s);
+
+// render.hip
+hipError_t launch_identify_ranges(int64_t R, const uint64_t* keys, uint32_t* ranges, int T,
+                                  hipStream_t s);
+hipError_t launch_bucket_tables(int T, const uint32_t* ranges, const uint32_t* boff, int bucket,
+                                uint32_t* bucket_ranges, uint32_t* bucket_tile, hipStream_t s);
+hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
+                                hipStream_t s);
+struct CompArgs {
+  int W, H, grid_x, T, bucket;
+  const uint32_t* ranges;
+  const uint32_t* vals;
+  const float* splat;
+  const float* bg;
+  const uint32_t* boff;
+  float* out;
+  float* tfinal;
+  int32_t* contrib;
+  float* pixc;
+  uint32_t* maxc;
+  float* acc_t;
+  float* acc_c;
+};
+hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
+hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);
+
+// backward.hip
+struct BwdArgs {
+  int W, H, grid_x, T, bucket, abs_grad;
+  int64_t NB;
+  const uint32_t* ranges;
+  const uint32_t* vals;
+  const float* splat;
+  const float* bg;
+  const uint32_t* boff;
+  const uint32_t* maxc;
+  const uint32_t* bucket_ranges;
+  const uint32_t* bucket_tile;
+  const float* tfinal;
+  const int32_t* contrib;
+  const float* pixc;
+  const float* acc_t;
+  const float* acc_c;
+  const float* dimg;
+  float* raw;
+};
+hipError_t launch_backward_plain(const BwdArgs& a, hipStream_t s);
+hipError_t launch_backward_fast(const BwdArgs& a, hipStream_t s);
+struct GaussBwdArgs {
+  GaussRef<float> G;
+  CamRef<float> cam;
+  int P;
+  const float* raw;
+  const int32_t* radii;
+  const float* splat;
+  float *dmean2D, *dconic, *dopacity, *dcolor, *dmean3D, *dcov3D, *ddc, *dsh, *dscale, *drot;
+};
+hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s);
+hipError_t launch_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* accum,
+                                float* accum_abs, float* max_radii, float* denom, hipStream_t s);
+
+}  // namespace fgs

@@ -1,0 +1,146 @@
+// Workspace carving shared by the C-ABI entry points (host code).
+//
+// HBM layout (all arrays 256-byte aligned, structure of arrays per stage):
+//   gauss workspace (depends on P and the tile grid T):
+//     geom   P x 32 B  (x, y, conic a, b | conic c, power, rect min, rect max)
+//     splat  P x 48 B  (x, y, ca, cb | cc, log2 op, r, g | b, depth, clamp, -)
+//     dkey / pcount / poff   P x 4 B each (+1 for the offsets' total)
+//     depth-sort keys/values P x (8+8+4+4) B (fast schedule)
+//     tile histogram, ranges, bucket offsets, per-tile max contributor  T x 20 B
+//     counters and scan/sort scratch
+//   pair workspace (depends on pairs R, buckets NB, pixels HW):
+//     keys (2 x R x 8 B), values (2 x R x 4 B), sort histogram scratch,
+//     bucket ranges / owner tile (NB x 12 B), per-bucket checkpoints
+//     (NB x 256 x (4 + 16) B, fast schedule), pixel colours (HW x 16 B).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fgs {
+
+constexpr size_t ALIGN = 256;
+inline size_t align_up(size_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
+
+// Radix sort / scan tiling (must match scan_sort.hip).
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+inline size_t scan_ws_bytes(int64_t n) {
+  const size_t nb = (size_t)((n + SCAN_TILE - 1) / SCAN_TILE);
+  return align_up((nb + 1) * sizeof(uint32_t));
+}
+inline size_t sort_ws_bytes(int64_t n) {
+  const size_t nb = (size_t)((n + SORT_TILE - 1) / SORT_TILE);
+  const size_t hist = 256 * (nb > 0 ? nb : 1);
+  return align_up(hist * sizeof(uint32_t)) + scan_ws_bytes((int64_t)hist) + ALIGN;
+}
+
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* b) : base((char*)b) {}
+  template <typename T>
+  T* take(size_t count) {
+    T* p = base ? (T*)(base + off) : nullptr;
+    off += align_up(count * sizeof(T));
+    return p;
+  }
+};
+
+struct GaussWs {
+  float* geom;
+  float* splat;
+  uint32_t* dkey;
+  uint32_t* pcount;
+  uint32_t* poff;
+  uint64_t* dsort_k0;
+  uint64_t* dsort_k1;
+  uint32_t* dsort_v0;
+  uint32_t* dsort_v1;
+  uint32_t* tcount;
+  uint32_t* ranges;
+  uint32_t* boff;
+  uint32_t* maxc;
+  uint32_t* counts;  // [0] R, [1] NB, [2] visible, [3] large tasks
+  uint32_t* tasks;   // split tasks (fast): P x 2
+  void* scratch;
+  size_t scratch_bytes;
+  size_t total;
+};
+
+inline GaussWs carve_gauss(void* base, int P, int T) {
+  Carver c(base);
+  GaussWs w;
+  const size_t p = (size_t)(P > 0 ? P : 1);
+  const size_t t = (size_t)(T > 0 ? T : 1);
+  w.geom = c.take<float>(8 * p);
+  w.splat = c.take<float>(12 * p);
+  w.dkey = c.take<uint32_t>(p);
+  w.pcount = c.take<uint32_t>(p);
+  w.poff = c.take<uint32_t>(p + 1);
+  w.dsort_k0 = c.take<uint64_t>(p);
+  w.dsort_k1 = c.take<uint64_t>(p);
+  w.dsort_v0 = c.take<uint32_t>(p);
+  w.dsort_v1 = c.take<uint32_t>(p);
+  w.tcount = c.take<uint32_t>(t + 1);
+  w.ranges = c.take<uint32_t>(2 * t);
+  w.boff = c.take<uint32_t>(t + 1);
+  w.maxc = c.take<uint32_t>(t);
+  w.counts = c.take<uint32_t>(16);
+  w.tasks = c.take<uint32_t>(2 * p + 2);
+  size_t sb = sort_ws_bytes((int64_t)p);
+  size_t s2 = scan_ws_bytes((int64_t)p + 1);
+  size_t s3 = scan_ws_bytes((int64_t)t + 1);
+  if (s2 > sb) sb = s2;
+  if (s3 > sb) sb = s3;
+  w.scratch_bytes = sb;
+  w.scratch = c.take<char>(sb);
+  w.total = c.off;
+  return w;
+}
+
+struct PairWs {
+  uint64_t* k0;
+  uint64_t* k1;
+  uint32_t* v0;
+  uint32_t* v1;
+  void* scratch;
+  size_t scratch_bytes;
+  uint32_t* bucket_ranges;
+  uint32_t* bucket_tile;
+  float* acc_t;
+  float* acc_c;
+  float* pixc;
+  size_t total;
+};
+
+inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int HW, bool checkpoints) {
+  Carver c(base);
+  PairWs w;
+  const size_t r = (size_t)(R > 0 ? R : 1);
+  const size_t nb = (size_t)(NB > 0 ? NB : 1);
+  w.k0 = c.take<uint64_t>(r);
+  w.k1 = c.take<uint64_t>(r);
+  w.v0 = c.take<uint32_t>(r);
+  w.v1 = c.take<uint32_t>(r);
+  w.scratch_bytes = sort_ws_bytes((int64_t)r);
+  w.scratch = c.take<char>(w.scratch_bytes);
+  w.bucket_ranges = c.take<uint32_t>(2 * nb);
+  w.bucket_tile = c.take<uint32_t>(nb);
+  if (checkpoints) {
+    w.acc_t = c.take<float>(nb * 256);
+    w.acc_c = c.take<float>(nb * 256 * 4);
+  } else {
+    w.acc_t = nullptr;
+    w.acc_c = nullptr;
+  }
+  w.pixc = c.take<float>((size_t)(HW > 0 ? HW : 1) * 4);
+  w.total = c.off;
+  return w;
+}
+
+}  // namespace fgs

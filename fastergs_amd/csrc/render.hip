@@ -1,0 +1,159 @@
+// Tile ranges, bucket tables and compositing.
+//
+// identifyTileRanges: FGS/csrc/cuda_rasterizer/render.cu:54-79.
+// perTileBucketCount / perBucketRange: render.cu:13-49.
+// Compositing semantics: render.cu:99-123 (alpha, blend), :130-275 (bucket
+// from T=1), :286-391 (merge with the T < 1e-4 cutoff after a bucket).
+//
+// PLAIN compositing: one 256-thread workgroup (4 wave64) per 16x16 tile, one
+// pixel per thread, one running (T, C) per pixel over the whole list, splats
+// staged 256 at a time in LDS, the T < 1e-4 test at every bucket boundary;
+// the workgroup leaves as soon as all of its pixels have stopped.
+#include <hip/hip_runtime.h>
+
+#include "fgs_device.h"
+#include "fgs_kernels.h"
+#include "fgs_layout.h"
+
+namespace fgs {
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, float v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
+  if (n > 0) k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+hipError_t fill_f32(float* p, float v, int64_t n, hipStream_t s) {
+  if (n > 0) k_fill_f32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+__global__ void k_identify_ranges(int64_t R, const uint64_t* __restrict__ keys,
+                                  uint32_t* __restrict__ ranges) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= R) return;
+  const uint32_t cur = (uint32_t)(keys[idx] >> 32);
+  if (idx == 0 || (uint32_t)(keys[idx - 1] >> 32) != cur) ranges[2 * cur] = (uint32_t)idx;
+  if (idx == R - 1 || (uint32_t)(keys[idx + 1] >> 32) != cur) ranges[2 * cur + 1] = (uint32_t)(idx + 1);
+}
+
+hipError_t launch_identify_ranges(int64_t R, const uint64_t* keys, uint32_t* ranges, int T,
+                                  hipStream_t s) {
+  hipError_t e = fill_u32(ranges, 0u, 2 * (int64_t)T, s);
+  if (e != hipSuccess) return e;
+  if (R > 0) k_identify_ranges<<<(unsigned)((R + 255) / 256), 256, 0, s>>>(R, keys, ranges);
+  return hipGetLastError();
+}
+
+__global__ void k_bucket_counts(int T, const uint32_t* __restrict__ ranges, int bucket,
+                                uint32_t* __restrict__ bcount) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= T) return;
+  const uint32_t len = ranges[2 * t + 1] - ranges[2 * t];
+  bcount[t] = (len + bucket - 1) / bucket;
+}
+hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
+                                hipStream_t s) {
+  if (T > 0) k_bucket_counts<<<(T + 255) / 256, 256, 0, s>>>(T, ranges, bucket, bcount);
+  return hipGetLastError();
+}
+
+// boff = inclusive prefix of bucket counts (boff[t] = end bucket of tile t).
+__global__ void k_bucket_tables(int T, const uint32_t* __restrict__ ranges,
+                                const uint32_t* __restrict__ boff, int bucket,
+                                uint32_t* __restrict__ bucket_ranges,
+                                uint32_t* __restrict__ bucket_tile) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= T) return;
+  uint32_t b = t == 0 ? 0u : boff[t - 1];
+  uint32_t s = ranges[2 * t];
+  const uint32_t e = ranges[2 * t + 1];
+  while (s < e) {
+    const uint32_t n = (e - s) < (uint32_t)bucket ? (e - s) : (uint32_t)bucket;
+    bucket_ranges[2 * b] = s;
+    bucket_ranges[2 * b + 1] = s + n;
+    bucket_tile[b] = (uint32_t)t;
+    s += n;
+    ++b;
+  }
+}
+hipError_t launch_bucket_tables(int T, const uint32_t* ranges, const uint32_t* boff, int bucket,
+                                uint32_t* bucket_ranges, uint32_t* bucket_tile, hipStream_t s) {
+  if (T > 0)
+    k_bucket_tables<<<(T + 255) / 256, 256, 0, s>>>(T, ranges, boff, bucket, bucket_ranges,
+                                                    bucket_tile);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------ PLAIN compositing
+__global__ __launch_bounds__(256) void k_composite_plain(CompArgs a) {
+  __shared__ float4 sA[256];
+  __shared__ float4 sB[256];
+  __shared__ float sC[256];
+  __shared__ int wmax[4];
+  const int tile = blockIdx.x;
+  const int tx = tile % a.grid_x, ty = tile / a.grid_x;
+  const int t = threadIdx.x;
+  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
+  const bool inside = px < a.W && py < a.H;
+  const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
+  const float fx = (float)px, fy = (float)py;
+  float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+  int cnt = 0;
+  bool done = !inside;
+  const int B = a.bucket;
+  for (uint32_t base = start; base < end; base += 256) {
+    if (__syncthreads_count(!done) == 0) break;
+    const uint32_t k = base + t;
+    if (k < end) {
+      const float4* s = (const float4*)(a.splat + 12 * (size_t)a.vals[k]);
+      sA[t] = s[0];
+      sB[t] = s[1];
+      sC[t] = s[2].x;
+    }
+    __syncthreads();
+    const int n = (int)min(256u, end - base);
+    if (!done) {
+      for (int j = 0; j < n; j += B) {
+        const int e = min(j + B, n);
+        for (int q = j; q < e; ++q) {
+          const float4 A = sA[q], Bv = sB[q];
+          const float al = splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx, fy);
+          blend(al, Bv.z, Bv.w, sC[q], T, C0, C1, C2);
+        }
+        cnt += e - j;
+        if (T < T_CUT_F) {
+          done = true;
+          break;
+        }
+      }
+    }
+  }
+  if (inside) {
+    const size_t HW = (size_t)a.W * a.H;
+    const size_t g = (size_t)py * a.W + px;
+    a.out[g] = fma_(T, a.bg[0], C0);
+    a.out[HW + g] = fma_(T, a.bg[1], C1);
+    a.out[2 * HW + g] = fma_(T, a.bg[2], C2);
+    a.tfinal[g] = T;
+    a.contrib[g] = cnt;
+    ((float4*)a.pixc)[g] = make_float4(C0, C1, C2, 0.0f);
+  }
+  const int m = wave_max_i(cnt);
+  if (lane_id() == 0) wmax[t / WAVE] = m;
+  __syncthreads();
+  if (t == 0) a.maxc[tile] = (uint32_t)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+}
+
+hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s) {
+  if (a.T > 0) k_composite_plain<<<a.T, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace fgs

@@ -1,0 +1,313 @@
+"""Drop-in Python API of the FasterGS rasterizer on gfx950.
+
+Mirrors gs_submodules/diff-gaussian-rasterization_faster/
+diff_gaussian_rasterization_faster/__init__.py of the FasterGS repository:
+``GaussianRasterizationSettings`` (:191-205), ``GaussianRasterizer`` (:207-264),
+``rasterize_gaussians`` (:41-73) and the autograd function (:76-187) with the
+same argument order, defaults, return values and in-place densification
+statistics. Differences, all deliberate:
+
+* workspaces are per call and kept by the autograd context (the reference
+  shares one static 2.4 GB buffer between calls, __init__.py:21-38, :77, so a
+  second forward before a backward corrupts the first);
+* there is no pair cap: buffers are sized from the exact pair count (the
+  reference raises a ``str`` above 2^24 pairs, __init__.py:118-119);
+* ``cov3D_precomp`` is rejected with ``ValueError`` (the reference accepts it
+  and ignores it, csrc/pybind.cpp:26-35);
+* argument errors raise ``ValueError`` instead of a bare ``Exception``.
+
+Compute runs in libfastergs.so (hand-written gfx950 kernels) through its C
+ABI; PyTorch only provides device memory and the current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import NamedTuple
+
+import torch
+import torch.nn as nn
+
+from . import _lib
+from ._lib import Camera, FrameView, Gaussians, Grads, Options, check, lib
+
+
+@dataclass
+class RasterOptions:
+    """Schedule knobs (not part of the reference settings tuple)."""
+    schedule: int = _lib.SCHEDULE_PLAIN
+    bucket_size: int = 32
+    split_tiles: int = 64
+
+    def to_c(self, abs_grad: bool = True) -> Options:
+        return Options(self.schedule, self.bucket_size, self.split_tiles, 1 if abs_grad else 0)
+
+
+_default_options = RasterOptions()
+
+
+def set_default_options(**kw) -> RasterOptions:
+    for k, v in kw.items():
+        if not hasattr(_default_options, k):
+            raise ValueError(f"unknown option {k}")
+        setattr(_default_options, k, v)
+    return _default_options
+
+
+def get_default_options() -> RasterOptions:
+    return _default_options
+
+
+class GaussianRasterizationSettings(NamedTuple):
+    image_height: int
+    image_width: int
+    tanfovx: float
+    tanfovy: float
+    bg: torch.Tensor
+    scale_modifier: float
+    viewmatrix: torch.Tensor
+    projmatrix: torch.Tensor
+    sh_degree: int
+    campos: torch.Tensor
+    mult: float
+    prefiltered: bool
+    debug: bool
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None or t.numel() == 0 else ctypes.c_void_p(t.data_ptr())
+
+
+def _f32(t: torch.Tensor, device) -> torch.Tensor:
+    return t.to(device=device, dtype=torch.float32).contiguous()
+
+
+def _stream(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class Frame:
+    """One forward evaluation: inputs, workspaces and outputs.
+
+    Owned by the autograd context between forward and backward; tests use it
+    to read intermediate products (sorted lists, ranges, buckets).
+    """
+
+    def __init__(self, settings: GaussianRasterizationSettings, means3D, opacities, scales,
+                 rotations, dc=None, shs=None, colors_precomp=None, culling=None,
+                 options: RasterOptions | None = None, abs_grad: bool = True):
+        if not means3D.is_cuda:
+            raise ValueError("means3D must be a GPU tensor")
+        dev = means3D.device
+        self.device = dev
+        self.opts = options or _default_options
+        self.copts = self.opts.to_c(abs_grad)
+        self.P = P = int(means3D.shape[0])
+        self.W, self.H = int(settings.image_width), int(settings.image_height)
+        self.settings = settings
+        self.means3D = _f32(means3D, dev).reshape(P, 3)
+        self.opacities = _f32(opacities, dev).reshape(P)
+        self.scales = _f32(scales, dev).reshape(P, 3)
+        self.rotations = _f32(rotations, dev).reshape(P, 4)
+        self.colors = None
+        self.dc = self.shs = None
+        self.M = 0
+        if colors_precomp is not None and colors_precomp.numel() > 0:
+            self.colors = _f32(colors_precomp, dev).reshape(P, 3)
+        else:
+            if dc is None or dc.numel() == 0:
+                raise ValueError("dc is required when colours come from SH")
+            self.dc = _f32(dc, dev).reshape(P, 3)
+            if shs is not None and shs.numel() > 0:
+                self.shs = _f32(shs, dev).reshape(P, -1, 3)
+                self.M = int(self.shs.shape[1])
+        self.culling = None
+        if culling is not None and culling.numel() > 0:
+            self.culling = culling.to(device=dev, dtype=torch.bool).contiguous().reshape(P)
+        self.view = _f32(settings.viewmatrix, dev).reshape(16)
+        self.proj = _f32(settings.projmatrix, dev).reshape(16)
+        self.campos = _f32(settings.campos, dev).reshape(3)
+        bg = settings.bg if torch.is_tensor(settings.bg) else torch.tensor(settings.bg)
+        self.bg = _f32(bg, dev).reshape(3)
+        self.cam = Camera(self.W, self.H, float(settings.tanfovx), float(settings.tanfovy),
+                          float(settings.scale_modifier), float(settings.mult),
+                          int(settings.sh_degree), _ptr(self.view), _ptr(self.proj),
+                          _ptr(self.campos), _ptr(self.bg))
+        self.gauss = Gaussians(P, self.M, _ptr(self.means3D), _ptr(self.opacities),
+                               _ptr(self.scales), _ptr(self.rotations), _ptr(self.dc),
+                               _ptr(self.shs), _ptr(self.colors), _ptr(self.culling))
+        self.R = self.NB = 0
+
+    # -- forward stages ---------------------------------------------------
+    def preprocess(self):
+        L, dev = lib(), self.device
+        self.radii = torch.zeros(self.P, dtype=torch.int32, device=dev)
+        n = L.fgs_gauss_ws_size(self.P, self.W, self.H, ctypes.byref(self.copts))
+        self.gws = torch.empty(n, dtype=torch.uint8, device=dev)
+        check(L.fgs_preprocess(ctypes.byref(self.cam), ctypes.byref(self.gauss),
+                               ctypes.byref(self.copts), _ptr(self.gws), n, _ptr(self.radii),
+                               ctypes.c_void_p(_stream(dev))), "fgs_preprocess")
+        counts = (ctypes.c_int64 * 2)()
+        check(L.fgs_read_counts(_ptr(self.gws), self.P, self.W, self.H, ctypes.byref(self.copts),
+                                counts, ctypes.c_void_p(_stream(dev))), "fgs_read_counts")
+        self.R, self.NB = int(counts[0]), int(counts[1])
+        return self
+
+    def render(self):
+        L, dev = lib(), self.device
+        n = L.fgs_pair_ws_size(self.P, self.W, self.H, self.R, self.NB, ctypes.byref(self.copts))
+        self.pws = torch.empty(n, dtype=torch.uint8, device=dev)
+        self.out_color = torch.empty((3, self.H, self.W), dtype=torch.float32, device=dev)
+        self.T_final = torch.empty((self.H, self.W, 1), dtype=torch.float32, device=dev)
+        self.last_contributor = torch.empty((self.H, self.W, 1), dtype=torch.int32, device=dev)
+        check(L.fgs_render(ctypes.byref(self.cam), ctypes.byref(self.gauss),
+                           ctypes.byref(self.copts), _ptr(self.gws), self.gws.numel(),
+                           _ptr(self.pws), n, self.R, self.NB, _ptr(self.out_color),
+                           _ptr(self.T_final), _ptr(self.last_contributor),
+                           ctypes.c_void_p(_stream(dev))), "fgs_render")
+        return self
+
+    def forward(self):
+        return self.preprocess().render()
+
+    # -- backward -----------------------------------------------------------
+    def backward(self, grad_out_color: torch.Tensor) -> dict:
+        L, dev, P = lib(), self.device, self.P
+        g = _f32(grad_out_color, dev).reshape(3, self.H, self.W)
+        e = lambda *s: torch.empty(s, dtype=torch.float32, device=dev)
+        out = {
+            "mean2D": e(P, 4), "conic": e(P, 2, 2), "opacities": e(P, 1), "colors": e(P, 3),
+            "means3D": e(P, 3), "cov3D": e(P, 6), "dc": e(P, 1, 3), "shs": e(P, self.M, 3),
+            "scales": e(P, 3), "rotations": e(P, 4),
+        }
+        grads = Grads(*[_ptr(out[k]) for k in ("mean2D", "conic", "opacities", "colors",
+                                                "means3D", "cov3D", "dc", "shs", "scales",
+                                                "rotations")])
+        gws = torch.empty(L.fgs_grad_ws_size(P), dtype=torch.uint8, device=dev)
+        check(L.fgs_backward(ctypes.byref(self.cam), ctypes.byref(self.gauss),
+                             ctypes.byref(self.copts), _ptr(self.gws), _ptr(self.pws), self.R,
+                             self.NB, _ptr(self.radii), _ptr(self.T_final),
+                             _ptr(self.last_contributor), _ptr(g), _ptr(gws),
+                             ctypes.byref(grads), ctypes.c_void_p(_stream(dev))), "fgs_backward")
+        return out
+
+    def densify_stats(self, dmean2D, accum, accum_abs, max_radii, denom):
+        check(lib().fgs_densify_stats(self.P, _ptr(self.radii), _ptr(dmean2D), _ptr(accum),
+                                      _ptr(accum_abs), _ptr(max_radii), _ptr(denom),
+                                      ctypes.c_void_p(_stream(self.device))),
+              "fgs_densify_stats")
+
+    # -- inspection -----------------------------------------------------------
+    def view_arrays(self) -> dict:
+        """Torch views of the intermediate arrays (valid until the frame dies)."""
+        v = FrameView()
+        check(lib().fgs_frame_view_get(_ptr(self.gws), _ptr(getattr(self, "pws", None)), self.P,
+                                       self.W, self.H, self.R, self.NB, ctypes.byref(self.copts),
+                                       ctypes.byref(v)), "fgs_frame_view_get")
+        T = ((self.W + 15) // 16) * ((self.H + 15) // 16)
+        spec = {
+            "geom": (torch.float32, self.P * 8), "splat": (torch.float32, self.P * 12),
+            "pair_count": (torch.int32, self.P), "tile_count": (torch.int32, T),
+            "ranges": (torch.int32, 2 * T), "bucket_off": (torch.int32, T),
+            "max_contrib": (torch.int32, T), "counts": (torch.int32, 16),
+            "sorted_keys": (torch.int64, self.R), "sorted_vals": (torch.int32, self.R),
+            "bucket_ranges": (torch.int32, 2 * self.NB), "bucket_tile": (torch.int32, self.NB),
+            "acc_t": (torch.float32, self.NB * 256), "acc_c": (torch.float32, self.NB * 256 * 4),
+            "pixel_colors": (torch.float32, self.H * self.W * 4),
+        }
+        res = {}
+        for name, (dt, n) in spec.items():
+            addr = getattr(v, name)
+            if not addr:
+                continue
+            res[name] = _tensor_at(addr, n, dt, self)
+        return res
+
+
+def _tensor_at(addr: int, n: int, dtype, frame: Frame) -> torch.Tensor:
+    """Copy n elements at a device address inside one of the frame's workspaces."""
+    item = torch.tensor([], dtype=dtype).element_size()
+    for buf in (frame.gws, getattr(frame, "pws", None)):
+        if buf is None:
+            continue
+        base = buf.data_ptr()
+        if base <= addr < base + buf.numel():
+            off = addr - base
+            return buf[off:off + n * item].view(dtype).clone()
+    raise ValueError("address outside the frame workspaces")
+
+
+def rasterize_gaussians(means3D, dc, shs, colors_precomp, opacities, scales, rotations,
+                        cov3Ds_precomp, raster_settings, culling, xyz_gradient_accum,
+                        xyz_gradient_accum_abs, max_radii, denom):
+    return _RasterizeGaussians.apply(means3D, dc, shs, colors_precomp, opacities, scales,
+                                     rotations, cov3Ds_precomp, raster_settings, culling,
+                                     xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom)
+
+
+class _RasterizeGaussians(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means3D, dc, shs, colors_precomp, opacities, scales, rotations,
+                cov3Ds_precomp, raster_settings, culling, xyz_gradient_accum,
+                xyz_gradient_accum_abs, max_radii, denom):
+        if cov3Ds_precomp is not None and cov3Ds_precomp.numel() > 0:
+            raise ValueError("cov3D_precomp is not supported: pass scales and rotations")
+        if scales is None or scales.numel() == 0 or rotations is None or rotations.numel() == 0:
+            raise ValueError("scales and rotations are required")
+        want_abs = xyz_gradient_accum_abs is not None and xyz_gradient_accum_abs.numel() > 0
+        frame = Frame(raster_settings, means3D, opacities, scales, rotations, dc=dc, shs=shs,
+                      colors_precomp=colors_precomp, culling=culling, abs_grad=want_abs).forward()
+        ctx.frame = frame
+        ctx.shapes = [t.shape if torch.is_tensor(t) else None for t in
+                      (means3D, dc, shs, colors_precomp, opacities, scales, rotations)]
+        ctx.stats = (xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom)
+        ctx.mark_non_differentiable(frame.radii)
+        return frame.out_color, frame.radii
+
+    @staticmethod
+    def backward(ctx, grad_out_color, _grad_radii):
+        frame: Frame = ctx.frame
+        g = frame.backward(grad_out_color)
+        accum, accum_abs, max_radii, denom = ctx.stats
+        if accum is not None and accum.numel() != 0:  # pybind.cpp:324
+            frame.densify_stats(g["mean2D"], accum, accum_abs, max_radii, denom)
+        ctx.frame = None
+
+        def shaped(t, shape):
+            if shape is not None and t.numel() == shape.numel() and t.numel() > 0:
+                return t.reshape(shape)
+            return t
+
+        s = ctx.shapes
+        return (shaped(g["means3D"], s[0]), shaped(g["dc"], s[1]), shaped(g["shs"], s[2]),
+                shaped(g["colors"], s[3]), shaped(g["opacities"], s[4]),
+                shaped(g["scales"], s[5]), shaped(g["rotations"], s[6]), g["cov3D"],
+                None, None, None, None, None, None)
+
+
+class GaussianRasterizer(nn.Module):
+    def __init__(self, raster_settings, **kwargs):
+        super().__init__()
+        self.raster_settings = raster_settings
+
+    def forward(self, means3D, opacities, dc=None, shs=None, colors_precomp=None, scales=None,
+                rotations=None, cov3D_precomp=None, culling=None, xyz_gradient_accum=None,
+                xyz_gradient_accum_abs=None, max_radii=None, denom=None):
+        if (shs is None and colors_precomp is None) or (shs is not None and colors_precomp is not None):
+            raise ValueError("Please provide excatly one of either SHs or precomputed colors!")
+        if ((scales is None or rotations is None) and cov3D_precomp is None) or \
+                ((scales is not None or rotations is not None) and cov3D_precomp is not None):
+            raise ValueError("Please provide exactly one of either scale/rotation pair or "
+                             "precomputed 3D covariance!")
+        empty = lambda dtype=torch.float32: torch.tensor([], dtype=dtype)
+        return rasterize_gaussians(
+            means3D, dc if dc is not None else empty(), shs if shs is not None else empty(),
+            colors_precomp if colors_precomp is not None else empty(), opacities,
+            scales if scales is not None else empty(),
+            rotations if rotations is not None else empty(),
+            cov3D_precomp if cov3D_precomp is not None else empty(), self.raster_settings,
+            culling if culling is not None else empty(torch.bool),
+            xyz_gradient_accum if xyz_gradient_accum is not None else empty(),
+            xyz_gradient_accum_abs if xyz_gradient_accum_abs is not None else empty(),
+            max_radii if max_radii is not None else empty(),
+            denom if denom is not None else empty())

@@ -1,0 +1,166 @@
+/* C ABI of the gfx950 Gaussian-splatting rasterizer (libfastergs.so).
+ *
+ * This is the boundary that replaces the reference backend's torch binding
+ * (FasterGS gs_submodules/diff-gaussian-rasterization_faster/csrc/pybind.cpp,
+ * the `_C.ops` module of :371-416). Every pointer is a device pointer unless
+ * stated otherwise; no torch types cross it. Entry points never allocate:
+ * callers pass workspaces sized by the *_ws_size queries, and every call
+ * takes the HIP stream to run on. Calls on distinct workspaces are
+ * independent (no device globals). All functions return FGS_OK or a negative
+ * fgs_status; fgs_last_error() gives a message for the calling thread.
+ *
+ * Forward sequence (mirrors _RasterizeGaussians.forward,
+ * diff_gaussian_rasterization_faster/__init__.py:80-155):
+ *   fgs_preprocess -> fgs_read_counts (the one host sync: pairs R, buckets NB)
+ *   -> fgs_render.
+ * Backward (__init__.py:159-187): fgs_backward, then optionally
+ * fgs_densify_stats.
+ */
+#ifndef FASTERGS_H
+#define FASTERGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fgs_status {
+  FGS_OK = 0,
+  FGS_ERR_ARG = -1,      /* invalid argument or shape */
+  FGS_ERR_HIP = -2,      /* a HIP runtime call failed */
+  FGS_ERR_CAPACITY = -3, /* workspace too small for the data */
+} fgs_status;
+
+/* Schedules. PLAIN: one thread per Gaussian, scan + duplicate, 64-bit key
+ * LSD radix sort, per-tile sequential compositing, per-pixel reverse
+ * backward. FAST: wave-packed / split preprocess, depth-presorted emission
+ * with a tile-only radix sort, bucketed compositing with per-tile bucket
+ * cutoff, bucket-checkpointed backward with wave reductions. */
+enum { FGS_SCHEDULE_PLAIN = 0, FGS_SCHEDULE_FAST = 1 };
+
+typedef struct fgs_camera {
+  int width, height;
+  float tanfovx, tanfovy;
+  float scale_modifier;
+  float mult; /* power multiplier of GaussianRasterizationSettings.mult */
+  int sh_degree;
+  const float* viewmatrix; /* 16, column-major world->view (3DGS world_view_transform) */
+  const float* projmatrix; /* 16, column-major full projection */
+  const float* campos;     /* 3 */
+  const float* bg;         /* 3 */
+} fgs_camera;
+
+typedef struct fgs_gaussians {
+  int P;
+  int max_coeffs;             /* rest SH coefficients per Gaussian (shs.size(1)) */
+  const float* means3D;       /* P x 3 */
+  const float* opacities;     /* P (activated) */
+  const float* scales;        /* P x 3 (activated) */
+  const float* rotations;     /* P x 4 (w,x,y,z), used as given */
+  const float* dc;            /* P x 3, required with shs */
+  const float* shs;           /* P x max_coeffs x 3 or NULL */
+  const float* colors_precomp;/* P x 3 or NULL (then SH is used) */
+  const uint8_t* culling;     /* P bool or NULL */
+} fgs_gaussians;
+
+typedef struct fgs_options {
+  int schedule;     /* FGS_SCHEDULE_* */
+  int bucket_size;  /* 16, 32, 64, 128 or 256 */
+  int split_tiles;  /* fast preprocess: rects with >= this many tiles are split */
+  int abs_grad;     /* backward accumulates |dL/dmean2D| terms (needed for xyz_gradient_accum_abs) */
+} fgs_options;
+
+/* Device pointers into the two workspaces, for inspection by tests and
+ * for callers that want the intermediate products. Arrays not produced by
+ * the chosen schedule are NULL. */
+typedef struct fgs_frame_view {
+  float* geom;            /* P x 8: x, y, conic a, b, c, power, rect min (x | y<<16), rect max */
+  float* splat;           /* P x 12: x, y, ca, cb, cc, log2 op, r, g, b, depth, clamp bits, 0 */
+  uint32_t* pair_count;   /* P */
+  uint32_t* tile_count;   /* T */
+  uint32_t* ranges;       /* T x 2 */
+  uint32_t* bucket_off;   /* T, inclusive prefix of per-tile bucket counts */
+  uint32_t* max_contrib;  /* T */
+  uint32_t* counts;       /* [0] pairs R, [1] buckets NB */
+  uint64_t* sorted_keys;  /* R, tile<<32 | depth bits (PLAIN) */
+  uint32_t* sorted_vals;  /* R, per-tile Gaussian lists */
+  uint32_t* bucket_ranges;/* NB x 2 */
+  uint32_t* bucket_tile;  /* NB */
+  float* acc_t;           /* NB x 256 transmittance after each bucket (FAST) */
+  float* acc_c;           /* NB x 256 x 4 colour after each bucket (FAST) */
+  float* pixel_colors;    /* H x W x 4 */
+} fgs_frame_view;
+
+/* Workspace sizes in bytes. The gauss workspace depends on P and the image;
+ * the pair workspace additionally on R and NB from fgs_read_counts. */
+size_t fgs_gauss_ws_size(int P, int width, int height, const fgs_options* opt);
+size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
+                        const fgs_options* opt);
+/* Raw gradient workspace of the backward: P x 12 floats. */
+size_t fgs_grad_ws_size(int P);
+
+/* Replaces faster::preprocess (preprocess.cu:928-962, pybind.cpp:26-76):
+ * projection, covariances, SH colour, tile rects, exact tile/ellipse pair
+ * counts and the per-tile pair histogram. Writes radii (P int32). */
+int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+                   void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream);
+
+/* Copies [R, NB] to host memory (synchronises the stream). Replaces the
+ * reference's curr_offset.cpu() and bucket_sum copies (__init__.py:117,
+ * render.cu:424). */
+int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fgs_options* opt,
+                    int64_t counts_host[2], void* stream);
+
+/* Replaces sort_gaussian + render_16x16 (sort.cu:28-44, render.cu:393-451):
+ * pair emission, sort, tile ranges, bucket tables and compositing. Writes
+ * out_color (3 x H x W), T_final (H x W), last_contributor (H x W int32). */
+int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+               void* gauss_ws, size_t gauss_ws_bytes, void* pair_ws, size_t pair_ws_bytes,
+               int64_t R, int64_t NB, float* out_color, float* T_final,
+               int32_t* last_contributor, void* stream);
+
+/* Replaces faster::render_backward + preprocess_backward (backward.cu:922-977,
+ * pybind.cpp:211-339). dL_dimage is 3 x H x W. Outputs are fully written
+ * (no pre-zeroing needed): dmean2D P x 4, dconic P x 4, dopacity P,
+ * dcolor P x 3, dmean3D P x 3, dcov3D P x 6, ddc P x 3, dsh P x max_coeffs x 3,
+ * dscale P x 3, drot P x 4. grad_ws is fgs_grad_ws_size(P) bytes. */
+typedef struct fgs_grads {
+  float *dmean2D, *dconic, *dopacity, *dcolor, *dmean3D, *dcov3D, *ddc, *dsh, *dscale, *drot;
+} fgs_grads;
+int fgs_backward(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+                 void* gauss_ws, void* pair_ws, int64_t R, int64_t NB, const int32_t* radii,
+                 const float* T_final, const int32_t* last_contributor, const float* dL_dimage,
+                 void* grad_ws, const fgs_grads* out, void* stream);
+
+/* Replaces faster::add_densification_stats (backward.cu:892-918,
+ * pybind.cpp:324-335). Any of the four outputs may be NULL. */
+int fgs_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* xyz_grad_accum,
+                      float* xyz_grad_accum_abs, float* max_radii, float* denom, void* stream);
+
+/* Views of the workspaces' arrays (host-side pointer arithmetic only). */
+int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                       int64_t NB, const fgs_options* opt, fgs_frame_view* view);
+
+/* Stage-level entry points used by tests and benchmarks. */
+/* Stable LSD radix sort of (u64 key, u32 value) pairs on key bits
+ * [begin_bit, end_bit); result lands in keys/vals (the *_alt buffers are
+ * scratch). Replaces cub::DeviceRadixSort::SortPairs (sort.cu:35-39). */
+size_t fgs_sort_ws_size(int64_t n);
+int fgs_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                   int64_t n, int begin_bit, int end_bit, void* ws, size_t ws_bytes,
+                   void* stream);
+/* Exclusive prefix sum of n uint32 (in place allowed); total written to
+ * *total_dev. Replaces cub::DeviceScan (render.cu:420-422). */
+size_t fgs_scan_ws_size(int64_t n);
+int fgs_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, void* ws,
+                 size_t ws_bytes, void* stream);
+
+const char* fgs_last_error(void);
+const char* fgs_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTERGS_H */

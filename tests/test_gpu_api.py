@@ -1,0 +1,118 @@
+"""GPU tests of the drop-in API (GaussianRasterizer / rasterize_gaussians)."""
+import numpy as np
+import pytest
+import torch
+
+from fastergs_amd import scenes
+from fastergs_amd.rasterizer import (GaussianRasterizer, get_default_options,
+                                     set_default_options)
+from oracle.pyoracle import Oracle
+from tests.helpers import grad_close, settings_for, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=[0, 1], ids=["plain", "fast"])
+def schedule(request):
+    old = get_default_options().schedule
+    set_default_options(schedule=request.param)
+    yield request.param
+    set_default_options(schedule=old)
+
+
+def _params(scene):
+    d = to_dev(scene)
+    return {k: v.clone().requires_grad_(True) for k, v in d.items()}
+
+
+def test_autograd_training_step(schedule):
+    scene = scenes.uniform_scene(20000, seed=5)
+    cam = scenes.look_at_camera((0.5, 0.2, -4.0), 320, 240)
+    p = _params(scene)
+    P = scene["means3D"].shape[0]
+    accum = torch.zeros(P, 1, device="cuda")
+    accum_abs = torch.zeros(P, 1, device="cuda")
+    max_radii = torch.zeros(P, device="cuda")
+    denom = torch.zeros(P, 1, device="cuda")
+    ras = GaussianRasterizer(settings_for(cam))
+    img, radii = ras(means3D=p["means3D"], opacities=p["opacities"], dc=p["dc"], shs=p["shs"],
+                     scales=p["scales"], rotations=p["rotations"], xyz_gradient_accum=accum,
+                     xyz_gradient_accum_abs=accum_abs, max_radii=max_radii, denom=denom)
+    assert img.shape == (3, 240, 320) and radii.dtype == torch.int32
+    tgt = torch.from_numpy(scenes.target_image(320, 240)).cuda()
+    loss = ((img - tgt) ** 2).mean()
+    loss.backward()
+    torch.cuda.synchronize()
+    mode = 1 if schedule == 0 else 0
+    orc = Oracle(scene, cam, bucket=32, mode=mode).forward()
+    np.testing.assert_array_equal(img.detach().cpu().numpy().view(np.uint32),
+                                  orc.image().view(np.uint32))
+    dimg = (2.0 * (orc.image() - scenes.target_image(320, 240)) / img.numel()).astype(np.float32)
+    orc.backward(dimg)
+    ref = orc.grads()
+    for k in ("means3D", "dc", "shs", "opacities", "scales", "rotations"):
+        grad_close(p[k].grad.cpu().numpy(), ref[k], name=k)
+    a = np.zeros(P, np.float32); ab = np.zeros(P, np.float32)
+    mr = np.zeros(P, np.float32); dn = np.zeros(P, np.float32)
+    orc.stats(a, ab, mr, dn)
+    np.testing.assert_array_equal(denom.cpu().numpy().ravel(), dn)
+    np.testing.assert_array_equal(max_radii.cpu().numpy(), mr)
+    grad_close(accum.cpu().numpy().ravel(), a, name="xyz_gradient_accum")
+    grad_close(accum_abs.cpu().numpy().ravel(), ab, name="xyz_gradient_accum_abs")
+
+
+def test_colors_precomp_and_culling(schedule):
+    scene = scenes.uniform_scene(8000, seed=6)
+    cam = scenes.look_at_camera((0.0, 0.0, -4.0), 256, 200)
+    rng = np.random.default_rng(0)
+    colors = rng.random((8000, 3)).astype(np.float32)
+    cull = rng.random(8000) < 0.3
+    d = to_dev(scene)
+    col = torch.from_numpy(colors).cuda().requires_grad_(True)
+    op = d["opacities"].clone().requires_grad_(True)
+    ras = GaussianRasterizer(settings_for(cam, bg=(0.2, 0.4, 0.6)))
+    img, radii = ras(means3D=d["means3D"], opacities=op, colors_precomp=col, scales=d["scales"],
+                     rotations=d["rotations"], culling=torch.from_numpy(cull).cuda())
+    (img * img).sum().backward()
+    mode = 1 if schedule == 0 else 0
+    orc = Oracle(scene, cam, bucket=32, mode=mode, colors=colors, culling=cull,
+                 bg=(0.2, 0.4, 0.6)).forward()
+    np.testing.assert_array_equal(img.detach().cpu().numpy().view(np.uint32),
+                                  orc.image().view(np.uint32))
+    assert (radii.cpu().numpy()[cull] == 0).all()
+    orc.backward(2.0 * orc.image())
+    ref = orc.grads()
+    grad_close(col.grad.cpu().numpy(), ref["colors"], name="colors")
+    grad_close(op.grad.cpu().numpy(), ref["opacities"], name="opacities")
+
+
+def test_empty_and_all_culled(schedule):
+    cam = scenes.look_at_camera((0.0, 0.0, -4.0), 100, 70)
+    scene = scenes.uniform_scene(500, seed=7)
+    d = to_dev(scene)
+    bg = (0.1, 0.2, 0.3)
+    ras = GaussianRasterizer(settings_for(cam, bg=bg))
+    # all behind the camera -> culled
+    img, radii = ras(means3D=d["means3D"] * 0 + torch.tensor([0.0, 0.0, -10.0], device="cuda"),
+                     opacities=d["opacities"], dc=d["dc"], shs=d["shs"], scales=d["scales"],
+                     rotations=d["rotations"])
+    assert (radii == 0).all()
+    exp = torch.tensor(bg, device="cuda").view(3, 1, 1).expand(3, 70, 100)
+    assert torch.equal(img, exp)
+    # no Gaussians at all
+    z = lambda *s: torch.zeros(*s, device="cuda")
+    img, radii = ras(means3D=z(0, 3), opacities=z(0, 1), dc=z(0, 1, 3), shs=z(0, 15, 3),
+                     scales=z(0, 3), rotations=z(0, 4))
+    assert radii.numel() == 0 and torch.equal(img, exp)
+
+
+def test_argument_errors():
+    cam = scenes.look_at_camera((0.0, 0.0, -4.0), 64, 64)
+    d = to_dev(scenes.uniform_scene(10, seed=8))
+    ras = GaussianRasterizer(settings_for(cam))
+    with pytest.raises(ValueError):
+        ras(means3D=d["means3D"], opacities=d["opacities"], scales=d["scales"],
+            rotations=d["rotations"])  # neither shs nor colours
+    with pytest.raises(ValueError):
+        ras(means3D=d["means3D"], opacities=d["opacities"], dc=d["dc"], shs=d["shs"],
+            cov3D_precomp=torch.zeros(10, 6, device="cuda"))
