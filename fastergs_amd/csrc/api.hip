@@ -56,6 +56,8 @@ int check_opts(const fgs_options* o) {
   const int b = o->bucket_size;
   if (!(b == 16 || b == 32 || b == 64 || b == 128 || b == 256))
     return fail(FGS_ERR_ARG, "bucket_size must be 16, 32, 64, 128 or 256");
+  if (o->split_tiles < 1 || o->split_tiles > (1 << 24))
+    return fail(FGS_ERR_ARG, "split_tiles must be in [1, 2^24]");
   return FGS_OK;
 }
 
@@ -189,18 +191,36 @@ int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_opti
   a.splat = w.splat;
   a.dkey = w.dkey;
   a.pcount = w.pcount;
-  a.tcount = w.tcount;
   a.radii = radii;
   a.P = g->P;
   a.split_tiles = opt->split_tiles;
-  a.tasks = w.tasks;
+  a.lchunk = w.lchunk;
+  a.dsort_k = w.dsort_k0;
+  a.dsort_v = w.dsort_v0;
   a.counts = w.counts;
   if (opt->schedule == FGS_SCHEDULE_PLAIN) {
     FGS_TRY(launch_preprocess_plain(a, s), "preprocess");
     FGS_TRY(scan_u32(w.pcount, w.poff, g->P, w.counts + 0, w.scratch, s), "pair scan");
-  } else {
-    return fail(FGS_ERR_ARG, "fast schedule not built");
+    return FGS_OK;
   }
+  // FAST: per-Gaussian math + lane-packed exact counts of small rects
+  FGS_TRY(launch_preprocess_fast(a, s), "preprocess");
+  // split tasks of large rects, counted by whole waves
+  FGS_TRY(scan_u32(w.lchunk, w.loff, g->P, w.counts + 3, w.scratch, s), "task scan");
+  FGS_TRY(launch_large_tiles(false, g->P, w.geom, w.loff, w.counts, opt->split_tiles,
+                             grid_x_of(cam->width), w.pcount, nullptr, nullptr, nullptr, s),
+          "large tiles");
+  // Gaussians in (depth, index) order; pair offsets assigned in that order so
+  // that a stable sort by tile alone yields the canonical per-tile order.
+  bool alt = false;
+  FGS_TRY(sort_pairs(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, g->P, 0, 32, w.scratch, s,
+                     alt),
+          "depth sort");
+  if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
+  FGS_TRY(launch_gather_scatter(true, g->P, w.dsort_v0, w.pcount, w.rcount, s), "gather");
+  FGS_TRY(scan_u32(w.rcount, w.roff, g->P, w.counts + 0, w.scratch, s), "pair scan");
+  FGS_TRY(launch_gather_scatter(false, g->P, w.dsort_v0, w.roff, w.poff, s), "scatter");
+  FGS_TRY(launch_nb_capacity(w.counts, opt->bucket_size, T, s), "bucket capacity");
   return FGS_OK;
 }
 
@@ -209,7 +229,7 @@ int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fg
   (void)opt;
   if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
   const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height));
-  uint32_t c[2] = {0, 0};
+  uint32_t c[2] = {0, 0};  // [0] pairs, [1] bucket-table capacity
   hipStream_t s = (hipStream_t)stream;
   FGS_TRY(hipMemcpyAsync(c, w.counts, sizeof(c), hipMemcpyDeviceToHost, s), "read counts");
   FGS_TRY(hipStreamSynchronize(s), "read counts sync");
@@ -234,17 +254,6 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   if (!pair_ws || pair_ws_bytes < pw.total) return fail(FGS_ERR_CAPACITY, "pair workspace too small");
   hipStream_t s = (hipStream_t)stream;
   const int gx = grid_x_of(W);
-  if (fast) return fail(FGS_ERR_ARG, "fast schedule not built");
-
-  // PLAIN: duplicate with keys, 64-bit radix sort, ranges, composite.
-  FGS_TRY(launch_duplicate_plain(g->P, gw.geom, gw.dkey, gw.pcount, gw.poff, gx, pw.k0, pw.v0, s),
-          "duplicate");
-  bool alt = false;
-  FGS_TRY(sort_pairs(pw.k0, pw.v0, pw.k1, pw.v1, R, 0, 32 + tile_bits(T), pw.scratch, s, alt),
-          "sort");
-  const uint64_t* keys = alt ? pw.k1 : pw.k0;
-  const uint32_t* vals = alt ? pw.v1 : pw.v0;
-  FGS_TRY(launch_identify_ranges(R, keys, gw.ranges, T, s), "ranges");
   CompArgs c;
   c.W = W;
   c.H = H;
@@ -252,7 +261,6 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   c.T = T;
   c.bucket = opt->bucket_size;
   c.ranges = gw.ranges;
-  c.vals = vals;
   c.splat = gw.splat;
   c.bg = cam->bg;
   c.boff = gw.boff;
@@ -261,8 +269,40 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   c.contrib = last_contributor;
   c.pixc = pw.pixc;
   c.maxc = gw.maxc;
-  c.acc_t = nullptr;
-  c.acc_c = nullptr;
+  c.acc = pw.acc;
+  if (fast) {
+    // emission in depth order, tile-only stable sort, lists, ranges, buckets
+    FGS_TRY(fill_u32(gw.lcur, 0u, g->P, s), "zero cursors");
+    FGS_TRY(launch_emit_small(g->P, gw.geom, opt->split_tiles, gx, gw.poff, pw.k0, s), "emit");
+    FGS_TRY(launch_large_tiles(true, g->P, gw.geom, gw.loff, gw.counts, opt->split_tiles, gx,
+                               nullptr, gw.poff, gw.lcur, pw.k0, s),
+            "emit large");
+    bool alt = false;
+    FGS_TRY(sort_pairs(pw.k0, nullptr, pw.k1, nullptr, R, 32, 32 + tile_bits(T), pw.scratch, s,
+                       alt),
+            "tile sort");
+    const uint64_t* packed = alt ? pw.k1 : pw.k0;
+    FGS_TRY(launch_unpack_vals(R, packed, pw.v0, s), "unpack");
+    FGS_TRY(launch_identify_ranges(R, packed, gw.ranges, T, s), "ranges");
+    FGS_TRY(launch_bucket_counts(T, gw.ranges, opt->bucket_size, gw.bcount, s), "bucket counts");
+    FGS_TRY(scan_u32(gw.bcount, gw.boff, T, gw.boff + T, gw.scratch, s), "bucket scan");
+    FGS_TRY(launch_bucket_tables(T, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
+                                 pw.bucket_tile, s),
+            "bucket tables");
+    c.vals = pw.v0;
+    FGS_TRY(launch_composite_fast(c, s), "composite");
+    return FGS_OK;
+  }
+
+  // PLAIN: duplicate with keys, 64-bit radix sort, ranges, composite.
+  FGS_TRY(launch_duplicate_plain(g->P, gw.geom, gw.dkey, gw.pcount, gw.poff, gx, pw.k0, pw.v0, s),
+          "duplicate");
+  bool alt = false;
+  FGS_TRY(sort_pairs(pw.k0, pw.v0, pw.k1, pw.v1, R, 0, 32 + tile_bits(T), pw.scratch, s, alt),
+          "sort");
+  const uint64_t* keys = alt ? pw.k1 : pw.k0;
+  c.vals = alt ? pw.v1 : pw.v0;
+  FGS_TRY(launch_identify_ranges(R, keys, gw.ranges, T, s), "ranges");
   FGS_TRY(launch_composite_plain(c, s), "composite");
   return FGS_OK;
 }
@@ -301,13 +341,19 @@ int fgs_backward(const fgs_camera* cam, const fgs_gaussians* g, const fgs_option
   b.tfinal = T_final;
   b.contrib = last_contributor;
   b.pixc = pw.pixc;
-  b.acc_t = pw.acc_t;
-  b.acc_c = pw.acc_c;
+  b.acc = pw.acc;
+  b.actoff = gw.actoff;
   b.dimg = dL_dimage;
   b.raw = raw;
-  if (fast) return fail(FGS_ERR_ARG, "fast schedule not built");
-  b.vals = plain_sorted_in_alt(T) ? pw.v1 : pw.v0;
-  FGS_TRY(launch_backward_plain(b, s), "raster backward");
+  if (fast) {
+    b.vals = pw.v0;
+    FGS_TRY(launch_active_counts(T, gw.maxc, opt->bucket_size, gw.bcount, s), "active buckets");
+    FGS_TRY(scan_u32(gw.bcount, gw.actoff, T, gw.actoff + T, gw.scratch, s), "active scan");
+    FGS_TRY(launch_backward_fast(b, s), "raster backward");
+  } else {
+    b.vals = plain_sorted_in_alt(T) ? pw.v1 : pw.v0;
+    FGS_TRY(launch_backward_plain(b, s), "raster backward");
+  }
   GaussBwdArgs q;
   q.G = make_gauss(g);
   q.cam = make_cam(cam, g);
@@ -351,9 +397,9 @@ int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int heig
   v->geom = gw.geom;
   v->splat = gw.splat;
   v->pair_count = gw.pcount;
-  v->tile_count = fast ? gw.tcount : nullptr;
+  v->tile_count = nullptr;
   v->ranges = gw.ranges;
-  v->bucket_off = fast ? gw.boff : nullptr;
+  v->bucket_off = fast ? gw.boff + 1 : nullptr;
   v->max_contrib = gw.maxc;
   v->counts = gw.counts;
   if (pair_ws) {
@@ -362,8 +408,8 @@ int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int heig
       v->sorted_vals = pw.v0;
       v->bucket_ranges = pw.bucket_ranges;
       v->bucket_tile = pw.bucket_tile;
-      v->acc_t = pw.acc_t;
-      v->acc_c = pw.acc_c;
+      v->acc_t = nullptr;
+      v->acc_c = pw.acc;
     } else {
       const bool alt = plain_sorted_in_alt(T);
       v->sorted_keys = alt ? pw.k1 : pw.k0;

@@ -98,6 +98,104 @@ hipError_t launch_backward_plain(const BwdArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ----------------------------------------------------------- FAST backward
+// One wave64 per needed bucket (backward.cu:695-878): the buckets of each
+// tile below its furthest contributor form a compact task list (prefix over
+// tiles, `actoff`), taken by persistent waves in a grid-stride loop. A wave
+// covers all 256 pixels of the tile (4 per lane), restarts each pixel from
+// the checkpoint stored after this bucket (T, C) and walks the bucket's
+// splats in reverse; the splat records are wave-uniform scalar loads, the
+// next one prefetched while the current one is processed. Per splat the 4
+// pixel contributions are summed in registers, then over the wave with DPP,
+// and one atomic instruction adds the 9 (or 11) raw sums.
+template <bool ABS>
+__global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
+  constexpr int NF = ABS ? RAW_N : RAW_POW + 1;
+  const uint32_t A = a.actoff[a.T];
+  const int lane = lane_id();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / WAVE);
+  const uint32_t nwaves = gridDim.x * 4;
+  const size_t HW = (size_t)a.W * a.H;
+  const int qx = 2 * (lane & 7), qy = 2 * (lane >> 3);
+  for (uint32_t u = wave; u < A; u += nwaves) {
+    int lo = 0, hi = a.T;  // actoff[lo] <= u < actoff[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.actoff[mid] <= u) lo = mid; else hi = mid;
+    }
+    const int tile = lo;
+    const uint32_t lb = u - a.actoff[tile];
+    const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
+    const uint32_t s = start + lb * (uint32_t)a.bucket;
+    const int n = (int)min((uint32_t)a.bucket, end - s);
+    const size_t bq = (size_t)(a.boff[tile] + lb) * TILE_PIX;
+    const int tx = tile % a.grid_x, ty = tile / a.grid_x;
+    float T[4], r0[4], r1[4], r2[4], d0[4], d1[4], d2[4], bgd[4], Tf[4], fx[4], fy[4];
+    bool act[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int lx = qx + (k & 1), ly = qy + (k >> 1);
+      const int px = tx * TILE_W + lx, py = ty * TILE_H + ly;
+      fx[k] = (float)px;
+      fy[k] = (float)py;
+      const bool inside = px < a.W && py < a.H;
+      const size_t g = inside ? (size_t)py * a.W + px : 0;
+      act[k] = inside && lb * (uint32_t)a.bucket < (uint32_t)a.contrib[g];
+      T[k] = 1.0f; r0[k] = r1[k] = r2[k] = 0.0f; d0[k] = d1[k] = d2[k] = 0.0f;
+      bgd[k] = 0.0f; Tf[k] = 0.0f;
+      if (act[k]) {
+        const float4 ck = ((const float4*)a.acc)[bq + ly * TILE_W + lx];
+        const float4 pc = ((const float4*)a.pixc)[g];
+        T[k] = ck.w;
+        r0[k] = pc.x - ck.x;
+        r1[k] = pc.y - ck.y;
+        r2[k] = pc.z - ck.z;
+        d0[k] = a.dimg[g];
+        d1[k] = a.dimg[HW + g];
+        d2[k] = a.dimg[2 * HW + g];
+        bgd[k] = a.bg[0] * d0[k] + a.bg[1] * d1[k] + a.bg[2] * d2[k];
+        Tf[k] = a.tfinal[g];
+      }
+    }
+    uint32_t id = a.vals[s + n - 1];
+    const float4* rp = (const float4*)(a.splat + 12 * (size_t)id);
+    float4 A0 = rp[0], B0 = rp[1];
+    float C0 = rp[2].x;
+    for (int k = n - 1; k >= 0; --k) {
+      // prefetch the previous splat of the bucket
+      const uint32_t idn = k > 0 ? a.vals[s + k - 1] : id;
+      const float4* rn = (const float4*)(a.splat + 12 * (size_t)idn);
+      const float4 A1 = rn[0], B1 = rn[1];
+      const float C1 = rn[2].x;
+      float acc[RAW_N];
+#pragma unroll
+      for (int f = 0; f < RAW_N; ++f) acc[f] = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (act[q])
+          pix_bwd(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q], fy[q], d0[q], d1[q],
+                  d2[q], bgd[q], Tf[q], T[q], r0[q], r1[q], r2[q], acc);
+      }
+      wave_flush<NF>(acc, a.raw, id);
+      id = idn;
+      A0 = A1;
+      B0 = B1;
+      C0 = C1;
+    }
+  }
+}
+
+hipError_t launch_backward_fast(const BwdArgs& a, hipStream_t s) {
+  if (a.T <= 0 || a.NB <= 0) return hipSuccess;
+  const int64_t want = (a.NB + 3) / 4;
+  const int blocks = (int)(want < 4096 ? want : 4096);
+  if (a.abs_grad)
+    k_backward_fast<true><<<blocks, 256, 0, s>>>(a);
+  else
+    k_backward_fast<false><<<blocks, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
 // --------------------------------------------------- per-Gaussian adjoint
 __global__ __launch_bounds__(256) void k_gauss_bwd(GaussBwdArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;

@@ -25,11 +25,12 @@ struct PreArgs {
   float* splat;
   uint32_t* dkey;
   uint32_t* pcount;
-  uint32_t* tcount;  // per-tile histogram (fast) or null
   int32_t* radii;
   int P;
   int split_tiles;
-  uint32_t* tasks;
+  uint32_t* lchunk;    // fast: split tasks per Gaussian (0 for small rects)
+  uint64_t* dsort_k;   // fast: depth keys for the Gaussian depth sort
+  uint32_t* dsort_v;   // fast: Gaussian indices for the depth sort
   uint32_t* counts;
 };
 hipError_t launch_preprocess_plain(const PreArgs& a, hipStream_t s);
@@ -37,10 +38,14 @@ hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
                                   uint64_t* keys, uint32_t* vals, hipStream_t s);
-hipError_t launch_emit_fast(int P, const float* geom, const uint32_t* order, const uint32_t* pcount,
-                            const uint32_t* poff_by_rank, int grid_x, int split_tiles,
-                            const uint32_t* tasks, const uint32_t* counts, uint64_t* packed,
-                            hipStream_t s);
+hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* loff,
+                              const uint32_t* counts, int split, int gx, uint32_t* pcount,
+                              const uint32_t* poff, uint32_t* lcur, uint64_t* packed,
+                              hipStream_t s);
+hipError_t launch_emit_small(int P, const float* geom, int split, int gx, const uint32_t* poff,
+                             uint64_t* packed, hipStream_t s);
+hipError_t launch_gather_scatter(bool gather, int P, const uint32_t* order, const uint32_t* src,
+                                 uint32_t* dst, hipStream_t s);
 
 // render.hip
 hipError_t launch_identify_ranges(int64_t R, const uint64_t* keys, uint32_t* ranges, int T,
@@ -49,20 +54,23 @@ hipError_t launch_bucket_tables(int T, const uint32_t* ranges, const uint32_t* b
                                 uint32_t* bucket_ranges, uint32_t* bucket_tile, hipStream_t s);
 hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
                                 hipStream_t s);
+hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
+                                hipStream_t s);
+hipError_t launch_unpack_vals(int64_t R, const uint64_t* packed, uint32_t* vals, hipStream_t s);
+hipError_t launch_nb_capacity(uint32_t* counts, int bucket, int T, hipStream_t s);
 struct CompArgs {
   int W, H, grid_x, T, bucket;
   const uint32_t* ranges;
   const uint32_t* vals;
   const float* splat;
   const float* bg;
-  const uint32_t* boff;
+  const uint32_t* boff;  // exclusive bucket offsets (fast)
   float* out;
   float* tfinal;
   int32_t* contrib;
   float* pixc;
   uint32_t* maxc;
-  float* acc_t;
-  float* acc_c;
+  float* acc;            // fast checkpoints
 };
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);
@@ -82,8 +90,8 @@ struct BwdArgs {
   const float* tfinal;
   const int32_t* contrib;
   const float* pixc;
-  const float* acc_t;
-  const float* acc_c;
+  const float* acc;
+  const uint32_t* actoff;  // fast: exclusive prefix of active buckets per tile, [T] = total
   const float* dimg;
   float* raw;
 };

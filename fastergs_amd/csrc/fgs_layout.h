@@ -57,16 +57,21 @@ struct GaussWs {
   uint32_t* dkey;
   uint32_t* pcount;
   uint32_t* poff;
-  uint64_t* dsort_k0;
+  uint64_t* dsort_k0;  // fast: depth keys, sorted in place
   uint64_t* dsort_k1;
-  uint32_t* dsort_v0;
+  uint32_t* dsort_v0;  // fast: Gaussian index by depth rank after the sort
   uint32_t* dsort_v1;
-  uint32_t* tcount;
-  uint32_t* ranges;
-  uint32_t* boff;
-  uint32_t* maxc;
-  uint32_t* counts;  // [0] R, [1] NB, [2] visible, [3] large tasks
-  uint32_t* tasks;   // split tasks (fast): P x 2
+  uint32_t* lchunk;    // fast: split tasks per Gaussian
+  uint32_t* loff;      // fast: exclusive prefix of lchunk
+  uint32_t* lcur;      // fast: per-Gaussian emission cursor of split tasks
+  uint32_t* rcount;    // fast: pair counts in depth order
+  uint32_t* roff;      // fast: their exclusive prefix
+  uint32_t* ranges;    // T x 2
+  uint32_t* boff;      // T + 1: exclusive prefix of per-tile bucket counts, [T] = NB
+  uint32_t* bcount;    // T + 1 scratch (bucket counts, then active bucket counts)
+  uint32_t* actoff;    // T + 1: exclusive prefix of per-tile active buckets, [T] = total
+  uint32_t* maxc;      // T
+  uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] split tasks
   void* scratch;
   size_t scratch_bytes;
   size_t total;
@@ -86,12 +91,17 @@ inline GaussWs carve_gauss(void* base, int P, int T) {
   w.dsort_k1 = c.take<uint64_t>(p);
   w.dsort_v0 = c.take<uint32_t>(p);
   w.dsort_v1 = c.take<uint32_t>(p);
-  w.tcount = c.take<uint32_t>(t + 1);
+  w.lchunk = c.take<uint32_t>(p);
+  w.loff = c.take<uint32_t>(p + 1);
+  w.lcur = c.take<uint32_t>(p);
+  w.rcount = c.take<uint32_t>(p);
+  w.roff = c.take<uint32_t>(p + 1);
   w.ranges = c.take<uint32_t>(2 * t);
   w.boff = c.take<uint32_t>(t + 1);
+  w.bcount = c.take<uint32_t>(t + 1);
+  w.actoff = c.take<uint32_t>(t + 1);
   w.maxc = c.take<uint32_t>(t);
   w.counts = c.take<uint32_t>(16);
-  w.tasks = c.take<uint32_t>(2 * p + 2);
   size_t sb = sort_ws_bytes((int64_t)p);
   size_t s2 = scan_ws_bytes((int64_t)p + 1);
   size_t s3 = scan_ws_bytes((int64_t)t + 1);
@@ -104,21 +114,20 @@ inline GaussWs carve_gauss(void* base, int P, int T) {
 }
 
 struct PairWs {
-  uint64_t* k0;
+  uint64_t* k0;  // plain: keys; fast: packed tile<<32 | index
   uint64_t* k1;
-  uint32_t* v0;
+  uint32_t* v0;  // sorted per-tile Gaussian lists (fast), values (plain)
   uint32_t* v1;
   void* scratch;
   size_t scratch_bytes;
   uint32_t* bucket_ranges;
   uint32_t* bucket_tile;
-  float* acc_t;
-  float* acc_c;
-  float* pixc;
+  float* acc;    // fast: NB x 256 x (C0, C1, C2, T) after each bucket
+  float* pixc;   // HW x (C0, C1, C2, 0)
   size_t total;
 };
 
-inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int HW, bool checkpoints) {
+inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int HW, bool fast) {
   Carver c(base);
   PairWs w;
   const size_t r = (size_t)(R > 0 ? R : 1);
@@ -126,17 +135,17 @@ inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int HW, bool checkpo
   w.k0 = c.take<uint64_t>(r);
   w.k1 = c.take<uint64_t>(r);
   w.v0 = c.take<uint32_t>(r);
-  w.v1 = c.take<uint32_t>(r);
+  w.v1 = fast ? nullptr : c.take<uint32_t>(r);
   w.scratch_bytes = sort_ws_bytes((int64_t)r);
   w.scratch = c.take<char>(w.scratch_bytes);
-  w.bucket_ranges = c.take<uint32_t>(2 * nb);
-  w.bucket_tile = c.take<uint32_t>(nb);
-  if (checkpoints) {
-    w.acc_t = c.take<float>(nb * 256);
-    w.acc_c = c.take<float>(nb * 256 * 4);
+  if (fast) {
+    w.bucket_ranges = c.take<uint32_t>(2 * nb);
+    w.bucket_tile = c.take<uint32_t>(nb);
+    w.acc = c.take<float>(nb * 256 * 4);
   } else {
-    w.acc_t = nullptr;
-    w.acc_c = nullptr;
+    w.bucket_ranges = nullptr;
+    w.bucket_tile = nullptr;
+    w.acc = nullptr;
   }
   w.pixc = c.take<float>((size_t)(HW > 0 ? HW : 1) * 4);
   w.total = c.off;

@@ -89,17 +89,22 @@ FGS_HD double ceil_(double a) { return ceil(a); }
 FGS_HD int trunc_i(float v) { return (int)fminf(fmaxf(v, -1e9f), 1e9f); }
 FGS_HD int trunc_i(double v) { return (int)fmin(fmax(v, -1e9), 1e9); }
 
+// Exact scaling by 2^e for a normal result (v_ldexp_f32 on the GPU).
+#if defined(__HIP_DEVICE_COMPILE__)
+FGS_HD float ldexp_(float v, int e) { return __builtin_amdgcn_ldexpf(v, e); }
+#else
+FGS_HD float ldexp_(float v, int e) { return ldexpf(v, e); }
+#endif
+
 // 2^x for float. Method: Cody-Waite style reduction x = n + f with
 // n = rint(x), f in [-1/2, 1/2] (the subtraction is exact), then a degree-6
 // polynomial for 2^f whose coefficients were fitted by iteratively
 // re-weighted least squares at Chebyshev nodes (Lawson's algorithm, which
 // converges to the minimax fit), evaluated by Horner's rule with fused
-// multiply-adds, and an exact scaling by 2^n built in the exponent field.
-// Maximum error on the reduced range is below 1 ulp. Results below 2^-125
-// flush to 0, which only the alpha < 1/255 threshold ever sees.
-FGS_HD float exp2_(float x) {
-  if (!(x >= -125.0f)) return (x != x) ? x : 0.0f;
-  if (x > 127.0f) return u2f(0x7f800000u);
+// multiply-adds, and an exact scaling by 2^n. Maximum error on the reduced
+// range is below 1 ulp. exp2_core needs x in [-125, 127]; exp2_ handles the
+// rest (flush to 0 below 2^-125, inf above 2^127, NaN propagates).
+FGS_HD float exp2_core(float x) {
   const float n = rintf(x);
   const float f = x - n;
   float p = 1.534580806037411e-4f;
@@ -109,8 +114,12 @@ FGS_HD float exp2_(float x) {
   p = fma_(p, f, 2.4022646248340607e-1f);
   p = fma_(p, f, 6.931471824645996e-1f);
   p = fma_(p, f, 1.0f);
-  const int e = (int)n;
-  return p * u2f((uint32_t)(e + 127) << 23);
+  return ldexp_(p, (int)n);
+}
+FGS_HD float exp2_(float x) {
+  if (!(x >= -125.0f)) return (x != x) ? x : 0.0f;
+  if (x > 127.0f) return u2f(0x7f800000u);
+  return exp2_core(x);
 }
 FGS_HD double exp2_(double x) { return exp2(x); }
 
@@ -432,12 +441,25 @@ FGS_HD void log2_conic(const R conic[3], R log2op, R& ca, R& cb, R& cc, R& w) {
 }
 
 // Alpha of a splat at integer pixel (px,py) (render.cu:99-110).
+// The exponent is clamped to [-9, 0] before exp2: every p > 0 gives alpha =
+// 0.99 and every p < -9 gives alpha < 1/255 -> 0 either way, so the result
+// equals min(0.99, 2^p) thresholded at 1/255 for all p (NaN included, which
+// becomes 0.99 like fminf(0.99, NaN)), and exp2 needs no range checks.
+FGS_HD float alpha_of(float p) {
+  const float pc = fmaxf(fminf(p, 0.0f), -9.0f);
+  const float a = fminf(ALPHA_CAP_F, exp2_core(pc));
+  return a < 1.0f / 255.0f ? 0.0f : a;
+}
+FGS_HD double alpha_of(double p) {
+  const double a = fmin(0.99, exp2(p));
+  return a < 1.0 / 255.0 ? 0.0 : a;
+}
+
 template <typename R>
 FGS_HD R splat_alpha(R gx, R gy, R ca, R cb, R cc, R w, R px, R py) {
   const R dx = gx - px, dy = gy - py;
   const R p = fma_(cb * dx, dy, fma_(cc * dy, dy, fma_(ca * dx, dx, w)));
-  const R a = min_(R(ALPHA_CAP_F), exp2_(p));
-  return a < R(1) / R(255) ? R(0) : a;
+  return alpha_of(p);
 }
 
 // Front-to-back blend of one splat (render.cu:112-115).
@@ -481,8 +503,7 @@ FGS_HD void pix_bwd(R gx, R gy, R ca, R cb, R cc, R w, R r, R g, R b, R px, R py
                     R& res2, R* acc) {
   const R dx = gx - px, dy = gy - py;
   const R p = fma_(cb * dx, dy, fma_(cc * dy, dy, fma_(ca * dx, dx, w)));
-  R a = min_(R(ALPHA_CAP_F), exp2_(p));
-  if (a < R(1) / R(255)) a = R(0);
+  const R a = alpha_of(p);
   const R inv = R(1) / (R(1) - a);
   T = min_(T * inv, R(1));
   const R wgt = a * T;

@@ -64,14 +64,14 @@ hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint3
   return hipGetLastError();
 }
 
-// boff = inclusive prefix of bucket counts (boff[t] = end bucket of tile t).
+// boff = exclusive prefix of bucket counts (boff[t] = first bucket of tile t).
 __global__ void k_bucket_tables(int T, const uint32_t* __restrict__ ranges,
                                 const uint32_t* __restrict__ boff, int bucket,
                                 uint32_t* __restrict__ bucket_ranges,
                                 uint32_t* __restrict__ bucket_tile) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= T) return;
-  uint32_t b = t == 0 ? 0u : boff[t - 1];
+  uint32_t b = boff[t];
   uint32_t s = ranges[2 * t];
   const uint32_t e = ranges[2 * t + 1];
   while (s < e) {
@@ -88,6 +88,39 @@ hipError_t launch_bucket_tables(int T, const uint32_t* ranges, const uint32_t* b
   if (T > 0)
     k_bucket_tables<<<(T + 255) / 256, 256, 0, s>>>(T, ranges, boff, bucket, bucket_ranges,
                                                     bucket_tile);
+  return hipGetLastError();
+}
+
+// Buckets of each tile that some pixel still needs in the backward:
+// ceil(max_contrib / B) (backward.cu:721-722 skips the others).
+__global__ void k_active_counts(int T, const uint32_t* __restrict__ maxc, int bucket,
+                                uint32_t* __restrict__ out) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < T) out[t] = (maxc[t] + bucket - 1) / bucket;
+}
+hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
+                                hipStream_t s) {
+  if (T > 0) k_active_counts<<<(T + 255) / 256, 256, 0, s>>>(T, maxc, bucket, out);
+  return hipGetLastError();
+}
+
+__global__ void k_unpack_vals(int64_t R, const uint64_t* __restrict__ packed,
+                              uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < R) vals[i] = (uint32_t)packed[i];
+}
+hipError_t launch_unpack_vals(int64_t R, const uint64_t* packed, uint32_t* vals, hipStream_t s) {
+  if (R > 0) k_unpack_vals<<<(unsigned)((R + 255) / 256), 256, 0, s>>>(R, packed, vals);
+  return hipGetLastError();
+}
+
+// Capacity for the bucket tables before the per-tile counts exist:
+// sum_t ceil(len_t / B) <= ceil(R / B) + T.
+__global__ void k_nb_capacity(uint32_t* counts, int bucket, int T) {
+  if (threadIdx.x == 0) counts[1] = (counts[0] + bucket - 1) / bucket + (uint32_t)T;
+}
+hipError_t launch_nb_capacity(uint32_t* counts, int bucket, int T, hipStream_t s) {
+  k_nb_capacity<<<1, 64, 0, s>>>(counts, bucket, T);
   return hipGetLastError();
 }
 
@@ -153,6 +186,113 @@ __global__ __launch_bounds__(256) void k_composite_plain(CompArgs a) {
 
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s) {
   if (a.T > 0) k_composite_plain<<<a.T, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------- FAST compositing
+// One 256-thread workgroup (4 wave64) per 16x16 tile. Each round the four
+// waves composite four consecutive buckets of the tile at once, every wave
+// over all 256 pixels (4 pixels per lane, 2x2 quads) starting from T = 1
+// (render.cu:130-275); the bucket splats are staged in LDS. The partial
+// (T_b, C_b) of the four buckets are then merged in order per pixel
+// (render.cu:329-362), the checkpoint after each bucket is stored for the
+// backward, and a pixel stops after the first bucket that leaves T < 1e-4.
+// The workgroup stops at the first round in which every pixel has stopped,
+// so buckets past the tile's cutoff (which the reference renders and then
+// discards) are never evaluated; the outputs are unchanged by this.
+template <int BMAX>
+__global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
+  __shared__ float4 sA[4][BMAX];
+  __shared__ float4 sB[4][BMAX];
+  __shared__ float sC[4][BMAX];
+  __shared__ float4 sP[4][TILE_PIX];
+  __shared__ int wmax[4];
+  const int tile = blockIdx.x;
+  const int tx = tile % a.grid_x, ty = tile / a.grid_x;
+  const int t = threadIdx.x;
+  const int w = t / WAVE, lane = t & (WAVE - 1);
+  // merge / output pixel of this thread
+  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
+  const bool inside = px < a.W && py < a.H;
+  // the 2x2 quad of this lane in the bucket pass
+  const int qx = 2 * (lane & 7), qy = 2 * (lane >> 3);
+  const float fx0 = (float)(tx * TILE_W + qx), fy0 = (float)(ty * TILE_H + qy);
+  const float fx1 = fx0 + 1.0f, fy1 = fy0 + 1.0f;
+  const int lp0 = qy * TILE_W + qx;
+  const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
+  const int B = a.bucket;
+  const int nb = (int)((end - start + B - 1) / B);
+  const uint32_t b0 = a.boff[tile];
+  float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+  int cnt = 0;
+  bool done = !inside;
+  for (int round = 0; round * 4 < nb; ++round) {
+    if (__syncthreads_count(!done) == 0) break;
+    const int bl = round * 4 + w;
+    const uint32_t s = start + (uint32_t)bl * B;
+    const int n = bl < nb ? (int)min((uint32_t)B, end - s) : 0;
+    for (int k = lane; k < n; k += WAVE) {
+      const float4* r = (const float4*)(a.splat + 12 * (size_t)a.vals[s + k]);
+      sA[w][k] = r[0];
+      sB[w][k] = r[1];
+      sC[w][k] = r[2].x;
+    }
+    __syncthreads();
+    if (n > 0) {
+      float T0 = 1.f, T1 = 1.f, T2 = 1.f, T3 = 1.f;
+      float c00 = 0.f, c01 = 0.f, c02 = 0.f, c10 = 0.f, c11 = 0.f, c12 = 0.f;
+      float c20 = 0.f, c21 = 0.f, c22 = 0.f, c30 = 0.f, c31 = 0.f, c32 = 0.f;
+      for (int k = 0; k < n; ++k) {
+        const float4 A = sA[w][k], Bv = sB[w][k];
+        const float cb = sC[w][k];
+        blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx0, fy0), Bv.z, Bv.w, cb, T0, c00, c01, c02);
+        blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx1, fy0), Bv.z, Bv.w, cb, T1, c10, c11, c12);
+        blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx0, fy1), Bv.z, Bv.w, cb, T2, c20, c21, c22);
+        blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx1, fy1), Bv.z, Bv.w, cb, T3, c30, c31, c32);
+      }
+      sP[w][lp0] = make_float4(c00, c01, c02, T0);
+      sP[w][lp0 + 1] = make_float4(c10, c11, c12, T1);
+      sP[w][lp0 + TILE_W] = make_float4(c20, c21, c22, T2);
+      sP[w][lp0 + TILE_W + 1] = make_float4(c30, c31, c32, T3);
+    }
+    __syncthreads();
+    if (!done) {
+      for (int q = 0; q < 4; ++q) {
+        const int bq = round * 4 + q;
+        if (bq >= nb) break;
+        const float4 v = sP[q][t];
+        merge(v.w, v.x, v.y, v.z, T, C0, C1, C2);
+        ((float4*)a.acc)[(size_t)(b0 + bq) * TILE_PIX + t] = make_float4(C0, C1, C2, T);
+        cnt += (int)min((uint32_t)B, end - (start + (uint32_t)bq * B));
+        if (T < T_CUT_F) {
+          done = true;
+          break;
+        }
+      }
+    }
+  }
+  if (inside) {
+    const size_t HW = (size_t)a.W * a.H;
+    const size_t g = (size_t)py * a.W + px;
+    a.out[g] = fma_(T, a.bg[0], C0);
+    a.out[HW + g] = fma_(T, a.bg[1], C1);
+    a.out[2 * HW + g] = fma_(T, a.bg[2], C2);
+    a.tfinal[g] = T;
+    a.contrib[g] = cnt;
+    ((float4*)a.pixc)[g] = make_float4(C0, C1, C2, 0.0f);
+  }
+  const int m = wave_max_i(cnt);
+  if (lane == 0) wmax[w] = m;
+  __syncthreads();
+  if (t == 0) a.maxc[tile] = (uint32_t)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+}
+
+hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
+  if (a.T <= 0) return hipSuccess;
+  if (a.bucket <= 64)
+    k_composite_fast<64><<<a.T, 256, 0, s>>>(a);
+  else
+    k_composite_fast<256><<<a.T, 256, 0, s>>>(a);
   return hipGetLastError();
 }
 
