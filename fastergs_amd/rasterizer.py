@@ -115,9 +115,9 @@ class Frame:
         if colors_precomp is not None and colors_precomp.numel() > 0:
             self.colors = _f32(colors_precomp, dev).reshape(P, 3)
         else:
-            if dc is None or dc.numel() == 0:
+            if P > 0 and (dc is None or dc.numel() == 0):
                 raise ValueError("dc is required when colours come from SH")
-            self.dc = _f32(dc, dev).reshape(P, 3)
+            self.dc = _f32(dc, dev).reshape(P, 3) if P > 0 else None
             if shs is not None and shs.numel() > 0:
                 self.shs = _f32(shs, dev).reshape(P, -1, 3)
                 self.M = int(self.shs.shape[1])
@@ -189,6 +189,7 @@ class Frame:
                              self.NB, _ptr(self.radii), _ptr(self.T_final),
                              _ptr(self.last_contributor), _ptr(g), _ptr(gws),
                              ctypes.byref(grads), ctypes.c_void_p(_stream(dev))), "fgs_backward")
+        out["raw"] = gws[:P * 48].view(torch.float32).reshape(P, 12)
         return out
 
     def densify_stats(self, dmean2D, accum, accum_abs, max_radii, denom):
@@ -252,7 +253,9 @@ class _RasterizeGaussians(torch.autograd.Function):
                 xyz_gradient_accum_abs, max_radii, denom):
         if cov3Ds_precomp is not None and cov3Ds_precomp.numel() > 0:
             raise ValueError("cov3D_precomp is not supported: pass scales and rotations")
-        if scales is None or scales.numel() == 0 or rotations is None or rotations.numel() == 0:
+        P = int(means3D.shape[0])
+        if scales is None or rotations is None or (P > 0 and (scales.numel() == 0 or
+                                                             rotations.numel() == 0)):
             raise ValueError("scales and rotations are required")
         want_abs = xyz_gradient_accum_abs is not None and xyz_gradient_accum_abs.numel() > 0
         frame = Frame(raster_settings, means3D, opacities, scales, rotations, dc=dc, shs=shs,

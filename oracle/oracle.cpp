@@ -306,7 +306,15 @@ void forward(State<R>& st) {
 }
 
 template <typename R>
-void backward(State<R>& st, const R* dimg) {
+void gauss_grads(State<R>& st, const float* raw_f);
+
+// accum: 0 = per-pair rows and the per-Gaussian reduction in double (the
+// checker), 1 = rows and reduction in float, pairs in sorted order, 2 = the
+// same with pairs in reverse order, 3 = rows in float and the reduction in
+// double (1 and 2 bound the error of float partial sums plus float atomics,
+// 3 that of float partial sums plus double atomics).
+template <typename R>
+void backward(State<R>& st, const R* dimg, int accum = 0) {
   const size_t HW = (size_t)st.W * st.H;
   const size_t Rn = st.vals.size();
   std::vector<double> rows(Rn * fgs::RAW_N, 0.0);
@@ -335,7 +343,11 @@ void backward(State<R>& st, const R* dimg) {
           fgs::pix_bwd(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], sp[6], sp[7], sp[8], fx, fy, dp0,
                        dp1, dp2, bgdot, Tf, T, r0, r1, r2, acc);
           double* row = &rows[k * fgs::RAW_N];
-          for (int f = 0; f < fgs::RAW_N; ++f) row[f] += (double)acc[f];
+          if (accum == 0) {
+            for (int f = 0; f < fgs::RAW_N; ++f) row[f] += (double)acc[f];
+          } else {
+            for (int f = 0; f < fgs::RAW_N; ++f) row[f] = (double)((float)row[f] + (float)acc[f]);
+          }
         };
         if (st.mode == 0) {
           for (uint32_t b = b0; b < b1; ++b) {
@@ -355,12 +367,24 @@ void backward(State<R>& st, const R* dimg) {
       }
   }
   st.raw.assign((size_t)st.P * fgs::RAW_N, 0.0);
-  for (size_t k = 0; k < Rn; ++k) {
+  for (size_t q = 0; q < Rn; ++q) {
+    const size_t k = accum == 2 ? Rn - 1 - q : q;
     double* dst = &st.raw[(size_t)st.vals[k] * fgs::RAW_N];
     const double* src = &rows[k * fgs::RAW_N];
-    for (int f = 0; f < fgs::RAW_N; ++f) dst[f] += src[f];
+    if (accum == 0 || accum == 3) {
+      for (int f = 0; f < fgs::RAW_N; ++f) dst[f] += src[f];
+    } else {
+      for (int f = 0; f < fgs::RAW_N; ++f) dst[f] = (double)((float)dst[f] + (float)src[f]);
+    }
   }
 
+  gauss_grads(st, nullptr);
+}
+
+// Per-Gaussian part of the backward (backward.cu:250-539) from raw sums
+// given as float rows of RAW_STRIDE (raw_f) or as the double sums in st.raw.
+template <typename R>
+void gauss_grads(State<R>& st, const float* raw_f) {
   const int P = st.P, M = st.M;
   st.dmean2d.assign((size_t)P * 4, R(0));
   st.dconic.assign((size_t)P * 4, R(0));
@@ -379,7 +403,8 @@ void backward(State<R>& st, const R* dimg) {
     R rawr[fgs::RAW_STRIDE];
     bool any = false;
     for (int f = 0; f < fgs::RAW_N; ++f) {
-      rawr[f] = (R)st.raw[(size_t)i * fgs::RAW_N + f];
+      rawr[f] = raw_f ? (R)raw_f[(size_t)i * fgs::RAW_STRIDE + f]
+                      : (R)st.raw[(size_t)i * fgs::RAW_N + f];
       any = any || rawr[f] != R(0);
     }
     if (!any) continue;
@@ -543,6 +568,12 @@ void* orc_create_f32(int P, const float* means, const float* opac, const float* 
 }
 void orc_forward_f32(void* h) { forward(*(State<float>*)h); }
 void orc_backward_f32(void* h, const float* dimg) { backward(*(State<float>*)h, dimg); }
+void orc_gauss_grads_from_raw_f32(void* h, const float* raw12) {
+  gauss_grads(*(State<float>*)h, raw12);
+}
+void orc_backward_f32_accum(void* h, const float* dimg, int accum) {
+  backward(*(State<float>*)h, dimg, accum);
+}
 long long orc_get_f32(void* h, const char* name, void* dst) {
   return get_named((State<float>*)h, name, dst);
 }

@@ -38,3 +38,47 @@ def grad_close(got: np.ndarray, ref: np.ndarray, rtol=1e-4, name=""):
         i = int(np.argmax(err / bound))
         raise AssertionError(f"{name}: {int(bad.sum())}/{ref.size} entries off; worst idx {i} "
                              f"got {got[i]:.8e} ref {ref[i]:.8e} (rms {scale:.3e})")
+
+
+WELL_CONDITIONED = ("opacities", "dc", "shs", "colors")
+GEOMETRIC = ("means3D", "scales", "rotations", "cov3D")
+
+
+def check_backward(got: dict, orc, dimg, abs_grad: bool = True):
+    """Backward parity in three parts.
+
+    1. The rasterizer adjoint's raw per-Gaussian sums (colour, mean2D, conic,
+       power; fgs_math.h RawField order) within 1e-4 of the oracle's double
+       sums: they differ only by float summation order.
+    2. The per-Gaussian adjoint, run by the oracle on the GPU's own raw sums,
+       reproduces the GPU's final gradients to float rounding: same
+       arithmetic, same inputs.
+    3. End to end against the oracle: raw-linear gradients (opacity, SH,
+       colour) elementwise within 1e-4; geometric gradients (means, scales,
+       rotations, cov3D) within 1e-3 in relative L2 norm. The geometric
+       adjoint of large splats amplifies 1e-7 relative changes of the conic
+       sums to ~1e-2 in single entries (conic -> 2D covariance adjoint with
+       1/det^2), in the reference's float32 arithmetic as in this one.
+    """
+    orc.backward(dimg)
+    ref = orc.grads()
+    raw_ref = orc.get("raw").reshape(-1, 11)
+    raw = got["raw"].detach().cpu().numpy()
+    nf = 11 if abs_grad else 9
+    for f in range(nf):
+        grad_close(raw[:, f], raw_ref[:, f], name=f"raw[{f}]")
+    if not abs_grad:
+        assert (raw[:, 9:] == 0).all()
+    mine = orc.gauss_grads_from_raw(raw)
+    for k in WELL_CONDITIONED + GEOMETRIC + ("mean2D", "conic"):
+        a = got[k].detach().cpu().numpy().reshape(mine[k].shape)
+        scale = float(np.sqrt(np.mean(mine[k].astype(np.float64) ** 2))) + 1e-30
+        assert np.abs(a - mine[k]).max() <= 1e-6 * scale, k
+    for k in WELL_CONDITIONED:
+        grad_close(got[k].detach().cpu().numpy(), ref[k], name=k)
+    for k in GEOMETRIC:
+        a = got[k].detach().cpu().numpy().ravel().astype(np.float64)
+        b = ref[k].ravel().astype(np.float64)
+        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        assert rel <= 1e-3, (k, rel)
+    return ref

@@ -14,7 +14,7 @@ from fastergs_amd import scenes
 from fastergs_amd._lib import SCHEDULE_PLAIN, check, lib
 from fastergs_amd.rasterizer import Frame, RasterOptions
 from oracle.pyoracle import Oracle
-from tests.helpers import grad_close, settings_for, to_dev
+from tests.helpers import check_backward, grad_close, settings_for, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -102,18 +102,10 @@ def test_backward_parity(case, abs_grad):
     tgt = scenes.target_image(cam.width, cam.height)
     img = orc.image()
     dimg = (2.0 * (img - tgt) / img.size).astype(np.float32)
-    orc.backward(dimg)
-    ref = orc.grads()
     fr.copts.abs_grad = 1 if abs_grad else 0
     got = fr.backward(torch.from_numpy(dimg).cuda())
     torch.cuda.synchronize()
-    for k in ("means3D", "dc", "shs", "opacities", "scales", "rotations", "cov3D", "colors"):
-        grad_close(got[k].cpu().numpy(), ref[k], name=k)
-    m2 = got["mean2D"].cpu().numpy().reshape(-1, 4)
-    grad_close(m2[:, :2], ref["mean2D"][:, :2], name="mean2D.xy")
-    if abs_grad:
-        grad_close(m2[:, 2:], ref["mean2D"][:, 2:], name="mean2D.abs")
-    grad_close(got["conic"].cpu().numpy().reshape(-1, 4), ref["conic"], name="conic")
+    check_backward(got, orc, dimg, abs_grad)
 
 
 def _sort_case(n, bits, seed, with_vals=True):
