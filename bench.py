@@ -103,6 +103,7 @@ def main():
     sched = SCHEDULE_FAST if args.schedule == "fast" else SCHEDULE_PLAIN
     ms, wall = timed(sched, args.steps, args.warmup)
     fwd_ms, _ = timed(sched, args.steps, 2, fwd_only=True)
+    stages, roofline = kernel_roofline(step, params, settings, sched, args, P)
     plain_ms = None
     if args.compare_plain and sched != SCHEDULE_PLAIN:
         plain_ms, _ = timed(SCHEDULE_PLAIN, max(5, args.steps // 2), 2)
@@ -132,10 +133,88 @@ def main():
         "wall_ms_per_step": round(wall, 4),
         "plain_ms_per_step": None if plain_ms is None else round(plain_ms, 4),
         "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms, 3),
-        "roofline": None,
+        "roofline": roofline,
+        "stages": stages,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+# Algorithmic work per launch of each timed kernel (bytes for HBM-bound
+# kernels, flops for VALU-bound ones). P Gaussians, M rest SH coefficients,
+# V visible Gaussians, R pairs, T tiles, E active (pixel, splat) pairs of
+# the backward (sum of last contributors).
+def kernel_units(name, P, M, V, R, T, E, fast):
+    sh = 12 + 12 * M  # dc + rest coefficients
+    table = {
+        # in: means, opacity, scales, rotations, SH; out: geom 32, splat 48,
+        # depth key 4 (+ sort key 8 and index 4 in the fast schedule), count 4, radius 4
+        "preprocess_fast": ("hbm", P * (12 + 4 + 12 + 16 + sh) + P * (32 + 48 + 4 + 8 + 4 + 4 + 4)),
+        "preprocess_plain": ("hbm", P * (12 + 4 + 12 + 16 + sh) + P * (32 + 48 + 4 + 4 + 4)),
+        # in: means, scales, rotations, opacity, rest SH, raw sums 48, radius, clamp;
+        # out: mean2D 16, conic 16, opacity 4, colour 12, mean3D 12, cov3D 24, dc 12, SH, scale 12, rot 16
+        "gauss_bwd": ("hbm", P * (12 + 12 + 16 + 4 + 12 * M + 48 + 4 + 4)
+                      + P * (16 + 16 + 4 + 12 + 12 + 24 + 12 + 12 * M + 12 + 16)),
+        # key-only scatter of the tile sort: read + write 8 B per pair; the
+        # last pass writes the 4 B list entry and the 4 B tile id instead
+        "radix_scatter_k": ("hbm", 16 * R),
+        "radix_scatter_k_last": ("hbm", 16 * R),
+        # sorted packed keys -> lists (4 B) and ranges
+        "unpack_ranges": ("hbm", 12 * R + 8 * T),
+        # ~70 flops per active (pixel, splat) in the reverse step (fgs_math.h pix_bwd)
+        "backward_fast": ("valu", 70 * E),
+        "backward_plain": ("valu", 70 * E),
+    }
+    if not fast:
+        table["radix_scatter_kv"] = ("hbm", 24 * R)  # 64-bit key + 32-bit value, read + write
+    return table.get(name)
+
+
+def kernel_roofline(step, params, settings, sched, args, P):
+    """Times every kernel with HIP events on its launch stream over a few
+    steps and reports the dominant one against its roofline."""
+    import torch
+    from fastergs_amd import _lib
+    from fastergs_amd.rasterizer import Frame, RasterOptions
+    fr = Frame(settings, params["means3D"].detach(), params["opacities"].detach(),
+               params["scales"].detach(), params["rotations"].detach(),
+               dc=params["dc"].detach(), shs=params["shs"].detach(),
+               options=RasterOptions(schedule=sched, bucket_size=args.bucket)).forward()
+    R, T = fr.R, ((fr.W + 15) // 16) * ((fr.H + 15) // 16)
+    V = int((fr.radii > 0).sum().item())
+    E = int(fr.last_contributor.sum().item())
+    M = int(params["shs"].shape[1])
+    del fr
+    steps = 5
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    stages = {}
+    best = None
+    for name, (tot_ms, launches) in prof.items():
+        avg_s = tot_ms / launches / 1e3
+        ent = {"ms_per_iter": round(tot_ms / steps, 4), "launches_per_iter": launches / steps,
+               "avg_us": round(avg_s * 1e6, 2)}
+        units = kernel_units(name, P, M, V, R, T, E, sched == 1)
+        if units is not None and launches % steps == 0:
+            bound, per_launch = units
+            ach = per_launch / avg_s / (1e9 if bound == "hbm" else 1e12)
+            peak = HBM_PEAK_GBS if bound == "hbm" else FP32_PEAK_TFLOPS
+            ent.update({"bound": bound, "achieved": round(ach, 2), "frac": round(ach / peak, 4)})
+            if best is None or tot_ms > best[1]:
+                best = (name, tot_ms, ach, peak, bound, per_launch)
+        stages[name] = ent
+    roof = None
+    if best is not None:
+        name, _, ach, peak, bound, per_launch = best
+        roof = {"kernel": name, "bound": bound, "achieved": round(ach, 2), "peak": peak,
+                "unit": "GB/s" if bound == "hbm" else "TFLOP/s", "frac": round(ach / peak, 4),
+                "traffic": None, "algorithmic_per_launch": per_launch}
+    return stages, roof
 
 
 def cpu_baseline(scene, cam):

@@ -80,6 +80,8 @@ SIGNATURES = {
                                       ctypes.c_int, vp, ctypes.c_size_t, vp]),
     "fgs_scan_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
     "fgs_scan_u32": (ctypes.c_int, [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_size_t, vp]),
+    "fgs_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "fgs_profile_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fgs_last_error": (ctypes.c_char_p, []),
     "fgs_build_info": (ctypes.c_char_p, []),
 }
@@ -112,3 +114,20 @@ def check(rc: int, what: str):
     if rc != 0:
         msg = lib().fgs_last_error().decode(errors="replace")
         raise FasterGSError(f"{what} failed ({rc}): {msg}")
+
+
+def profile_enable(on: bool = True):
+    lib().fgs_profile_enable(1 if on else 0)
+
+
+def profile_read() -> dict:
+    """{kernel name: (total ms, launches)} since the last profile_enable()."""
+    L = lib()
+    n = L.fgs_profile_read(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    L.fgs_profile_read(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, ms, cnt = line.split()
+        out[name] = (float(ms), int(cnt))
+    return out

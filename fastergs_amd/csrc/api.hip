@@ -3,11 +3,15 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstring>
+#include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/fastergs.h"
 #include "fgs_kernels.h"
 #include "fgs_layout.h"
+#include "fgs_prof.h"
 
 using namespace fgs;
 
@@ -120,9 +124,79 @@ GaussRef<float> make_gauss(const fgs_gaussians* g) {
   return r;
 }
 
+struct ProfRecord {
+  const char* name;
+  hipEvent_t start, stop;
+};
+struct Profiler {
+  bool on = false;
+  std::vector<ProfRecord> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+};
+Profiler g_prof;
+
 }  // namespace
 
+namespace fgs {
+bool prof_on() { return g_prof.on; }
+hipEvent_t prof_begin(hipStream_t s) {
+  hipEvent_t e = g_prof.take();
+  if (e) hipEventRecord(e, s);
+  return e;
+}
+void prof_end(const char* name, hipEvent_t start, hipStream_t s) {
+  hipEvent_t e = g_prof.take();
+  if (!e) return;
+  hipEventRecord(e, s);
+  g_prof.recs.push_back({name, start, e});
+}
+}  // namespace fgs
+
 extern "C" {
+
+int fgs_profile_enable(int on) {
+  for (auto& r : g_prof.recs) {
+    g_prof.pool.push_back(r.start);
+    g_prof.pool.push_back(r.stop);
+  }
+  g_prof.recs.clear();
+  g_prof.on = on != 0;
+  return FGS_OK;
+}
+
+int fgs_profile_read(char* buf, size_t cap) {
+  std::map<std::string, std::pair<double, int>> agg;
+  for (auto& r : g_prof.recs) {
+    if (hipEventSynchronize(r.stop) != hipSuccess) return fail(FGS_ERR_HIP, "profile sync");
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, r.start, r.stop) != hipSuccess) return fail(FGS_ERR_HIP, "profile time");
+    auto& a = agg[r.name];
+    a.first += ms;
+    a.second += 1;
+  }
+  std::string out;
+  char line[256];
+  for (auto& kv : agg) {
+    snprintf(line, sizeof(line), "%s %.6f %d\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    out += line;
+  }
+  if (buf && cap) {
+    const size_t n = out.size() < cap - 1 ? out.size() : cap - 1;
+    memcpy(buf, out.data(), n);
+    buf[n] = 0;
+  }
+  return (int)out.size();
+}
 
 const char* fgs_last_error(void) { return g_err.c_str(); }
 
@@ -194,7 +268,7 @@ int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_opti
   a.radii = radii;
   a.P = g->P;
   a.split_tiles = opt->split_tiles;
-  a.lchunk = w.lchunk;
+  a.lflag = w.lflag;
   a.dsort_k = w.dsort_k0;
   a.dsort_v = w.dsort_v0;
   a.counts = w.counts;
@@ -205,10 +279,11 @@ int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_opti
   }
   // FAST: per-Gaussian math + lane-packed exact counts of small rects
   FGS_TRY(launch_preprocess_fast(a, s), "preprocess");
-  // split tasks of large rects, counted by whole waves
-  FGS_TRY(scan_u32(w.lchunk, w.loff, g->P, w.counts + 3, w.scratch, s), "task scan");
-  FGS_TRY(launch_large_tiles(false, g->P, w.geom, w.loff, w.counts, opt->split_tiles,
-                             grid_x_of(cam->width), w.pcount, nullptr, nullptr, nullptr, s),
+  // large rects, compacted (count in counts[3]) and walked by whole workgroups
+  FGS_TRY(scan_u32(w.lflag, w.lpos, g->P, w.counts + 3, w.scratch, s), "large scan");
+  FGS_TRY(launch_compact_large(g->P, w.lflag, w.lpos, w.large_ids, s), "large compact");
+  FGS_TRY(launch_large_tiles(false, g->P, w.geom, w.large_ids, w.counts, grid_x_of(cam->width),
+                             w.pcount, nullptr, nullptr, nullptr, s),
           "large tiles");
   // Gaussians in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by tile alone yields the canonical per-tile order.
@@ -274,19 +349,22 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
     // emission in depth order, tile-only stable sort, lists, ranges, buckets
     FGS_TRY(fill_u32(gw.lcur, 0u, g->P, s), "zero cursors");
     FGS_TRY(launch_emit_small(g->P, gw.geom, opt->split_tiles, gx, gw.poff, pw.k0, s), "emit");
-    FGS_TRY(launch_large_tiles(true, g->P, gw.geom, gw.loff, gw.counts, opt->split_tiles, gx,
-                               nullptr, gw.poff, gw.lcur, pw.k0, s),
+    FGS_TRY(launch_large_tiles(true, g->P, gw.geom, gw.large_ids, gw.counts, gx, nullptr, gw.poff,
+                               gw.lcur, pw.k0, s),
             "emit large");
-    bool alt = false;
-    FGS_TRY(sort_pairs(pw.k0, nullptr, pw.k1, nullptr, R, 32, 32 + tile_bits(T), pw.scratch, s,
-                       alt),
-            "tile sort");
-    const uint64_t* packed = alt ? pw.k1 : pw.k0;
-    FGS_TRY(launch_unpack_vals(R, packed, pw.v0, s), "unpack");
-    FGS_TRY(launch_identify_ranges(R, packed, gw.ranges, T, s), "ranges");
+    if (R > 1) {
+      // the last pass writes the lists (low halves) and tile ids (high halves)
+      bool alt = false;
+      FGS_TRY(sort_pairs(pw.k0, nullptr, pw.k1, nullptr, R, 32, 32 + tile_bits(T), pw.scratch, s,
+                         alt, pw.v0, pw.tiles),
+              "tile sort");
+      FGS_TRY(launch_tile_ranges(R, pw.tiles, gw.ranges, T, s), "ranges");
+    } else {
+      FGS_TRY(launch_unpack_ranges(R, pw.k0, pw.v0, gw.ranges, T, s), "lists and ranges");
+    }
     FGS_TRY(launch_bucket_counts(T, gw.ranges, opt->bucket_size, gw.bcount, s), "bucket counts");
     FGS_TRY(scan_u32(gw.bcount, gw.boff, T, gw.boff + T, gw.scratch, s), "bucket scan");
-    FGS_TRY(launch_bucket_tables(T, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
+    FGS_TRY(launch_bucket_tables(T, NB, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
                                  pw.bucket_tile, s),
             "bucket tables");
     c.vals = pw.v0;

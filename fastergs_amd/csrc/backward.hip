@@ -13,6 +13,7 @@
 
 #include "fgs_device.h"
 #include "fgs_kernels.h"
+#include "fgs_prof.h"
 #include "fgs_layout.h"
 
 namespace fgs {
@@ -92,9 +93,9 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
 hipError_t launch_backward_plain(const BwdArgs& a, hipStream_t s) {
   if (a.T <= 0) return hipSuccess;
   if (a.abs_grad)
-    k_backward_plain<true><<<a.T, 256, 0, s>>>(a);
+    FGS_TIMED("backward_plain", s, (k_backward_plain<true><<<a.T, 256, 0, s>>>(a)));
   else
-    k_backward_plain<false><<<a.T, 256, 0, s>>>(a);
+    FGS_TIMED("backward_plain", s, (k_backward_plain<false><<<a.T, 256, 0, s>>>(a)));
   return hipGetLastError();
 }
 
@@ -190,57 +191,82 @@ hipError_t launch_backward_fast(const BwdArgs& a, hipStream_t s) {
   const int64_t want = (a.NB + 3) / 4;
   const int blocks = (int)(want < 4096 ? want : 4096);
   if (a.abs_grad)
-    k_backward_fast<true><<<blocks, 256, 0, s>>>(a);
+    FGS_TIMED("backward_fast", s, (k_backward_fast<true><<<blocks, 256, 0, s>>>(a)));
   else
-    k_backward_fast<false><<<blocks, 256, 0, s>>>(a);
+    FGS_TIMED("backward_fast", s, (k_backward_fast<false><<<blocks, 256, 0, s>>>(a)));
   return hipGetLastError();
 }
 
 // --------------------------------------------------- per-Gaussian adjoint
-__global__ __launch_bounds__(256) void k_gauss_bwd(GaussBwdArgs a) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.P) return;
-  const float4* r4 = (const float4*)(a.raw + (size_t)RAW_STRIDE * i);
-  float raw[RAW_STRIDE];
-  const float4 q0 = r4[0], q1 = r4[1], q2 = r4[2];
-  raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
-  raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
-  raw[8] = q2.x; raw[9] = q2.y; raw[10] = q2.z; raw[11] = q2.w;
-  Grad2D<float> g2;
-  finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
-  if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
-  ((float4*)a.dmean2D)[i] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
-  ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
-  a.dopacity[i] = g2.opacity;
-  a.dcolor[3 * (size_t)i] = g2.color[0];
-  a.dcolor[3 * (size_t)i + 1] = g2.color[1];
-  a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+// 128 Gaussians per workgroup; their SH rest rows are staged in LDS with
+// coalesced loads, the adjoint overwrites each row in place with its
+// gradient, and the rows are stored back coalesced.
+constexpr int GBWD_THREADS = 128;
+__global__ __launch_bounds__(GBWD_THREADS) void k_gauss_bwd(GaussBwdArgs a) {
+  extern __shared__ float rows[];  // GBWD_THREADS x 3M
+  const int g0 = blockIdx.x * GBWD_THREADS;
+  const int i = g0 + threadIdx.x;
   const int M = a.cam.max_coeffs;
-  float* ddc = a.ddc + 3 * (size_t)i;
-  float* dsh = a.dsh ? a.dsh + (size_t)i * M * 3 : nullptr;
-  for (int k = 0; k < 3; ++k) ddc[k] = 0.0f;
-  if (dsh)
-    for (int k = 0; k < 3 * M; ++k) dsh[k] = 0.0f;
-  GaussGrad<float> out;
-  out.dmean3d = a.dmean3D + 3 * (size_t)i;
-  out.dcov3d = a.dcov3D + 6 * (size_t)i;
-  out.dscale = a.dscale + 3 * (size_t)i;
-  out.drot = a.drot + 4 * (size_t)i;
-  out.ddc = ddc;
-  out.dsh = dsh;
-  if (a.radii[i] > 0) {
-    const int clamp = (int)f2u(a.splat[12 * (size_t)i + 10]);
-    gauss_bwd(i, a.G, a.cam, clamp, g2, out);
-  } else {
-    for (int k = 0; k < 3; ++k) out.dmean3d[k] = 0.0f;
-    for (int k = 0; k < 6; ++k) out.dcov3d[k] = 0.0f;
-    for (int k = 0; k < 3; ++k) out.dscale[k] = 0.0f;
-    for (int k = 0; k < 4; ++k) out.drot[k] = 0.0f;
+  const bool sh = a.dsh != nullptr && M > 0;
+  const int nb = min(GBWD_THREADS, a.P - g0);
+  if (sh)
+    for (int k = threadIdx.x; k < 3 * M * nb; k += GBWD_THREADS)
+      rows[k] = a.G.shs[(size_t)3 * M * g0 + k];
+  __syncthreads();
+  if (i < a.P) {
+    const float4* r4 = (const float4*)(a.raw + (size_t)RAW_STRIDE * i);
+    float raw[RAW_STRIDE];
+    const float4 q0 = r4[0], q1 = r4[1], q2 = r4[2];
+    raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
+    raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
+    raw[8] = q2.x; raw[9] = q2.y; raw[10] = q2.z; raw[11] = q2.w;
+    Grad2D<float> g2;
+    finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
+    if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
+    ((float4*)a.dmean2D)[i] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
+    ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
+    a.dopacity[i] = g2.opacity;
+    a.dcolor[3 * (size_t)i] = g2.color[0];
+    a.dcolor[3 * (size_t)i + 1] = g2.color[1];
+    a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+    float* row = rows + 3 * M * threadIdx.x;
+    float ddc[3] = {0.0f, 0.0f, 0.0f};
+    GaussGrad<float> out;
+    out.dmean3d = a.dmean3D + 3 * (size_t)i;
+    out.dcov3d = a.dcov3D + 6 * (size_t)i;
+    out.dscale = a.dscale + 3 * (size_t)i;
+    out.drot = a.drot + 4 * (size_t)i;
+    out.ddc = ddc;
+    out.dsh = sh ? row : nullptr;
+    int written = 0;  // leading SH entries holding gradients
+    if (a.radii[i] > 0) {
+      const int clamp = (int)f2u(a.splat[12 * (size_t)i + 10]);
+      gauss_bwd(i, a.G, a.cam, clamp, g2, out, sh ? row : nullptr);
+      if (a.G.shs != nullptr && a.G.colors == nullptr) written = 3 * (sh_count(a.cam.deg) - 1);
+    } else {
+      for (int k = 0; k < 3; ++k) out.dmean3d[k] = 0.0f;
+      for (int k = 0; k < 6; ++k) out.dcov3d[k] = 0.0f;
+      for (int k = 0; k < 3; ++k) out.dscale[k] = 0.0f;
+      for (int k = 0; k < 4; ++k) out.drot[k] = 0.0f;
+    }
+    if (sh)
+      for (int k = written; k < 3 * M; ++k) row[k] = 0.0f;
+    a.ddc[3 * (size_t)i] = ddc[0];
+    a.ddc[3 * (size_t)i + 1] = ddc[1];
+    a.ddc[3 * (size_t)i + 2] = ddc[2];
   }
+  __syncthreads();
+  if (sh)
+    for (int k = threadIdx.x; k < 3 * M * nb; k += GBWD_THREADS)
+      a.dsh[(size_t)3 * M * g0 + k] = rows[k];
 }
 
 hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
-  if (a.P > 0) k_gauss_bwd<<<(a.P + 255) / 256, 256, 0, s>>>(a);
+  if (a.P <= 0) return hipSuccess;
+  const int M = a.cam.max_coeffs;
+  const size_t shmem = (a.dsh != nullptr && M > 0) ? (size_t)GBWD_THREADS * 3 * M * sizeof(float) : 0;
+  FGS_TIMED("gauss_bwd", s,
+            (k_gauss_bwd<<<(a.P + GBWD_THREADS - 1) / GBWD_THREADS, GBWD_THREADS, shmem, s>>>(a)));
   return hipGetLastError();
 }
 

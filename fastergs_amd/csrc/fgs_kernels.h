@@ -10,8 +10,11 @@ namespace fgs {
 // scan_sort.hip
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
                     hipStream_t s);
+// last_lo / last_hi (key-only sorts): the final pass writes the low / high
+// 32 bits of the sorted keys there instead of a key array.
 hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
-                      int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt);
+                      int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
+                      uint32_t* last_lo = nullptr, uint32_t* last_hi = nullptr);
 
 // fill.hip-style helpers (render.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
@@ -28,7 +31,7 @@ struct PreArgs {
   int32_t* radii;
   int P;
   int split_tiles;
-  uint32_t* lchunk;    // fast: split tasks per Gaussian (0 for small rects)
+  uint32_t* lflag;     // fast: 1 for Gaussians whose rect is walked by whole workgroups
   uint64_t* dsort_k;   // fast: depth keys for the Gaussian depth sort
   uint32_t* dsort_v;   // fast: Gaussian indices for the depth sort
   uint32_t* counts;
@@ -38,25 +41,31 @@ hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
                                   uint64_t* keys, uint32_t* vals, hipStream_t s);
-hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* loff,
-                              const uint32_t* counts, int split, int gx, uint32_t* pcount,
+hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* large_ids,
+                              const uint32_t* counts, int gx, uint32_t* pcount,
                               const uint32_t* poff, uint32_t* lcur, uint64_t* packed,
                               hipStream_t s);
 hipError_t launch_emit_small(int P, const float* geom, int split, int gx, const uint32_t* poff,
                              uint64_t* packed, hipStream_t s);
+hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lpos,
+                                uint32_t* large_ids, hipStream_t s);
 hipError_t launch_gather_scatter(bool gather, int P, const uint32_t* order, const uint32_t* src,
                                  uint32_t* dst, hipStream_t s);
 
 // render.hip
 hipError_t launch_identify_ranges(int64_t R, const uint64_t* keys, uint32_t* ranges, int T,
                                   hipStream_t s);
-hipError_t launch_bucket_tables(int T, const uint32_t* ranges, const uint32_t* boff, int bucket,
-                                uint32_t* bucket_ranges, uint32_t* bucket_tile, hipStream_t s);
+hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, const uint32_t* boff,
+                                int bucket, uint32_t* bucket_ranges, uint32_t* bucket_tile,
+                                hipStream_t s);
+hipError_t launch_unpack_ranges(int64_t R, const uint64_t* keys, uint32_t* vals, uint32_t* ranges,
+                                int T, hipStream_t s);
+hipError_t launch_tile_ranges(int64_t R, const uint32_t* tiles, uint32_t* ranges, int T,
+                              hipStream_t s);
 hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
                                 hipStream_t s);
 hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
                                 hipStream_t s);
-hipError_t launch_unpack_vals(int64_t R, const uint64_t* packed, uint32_t* vals, hipStream_t s);
 hipError_t launch_nb_capacity(uint32_t* counts, int bucket, int T, hipStream_t s);
 struct CompArgs {
   int W, H, grid_x, T, bucket;

@@ -61,9 +61,10 @@ struct GaussWs {
   uint64_t* dsort_k1;
   uint32_t* dsort_v0;  // fast: Gaussian index by depth rank after the sort
   uint32_t* dsort_v1;
-  uint32_t* lchunk;    // fast: split tasks per Gaussian
-  uint32_t* loff;      // fast: exclusive prefix of lchunk
-  uint32_t* lcur;      // fast: per-Gaussian emission cursor of split tasks
+  uint32_t* lflag;     // fast: rect walked by whole workgroups (1) or lane-packed (0)
+  uint32_t* lpos;      // fast: exclusive prefix of lflag
+  uint32_t* large_ids; // fast: flagged Gaussians in index order
+  uint32_t* lcur;      // fast: per-Gaussian emission cursor of those walks
   uint32_t* rcount;    // fast: pair counts in depth order
   uint32_t* roff;      // fast: their exclusive prefix
   uint32_t* ranges;    // T x 2
@@ -71,7 +72,7 @@ struct GaussWs {
   uint32_t* bcount;    // T + 1 scratch (bucket counts, then active bucket counts)
   uint32_t* actoff;    // T + 1: exclusive prefix of per-tile active buckets, [T] = total
   uint32_t* maxc;      // T
-  uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] split tasks
+  uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects
   void* scratch;
   size_t scratch_bytes;
   size_t total;
@@ -91,8 +92,9 @@ inline GaussWs carve_gauss(void* base, int P, int T) {
   w.dsort_k1 = c.take<uint64_t>(p);
   w.dsort_v0 = c.take<uint32_t>(p);
   w.dsort_v1 = c.take<uint32_t>(p);
-  w.lchunk = c.take<uint32_t>(p);
-  w.loff = c.take<uint32_t>(p + 1);
+  w.lflag = c.take<uint32_t>(p);
+  w.lpos = c.take<uint32_t>(p + 1);
+  w.large_ids = c.take<uint32_t>(p);
   w.lcur = c.take<uint32_t>(p);
   w.rcount = c.take<uint32_t>(p);
   w.roff = c.take<uint32_t>(p + 1);
@@ -118,6 +120,7 @@ struct PairWs {
   uint64_t* k1;
   uint32_t* v0;  // sorted per-tile Gaussian lists (fast), values (plain)
   uint32_t* v1;
+  uint32_t* tiles;  // fast: tile id of each sorted pair
   void* scratch;
   size_t scratch_bytes;
   uint32_t* bucket_ranges;
@@ -136,6 +139,7 @@ inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int HW, bool fast) {
   w.k1 = c.take<uint64_t>(r);
   w.v0 = c.take<uint32_t>(r);
   w.v1 = fast ? nullptr : c.take<uint32_t>(r);
+  w.tiles = fast ? c.take<uint32_t>(r) : nullptr;
   w.scratch_bytes = sort_ws_bytes((int64_t)r);
   w.scratch = c.take<char>(w.scratch_bytes);
   if (fast) {

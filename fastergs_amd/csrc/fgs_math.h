@@ -307,25 +307,35 @@ FGS_HD void sh_basis(R x, R y, R z, R bas[16]) {
 
 FGS_HD int sh_count(int deg) { return deg <= 0 ? 1 : (deg == 1 ? 4 : (deg == 2 ? 9 : 16)); }
 
-// View-dependent colour of Gaussian i: sum_k basis_k coef_k + 0.5, clamped at 0.
+// View-dependent colour: sum_k basis_k coef_k + 0.5, clamped at 0. d0 is the
+// Gaussian's DC row (3), sh its rest rows (3 per coefficient); loops have
+// static bounds so the arrays stay in registers.
 template <typename R>
-FGS_HD void sh_color(int i, const GaussRef<R>& G, const CamRef<R>& cam, R rgb[3], int& clamp) {
-  const R* m = G.means + 3 * i;
-  const R dx = m[0] - cam.campos[0], dy = m[1] - cam.campos[1], dz = m[2] - cam.campos[2];
-  const R len = sqrt_(dx * dx + dy * dy + dz * dz);
+FGS_HD void sh_color_rows(R ox, R oy, R oz, const R* d0, const R* sh, int deg, R rgb[3],
+                          int& clamp) {
+  const R len = sqrt_(ox * ox + oy * oy + oz * oz);
   R bas[16];
-  sh_basis(dx / len, dy / len, dz / len, bas);
-  const int n = sh_count(cam.deg);
-  const R* d0 = G.dc + 3 * i;
-  const R* sh = G.shs + (size_t)i * cam.max_coeffs * 3;
+  sh_basis(ox / len, oy / len, oz / len, bas);
+  const int n = sh_count(deg);
   clamp = 0;
   for (int ch = 0; ch < 3; ++ch) {
     R acc = bas[0] * d0[ch];
-    for (int k = 1; k < n; ++k) acc = acc + bas[k] * sh[3 * (k - 1) + ch];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int k = 1; k < 16; ++k)
+      if (k < n) acc = acc + bas[k] * sh[3 * (k - 1) + ch];
     acc = acc + R(0.5f);
     if (acc < R(0)) clamp |= 1 << ch;
     rgb[ch] = max_(acc, R(0));
   }
+}
+
+template <typename R>
+FGS_HD void sh_color(int i, const GaussRef<R>& G, const CamRef<R>& cam, R rgb[3], int& clamp) {
+  const R* m = G.means + 3 * i;
+  sh_color_rows(m[0] - cam.campos[0], m[1] - cam.campos[1], m[2] - cam.campos[2], G.dc + 3 * i,
+                G.shs + (size_t)i * cam.max_coeffs * 3, cam.deg, rgb, clamp);
 }
 
 // Tile rect of a splat centred at (x,y) with integer half-extent (w,h)
@@ -342,7 +352,8 @@ FGS_HD void tile_rect(R x, R y, int w, int h, int gx, int gy, int rect[4]) {
 
 // Full per-Gaussian forward (preprocess.cu:450-499 and :698-720).
 template <typename R>
-FGS_HD void preprocess_gauss(int i, const GaussRef<R>& G, const CamRef<R>& cam, Proj<R>& o) {
+FGS_HD void preprocess_gauss(int i, const GaussRef<R>& G, const CamRef<R>& cam, Proj<R>& o,
+                             const R* dc_row = nullptr, const R* sh_row = nullptr) {
   o.valid = 0;
   o.radius = 0;
   o.clamp = 0;
@@ -399,6 +410,9 @@ FGS_HD void preprocess_gauss(int i, const GaussRef<R>& G, const CamRef<R>& cam, 
   o.log2op = l2op;
   if (G.colors != nullptr) {
     for (int k = 0; k < 3; ++k) o.rgb[k] = G.colors[3 * i + k];
+  } else if (dc_row != nullptr) {
+    sh_color_rows(pm[0] - cam.campos[0], pm[1] - cam.campos[1], pm[2] - cam.campos[2], dc_row,
+                  sh_row, cam.deg, o.rgb, o.clamp);
   } else {
     sh_color(i, G, cam, o.rgb, o.clamp);
   }
@@ -581,7 +595,7 @@ struct GaussGrad {
 // is w.r.t. the unnormalised quaternion.
 template <typename R>
 FGS_HD void gauss_bwd(int i, const GaussRef<R>& G, const CamRef<R>& cam, int clamp,
-                      const Grad2D<R>& g2, const GaussGrad<R>& out) {
+                      const Grad2D<R>& g2, const GaussGrad<R>& out, const R* sh_row = nullptr) {
   const R* pm = G.means + 3 * i;
   R t[3];
   xform_point(cam.view, pm[0], pm[1], pm[2], t);
@@ -662,41 +676,49 @@ FGS_HD void gauss_bwd(int i, const GaussRef<R>& G, const CamRef<R>& cam, int cla
     R dRGB[3];
     for (int ch = 0; ch < 3; ++ch) dRGB[ch] = (clamp >> ch) & 1 ? R(0) : g2.color[ch];
     const int n = sh_count(cam.deg);
-    for (int ch = 0; ch < 3; ++ch) out.ddc[ch] = bas[0] * dRGB[ch];
-    for (int k = 1; k < n; ++k)
-      for (int ch = 0; ch < 3; ++ch) out.dsh[3 * (k - 1) + ch] = bas[k] * dRGB[ch];
-    // d basis / d(x,y,z), zero rows for unused degrees
+    // d basis / d(x,y,z) (rows for unused degrees are skipped by k < n)
     const R xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-    R db[16][3];
-    for (int k = 0; k < 16; ++k) db[k][0] = db[k][1] = db[k][2] = R(0);
-    db[1][1] = -R(SH_C1);
-    db[2][2] = R(SH_C1);
-    db[3][0] = -R(SH_C1);
-    db[4][0] = R(SH_C2_0) * y; db[4][1] = R(SH_C2_0) * x;
-    db[5][1] = R(SH_C2_1) * z; db[5][2] = R(SH_C2_1) * y;
-    db[6][0] = R(-2) * R(SH_C2_2) * x; db[6][1] = R(-2) * R(SH_C2_2) * y; db[6][2] = R(4) * R(SH_C2_2) * z;
-    db[7][0] = R(SH_C2_3) * z; db[7][2] = R(SH_C2_3) * x;
-    db[8][0] = R(2) * R(SH_C2_4) * x; db[8][1] = R(-2) * R(SH_C2_4) * y;
-    db[9][0] = R(6) * R(SH_C3_0) * xy; db[9][1] = R(SH_C3_0) * (R(3) * xx - R(3) * yy);
-    db[10][0] = R(SH_C3_1) * yz; db[10][1] = R(SH_C3_1) * xz; db[10][2] = R(SH_C3_1) * xy;
-    db[11][0] = R(-2) * R(SH_C3_2) * xy; db[11][1] = R(SH_C3_2) * (R(4) * zz - xx - R(3) * yy);
-    db[11][2] = R(8) * R(SH_C3_2) * yz;
-    db[12][0] = R(-6) * R(SH_C3_3) * xz; db[12][1] = R(-6) * R(SH_C3_3) * yz;
-    db[12][2] = R(SH_C3_3) * (R(6) * zz - R(3) * xx - R(3) * yy);
-    db[13][0] = R(SH_C3_4) * (R(4) * zz - R(3) * xx - yy); db[13][1] = R(-2) * R(SH_C3_4) * xy;
-    db[13][2] = R(8) * R(SH_C3_4) * xz;
-    db[14][0] = R(2) * R(SH_C3_5) * xz; db[14][1] = R(-2) * R(SH_C3_5) * yz;
-    db[14][2] = R(SH_C3_5) * (xx - yy);
-    db[15][0] = R(SH_C3_6) * (R(3) * xx - R(3) * yy); db[15][1] = R(-6) * R(SH_C3_6) * xy;
-    const R* sh = G.shs + (size_t)i * cam.max_coeffs * 3;
+    const R db[16][3] = {
+        {R(0), R(0), R(0)},
+        {R(0), -R(SH_C1), R(0)},
+        {R(0), R(0), R(SH_C1)},
+        {-R(SH_C1), R(0), R(0)},
+        {R(SH_C2_0) * y, R(SH_C2_0) * x, R(0)},
+        {R(0), R(SH_C2_1) * z, R(SH_C2_1) * y},
+        {R(-2) * R(SH_C2_2) * x, R(-2) * R(SH_C2_2) * y, R(4) * R(SH_C2_2) * z},
+        {R(SH_C2_3) * z, R(0), R(SH_C2_3) * x},
+        {R(2) * R(SH_C2_4) * x, R(-2) * R(SH_C2_4) * y, R(0)},
+        {R(6) * R(SH_C3_0) * xy, R(SH_C3_0) * (R(3) * xx - R(3) * yy), R(0)},
+        {R(SH_C3_1) * yz, R(SH_C3_1) * xz, R(SH_C3_1) * xy},
+        {R(-2) * R(SH_C3_2) * xy, R(SH_C3_2) * (R(4) * zz - xx - R(3) * yy), R(8) * R(SH_C3_2) * yz},
+        {R(-6) * R(SH_C3_3) * xz, R(-6) * R(SH_C3_3) * yz,
+         R(SH_C3_3) * (R(6) * zz - R(3) * xx - R(3) * yy)},
+        {R(SH_C3_4) * (R(4) * zz - R(3) * xx - yy), R(-2) * R(SH_C3_4) * xy, R(8) * R(SH_C3_4) * xz},
+        {R(2) * R(SH_C3_5) * xz, R(-2) * R(SH_C3_5) * yz, R(SH_C3_5) * (xx - yy)},
+        {R(SH_C3_6) * (R(3) * xx - R(3) * yy), R(-6) * R(SH_C3_6) * xy, R(0)}};
+    const R* sh = sh_row ? sh_row : G.shs + (size_t)i * cam.max_coeffs * 3;
     R ddir[3] = {R(0), R(0), R(0)};
-    for (int k = 1; k < n; ++k) {
-      const R cdot = sh[3 * (k - 1)] * dRGB[0] + sh[3 * (k - 1) + 1] * dRGB[1] +
-                     sh[3 * (k - 1) + 2] * dRGB[2];
-      ddir[0] += db[k][0] * cdot;
-      ddir[1] += db[k][1] * cdot;
-      ddir[2] += db[k][2] * cdot;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int k = 1; k < 16; ++k) {
+      if (k < n) {
+        const R cdot = sh[3 * (k - 1)] * dRGB[0] + sh[3 * (k - 1) + 1] * dRGB[1] +
+                       sh[3 * (k - 1) + 2] * dRGB[2];
+        ddir[0] += db[k][0] * cdot;
+        ddir[1] += db[k][1] * cdot;
+        ddir[2] += db[k][2] * cdot;
+      }
     }
+    // SH coefficient gradients (written after sh has been read, so sh and dsh
+    // may share storage)
+    for (int ch = 0; ch < 3; ++ch) out.ddc[ch] = bas[0] * dRGB[ch];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int k = 1; k < 16; ++k)
+      if (k < n)
+        for (int ch = 0; ch < 3; ++ch) out.dsh[3 * (k - 1) + ch] = bas[k] * dRGB[ch];
     // through the normalisation dir = o/|o| (backward.cu:88-103)
     const R s2 = ox * ox + oy * oy + oz * oz;
     const R inv32 = R(1) / sqrt_(s2 * s2 * s2);

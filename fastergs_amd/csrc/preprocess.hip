@@ -21,6 +21,7 @@
 
 #include "fgs_device.h"
 #include "fgs_kernels.h"
+#include "fgs_prof.h"
 #include "fgs_layout.h"
 
 namespace fgs {
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void k_preprocess_plain(PreArgs a) {
 }
 
 hipError_t launch_preprocess_plain(const PreArgs& a, hipStream_t s) {
-  if (a.P > 0) k_preprocess_plain<<<(a.P + 255) / 256, 256, 0, s>>>(a);
+  if (a.P > 0) FGS_TIMED("preprocess_plain", s, (k_preprocess_plain<<<(a.P + 255) / 256, 256, 0, s>>>(a)));
   return hipGetLastError();
 }
 
@@ -122,8 +123,9 @@ hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
                                   uint64_t* keys, uint32_t* vals, hipStream_t s) {
   if (P > 0)
-    k_duplicate_plain<<<(P + 255) / 256, 256, 0, s>>>(P, geom, dkey, pcount, poff, grid_x, keys,
-                                                      vals);
+    FGS_TIMED("duplicate_plain", s,
+              (k_duplicate_plain<<<(P + 255) / 256, 256, 0, s>>>(P, geom, dkey, pcount, poff, grid_x,
+                                                                 keys, vals)));
   return hipGetLastError();
 }
 
@@ -190,16 +192,34 @@ __device__ __forceinline__ void stage_small(WaveTiles& L, int lane, bool small, 
   S = (uint32_t)__shfl(inc, WAVE - 1, WAVE);
 }
 
-__global__ __launch_bounds__(256) void k_preprocess_fast(PreArgs a) {
-  __shared__ WaveTiles lds[4];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int lane = lane_id();
-  WaveTiles& L = lds[threadIdx.x / WAVE];
-  const int gbase = blockIdx.x * 256 + (threadIdx.x / WAVE) * WAVE;
+// One wave64 per workgroup: the dc / SH rows of its 64 Gaussians are staged
+// in LDS with coalesced loads (a thread's own 192-byte row would otherwise
+// be 48 scattered loads per wave), then the same LDS holds the wave's
+// lane-packed tile walk. 12 KB of LDS per workgroup keeps occupancy high.
+constexpr int PRE_THREADS = WAVE;
+__global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a) {
+  extern __shared__ float rows[];  // 64 x 3 dc, then 64 x 3M rest
+  const int i = blockIdx.x * PRE_THREADS + threadIdx.x;
+  const int lane = threadIdx.x;
+  WaveTiles& L = *(WaveTiles*)rows;
+  const int g0 = blockIdx.x * PRE_THREADS;
+  const int M = a.cam.max_coeffs;
+  const bool staged = a.G.colors == nullptr && a.G.dc != nullptr;
+  float* sdc = rows;
+  float* ssh = rows + 3 * PRE_THREADS;
+  if (staged) {
+    const int nb = min(PRE_THREADS, a.P - g0);
+    for (int k = lane; k < 3 * nb; k += PRE_THREADS) sdc[k] = a.G.dc[3 * (size_t)g0 + k];
+    if (M > 0)
+      for (int k = lane; k < 3 * M * nb; k += PRE_THREADS)
+        ssh[k] = a.G.shs[(size_t)3 * M * g0 + k];
+  }
+  __syncthreads();
   Proj<float> o;
   o.valid = 0;
   if (i < a.P) {
-    preprocess_gauss(i, a.G, a.cam, o);
+    preprocess_gauss(i, a.G, a.cam, o, staged ? sdc + 3 * lane : nullptr,
+                     staged ? ssh + 3 * M * lane : nullptr);
     a.radii[i] = o.valid ? o.radius : 0;
     const uint32_t dk = o.valid ? depth_bits(o.depth) : 0xffffffffu;
     a.dkey[i] = dk;
@@ -212,58 +232,55 @@ __global__ __launch_bounds__(256) void k_preprocess_fast(PreArgs a) {
   }
   const int area = o.valid ? (o.rect[2] - o.rect[0]) * (o.rect[3] - o.rect[1]) : 0;
   const bool small = o.valid && area < a.split_tiles;
-  if (i < a.P) {
-    a.lchunk[i] = (o.valid && !small) ? (uint32_t)((area + a.split_tiles - 1) / a.split_tiles) : 0u;
-  }
+  // large rects: flagged here, compacted by a scan (no same-address atomics)
+  if (i < a.P) a.lflag[i] = (o.valid && !small) ? 1u : 0u;
   uint32_t S;
+  __syncthreads();  // the SH rows are dead; the tile walk reuses the LDS
   stage_small(L, lane, small, o.x, o.y, o.conic, o.power, o.rect[0], o.rect[1], o.rect[2],
               o.rect[3], S);
   __syncthreads();
-  walk_small<false>(L, S, gbase, a.cam.grid_x, nullptr, nullptr);
+  walk_small<false>(L, S, g0, a.cam.grid_x, nullptr, nullptr);
   __syncthreads();
   if (i < a.P) a.pcount[i] = L.cnt[lane];
 }
 
 hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s) {
-  if (a.P > 0) k_preprocess_fast<<<(a.P + 255) / 256, 256, 0, s>>>(a);
+  if (a.P <= 0) return hipSuccess;
+  const bool staged = a.G.colors == nullptr && a.G.dc != nullptr;
+  size_t shmem =
+      staged ? (size_t)PRE_THREADS * 3 * (1 + a.cam.max_coeffs) * sizeof(float) : 0;
+  if (shmem < sizeof(WaveTiles)) shmem = sizeof(WaveTiles);
+  FGS_TIMED("preprocess_fast", s,
+            (k_preprocess_fast<<<(a.P + PRE_THREADS - 1) / PRE_THREADS, PRE_THREADS, shmem, s>>>(a)));
   return hipGetLastError();
 }
 
-// Large rects: tasks of `split` tiles each (task t belongs to the Gaussian
-// whose [loff, loff + lchunk) holds t); waves take tasks in a grid-stride
-// loop over the device-side task count and walk 64 tiles per step.
+// Large rects: one workgroup per Gaussian of the compact list (grid-stride
+// over the device-side count); its four waves split the rect's tiles, 64
+// per wave step, 256 per workgroup step. One atomic per wave for the count;
+// emission reserves slots in the Gaussian's block with one atomic per step.
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_large_tiles(int P, const float* __restrict__ geom,
-                                                     const uint32_t* __restrict__ loff,
-                                                     const uint32_t* __restrict__ counts,
-                                                     int split, int gx, uint32_t* pcount,
+__global__ __launch_bounds__(256) void k_large_tiles(const float* __restrict__ geom,
+                                                     const uint32_t* __restrict__ large_ids,
+                                                     const uint32_t* __restrict__ counts, int gx,
+                                                     uint32_t* pcount,
                                                      const uint32_t* __restrict__ poff,
                                                      uint32_t* lcur, uint64_t* packed) {
-  const uint32_t L = counts[3];
+  const uint32_t NL = counts[3];
   const int lane = lane_id();
-  const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
-  for (uint32_t t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; t < L; t += nwaves) {
-    // last Gaussian with loff <= t
-    int lo = 0, hi = P;  // loff[lo] <= t < loff[hi] (loff[P] = L)
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (loff[mid] <= t) lo = mid; else hi = mid;
-    }
-    const int g = lo;
+  const int w = threadIdx.x / WAVE;
+  for (uint32_t j = blockIdx.x; j < NL; j += gridDim.x) {
+    const int g = (int)large_ids[j];
     const GeomRow r = load_geom(geom, g);
-    const int w = r.r2 - r.r0;
-    const int area = w * (r.r3 - r.r1);
-    const int c = (int)(t - loff[g]);
-    const int start = c * split;
-    const int end = min(area, start + split);
-    int l = start + lane;
-    int ty = l / w;
-    int tx = l - ty * w;
-    const int sy = WAVE / w, sx = WAVE - sy * w;
+    const int wr = r.r2 - r.r0;
+    const int area = wr * (r.r3 - r.r1);
+    int l = w * WAVE + lane;
+    int ty = l / wr;
+    int tx = l - ty * wr;
+    const int sy = 4 * WAVE / wr, sx = 4 * WAVE - sy * wr;
     uint32_t mine = 0;
-    for (int k = start; k < end; k += WAVE) {
-      const bool in = l < end;
-      const bool hit = in && tile_hit(r.r0 + tx, r.r1 + ty, r.x, r.y, r.conic, r.power);
+    for (int k = w * WAVE; k < area; k += 4 * WAVE) {
+      const bool hit = l < area && tile_hit(r.r0 + tx, r.r1 + ty, r.x, r.y, r.conic, r.power);
       const uint64_t m = __ballot(hit);
       if (!EMIT) {
         mine += (uint32_t)__popcll(m);
@@ -275,11 +292,11 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, const float* __restr
           packed[poff[g] + slot0 + lanes_below(m)] =
               ((uint64_t)(uint32_t)((r.r1 + ty) * gx + r.r0 + tx) << 32) | (uint32_t)g;
       }
-      l += WAVE;
+      l += 4 * WAVE;
       tx += sx;
       ty += sy;
-      if (tx >= w) {
-        tx -= w;
+      if (tx >= wr) {
+        tx -= wr;
         ty += 1;
       }
     }
@@ -287,18 +304,20 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, const float* __restr
   }
 }
 
-hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* loff,
-                              const uint32_t* counts, int split, int gx, uint32_t* pcount,
+hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* large_ids,
+                              const uint32_t* counts, int gx, uint32_t* pcount,
                               const uint32_t* poff, uint32_t* lcur, uint64_t* packed,
                               hipStream_t s) {
   if (P <= 0) return hipSuccess;
-  const int blocks = 1024;  // 4096 persistent waves; each exits when the task count is reached
+  const int blocks = 2048;  // persistent workgroups; each exits past the device-side count
   if (emit)
-    k_large_tiles<true><<<blocks, 256, 0, s>>>(P, geom, loff, counts, split, gx, pcount, poff,
-                                               lcur, packed);
+    FGS_TIMED("large_tiles_emit", s,
+              (k_large_tiles<true><<<blocks, 256, 0, s>>>(geom, large_ids, counts, gx, pcount, poff,
+                                                          lcur, packed)));
   else
-    k_large_tiles<false><<<blocks, 256, 0, s>>>(P, geom, loff, counts, split, gx, pcount, poff,
-                                                lcur, packed);
+    FGS_TIMED("large_tiles_count", s,
+              (k_large_tiles<false><<<blocks, 256, 0, s>>>(geom, large_ids, counts, gx, pcount,
+                                                           poff, lcur, packed)));
   return hipGetLastError();
 }
 
@@ -326,7 +345,22 @@ __global__ __launch_bounds__(256) void k_emit_small(int P, const float* __restri
 
 hipError_t launch_emit_small(int P, const float* geom, int split, int gx, const uint32_t* poff,
                              uint64_t* packed, hipStream_t s) {
-  if (P > 0) k_emit_small<<<(P + 255) / 256, 256, 0, s>>>(P, geom, split, gx, poff, packed);
+  if (P > 0)
+    FGS_TIMED("emit_small", s,
+              (k_emit_small<<<(P + 255) / 256, 256, 0, s>>>(P, geom, split, gx, poff, packed)));
+  return hipGetLastError();
+}
+
+// large_ids[lpos[i]] = i for every flagged Gaussian (lpos = exclusive scan of
+// the flags, so the list is in index order).
+__global__ void k_compact_large(int P, const uint32_t* __restrict__ lflag,
+                                const uint32_t* __restrict__ lpos, uint32_t* __restrict__ large_ids) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < P && lflag[i]) large_ids[lpos[i]] = (uint32_t)i;
+}
+hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lpos,
+                                uint32_t* large_ids, hipStream_t s) {
+  if (P > 0) k_compact_large<<<(P + 255) / 256, 256, 0, s>>>(P, lflag, lpos, large_ids);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 
 #include "fgs_device.h"
 #include "fgs_kernels.h"
+#include "fgs_prof.h"
 #include "fgs_layout.h"
 
 namespace fgs {
@@ -64,30 +65,78 @@ hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint3
   return hipGetLastError();
 }
 
-// boff = exclusive prefix of bucket counts (boff[t] = first bucket of tile t).
-__global__ void k_bucket_tables(int T, const uint32_t* __restrict__ ranges,
+// boff = exclusive prefix of bucket counts (boff[t] = first bucket of tile
+// t, boff[T] = NB). One thread per bucket: its tile is the last t with
+// boff[t] <= b (binary search over the T + 1 offsets).
+__global__ void k_bucket_tables(int T, int64_t cap, const uint32_t* __restrict__ ranges,
                                 const uint32_t* __restrict__ boff, int bucket,
                                 uint32_t* __restrict__ bucket_ranges,
                                 uint32_t* __restrict__ bucket_tile) {
+  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= cap || b >= (int64_t)boff[T]) return;
+  int lo = 0, hi = T;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)boff[mid] <= b) lo = mid; else hi = mid;
+  }
+  const uint32_t s = ranges[2 * lo] + (uint32_t)(b - boff[lo]) * (uint32_t)bucket;
+  const uint32_t e = ranges[2 * lo + 1];
+  bucket_ranges[2 * b] = s;
+  bucket_ranges[2 * b + 1] = min(s + (uint32_t)bucket, e);
+  bucket_tile[b] = (uint32_t)lo;
+}
+hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, const uint32_t* boff,
+                                int bucket, uint32_t* bucket_ranges, uint32_t* bucket_tile,
+                                hipStream_t s) {
+  if (T > 0 && cap > 0)
+    k_bucket_tables<<<(unsigned)((cap + 255) / 256), 256, 0, s>>>(T, cap, ranges, boff, bucket,
+                                                                  bucket_ranges, bucket_tile);
+  return hipGetLastError();
+}
+
+// Sorted packed keys -> per-tile lists and tile ranges in one pass.
+__global__ void k_unpack_ranges(int64_t R, const uint64_t* __restrict__ keys,
+                                uint32_t* __restrict__ vals, uint32_t* __restrict__ ranges) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= R) return;
+  const uint64_t key = keys[idx];
+  vals[idx] = (uint32_t)key;
+  const uint32_t cur = (uint32_t)(key >> 32);
+  if (idx == 0 || (uint32_t)(keys[idx - 1] >> 32) != cur) ranges[2 * cur] = (uint32_t)idx;
+  if (idx == R - 1 || (uint32_t)(keys[idx + 1] >> 32) != cur) ranges[2 * cur + 1] = (uint32_t)(idx + 1);
+}
+hipError_t launch_unpack_ranges(int64_t R, const uint64_t* keys, uint32_t* vals, uint32_t* ranges,
+                                int T, hipStream_t s) {
+  hipError_t e = fill_u32(ranges, 0u, 2 * (int64_t)T, s);
+  if (e != hipSuccess) return e;
+  if (R > 0)
+    FGS_TIMED("unpack_ranges", s,
+              (k_unpack_ranges<<<(unsigned)((R + 255) / 256), 256, 0, s>>>(R, keys, vals, ranges)));
+  return hipGetLastError();
+}
+
+// Ranges from the sorted tile ids by binary search (one thread per tile):
+// ranges[t] = [lower_bound(t), lower_bound(t + 1)).
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+__global__ void k_tile_ranges(uint32_t R, const uint32_t* __restrict__ tiles,
+                              uint32_t* __restrict__ ranges, int T) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= T) return;
-  uint32_t b = boff[t];
-  uint32_t s = ranges[2 * t];
-  const uint32_t e = ranges[2 * t + 1];
-  while (s < e) {
-    const uint32_t n = (e - s) < (uint32_t)bucket ? (e - s) : (uint32_t)bucket;
-    bucket_ranges[2 * b] = s;
-    bucket_ranges[2 * b + 1] = s + n;
-    bucket_tile[b] = (uint32_t)t;
-    s += n;
-    ++b;
-  }
+  ranges[2 * t] = lower_bound_u32(tiles, R, (uint32_t)t);
+  ranges[2 * t + 1] = lower_bound_u32(tiles, R, (uint32_t)t + 1);
 }
-hipError_t launch_bucket_tables(int T, const uint32_t* ranges, const uint32_t* boff, int bucket,
-                                uint32_t* bucket_ranges, uint32_t* bucket_tile, hipStream_t s) {
+hipError_t launch_tile_ranges(int64_t R, const uint32_t* tiles, uint32_t* ranges, int T,
+                              hipStream_t s) {
   if (T > 0)
-    k_bucket_tables<<<(T + 255) / 256, 256, 0, s>>>(T, ranges, boff, bucket, bucket_ranges,
-                                                    bucket_tile);
+    FGS_TIMED("tile_ranges", s,
+              (k_tile_ranges<<<(T + 255) / 256, 256, 0, s>>>((uint32_t)R, tiles, ranges, T)));
   return hipGetLastError();
 }
 
@@ -101,16 +150,6 @@ __global__ void k_active_counts(int T, const uint32_t* __restrict__ maxc, int bu
 hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
                                 hipStream_t s) {
   if (T > 0) k_active_counts<<<(T + 255) / 256, 256, 0, s>>>(T, maxc, bucket, out);
-  return hipGetLastError();
-}
-
-__global__ void k_unpack_vals(int64_t R, const uint64_t* __restrict__ packed,
-                              uint32_t* __restrict__ vals) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < R) vals[i] = (uint32_t)packed[i];
-}
-hipError_t launch_unpack_vals(int64_t R, const uint64_t* packed, uint32_t* vals, hipStream_t s) {
-  if (R > 0) k_unpack_vals<<<(unsigned)((R + 255) / 256), 256, 0, s>>>(R, packed, vals);
   return hipGetLastError();
 }
 
@@ -185,7 +224,7 @@ __global__ __launch_bounds__(256) void k_composite_plain(CompArgs a) {
 }
 
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s) {
-  if (a.T > 0) k_composite_plain<<<a.T, 256, 0, s>>>(a);
+  if (a.T > 0) FGS_TIMED("composite_plain", s, (k_composite_plain<<<a.T, 256, 0, s>>>(a)));
   return hipGetLastError();
 }
 
@@ -290,9 +329,9 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
   if (a.T <= 0) return hipSuccess;
   if (a.bucket <= 64)
-    k_composite_fast<64><<<a.T, 256, 0, s>>>(a);
+    FGS_TIMED("composite_fast", s, (k_composite_fast<64><<<a.T, 256, 0, s>>>(a)));
   else
-    k_composite_fast<256><<<a.T, 256, 0, s>>>(a);
+    FGS_TIMED("composite_fast", s, (k_composite_fast<256><<<a.T, 256, 0, s>>>(a)));
   return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@
 #include "fgs_device.h"
 #include "fgs_layout.h"
 #include "fgs_kernels.h"
+#include "fgs_prof.h"
 
 namespace fgs {
 
@@ -99,30 +100,57 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t* __r
   hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
-template <bool VALS>
+// Stable scatter of one 4096-element tile: the block first sorts its
+// elements by digit in LDS (stable ranks from wave ballots + per-wave digit
+// counts), then writes each digit's run contiguously to its global offset,
+// so stores are coalesced runs instead of one element per digit per wave.
+// SPLIT (key-only sorts): write the low and high 32-bit halves of each key
+// to two arrays instead of the key (used by the last pass of the tile sort to
+// emit the per-tile lists and the tile ids directly).
+template <bool VALS, bool SPLIT = false>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift, uint32_t mask,
-    const uint32_t* __restrict__ hist, int nb) {
+    const uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ lo_out = nullptr,
+    uint32_t* __restrict__ hi_out = nullptr) {
   constexpr int NW = SORT_THREADS / WAVE;
+  __shared__ uint64_t sk[SORT_TILE];
+  __shared__ uint32_t sv[VALS ? SORT_TILE : 1];
   __shared__ uint32_t gofs[256];
+  __shared__ uint32_t bstart[256];
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[NW][256];
   const int t = threadIdx.x;
-  gofs[t] = hist[(size_t)t * nb + blockIdx.x];
-  run[t] = 0;
-#pragma unroll
-  for (int w = 0; w < NW; ++w) wcnt[w][t] = 0;
-  __syncthreads();
   const int w = t / WAVE;
   const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+  const int nvalid = (int)min((int64_t)SORT_TILE, n - base);
+  uint64_t k[SORT_ITEMS];
+  uint32_t v[SORT_ITEMS];
+#pragma unroll
   for (int r = 0; r < SORT_ITEMS; ++r) {
-    const int64_t i = base + r * SORT_THREADS + t;
-    const bool valid = i < n;
-    const uint64_t k = valid ? kin[i] : 0ull;
-    uint32_t v = 0;
-    if (VALS && valid) v = vin[i];
-    const uint32_t d = (uint32_t)(k >> shift) & mask;
+    const int e = r * SORT_THREADS + t;
+    k[r] = e < nvalid ? kin[base + e] : 0ull;
+    v[r] = (VALS && e < nvalid) ? vin[base + e] : 0u;
+  }
+  run[t] = 0;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) wcnt[q][t] = 0;
+  gofs[t] = hist[(size_t)t * nb + blockIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r)
+    if (r * SORT_THREADS + t < nvalid) atomicAdd(&run[(uint32_t)(k[r] >> shift) & mask], 1u);
+  __syncthreads();
+  uint32_t tot;
+  const uint32_t my = run[t];
+  const uint32_t ex = block_excl_scan<SORT_THREADS>(my, tot);
+  bstart[t] = ex;
+  run[t] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    const bool valid = r * SORT_THREADS + t < nvalid;
+    const uint32_t d = (uint32_t)(k[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -134,12 +162,11 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     if (valid && wrank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
     if (valid) {
-      uint32_t off = run[d] + wrank;
+      uint32_t pos = bstart[d] + run[d] + wrank;
 #pragma unroll
-      for (int q = 0; q < NW; ++q) off += (q < w) ? wcnt[q][d] : 0u;
-      const uint32_t dst = gofs[d] + off;
-      kout[dst] = k;
-      if (VALS) vout[dst] = v;
+      for (int q = 0; q < NW; ++q) pos += (q < w) ? wcnt[q][d] : 0u;
+      sk[pos] = k[r];
+      if (VALS) sv[pos] = v[r];
     }
     __syncthreads();
     uint32_t add = 0;
@@ -151,11 +178,23 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     run[t] += add;
     __syncthreads();
   }
+  for (int e = t; e < nvalid; e += SORT_THREADS) {
+    const uint64_t key = sk[e];
+    const uint32_t d = (uint32_t)(key >> shift) & mask;
+    const uint32_t dst = gofs[d] + (uint32_t)e - bstart[d];
+    if (SPLIT) {
+      lo_out[dst] = (uint32_t)key;
+      hi_out[dst] = (uint32_t)(key >> 32);
+    } else {
+      kout[dst] = key;
+    }
+    if (VALS) vout[dst] = sv[e];
+  }
 }
 
 hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                       int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
-                      bool& in_alt) {
+                      bool& in_alt, uint32_t* last_lo, uint32_t* last_hi) {
   in_alt = false;
   if (n <= 1 || end_bit <= begin_bit) return hipSuccess;
   const int nb = (int)((n + SORT_TILE - 1) / SORT_TILE);
@@ -166,14 +205,22 @@ hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
     const uint32_t mask = (1u << bits) - 1u;
-    k_radix_hist<<<nb, SORT_THREADS, 0, s>>>(ka, n, shift, mask, hist, nb);
+    FGS_TIMED("radix_hist", s, (k_radix_hist<<<nb, SORT_THREADS, 0, s>>>(ka, n, shift, mask, hist, nb)));
     hipError_t e = scan_u32(hist, hist, (int64_t)256 * nb, nullptr, scan_ws, s);
     if (e != hipSuccess) return e;
+    const bool last = shift + 8 >= end_bit;
     if (vals)
-      k_radix_scatter<true><<<nb, SORT_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, mask, hist, nb);
+      FGS_TIMED("radix_scatter_kv", s,
+                (k_radix_scatter<true><<<nb, SORT_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, mask,
+                                                                    hist, nb)));
+    else if (last && last_lo)
+      FGS_TIMED("radix_scatter_k_last", s,
+                (k_radix_scatter<false, true><<<nb, SORT_THREADS, 0, s>>>(
+                    ka, nullptr, kb, nullptr, n, shift, mask, hist, nb, last_lo, last_hi)));
     else
-      k_radix_scatter<false><<<nb, SORT_THREADS, 0, s>>>(ka, nullptr, kb, nullptr, n, shift, mask,
-                                                         hist, nb);
+      FGS_TIMED("radix_scatter_k", s,
+                (k_radix_scatter<false><<<nb, SORT_THREADS, 0, s>>>(ka, nullptr, kb, nullptr, n,
+                                                                     shift, mask, hist, nb)));
     uint64_t* tk = ka; ka = kb; kb = tk;
     uint32_t* tv = va; va = vb; vb = tv;
     in_alt = !in_alt;

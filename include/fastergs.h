@@ -157,6 +157,13 @@ size_t fgs_scan_ws_size(int64_t n);
 int fgs_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, void* ws,
                  size_t ws_bytes, void* stream);
 
+/* Per-kernel timing for measurement runs: while enabled, kernel launches are
+ * bracketed by HIP events on their stream; fgs_profile_read synchronises and
+ * writes "name total_ms launches\n" lines (aggregated since the last enable)
+ * into buf and returns the full length. Not thread-safe; off by default. */
+int fgs_profile_enable(int on);
+int fgs_profile_read(char* buf, size_t cap);
+
 const char* fgs_last_error(void);
 const char* fgs_build_info(void);
 

