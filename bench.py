@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--bucket", type=int, default=32)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--compare-plain", type=int, default=1)
+    ap.add_argument("--render-loop", type=int, default=1,
+                    help="also run the graph-captured 120-frame orbit of the dense workload")
     return ap.parse_args()
 
 
@@ -108,6 +110,9 @@ def main():
     if args.compare_plain and sched != SCHEDULE_PLAIN:
         plain_ms, _ = timed(SCHEDULE_PLAIN, max(5, args.steps // 2), 2)
 
+    graph_fps = graph_render_fps(settings, params, sched, args)
+    loop = render_loop(args) if args.render_loop else None
+
     cpu = None
     if args.cpu_baseline:
         cpu = cpu_baseline(scene, cam)
@@ -130,11 +135,13 @@ def main():
                    "parallelism": "single GPU"},
         "render_fps": round(1000.0 / fwd_ms, 2),
         "fwd_ms": round(fwd_ms, 4),
+        "render_fps_graph": graph_fps,
         "wall_ms_per_step": round(wall, 4),
         "plain_ms_per_step": None if plain_ms is None else round(plain_ms, 4),
         "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms, 3),
         "roofline": roofline,
         "stages": stages,
+        "render_loop": loop,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -147,9 +154,11 @@ def main():
 def kernel_units(name, P, M, V, R, T, E, fast):
     sh = 12 + 12 * M  # dc + rest coefficients
     table = {
-        # in: means, opacity, scales, rotations, SH; out: geom 32, splat 48,
-        # depth key 4 (+ sort key 8 and index 4 in the fast schedule), count 4, radius 4
-        "preprocess_fast": ("hbm", P * (12 + 4 + 12 + 16 + sh) + P * (32 + 48 + 4 + 8 + 4 + 4 + 4)),
+        # geometry only: in means, opacity, scales, rotations; out geom 32,
+        # splat 48, depth key 4, depth-sort key 8 + index 4, large flag 4, count 4, radius 4
+        "preprocess_fast": ("hbm", P * (12 + 4 + 12 + 16) + P * (32 + 48 + 4 + 8 + 4 + 4 + 4 + 4)),
+        # colour pass: in dc + rest SH, means, depth key; out rgb + clamp 16
+        "sh_color": ("hbm", P * (sh + 12 + 4) + V * 16),
         "preprocess_plain": ("hbm", P * (12 + 4 + 12 + 16 + sh) + P * (32 + 48 + 4 + 4 + 4)),
         # in: means, scales, rotations, opacity, rest SH, raw sums 48, radius, clamp;
         # out: mean2D 16, conic 16, opacity 4, colour 12, mean3D 12, cov3D 24, dc 12, SH, scale 12, rot 16
@@ -165,8 +174,11 @@ def kernel_units(name, P, M, V, R, T, E, fast):
         "backward_fast": ("valu", 70 * E),
         "backward_plain": ("valu", 70 * E),
     }
-    if not fast:
-        table["radix_scatter_kv"] = ("hbm", 24 * R)  # 64-bit key + 32-bit value, read + write
+    # 64-bit key + 32-bit value, read + write: the pair sort (plain) or the
+    # Gaussian depth sort (fast); histograms read the 8 B keys
+    table["radix_scatter_kv"] = ("hbm", 24 * (P if fast else R))
+    table["radix_hist_kv"] = ("hbm", 8 * (P if fast else R))
+    table["radix_hist_k"] = ("hbm", 8 * R)
     return table.get(name)
 
 
@@ -215,6 +227,100 @@ def kernel_roofline(step, params, settings, sched, args, P):
                 "unit": "GB/s" if bound == "hbm" else "TFLOP/s", "frac": round(ach / peak, 4),
                 "traffic": None, "algorithmic_per_launch": per_launch}
     return stages, roof
+
+
+def _time_replays(rend, camt, reps):
+    import torch
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(reps):
+        rend.cambuf.copy_(camt[k % camt.shape[0]])
+        rend.graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
+def graph_render_fps(settings, params, sched, args):
+    """Forward-only FPS of the heavy workload with the frame captured in one HIP graph."""
+    import torch
+    from fastergs_amd.graph import CapturedRenderer
+    from fastergs_amd.rasterizer import RasterOptions
+    p = {k: v.detach() for k, v in params.items()}
+    rend = CapturedRenderer(settings, p["means3D"], p["opacities"], p["scales"], p["rotations"],
+                            dc=p["dc"], shs=p["shs"],
+                            options=RasterOptions(schedule=sched, bucket_size=args.bucket))
+    camt = rend.cambuf.clone().reshape(1, 35)
+    _time_replays(rend, camt, 5)
+    ms = _time_replays(rend, camt, 50)
+    rend.check()
+    return round(50 * 1000.0 / ms, 2)
+
+
+def render_loop(args, frames=120):
+    """Render-only loop of the dense workload (3M Gaussians, 1920x1080, 120-frame
+    orbit at radius 3.5): each frame is one replay of a captured HIP graph
+    after copying the frame's camera into the buffers the graph reads.
+    Reports FPS over the orbit and the tile sort's HBM bandwidth (algorithmic
+    bytes: per 8-bit pass a histogram read of 8 B and a scatter read + write
+    of 8 + 8 B per pair) from per-kernel HIP events of eager frames."""
+    import torch
+    from fastergs_amd import _lib, scenes
+    from fastergs_amd._lib import SCHEDULE_FAST
+    from fastergs_amd.graph import CapturedRenderer
+    from fastergs_amd.rasterizer import Frame, GaussianRasterizationSettings, RasterOptions
+    dev = torch.device("cuda:0")
+    scene, cam0 = scenes.workload("dense", seed=0)
+    cams = scenes.orbit_cameras(frames, cam0.width, cam0.height, radius=3.5)
+    P = scene["means3D"].shape[0]
+    p = {k: torch.from_numpy(v).to(dev) for k, v in scene.items()}
+    del scene
+
+    def settings_of(c):
+        return GaussianRasterizationSettings(
+            image_height=c.height, image_width=c.width, tanfovx=c.tanfovx, tanfovy=c.tanfovy,
+            bg=torch.zeros(3, device=dev), scale_modifier=1.0,
+            viewmatrix=torch.from_numpy(c.viewmatrix).to(dev),
+            projmatrix=torch.from_numpy(c.projmatrix).to(dev), sh_degree=3,
+            campos=torch.from_numpy(c.campos).to(dev), mult=1.0, prefiltered=False, debug=False)
+
+    camt = torch.stack([torch.from_numpy(np.concatenate(
+        [c.viewmatrix.reshape(16), c.projmatrix.reshape(16), c.campos.reshape(3)]))
+        for c in cams]).to(dev)
+    opts = RasterOptions(schedule=SCHEDULE_FAST, bucket_size=args.bucket)
+    rend = CapturedRenderer(settings_of(cams[0]), p["means3D"], p["opacities"], p["scales"],
+                            p["rotations"], dc=p["dc"], shs=p["shs"], options=opts)
+    for _ in range(4):  # size the capacity for the whole orbit
+        _time_replays(rend, camt, frames)
+        if rend.check(grow=True):
+            break
+    ms = _time_replays(rend, camt, frames)
+    rend.check()
+    R_cap, NB_cap = rend.capacity()
+    del rend
+    torch.cuda.empty_cache()
+    # sort bandwidth and pair counts from eager frames at four orbit positions
+    sort_ms, sort_bytes, Rs = 0.0, 0, []
+    for k in range(0, frames, frames // 4):
+        _lib.profile_enable(True)
+        fr = Frame(settings_of(cams[k]), p["means3D"], p["opacities"], p["scales"],
+                   p["rotations"], dc=p["dc"], shs=p["shs"], options=opts).forward()
+        prof = _lib.profile_read()
+        _lib.profile_enable(False)
+        passes = sum(prof[n][1] for n in ("radix_scatter_k", "radix_scatter_k_last") if n in prof)
+        sort_ms += sum(prof[n][0] for n in ("radix_hist_k", "radix_scatter_k",
+                                            "radix_scatter_k_last") if n in prof)
+        sort_bytes += passes * 24 * fr.R
+        Rs.append(fr.R)
+        del fr
+    torch.cuda.empty_cache()
+    return {"workload": f"dense: {P} Gaussians, {cams[0].width}x{cams[0].height}, SH3, "
+                        f"{frames}-frame orbit r=3.5, one HIP graph per frame",
+            "fps": round(frames * 1000.0 / ms, 2), "ms_per_frame": round(ms / frames, 4),
+            "pairs_sampled": Rs, "pair_capacity": R_cap, "bucket_capacity": NB_cap,
+            "sort_gbs": round(sort_bytes / (sort_ms / 1e3) / 1e9, 1),
+            "sort_frac_hbm": round(sort_bytes / (sort_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def cpu_baseline(scene, cam):

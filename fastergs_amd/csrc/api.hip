@@ -283,7 +283,7 @@ int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_opti
   FGS_TRY(scan_u32(w.lflag, w.lpos, g->P, w.counts + 3, w.scratch, s), "large scan");
   FGS_TRY(launch_compact_large(g->P, w.lflag, w.lpos, w.large_ids, s), "large compact");
   FGS_TRY(launch_large_tiles(false, g->P, w.geom, w.large_ids, w.counts, grid_x_of(cam->width),
-                             w.pcount, nullptr, nullptr, nullptr, s),
+                             w.pcount, nullptr, nullptr, 0u, nullptr, s),
           "large tiles");
   // Gaussians in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by tile alone yields the canonical per-tile order.
@@ -348,19 +348,22 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   if (fast) {
     // emission in depth order, tile-only stable sort, lists, ranges, buckets
     FGS_TRY(fill_u32(gw.lcur, 0u, g->P, s), "zero cursors");
-    FGS_TRY(launch_emit_small(g->P, gw.geom, opt->split_tiles, gx, gw.poff, pw.k0, s), "emit");
+    // R is the pair capacity; the pair count itself (counts[0]) stays on
+    // the device, so nothing here waits for the host.
+    FGS_TRY(launch_emit_small(g->P, gw.geom, opt->split_tiles, gx, gw.poff, (uint32_t)R, pw.k0, s),
+            "emit");
     FGS_TRY(launch_large_tiles(true, g->P, gw.geom, gw.large_ids, gw.counts, gx, nullptr, gw.poff,
-                               gw.lcur, pw.k0, s),
+                               gw.lcur, (uint32_t)R, pw.k0, s),
             "emit large");
     if (R > 1) {
       // the last pass writes the lists (low halves) and tile ids (high halves)
       bool alt = false;
       FGS_TRY(sort_pairs(pw.k0, nullptr, pw.k1, nullptr, R, 32, 32 + tile_bits(T), pw.scratch, s,
-                         alt, pw.v0, pw.tiles),
+                         alt, pw.v0, pw.tiles, gw.counts),
               "tile sort");
-      FGS_TRY(launch_tile_ranges(R, pw.tiles, gw.ranges, T, s), "ranges");
+      FGS_TRY(launch_tile_ranges(R, gw.counts, pw.tiles, gw.ranges, T, s), "ranges");
     } else {
-      FGS_TRY(launch_unpack_ranges(R, pw.k0, pw.v0, gw.ranges, T, s), "lists and ranges");
+      FGS_TRY(launch_unpack_ranges(R, gw.counts, pw.k0, pw.v0, gw.ranges, T, s), "lists and ranges");
     }
     FGS_TRY(launch_bucket_counts(T, gw.ranges, opt->bucket_size, gw.bcount, s), "bucket counts");
     FGS_TRY(scan_u32(gw.bcount, gw.boff, T, gw.boff + T, gw.scratch, s), "bucket scan");
@@ -373,14 +376,17 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   }
 
   // PLAIN: duplicate with keys, 64-bit radix sort, ranges, composite.
-  FGS_TRY(launch_duplicate_plain(g->P, gw.geom, gw.dkey, gw.pcount, gw.poff, gx, pw.k0, pw.v0, s),
+  FGS_TRY(launch_duplicate_plain(g->P, gw.geom, gw.dkey, gw.pcount, gw.poff, gx, (uint32_t)R,
+                                 pw.k0, pw.v0, s),
           "duplicate");
   bool alt = false;
-  FGS_TRY(sort_pairs(pw.k0, pw.v0, pw.k1, pw.v1, R, 0, 32 + tile_bits(T), pw.scratch, s, alt),
+  FGS_TRY(sort_pairs(pw.k0, pw.v0, pw.k1, pw.v1, R, 0, 32 + tile_bits(T), pw.scratch, s, alt,
+                     nullptr, nullptr, gw.counts),
           "sort");
+  if (R > 0 && alt != plain_sorted_in_alt(T)) return fail(FGS_ERR_HIP, "sort parity");
   const uint64_t* keys = alt ? pw.k1 : pw.k0;
   c.vals = alt ? pw.v1 : pw.v0;
-  FGS_TRY(launch_identify_ranges(R, keys, gw.ranges, T, s), "ranges");
+  FGS_TRY(launch_identify_ranges(R, gw.counts, keys, gw.ranges, T, s), "ranges");
   FGS_TRY(launch_composite_plain(c, s), "composite");
   return FGS_OK;
 }

@@ -89,4 +89,11 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
 
 __device__ __forceinline__ void atomic_add_f(float* p, float v) { unsafeAtomicAdd(p, v); }
 
+// Element count of a stage whose size is produced on the device: the value
+// at n_dev (when given) bounded by the host-side capacity cap the launch
+// grid and the buffers were sized for.
+__device__ __forceinline__ int64_t dev_count(const uint32_t* n_dev, int64_t cap) {
+  return n_dev ? min((int64_t)*n_dev, cap) : cap;
+}
+
 }  // namespace fgs

@@ -11,10 +11,13 @@ namespace fgs {
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
                     hipStream_t s);
 // last_lo / last_hi (key-only sorts): the final pass writes the low / high
-// 32 bits of the sorted keys there instead of a key array.
+// 32 bits of the sorted keys there instead of a key array. n_dev: when
+// given, the element count is min(*n_dev, n) read on the device (n is then
+// the capacity the grid is sized for); no host synchronisation either way.
 hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                       int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
-                      uint32_t* last_lo = nullptr, uint32_t* last_hi = nullptr);
+                      uint32_t* last_lo = nullptr, uint32_t* last_hi = nullptr,
+                      const uint32_t* n_dev = nullptr);
 
 // fill.hip-style helpers (render.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
@@ -38,30 +41,33 @@ struct PreArgs {
 };
 hipError_t launch_preprocess_plain(const PreArgs& a, hipStream_t s);
 hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
+// Pair emission writes only slots below cap (the pair capacity); a frame
+// whose pair count exceeds it is detected on the host from the count.
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
-                                  uint64_t* keys, uint32_t* vals, hipStream_t s);
+                                  uint32_t cap, uint64_t* keys, uint32_t* vals, hipStream_t s);
 hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* large_ids,
                               const uint32_t* counts, int gx, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t* lcur, uint64_t* packed,
-                              hipStream_t s);
+                              const uint32_t* poff, uint32_t* lcur, uint32_t cap,
+                              uint64_t* packed, hipStream_t s);
 hipError_t launch_emit_small(int P, const float* geom, int split, int gx, const uint32_t* poff,
-                             uint64_t* packed, hipStream_t s);
+                             uint32_t cap, uint64_t* packed, hipStream_t s);
 hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lpos,
                                 uint32_t* large_ids, hipStream_t s);
 hipError_t launch_gather_scatter(bool gather, int P, const uint32_t* order, const uint32_t* src,
                                  uint32_t* dst, hipStream_t s);
 
 // render.hip
-hipError_t launch_identify_ranges(int64_t R, const uint64_t* keys, uint32_t* ranges, int T,
-                                  hipStream_t s);
+// Range kernels take the pair count as (capacity R, device count n_dev).
+hipError_t launch_identify_ranges(int64_t R, const uint32_t* n_dev, const uint64_t* keys,
+                                  uint32_t* ranges, int T, hipStream_t s);
 hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, const uint32_t* boff,
                                 int bucket, uint32_t* bucket_ranges, uint32_t* bucket_tile,
                                 hipStream_t s);
-hipError_t launch_unpack_ranges(int64_t R, const uint64_t* keys, uint32_t* vals, uint32_t* ranges,
-                                int T, hipStream_t s);
-hipError_t launch_tile_ranges(int64_t R, const uint32_t* tiles, uint32_t* ranges, int T,
-                              hipStream_t s);
+hipError_t launch_unpack_ranges(int64_t R, const uint32_t* n_dev, const uint64_t* keys,
+                                uint32_t* vals, uint32_t* ranges, int T, hipStream_t s);
+hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const uint32_t* tiles,
+                              uint32_t* ranges, int T, hipStream_t s);
 hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
                                 hipStream_t s);
 hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,

@@ -350,10 +350,12 @@ FGS_HD void tile_rect(R x, R y, int w, int h, int gx, int gy, int rect[4]) {
   v = trunc_i((y + (R)h) / by) + 1; rect[3] = v < 0 ? 0 : (v > gy ? gy : v);
 }
 
-// Full per-Gaussian forward (preprocess.cu:450-499 and :698-720).
+// Full per-Gaussian forward (preprocess.cu:450-499 and :698-720). With
+// color == false the colour is left at zero (a separate pass evaluates it).
 template <typename R>
 FGS_HD void preprocess_gauss(int i, const GaussRef<R>& G, const CamRef<R>& cam, Proj<R>& o,
-                             const R* dc_row = nullptr, const R* sh_row = nullptr) {
+                             const R* dc_row = nullptr, const R* sh_row = nullptr,
+                             bool color = true) {
   o.valid = 0;
   o.radius = 0;
   o.clamp = 0;
@@ -408,7 +410,9 @@ FGS_HD void preprocess_gauss(int i, const GaussRef<R>& G, const CamRef<R>& cam, 
   o.depth = t[2];
   o.power = power;
   o.log2op = l2op;
-  if (G.colors != nullptr) {
+  if (!color) {
+    o.rgb[0] = o.rgb[1] = o.rgb[2] = R(0);
+  } else if (G.colors != nullptr) {
     for (int k = 0; k < 3; ++k) o.rgb[k] = G.colors[3 * i + k];
   } else if (dc_row != nullptr) {
     sh_color_rows(pm[0] - cam.campos[0], pm[1] - cam.campos[1], pm[2] - cam.campos[2], dc_row,

@@ -103,7 +103,8 @@ __global__ __launch_bounds__(256) void k_duplicate_plain(int P, const float* __r
                                                          const uint32_t* __restrict__ dkey,
                                                          const uint32_t* __restrict__ pcount,
                                                          const uint32_t* __restrict__ poff,
-                                                         int grid_x, uint64_t* __restrict__ keys,
+                                                         int grid_x, uint32_t cap,
+                                                         uint64_t* __restrict__ keys,
                                                          uint32_t* __restrict__ vals) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P || pcount[i] == 0) return;
@@ -113,19 +114,21 @@ __global__ __launch_bounds__(256) void k_duplicate_plain(int P, const float* __r
   for (int ty = g.r1; ty < g.r3; ++ty)
     for (int tx = g.r0; tx < g.r2; ++tx)
       if (tile_hit(tx, ty, g.x, g.y, g.conic, g.power)) {
-        keys[k] = ((uint64_t)(uint32_t)(ty * grid_x + tx) << 32) | d;
-        vals[k] = (uint32_t)i;
+        if (k < cap) {
+          keys[k] = ((uint64_t)(uint32_t)(ty * grid_x + tx) << 32) | d;
+          vals[k] = (uint32_t)i;
+        }
         ++k;
       }
 }
 
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
-                                  uint64_t* keys, uint32_t* vals, hipStream_t s) {
+                                  uint32_t cap, uint64_t* keys, uint32_t* vals, hipStream_t s) {
   if (P > 0)
     FGS_TIMED("duplicate_plain", s,
               (k_duplicate_plain<<<(P + 255) / 256, 256, 0, s>>>(P, geom, dkey, pcount, poff, grid_x,
-                                                                 keys, vals)));
+                                                                 cap, keys, vals)));
   return hipGetLastError();
 }
 
@@ -145,7 +148,7 @@ struct WaveTiles {
 // is free: its pairs are in distinct tiles and the tile sort is stable.
 template <bool EMIT>
 __device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, int gbase, int gx,
-                                           const uint32_t* __restrict__ poff,
+                                           const uint32_t* __restrict__ poff, uint32_t cap,
                                            uint64_t* __restrict__ packed) {
   const int lane = lane_id();
   for (uint32_t base = 0; base < S; base += WAVE) {
@@ -165,7 +168,8 @@ __device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, int gbase, 
         const uint32_t k = atomicAdd(&L.cnt[j], 1u);
         if (EMIT) {
           const uint32_t g = (uint32_t)(gbase + j);
-          packed[poff[g] + k] = ((uint64_t)(uint32_t)(ty * gx + tx) << 32) | g;
+          const uint32_t pos = poff[g] + k;
+          if (pos < cap) packed[pos] = ((uint64_t)(uint32_t)(ty * gx + tx) << 32) | g;
         }
       }
     }
@@ -192,34 +196,20 @@ __device__ __forceinline__ void stage_small(WaveTiles& L, int lane, bool small, 
   S = (uint32_t)__shfl(inc, WAVE - 1, WAVE);
 }
 
-// One wave64 per workgroup: the dc / SH rows of its 64 Gaussians are staged
-// in LDS with coalesced loads (a thread's own 192-byte row would otherwise
-// be 48 scattered loads per wave), then the same LDS holds the wave's
-// lane-packed tile walk. 12 KB of LDS per workgroup keeps occupancy high.
+// Geometry pass of the fast preprocess: one thread per Gaussian for the math
+// (colour comes from k_sh_color), then the wave's lane-packed tile walk.
+// Only 3 KB of LDS per wave64 workgroup, so occupancy is set by registers.
 constexpr int PRE_THREADS = WAVE;
 __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a) {
-  extern __shared__ float rows[];  // 64 x 3 dc, then 64 x 3M rest
+  __shared__ WaveTiles L;
   const int i = blockIdx.x * PRE_THREADS + threadIdx.x;
   const int lane = threadIdx.x;
-  WaveTiles& L = *(WaveTiles*)rows;
   const int g0 = blockIdx.x * PRE_THREADS;
-  const int M = a.cam.max_coeffs;
-  const bool staged = a.G.colors == nullptr && a.G.dc != nullptr;
-  float* sdc = rows;
-  float* ssh = rows + 3 * PRE_THREADS;
-  if (staged) {
-    const int nb = min(PRE_THREADS, a.P - g0);
-    for (int k = lane; k < 3 * nb; k += PRE_THREADS) sdc[k] = a.G.dc[3 * (size_t)g0 + k];
-    if (M > 0)
-      for (int k = lane; k < 3 * M * nb; k += PRE_THREADS)
-        ssh[k] = a.G.shs[(size_t)3 * M * g0 + k];
-  }
-  __syncthreads();
   Proj<float> o;
   o.valid = 0;
   if (i < a.P) {
-    preprocess_gauss(i, a.G, a.cam, o, staged ? sdc + 3 * lane : nullptr,
-                     staged ? ssh + 3 * M * lane : nullptr);
+    // SH colour is evaluated by k_sh_color; precomputed colours are copied here
+    preprocess_gauss<float>(i, a.G, a.cam, o, nullptr, nullptr, a.G.colors != nullptr);
     a.radii[i] = o.valid ? o.radius : 0;
     const uint32_t dk = o.valid ? depth_bits(o.depth) : 0xffffffffu;
     a.dkey[i] = dk;
@@ -235,23 +225,66 @@ __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a) {
   // large rects: flagged here, compacted by a scan (no same-address atomics)
   if (i < a.P) a.lflag[i] = (o.valid && !small) ? 1u : 0u;
   uint32_t S;
-  __syncthreads();  // the SH rows are dead; the tile walk reuses the LDS
   stage_small(L, lane, small, o.x, o.y, o.conic, o.power, o.rect[0], o.rect[1], o.rect[2],
               o.rect[3], S);
   __syncthreads();
-  walk_small<false>(L, S, g0, a.cam.grid_x, nullptr, nullptr);
+  walk_small<false>(L, S, g0, a.cam.grid_x, nullptr, 0u, nullptr);
   __syncthreads();
   if (i < a.P) a.pcount[i] = L.cnt[lane];
 }
 
+// Colour pass: SH degree <= 3 of every valid Gaussian (preprocess.cu:698-707)
+// into its splat record (r, g, b, clamp bits). One wave64 per workgroup; the
+// dc / SH rows of its 64 Gaussians are staged in LDS with coalesced loads (a
+// thread's own 192-byte row would otherwise be 48 scattered loads per wave).
+__global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
+  extern __shared__ float rows[];  // 64 x 3 dc, then 64 x 3M rest
+  const int lane = threadIdx.x;
+  const int g0 = blockIdx.x * PRE_THREADS;
+  const int i = g0 + lane;
+  const int M = a.cam.max_coeffs;
+  float* sdc = rows;
+  float* ssh = rows + 3 * PRE_THREADS;
+  const int nb = min(PRE_THREADS, a.P - g0);
+  const float* gdc = a.G.dc + 3 * (size_t)g0;
+  const float* gsh = a.G.shs + (size_t)3 * M * g0;
+  // full groups of 64 rows start 16-byte aligned when the arrays do
+  // (3 * 64 and 3 * M * 64 floats are multiples of 4): copy as float4
+  const bool vec = nb == PRE_THREADS &&
+                   (((uintptr_t)a.G.dc | (uintptr_t)(M > 0 ? a.G.shs : a.G.dc)) & 15) == 0;
+  if (vec) {
+    const float4* d4 = (const float4*)gdc;
+    float4* s4 = (float4*)sdc;
+    if (lane < 3 * PRE_THREADS / 4) s4[lane] = d4[lane];
+    const float4* h4 = (const float4*)gsh;
+    float4* t4 = (float4*)ssh;
+    for (int k = lane; k < 3 * M * PRE_THREADS / 4; k += PRE_THREADS) t4[k] = h4[k];
+  } else {
+    for (int k = lane; k < 3 * nb; k += PRE_THREADS) sdc[k] = gdc[k];
+    for (int k = lane; k < 3 * M * nb; k += PRE_THREADS) ssh[k] = gsh[k];
+  }
+  __syncthreads();
+  if (i >= a.P || a.dkey[i] == 0xffffffffu) return;
+  const float* pm = a.G.means + 3 * (size_t)i;
+  float rgb[3];
+  int clamp;
+  sh_color_rows(pm[0] - a.cam.campos[0], pm[1] - a.cam.campos[1], pm[2] - a.cam.campos[2],
+                sdc + 3 * lane, ssh + 3 * M * lane, a.cam.deg, rgb, clamp);
+  float* sp = a.splat + 12 * (size_t)i;
+  sp[6] = rgb[0];
+  sp[7] = rgb[1];
+  sp[8] = rgb[2];
+  sp[10] = u2f((uint32_t)clamp);
+}
+
 hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s) {
   if (a.P <= 0) return hipSuccess;
-  const bool staged = a.G.colors == nullptr && a.G.dc != nullptr;
-  size_t shmem =
-      staged ? (size_t)PRE_THREADS * 3 * (1 + a.cam.max_coeffs) * sizeof(float) : 0;
-  if (shmem < sizeof(WaveTiles)) shmem = sizeof(WaveTiles);
-  FGS_TIMED("preprocess_fast", s,
-            (k_preprocess_fast<<<(a.P + PRE_THREADS - 1) / PRE_THREADS, PRE_THREADS, shmem, s>>>(a)));
+  const unsigned blocks = (a.P + PRE_THREADS - 1) / PRE_THREADS;
+  FGS_TIMED("preprocess_fast", s, (k_preprocess_fast<<<blocks, PRE_THREADS, 0, s>>>(a)));
+  if (a.G.colors == nullptr && a.G.dc != nullptr) {
+    const size_t shmem = (size_t)PRE_THREADS * 3 * (1 + a.cam.max_coeffs) * sizeof(float);
+    FGS_TIMED("sh_color", s, (k_sh_color<<<blocks, PRE_THREADS, shmem, s>>>(a)));
+  }
   return hipGetLastError();
 }
 
@@ -265,7 +298,8 @@ __global__ __launch_bounds__(256) void k_large_tiles(const float* __restrict__ g
                                                      const uint32_t* __restrict__ counts, int gx,
                                                      uint32_t* pcount,
                                                      const uint32_t* __restrict__ poff,
-                                                     uint32_t* lcur, uint64_t* packed) {
+                                                     uint32_t* lcur, uint32_t cap,
+                                                     uint64_t* packed) {
   const uint32_t NL = counts[3];
   const int lane = lane_id();
   const int w = threadIdx.x / WAVE;
@@ -288,9 +322,9 @@ __global__ __launch_bounds__(256) void k_large_tiles(const float* __restrict__ g
         uint32_t slot0 = 0;
         if (lane == 0) slot0 = atomicAdd(&lcur[g], (uint32_t)__popcll(m));
         slot0 = __shfl(slot0, 0, WAVE);
-        if (hit)
-          packed[poff[g] + slot0 + lanes_below(m)] =
-              ((uint64_t)(uint32_t)((r.r1 + ty) * gx + r.r0 + tx) << 32) | (uint32_t)g;
+        const uint32_t pos = poff[g] + slot0 + lanes_below(m);
+        if (hit && pos < cap)
+          packed[pos] = ((uint64_t)(uint32_t)((r.r1 + ty) * gx + r.r0 + tx) << 32) | (uint32_t)g;
       }
       l += 4 * WAVE;
       tx += sx;
@@ -306,24 +340,24 @@ __global__ __launch_bounds__(256) void k_large_tiles(const float* __restrict__ g
 
 hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* large_ids,
                               const uint32_t* counts, int gx, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t* lcur, uint64_t* packed,
-                              hipStream_t s) {
+                              const uint32_t* poff, uint32_t* lcur, uint32_t cap,
+                              uint64_t* packed, hipStream_t s) {
   if (P <= 0) return hipSuccess;
   const int blocks = 2048;  // persistent workgroups; each exits past the device-side count
   if (emit)
     FGS_TIMED("large_tiles_emit", s,
               (k_large_tiles<true><<<blocks, 256, 0, s>>>(geom, large_ids, counts, gx, pcount, poff,
-                                                          lcur, packed)));
+                                                          lcur, cap, packed)));
   else
     FGS_TIMED("large_tiles_count", s,
               (k_large_tiles<false><<<blocks, 256, 0, s>>>(geom, large_ids, counts, gx, pcount,
-                                                           poff, lcur, packed)));
+                                                           poff, lcur, cap, packed)));
   return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void k_emit_small(int P, const float* __restrict__ geom,
                                                     int split, int gx,
-                                                    const uint32_t* __restrict__ poff,
+                                                    const uint32_t* __restrict__ poff, uint32_t cap,
                                                     uint64_t* __restrict__ packed) {
   __shared__ WaveTiles lds[4];
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -340,14 +374,15 @@ __global__ __launch_bounds__(256) void k_emit_small(int P, const float* __restri
   uint32_t S;
   stage_small(L, lane, small, r.x, r.y, r.conic, r.power, r.r0, r.r1, r.r2, r.r3, S);
   __syncthreads();
-  walk_small<true>(L, S, gbase, gx, poff, packed);
+  walk_small<true>(L, S, gbase, gx, poff, cap, packed);
 }
 
 hipError_t launch_emit_small(int P, const float* geom, int split, int gx, const uint32_t* poff,
-                             uint64_t* packed, hipStream_t s) {
+                             uint32_t cap, uint64_t* packed, hipStream_t s) {
   if (P > 0)
     FGS_TIMED("emit_small", s,
-              (k_emit_small<<<(P + 255) / 256, 256, 0, s>>>(P, geom, split, gx, poff, packed)));
+              (k_emit_small<<<(P + 255) / 256, 256, 0, s>>>(P, geom, split, gx, poff, cap,
+                                                             packed)));
   return hipGetLastError();
 }
 

@@ -85,9 +85,12 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 
 // ------------------------------------------------------------------ radix
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
-                                                             int64_t n, int shift, uint32_t mask,
+                                                             int64_t n_cap,
+                                                             const uint32_t* __restrict__ n_dev,
+                                                             int shift, uint32_t mask,
                                                              uint32_t* __restrict__ hist, int nb) {
   __shared__ uint32_t cnt[256];
+  const int64_t n = dev_count(n_dev, n_cap);
   cnt[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
@@ -110,8 +113,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t* __r
 template <bool VALS, bool SPLIT = false>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, int64_t n, int shift, uint32_t mask,
-    const uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ lo_out = nullptr,
+    uint32_t* __restrict__ vout, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
+    uint32_t mask, const uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ lo_out = nullptr,
     uint32_t* __restrict__ hi_out = nullptr) {
   constexpr int NW = SORT_THREADS / WAVE;
   __shared__ uint64_t sk[SORT_TILE];
@@ -123,6 +126,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
   const int t = threadIdx.x;
   const int w = t / WAVE;
   const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+  const int64_t n = dev_count(n_dev, n_cap);
+  if (base >= n) return;  // whole workgroup: past the device-side count
   const int nvalid = (int)min((int64_t)SORT_TILE, n - base);
   uint64_t k[SORT_ITEMS];
   uint32_t v[SORT_ITEMS];
@@ -194,9 +199,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
 
 hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                       int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
-                      bool& in_alt, uint32_t* last_lo, uint32_t* last_hi) {
+                      bool& in_alt, uint32_t* last_lo, uint32_t* last_hi, const uint32_t* n_dev) {
   in_alt = false;
-  if (n <= 1 || end_bit <= begin_bit) return hipSuccess;
+  // with a device-side count the passes always run (the count may be 0 or 1)
+  if ((n <= 1 && !n_dev) || n <= 0 || end_bit <= begin_bit) return hipSuccess;
   const int nb = (int)((n + SORT_TILE - 1) / SORT_TILE);
   uint32_t* hist = (uint32_t*)ws;
   void* scan_ws = (char*)ws + align_up((size_t)256 * nb * sizeof(uint32_t));
@@ -205,22 +211,23 @@ hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
     const uint32_t mask = (1u << bits) - 1u;
-    FGS_TIMED("radix_hist", s, (k_radix_hist<<<nb, SORT_THREADS, 0, s>>>(ka, n, shift, mask, hist, nb)));
+    FGS_TIMED(vals ? "radix_hist_kv" : "radix_hist_k", s,
+              (k_radix_hist<<<nb, SORT_THREADS, 0, s>>>(ka, n, n_dev, shift, mask, hist, nb)));
     hipError_t e = scan_u32(hist, hist, (int64_t)256 * nb, nullptr, scan_ws, s);
     if (e != hipSuccess) return e;
     const bool last = shift + 8 >= end_bit;
     if (vals)
       FGS_TIMED("radix_scatter_kv", s,
-                (k_radix_scatter<true><<<nb, SORT_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, mask,
-                                                                    hist, nb)));
+                (k_radix_scatter<true><<<nb, SORT_THREADS, 0, s>>>(ka, va, kb, vb, n, n_dev, shift,
+                                                                    mask, hist, nb)));
     else if (last && last_lo)
       FGS_TIMED("radix_scatter_k_last", s,
                 (k_radix_scatter<false, true><<<nb, SORT_THREADS, 0, s>>>(
-                    ka, nullptr, kb, nullptr, n, shift, mask, hist, nb, last_lo, last_hi)));
+                    ka, nullptr, kb, nullptr, n, n_dev, shift, mask, hist, nb, last_lo, last_hi)));
     else
       FGS_TIMED("radix_scatter_k", s,
                 (k_radix_scatter<false><<<nb, SORT_THREADS, 0, s>>>(ka, nullptr, kb, nullptr, n,
-                                                                     shift, mask, hist, nb)));
+                                                                     n_dev, shift, mask, hist, nb)));
     uint64_t* tk = ka; ka = kb; kb = tk;
     uint32_t* tv = va; va = vb; vb = tv;
     in_alt = !in_alt;

@@ -13,6 +13,14 @@
  * diff_gaussian_rasterization_faster/__init__.py:80-155):
  *   fgs_preprocess -> fgs_read_counts (the one host sync: pairs R, buckets NB)
  *   -> fgs_render.
+ * fgs_render and fgs_backward take R and NB as capacities and read the
+ * actual counts on the device, so with fixed capacities (e.g. from a
+ * previous frame plus headroom) preprocess + render + backward contain no
+ * host synchronisation and can be captured into one hipGraph; camera
+ * matrices are read through device pointers at run time, so a replay after
+ * rewriting them renders the new view. Such a frame is valid iff the counts
+ * that fgs_read_counts returns afterwards are <= the capacities used
+ * (pairs beyond the capacity are dropped, never written out of bounds).
  * Backward (__init__.py:159-187): fgs_backward, then optionally
  * fgs_densify_stats.
  */
@@ -115,7 +123,9 @@ int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fg
 
 /* Replaces sort_gaussian + render_16x16 (sort.cu:28-44, render.cu:393-451):
  * pair emission, sort, tile ranges, bucket tables and compositing. Writes
- * out_color (3 x H x W), T_final (H x W), last_contributor (H x W int32). */
+ * out_color (3 x H x W), T_final (H x W), last_contributor (H x W int32).
+ * R, NB: pair and bucket-table capacities (>= fgs_read_counts' values for
+ * a complete frame); pair_ws must be fgs_pair_ws_size(..., R, NB, ...). */
 int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
                void* gauss_ws, size_t gauss_ws_bytes, void* pair_ws, size_t pair_ws_bytes,
                int64_t R, int64_t NB, float* out_color, float* T_final,
