@@ -15,6 +15,13 @@ LIB_PATH = os.path.join(HERE, "libfastergs.so")
 SCHEDULE_PLAIN = 0
 SCHEDULE_FAST = 1
 
+TILES_RASTER = 0
+TILES_HEURISTIC = 1
+TILES_LEARNED = 2
+MLP_IN, MLP_HIDDEN, MLP_OUT = 8, 64, 2
+MLP_NPARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN \
+    + MLP_OUT * MLP_HIDDEN + MLP_OUT  # 4866
+
 vp = ctypes.c_void_p
 
 
@@ -32,16 +39,22 @@ class Gaussians(ctypes.Structure):
                 ("dc", vp), ("shs", vp), ("colors_precomp", vp), ("culling", vp)]
 
 
+class TileSched(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("split_len", ctypes.c_int), ("mlp_params", vp),
+                ("features", ctypes.c_int), ("log_cycles", ctypes.c_int)]
+
+
 class Options(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int), ("bucket_size", ctypes.c_int),
-                ("split_tiles", ctypes.c_int), ("abs_grad", ctypes.c_int)]
+                ("split_tiles", ctypes.c_int), ("abs_grad", ctypes.c_int),
+                ("tile_sched", ctypes.POINTER(TileSched))]
 
 
 class FrameView(ctypes.Structure):
     _fields_ = [(n, vp) for n in (
         "geom", "splat", "pair_count", "tile_count", "ranges", "bucket_off", "max_contrib",
         "counts", "sorted_keys", "sorted_vals", "bucket_ranges", "bucket_tile", "acc_t", "acc_c",
-        "pixel_colors")]
+        "pixel_colors", "tile_features", "tile_pred", "work_items", "item_ticks")]
 
 
 class Grads(ctypes.Structure):
@@ -80,6 +93,13 @@ SIGNATURES = {
                                       ctypes.c_int, vp, ctypes.c_size_t, vp]),
     "fgs_scan_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
     "fgs_scan_u32": (ctypes.c_int, [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_size_t, vp]),
+    "fgs_mlp_forward": (ctypes.c_int, [vp, vp, ctypes.c_int64, vp, ctypes.c_int, vp]),
+    "fgs_mlp_grad_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
+    "fgs_mlp_loss_grad": (ctypes.c_int, [vp, vp, vp, ctypes.c_int64, ctypes.c_float, vp, vp, vp,
+                                         ctypes.c_size_t, vp]),
+    "fgs_adam_update": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_int, vp]),
     "fgs_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fgs_profile_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fgs_last_error": (ctypes.c_char_p, []),

@@ -10,6 +10,7 @@
 
 #include "../../include/fastergs.h"
 #include "fgs_kernels.h"
+#include "fgs_sched.h"
 #include "fgs_layout.h"
 #include "fgs_prof.h"
 
@@ -62,6 +63,12 @@ int check_opts(const fgs_options* o) {
     return fail(FGS_ERR_ARG, "bucket_size must be 16, 32, 64, 128 or 256");
   if (o->split_tiles < 1 || o->split_tiles > (1 << 24))
     return fail(FGS_ERR_ARG, "split_tiles must be in [1, 2^24]");
+  if (const fgs_tile_sched* ts = o->tile_sched) {
+    if (ts->mode < FGS_TILES_RASTER || ts->mode > FGS_TILES_LEARNED)
+      return fail(FGS_ERR_ARG, "unknown tile schedule mode");
+    if (ts->mode == FGS_TILES_LEARNED && !ts->mlp_params)
+      return fail(FGS_ERR_ARG, "learned tile schedule needs mlp_params");
+  }
   return FGS_OK;
 }
 
@@ -345,6 +352,9 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   c.pixc = pw.pixc;
   c.maxc = gw.maxc;
   c.acc = pw.acc;
+  c.items = nullptr;
+  c.n_items = nullptr;
+  c.cycles = nullptr;
   if (fast) {
     // emission in depth order, tile-only stable sort, lists, ranges, buckets
     FGS_TRY(fill_u32(gw.lcur, 0u, g->P, s), "zero cursors");
@@ -371,6 +381,21 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
                                  pw.bucket_tile, s),
             "bucket tables");
     c.vals = pw.v0;
+    if (const fgs_tile_sched* ts = opt->tile_sched) {
+      const bool learned = ts->mode == FGS_TILES_LEARNED;
+      if (learned || ts->features)
+        FGS_TRY(launch_tile_features(T, gw.ranges, pw.v0, gw.geom, gw.splat, gw.feat, s),
+                "tile features");
+      if (learned) FGS_TRY(launch_mlp_infer(ts->mlp_params, gw.feat, T, gw.pred, true, s), "cost model");
+      if (ts->mode != FGS_TILES_RASTER) {
+        FGS_TRY(launch_sched_items(T, gw.ranges, opt->bucket_size, learned ? gw.pred : nullptr,
+                                   ts->split_len, gw.items, gw.counts + 4, gw.maxc, s),
+                "work items");
+        c.items = gw.items;
+        c.n_items = gw.counts + 4;
+      }
+      if (ts->log_cycles) c.cycles = gw.cycles;
+    }
     FGS_TRY(launch_composite_fast(c, s), "composite");
     return FGS_OK;
   }
@@ -501,6 +526,49 @@ int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int heig
     }
     v->pixel_colors = pw.pixc;
   }
+  if (fast) {
+    v->tile_features = gw.feat;
+    v->tile_pred = gw.pred;
+    v->work_items = gw.items;
+    v->item_ticks = gw.cycles;
+  }
+  return FGS_OK;
+}
+
+int fgs_mlp_forward(const float* params, const float* x, int64_t n, float* y, int bf16,
+                    void* stream) {
+  if (n < 0 || (n > 0 && (!params || !x || !y))) return fail(FGS_ERR_ARG, "bad MLP arguments");
+  FGS_TRY(launch_mlp_infer(params, x, n, y, bf16 != 0, (hipStream_t)stream), "MLP forward");
+  return FGS_OK;
+}
+
+size_t fgs_mlp_grad_ws_size(int64_t n) { return mlp_grad_ws_bytes(n); }
+
+int fgs_mlp_loss_grad(const float* params, const float* x, const float* target, int64_t n,
+                      float split_weight, float* grad, float* loss, void* ws, size_t ws_bytes,
+                      void* stream) {
+  if (n < 0 || !params || !grad || (n > 0 && (!x || !target)))
+    return fail(FGS_ERR_ARG, "bad MLP arguments");
+  if (n > 0 && (!ws || ws_bytes < mlp_grad_ws_bytes(n)))
+    return fail(FGS_ERR_CAPACITY, "MLP workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    FGS_TRY(fill_f32(grad, 0.0f, MLP_NPARAMS, s), "MLP zero grad");
+    if (loss) FGS_TRY(fill_f32(loss, 0.0f, 1, s), "MLP zero loss");
+    return FGS_OK;
+  }
+  FGS_TRY(launch_mlp_loss_grad(params, x, target, n, split_weight, grad, loss, ws, s), "MLP grad");
+  return FGS_OK;
+}
+
+int fgs_adam_update(float* params, const float* grad, float* exp_avg, float* exp_avg_sq,
+                    int64_t n, float lr, float beta1, float beta2, float eps, int step,
+                    void* stream) {
+  if (n < 0 || step < 1 || (n > 0 && (!params || !grad || !exp_avg || !exp_avg_sq)))
+    return fail(FGS_ERR_ARG, "bad Adam arguments");
+  FGS_TRY(launch_adam(params, grad, exp_avg, exp_avg_sq, n, lr, beta1, beta2, eps, step,
+                      (hipStream_t)stream),
+          "Adam");
   return FGS_OK;
 }
 

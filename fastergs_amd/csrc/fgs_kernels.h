@@ -86,9 +86,30 @@ struct CompArgs {
   float* pixc;
   uint32_t* maxc;
   float* acc;            // fast checkpoints
+  // fast: optional work list (tile << 2 | part: 0 whole tile, 1 / 2 upper /
+  // lower 16x8 band) of n_items[0] <= 2T items, and per-item duration log
+  const uint32_t* items;
+  const uint32_t* n_items;
+  uint32_t* cycles;
 };
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);
+
+// sched.hip
+hipError_t launch_tile_features(int T, const uint32_t* ranges, const uint32_t* vals,
+                                const float* geom, const float* splat, float* feat,
+                                hipStream_t s);
+hipError_t launch_mlp_infer(const float* params, const float* x, int64_t n, float* y, bool bf16,
+                            hipStream_t s);
+size_t mlp_grad_ws_bytes(int64_t n);
+hipError_t launch_mlp_loss_grad(const float* params, const float* x, const float* target,
+                                int64_t n, float split_weight, float* grad, float* loss, void* ws,
+                                hipStream_t s);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
+                       float b1, float b2, float eps, int step, hipStream_t s);
+hipError_t launch_sched_items(int T, const uint32_t* ranges, int bucket, const float* pred,
+                              int split_len, uint32_t* items, uint32_t* n_items, uint32_t* maxc,
+                              hipStream_t s);
 
 // backward.hip
 struct BwdArgs {

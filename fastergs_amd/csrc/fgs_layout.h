@@ -7,6 +7,8 @@
 //     dkey / pcount / poff   P x 4 B each (+1 for the offsets' total)
 //     depth-sort keys/values P x (8+8+4+4) B (fast schedule)
 //     tile histogram, ranges, bucket offsets, per-tile max contributor  T x 20 B
+//     tile schedule: features T x 32 B, predictions T x 8 B, work items and
+//     their timing log 2T x 8 B
 //     counters and scan/sort scratch
 //   pair workspace (depends on pairs R, buckets NB, pixels HW):
 //     keys (2 x R x 8 B), values (2 x R x 4 B), sort histogram scratch,
@@ -72,7 +74,11 @@ struct GaussWs {
   uint32_t* bcount;    // T + 1 scratch (bucket counts, then active bucket counts)
   uint32_t* actoff;    // T + 1: exclusive prefix of per-tile active buckets, [T] = total
   uint32_t* maxc;      // T
-  uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects
+  uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects, [4] work items
+  float* feat;         // T x 8 tile features (scheduled compositing)
+  float* pred;         // T x 2 cost-model outputs (predicted log2 cost, split logit)
+  uint32_t* items;     // 2T compositing work items (tile << 2 | part)
+  uint32_t* cycles;    // 2T per-item timing log (100 MHz ticks)
   void* scratch;
   size_t scratch_bytes;
   size_t total;
@@ -104,6 +110,10 @@ inline GaussWs carve_gauss(void* base, int P, int T) {
   w.actoff = c.take<uint32_t>(t + 1);
   w.maxc = c.take<uint32_t>(t);
   w.counts = c.take<uint32_t>(16);
+  w.feat = c.take<float>(8 * t);
+  w.pred = c.take<float>(2 * t);
+  w.items = c.take<uint32_t>(2 * t);
+  w.cycles = c.take<uint32_t>(2 * t);
   size_t sb = sort_ws_bytes((int64_t)p);
   size_t s2 = scan_ws_bytes((int64_t)p + 1);
   size_t s3 = scan_ws_bytes((int64_t)t + 1);

@@ -240,16 +240,45 @@ hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s) {
 }
 
 // ------------------------------------------------------- FAST compositing
-// One 256-thread workgroup (4 wave64) per 16x16 tile. Each round the four
-// waves composite four consecutive buckets of the tile at once, every wave
-// over all 256 pixels (4 pixels per lane, 2x2 quads) starting from T = 1
-// (render.cu:130-275); the bucket splats are staged in LDS. The partial
-// (T_b, C_b) of the four buckets are then merged in order per pixel
+// One 256-thread workgroup (4 wave64) per work item: a whole 16x16 tile, or
+// one of its two 16x8 row bands when the tile schedule splits it (pixels
+// are independent, so a band does exactly the per-pixel work of the whole
+// tile for its rows). Each round the four waves composite four consecutive
+// buckets of the tile at once, every wave over all pixels of the item (4
+// pixels per lane in 2x2 quads, or 2 per lane for a band) starting from
+// T = 1 (render.cu:130-275); the bucket splats are staged in LDS. The
+// partial (T_b, C_b) of the four buckets are then merged in order per pixel
 // (render.cu:329-362), the checkpoint after each bucket is stored for the
 // backward, and a pixel stops after the first bucket that leaves T < 1e-4.
-// The workgroup stops at the first round in which every pixel has stopped,
-// so buckets past the tile's cutoff (which the reference renders and then
-// discards) are never evaluated; the outputs are unchanged by this.
+// The workgroup stops at the first round in which every pixel of the item
+// has stopped, so buckets past the cutoff (which the reference renders and
+// then discards) are never evaluated; the outputs are unchanged by this,
+// and by the item order and the splitting.
+template <int NPX>
+__device__ __forceinline__ void bucket_pass(const float4* sA, const float4* sB, const float* sC,
+                                            int n, float fx0, float fy0, float4* out, int lp0) {
+  const float fx1 = fx0 + 1.0f, fy1 = fy0 + 1.0f;
+  float T0 = 1.f, T1 = 1.f, T2 = 1.f, T3 = 1.f;
+  float c00 = 0.f, c01 = 0.f, c02 = 0.f, c10 = 0.f, c11 = 0.f, c12 = 0.f;
+  float c20 = 0.f, c21 = 0.f, c22 = 0.f, c30 = 0.f, c31 = 0.f, c32 = 0.f;
+  for (int k = 0; k < n; ++k) {
+    const float4 A = sA[k], Bv = sB[k];
+    const float cb = sC[k];
+    blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx0, fy0), Bv.z, Bv.w, cb, T0, c00, c01, c02);
+    blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx1, fy0), Bv.z, Bv.w, cb, T1, c10, c11, c12);
+    if (NPX == 4) {
+      blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx0, fy1), Bv.z, Bv.w, cb, T2, c20, c21, c22);
+      blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx1, fy1), Bv.z, Bv.w, cb, T3, c30, c31, c32);
+    }
+  }
+  out[lp0] = make_float4(c00, c01, c02, T0);
+  out[lp0 + 1] = make_float4(c10, c11, c12, T1);
+  if (NPX == 4) {
+    out[lp0 + TILE_W] = make_float4(c20, c21, c22, T2);
+    out[lp0 + TILE_W + 1] = make_float4(c30, c31, c32, T3);
+  }
+}
+
 template <int BMAX>
 __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   __shared__ float4 sA[4][BMAX];
@@ -257,17 +286,28 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   __shared__ float sC[4][BMAX];
   __shared__ float4 sP[4][TILE_PIX];
   __shared__ int wmax[4];
-  const int tile = blockIdx.x;
+  const uint64_t t_start = a.cycles ? wall_clock64() : 0;
+  uint32_t item = (uint32_t)blockIdx.x << 2;
+  if (a.items) {
+    if (blockIdx.x >= a.n_items[0]) return;  // whole workgroup
+    item = a.items[blockIdx.x];
+  }
+  const int tile = (int)(item >> 2), part = (int)(item & 3u);
+  const bool band = part != 0;
+  const int row0 = part == 2 ? TILE_H / 2 : 0;
+  const int nrows = band ? TILE_H / 2 : TILE_H;
   const int tx = tile % a.grid_x, ty = tile / a.grid_x;
   const int t = threadIdx.x;
   const int w = t / WAVE, lane = t & (WAVE - 1);
-  // merge / output pixel of this thread
-  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
-  const bool inside = px < a.W && py < a.H;
-  // the 2x2 quad of this lane in the bucket pass
-  const int qx = 2 * (lane & 7), qy = 2 * (lane >> 3);
+  // merge / output pixel of this thread (lp: index inside the tile)
+  const int mrow = row0 + (t >> 4);
+  const int lp = mrow * TILE_W + (t & 15);
+  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + mrow;
+  const bool merger = (t >> 4) < nrows;
+  const bool inside = merger && px < a.W && py < a.H;
+  // the pixels of this lane in the bucket pass: a 2x2 quad, or a 2x1 pair in a band
+  const int qx = 2 * (lane & 7), qy = band ? row0 + (lane >> 3) : 2 * (lane >> 3);
   const float fx0 = (float)(tx * TILE_W + qx), fy0 = (float)(ty * TILE_H + qy);
-  const float fx1 = fx0 + 1.0f, fy1 = fy0 + 1.0f;
   const int lp0 = qy * TILE_W + qx;
   const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
   const int B = a.bucket;
@@ -289,30 +329,19 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
     }
     __syncthreads();
     if (n > 0) {
-      float T0 = 1.f, T1 = 1.f, T2 = 1.f, T3 = 1.f;
-      float c00 = 0.f, c01 = 0.f, c02 = 0.f, c10 = 0.f, c11 = 0.f, c12 = 0.f;
-      float c20 = 0.f, c21 = 0.f, c22 = 0.f, c30 = 0.f, c31 = 0.f, c32 = 0.f;
-      for (int k = 0; k < n; ++k) {
-        const float4 A = sA[w][k], Bv = sB[w][k];
-        const float cb = sC[w][k];
-        blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx0, fy0), Bv.z, Bv.w, cb, T0, c00, c01, c02);
-        blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx1, fy0), Bv.z, Bv.w, cb, T1, c10, c11, c12);
-        blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx0, fy1), Bv.z, Bv.w, cb, T2, c20, c21, c22);
-        blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx1, fy1), Bv.z, Bv.w, cb, T3, c30, c31, c32);
-      }
-      sP[w][lp0] = make_float4(c00, c01, c02, T0);
-      sP[w][lp0 + 1] = make_float4(c10, c11, c12, T1);
-      sP[w][lp0 + TILE_W] = make_float4(c20, c21, c22, T2);
-      sP[w][lp0 + TILE_W + 1] = make_float4(c30, c31, c32, T3);
+      if (band)
+        bucket_pass<2>(sA[w], sB[w], sC[w], n, fx0, fy0, sP[w], lp0);
+      else
+        bucket_pass<4>(sA[w], sB[w], sC[w], n, fx0, fy0, sP[w], lp0);
     }
     __syncthreads();
     if (!done) {
       for (int q = 0; q < 4; ++q) {
         const int bq = round * 4 + q;
         if (bq >= nb) break;
-        const float4 v = sP[q][t];
+        const float4 v = sP[q][lp];
         merge(v.w, v.x, v.y, v.z, T, C0, C1, C2);
-        ((float4*)a.acc)[(size_t)(b0 + bq) * TILE_PIX + t] = make_float4(C0, C1, C2, T);
+        ((float4*)a.acc)[(size_t)(b0 + bq) * TILE_PIX + lp] = make_float4(C0, C1, C2, T);
         cnt += (int)min((uint32_t)B, end - (start + (uint32_t)bq * B));
         if (T < T_CUT_F) {
           done = true;
@@ -334,15 +363,24 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   const int m = wave_max_i(cnt);
   if (lane == 0) wmax[w] = m;
   __syncthreads();
-  if (t == 0) a.maxc[tile] = (uint32_t)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+  if (t == 0) {
+    const uint32_t mx = (uint32_t)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    // the two bands of a split tile combine into the zero-initialised entry
+    if (band)
+      atomicMax(&a.maxc[tile], mx);
+    else
+      a.maxc[tile] = mx;
+    if (a.cycles) a.cycles[blockIdx.x] = (uint32_t)(wall_clock64() - t_start);
+  }
 }
 
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
   if (a.T <= 0) return hipSuccess;
+  const unsigned grid = a.items ? 2u * (unsigned)a.T : (unsigned)a.T;
   if (a.bucket <= 64)
-    FGS_TIMED("composite_fast", s, (k_composite_fast<64><<<a.T, 256, 0, s>>>(a)));
+    FGS_TIMED("composite_fast", s, (k_composite_fast<64><<<grid, 256, 0, s>>>(a)));
   else
-    FGS_TIMED("composite_fast", s, (k_composite_fast<256><<<a.T, 256, 0, s>>>(a)));
+    FGS_TIMED("composite_fast", s, (k_composite_fast<256><<<grid, 256, 0, s>>>(a)));
   return hipGetLastError();
 }
 

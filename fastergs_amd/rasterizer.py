@@ -34,13 +34,24 @@ from ._lib import Camera, FrameView, Gaussians, Grads, Options, check, lib
 
 @dataclass
 class RasterOptions:
-    """Schedule knobs (not part of the reference settings tuple)."""
+    """Schedule knobs (not part of the reference settings tuple).
+
+    ``tile_sched``: work schedule of the FAST compositing pass, an object
+    with a ``to_c()`` returning a ``_lib.TileSched`` (see
+    ``fastergs_amd.scheduler``); None = one workgroup per tile in tile order.
+    """
     schedule: int = _lib.SCHEDULE_PLAIN
     bucket_size: int = 32
     split_tiles: int = 64
+    tile_sched: object = None
 
     def to_c(self, abs_grad: bool = True) -> Options:
-        return Options(self.schedule, self.bucket_size, self.split_tiles, 1 if abs_grad else 0)
+        o = Options(self.schedule, self.bucket_size, self.split_tiles, 1 if abs_grad else 0, None)
+        if self.tile_sched is not None:
+            ts = self.tile_sched.to_c()
+            o.tile_sched = ctypes.pointer(ts)
+            o._keep = ts  # the struct must outlive the calls that read it
+        return o
 
 
 _default_options = RasterOptions()
@@ -215,6 +226,8 @@ class Frame:
             "bucket_ranges": (torch.int32, 2 * self.NB), "bucket_tile": (torch.int32, self.NB),
             "acc_t": (torch.float32, self.NB * 256), "acc_c": (torch.float32, self.NB * 256 * 4),
             "pixel_colors": (torch.float32, self.H * self.W * 4),
+            "tile_features": (torch.float32, T * 8), "tile_pred": (torch.float32, T * 2),
+            "work_items": (torch.int32, 2 * T), "item_ticks": (torch.int32, 2 * T),
         }
         res = {}
         for name, (dt, n) in spec.items():

@@ -73,11 +73,32 @@ typedef struct fgs_gaussians {
   const uint8_t* culling;     /* P bool or NULL */
 } fgs_gaussians;
 
+/* Work schedule of the FAST compositing pass (outputs never depend on it).
+ * RASTER: one workgroup per tile in tile order. HEURISTIC: tile order, tiles
+ * whose list has >= split_len entries are composited as two 16x8 row bands
+ * by two workgroups. LEARNED: a cost model (MLP 8 -> 64 -> 64 -> 2, bf16
+ * inference) predicts each tile's log2 duration and a split flag from 8
+ * features of its list; items are dispatched longest-predicted first. */
+enum { FGS_TILES_RASTER = 0, FGS_TILES_HEURISTIC = 1, FGS_TILES_LEARNED = 2 };
+#define FGS_MLP_IN 8
+#define FGS_MLP_HIDDEN 64
+#define FGS_MLP_OUT 2
+/* W1[64][8], b1[64], W2[64][64], b2[64], W3[2][64], b3[2] (fp32) */
+#define FGS_MLP_NPARAMS 4866
+typedef struct fgs_tile_sched {
+  int mode;                /* FGS_TILES_* */
+  int split_len;           /* HEURISTIC split threshold in list entries (<= 0: never split) */
+  const float* mlp_params; /* LEARNED: FGS_MLP_NPARAMS floats (device) */
+  int features;            /* compute the tile features even when not LEARNED */
+  int log_cycles;          /* log each work item's duration (100 MHz ticks) */
+} fgs_tile_sched;
+
 typedef struct fgs_options {
   int schedule;     /* FGS_SCHEDULE_* */
   int bucket_size;  /* 16, 32, 64, 128 or 256 */
   int split_tiles;  /* fast preprocess: rects with >= this many tiles are split */
   int abs_grad;     /* backward accumulates |dL/dmean2D| terms (needed for xyz_gradient_accum_abs) */
+  const fgs_tile_sched* tile_sched; /* FAST compositing schedule; NULL = RASTER */
 } fgs_options;
 
 /* Device pointers into the two workspaces, for inspection by tests and
@@ -99,6 +120,10 @@ typedef struct fgs_frame_view {
   float* acc_t;           /* NB x 256 transmittance after each bucket (FAST) */
   float* acc_c;           /* NB x 256 x 4 colour after each bucket (FAST) */
   float* pixel_colors;    /* H x W x 4 */
+  float* tile_features;   /* T x 8 (FAST, when computed) */
+  float* tile_pred;       /* T x 2 cost-model outputs (LEARNED) */
+  uint32_t* work_items;   /* counts[4] items: tile << 2 | part (0 whole, 1 / 2 upper / lower band) */
+  uint32_t* item_ticks;   /* per-item duration, 100 MHz ticks (log_cycles) */
 } fgs_frame_view;
 
 /* Workspace sizes in bytes. The gauss workspace depends on P and the image;
@@ -166,6 +191,23 @@ int fgs_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t*
 size_t fgs_scan_ws_size(int64_t n);
 int fgs_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, void* ws,
                  size_t ws_bytes, void* stream);
+
+/* Cost-model kernels. fgs_mlp_forward: y (n x 2) from x (n x 8), bf16 != 0
+ * rounds weights and activations to bfloat16 (fp32 accumulation).
+ * fgs_mlp_loss_grad: loss = mean over rows of (y0 - t0)^2 + split_weight *
+ * BCE(sigmoid(y1), t1) with targets t (n x 2); writes the full parameter
+ * gradient (FGS_MLP_NPARAMS) and the loss (1 float, may be NULL), with
+ * fixed-order (deterministic) reductions. fgs_adam_update: one Adam step
+ * (torch.optim.Adam rule, bias-corrected, step >= 1) over n parameters. */
+int fgs_mlp_forward(const float* params, const float* x, int64_t n, float* y, int bf16,
+                    void* stream);
+size_t fgs_mlp_grad_ws_size(int64_t n);
+int fgs_mlp_loss_grad(const float* params, const float* x, const float* target, int64_t n,
+                      float split_weight, float* grad, float* loss, void* ws, size_t ws_bytes,
+                      void* stream);
+int fgs_adam_update(float* params, const float* grad, float* exp_avg, float* exp_avg_sq,
+                    int64_t n, float lr, float beta1, float beta2, float eps, int step,
+                    void* stream);
 
 /* Per-kernel timing for measurement runs: while enabled, kernel launches are
  * bracketed by HIP events on their stream; fgs_profile_read synchronises and

@@ -1,0 +1,188 @@
+"""Tile schedules and the cost model on the GPU.
+
+* HIP cost-model kernels against the numpy oracle: fp32 forward, bf16
+  forward, loss gradient (1e-4 relative), Adam step.
+* Tile features against a numpy computation from the frame's own lists.
+* Work lists are a partition of the tiles (whole or two bands); the
+  heuristic list is in tile order.
+* Outputs do not depend on the schedule: images, T_final, last
+  contributors and per-tile max contributors bit-identical across raster,
+  heuristic (every tile split / some split) and learned schedules; the
+  backward's raw sums agree to float reassociation.
+* Online training on logged tile durations lowers the cost-model loss.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fastergs_amd import _lib, scenes
+from fastergs_amd._lib import SCHEDULE_FAST
+from fastergs_amd.rasterizer import Frame, RasterOptions
+from fastergs_amd.scheduler import CostModel, OnlineTrainer, TileSchedule, tile_targets
+from oracle import mlp
+from tests.helpers import grad_close, settings_for, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _xt(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0.0, 1.0, (n, 8)).astype(np.float32)
+    t = np.stack([rng.normal(8.0, 2.0, n), (rng.random(n) < 0.3)], 1).astype(np.float32)
+    return x, t
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 5000])
+def test_mlp_forward_and_grad(n):
+    print(f"\n[mlp] n={n}", flush=True)
+    model = CostModel(seed=n)
+    p = model.params.cpu().numpy().astype(np.float64)
+    x, t = _xt(n, n)
+    y32 = model.forward(torch.from_numpy(x), bf16=False).cpu().numpy()
+    ref32 = mlp.forward(p, x)
+    scale = np.sqrt(np.mean(ref32 ** 2))
+    np.testing.assert_allclose(y32, ref32, rtol=1e-5, atol=1e-5 * scale)
+    y16 = model.forward(torch.from_numpy(x), bf16=True).cpu().numpy()
+    ref16 = mlp.forward(p, x, bf16=True)
+    err16 = np.abs(y16 - ref16).max() / scale
+    assert err16 < 4e-3, err16  # rounding points identical; fp32 vs f64 accumulation only
+    loss, g = model.loss_grad(torch.from_numpy(x), torch.from_numpy(t), split_weight=0.5)
+    l_ref, g_ref = mlp.loss_grad(p, x, t, 0.5)
+    assert abs(loss.item() - l_ref) <= 1e-5 * abs(l_ref)
+    grad_close(g.cpu().numpy(), g_ref, rtol=1e-4, name="mlp grad")
+
+
+def test_adam_kernel():
+    model = CostModel(seed=3, lr=5e-3)
+    x, t = _xt(700, 4)
+    p = model.params.cpu().numpy().astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for step in range(1, 4):
+        _, g = model.loss_grad(torch.from_numpy(x), torch.from_numpy(t))
+        g = g.cpu().numpy().astype(np.float64)
+        model.step(torch.from_numpy(x), torch.from_numpy(t))
+        p, m, v = mlp.adam(p, g, m, v, step, lr=5e-3)
+        np.testing.assert_allclose(model.params.cpu().numpy(), p, rtol=1e-5, atol=1e-7)
+
+
+def _frame(d, cam, sched, bucket=32):
+    fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+               dc=d["dc"], shs=d["shs"],
+               options=RasterOptions(schedule=SCHEDULE_FAST, bucket_size=bucket,
+                                     tile_sched=sched)).forward()
+    torch.cuda.synchronize()
+    return fr
+
+
+@pytest.fixture(scope="module")
+def scene_cam():
+    scene = scenes.cluster_scene(40000, seed=11)
+    cam = scenes.look_at_camera((0.2, 0.1, -4.0), 480, 320)
+    return to_dev(scene), cam
+
+
+def test_tile_features(scene_cam):
+    d, cam = scene_cam
+    fr = _frame(d, cam, TileSchedule.raster(features=True))
+    v = fr.view_arrays()
+    T = v["ranges"].numel() // 2
+    feat = v["tile_features"].cpu().numpy().reshape(T, 8)
+    rng = v["ranges"].cpu().numpy().view(np.uint32).reshape(T, 2)
+    vals = v["sorted_vals"].cpu().numpy().view(np.uint32)
+    geom = v["geom"].cpu().numpy().reshape(-1, 8)
+    splat = v["splat"].cpu().numpy().reshape(-1, 12)
+    rmin = geom[:, 6].view(np.uint32).astype(np.int64)
+    rmax = geom[:, 7].view(np.uint32).astype(np.int64)
+    area = ((rmax & 0xFFFF) - (rmin & 0xFFFF)) * ((rmax >> 16) - (rmin >> 16))
+    checked = 0
+    for tile in range(T):
+        s, e = int(rng[tile, 0]), int(rng[tile, 1])
+        f = feat[tile]
+        assert abs(f[0] - np.log2(1.0 + (e - s))) < 1e-5
+        if e == s:
+            assert np.all(f[1:] == 0)
+            continue
+        ids = vals[s:min(e, s + 256)]
+        la = np.log2(np.maximum(area[ids], 1).astype(np.float64))
+        assert abs(f[1] - la.mean()) < 1e-4 and abs(f[2] - la.max()) < 1e-5
+        op = np.exp2(splat[ids, 5].astype(np.float64))
+        bins = np.minimum((op * 16).astype(int), 15)
+        srt = np.sort(bins)
+        for k, q in enumerate((0.25, 0.5, 0.75, 1.0)):
+            b = srt[int(np.ceil(q * len(ids))) - 1]
+            # a sample exactly on a bin edge may land in either neighbour
+            assert abs(f[3 + k] - (b + 0.5) / 16) <= 1.0 / 16 + 1e-6
+        dep = splat[ids, 9].astype(np.float64)
+        assert abs(f[7] - np.log2(dep.max() / dep.min())) < 1e-4
+        checked += 1
+    assert checked > 50
+
+
+def _items(fr):
+    v = fr.view_arrays()
+    n = int(v["counts"][4].item())
+    return v["work_items"][:n].cpu().numpy().view(np.uint32), v
+
+
+def test_work_lists_partition_tiles(scene_cam):
+    d, cam = scene_cam
+    model = CostModel(seed=1)
+    for sched in (TileSchedule.heuristic(split_len=1), TileSchedule.heuristic(split_len=200),
+                  TileSchedule.learned(model)):
+        fr = _frame(d, cam, sched)
+        items, v = _items(fr)
+        T = v["ranges"].numel() // 2
+        tiles, parts = items >> 2, items & 3
+        assert parts.max() <= 2
+        whole = np.bincount(tiles[parts == 0], minlength=T)
+        up = np.bincount(tiles[parts == 1], minlength=T)
+        low = np.bincount(tiles[parts == 2], minlength=T)
+        assert np.all(whole + up == 1) and np.all(up == low)
+        if sched.mode == _lib.TILES_HEURISTIC:
+            order = tiles[parts != 2]
+            assert np.all(np.diff(order.astype(np.int64)) > 0)
+            rng = v["ranges"].cpu().numpy().view(np.uint32).reshape(T, 2)
+            ln = rng[:, 1] - rng[:, 0]
+            np.testing.assert_array_equal(up == 1, (ln >= sched.split_len))
+
+
+def test_outputs_independent_of_schedule(scene_cam):
+    d, cam = scene_cam
+    ref = _frame(d, cam, None)
+    g = torch.from_numpy(scenes.target_image(cam.width, cam.height)).cuda()
+    ref_b = ref.backward(g)
+    rv = ref.view_arrays()
+    model = CostModel(seed=2)
+    for name, sched in [("raster_logged", TileSchedule.raster(log_cycles=True, features=True)),
+                        ("split_all", TileSchedule.heuristic(split_len=1)),
+                        ("split_some", TileSchedule.heuristic(split_len=300)),
+                        ("learned", TileSchedule.learned(model, log_cycles=True))]:
+        print(f"[sched] {name}", flush=True)
+        fr = _frame(d, cam, sched)
+        np.testing.assert_array_equal(fr.out_color.cpu().numpy(), ref.out_color.cpu().numpy())
+        np.testing.assert_array_equal(fr.T_final.cpu().numpy(), ref.T_final.cpu().numpy())
+        np.testing.assert_array_equal(fr.last_contributor.cpu().numpy(),
+                                      ref.last_contributor.cpu().numpy())
+        v = fr.view_arrays()
+        np.testing.assert_array_equal(v["max_contrib"].cpu().numpy(), rv["max_contrib"].cpu().numpy())
+        b = fr.backward(g)
+        grad_close(b["raw"].cpu().numpy(), ref_b["raw"].cpu().numpy(), rtol=1e-5, name=name)
+
+
+def test_online_training_lowers_loss(scene_cam):
+    d, cam = scene_cam
+    model = CostModel(seed=4, lr=3e-3)
+    trainer = OnlineTrainer(model, batch=1024)
+    cams = scenes.orbit_cameras(6, cam.width, cam.height, radius=4.0)
+    for c in cams:
+        fr = _frame(d, c, TileSchedule.heuristic(split_len=2048, features=True, log_cycles=True))
+        feat, tgt, cost = tile_targets(fr)
+        assert torch.isfinite(feat).all() and (cost >= 0).all() and cost.sum() > 0
+        trainer.observe(fr)
+    x, t = trainer.x[:trainer.size], trainer.t[:trainer.size]
+    l0 = model.loss_grad(x, t)[0].item()
+    trainer.train(300)
+    l1 = model.loss_grad(x, t)[0].item()
+    print(f"[train] loss {l0:.4f} -> {l1:.4f}", flush=True)
+    assert l1 < 0.5 * l0
