@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--compare-plain", type=int, default=1)
     ap.add_argument("--render-loop", type=int, default=1,
                     help="also run the graph-captured 120-frame orbit of the dense workload")
+    ap.add_argument("--sched-eval", type=int, default=1,
+                    help="also train the tile cost model online and score it against the heuristic")
     return ap.parse_args()
 
 
@@ -112,6 +114,7 @@ def main():
 
     graph_fps = graph_render_fps(settings, params, sched, args)
     loop = render_loop(args) if args.render_loop else None
+    sched_eval = learned_schedule_eval(args) if args.sched_eval else None
 
     cpu = None
     if args.cpu_baseline:
@@ -142,6 +145,7 @@ def main():
         "roofline": roofline,
         "stages": stages,
         "render_loop": loop,
+        "learned_scheduler": sched_eval,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -321,6 +325,85 @@ def render_loop(args, frames=120):
             "pairs_sampled": Rs, "pair_capacity": R_cap, "bucket_capacity": NB_cap,
             "sort_gbs": round(sort_bytes / (sort_ms / 1e3) / 1e9, 1),
             "sort_frac_hbm": round(sort_bytes / (sort_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def learned_schedule_eval(args, train_views=60, eval_views=40, n=None):
+    """Online training of the tile cost model on a seeded camera trajectory
+    of the large workload (6M Gaussians, 1237x822), then, on views it was not
+    trained on, compositing time (composite + its scheduling kernels, HIP
+    events, best of 3) of the raster, heuristic and learned schedules. The
+    heuristic (tile order, fixed split threshold) is the judge: regret =
+    learned - heuristic per view (negative = the model gains)."""
+    import torch
+    from fastergs_amd import _lib, scenes
+    from fastergs_amd._lib import SCHEDULE_FAST
+    from fastergs_amd.rasterizer import Frame, GaussianRasterizationSettings, RasterOptions
+    from fastergs_amd.scheduler import CostModel, OnlineTrainer, TileSchedule
+    dev = torch.device("cuda:0")
+    scene, cam0 = scenes.workload("large", seed=0, n=n)
+    P = scene["means3D"].shape[0]
+    p = {k: torch.from_numpy(v).to(dev) for k, v in scene.items()}
+    del scene
+    cams = scenes.sphere_cameras(train_views + eval_views, seed=3, rmin=3.0, rmax=5.0,
+                                 sizes=((cam0.width, cam0.height),))
+
+    def frame(c, sched):
+        st = GaussianRasterizationSettings(
+            image_height=c.height, image_width=c.width, tanfovx=c.tanfovx, tanfovy=c.tanfovy,
+            bg=torch.zeros(3, device=dev), scale_modifier=1.0,
+            viewmatrix=torch.from_numpy(c.viewmatrix).to(dev),
+            projmatrix=torch.from_numpy(c.projmatrix).to(dev), sh_degree=3,
+            campos=torch.from_numpy(c.campos).to(dev), mult=1.0, prefiltered=False, debug=False)
+        return Frame(st, p["means3D"], p["opacities"], p["scales"], p["rotations"], dc=p["dc"],
+                     shs=p["shs"], options=RasterOptions(schedule=SCHEDULE_FAST,
+                                                         bucket_size=args.bucket,
+                                                         tile_sched=sched)).forward()
+
+    model = CostModel(seed=0, lr=2e-3)
+    trainer = OnlineTrainer(model, batch=2048)
+    split_len = 4096
+    log = TileSchedule.heuristic(split_len=split_len, features=True, log_cycles=True)
+    t0 = time.perf_counter()
+    losses = []
+    for c in cams[:train_views]:
+        fr = frame(c, log)
+        torch.cuda.synchronize()
+        trainer.observe(fr)
+        del fr
+        losses.append(trainer.train(20))
+    train_s = time.perf_counter() - t0
+    names = ("composite_fast", "sched_items", "tile_features", "mlp_infer")
+    scheds = {"raster": TileSchedule.raster(), "heuristic": TileSchedule.heuristic(split_len),
+              "learned": TileSchedule.learned(model),
+              "learned_ordered": TileSchedule.learned(model, order=True)}
+    res = {k: [] for k in scheds}
+    for c in cams[train_views:]:
+        for k, sc in scheds.items():
+            best = None
+            for _ in range(3):
+                _lib.profile_enable(True)
+                fr = frame(c, sc)
+                prof = _lib.profile_read()
+                _lib.profile_enable(False)
+                del fr
+                us = 1e3 * sum(prof[nm][0] for nm in names if nm in prof)
+                best = us if best is None else min(best, us)
+            res[k].append(best)
+    torch.cuda.empty_cache()
+    h = np.array(res["heuristic"])
+    lo = np.array(res["learned_ordered"])
+    l = np.array(res["learned"])
+    if lo.sum() < l.sum():  # report the better learned variant; both are listed
+        l = lo
+    r = np.array(res["raster"])
+    return {"workload": f"large: {P} Gaussians, {cam0.width}x{cam0.height}, {train_views} training "
+                        f"+ {eval_views} held-out views on a seeded sphere r=3-5",
+            "train_seconds": round(train_s, 2), "loss_first": round(losses[0], 4),
+            "loss_last": round(losses[-1], 4),
+            "composite_us_mean": {k: round(float(np.mean(v)), 2) for k, v in res.items()},
+            "regret_us_mean": round(float((l - h).mean()), 2),
+            "learned_wins_frac": round(float((l < h).mean()), 3),
+            "gain_vs_heuristic": round(float(h.sum() / l.sum()), 4)}
 
 
 def cpu_baseline(scene, cam):

@@ -354,6 +354,9 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   c.acc = pw.acc;
   c.items = nullptr;
   c.n_items = nullptr;
+  c.slots = 0;
+  c.split_len = 0;
+  c.split_pred = nullptr;
   c.cycles = nullptr;
   if (fast) {
     // emission in depth order, tile-only stable sort, lists, ranges, buckets
@@ -375,7 +378,8 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
     } else {
       FGS_TRY(launch_unpack_ranges(R, gw.counts, pw.k0, pw.v0, gw.ranges, T, s), "lists and ranges");
     }
-    FGS_TRY(launch_bucket_counts(T, gw.ranges, opt->bucket_size, gw.bcount, s), "bucket counts");
+    FGS_TRY(launch_bucket_counts(T, gw.ranges, opt->bucket_size, gw.bcount, gw.maxc, s),
+            "bucket counts");
     FGS_TRY(scan_u32(gw.bcount, gw.boff, T, gw.boff + T, gw.scratch, s), "bucket scan");
     FGS_TRY(launch_bucket_tables(T, NB, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
                                  pw.bucket_tile, s),
@@ -387,12 +391,16 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
         FGS_TRY(launch_tile_features(T, gw.ranges, pw.v0, gw.geom, gw.splat, gw.feat, s),
                 "tile features");
       if (learned) FGS_TRY(launch_mlp_infer(ts->mlp_params, gw.feat, T, gw.pred, true, s), "cost model");
-      if (ts->mode != FGS_TILES_RASTER) {
-        FGS_TRY(launch_sched_items(T, gw.ranges, opt->bucket_size, learned ? gw.pred : nullptr,
-                                   ts->split_len, gw.items, gw.counts + 4, gw.maxc, s),
+      if (learned && ts->order) {
+        FGS_TRY(launch_sched_items(T, gw.ranges, opt->bucket_size, gw.pred, gw.items,
+                                   gw.counts + 4, s),
                 "work items");
         c.items = gw.items;
         c.n_items = gw.counts + 4;
+      } else if (ts->mode != FGS_TILES_RASTER) {
+        c.slots = 1;
+        c.split_len = ts->split_len;
+        c.split_pred = learned ? gw.pred : nullptr;
       }
       if (ts->log_cycles) c.cycles = gw.cycles;
     }

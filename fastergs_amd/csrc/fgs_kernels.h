@@ -69,7 +69,7 @@ hipError_t launch_unpack_ranges(int64_t R, const uint32_t* n_dev, const uint64_t
 hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const uint32_t* tiles,
                               uint32_t* ranges, int T, hipStream_t s);
 hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
-                                hipStream_t s);
+                                uint32_t* maxc, hipStream_t s);
 hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
                                 hipStream_t s);
 hipError_t launch_nb_capacity(uint32_t* counts, int bucket, int T, hipStream_t s);
@@ -87,9 +87,15 @@ struct CompArgs {
   uint32_t* maxc;
   float* acc;            // fast checkpoints
   // fast: optional work list (tile << 2 | part: 0 whole tile, 1 / 2 upper /
-  // lower 16x8 band) of n_items[0] <= 2T items, and per-item duration log
+  // lower 16x8 band) of n_items[0] <= 2T items; or, with slots set, two grid
+  // slots per tile in tile order, tiles split when split_pred[2t + 1] > 0
+  // (if given) or their list has >= split_len (> 0) entries; optional
+  // per-workgroup duration log
   const uint32_t* items;
   const uint32_t* n_items;
+  int slots;
+  int split_len;
+  const float* split_pred;
   uint32_t* cycles;
 };
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
@@ -108,8 +114,7 @@ hipError_t launch_mlp_loss_grad(const float* params, const float* x, const float
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
                        float b1, float b2, float eps, int step, hipStream_t s);
 hipError_t launch_sched_items(int T, const uint32_t* ranges, int bucket, const float* pred,
-                              int split_len, uint32_t* items, uint32_t* n_items, uint32_t* maxc,
-                              hipStream_t s);
+                              uint32_t* items, uint32_t* n_items, hipStream_t s);
 
 // backward.hip
 struct BwdArgs {

@@ -55,16 +55,19 @@ hipError_t launch_identify_ranges(int64_t R, const uint32_t* n_dev, const uint64
   return hipGetLastError();
 }
 
+// Also clears the per-tile max contributor that compositing combines with
+// atomicMax when a tile is split into row bands.
 __global__ void k_bucket_counts(int T, const uint32_t* __restrict__ ranges, int bucket,
-                                uint32_t* __restrict__ bcount) {
+                                uint32_t* __restrict__ bcount, uint32_t* __restrict__ maxc) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= T) return;
   const uint32_t len = ranges[2 * t + 1] - ranges[2 * t];
   bcount[t] = (len + bucket - 1) / bucket;
+  if (maxc) maxc[t] = 0u;
 }
 hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
-                                hipStream_t s) {
-  if (T > 0) k_bucket_counts<<<(T + 255) / 256, 256, 0, s>>>(T, ranges, bucket, bcount);
+                                uint32_t* maxc, hipStream_t s) {
+  if (T > 0) k_bucket_counts<<<(T + 255) / 256, 256, 0, s>>>(T, ranges, bucket, bcount, maxc);
   return hipGetLastError();
 }
 
@@ -287,10 +290,32 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   __shared__ float4 sP[4][TILE_PIX];
   __shared__ int wmax[4];
   const uint64_t t_start = a.cycles ? wall_clock64() : 0;
-  uint32_t item = (uint32_t)blockIdx.x << 2;
+  uint32_t item;
+  int log_idx;  // where the duration is logged: by work item, by (slot, tile) or by tile
   if (a.items) {
     if (blockIdx.x >= a.n_items[0]) return;  // whole workgroup
     item = a.items[blockIdx.x];
+    log_idx = (int)blockIdx.x;
+  } else if (a.slots) {
+    // two grid slots per tile: the upper and lower band of a split tile, or
+    // the whole tile in slot 0. Workgroup b runs on XCD b % 8, so the slots
+    // are laid out [all first slots | all second slots]: interleaving them
+    // would leave the mostly idle second slots to half of the XCDs.
+    const int slot = (int)blockIdx.x >= a.T ? 1 : 0;
+    const int tl = (int)blockIdx.x - slot * a.T;
+
+    const uint32_t len = a.ranges[2 * tl + 1] - a.ranges[2 * tl];
+    const bool split = a.split_pred ? (a.split_pred[2 * tl + 1] > 0.0f && len > (uint32_t)a.bucket)
+                                    : (a.split_len > 0 && len >= (uint32_t)a.split_len);
+    log_idx = slot * a.T + tl;
+    if (!split && slot) {
+      if (a.cycles && threadIdx.x == 0) a.cycles[log_idx] = 0u;
+      return;  // whole workgroup
+    }
+    item = ((uint32_t)tl << 2) | (split ? 1u + (uint32_t)slot : 0u);
+  } else {
+    item = (uint32_t)blockIdx.x << 2;
+    log_idx = (int)blockIdx.x;
   }
   const int tile = (int)(item >> 2), part = (int)(item & 3u);
   const bool band = part != 0;
@@ -365,18 +390,18 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   __syncthreads();
   if (t == 0) {
     const uint32_t mx = (uint32_t)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-    // the two bands of a split tile combine into the zero-initialised entry
+    // the two bands of a split tile combine into the entry k_bucket_counts cleared
     if (band)
       atomicMax(&a.maxc[tile], mx);
     else
       a.maxc[tile] = mx;
-    if (a.cycles) a.cycles[blockIdx.x] = (uint32_t)(wall_clock64() - t_start);
+    if (a.cycles) a.cycles[log_idx] = (uint32_t)(wall_clock64() - t_start);
   }
 }
 
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
   if (a.T <= 0) return hipSuccess;
-  const unsigned grid = a.items ? 2u * (unsigned)a.T : (unsigned)a.T;
+  const unsigned grid = (a.items || a.slots) ? 2u * (unsigned)a.T : (unsigned)a.T;
   if (a.bucket <= 64)
     FGS_TIMED("composite_fast", s, (k_composite_fast<64><<<grid, 256, 0, s>>>(a)));
   else

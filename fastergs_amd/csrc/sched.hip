@@ -7,12 +7,14 @@
 // independent and each keeps its own bucket order), only how evenly the
 // work lands on the 256 CUs. Three schedules:
 //   raster     one item per tile, tile order (the reference's render.cu:418)
-//   heuristic  raster order, tiles whose list has >= split_len entries split
+//   heuristic  tile order, tiles whose list has >= split_len entries split
 //              (a fixed threshold, like the reference's fixed 128-tile split
-//              of large Gaussians, preprocess.cu:502-504)
+//              of large Gaussians, preprocess.cu:502-504); decided inside
+//              the compositing kernel, two grid slots per tile
 //   learned    a small MLP predicts each tile's cost (log2 of its measured
-//              duration) and a split flag from 8 tile features; items are
-//              dispatched longest-predicted first
+//              duration) and a split flag from 8 tile features; split flags
+//              applied in tile order like the heuristic, or (order) items
+//              dispatched longest-predicted first from a work list
 // The MLP (8 -> 64 -> 64 -> 2, ReLU) is trained on the durations the
 // compositing pass logs for every item (wall clock, 100 MHz ticks), with the
 // kernels below: per-row forward/backward, fixed-order weight-gradient
@@ -35,7 +37,7 @@ namespace fgs {
 //   1 mean and 2 max of log2(rect area in tiles) of the sampled splats
 //   3..6 opacity quantiles 0.25 / 0.5 / 0.75 / 1.0 (16-bin histogram)
 //   7 log2(max depth / min depth) of the sample
-constexpr int FEAT_SAMPLE = 256;
+constexpr int FEAT_SAMPLE = 128;
 
 __global__ __launch_bounds__(FEAT_SAMPLE) void k_tile_features(
     int T, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ vals,
@@ -120,48 +122,52 @@ __device__ __forceinline__ float bf16_round(float x) {
   return u2f(u & 0xffff0000u);
 }
 
-// Inference, one thread per row; BF16: weights and activations rounded to
-// bfloat16, fp32 accumulation; else plain fp32.
+// Inference. One wave evaluates a stream of rows: lane j owns hidden unit j
+// of both layers (its rows of W1 and W2 and its two output weights stay in
+// registers), the first layer's outputs reach every lane through readlane,
+// and the two outputs are wave sums. BF16: weights, inputs and hidden
+// activations rounded to bfloat16, fp32 accumulation; else plain fp32.
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ params,
                                                    const float* __restrict__ x, int64_t n,
                                                    float* __restrict__ y) {
-  __shared__ float w[MLP_NPARAMS];
-  for (int k = threadIdx.x; k < MLP_NPARAMS; k += 256)
-    w[k] = BF16 ? bf16_round(params[k]) : params[k];
-  __syncthreads();
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= n) return;
-  float xi[MLP_IN];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE;
+  const int64_t nwaves = (int64_t)gridDim.x * (256 / WAVE);
+  auto rnd = [](float v) { return BF16 ? bf16_round(v) : v; };
+  float w1[MLP_IN], w2[MLP_HID];
 #pragma unroll
-  for (int k = 0; k < MLP_IN; ++k) xi[k] = BF16 ? bf16_round(x[r * MLP_IN + k]) : x[r * MLP_IN + k];
-  float h1[MLP_HID];
+  for (int k = 0; k < MLP_IN; ++k) w1[k] = rnd(params[MLP_W1 + lane * MLP_IN + k]);
 #pragma unroll
-  for (int j = 0; j < MLP_HID; ++j) {
-    float z = w[MLP_B1 + j];
+  for (int k = 0; k < MLP_HID; ++k) w2[k] = rnd(params[MLP_W2 + lane * MLP_HID + k]);
+  const float b1 = rnd(params[MLP_B1 + lane]), b2 = rnd(params[MLP_B2 + lane]);
+  const float w30 = rnd(params[MLP_W3 + lane]), w31 = rnd(params[MLP_W3 + MLP_HID + lane]);
+  const float b30 = rnd(params[MLP_B3]), b31 = rnd(params[MLP_B3 + 1]);
+  for (int64_t r = wave; r < n; r += nwaves) {
+    float z1 = b1;
 #pragma unroll
-    for (int k = 0; k < MLP_IN; ++k) z = fmaf(w[MLP_W1 + j * MLP_IN + k], xi[k], z);
-    z = fmaxf(z, 0.0f);
-    h1[j] = BF16 ? bf16_round(z) : z;
+    for (int k = 0; k < MLP_IN; ++k) z1 = fmaf(w1[k], rnd(x[r * MLP_IN + k]), z1);
+    const float h1 = rnd(fmaxf(z1, 0.0f));
+    float z2 = b2;
+#pragma unroll
+    for (int k = 0; k < MLP_HID; ++k)
+      z2 = fmaf(w2[k], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), k)), z2);
+    const float h2 = rnd(fmaxf(z2, 0.0f));
+    const float o0 = wave_sum(w30 * h2), o1 = wave_sum(w31 * h2);
+    if (lane == 0) {
+      y[2 * r] = o0 + b30;
+      y[2 * r + 1] = o1 + b31;
+    }
   }
-  float o0 = w[MLP_B3], o1 = w[MLP_B3 + 1];
-  for (int j = 0; j < MLP_HID; ++j) {
-    float z = w[MLP_B2 + j];
-#pragma unroll
-    for (int k = 0; k < MLP_HID; ++k) z = fmaf(w[MLP_W2 + j * MLP_HID + k], h1[k], z);
-    z = fmaxf(z, 0.0f);
-    if (BF16) z = bf16_round(z);
-    o0 = fmaf(w[MLP_W3 + j], z, o0);
-    o1 = fmaf(w[MLP_W3 + MLP_HID + j], z, o1);
-  }
-  y[2 * r] = o0;
-  y[2 * r + 1] = o1;
 }
 
 hipError_t launch_mlp_infer(const float* params, const float* x, int64_t n, float* y, bool bf16,
                             hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((n + 255) / 256);
+  // up to 4096 waves (one row each for a frame's tiles): the rows of a wave
+  // are sequential, so spreading them wins over reloading the weights
+  const int64_t want = (n + 3) / 4;
+  const unsigned blocks = (unsigned)(want < 1024 ? want : 1024);
   if (bf16)
     FGS_TIMED("mlp_infer", s, (k_mlp_infer<true><<<blocks, 256, 0, s>>>(params, x, n, y)));
   else
@@ -370,25 +376,20 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, 
 }
 
 // ------------------------------------------------------------ work list
-// One 1024-thread workgroup builds the compositing work list.
-//   heuristic (pred == nullptr): tiles in raster order, split when the list
-//   has >= split_len entries (split_len <= 0: never); positions by a block
-//   scan, so the list is deterministic.
-//   learned: tiles binned by predicted log2 cost (bins of 1/8, clamped to
-//   256 bins), highest bin first, split when the split logit is > 0 and the
-//   tile has at least two buckets; the order inside a bin is arbitrary.
-// Split tiles get maxc = 0 here (their bands combine with atomicMax).
+// Work list of the learned schedule, built by one 1024-thread workgroup:
+// tiles binned by predicted log2 cost (bins of 1/8, 256 bins), highest bin
+// first (longest predicted first); a tile is split into its two row bands
+// when its split logit is > 0 and it has more than one bucket. The order
+// inside a bin is arbitrary (LDS atomics); outputs never depend on it.
 constexpr int ITEM_THREADS = 1024;
 __global__ __launch_bounds__(ITEM_THREADS) void k_sched_items(
     int T, const uint32_t* __restrict__ ranges, int bucket, const float* __restrict__ pred,
-    int split_len, uint32_t* __restrict__ items, uint32_t* __restrict__ n_items,
-    uint32_t* __restrict__ maxc) {
+    uint32_t* __restrict__ items, uint32_t* __restrict__ n_items) {
   __shared__ uint32_t cnt[256];
   __shared__ uint32_t off[256];
   const int t = threadIdx.x;
   auto split_of = [&](int tile) -> bool {
     const uint32_t len = ranges[2 * tile + 1] - ranges[2 * tile];
-    if (pred == nullptr) return split_len > 0 && len >= (uint32_t)split_len;
     return pred[2 * tile + 1] > 0.0f && len > (uint32_t)bucket;
   };
   auto bin_of = [&](int tile) -> int {
@@ -396,29 +397,6 @@ __global__ __launch_bounds__(ITEM_THREADS) void k_sched_items(
     const int b = c != c ? 0 : (int)fminf(fmaxf(c, 0.0f), 255.0f);
     return 255 - b;  // descending cost
   };
-  if (pred == nullptr) {
-    uint32_t carry = 0;
-    for (int base = 0; base < T; base += ITEM_THREADS) {
-      const int tile = base + t;
-      const bool sp = tile < T && split_of(tile);
-      const uint32_t k = tile < T ? (sp ? 2u : 1u) : 0u;
-      uint32_t tot;
-      const uint32_t ex = block_excl_scan<ITEM_THREADS>(k, tot);
-      if (tile < T) {
-        const uint32_t pos = carry + ex;
-        if (sp) {
-          items[pos] = ((uint32_t)tile << 2) | 1u;
-          items[pos + 1] = ((uint32_t)tile << 2) | 2u;
-          maxc[tile] = 0u;
-        } else {
-          items[pos] = (uint32_t)tile << 2;
-        }
-      }
-      carry += tot;
-    }
-    if (t == 0) n_items[0] = carry;
-    return;
-  }
   if (t < 256) cnt[t] = 0;
   __syncthreads();
   for (int tile = t; tile < T; tile += ITEM_THREADS)
@@ -438,7 +416,6 @@ __global__ __launch_bounds__(ITEM_THREADS) void k_sched_items(
     if (sp) {
       items[pos] = ((uint32_t)tile << 2) | 1u;
       items[pos + 1] = ((uint32_t)tile << 2) | 2u;
-      maxc[tile] = 0u;
     } else {
       items[pos] = (uint32_t)tile << 2;
     }
@@ -446,12 +423,10 @@ __global__ __launch_bounds__(ITEM_THREADS) void k_sched_items(
 }
 
 hipError_t launch_sched_items(int T, const uint32_t* ranges, int bucket, const float* pred,
-                              int split_len, uint32_t* items, uint32_t* n_items, uint32_t* maxc,
-                              hipStream_t s) {
+                              uint32_t* items, uint32_t* n_items, hipStream_t s) {
   if (T <= 0) return hipSuccess;
   FGS_TIMED("sched_items", s,
-            (k_sched_items<<<1, ITEM_THREADS, 0, s>>>(T, ranges, bucket, pred, split_len, items,
-                                                       n_items, maxc)));
+            (k_sched_items<<<1, ITEM_THREADS, 0, s>>>(T, ranges, bucket, pred, items, n_items)));
   return hipGetLastError();
 }
 

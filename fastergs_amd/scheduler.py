@@ -10,9 +10,10 @@ the kernel time does.
   reference's launch, render.cu:418).
 * ``TileSchedule.heuristic(split_len)``: tile order, tiles with at least
   ``split_len`` list entries split -- a fixed threshold, the fallback.
-* ``TileSchedule.learned(model)``: a cost model predicts each tile's log2
-  duration and a split flag from 8 features of the tile's list; items are
-  dispatched longest-predicted first.
+* ``TileSchedule.learned(model, order=False)``: a cost model predicts each
+  tile's log2 duration and a split flag from 8 features of the tile's list;
+  the split flags are applied in tile order, or with ``order=True`` the
+  items are dispatched longest-predicted first.
 
 ``CostModel`` is the MLP (8 -> 64 -> 64 -> 2, fp32 weights, bf16
 inference). Its forward, loss gradient and Adam update are HIP kernels of
@@ -96,11 +97,15 @@ class TileSchedule:
     """What ``RasterOptions.tile_sched`` takes."""
 
     def __init__(self, mode: int, split_len: int = 0, model: CostModel | None = None,
-                 features: bool = False, log_cycles: bool = False):
+                 features: bool = False, log_cycles: bool = False, order: bool = False):
         if mode == _lib.TILES_LEARNED and model is None:
             raise ValueError("the learned schedule needs a CostModel")
         self.mode, self.split_len, self.model = mode, int(split_len), model
-        self.features, self.log_cycles = features, log_cycles
+        self.features, self.log_cycles, self.order = features, log_cycles, order
+
+    @property
+    def uses_list(self) -> bool:
+        return self.mode == _lib.TILES_LEARNED and self.order
 
     @classmethod
     def raster(cls, **kw):
@@ -117,7 +122,8 @@ class TileSchedule:
     def to_c(self) -> TileSched:
         return TileSched(self.mode, self.split_len,
                          _p(self.model.params) if self.model is not None else None,
-                         1 if self.features else 0, 1 if self.log_cycles else 0)
+                         1 if self.features else 0, 1 if self.log_cycles else 0,
+                         1 if self.order else 0)
 
 
 # Workgroups of the compositing pass resident at once on an MI355X: 256 CUs x
@@ -131,14 +137,20 @@ def tile_targets(frame, slots: int = RESIDENT_SLOTS):
     log_cycles on: target 0 = log2(1 + ticks of the tile's items), target 1 =
     1 when the tile's ticks exceed the total over ``slots``."""
     v = frame.view_arrays()
+    sched = frame.opts.tile_sched
     T = v["tile_features"].shape[0] // 8
     feat = v["tile_features"].reshape(T, 8)
     ticks = v["item_ticks"].to(torch.int64) & 0xFFFFFFFF
-    n_items = int(v["counts"][4].item()) if frame.opts.tile_sched.mode != _lib.TILES_RASTER else T
-    if frame.opts.tile_sched.mode == _lib.TILES_RASTER:
-        tile = torch.arange(T, device=ticks.device)
-    else:
+    dev = ticks.device
+    if sched.uses_list:  # work items
+        n_items = int(v["counts"][4].item())
         tile = (v["work_items"][:n_items].to(torch.int64) & 0xFFFFFFFF) >> 2
+    elif sched.mode == _lib.TILES_RASTER:  # one workgroup per tile
+        n_items = T
+        tile = torch.arange(T, device=dev)
+    else:  # two grid slots per tile, logged at slot * T + tile
+        n_items = 2 * T
+        tile = torch.arange(2 * T, device=dev) % T
     cost = torch.zeros(T, dtype=torch.float64, device=ticks.device)
     cost.index_add_(0, tile, ticks[:n_items].to(torch.float64))
     per_slot = cost.sum() / slots

@@ -76,9 +76,11 @@ typedef struct fgs_gaussians {
 /* Work schedule of the FAST compositing pass (outputs never depend on it).
  * RASTER: one workgroup per tile in tile order. HEURISTIC: tile order, tiles
  * whose list has >= split_len entries are composited as two 16x8 row bands
- * by two workgroups. LEARNED: a cost model (MLP 8 -> 64 -> 64 -> 2, bf16
- * inference) predicts each tile's log2 duration and a split flag from 8
- * features of its list; items are dispatched longest-predicted first. */
+ * by two workgroups (two grid slots per tile, decided in the kernel).
+ * LEARNED: a cost model (MLP 8 -> 64 -> 64 -> 2, bf16 inference) predicts
+ * each tile's log2 duration and a split flag from 8 features of its list;
+ * the split flags are applied in tile order like HEURISTIC, or with order
+ * set the items are dispatched longest-predicted first from a work list. */
 enum { FGS_TILES_RASTER = 0, FGS_TILES_HEURISTIC = 1, FGS_TILES_LEARNED = 2 };
 #define FGS_MLP_IN 8
 #define FGS_MLP_HIDDEN 64
@@ -90,7 +92,8 @@ typedef struct fgs_tile_sched {
   int split_len;           /* HEURISTIC split threshold in list entries (<= 0: never split) */
   const float* mlp_params; /* LEARNED: FGS_MLP_NPARAMS floats (device) */
   int features;            /* compute the tile features even when not LEARNED */
-  int log_cycles;          /* log each work item's duration (100 MHz ticks) */
+  int log_cycles;          /* log each workgroup's duration (100 MHz ticks) */
+  int order;               /* LEARNED: dispatch longest-predicted first */
 } fgs_tile_sched;
 
 typedef struct fgs_options {
@@ -122,8 +125,10 @@ typedef struct fgs_frame_view {
   float* pixel_colors;    /* H x W x 4 */
   float* tile_features;   /* T x 8 (FAST, when computed) */
   float* tile_pred;       /* T x 2 cost-model outputs (LEARNED) */
-  uint32_t* work_items;   /* counts[4] items: tile << 2 | part (0 whole, 1 / 2 upper / lower band) */
-  uint32_t* item_ticks;   /* per-item duration, 100 MHz ticks (log_cycles) */
+  uint32_t* work_items;   /* LEARNED with order: counts[4] items, tile << 2 | part
+                             (0 whole, 1 / 2 upper / lower band) */
+  uint32_t* item_ticks;   /* per-workgroup duration, 100 MHz ticks (log_cycles): by
+                             work item, by grid slot (2 per tile) or by tile (RASTER) */
 } fgs_frame_view;
 
 /* Workspace sizes in bytes. The gauss workspace depends on P and the image;
