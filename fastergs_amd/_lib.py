@@ -50,6 +50,13 @@ class Options(ctypes.Structure):
                 ("tile_sched", ctypes.POINTER(TileSched))]
 
 
+class Batch(ctypes.Structure):
+    _fields_ = [("n_views", ctypes.c_int), ("cameras", ctypes.POINTER(Camera))]
+
+
+MAX_VIEWS = 32
+
+
 class FrameView(ctypes.Structure):
     _fields_ = [(n, vp) for n in (
         "geom", "splat", "pair_count", "tile_count", "ranges", "bucket_off", "max_contrib",
@@ -100,6 +107,32 @@ SIGNATURES = {
     "fgs_adam_update": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_float,
                                        ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_int, vp]),
+    "fgs_batch_gauss_ws_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.POINTER(Batch),
+                                                  ctypes.POINTER(Options)]),
+    "fgs_batch_pair_ws_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.POINTER(Batch),
+                                                 ctypes.c_int64, ctypes.c_int64,
+                                                 ctypes.POINTER(Options)]),
+    "fgs_batch_grad_ws_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
+    "fgs_batch_preprocess": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.POINTER(Gaussians),
+                                            ctypes.POINTER(Options), vp, ctypes.c_size_t, vp,
+                                            vp]),
+    "fgs_batch_read_counts": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(Batch),
+                                             ctypes.POINTER(Options),
+                                             ctypes.POINTER(ctypes.c_int64), vp]),
+    "fgs_batch_render": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.POINTER(Gaussians),
+                                        ctypes.POINTER(Options), vp, ctypes.c_size_t, vp,
+                                        ctypes.c_size_t, ctypes.c_int64, ctypes.c_int64, vp, vp,
+                                        vp, vp]),
+    "fgs_batch_backward": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.POINTER(Gaussians),
+                                          ctypes.POINTER(Options), vp, vp, ctypes.c_int64,
+                                          ctypes.c_int64, vp, vp, vp, vp, vp,
+                                          ctypes.POINTER(Grads), vp]),
+    "fgs_batch_densify_stats": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
+                                               vp, vp]),
+    "fgs_batch_frame_view_get": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.POINTER(Batch),
+                                                ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.POINTER(Options),
+                                                ctypes.POINTER(FrameView)]),
     "fgs_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fgs_profile_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fgs_last_error": (ctypes.c_char_p, []),

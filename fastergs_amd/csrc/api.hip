@@ -118,6 +118,61 @@ CamRef<float> make_cam(const fgs_camera* c, const fgs_gaussians* g) {
   return r;
 }
 
+// A batch of views (FAST); a single view is a batch of one.
+struct Batch {
+  int V = 0;
+  const fgs_camera* cams = nullptr;
+  int T = 0;        // tiles of all views
+  int64_t HW = 0;   // pixels of all views
+  ViewSetup setup;  // kernel-argument description of the view table
+};
+
+// make_batch without options only sizes the batch (workspace queries, views).
+int make_batch(const fgs_batch* batch, Batch& b, const fgs_options* opt = nullptr) {
+  static_assert(sizeof(ViewRec) <= VIEW_REC_BYTES, "view record too large");
+  static_assert(MAX_VIEWS <= VIEW_TABLE_VIEWS, "view table too small");
+  if (!batch || !batch->cameras) return fail(FGS_ERR_ARG, "batch is NULL");
+  if (batch->n_views < 1 || batch->n_views > MAX_VIEWS)
+    return fail(FGS_ERR_ARG, "n_views must be in [1, 32]");
+  if (opt && opt->schedule == FGS_SCHEDULE_PLAIN && batch->n_views != 1)
+    return fail(FGS_ERR_ARG, "the PLAIN schedule renders one view per call");
+  b.V = batch->n_views;
+  b.cams = batch->cameras;
+  int64_t T = 0, HW = 0;
+  b.setup.n = b.V;
+  for (int v = 0; v < b.V; ++v) {
+    const fgs_camera* c = &b.cams[v];
+    if (opt) {
+      int rc = check_cam(c);
+      if (rc) return rc;
+      if (v > 0 && (c->sh_degree != b.cams[0].sh_degree ||
+                    c->scale_modifier != b.cams[0].scale_modifier || c->mult != b.cams[0].mult))
+        return fail(FGS_ERR_ARG, "views of a batch must share sh_degree, scale_modifier and mult");
+    } else if (c->width <= 0 || c->height <= 0) {
+      return fail(FGS_ERR_ARG, "image size must be positive");
+    }
+    ViewSrc& src = b.setup.v[v];
+    src.view = c->viewmatrix;
+    src.proj = c->projmatrix;
+    src.campos = c->campos;
+    src.bg = c->bg;
+    src.tanfovx = c->tanfovx;
+    src.tanfovy = c->tanfovy;
+    src.focal_x = (float)c->width / (2.0f * c->tanfovx);  // preprocess.cu:941-942
+    src.focal_y = (float)c->height / (2.0f * c->tanfovy);
+    src.W = c->width;
+    src.H = c->height;
+    src.tile_off = (uint32_t)T;
+    src.pix_off = (uint32_t)HW;
+    T += tiles_of(c->width, c->height);
+    HW += (int64_t)c->width * c->height;
+  }
+  if (T >= (1 << 24) || HW >= (int64_t)1 << 32) return fail(FGS_ERR_ARG, "batch too large");
+  b.T = (int)T;
+  b.HW = HW;
+  return FGS_OK;
+}
+
 GaussRef<float> make_gauss(const fgs_gaussians* g) {
   GaussRef<float> r;
   r.means = g->means3D;
@@ -220,10 +275,30 @@ size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
                         const fgs_options* opt) {
   (void)P;
   const bool ck = opt && opt->schedule == FGS_SCHEDULE_FAST;
-  return carve_pair(nullptr, R, NB, width * height, ck).total;
+  return carve_pair(nullptr, R, NB, (int64_t)width * height, ck).total;
 }
 
 size_t fgs_grad_ws_size(int P) { return align_up((size_t)(P > 0 ? P : 1) * RAW_STRIDE * 4); }
+
+size_t fgs_batch_gauss_ws_size(int P, const fgs_batch* batch, const fgs_options* opt) {
+  (void)opt;
+  Batch b;
+  if (make_batch(batch, b) != FGS_OK) return 0;
+  return carve_gauss(nullptr, (int64_t)P * b.V, b.T).total;
+}
+
+size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t NB,
+                              const fgs_options* opt) {
+  (void)P;
+  (void)opt;
+  Batch b;
+  if (make_batch(batch, b) != FGS_OK) return 0;
+  return carve_pair(nullptr, R, NB, b.HW, true).total;
+}
+
+size_t fgs_batch_grad_ws_size(int P, int n_views) {
+  return align_up((size_t)(P > 0 ? P : 1) * (size_t)(n_views > 0 ? n_views : 1) * RAW_STRIDE * 4);
+}
 
 size_t fgs_sort_ws_size(int64_t n) { return sort_ws_bytes(n); }
 size_t fgs_scan_ws_size(int64_t n) { return scan_ws_bytes(n); }
@@ -254,17 +329,21 @@ int fgs_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t*
   return FGS_OK;
 }
 
-int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
-                   void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream) {
+int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const fgs_options* opt,
+                         void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream) {
+  Batch b;
   int rc;
-  if ((rc = check_opts(opt)) || (rc = check_cam(cam)) || (rc = check_gauss(g, cam))) return rc;
-  const int T = tiles_of(cam->width, cam->height);
-  const GaussWs w = carve_gauss(gauss_ws, g->P, T);
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt)) ||
+      (rc = check_gauss(g, &b.cams[0])))
+    return rc;
+  const int64_t PV = (int64_t)g->P * b.V;
+  if (PV >= ((int64_t)1 << 31)) return fail(FGS_ERR_ARG, "P x n_views must be < 2^31");
+  const GaussWs w = carve_gauss(gauss_ws, PV, b.T);
   if (!gauss_ws || gauss_ws_bytes < w.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
   if (g->P > 0 && !radii) return fail(FGS_ERR_ARG, "radii is NULL");
   hipStream_t s = (hipStream_t)stream;
   FGS_TRY(fill_u32(w.counts, 0u, 16, s), "fill counts");
-  if (g->P == 0) return FGS_OK;
+  const fgs_camera* cam = &b.cams[0];
   PreArgs a;
   a.G = make_gauss(g);
   a.cam = make_cam(cam, g);
@@ -279,38 +358,47 @@ int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_opti
   a.dsort_k = w.dsort_k0;
   a.dsort_v = w.dsort_v0;
   a.counts = w.counts;
+  a.views = (const ViewRec*)w.views;
+  a.V = b.V;
   if (opt->schedule == FGS_SCHEDULE_PLAIN) {
+    if (g->P == 0) return FGS_OK;
     FGS_TRY(launch_preprocess_plain(a, s), "preprocess");
     FGS_TRY(scan_u32(w.pcount, w.poff, g->P, w.counts + 0, w.scratch, s), "pair scan");
     return FGS_OK;
   }
-  // FAST: per-Gaussian math + lane-packed exact counts of small rects
+  // FAST: the view table, then per-Gaussian math + lane-packed exact counts
+  // of small rects for every view
+  FGS_TRY(launch_views_setup(b.setup, (ViewRec*)w.views, s), "views");
+  if (g->P == 0) return FGS_OK;
   FGS_TRY(launch_preprocess_fast(a, s), "preprocess");
   // large rects, compacted (count in counts[3]) and walked by whole workgroups
-  FGS_TRY(scan_u32(w.lflag, w.lpos, g->P, w.counts + 3, w.scratch, s), "large scan");
-  FGS_TRY(launch_compact_large(g->P, w.lflag, w.lpos, w.large_ids, s), "large compact");
-  FGS_TRY(launch_large_tiles(false, g->P, w.geom, w.large_ids, w.counts, grid_x_of(cam->width),
-                             w.pcount, nullptr, nullptr, 0u, nullptr, s),
+  FGS_TRY(scan_u32(w.lflag, w.lpos, PV, w.counts + 3, w.scratch, s), "large scan");
+  FGS_TRY(launch_compact_large((int)PV, w.lflag, w.lpos, w.large_ids, s), "large compact");
+  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
+                             w.counts, w.pcount, nullptr, nullptr, 0u, nullptr, s),
           "large tiles");
-  // Gaussians in (depth, index) order; pair offsets assigned in that order so
-  // that a stable sort by tile alone yields the canonical per-tile order.
+  // records in (depth, index) order; pair offsets assigned in that order so
+  // that a stable sort by (global) tile alone yields the canonical per-tile
+  // order of every view
   bool alt = false;
-  FGS_TRY(sort_pairs(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, g->P, 0, 32, w.scratch, s,
-                     alt),
+  FGS_TRY(sort_pairs(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, PV, 0, 32, w.scratch, s, alt),
           "depth sort");
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
-  FGS_TRY(launch_gather_scatter(true, g->P, w.dsort_v0, w.pcount, w.rcount, s), "gather");
-  FGS_TRY(scan_u32(w.rcount, w.roff, g->P, w.counts + 0, w.scratch, s), "pair scan");
-  FGS_TRY(launch_gather_scatter(false, g->P, w.dsort_v0, w.roff, w.poff, s), "scatter");
-  FGS_TRY(launch_nb_capacity(w.counts, opt->bucket_size, T, s), "bucket capacity");
+  FGS_TRY(launch_gather_scatter(true, (int)PV, w.dsort_v0, w.pcount, w.rcount, s), "gather");
+  FGS_TRY(scan_u32(w.rcount, w.roff, PV, w.counts + 0, w.scratch, s), "pair scan");
+  FGS_TRY(launch_gather_scatter(false, (int)PV, w.dsort_v0, w.roff, w.poff, s), "scatter");
+  FGS_TRY(launch_nb_capacity(w.counts, opt->bucket_size, b.T, s), "bucket capacity");
   return FGS_OK;
 }
 
-int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fgs_options* opt,
-                    int64_t counts_host[2], void* stream) {
+int fgs_batch_read_counts(const void* gauss_ws, int P, const fgs_batch* batch,
+                          const fgs_options* opt, int64_t counts_host[2], void* stream) {
   (void)opt;
+  Batch b;
+  int rc;
+  if ((rc = make_batch(batch, b))) return rc;
   if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
-  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height));
+  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), (int64_t)P * b.V, b.T);
   uint32_t c[2] = {0, 0};  // [0] pairs, [1] bucket-table capacity
   hipStream_t s = (hipStream_t)stream;
   FGS_TRY(hipMemcpyAsync(c, w.counts, sizeof(c), hipMemcpyDeviceToHost, s), "read counts");
@@ -320,21 +408,26 @@ int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fg
   return FGS_OK;
 }
 
-int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
-               void* gauss_ws, size_t gauss_ws_bytes, void* pair_ws, size_t pair_ws_bytes,
-               int64_t R, int64_t NB, float* out_color, float* T_final,
-               int32_t* last_contributor, void* stream) {
+int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_options* opt,
+                     void* gauss_ws, size_t gauss_ws_bytes, void* pair_ws, size_t pair_ws_bytes,
+                     int64_t R, int64_t NB, float* out_color, float* T_final,
+                     int32_t* last_contributor, void* stream) {
+  Batch b;
   int rc;
-  if ((rc = check_opts(opt)) || (rc = check_cam(cam)) || (rc = check_gauss(g, cam))) return rc;
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt)) ||
+      (rc = check_gauss(g, &b.cams[0])))
+    return rc;
   if (!out_color || !T_final || !last_contributor) return fail(FGS_ERR_ARG, "NULL output");
   if (R < 0 || NB < 0 || R >= (int64_t)0xffffffffll) return fail(FGS_ERR_ARG, "bad R / NB");
-  const int W = cam->width, H = cam->height, T = tiles_of(W, H);
-  const GaussWs gw = carve_gauss(gauss_ws, g->P, T);
+  const int T = b.T;
+  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)g->P * b.V, T);
   if (!gauss_ws || gauss_ws_bytes < gw.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, W * H, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, fast);
   if (!pair_ws || pair_ws_bytes < pw.total) return fail(FGS_ERR_CAPACITY, "pair workspace too small");
   hipStream_t s = (hipStream_t)stream;
+  const fgs_camera* cam = &b.cams[0];
+  const int W = cam->width, H = cam->height;
   const int gx = grid_x_of(W);
   CompArgs c;
   c.W = W;
@@ -358,15 +451,19 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   c.split_len = 0;
   c.split_pred = nullptr;
   c.cycles = nullptr;
+  c.views = (const ViewRec*)gw.views;
+  c.V = b.V;
   if (fast) {
-    // emission in depth order, tile-only stable sort, lists, ranges, buckets
-    FGS_TRY(fill_u32(gw.lcur, 0u, g->P, s), "zero cursors");
+    const ViewRec* views = (const ViewRec*)gw.views;
+    // emission in depth order, tile-only stable sort, lists, ranges, buckets.
     // R is the pair capacity; the pair count itself (counts[0]) stays on
     // the device, so nothing here waits for the host.
-    FGS_TRY(launch_emit_small(g->P, gw.geom, opt->split_tiles, gx, gw.poff, (uint32_t)R, pw.k0, s),
+    FGS_TRY(fill_u32(gw.lcur, 0u, (int64_t)g->P * b.V, s), "zero cursors");
+    FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.poff, (uint32_t)R,
+                              pw.k0, s),
             "emit");
-    FGS_TRY(launch_large_tiles(true, g->P, gw.geom, gw.large_ids, gw.counts, gx, nullptr, gw.poff,
-                               gw.lcur, (uint32_t)R, pw.k0, s),
+    FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts, nullptr,
+                               gw.poff, gw.lcur, (uint32_t)R, pw.k0, s),
             "emit large");
     if (R > 1) {
       // the last pass writes the lists (low halves) and tile ids (high halves)
@@ -424,56 +521,71 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
   return FGS_OK;
 }
 
-int fgs_backward(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
-                 void* gauss_ws, void* pair_ws, int64_t R, int64_t NB, const int32_t* radii,
-                 const float* T_final, const int32_t* last_contributor, const float* dL_dimage,
-                 void* grad_ws, const fgs_grads* out, void* stream) {
+int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs_options* opt,
+                       void* gauss_ws, void* pair_ws, int64_t R, int64_t NB, const int32_t* radii,
+                       const float* T_final, const int32_t* last_contributor,
+                       const float* dL_dimage, void* grad_ws, const fgs_grads* out,
+                       void* stream) {
+  Batch b;
   int rc;
-  if ((rc = check_opts(opt)) || (rc = check_cam(cam)) || (rc = check_gauss(g, cam))) return rc;
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt)) ||
+      (rc = check_gauss(g, &b.cams[0])))
+    return rc;
   if (!out || !grad_ws || !dL_dimage || !T_final || !last_contributor)
     return fail(FGS_ERR_ARG, "NULL argument");
   if (g->P == 0) return FGS_OK;
-  const int W = cam->width, H = cam->height, T = tiles_of(W, H);
-  const GaussWs gw = carve_gauss(gauss_ws, g->P, T);
+  const fgs_camera* cam = &b.cams[0];
+  const int T = b.T;
+  const int64_t PV = (int64_t)g->P * b.V;
+  const GaussWs gw = carve_gauss(gauss_ws, PV, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, W * H, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, fast);
   hipStream_t s = (hipStream_t)stream;
   float* raw = (float*)grad_ws;
-  FGS_TRY(fill_f32(raw, 0.0f, (int64_t)g->P * RAW_STRIDE, s), "zero raw");
-  BwdArgs b;
-  b.W = W;
-  b.H = H;
-  b.grid_x = grid_x_of(W);
-  b.T = T;
-  b.bucket = opt->bucket_size;
-  b.abs_grad = opt->abs_grad;
-  b.NB = NB;
-  b.ranges = gw.ranges;
-  b.splat = gw.splat;
-  b.bg = cam->bg;
-  b.boff = gw.boff;
-  b.maxc = gw.maxc;
-  b.bucket_ranges = pw.bucket_ranges;
-  b.bucket_tile = pw.bucket_tile;
-  b.tfinal = T_final;
-  b.contrib = last_contributor;
-  b.pixc = pw.pixc;
-  b.acc = pw.acc;
-  b.actoff = gw.actoff;
-  b.dimg = dL_dimage;
-  b.raw = raw;
+  FGS_TRY(fill_f32(raw, 0.0f, PV * RAW_STRIDE, s), "zero raw");
+  BwdArgs bw;
+  bw.W = cam->width;
+  bw.H = cam->height;
+  bw.grid_x = grid_x_of(cam->width);
+  bw.T = T;
+  bw.bucket = opt->bucket_size;
+  bw.abs_grad = opt->abs_grad;
+  bw.NB = NB;
+  bw.ranges = gw.ranges;
+  bw.splat = gw.splat;
+  bw.bg = cam->bg;
+  bw.boff = gw.boff;
+  bw.maxc = gw.maxc;
+  bw.bucket_ranges = pw.bucket_ranges;
+  bw.bucket_tile = pw.bucket_tile;
+  bw.tfinal = T_final;
+  bw.contrib = last_contributor;
+  bw.pixc = pw.pixc;
+  bw.acc = pw.acc;
+  bw.actoff = gw.actoff;
+  bw.dimg = dL_dimage;
+  bw.raw = raw;
+  bw.views = (const ViewRec*)gw.views;
+  bw.V = b.V;
   if (fast) {
-    b.vals = pw.v0;
+    bw.vals = pw.v0;
     FGS_TRY(launch_active_counts(T, gw.maxc, opt->bucket_size, gw.bcount, s), "active buckets");
     FGS_TRY(scan_u32(gw.bcount, gw.actoff, T, gw.actoff + T, gw.scratch, s), "active scan");
-    FGS_TRY(launch_backward_fast(b, s), "raster backward");
+    FGS_TRY(launch_backward_fast(bw, s), "raster backward");
   } else {
-    b.vals = plain_sorted_in_alt(T) ? pw.v1 : pw.v0;
-    FGS_TRY(launch_backward_plain(b, s), "raster backward");
+    bw.vals = plain_sorted_in_alt(T) ? pw.v1 : pw.v0;
+    FGS_TRY(launch_backward_plain(bw, s), "raster backward");
   }
   GaussBwdArgs q;
   q.G = make_gauss(g);
   q.cam = make_cam(cam, g);
+  if (fast) {
+    // the camera the forward used: the view table in the workspace
+    const ViewRec* v0 = (const ViewRec*)gw.views;
+    q.cam.view = v0->view;
+    q.cam.proj = v0->proj;
+    q.cam.campos = v0->campos;
+  }
   q.P = g->P;
   q.raw = raw;
   q.radii = radii;
@@ -488,28 +600,33 @@ int fgs_backward(const fgs_camera* cam, const fgs_gaussians* g, const fgs_option
   q.dsh = q.cam.max_coeffs > 0 ? out->dsh : nullptr;
   q.dscale = out->dscale;
   q.drot = out->drot;
+  q.views = fast ? (const ViewRec*)gw.views : nullptr;
+  q.V = b.V;
   FGS_TRY(launch_gauss_bwd(q, s), "preprocess backward");
   return FGS_OK;
 }
 
-int fgs_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* xyz_grad_accum,
-                      float* xyz_grad_accum_abs, float* max_radii, float* denom, void* stream) {
-  if (P < 0 || (P > 0 && (!radii || !dmean2D))) return fail(FGS_ERR_ARG, "bad stats arguments");
-  FGS_TRY(launch_densify_stats(P, radii, dmean2D, xyz_grad_accum, xyz_grad_accum_abs, max_radii,
-                               denom, (hipStream_t)stream),
+int fgs_batch_densify_stats(int P, int n_views, const int32_t* radii, const float* dmean2D,
+                            float* xyz_grad_accum, float* xyz_grad_accum_abs, float* max_radii,
+                            float* denom, void* stream) {
+  if (P < 0 || n_views < 1 || (P > 0 && (!radii || !dmean2D)))
+    return fail(FGS_ERR_ARG, "bad stats arguments");
+  FGS_TRY(launch_densify_stats(P, n_views, radii, dmean2D, xyz_grad_accum, xyz_grad_accum_abs,
+                               max_radii, denom, (hipStream_t)stream),
           "densify stats");
   return FGS_OK;
 }
 
-int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
-                       int64_t NB, const fgs_options* opt, fgs_frame_view* v) {
+int fgs_batch_frame_view_get(void* gauss_ws, void* pair_ws, int P, const fgs_batch* batch,
+                             int64_t R, int64_t NB, const fgs_options* opt, fgs_frame_view* v) {
+  Batch b;
   int rc;
-  if ((rc = check_opts(opt))) return rc;
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b))) return rc;
   if (!v) return fail(FGS_ERR_ARG, "view is NULL");
-  const int T = tiles_of(width, height);
-  const GaussWs gw = carve_gauss(gauss_ws, P, T);
+  const int T = b.T;
+  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, width * height, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, fast);
   *v = fgs_frame_view{};
   v->geom = gw.geom;
   v->splat = gw.splat;
@@ -541,6 +658,62 @@ int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int heig
     v->item_ticks = gw.cycles;
   }
   return FGS_OK;
+}
+
+// Single-view entry points: a batch of one view.
+int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+                   void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream) {
+  const fgs_batch b{1, cam};
+  return fgs_batch_preprocess(&b, g, opt, gauss_ws, gauss_ws_bytes, radii, stream);
+}
+
+int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fgs_options* opt,
+                    int64_t counts_host[2], void* stream) {
+  (void)opt;
+  if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
+  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height));
+  uint32_t c[2] = {0, 0};  // [0] pairs, [1] bucket-table capacity
+  hipStream_t s = (hipStream_t)stream;
+  FGS_TRY(hipMemcpyAsync(c, w.counts, sizeof(c), hipMemcpyDeviceToHost, s), "read counts");
+  FGS_TRY(hipStreamSynchronize(s), "read counts sync");
+  counts_host[0] = c[0];
+  counts_host[1] = c[1];
+  return FGS_OK;
+}
+
+int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+               void* gauss_ws, size_t gauss_ws_bytes, void* pair_ws, size_t pair_ws_bytes,
+               int64_t R, int64_t NB, float* out_color, float* T_final,
+               int32_t* last_contributor, void* stream) {
+  const fgs_batch b{1, cam};
+  return fgs_batch_render(&b, g, opt, gauss_ws, gauss_ws_bytes, pair_ws, pair_ws_bytes, R, NB,
+                          out_color, T_final, last_contributor, stream);
+}
+
+int fgs_backward(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
+                 void* gauss_ws, void* pair_ws, int64_t R, int64_t NB, const int32_t* radii,
+                 const float* T_final, const int32_t* last_contributor, const float* dL_dimage,
+                 void* grad_ws, const fgs_grads* out, void* stream) {
+  const fgs_batch b{1, cam};
+  return fgs_batch_backward(&b, g, opt, gauss_ws, pair_ws, R, NB, radii, T_final,
+                            last_contributor, dL_dimage, grad_ws, out, stream);
+}
+
+int fgs_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* xyz_grad_accum,
+                      float* xyz_grad_accum_abs, float* max_radii, float* denom, void* stream) {
+  return fgs_batch_densify_stats(P, 1, radii, dmean2D, xyz_grad_accum, xyz_grad_accum_abs,
+                                 max_radii, denom, stream);
+}
+
+int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                       int64_t NB, const fgs_options* opt, fgs_frame_view* v) {
+  // only the sizes of the camera matter here
+  fgs_camera c{};
+  c.width = width;
+  c.height = height;
+  c.tanfovx = c.tanfovy = 1.0f;
+  const fgs_batch b{1, &c};
+  return fgs_batch_frame_view_get(gauss_ws, pair_ws, P, &b, R, NB, opt, v);
 }
 
 int fgs_mlp_forward(const float* params, const float* x, int64_t n, float* y, int bf16,

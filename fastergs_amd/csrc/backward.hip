@@ -116,7 +116,6 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
   const int lane = lane_id();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / WAVE);
   const uint32_t nwaves = gridDim.x * 4;
-  const size_t HW = (size_t)a.W * a.H;
   const int qx = 2 * (lane & 7), qy = 2 * (lane >> 3);
   for (uint32_t u = wave; u < A; u += nwaves) {
     int lo = 0, hi = a.T;  // actoff[lo] <= u < actoff[hi]
@@ -130,7 +129,14 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
     const uint32_t s = start + lb * (uint32_t)a.bucket;
     const int n = (int)min((uint32_t)a.bucket, end - s);
     const size_t bq = (size_t)(a.boff[tile] + lb) * TILE_PIX;
-    const int tx = tile % a.grid_x, ty = tile / a.grid_x;
+    // the tile's view: size, tile grid, background and image offsets
+    const ViewRec& vr = a.views[a.V > 1 ? view_of_tile(a.views, a.V, (uint32_t)tile) : 0];
+    const int W = vr.W, H = vr.H, gx = vr.grid_x;
+    const size_t HW = (size_t)W * H;
+    const size_t poff = vr.pix_off;
+    const float* dimg = a.dimg + 3 * poff;
+    const int lt = tile - (int)vr.tile_off;
+    const int tx = lt % gx, ty = lt / gx;
     float T[4], r0[4], r1[4], r2[4], d0[4], d1[4], d2[4], bgd[4], Tf[4], fx[4], fy[4];
     bool act[4];
 #pragma unroll
@@ -139,8 +145,9 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
       const int px = tx * TILE_W + lx, py = ty * TILE_H + ly;
       fx[k] = (float)px;
       fy[k] = (float)py;
-      const bool inside = px < a.W && py < a.H;
-      const size_t g = inside ? (size_t)py * a.W + px : 0;
+      const bool inside = px < W && py < H;
+      const size_t gl = inside ? (size_t)py * W + px : 0;  // pixel in the view
+      const size_t g = poff + gl;                            // pixel in the batch
       act[k] = inside && lb * (uint32_t)a.bucket < (uint32_t)a.contrib[g];
       T[k] = 1.0f; r0[k] = r1[k] = r2[k] = 0.0f; d0[k] = d1[k] = d2[k] = 0.0f;
       bgd[k] = 0.0f; Tf[k] = 0.0f;
@@ -151,10 +158,10 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
         r0[k] = pc.x - ck.x;
         r1[k] = pc.y - ck.y;
         r2[k] = pc.z - ck.z;
-        d0[k] = a.dimg[g];
-        d1[k] = a.dimg[HW + g];
-        d2[k] = a.dimg[2 * HW + g];
-        bgd[k] = a.bg[0] * d0[k] + a.bg[1] * d1[k] + a.bg[2] * d2[k];
+        d0[k] = dimg[gl];
+        d1[k] = dimg[HW + gl];
+        d2[k] = dimg[2 * HW + gl];
+        bgd[k] = vr.bg[0] * d0[k] + vr.bg[1] * d1[k] + vr.bg[2] * d2[k];
         Tf[k] = a.tfinal[g];
       }
     }
@@ -261,33 +268,130 @@ __global__ __launch_bounds__(GBWD_THREADS) void k_gauss_bwd(GaussBwdArgs a) {
       a.dsh[(size_t)3 * M * g0 + k] = rows[k];
 }
 
+// Batch of V > 1 views: one thread per Gaussian runs the adjoint of every
+// view (records v * P + i) and sums the parameter gradients in view order;
+// the screen-space gradients (dmean2D, dconic) are written per view. The SH
+// rest rows are staged once; per view the adjoint writes its SH gradient to
+// a scratch row that is added to an accumulator row (both in LDS).
+constexpr int GBWDV_THREADS = 64;
+__global__ __launch_bounds__(GBWDV_THREADS) void k_gauss_bwd_views(GaussBwdArgs a) {
+  extern __shared__ float rows[];  // 3 x (GBWDV_THREADS x 3M): input, scratch, sum
+  const int g0 = blockIdx.x * GBWDV_THREADS;
+  const int i = g0 + threadIdx.x;
+  const int M = a.cam.max_coeffs;
+  const int RM = 3 * M;
+  const bool sh = a.dsh != nullptr && M > 0;
+  const int nb = min(GBWDV_THREADS, a.P - g0);
+  float* shin = rows;
+  float* shtmp = rows + GBWDV_THREADS * RM;
+  float* shacc = rows + 2 * GBWDV_THREADS * RM;
+  if (sh)
+    for (int k = threadIdx.x; k < RM * nb; k += GBWDV_THREADS) {
+      shin[k] = a.G.shs[(size_t)RM * g0 + k];
+      shacc[k] = 0.0f;
+    }
+  __syncthreads();
+  if (i < a.P) {
+    float sm[3] = {0, 0, 0}, sc[6] = {0, 0, 0, 0, 0, 0}, ss[3] = {0, 0, 0}, sr[4] = {0, 0, 0, 0};
+    float sdc[3] = {0, 0, 0}, scol[3] = {0, 0, 0}, sop = 0.0f;
+    float* row_in = shin + RM * threadIdx.x;
+    float* row_tmp = shtmp + RM * threadIdx.x;
+    float* row_acc = shacc + RM * threadIdx.x;
+    for (int v = 0; v < a.V; ++v) {
+      const ViewRec& vr = a.views[v];
+      const size_t ri = (size_t)v * a.P + i;
+      const float4* r4 = (const float4*)(a.raw + (size_t)RAW_STRIDE * ri);
+      float raw[RAW_STRIDE];
+      const float4 q0 = r4[0], q1 = r4[1], q2 = r4[2];
+      raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
+      raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
+      raw[8] = q2.x; raw[9] = q2.y; raw[10] = q2.z; raw[11] = q2.w;
+      Grad2D<float> g2;
+      finalize_raw(raw, a.G.opac[i], vr.W, vr.H, g2);
+      if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
+      ((float4*)a.dmean2D)[ri] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
+      ((float4*)a.dconic)[ri] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
+      sop += g2.opacity;
+      for (int k = 0; k < 3; ++k) scol[k] += g2.color[k];
+      if (a.radii[ri] > 0) {
+        float tm[3], tc[6], ts[3], tr[4], tdc[3] = {0.0f, 0.0f, 0.0f};
+        GaussGrad<float> out;
+        out.dmean3d = tm;
+        out.dcov3d = tc;
+        out.dscale = ts;
+        out.drot = tr;
+        out.ddc = tdc;
+        out.dsh = sh ? row_tmp : nullptr;
+        const int clamp = (int)f2u(a.splat[12 * ri + 10]);
+        gauss_bwd(i, a.G, view_cam(a.cam, vr), clamp, g2, out, sh ? row_in : nullptr);
+        for (int k = 0; k < 3; ++k) sm[k] += tm[k];
+        for (int k = 0; k < 6; ++k) sc[k] += tc[k];
+        for (int k = 0; k < 3; ++k) ss[k] += ts[k];
+        for (int k = 0; k < 4; ++k) sr[k] += tr[k];
+        for (int k = 0; k < 3; ++k) sdc[k] += tdc[k];
+        if (sh && a.G.shs != nullptr && a.G.colors == nullptr) {
+          const int written = 3 * (sh_count(a.cam.deg) - 1);
+          for (int k = 0; k < written; ++k) row_acc[k] += row_tmp[k];
+        }
+      }
+    }
+    a.dopacity[i] = sop;
+    for (int k = 0; k < 3; ++k) {
+      a.dcolor[3 * (size_t)i + k] = scol[k];
+      a.dmean3D[3 * (size_t)i + k] = sm[k];
+      a.dscale[3 * (size_t)i + k] = ss[k];
+      a.ddc[3 * (size_t)i + k] = sdc[k];
+    }
+    for (int k = 0; k < 6; ++k) a.dcov3D[6 * (size_t)i + k] = sc[k];
+    for (int k = 0; k < 4; ++k) a.drot[4 * (size_t)i + k] = sr[k];
+  }
+  __syncthreads();
+  if (sh)
+    for (int k = threadIdx.x; k < RM * nb; k += GBWDV_THREADS)
+      a.dsh[(size_t)RM * g0 + k] = shacc[k];
+}
+
 hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
   if (a.P <= 0) return hipSuccess;
   const int M = a.cam.max_coeffs;
-  const size_t shmem = (a.dsh != nullptr && M > 0) ? (size_t)GBWD_THREADS * 3 * M * sizeof(float) : 0;
+  const bool sh = a.dsh != nullptr && M > 0;
+  if (a.views != nullptr && a.V > 1) {
+    const size_t shmem = sh ? (size_t)3 * GBWDV_THREADS * 3 * M * sizeof(float) : 0;
+    FGS_TIMED("gauss_bwd_views", s,
+              (k_gauss_bwd_views<<<(a.P + GBWDV_THREADS - 1) / GBWDV_THREADS, GBWDV_THREADS, shmem,
+                                   s>>>(a)));
+    return hipGetLastError();
+  }
+  const size_t shmem = sh ? (size_t)GBWD_THREADS * 3 * M * sizeof(float) : 0;
   FGS_TIMED("gauss_bwd", s,
             (k_gauss_bwd<<<(a.P + GBWD_THREADS - 1) / GBWD_THREADS, GBWD_THREADS, shmem, s>>>(a)));
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_densify_stats(int P, const int32_t* __restrict__ radii,
+__global__ __launch_bounds__(256) void k_densify_stats(int P, int V,
+                                                       const int32_t* __restrict__ radii,
                                                        const float* __restrict__ dmean2D,
                                                        float* accum, float* accum_abs,
                                                        float* max_radii, float* denom) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= P || !(radii[i] > 0)) return;
-  const float4 g = ((const float4*)dmean2D)[i];
-  if (denom) denom[i] += 1.0f;
-  if (max_radii) max_radii[i] = fmaxf(max_radii[i], (float)radii[i]);
-  if (accum) accum[i] += sqrtf(g.x * g.x + g.y * g.y);
-  if (accum_abs) accum_abs[i] += sqrtf(g.z * g.z + g.w * g.w);
+  if (i >= P) return;
+  for (int v = 0; v < V; ++v) {  // one view after the other, as separate calls would
+    const size_t ri = (size_t)v * P + i;
+    if (!(radii[ri] > 0)) continue;
+    const float4 g = ((const float4*)dmean2D)[ri];
+    if (denom) denom[i] += 1.0f;
+    if (max_radii) max_radii[i] = fmaxf(max_radii[i], (float)radii[ri]);
+    if (accum) accum[i] += sqrtf(g.x * g.x + g.y * g.y);
+    if (accum_abs) accum_abs[i] += sqrtf(g.z * g.z + g.w * g.w);
+  }
 }
 
-hipError_t launch_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* accum,
-                                float* accum_abs, float* max_radii, float* denom, hipStream_t s) {
-  if (P > 0)
-    k_densify_stats<<<(P + 255) / 256, 256, 0, s>>>(P, radii, dmean2D, accum, accum_abs, max_radii,
-                                                    denom);
+hipError_t launch_densify_stats(int P, int V, const int32_t* radii, const float* dmean2D,
+                                float* accum, float* accum_abs, float* max_radii, float* denom,
+                                hipStream_t s) {
+  if (P > 0 && V > 0)
+    k_densify_stats<<<(P + 255) / 256, 256, 0, s>>>(P, V, radii, dmean2D, accum, accum_abs,
+                                                    max_radii, denom);
   return hipGetLastError();
 }
 

@@ -7,6 +7,66 @@
 
 namespace fgs {
 
+// ------------------------------------------------------------------ views
+// The FAST schedule renders a batch of V views in one launch per stage.
+// Per-Gaussian records are indexed view-major (record v * P + i for
+// Gaussian i in view v); the tiles of all views form one index space (view
+// v owns tiles [tile_off, tile_off + grid_x * grid_y)), and per-pixel
+// arrays are the views' images concatenated (view v from pixel pix_off;
+// colour planes at 3 * pix_off). A single view is a batch of one.
+constexpr int MAX_VIEWS = 32;  // ViewSetup travels as kernel arguments (< 4 KB)
+struct ViewRec {
+  float view[16];    // world -> view, column-major
+  float proj[16];    // world -> clip, column-major
+  float campos[4];
+  float bg[4];
+  float tanfovx, tanfovy, focal_x, focal_y;
+  int W, H, grid_x, grid_y;
+  uint32_t tile_off, pix_off, pad0, pad1;
+};
+struct ViewSrc {     // host-side description; matrices are device pointers
+  const float* view;
+  const float* proj;
+  const float* campos;
+  const float* bg;
+  float tanfovx, tanfovy, focal_x, focal_y;
+  int W, H;
+  uint32_t tile_off, pix_off;
+};
+struct ViewSetup {
+  ViewSrc v[MAX_VIEWS];
+  int n;
+};
+// Copies the camera data into the device table (kernel arguments only, so
+// a captured graph re-reads the camera buffers on every replay).
+hipError_t launch_views_setup(const ViewSetup& src, ViewRec* out, hipStream_t s);
+
+// Camera of view r for the shared per-Gaussian math.
+__device__ __forceinline__ CamRef<float> view_cam(const CamRef<float>& base, const ViewRec& r) {
+  CamRef<float> c = base;
+  c.view = r.view;
+  c.proj = r.proj;
+  c.campos = r.campos;
+  c.tanfovx = r.tanfovx;
+  c.tanfovy = r.tanfovy;
+  c.focal_x = r.focal_x;
+  c.focal_y = r.focal_y;
+  c.W = r.W;
+  c.H = r.H;
+  c.grid_x = r.grid_x;
+  c.grid_y = r.grid_y;
+  return c;
+}
+// View owning global tile t (binary search over the views' first tiles).
+__device__ __forceinline__ int view_of_tile(const ViewRec* views, int V, uint32_t t) {
+  int lo = 0, hi = V;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (views[mid].tile_off <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // scan_sort.hip
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
                     hipStream_t s);
@@ -38,6 +98,8 @@ struct PreArgs {
   uint64_t* dsort_k;   // fast: depth keys for the Gaussian depth sort
   uint32_t* dsort_v;   // fast: Gaussian indices for the depth sort
   uint32_t* counts;
+  const ViewRec* views;  // fast: V views (records are V x P)
+  int V;
 };
 hipError_t launch_preprocess_plain(const PreArgs& a, hipStream_t s);
 hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
@@ -46,12 +108,14 @@ hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
                                   uint32_t cap, uint64_t* keys, uint32_t* vals, hipStream_t s);
-hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* large_ids,
-                              const uint32_t* counts, int gx, uint32_t* pcount,
+// FAST emission over V views of P Gaussians (records v * P + i); pair
+// values are record indices, tile ids global.
+hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
+                              const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
                               const uint32_t* poff, uint32_t* lcur, uint32_t cap,
                               uint64_t* packed, hipStream_t s);
-hipError_t launch_emit_small(int P, const float* geom, int split, int gx, const uint32_t* poff,
-                             uint32_t cap, uint64_t* packed, hipStream_t s);
+hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
+                             const uint32_t* poff, uint32_t cap, uint64_t* packed, hipStream_t s);
 hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lpos,
                                 uint32_t* large_ids, hipStream_t s);
 hipError_t launch_gather_scatter(bool gather, int P, const uint32_t* order, const uint32_t* src,
@@ -97,6 +161,8 @@ struct CompArgs {
   int split_len;
   const float* split_pred;
   uint32_t* cycles;
+  const ViewRec* views;  // fast: tiles / pixels of V views (W, H, grid_x, bg per view)
+  int V;
 };
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);
@@ -135,9 +201,14 @@ struct BwdArgs {
   const uint32_t* actoff;  // fast: exclusive prefix of active buckets per tile, [T] = total
   const float* dimg;
   float* raw;
+  const ViewRec* views;  // fast: V views
+  int V;
 };
 hipError_t launch_backward_plain(const BwdArgs& a, hipStream_t s);
 hipError_t launch_backward_fast(const BwdArgs& a, hipStream_t s);
+// Per-Gaussian adjoint. With views (V >= 1 of them, records v * P + i):
+// dmean2D / dconic are per view (V x P rows), every other gradient is the
+// sum over the views (fixed order). Without views: the single camera cam.
 struct GaussBwdArgs {
   GaussRef<float> G;
   CamRef<float> cam;
@@ -146,9 +217,14 @@ struct GaussBwdArgs {
   const int32_t* radii;
   const float* splat;
   float *dmean2D, *dconic, *dopacity, *dcolor, *dmean3D, *dcov3D, *ddc, *dsh, *dscale, *drot;
+  const ViewRec* views;
+  int V;
 };
 hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s);
-hipError_t launch_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* accum,
-                                float* accum_abs, float* max_radii, float* denom, hipStream_t s);
+// Densification statistics of V views (radii / dmean2D rows v * P + i) added
+// in view order into the P-sized statistics.
+hipError_t launch_densify_stats(int P, int V, const int32_t* radii, const float* dmean2D,
+                                float* accum, float* accum_abs, float* max_radii, float* denom,
+                                hipStream_t s);
 
 }  // namespace fgs

@@ -21,6 +21,10 @@
 namespace fgs {
 
 constexpr size_t ALIGN = 256;
+// Per-view camera table at the start of the gauss workspace (32 views of at
+// most VIEW_REC_BYTES each; the record type lives with the kernels).
+constexpr int VIEW_TABLE_VIEWS = 32;
+constexpr size_t VIEW_REC_BYTES = 256;
 inline size_t align_up(size_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
 
 // Radix sort / scan tiling (must match scan_sort.hip).
@@ -79,14 +83,17 @@ struct GaussWs {
   float* pred;         // T x 2 cost-model outputs (predicted log2 cost, split logit)
   uint32_t* items;     // 2T compositing work items (tile << 2 | part)
   uint32_t* cycles;    // 2T per-item timing log (100 MHz ticks)
+  void* views;         // view table (FAST)
   void* scratch;
   size_t scratch_bytes;
   size_t total;
 };
 
-inline GaussWs carve_gauss(void* base, int P, int T) {
+// P: records (views x Gaussians), T: tiles of all views.
+inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   Carver c(base);
   GaussWs w;
+  w.views = c.take<char>(VIEW_TABLE_VIEWS * VIEW_REC_BYTES);
   const size_t p = (size_t)(P > 0 ? P : 1);
   const size_t t = (size_t)(T > 0 ? T : 1);
   w.geom = c.take<float>(8 * p);
@@ -140,7 +147,7 @@ struct PairWs {
   size_t total;
 };
 
-inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int HW, bool fast) {
+inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int64_t HW, bool fast) {
   Carver c(base);
   PairWs w;
   const size_t r = (size_t)(R > 0 ? R : 1);

@@ -26,7 +26,7 @@
 
 namespace fgs {
 
-__device__ __forceinline__ void store_records(int i, const Proj<float>& o, float* geom,
+__device__ __forceinline__ void store_records(size_t i, const Proj<float>& o, float* geom,
                                               float* splat) {
   float4* g4 = (float4*)(geom + 8 * (size_t)i);
   const uint32_t rmin = (uint32_t)o.rect[0] | ((uint32_t)o.rect[1] << 16);
@@ -41,7 +41,7 @@ __device__ __forceinline__ void store_records(int i, const Proj<float>& o, float
   s4[2] = make_float4(o.rgb[2], o.depth, u2f((uint32_t)o.clamp), 0.0f);
 }
 
-__device__ __forceinline__ void store_invalid(int i, float* geom, float* splat) {
+__device__ __forceinline__ void store_invalid(size_t i, float* geom, float* splat) {
   float4* g4 = (float4*)(geom + 8 * (size_t)i);
   g4[0] = make_float4(0.f, 0.f, 0.f, 0.f);
   g4[1] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -54,7 +54,7 @@ struct GeomRow {
   int r0, r1, r2, r3;
 };
 
-__device__ __forceinline__ GeomRow load_geom(const float* geom, int i) {
+__device__ __forceinline__ GeomRow load_geom(const float* geom, size_t i) {
   const float4* g4 = (const float4*)(geom + 8 * (size_t)i);
   const float4 a = g4[0], b = g4[1];
   GeomRow r;
@@ -147,9 +147,9 @@ struct WaveTiles {
 // the exact tile test. Within a Gaussian's block of the pair array the order
 // is free: its pairs are in distinct tiles and the tile sort is stable.
 template <bool EMIT>
-__device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, int gbase, int gx,
-                                           const uint32_t* __restrict__ poff, uint32_t cap,
-                                           uint64_t* __restrict__ packed) {
+__device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, uint32_t gbase, int gx,
+                                           uint32_t toff, const uint32_t* __restrict__ poff,
+                                           uint32_t cap, uint64_t* __restrict__ packed) {
   const int lane = lane_id();
   for (uint32_t base = 0; base < S; base += WAVE) {
     const uint32_t f = base + lane;
@@ -167,9 +167,9 @@ __device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, int gbase, 
       if (tile_hit(tx, ty, L.x[j], L.y[j], conic, L.pw[j])) {
         const uint32_t k = atomicAdd(&L.cnt[j], 1u);
         if (EMIT) {
-          const uint32_t g = (uint32_t)(gbase + j);
+          const uint32_t g = gbase + (uint32_t)j;
           const uint32_t pos = poff[g] + k;
-          if (pos < cap) packed[pos] = ((uint64_t)(uint32_t)(ty * gx + tx) << 32) | g;
+          if (pos < cap) packed[pos] = ((uint64_t)(toff + (uint32_t)(ty * gx + tx)) << 32) | g;
         }
       }
     }
@@ -200,37 +200,40 @@ __device__ __forceinline__ void stage_small(WaveTiles& L, int lane, bool small, 
 // (colour comes from k_sh_color), then the wave's lane-packed tile walk.
 // Only 3 KB of LDS per wave64 workgroup, so occupancy is set by registers.
 constexpr int PRE_THREADS = WAVE;
+// Grid (ceil(P / 64), V): blockIdx.y is the view, records v * P + i.
 __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a) {
   __shared__ WaveTiles L;
+  const int v = blockIdx.y;
+  const CamRef<float> cam = view_cam(a.cam, a.views[v]);
   const int i = blockIdx.x * PRE_THREADS + threadIdx.x;
   const int lane = threadIdx.x;
-  const int g0 = blockIdx.x * PRE_THREADS;
+  const size_t ri = (size_t)v * a.P + i;
   Proj<float> o;
   o.valid = 0;
   if (i < a.P) {
     // SH colour is evaluated by k_sh_color; precomputed colours are copied here
-    preprocess_gauss<float>(i, a.G, a.cam, o, nullptr, nullptr, a.G.colors != nullptr);
-    a.radii[i] = o.valid ? o.radius : 0;
+    preprocess_gauss<float>(i, a.G, cam, o, nullptr, nullptr, a.G.colors != nullptr);
+    a.radii[ri] = o.valid ? o.radius : 0;
     const uint32_t dk = o.valid ? depth_bits(o.depth) : 0xffffffffu;
-    a.dkey[i] = dk;
-    a.dsort_k[i] = dk;
-    a.dsort_v[i] = (uint32_t)i;
+    a.dkey[ri] = dk;
+    a.dsort_k[ri] = dk;
+    a.dsort_v[ri] = (uint32_t)ri;
     if (o.valid)
-      store_records(i, o, a.geom, a.splat);
+      store_records(ri, o, a.geom, a.splat);
     else
-      store_invalid(i, a.geom, a.splat);
+      store_invalid(ri, a.geom, a.splat);
   }
   const int area = o.valid ? (o.rect[2] - o.rect[0]) * (o.rect[3] - o.rect[1]) : 0;
   const bool small = o.valid && area < a.split_tiles;
   // large rects: flagged here, compacted by a scan (no same-address atomics)
-  if (i < a.P) a.lflag[i] = (o.valid && !small) ? 1u : 0u;
+  if (i < a.P) a.lflag[ri] = (o.valid && !small) ? 1u : 0u;
   uint32_t S;
   stage_small(L, lane, small, o.x, o.y, o.conic, o.power, o.rect[0], o.rect[1], o.rect[2],
               o.rect[3], S);
   __syncthreads();
-  walk_small<false>(L, S, g0, a.cam.grid_x, nullptr, 0u, nullptr);
+  walk_small<false>(L, S, 0u, cam.grid_x, 0u, nullptr, 0u, nullptr);
   __syncthreads();
-  if (i < a.P) a.pcount[i] = L.cnt[lane];
+  if (i < a.P) a.pcount[ri] = L.cnt[lane];
 }
 
 // Colour pass: SH degree <= 3 of every valid Gaussian (preprocess.cu:698-707)
@@ -264,23 +267,30 @@ __global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
     for (int k = lane; k < 3 * M * nb; k += PRE_THREADS) ssh[k] = gsh[k];
   }
   __syncthreads();
-  if (i >= a.P || a.dkey[i] == 0xffffffffu) return;
+  if (i >= a.P) return;
   const float* pm = a.G.means + 3 * (size_t)i;
-  float rgb[3];
-  int clamp;
-  sh_color_rows(pm[0] - a.cam.campos[0], pm[1] - a.cam.campos[1], pm[2] - a.cam.campos[2],
-                sdc + 3 * lane, ssh + 3 * M * lane, a.cam.deg, rgb, clamp);
-  float* sp = a.splat + 12 * (size_t)i;
-  sp[6] = rgb[0];
-  sp[7] = rgb[1];
-  sp[8] = rgb[2];
-  sp[10] = u2f((uint32_t)clamp);
+  // the coefficients are staged once for all views of the batch
+  for (int v = 0; v < a.V; ++v) {
+    const size_t ri = (size_t)v * a.P + i;
+    if (a.dkey[ri] == 0xffffffffu) continue;
+    const float* cp = a.views[v].campos;
+    float rgb[3];
+    int clamp;
+    sh_color_rows(pm[0] - cp[0], pm[1] - cp[1], pm[2] - cp[2], sdc + 3 * lane,
+                  ssh + 3 * M * lane, a.cam.deg, rgb, clamp);
+    float* sp = a.splat + 12 * ri;
+    sp[6] = rgb[0];
+    sp[7] = rgb[1];
+    sp[8] = rgb[2];
+    sp[10] = u2f((uint32_t)clamp);
+  }
 }
 
 hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s) {
   if (a.P <= 0) return hipSuccess;
   const unsigned blocks = (a.P + PRE_THREADS - 1) / PRE_THREADS;
-  FGS_TIMED("preprocess_fast", s, (k_preprocess_fast<<<blocks, PRE_THREADS, 0, s>>>(a)));
+  FGS_TIMED("preprocess_fast", s,
+            (k_preprocess_fast<<<dim3(blocks, (unsigned)a.V), PRE_THREADS, 0, s>>>(a)));
   if (a.G.colors == nullptr && a.G.dc != nullptr) {
     const size_t shmem = (size_t)PRE_THREADS * 3 * (1 + a.cam.max_coeffs) * sizeof(float);
     FGS_TIMED("sh_color", s, (k_sh_color<<<blocks, PRE_THREADS, shmem, s>>>(a)));
@@ -293,9 +303,10 @@ hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s) {
 // per wave step, 256 per workgroup step. One atomic per wave for the count;
 // emission reserves slots in the Gaussian's block with one atomic per step.
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_large_tiles(const float* __restrict__ geom,
+__global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec* __restrict__ views,
+                                                     const float* __restrict__ geom,
                                                      const uint32_t* __restrict__ large_ids,
-                                                     const uint32_t* __restrict__ counts, int gx,
+                                                     const uint32_t* __restrict__ counts,
                                                      uint32_t* pcount,
                                                      const uint32_t* __restrict__ poff,
                                                      uint32_t* lcur, uint32_t cap,
@@ -304,7 +315,10 @@ __global__ __launch_bounds__(256) void k_large_tiles(const float* __restrict__ g
   const int lane = lane_id();
   const int w = threadIdx.x / WAVE;
   for (uint32_t j = blockIdx.x; j < NL; j += gridDim.x) {
-    const int g = (int)large_ids[j];
+    const uint32_t g = large_ids[j];  // record index v * P + i
+    const int v = V > 1 ? (int)(g / (uint32_t)P) : 0;
+    const int gx = views[v].grid_x;
+    const uint32_t toff = views[v].tile_off;
     const GeomRow r = load_geom(geom, g);
     const int wr = r.r2 - r.r0;
     const int area = wr * (r.r3 - r.r1);
@@ -324,7 +338,7 @@ __global__ __launch_bounds__(256) void k_large_tiles(const float* __restrict__ g
         slot0 = __shfl(slot0, 0, WAVE);
         const uint32_t pos = poff[g] + slot0 + lanes_below(m);
         if (hit && pos < cap)
-          packed[pos] = ((uint64_t)(uint32_t)((r.r1 + ty) * gx + r.r0 + tx) << 32) | (uint32_t)g;
+          packed[pos] = ((uint64_t)(toff + (uint32_t)((r.r1 + ty) * gx + r.r0 + tx)) << 32) | g;
       }
       l += 4 * WAVE;
       tx += sx;
@@ -338,51 +352,89 @@ __global__ __launch_bounds__(256) void k_large_tiles(const float* __restrict__ g
   }
 }
 
-hipError_t launch_large_tiles(bool emit, int P, const float* geom, const uint32_t* large_ids,
-                              const uint32_t* counts, int gx, uint32_t* pcount,
+hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
+                              const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
                               const uint32_t* poff, uint32_t* lcur, uint32_t cap,
                               uint64_t* packed, hipStream_t s) {
   if (P <= 0) return hipSuccess;
   const int blocks = 2048;  // persistent workgroups; each exits past the device-side count
   if (emit)
     FGS_TIMED("large_tiles_emit", s,
-              (k_large_tiles<true><<<blocks, 256, 0, s>>>(geom, large_ids, counts, gx, pcount, poff,
-                                                          lcur, cap, packed)));
+              (k_large_tiles<true><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
+                                                          pcount, poff, lcur, cap, packed)));
   else
     FGS_TIMED("large_tiles_count", s,
-              (k_large_tiles<false><<<blocks, 256, 0, s>>>(geom, large_ids, counts, gx, pcount,
-                                                           poff, lcur, cap, packed)));
+              (k_large_tiles<false><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
+                                                           pcount, poff, lcur, cap, packed)));
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_emit_small(int P, const float* __restrict__ geom,
-                                                    int split, int gx,
+// Grid (ceil(P / 256), V): blockIdx.y is the view; each wave walks the small
+// rects of 64 consecutive Gaussians of one view.
+__global__ __launch_bounds__(256) void k_emit_small(int P, const ViewRec* __restrict__ views,
+                                                    const float* __restrict__ geom, int split,
                                                     const uint32_t* __restrict__ poff, uint32_t cap,
                                                     uint64_t* __restrict__ packed) {
   __shared__ WaveTiles lds[4];
+  const int v = blockIdx.y;
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int lane = lane_id();
   WaveTiles& L = lds[threadIdx.x / WAVE];
-  const int gbase = blockIdx.x * 256 + (threadIdx.x / WAVE) * WAVE;
+  const uint32_t gbase = (uint32_t)((size_t)v * P) + blockIdx.x * 256 + (threadIdx.x / WAVE) * WAVE;
   GeomRow r{};
   bool small = false;
   if (i < P) {
-    r = load_geom(geom, i);
+    r = load_geom(geom, (size_t)v * P + i);
     const int area = (r.r2 - r.r0) * (r.r3 - r.r1);
     small = area > 0 && area < split;
   }
   uint32_t S;
   stage_small(L, lane, small, r.x, r.y, r.conic, r.power, r.r0, r.r1, r.r2, r.r3, S);
   __syncthreads();
-  walk_small<true>(L, S, gbase, gx, poff, cap, packed);
+  walk_small<true>(L, S, gbase, views[v].grid_x, views[v].tile_off, poff, cap, packed);
 }
 
-hipError_t launch_emit_small(int P, const float* geom, int split, int gx, const uint32_t* poff,
-                             uint32_t cap, uint64_t* packed, hipStream_t s) {
+hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
+                             const uint32_t* poff, uint32_t cap, uint64_t* packed, hipStream_t s) {
   if (P > 0)
     FGS_TIMED("emit_small", s,
-              (k_emit_small<<<(P + 255) / 256, 256, 0, s>>>(P, geom, split, gx, poff, cap,
-                                                             packed)));
+              (k_emit_small<<<dim3((P + 255) / 256, (unsigned)V), 256, 0, s>>>(P, views, geom, split,
+                                                                               poff, cap, packed)));
+  return hipGetLastError();
+}
+
+// View table from kernel arguments (see ViewSetup).
+__global__ void k_views_setup(ViewSetup src, ViewRec* out) {
+  const int v = blockIdx.x;
+  const int t = threadIdx.x;
+  if (v >= src.n) return;
+  const ViewSrc& s = src.v[v];
+  ViewRec& r = out[v];
+  if (t < 16) {
+    r.view[t] = s.view[t];
+    r.proj[t] = s.proj[t];
+  } else if (t < 19) {
+    r.campos[t - 16] = s.campos[t - 16];
+    r.bg[t - 16] = s.bg ? s.bg[t - 16] : 0.0f;
+  } else if (t == 19) {
+    r.campos[3] = 0.0f;
+    r.bg[3] = 0.0f;
+    r.tanfovx = s.tanfovx;
+    r.tanfovy = s.tanfovy;
+    r.focal_x = s.focal_x;
+    r.focal_y = s.focal_y;
+    r.W = s.W;
+    r.H = s.H;
+    r.grid_x = (s.W + TILE_W - 1) / TILE_W;
+    r.grid_y = (s.H + TILE_H - 1) / TILE_H;
+    r.tile_off = s.tile_off;
+    r.pix_off = s.pix_off;
+    r.pad0 = r.pad1 = 0u;
+  }
+}
+
+hipError_t launch_views_setup(const ViewSetup& src, ViewRec* out, hipStream_t s) {
+  if (src.n > 0) k_views_setup<<<src.n, 64, 0, s>>>(src, out);
   return hipGetLastError();
 }
 

@@ -321,7 +321,11 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   const bool band = part != 0;
   const int row0 = part == 2 ? TILE_H / 2 : 0;
   const int nrows = band ? TILE_H / 2 : TILE_H;
-  const int tx = tile % a.grid_x, ty = tile / a.grid_x;
+  // the tile's view: its size, tile grid, background and image offsets
+  const ViewRec& vr = a.views[a.V > 1 ? view_of_tile(a.views, a.V, (uint32_t)tile) : 0];
+  const int W = vr.W, H = vr.H, gx = vr.grid_x;
+  const int lt = tile - (int)vr.tile_off;
+  const int tx = lt % gx, ty = lt / gx;
   const int t = threadIdx.x;
   const int w = t / WAVE, lane = t & (WAVE - 1);
   // merge / output pixel of this thread (lp: index inside the tile)
@@ -329,7 +333,7 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   const int lp = mrow * TILE_W + (t & 15);
   const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + mrow;
   const bool merger = (t >> 4) < nrows;
-  const bool inside = merger && px < a.W && py < a.H;
+  const bool inside = merger && px < W && py < H;
   // the pixels of this lane in the bucket pass: a 2x2 quad, or a 2x1 pair in a band
   const int qx = 2 * (lane & 7), qy = band ? row0 + (lane >> 3) : 2 * (lane >> 3);
   const float fx0 = (float)(tx * TILE_W + qx), fy0 = (float)(ty * TILE_H + qy);
@@ -376,11 +380,12 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
     }
   }
   if (inside) {
-    const size_t HW = (size_t)a.W * a.H;
-    const size_t g = (size_t)py * a.W + px;
-    a.out[g] = fma_(T, a.bg[0], C0);
-    a.out[HW + g] = fma_(T, a.bg[1], C1);
-    a.out[2 * HW + g] = fma_(T, a.bg[2], C2);
+    const size_t HW = (size_t)W * H;
+    const size_t g = (size_t)vr.pix_off + (size_t)py * W + px;
+    float* out = a.out + 3 * (size_t)vr.pix_off + (size_t)py * W + px;
+    out[0] = fma_(T, vr.bg[0], C0);
+    out[HW] = fma_(T, vr.bg[1], C1);
+    out[2 * HW] = fma_(T, vr.bg[2], C2);
     a.tfinal[g] = T;
     a.contrib[g] = cnt;
     ((float4*)a.pixc)[g] = make_float4(C0, C1, C2, 0.0f);

@@ -214,6 +214,44 @@ int fgs_adam_update(float* params, const float* grad, float* exp_avg, float* exp
                     int64_t n, float lr, float beta1, float beta2, float eps, int step,
                     void* stream);
 
+/* Batches of views (FAST schedule): n_views <= 32 cameras (same sh_degree,
+ * scale_modifier and mult; any image sizes) render the same Gaussians with
+ * one launch per stage -- the multi-view training step. Per-view arrays are
+ * concatenated in view order:
+ *   radii, dmean2D, dconic        n_views x P rows
+ *   out_color and dL_dimage      view v: 3 x H_v x W_v from 3 * pix_off(v)
+ *   T_final, last_contributor    view v: H_v x W_v from pix_off(v)
+ * where pix_off(v) = sum over u < v of H_u * W_u. Every other gradient
+ * (dopacity, dcolor, dmean3D, dcov3D, ddc, dsh, dscale, drot) is the sum
+ * over the views, added in view order; fgs_batch_densify_stats adds the
+ * views' statistics in view order. Each single-view entry point above is
+ * the batch of its one camera. */
+typedef struct fgs_batch {
+  int n_views;
+  const fgs_camera* cameras; /* host array of n_views cameras */
+} fgs_batch;
+size_t fgs_batch_gauss_ws_size(int P, const fgs_batch* batch, const fgs_options* opt);
+size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t NB,
+                              const fgs_options* opt);
+size_t fgs_batch_grad_ws_size(int P, int n_views);
+int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const fgs_options* opt,
+                         void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream);
+int fgs_batch_read_counts(const void* gauss_ws, int P, const fgs_batch* batch,
+                          const fgs_options* opt, int64_t counts_host[2], void* stream);
+int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_options* opt,
+                     void* gauss_ws, size_t gauss_ws_bytes, void* pair_ws, size_t pair_ws_bytes,
+                     int64_t R, int64_t NB, float* out_color, float* T_final,
+                     int32_t* last_contributor, void* stream);
+int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs_options* opt,
+                       void* gauss_ws, void* pair_ws, int64_t R, int64_t NB, const int32_t* radii,
+                       const float* T_final, const int32_t* last_contributor,
+                       const float* dL_dimage, void* grad_ws, const fgs_grads* out, void* stream);
+int fgs_batch_densify_stats(int P, int n_views, const int32_t* radii, const float* dmean2D,
+                            float* xyz_grad_accum, float* xyz_grad_accum_abs, float* max_radii,
+                            float* denom, void* stream);
+int fgs_batch_frame_view_get(void* gauss_ws, void* pair_ws, int P, const fgs_batch* batch,
+                             int64_t R, int64_t NB, const fgs_options* opt, fgs_frame_view* view);
+
 /* Per-kernel timing for measurement runs: while enabled, kernel launches are
  * bracketed by HIP events on their stream; fgs_profile_read synchronises and
  * writes "name total_ms launches\n" lines (aggregated since the last enable)

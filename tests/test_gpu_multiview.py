@@ -1,0 +1,137 @@
+"""Batches of views (one launch per stage) against single-view frames.
+
+For a batch of cameras with different image sizes:
+* every view's image, T_final, last contributors and radii are bit-identical
+  to a single-view FAST frame of that camera, and its per-tile lists are
+  the single-view lists (record indices v * P + i mapped back to i);
+* the screen-space gradients of each view match the single-view ones and
+  the parameter gradients match the sum of the single-view gradients
+  (1e-4 relative, float atomics reassociate sums);
+* the densification statistics equal consecutive single-view updates;
+* the autograd entry point returns per-view images and the same gradients.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fastergs_amd import scenes
+from fastergs_amd._lib import SCHEDULE_FAST
+from fastergs_amd.multiview import BatchFrame, rasterize_views
+from fastergs_amd.rasterizer import Frame, RasterOptions
+from tests.helpers import grad_close, settings_for, to_dev
+
+pytestmark = pytest.mark.gpu
+
+SIZES = ((320, 240), (200, 150), (416, 300), (256, 256))
+
+
+@pytest.fixture(scope="module")
+def batch_case():
+    scene = scenes.cluster_scene(30000, seed=21, clusters=16, radius=1.5)
+    cams = scenes.sphere_cameras(len(SIZES), seed=4, rmin=3.0, rmax=4.5, sizes=SIZES)
+    d = to_dev(scene)
+    opts = RasterOptions(schedule=SCHEDULE_FAST)
+    bf = BatchFrame([settings_for(c) for c in cams], d["means3D"], d["opacities"], d["scales"],
+                    d["rotations"], dc=d["dc"], shs=d["shs"], options=opts).forward()
+    singles = [Frame(settings_for(c), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+                     dc=d["dc"], shs=d["shs"], options=opts).forward() for c in cams]
+    torch.cuda.synchronize()
+    return scene, cams, d, bf, singles
+
+
+def test_forward_matches_single_views(batch_case):
+    print("\n[multiview] forward", flush=True)
+    scene, cams, d, bf, singles = batch_case
+    imgs = bf.images()
+    tf = bf.per_view(bf.T_final)
+    lc = bf.per_view(bf.last_contributor)
+    assert bf.R == sum(s.R for s in singles)
+    for v, s in enumerate(singles):
+        np.testing.assert_array_equal(imgs[v].cpu().numpy(), s.out_color.cpu().numpy())
+        np.testing.assert_array_equal(tf[v].cpu().numpy(), s.T_final[..., 0].cpu().numpy())
+        np.testing.assert_array_equal(lc[v].cpu().numpy(), s.last_contributor[..., 0].cpu().numpy())
+        np.testing.assert_array_equal(bf.radii[v].cpu().numpy(), s.radii.cpu().numpy())
+
+
+def test_lists_match_single_views(batch_case):
+    scene, cams, d, bf, singles = batch_case
+    bv = bf.view_arrays()
+    rng = bv["ranges"].cpu().numpy().view(np.uint32).reshape(-1, 2)
+    vals = bv["sorted_vals"].cpu().numpy().view(np.uint32)
+    P = bf.P
+    toff = 0
+    for v, s in enumerate(singles):
+        sv = s.view_arrays()
+        srng = sv["ranges"].cpu().numpy().view(np.uint32).reshape(-1, 2)
+        svals = sv["sorted_vals"].cpu().numpy().view(np.uint32)
+        T = srng.shape[0]
+        for t in range(T):
+            a0, a1 = rng[toff + t]
+            b0, b1 = srng[t]
+            assert a1 - a0 == b1 - b0, (v, t)
+            if a1 > a0:
+                got = vals[a0:a1].astype(np.int64) - v * P
+                np.testing.assert_array_equal(got, svals[b0:b1])
+        toff += T
+    assert toff == rng.shape[0]
+
+
+def test_backward_matches_single_views(batch_case):
+    print("\n[multiview] backward", flush=True)
+    scene, cams, d, bf, singles = batch_case
+    targets = [torch.from_numpy(scenes.target_image(c.width, c.height, seed=v)).cuda()
+               for v, c in enumerate(cams)]
+    grads_flat = torch.cat([(2.0 * (img - t) / img.numel()).reshape(-1)
+                            for img, t in zip(bf.images(), targets)])
+    gb = bf.backward(grads_flat)
+    sums = None
+    for v, (s, t) in enumerate(zip(singles, targets)):
+        gs = s.backward(2.0 * (s.out_color - t) / s.out_color.numel())
+        grad_close(gb["mean2D"][v].cpu().numpy(), gs["mean2D"].cpu().numpy(), 1e-4, f"mean2D[{v}]")
+        grad_close(gb["conic"][v].cpu().numpy(), gs["conic"].cpu().numpy(), 1e-4, f"conic[{v}]")
+        grad_close(gb["raw"][v].cpu().numpy(), gs["raw"].cpu().numpy(), 1e-4, f"raw[{v}]")
+        if sums is None:
+            sums = {k: gs[k].double().clone() for k in ("means3D", "dc", "shs", "opacities",
+                                                         "scales", "rotations", "cov3D")}
+        else:
+            for k in sums:
+                sums[k] += gs[k].double()
+    for k, ref in sums.items():
+        # geometric terms amplify float reassociation of the raw sums (see
+        # the single-view parity tests); linear ones stay within 1e-4
+        tol = 1e-4 if k in ("dc", "shs", "opacities") else 1e-2
+        got = gb[k].double().cpu().numpy().ravel()
+        ref = ref.cpu().numpy().ravel()
+        if tol == 1e-4:
+            grad_close(got, ref, tol, k)
+        else:
+            assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref), k
+
+
+def test_densify_stats_match_consecutive_views(batch_case):
+    scene, cams, d, bf, singles = batch_case
+    P = bf.P
+    dm = torch.randn(len(singles), P, 4, device="cuda")
+    acc_b, accabs_b, mr_b, den_b = (torch.zeros(P, device="cuda") for _ in range(4))
+    bf.densify_stats(dm, acc_b, accabs_b, mr_b, den_b)
+    acc_s, accabs_s, mr_s, den_s = (torch.zeros(P, device="cuda") for _ in range(4))
+    for v, s in enumerate(singles):
+        s.densify_stats(dm[v].contiguous(), acc_s, accabs_s, mr_s, den_s)
+    for a, b in ((acc_b, acc_s), (accabs_b, accabs_s), (mr_b, mr_s), (den_b, den_s)):
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
+def test_autograd_entry_point(batch_case):
+    scene, cams, d, bf, singles = batch_case
+    params = {k: v.clone().requires_grad_(True) for k, v in d.items()}
+    imgs, radii = rasterize_views([settings_for(c) for c in cams], params["means3D"],
+                                  params["opacities"], dc=params["dc"], shs=params["shs"],
+                                  scales=params["scales"], rotations=params["rotations"])
+    assert radii.shape == (len(cams), d["means3D"].shape[0])
+    for v, s in enumerate(singles):
+        np.testing.assert_array_equal(imgs[v].detach().cpu().numpy(), s.out_color.cpu().numpy())
+    loss = sum((img ** 2).mean() for img in imgs)
+    loss.backward()
+    for k, p in params.items():
+        assert p.grad is not None and torch.isfinite(p.grad).all(), k
+    assert params["dc"].grad.abs().sum() > 0
