@@ -32,12 +32,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="heavy")
-    ap.add_argument("--schedule", default="plain", choices=["plain", "fast"])
+    ap.add_argument("--schedule", default="fast", choices=["plain", "fast"])
     ap.add_argument("--bucket", type=int, default=32)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--compare-plain", type=int, default=1)
     ap.add_argument("--render-loop", type=int, default=1,
                     help="also run the graph-captured 120-frame orbit of the dense workload")
+    ap.add_argument("--multiview", type=int, default=1,
+                    help="also time the 16-view batched training step (autotuned)")
     ap.add_argument("--sched-eval", type=int, default=1,
                     help="also train the tile cost model online and score it against the heuristic")
     return ap.parse_args()
@@ -114,6 +116,9 @@ def main():
 
     graph_fps = graph_render_fps(settings, params, sched, args)
     loop = render_loop(args) if args.render_loop else None
+    torch.cuda.empty_cache()
+    mv = multiview_step(args) if args.multiview else None
+    torch.cuda.empty_cache()
     sched_eval = learned_schedule_eval(args) if args.sched_eval else None
 
     cpu = None
@@ -145,6 +150,7 @@ def main():
         "roofline": roofline,
         "stages": stages,
         "render_loop": loop,
+        "multiview": mv,
         "learned_scheduler": sched_eval,
         "cpu_baseline": cpu,
     }
@@ -325,6 +331,82 @@ def render_loop(args, frames=120):
             "pairs_sampled": Rs, "pair_capacity": R_cap, "bucket_capacity": NB_cap,
             "sort_gbs": round(sort_bytes / (sort_ms / 1e3) / 1e9, 1),
             "sort_frac_hbm": round(sort_bytes / (sort_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def multiview_step(args, views=16, steps=5):
+    """Batched multi-view training step: 2M Gaussians (heavy-tailed cluster
+    distribution), 16 views on a seeded sphere r=3-5 with sizes drawn from
+    {640x480, 960x720, 1280x840, 1920x1080}; forward + backward of all views
+    with one launch per stage (rasterize_views), L2 loss per view. The
+    autotuner picks bucket size and split threshold; the same step is also
+    timed as 16 consecutive single-view steps with the chosen options."""
+    import torch
+    from fastergs_amd import scenes
+    from fastergs_amd._lib import SCHEDULE_FAST
+    from fastergs_amd.autotune import Autotuner
+    from fastergs_amd.multiview import rasterize_views
+    from fastergs_amd.rasterizer import (GaussianRasterizationSettings, RasterOptions,
+                                         rasterize_gaussians)
+    dev = torch.device("cuda:0")
+    scene = scenes.cluster_scene(2_000_000, seed=0)
+    P = scene["means3D"].shape[0]
+    cams = scenes.sphere_cameras(views, seed=0)
+    params = {k: torch.from_numpy(v).to(dev).requires_grad_(True) for k, v in scene.items()}
+    del scene
+    settings = [GaussianRasterizationSettings(
+        image_height=c.height, image_width=c.width, tanfovx=c.tanfovx, tanfovy=c.tanfovy,
+        bg=torch.zeros(3, device=dev), scale_modifier=1.0,
+        viewmatrix=torch.from_numpy(c.viewmatrix).to(dev),
+        projmatrix=torch.from_numpy(c.projmatrix).to(dev), sh_degree=3,
+        campos=torch.from_numpy(c.campos).to(dev), mult=1.0, prefiltered=False, debug=False)
+        for c in cams]
+    targets = [torch.from_numpy(scenes.target_image(c.width, c.height, seed=v)).to(dev)
+               for v, c in enumerate(cams)]
+
+    def batched(opts):
+        for t in params.values():
+            t.grad = None
+        imgs, _ = rasterize_views(settings, params["means3D"], params["opacities"],
+                                  dc=params["dc"], shs=params["shs"], scales=params["scales"],
+                                  rotations=params["rotations"], options=opts)
+        sum(((img - t) ** 2).mean() for img, t in zip(imgs, targets)).backward()
+
+    def sequential(opts):
+        for t in params.values():
+            t.grad = None
+        from fastergs_amd.rasterizer import set_default_options
+        set_default_options(schedule=SCHEDULE_FAST, bucket_size=opts.bucket_size,
+                            split_tiles=opts.split_tiles)
+        for st, t in zip(settings, targets):
+            img, _ = rasterize_gaussians(params["means3D"], params["dc"], params["shs"], None,
+                                         params["opacities"], params["scales"],
+                                         params["rotations"], None, st, None, None, None, None,
+                                         None)
+            ((img - t) ** 2).mean().backward()
+
+    tuner = Autotuner(buckets=(32, 64), splits=(32, 64, 128), reps=2)
+    best, table = tuner.tune(batched)
+    times = {}
+    for name, fn in (("batched", batched), ("sequential", sequential)):
+        fn(best)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn(best)
+        e1.record()
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / steps
+    del params
+    return {"workload": f"multiview: {P} Gaussians, {views} views "
+                        f"({sum(c.width * c.height for c in cams)} pixels, sizes "
+                        f"{sorted(set((c.width, c.height) for c in cams))}), fwd+bwd per step",
+            "ms_per_step_batched": round(times["batched"], 3),
+            "ms_per_step_sequential": round(times["sequential"], 3),
+            "views_per_s": round(views * 1000.0 / times["batched"], 1),
+            "speedup_vs_sequential": round(times["sequential"] / times["batched"], 3),
+            "autotuned": {"bucket_size": best.bucket_size, "split_tiles": best.split_tiles},
+            "autotune_table_ms": table}
 
 
 def learned_schedule_eval(args, train_views=60, eval_views=40, n=None):

@@ -275,7 +275,8 @@ class _RasterizeGaussians(torch.autograd.Function):
                       colors_precomp=colors_precomp, culling=culling, abs_grad=want_abs).forward()
         ctx.frame = frame
         ctx.shapes = [t.shape if torch.is_tensor(t) else None for t in
-                      (means3D, dc, shs, colors_precomp, opacities, scales, rotations)]
+                      (means3D, dc, shs, colors_precomp, opacities, scales, rotations,
+                       cov3Ds_precomp)]
         ctx.stats = (xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom)
         ctx.mark_non_differentiable(frame.radii)
         return frame.out_color, frame.radii
@@ -290,15 +291,17 @@ class _RasterizeGaussians(torch.autograd.Function):
         ctx.frame = None
 
         def shaped(t, shape):
-            if shape is not None and t.numel() == shape.numel() and t.numel() > 0:
+            if shape is None:  # the input was None (not a tensor): no gradient
+                return None
+            if t.numel() == shape.numel() and t.numel() > 0:
                 return t.reshape(shape)
             return t
 
         s = ctx.shapes
         return (shaped(g["means3D"], s[0]), shaped(g["dc"], s[1]), shaped(g["shs"], s[2]),
                 shaped(g["colors"], s[3]), shaped(g["opacities"], s[4]),
-                shaped(g["scales"], s[5]), shaped(g["rotations"], s[6]), g["cov3D"],
-                None, None, None, None, None, None)
+                shaped(g["scales"], s[5]), shaped(g["rotations"], s[6]),
+                None if s[7] is None else g["cov3D"], None, None, None, None, None, None)
 
 
 class GaussianRasterizer(nn.Module):

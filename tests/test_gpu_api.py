@@ -120,3 +120,19 @@ def test_argument_errors():
     with pytest.raises(ValueError):
         ras(means3D=d["means3D"], opacities=d["opacities"], dc=d["dc"], shs=d["shs"],
             cov3D_precomp=torch.zeros(10, 6, device="cuda"))
+
+
+def test_rasterize_gaussians_with_none_inputs(schedule):
+    """Direct calls of rasterize_gaussians may pass None for the unused
+    inputs (colors_precomp, cov3Ds_precomp, culling, statistics); backward
+    must then return no gradient for them."""
+    from fastergs_amd.rasterizer import rasterize_gaussians
+    scene = scenes.uniform_scene(5000, seed=8)
+    cam = scenes.look_at_camera((0.0, 0.3, -4.0), 160, 120)
+    p = _params(scene)
+    img, radii = rasterize_gaussians(p["means3D"], p["dc"], p["shs"], None, p["opacities"],
+                                     p["scales"], p["rotations"], None, settings_for(cam), None,
+                                     None, None, None, None)
+    (img ** 2).mean().backward()
+    for k in ("means3D", "dc", "shs", "opacities", "scales", "rotations"):
+        assert p[k].grad is not None and torch.isfinite(p[k].grad).all(), k

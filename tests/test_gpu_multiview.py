@@ -135,3 +135,34 @@ def test_autograd_entry_point(batch_case):
     for k, p in params.items():
         assert p.grad is not None and torch.isfinite(p.grad).all(), k
     assert params["dc"].grad.abs().sum() > 0
+
+
+def test_autotuner_picks_a_candidate(batch_case, tmp_path):
+    from fastergs_amd.autotune import Autotuner
+    scene, cams, d, bf, singles = batch_case
+    settings = [settings_for(c) for c in cams]
+    calls = []
+
+    def step(opts):
+        calls.append((opts.bucket_size, opts.split_tiles))
+        imgs, _ = rasterize_views(settings, d["means3D"], d["opacities"], dc=d["dc"],
+                                  shs=d["shs"], scales=d["scales"], rotations=d["rotations"],
+                                  options=opts)
+        torch.cuda.synchronize()
+
+    cache = str(tmp_path / "tune.json")
+    tuner = Autotuner(buckets=(16, 32), splits=(32, 128), reps=1, cache_path=cache)
+    best, table = tuner.tune(step, key="k")
+    assert len(table) == 4 and table[0][1] <= table[-1][1]
+    assert (best.bucket_size, best.split_tiles) in {(16, 32), (16, 128), (32, 32), (32, 128)}
+    n = len(calls)
+    again, _ = Autotuner(cache_path=cache).tune(step, key="k")
+    assert len(calls) == n  # served from the cache file
+    assert (again.bucket_size, again.split_tiles) == (best.bucket_size, best.split_tiles)
+    # the knobs never change the image
+    ref = bf.images()
+    for o in (best, again):
+        imgs, _ = rasterize_views(settings, d["means3D"], d["opacities"], dc=d["dc"], shs=d["shs"],
+                                  scales=d["scales"], rotations=d["rotations"], options=o)
+        for a, b in zip(imgs, ref):
+            np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
