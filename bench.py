@@ -40,6 +40,9 @@ def parse():
                     help="also run the graph-captured 120-frame orbit of the dense workload")
     ap.add_argument("--multiview", type=int, default=1,
                     help="also time the 16-view batched training step (autotuned)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run a few training steps and a 256 MiB copy (for rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE runs); prints no JSON")
     ap.add_argument("--sched-eval", type=int, default=1,
                     help="also train the tile cost model online and score it against the heuristic")
     return ap.parse_args()
@@ -84,6 +87,19 @@ def main():
         loss = ((img - target) ** 2).mean()
         loss.backward()
         return img
+
+    if args.traffic_probe:
+        set_default_options(schedule=SCHEDULE_FAST if args.schedule == "fast" else SCHEDULE_PLAIN,
+                            bucket_size=args.bucket)
+        for _ in range(3):
+            step()
+        # calibration: a copy of known size (256 MiB read + 256 MiB written)
+        src = torch.ones(64 << 20, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("traffic probe done", flush=True)
+        return
 
     def timed(schedule: int, steps: int, warmup: int, fwd_only=False):
         set_default_options(schedule=schedule, bucket_size=args.bucket)
@@ -235,8 +251,23 @@ def kernel_roofline(step, params, settings, sched, args, P):
         name, _, ach, peak, bound, per_launch = best
         roof = {"kernel": name, "bound": bound, "achieved": round(ach, 2), "peak": peak,
                 "unit": "GB/s" if bound == "hbm" else "TFLOP/s", "frac": round(ach / peak, 4),
-                "traffic": None, "algorithmic_per_launch": per_launch}
+                "traffic": measured_traffic(name, args), "algorithmic_per_launch": per_launch}
     return stages, roof
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE runs of `bench.py --traffic-probe`,
+    scaled by a copy of known size), for the configuration it was measured
+    on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "pmc_heavy_1M_1280x840_fast.json")
+    if args.workload != "heavy" or args.schedule != "fast" or args.bucket != 32:
+        return None
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return None if k is None else k["traffic_bytes"]
 
 
 def _time_replays(rend, camt, reps):

@@ -73,8 +73,42 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* in,
   }
 }
 
+// Small inputs (tile counts, per-digit histograms of short sorts): one
+// 1024-thread workgroup walks the array in chunks of 8192 with a running
+// carry -- one launch instead of three.
+constexpr int SCAN1_THREADS = 1024;
+constexpr int SCAN1_ITEMS = 8;
+constexpr int64_t SCAN1_MAX = 1 << 17;
+__global__ __launch_bounds__(SCAN1_THREADS) void k_scan_single(const uint32_t* in, uint32_t* out,
+                                                               int64_t n, uint32_t* total) {
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < n; base += SCAN1_THREADS * SCAN1_ITEMS) {
+    const int64_t b = base + (int64_t)threadIdx.x * SCAN1_ITEMS;
+    uint32_t v[SCAN1_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; ++k) {
+      v[k] = b + k < n ? in[b + k] : 0u;
+      sum += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = carry + block_excl_scan<SCAN1_THREADS>(sum, tot);
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; ++k) {
+      if (b + k < n) out[b + k] = run;
+      run += v[k];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
                     hipStream_t s) {
+  if (n <= SCAN1_MAX) {
+    k_scan_single<<<1, SCAN1_THREADS, 0, s>>>(in, out, n, total);
+    return hipGetLastError();
+  }
   const int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
   uint32_t* partial = (uint32_t*)ws;
   if (nb > 0) k_scan_reduce<<<nb, SCAN_THREADS, 0, s>>>(in, n, partial);
