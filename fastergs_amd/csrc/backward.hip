@@ -18,6 +18,13 @@
 
 namespace fgs {
 
+// v_rcp_f32 (1 ulp) for 1 / (1 - alpha) in the reverse walk: alpha <= 0.99,
+// so the argument is in [0.01, 1] and never denormal.
+struct FastRcp {
+  __device__ float operator()(float x) const { return __builtin_amdgcn_rcpf(x); }
+};
+
+
 // Reduce NF raw fields over the wave and add them to raw[12*id + f] with one
 // atomic instruction (lane f carries field f).
 template <int NF>
@@ -58,7 +65,7 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
   const float dp0 = inside ? a.dimg[g] : 0.0f;
   const float dp1 = inside ? a.dimg[HW + g] : 0.0f;
   const float dp2 = inside ? a.dimg[2 * HW + g] : 0.0f;
-  const float bgdot = a.bg[0] * dp0 + a.bg[1] * dp1 + a.bg[2] * dp2;
+  const float tb = Tf * (a.bg[0] * dp0 + a.bg[1] * dp1 + a.bg[2] * dp2);
   const float fx = (float)px, fy = (float)py;
   float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
   for (int top = mc; top > 0; top -= 256) {
@@ -82,8 +89,8 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
       for (int f = 0; f < RAW_N; ++f) acc[f] = 0.0f;
       if (act) {
         const float4 A = sA[j], Bv = sB[j];
-        pix_bwd(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, sC[j], fx, fy, dp0, dp1, dp2, bgdot, Tf,
-                T, r0, r1, r2, acc);
+        pix_bwd(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, sC[j], fx, fy, dp0, dp1, dp2, tb, T,
+                r0, r1, r2, acc);
       }
       wave_flush<NF>(acc, a.raw, sId[j]);
     }
@@ -137,7 +144,7 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
     const float* dimg = a.dimg + 3 * poff;
     const int lt = tile - (int)vr.tile_off;
     const int tx = lt % gx, ty = lt / gx;
-    float T[4], r0[4], r1[4], r2[4], d0[4], d1[4], d2[4], bgd[4], Tf[4], fx[4], fy[4];
+    float T[4], r0[4], r1[4], r2[4], d0[4], d1[4], d2[4], tb[4], fx[4], fy[4];
     bool act[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -150,7 +157,7 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
       const size_t g = poff + gl;                            // pixel in the batch
       act[k] = inside && lb * (uint32_t)a.bucket < (uint32_t)a.contrib[g];
       T[k] = 1.0f; r0[k] = r1[k] = r2[k] = 0.0f; d0[k] = d1[k] = d2[k] = 0.0f;
-      bgd[k] = 0.0f; Tf[k] = 0.0f;
+      tb[k] = 0.0f;
       if (act[k]) {
         const float4 ck = ((const float4*)a.acc)[bq + ly * TILE_W + lx];
         const float4 pc = ((const float4*)a.pixc)[g];
@@ -161,8 +168,7 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
         d0[k] = dimg[gl];
         d1[k] = dimg[HW + gl];
         d2[k] = dimg[2 * HW + gl];
-        bgd[k] = vr.bg[0] * d0[k] + vr.bg[1] * d1[k] + vr.bg[2] * d2[k];
-        Tf[k] = a.tfinal[g];
+        tb[k] = a.tfinal[g] * (vr.bg[0] * d0[k] + vr.bg[1] * d1[k] + vr.bg[2] * d2[k]);
       }
     }
     uint32_t id = a.vals[s + n - 1];
@@ -181,8 +187,8 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (act[q])
-          pix_bwd(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q], fy[q], d0[q], d1[q],
-                  d2[q], bgd[q], Tf[q], T[q], r0[q], r1[q], r2[q], acc);
+          pix_bwd<float, FastRcp>(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q], fy[q],
+                                  d0[q], d1[q], d2[q], tb[q], T[q], r0[q], r1[q], r2[q], acc);
       }
       wave_flush<NF>(acc, a.raw, id);
       id = idn;

@@ -515,14 +515,22 @@ constexpr int RAW_STRIDE = 12;
 // One reverse step of the backward for one pixel (backward.cu:549-632):
 // recovers T before the splat, updates the residual colour and accumulates
 // the raw sums into acc[RAW_N].
-template <typename R>
+// Reciprocal of the per-pixel backward: IEEE division by default; the GPU
+// backward passes a one-instruction approximate reciprocal (the result only
+// enters gradient sums, which are compared within a tolerance).
+struct IeeeRcp {
+  template <typename R>
+  FGS_HD R operator()(R x) const { return R(1) / x; }
+};
+
+template <typename R, typename Rcp = IeeeRcp>
 FGS_HD void pix_bwd(R gx, R gy, R ca, R cb, R cc, R w, R r, R g, R b, R px, R py,
-                    R dp0, R dp1, R dp2, R bg_dot, R T_final, R& T, R& res0, R& res1,
+                    R dp0, R dp1, R dp2, R tb, R& T, R& res0, R& res1,
                     R& res2, R* acc) {
   const R dx = gx - px, dy = gy - py;
   const R p = fma_(cb * dx, dy, fma_(cc * dy, dy, fma_(ca * dx, dx, w)));
   const R a = alpha_of(p);
-  const R inv = R(1) / (R(1) - a);
+  const R inv = Rcp()(R(1) - a);
   T = min_(T * inv, R(1));
   const R wgt = a * T;
   R dLda = (T * r - res0 * inv) * dp0;
@@ -534,7 +542,7 @@ FGS_HD void pix_bwd(R gx, R gy, R ca, R cb, R cc, R w, R r, R g, R b, R px, R py
   acc[RAW_COL0] = fma_(wgt, dp0, acc[RAW_COL0]);
   acc[RAW_COL1] = fma_(wgt, dp1, acc[RAW_COL1]);
   acc[RAW_COL2] = fma_(wgt, dp2, acc[RAW_COL2]);
-  dLda = fma_(-T_final * inv, bg_dot, dLda);
+  dLda = fma_(-tb, inv, dLda);  // tb = T_final * (bg . dL/dpixel), per pixel
   const R dLdp = dLda * (a * R(LN2_F));
   const R mx = dLdp * (R(2) * ca * dx + cb * dy);
   const R my = dLdp * (R(2) * cc * dy + cb * dx);

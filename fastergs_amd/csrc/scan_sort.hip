@@ -73,12 +73,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* in,
   }
 }
 
-// Small inputs (tile counts, per-digit histograms of short sorts): one
-// 1024-thread workgroup walks the array in chunks of 8192 with a running
-// carry -- one launch instead of three.
+// Small inputs (per-tile counts): one 1024-thread workgroup, one launch
+// instead of three. Limited to one chunk of 8192: a single workgroup walking
+// longer inputs chunk by chunk is latency-bound on its one CU and slower
+// than the three-launch scan.
 constexpr int SCAN1_THREADS = 1024;
 constexpr int SCAN1_ITEMS = 8;
-constexpr int64_t SCAN1_MAX = 1 << 17;
+constexpr int64_t SCAN1_MAX = SCAN1_THREADS * SCAN1_ITEMS;
 __global__ __launch_bounds__(SCAN1_THREADS) void k_scan_single(const uint32_t* in, uint32_t* out,
                                                                int64_t n, uint32_t* total) {
   uint32_t carry = 0;

@@ -333,15 +333,15 @@ void backward(State<R>& st, const R* dimg, int accum = 0) {
         const size_t g = (size_t)py * st.W + px;
         const R fx = (R)px, fy = (R)py;
         const R dp0 = dimg[g], dp1 = dimg[HW + g], dp2 = dimg[2 * HW + g];
-        const R bgdot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
         const R Tf = st.tfinal[g];
+        const R tb = Tf * (bg0 * dp0 + bg1 * dp1 + bg2 * dp2);
         const uint32_t cnt = (uint32_t)st.contrib[g];
         auto step = [&](uint32_t k, R& T, R& r0, R& r1, R& r2) {
           const R* sp = &st.splat[9 * (size_t)st.vals[k]];
           R acc[fgs::RAW_N];
           for (int f = 0; f < fgs::RAW_N; ++f) acc[f] = R(0);
           fgs::pix_bwd(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], sp[6], sp[7], sp[8], fx, fy, dp0,
-                       dp1, dp2, bgdot, Tf, T, r0, r1, r2, acc);
+                       dp1, dp2, tb, T, r0, r1, r2, acc);
           double* row = &rows[k * fgs::RAW_N];
           if (accum == 0) {
             for (int f = 0; f < fgs::RAW_N; ++f) row[f] += (double)acc[f];
