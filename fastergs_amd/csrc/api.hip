@@ -278,7 +278,8 @@ size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
   return carve_pair(nullptr, R, NB, (int64_t)width * height, ck).total;
 }
 
-size_t fgs_grad_ws_size(int P) { return align_up((size_t)(P > 0 ? P : 1) * RAW_STRIDE * 4); }
+// raw sums (P x 12) and the split SH pass's mean term (P x 4)
+size_t fgs_grad_ws_size(int P) { return align_up((size_t)(P > 0 ? P : 1) * (RAW_STRIDE + 4) * 4); }
 
 size_t fgs_batch_gauss_ws_size(int P, const fgs_batch* batch, const fgs_options* opt) {
   (void)opt;
@@ -297,7 +298,8 @@ size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t 
 }
 
 size_t fgs_batch_grad_ws_size(int P, int n_views) {
-  return align_up((size_t)(P > 0 ? P : 1) * (size_t)(n_views > 0 ? n_views : 1) * RAW_STRIDE * 4);
+  const size_t p = (size_t)(P > 0 ? P : 1), v = (size_t)(n_views > 0 ? n_views : 1);
+  return align_up(p * v * RAW_STRIDE * 4 + (v == 1 ? p * 4 * 4 : 0));
 }
 
 size_t fgs_sort_ws_size(int64_t n) { return sort_ws_bytes(n); }
@@ -602,6 +604,7 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   q.drot = out->drot;
   q.views = fast ? (const ViewRec*)gw.views : nullptr;
   q.V = b.V;
+  q.sh_dm = b.V == 1 ? raw + (size_t)g->P * RAW_STRIDE : nullptr;
   FGS_TRY(launch_gauss_bwd(q, s), "preprocess backward");
   return FGS_OK;
 }

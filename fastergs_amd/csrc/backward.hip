@@ -357,10 +357,148 @@ __global__ __launch_bounds__(GBWDV_THREADS) void k_gauss_bwd_views(GaussBwdArgs 
       a.dsh[(size_t)RM * g0 + k] = shacc[k];
 }
 
+// Split adjoint of one view, pass 1: the SH part. One wave64 per workgroup,
+// one Gaussian per lane; the 64 coefficient rows (64 x 3M floats, contiguous)
+// are staged in LDS with float4 loads, each lane reads its row (stride 3M
+// words: odd for M = 15, so conflict-free), overwrites it with the
+// coefficient gradients, and the rows are stored back with float4 stores.
+// Lane i also writes ddc and the mean-gradient term sh_dm of its Gaussian.
+constexpr int SHB_THREADS = WAVE;
+__global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
+  extern __shared__ float rows[];  // SHB_THREADS x 3M
+  const int lane = threadIdx.x;
+  const int g0 = blockIdx.x * SHB_THREADS;
+  const int i = g0 + lane;
+  const int M = a.cam.max_coeffs;
+  const int RM = 3 * M;
+  const int nb = min(SHB_THREADS, a.P - g0);
+  const float* src = a.G.shs + (size_t)RM * g0;
+  float* dst = a.dsh + (size_t)RM * g0;
+  // full groups of 64 rows start 16-byte aligned when the arrays do
+  const bool vec = M > 0 && nb == SHB_THREADS && RM * SHB_THREADS % 4 == 0 &&
+                   (((uintptr_t)a.G.shs | (uintptr_t)a.dsh) & 15) == 0;
+  if (vec) {
+    for (int j = lane; j < RM * SHB_THREADS / 4; j += SHB_THREADS)
+      ((float4*)rows)[j] = ((const float4*)src)[j];
+  } else {
+    for (int j = lane; j < RM * nb; j += SHB_THREADS) rows[j] = src[j];
+  }
+  __syncthreads();
+  if (i < a.P) {
+    float* row = rows + RM * lane;
+    const bool vis = a.radii[i] > 0;
+    float dm[3] = {0.0f, 0.0f, 0.0f}, ddc[3] = {0.0f, 0.0f, 0.0f};
+    if (vis) {
+      const float* pm = a.G.means + 3 * (size_t)i;
+      const float ox = pm[0] - a.cam.campos[0], oy = pm[1] - a.cam.campos[1],
+                  oz = pm[2] - a.cam.campos[2];
+      const float col[3] = {a.raw[RAW_STRIDE * (size_t)i + RAW_COL0],
+                            a.raw[RAW_STRIDE * (size_t)i + RAW_COL1],
+                            a.raw[RAW_STRIDE * (size_t)i + RAW_COL2]};
+      float dRGB[3];
+      sh_drgb((int)f2u(a.splat[12 * (size_t)i + 10]), col, dRGB);
+      const float len = sqrt_(ox * ox + oy * oy + oz * oz);
+      const float x = ox / len, y = oy / len, z = oz / len;
+      float bas[16], db[16][3];
+      sh_basis(x, y, z, bas);
+      sh_dbasis(x, y, z, db);
+      const int n = sh_count(a.cam.deg);
+      float ddir[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int k = 1; k < 16; ++k) {
+        if (k < n) {
+          const float cdot = row[3 * (k - 1)] * dRGB[0] + row[3 * (k - 1) + 1] * dRGB[1] +
+                             row[3 * (k - 1) + 2] * dRGB[2];
+          ddir[0] += db[k][0] * cdot;
+          ddir[1] += db[k][1] * cdot;
+          ddir[2] += db[k][2] * cdot;
+          for (int ch = 0; ch < 3; ++ch) row[3 * (k - 1) + ch] = bas[k] * dRGB[ch];
+        }
+      }
+      for (int ch = 0; ch < 3; ++ch) ddc[ch] = bas[0] * dRGB[ch];
+      for (int j = 3 * (n - 1); j < RM; ++j) row[j] = 0.0f;
+      sh_dmean(ox, oy, oz, ddir, dm);
+    } else {
+      for (int j = 0; j < RM; ++j) row[j] = 0.0f;
+    }
+    ((float4*)a.sh_dm)[i] = make_float4(dm[0], dm[1], dm[2], 0.0f);
+    for (int ch = 0; ch < 3; ++ch) a.ddc[3 * (size_t)i + ch] = ddc[ch];
+  }
+  __syncthreads();
+  if (vec) {
+    for (int j = lane; j < RM * SHB_THREADS / 4; j += SHB_THREADS)
+      ((float4*)dst)[j] = ((const float4*)rows)[j];
+  } else {
+    for (int j = lane; j < RM * nb; j += SHB_THREADS) dst[j] = rows[j];
+  }
+}
+
+// Split adjoint of one view, pass 2: everything but the SH coefficients,
+// one thread per Gaussian without LDS (full occupancy), adding pass 1's
+// mean-gradient term.
+__global__ __launch_bounds__(256) void k_gauss_bwd_geom(GaussBwdArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.P) return;
+  const float4* r4 = (const float4*)(a.raw + (size_t)RAW_STRIDE * i);
+  float raw[RAW_STRIDE];
+  const float4 q0 = r4[0], q1 = r4[1], q2 = r4[2];
+  raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
+  raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
+  raw[8] = q2.x; raw[9] = q2.y; raw[10] = q2.z; raw[11] = q2.w;
+  Grad2D<float> g2;
+  finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
+  if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
+  ((float4*)a.dmean2D)[i] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
+  ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
+  a.dopacity[i] = g2.opacity;
+  a.dcolor[3 * (size_t)i] = g2.color[0];
+  a.dcolor[3 * (size_t)i + 1] = g2.color[1];
+  a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+  float tm[3], tc[6], ts[3], tr[4], tdc[3];
+  GaussGrad<float> out;
+  out.dmean3d = tm;
+  out.dcov3d = tc;
+  out.dscale = ts;
+  out.drot = tr;
+  out.ddc = tdc;
+  out.dsh = nullptr;
+  if (a.radii[i] > 0) {
+    const float4 sd = ((const float4*)a.sh_dm)[i];
+    const float dms[3] = {sd.x, sd.y, sd.z};
+    const int clamp = (int)f2u(a.splat[12 * (size_t)i + 10]);
+    gauss_bwd(i, a.G, a.cam, clamp, g2, out, (const float*)nullptr, dms);
+  } else {
+    for (int k = 0; k < 3; ++k) tm[k] = ts[k] = 0.0f;
+    for (int k = 0; k < 6; ++k) tc[k] = 0.0f;
+    for (int k = 0; k < 4; ++k) tr[k] = 0.0f;
+  }
+  for (int k = 0; k < 3; ++k) {
+    a.dmean3D[3 * (size_t)i + k] = tm[k];
+    a.dscale[3 * (size_t)i + k] = ts[k];
+  }
+  for (int k = 0; k < 6; ++k) a.dcov3D[6 * (size_t)i + k] = tc[k];
+  ((float4*)a.drot)[i] = make_float4(tr[0], tr[1], tr[2], tr[3]);
+}
+
 hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
   if (a.P <= 0) return hipSuccess;
   const int M = a.cam.max_coeffs;
   const bool sh = a.dsh != nullptr && M > 0;
+  const bool colour_sh = a.G.colors == nullptr && a.G.dc != nullptr;
+  if (a.sh_dm != nullptr && (a.views == nullptr || a.V <= 1) && M <= 15) {
+    if (colour_sh) {
+      const size_t shmem = (size_t)SHB_THREADS * 3 * M * sizeof(float);
+      FGS_TIMED("sh_bwd", s,
+                (k_sh_bwd<<<(unsigned)((a.P + SHB_THREADS - 1) / SHB_THREADS), SHB_THREADS,
+                             shmem, s>>>(a)));
+    } else {  // precomputed colours: no SH term, zero dc gradient
+      hipError_t e = fill_f32(a.sh_dm, 0.0f, (int64_t)a.P * 4, s);
+      if (e == hipSuccess && a.ddc) e = fill_f32(a.ddc, 0.0f, (int64_t)a.P * 3, s);
+      if (e != hipSuccess) return e;
+    }
+    FGS_TIMED("gauss_bwd_geom", s, (k_gauss_bwd_geom<<<(a.P + 255) / 256, 256, 0, s>>>(a)));
+    return hipGetLastError();
+  }
   if (a.views != nullptr && a.V > 1) {
     const size_t shmem = sh ? (size_t)3 * GBWDV_THREADS * 3 * M * sizeof(float) : 0;
     FGS_TIMED("gauss_bwd_views", s,

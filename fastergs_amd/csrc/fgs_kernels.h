@@ -219,6 +219,7 @@ struct GaussBwdArgs {
   float *dmean2D, *dconic, *dopacity, *dcolor, *dmean3D, *dcov3D, *ddc, *dsh, *dscale, *drot;
   const ViewRec* views;
   int V;
+  float* sh_dm;  // one view: P x 4 scratch for the split SH pass (nullptr: fused kernel)
 };
 hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s);
 // Densification statistics of V views (radii / dmean2D rows v * P + i) added

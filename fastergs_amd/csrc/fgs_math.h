@@ -599,15 +599,61 @@ struct GaussGrad {
   R* dsh;      // max_coeffs x 3
 };
 
+// Derivatives of the 16 real SH basis functions w.r.t. the unit direction
+// (x, y, z) (backward.cu:121-245; rows of unused degrees are ignored).
+template <typename R>
+FGS_HD void sh_dbasis(R x, R y, R z, R db[16][3]) {
+  const R xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+  const R t[16][3] = {
+      {R(0), R(0), R(0)},
+      {R(0), -R(SH_C1), R(0)},
+      {R(0), R(0), R(SH_C1)},
+      {-R(SH_C1), R(0), R(0)},
+      {R(SH_C2_0) * y, R(SH_C2_0) * x, R(0)},
+      {R(0), R(SH_C2_1) * z, R(SH_C2_1) * y},
+      {R(-2) * R(SH_C2_2) * x, R(-2) * R(SH_C2_2) * y, R(4) * R(SH_C2_2) * z},
+      {R(SH_C2_3) * z, R(0), R(SH_C2_3) * x},
+      {R(2) * R(SH_C2_4) * x, R(-2) * R(SH_C2_4) * y, R(0)},
+      {R(6) * R(SH_C3_0) * xy, R(SH_C3_0) * (R(3) * xx - R(3) * yy), R(0)},
+      {R(SH_C3_1) * yz, R(SH_C3_1) * xz, R(SH_C3_1) * xy},
+      {R(-2) * R(SH_C3_2) * xy, R(SH_C3_2) * (R(4) * zz - xx - R(3) * yy), R(8) * R(SH_C3_2) * yz},
+      {R(-6) * R(SH_C3_3) * xz, R(-6) * R(SH_C3_3) * yz,
+       R(SH_C3_3) * (R(6) * zz - R(3) * xx - R(3) * yy)},
+      {R(SH_C3_4) * (R(4) * zz - R(3) * xx - yy), R(-2) * R(SH_C3_4) * xy, R(8) * R(SH_C3_4) * xz},
+      {R(2) * R(SH_C3_5) * xz, R(-2) * R(SH_C3_5) * yz, R(SH_C3_5) * (xx - yy)},
+      {R(SH_C3_6) * (R(3) * xx - R(3) * yy), R(-6) * R(SH_C3_6) * xy, R(0)}};
+  for (int k = 0; k < 16; ++k)
+    for (int c = 0; c < 3; ++c) db[k][c] = t[k][c];
+}
+
+// Colour gradient seen by the SH coefficients: zero on clamped channels.
+template <typename R>
+FGS_HD void sh_drgb(int clamp, const R color[3], R dRGB[3]) {
+  for (int ch = 0; ch < 3; ++ch) dRGB[ch] = (clamp >> ch) & 1 ? R(0) : color[ch];
+}
+
+// Mean gradient through the normalisation dir = o / |o| (backward.cu:88-103).
+template <typename R>
+FGS_HD void sh_dmean(R ox, R oy, R oz, const R ddir[3], R dm[3]) {
+  const R s2 = ox * ox + oy * oy + oz * oz;
+  const R inv32 = R(1) / sqrt_(s2 * s2 * s2);
+  dm[0] = ((s2 - ox * ox) * ddir[0] - oy * ox * ddir[1] - oz * ox * ddir[2]) * inv32;
+  dm[1] = (-ox * oy * ddir[0] + (s2 - oy * oy) * ddir[1] - oz * oy * ddir[2]) * inv32;
+  dm[2] = (-ox * oz * ddir[0] - oy * oz * ddir[1] + (s2 - oz * oz) * ddir[2]) * inv32;
+}
+
 // Per-Gaussian adjoint of preprocess for a Gaussian with radius > 0:
 // cov2D path (backward.cu:250-403), projection path (:504-526), SH path
 // (:121-245) and cov3D path (:407-483). Like the reference, the scale
 // gradient is taken w.r.t. s = scale_modifier * scale (it equals the
 // gradient w.r.t. scale when the modifier is 1), and the rotation gradient
 // is w.r.t. the unnormalised quaternion.
+// With sh_dm given, the SH path is left out (a separate pass computed its
+// coefficient gradients) and sh_dm is added as its mean-gradient term.
 template <typename R>
 FGS_HD void gauss_bwd(int i, const GaussRef<R>& G, const CamRef<R>& cam, int clamp,
-                      const Grad2D<R>& g2, const GaussGrad<R>& out, const R* sh_row = nullptr) {
+                      const Grad2D<R>& g2, const GaussGrad<R>& out, const R* sh_row = nullptr,
+                      const R* sh_dm = nullptr) {
   const R* pm = G.means + 3 * i;
   R t[3];
   xform_point(cam.view, pm[0], pm[1], pm[2], t);
@@ -678,36 +724,20 @@ FGS_HD void gauss_bwd(int i, const GaussRef<R>& G, const CamRef<R>& cam, int cla
   dm[1] += (P[4] * mw - P[7] * mul1) * gx + (P[5] * mw - P[7] * mul2) * gy;
   dm[2] += (P[8] * mw - P[11] * mul1) * gx + (P[9] * mw - P[11] * mul2) * gy;
 
-  // SH path
-  if (G.shs != nullptr && G.colors == nullptr) {
+  // SH path (dc alone at degree 0; rest coefficients for k < sh_count(deg))
+  if (sh_dm != nullptr) {
+    for (int k = 0; k < 3; ++k) dm[k] += sh_dm[k];
+  } else if (G.colors == nullptr && G.dc != nullptr) {
     const R ox = pm[0] - cam.campos[0], oy = pm[1] - cam.campos[1], oz = pm[2] - cam.campos[2];
     const R len = sqrt_(ox * ox + oy * oy + oz * oz);
     const R x = ox / len, y = oy / len, z = oz / len;
     R bas[16];
     sh_basis(x, y, z, bas);
     R dRGB[3];
-    for (int ch = 0; ch < 3; ++ch) dRGB[ch] = (clamp >> ch) & 1 ? R(0) : g2.color[ch];
+    sh_drgb(clamp, g2.color, dRGB);
     const int n = sh_count(cam.deg);
-    // d basis / d(x,y,z) (rows for unused degrees are skipped by k < n)
-    const R xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-    const R db[16][3] = {
-        {R(0), R(0), R(0)},
-        {R(0), -R(SH_C1), R(0)},
-        {R(0), R(0), R(SH_C1)},
-        {-R(SH_C1), R(0), R(0)},
-        {R(SH_C2_0) * y, R(SH_C2_0) * x, R(0)},
-        {R(0), R(SH_C2_1) * z, R(SH_C2_1) * y},
-        {R(-2) * R(SH_C2_2) * x, R(-2) * R(SH_C2_2) * y, R(4) * R(SH_C2_2) * z},
-        {R(SH_C2_3) * z, R(0), R(SH_C2_3) * x},
-        {R(2) * R(SH_C2_4) * x, R(-2) * R(SH_C2_4) * y, R(0)},
-        {R(6) * R(SH_C3_0) * xy, R(SH_C3_0) * (R(3) * xx - R(3) * yy), R(0)},
-        {R(SH_C3_1) * yz, R(SH_C3_1) * xz, R(SH_C3_1) * xy},
-        {R(-2) * R(SH_C3_2) * xy, R(SH_C3_2) * (R(4) * zz - xx - R(3) * yy), R(8) * R(SH_C3_2) * yz},
-        {R(-6) * R(SH_C3_3) * xz, R(-6) * R(SH_C3_3) * yz,
-         R(SH_C3_3) * (R(6) * zz - R(3) * xx - R(3) * yy)},
-        {R(SH_C3_4) * (R(4) * zz - R(3) * xx - yy), R(-2) * R(SH_C3_4) * xy, R(8) * R(SH_C3_4) * xz},
-        {R(2) * R(SH_C3_5) * xz, R(-2) * R(SH_C3_5) * yz, R(SH_C3_5) * (xx - yy)},
-        {R(SH_C3_6) * (R(3) * xx - R(3) * yy), R(-6) * R(SH_C3_6) * xy, R(0)}};
+    R db[16][3];
+    sh_dbasis(x, y, z, db);
     const R* sh = sh_row ? sh_row : G.shs + (size_t)i * cam.max_coeffs * 3;
     R ddir[3] = {R(0), R(0), R(0)};
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -731,12 +761,9 @@ FGS_HD void gauss_bwd(int i, const GaussRef<R>& G, const CamRef<R>& cam, int cla
     for (int k = 1; k < 16; ++k)
       if (k < n)
         for (int ch = 0; ch < 3; ++ch) out.dsh[3 * (k - 1) + ch] = bas[k] * dRGB[ch];
-    // through the normalisation dir = o/|o| (backward.cu:88-103)
-    const R s2 = ox * ox + oy * oy + oz * oz;
-    const R inv32 = R(1) / sqrt_(s2 * s2 * s2);
-    dm[0] += ((s2 - ox * ox) * ddir[0] - oy * ox * ddir[1] - oz * ox * ddir[2]) * inv32;
-    dm[1] += (-ox * oy * ddir[0] + (s2 - oy * oy) * ddir[1] - oz * oy * ddir[2]) * inv32;
-    dm[2] += (-ox * oz * ddir[0] - oy * oz * ddir[1] + (s2 - oz * oz) * ddir[2]) * inv32;
+    R dms[3];
+    sh_dmean(ox, oy, oz, ddir, dms);
+    for (int k = 0; k < 3; ++k) dm[k] += dms[k];
   }
   for (int k = 0; k < 3; ++k) out.dmean3d[k] = dm[k];
 

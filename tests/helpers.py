@@ -71,11 +71,13 @@ def check_backward(got: dict, orc, dimg, abs_grad: bool = True):
         grad_close(raw[:, f], raw_ref[:, f], name=f"raw[{f}]")
     if not abs_grad:
         assert (raw[:, 9:] == 0).all()
+    # the oracle's per-Gaussian adjoint applied to the GPU's own raw sums: the
+    # GPU reduces the SH direction gradient over 16 lanes (a tree) where the
+    # oracle adds the 15 terms in order, so agreement is to float rounding
     mine = orc.gauss_grads_from_raw(raw)
     for k in WELL_CONDITIONED + GEOMETRIC + ("mean2D", "conic"):
         a = got[k].detach().cpu().numpy().reshape(mine[k].shape)
-        scale = float(np.sqrt(np.mean(mine[k].astype(np.float64) ** 2))) + 1e-30
-        assert np.abs(a - mine[k]).max() <= 1e-6 * scale, k
+        grad_close(a, mine[k], rtol=1e-5, name=f"{k} (adjoint of the GPU raw sums)")
     for k in WELL_CONDITIONED:
         grad_close(got[k].detach().cpu().numpy(), ref[k], name=k)
     for k in GEOMETRIC:

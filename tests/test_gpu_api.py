@@ -136,3 +136,25 @@ def test_rasterize_gaussians_with_none_inputs(schedule):
     (img ** 2).mean().backward()
     for k in ("means3D", "dc", "shs", "opacities", "scales", "rotations"):
         assert p[k].grad is not None and torch.isfinite(p[k].grad).all(), k
+
+
+def test_degree0_dc_gradient(schedule):
+    """SH degree 0 without rest coefficients: colour = C0 dc + 1/2 (clamped
+    at 0), so dL/ddc = C0 dL/dcolour wherever the colour was not clamped."""
+    from fastergs_amd.rasterizer import Frame, get_default_options
+    from tests.helpers import settings_for as sf
+    scene = scenes.uniform_scene(4000, seed=9)
+    cam = scenes.look_at_camera((0.0, 0.0, -4.0), 160, 128)
+    d = to_dev(scene)
+    fr = Frame(sf(cam, sh_degree=0), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+               dc=d["dc"], shs=None, options=get_default_options()).forward()
+    g = fr.backward(torch.ones_like(fr.out_color))
+    C0 = 0.28209479177387814
+    col = C0 * d["dc"].reshape(-1, 3) + 0.5
+    vis = (fr.radii > 0)[:, None] & (col > 0)
+    ddc = g["dc"].reshape(-1, 3)
+    dcol = g["colors"].reshape(-1, 3)
+    assert vis.sum() > 100
+    np.testing.assert_allclose(ddc[vis].cpu().numpy(), (C0 * dcol[vis]).cpu().numpy(),
+                               rtol=1e-6, atol=1e-12)
+    assert (ddc[~vis] == 0).all()
