@@ -75,9 +75,10 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
     if (t < n) {
       const uint32_t id = a.vals[start + lo + t];
       const float4* s = (const float4*)(a.splat + 12 * (size_t)id);
+      const float4 s1 = s[1], s2 = s[2];
       sA[t] = s[0];
-      sB[t] = s[1];
-      sC[t] = s[2].x;
+      sB[t] = make_float4(s1.x, s1.y, s2.x, s2.y);  // cc, log2 op, r, g
+      sC[t] = s2.z;
       sId[t] = id;
     }
     __syncthreads();
@@ -172,15 +173,22 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
       }
     }
     uint32_t id = a.vals[s + n - 1];
+    // record: (x, y, ca, cb) (cc, log2 op, depth, -) (r, g, b, clamp); the
+    // walk uses A = first, B = (cc, log2 op, r, g), C = b
     const float4* rp = (const float4*)(a.splat + 12 * (size_t)id);
-    float4 A0 = rp[0], B0 = rp[1];
-    float C0 = rp[2].x;
+    float4 A0 = rp[0], B0;
+    {
+      const float4 b1 = rp[1], c1 = rp[2];
+      B0 = make_float4(b1.x, b1.y, c1.x, c1.y);
+    }
+    float C0 = rp[2].z;
     for (int k = n - 1; k >= 0; --k) {
       // prefetch the previous splat of the bucket
       const uint32_t idn = k > 0 ? a.vals[s + k - 1] : id;
       const float4* rn = (const float4*)(a.splat + 12 * (size_t)idn);
-      const float4 A1 = rn[0], B1 = rn[1];
-      const float C1 = rn[2].x;
+      const float4 A1 = rn[0], n1 = rn[1], n2 = rn[2];
+      const float4 B1 = make_float4(n1.x, n1.y, n2.x, n2.y);
+      const float C1 = rn[2].z;
       float acc[RAW_N];
 #pragma unroll
       for (int f = 0; f < RAW_N; ++f) acc[f] = 0.0f;
@@ -253,7 +261,7 @@ __global__ __launch_bounds__(GBWD_THREADS) void k_gauss_bwd(GaussBwdArgs a) {
     out.dsh = sh ? row : nullptr;
     int written = 0;  // leading SH entries holding gradients
     if (a.radii[i] > 0) {
-      const int clamp = (int)f2u(a.splat[12 * (size_t)i + 10]);
+      const int clamp = (int)f2u(a.splat[12 * (size_t)i + 11]);
       gauss_bwd(i, a.G, a.cam, clamp, g2, out, sh ? row : nullptr);
       if (a.G.shs != nullptr && a.G.colors == nullptr) written = 3 * (sh_count(a.cam.deg) - 1);
     } else {
@@ -328,7 +336,7 @@ __global__ __launch_bounds__(GBWDV_THREADS) void k_gauss_bwd_views(GaussBwdArgs 
         out.drot = tr;
         out.ddc = tdc;
         out.dsh = sh ? row_tmp : nullptr;
-        const int clamp = (int)f2u(a.splat[12 * ri + 10]);
+        const int clamp = (int)f2u(a.splat[12 * ri + 11]);
         gauss_bwd(i, a.G, view_cam(a.cam, vr), clamp, g2, out, sh ? row_in : nullptr);
         for (int k = 0; k < 3; ++k) sm[k] += tm[k];
         for (int k = 0; k < 6; ++k) sc[k] += tc[k];
@@ -396,7 +404,7 @@ __global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
                             a.raw[RAW_STRIDE * (size_t)i + RAW_COL1],
                             a.raw[RAW_STRIDE * (size_t)i + RAW_COL2]};
       float dRGB[3];
-      sh_drgb((int)f2u(a.splat[12 * (size_t)i + 10]), col, dRGB);
+      sh_drgb((int)f2u(a.splat[12 * (size_t)i + 11]), col, dRGB);
       const float len = sqrt_(ox * ox + oy * oy + oz * oz);
       const float x = ox / len, y = oy / len, z = oz / len;
       float bas[16], db[16][3];
@@ -465,7 +473,7 @@ __global__ __launch_bounds__(256) void k_gauss_bwd_geom(GaussBwdArgs a) {
   if (a.radii[i] > 0) {
     const float4 sd = ((const float4*)a.sh_dm)[i];
     const float dms[3] = {sd.x, sd.y, sd.z};
-    const int clamp = (int)f2u(a.splat[12 * (size_t)i + 10]);
+    const int clamp = (int)f2u(a.splat[12 * (size_t)i + 11]);
     gauss_bwd(i, a.G, a.cam, clamp, g2, out, (const float*)nullptr, dms);
   } else {
     for (int k = 0; k < 3; ++k) tm[k] = ts[k] = 0.0f;

@@ -3,7 +3,7 @@
 // HBM layout (all arrays 256-byte aligned, structure of arrays per stage):
 //   gauss workspace (depends on P and the tile grid T):
 //     geom   P x 32 B  (x, y, conic a, b | conic c, power, rect min, rect max)
-//     splat  P x 48 B  (x, y, ca, cb | cc, log2 op, r, g | b, depth, clamp, -)
+//     splat  P x 48 B  (x, y, ca, cb | cc, log2 op, depth, - | r, g, b, clamp)
 //     dkey / pcount / poff   P x 4 B each (+1 for the offsets' total)
 //     depth-sort keys/values P x (8+8+4+4) B (fast schedule)
 //     tile histogram, ranges, bucket offsets, per-tile max contributor  T x 20 B

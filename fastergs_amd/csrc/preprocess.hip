@@ -7,7 +7,8 @@
 //
 // Records written per Gaussian (structure of arrays, 16-byte aligned rows):
 //   geom  8 floats: x, y, conic a, b, c, power, rect min, rect max (packed u16x2)
-//   splat 12 floats: x, y, ca, cb, cc, log2 op, r, g, b, depth, clamp bits, 0
+//   splat 12 floats: x, y, ca, cb | cc, log2 op, depth, 0 | r, g, b, clamp bits
+//   (the colour is one aligned float4, written by the colour pass in one store)
 // The splat row is what compositing and its backward read (48 B per
 // Gaussian; the reference reads fields 0-9 of a 128 B row).
 //
@@ -37,8 +38,8 @@ __device__ __forceinline__ void store_records(size_t i, const Proj<float>& o, fl
   log2_conic(o.conic, o.log2op, ca, cb, cc, w);
   float4* s4 = (float4*)(splat + 12 * (size_t)i);
   s4[0] = make_float4(o.x, o.y, ca, cb);
-  s4[1] = make_float4(cc, w, o.rgb[0], o.rgb[1]);
-  s4[2] = make_float4(o.rgb[2], o.depth, u2f((uint32_t)o.clamp), 0.0f);
+  s4[1] = make_float4(cc, w, o.depth, 0.0f);
+  s4[2] = make_float4(o.rgb[0], o.rgb[1], o.rgb[2], u2f((uint32_t)o.clamp));
 }
 
 __device__ __forceinline__ void store_invalid(size_t i, float* geom, float* splat) {
@@ -46,6 +47,7 @@ __device__ __forceinline__ void store_invalid(size_t i, float* geom, float* spla
   g4[0] = make_float4(0.f, 0.f, 0.f, 0.f);
   g4[1] = make_float4(0.f, 0.f, 0.f, 0.f);
   float4* s4 = (float4*)(splat + 12 * (size_t)i);
+  s4[1] = make_float4(0.f, 0.f, 0.f, 0.f);
   s4[2] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -278,11 +280,7 @@ __global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
     int clamp;
     sh_color_rows(pm[0] - cp[0], pm[1] - cp[1], pm[2] - cp[2], sdc + 3 * lane,
                   ssh + 3 * M * lane, a.cam.deg, rgb, clamp);
-    float* sp = a.splat + 12 * ri;
-    sp[6] = rgb[0];
-    sp[7] = rgb[1];
-    sp[8] = rgb[2];
-    sp[10] = u2f((uint32_t)clamp);
+    ((float4*)(a.splat + 12 * ri))[2] = make_float4(rgb[0], rgb[1], rgb[2], u2f((uint32_t)clamp));
   }
 }
 

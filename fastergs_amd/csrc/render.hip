@@ -199,9 +199,10 @@ __global__ __launch_bounds__(256) void k_composite_plain(CompArgs a) {
     const uint32_t k = base + t;
     if (k < end) {
       const float4* s = (const float4*)(a.splat + 12 * (size_t)a.vals[k]);
+      const float4 s1 = s[1], s2 = s[2];
       sA[t] = s[0];
-      sB[t] = s[1];
-      sC[t] = s[2].x;
+      sB[t] = make_float4(s1.x, s1.y, s2.x, s2.y);  // cc, log2 op, r, g
+      sC[t] = s2.z;
     }
     __syncthreads();
     const int n = (int)min(256u, end - base);
@@ -352,9 +353,10 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
     const int n = bl < nb ? (int)min((uint32_t)B, end - s) : 0;
     for (int k = lane; k < n; k += WAVE) {
       const float4* r = (const float4*)(a.splat + 12 * (size_t)a.vals[s + k]);
+      const float4 r1 = r[1], r2 = r[2];
       sA[w][k] = r[0];
-      sB[w][k] = r[1];
-      sC[w][k] = r[2].x;
+      sB[w][k] = make_float4(r1.x, r1.y, r2.x, r2.y);  // cc, log2 op, r, g
+      sC[w][k] = r2.z;
     }
     __syncthreads();
     if (n > 0) {

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(FEAT_SAMPLE) void k_tile_features(
     lmax = la;
     const float* sp = splat + 12 * (size_t)g;
     const float op = exp2f(sp[5]);
-    const float d = sp[9];
+    const float d = sp[6];
     dmin = d;
     dmax = d;
     atomicAdd(&hist[min((int)(op * 16.0f), 15)], 1u);

@@ -109,7 +109,7 @@ typedef struct fgs_options {
  * the chosen schedule are NULL. */
 typedef struct fgs_frame_view {
   float* geom;            /* P x 8: x, y, conic a, b, c, power, rect min (x | y<<16), rect max */
-  float* splat;           /* P x 12: x, y, ca, cb, cc, log2 op, r, g, b, depth, clamp bits, 0 */
+  float* splat;           /* P x 12: x, y, ca, cb, cc, log2 op, depth, 0, r, g, b, clamp bits */
   uint32_t* pair_count;   /* P */
   uint32_t* tile_count;   /* T */
   uint32_t* ranges;       /* T x 2 */

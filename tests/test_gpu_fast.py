@@ -49,7 +49,7 @@ def rebuilt_keys(fr, v):
     vals = v["sorted_vals"].cpu().numpy().view(np.uint32)
     rng = v["ranges"].cpu().numpy().view(np.uint32).reshape(-1, 2)
     tiles = np.repeat(np.arange(rng.shape[0], dtype=np.uint64), (rng[:, 1] - rng[:, 0]))
-    depth = v["splat"].cpu().numpy().reshape(-1, 12)[:, 9].view(np.uint32).astype(np.uint64)
+    depth = v["splat"].cpu().numpy().reshape(-1, 12)[:, 6].view(np.uint32).astype(np.uint64)
     return (tiles << np.uint64(32)) | depth[vals]
 
 

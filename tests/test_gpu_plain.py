@@ -65,15 +65,15 @@ def test_records_bit_exact(case):
     np.testing.assert_array_equal(geom[:, 1].view(np.uint32), p[:, 7].view(np.uint32))  # y
     np.testing.assert_array_equal(geom[:, 2:5].view(np.uint32), p[:, 9:12].view(np.uint32))
     np.testing.assert_array_equal(geom[:, 5].view(np.uint32), p[:, 12].view(np.uint32))
-    np.testing.assert_array_equal(spl[:, 9].view(np.uint32), p[:, 8].view(np.uint32))  # depth
-    np.testing.assert_array_equal(spl[:, 6:9].view(np.uint32), p[:, 13:16].view(np.uint32))
+    np.testing.assert_array_equal(spl[:, 6].view(np.uint32), p[:, 8].view(np.uint32))  # depth
+    np.testing.assert_array_equal(spl[:, 8:11].view(np.uint32), p[:, 13:16].view(np.uint32))
     rmin = geom[:, 6].view(np.uint32)
     rmax = geom[:, 7].view(np.uint32)
     np.testing.assert_array_equal(rmin & 0xFFFF, p[:, 1].astype(np.uint32))
     np.testing.assert_array_equal(rmin >> 16, p[:, 2].astype(np.uint32))
     np.testing.assert_array_equal(rmax & 0xFFFF, p[:, 3].astype(np.uint32))
     np.testing.assert_array_equal(rmax >> 16, p[:, 4].astype(np.uint32))
-    np.testing.assert_array_equal(spl[:, 10].view(np.uint32), p[:, 5].astype(np.uint32))
+    np.testing.assert_array_equal(spl[:, 11].view(np.uint32), p[:, 5].astype(np.uint32))
 
 
 def test_sorted_pairs_and_ranges(case):

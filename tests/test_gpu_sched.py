@@ -113,7 +113,7 @@ def test_tile_features(scene_cam):
             b = srt[int(np.ceil(q * len(ids))) - 1]
             # a sample exactly on a bin edge may land in either neighbour
             assert abs(f[3 + k] - (b + 0.5) / 16) <= 1.0 / 16 + 1e-6
-        dep = splat[ids, 9].astype(np.float64)
+        dep = splat[ids, 6].astype(np.float64)
         assert abs(f[7] - np.log2(dep.max() / dep.min())) < 1e-4
         checked += 1
     assert checked > 50
