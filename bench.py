@@ -131,6 +131,7 @@ def main():
         plain_ms, _ = timed(SCHEDULE_PLAIN, max(5, args.steps // 2), 2)
 
     graph_fps = graph_render_fps(settings, params, sched, args)
+    adam = sparse_adam_step(rast, params)
     loop = render_loop(args) if args.render_loop else None
     torch.cuda.empty_cache()
     mv = multiview_step(args) if args.multiview else None
@@ -165,6 +166,7 @@ def main():
         "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms, 3),
         "roofline": roofline,
         "stages": stages,
+        "sparse_adam": adam,
         "render_loop": loop,
         "multiview": mv,
         "learned_scheduler": sched_eval,
@@ -276,6 +278,39 @@ def measured_traffic(kernel, args):
     with open(path) as f:
         k = json.load(f)["kernels"].get(kernel)
     return None if k is None else k["traffic_bytes"]
+
+
+def sparse_adam_step(rast, params, reps=20):
+    """SparseGaussianAdam.step over the six parameter tensors of the heavy
+    model (59 floats per Gaussian) with the visibility of the bench camera;
+    HBM-bound: 28 B per updated element (read p, g, m, v; write p, m, v)."""
+    import torch
+    from fastergs_amd.optim import SparseGaussianAdam
+    P = params["means3D"].shape[0]
+    with torch.no_grad():
+        _, radii = rast(means3D=params["means3D"], opacities=params["opacities"],
+                        dc=params["dc"], shs=params["shs"], scales=params["scales"],
+                        rotations=params["rotations"])
+    vis = radii > 0
+    ps = {k: v.detach().clone().requires_grad_(True) for k, v in params.items()}
+    for k, p in ps.items():
+        p.grad = torch.randn_like(p) * 1e-3
+    opt = SparseGaussianAdam([{"params": [p], "lr": 1e-3} for p in ps.values()], lr=0.0, eps=1e-15)
+    opt.step(vis, P)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        opt.step(vis, P)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    per = sum(p.numel() for p in ps.values()) // P
+    n_vis = int(vis.sum().item())
+    gbs = (n_vis * per * 28 + P) / (ms * 1e-3) / 1e9
+    return {"ms_per_step": round(ms, 4), "gaussians": P, "visible": n_vis,
+            "floats_per_gaussian": per, "achieved_gbs": round(gbs, 1),
+            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def _time_replays(rend, camt, reps):

@@ -1,6 +1,7 @@
 """Import name used by the FasterGS training/rendering scripts
 (``from diff_gaussian_rasterization_faster import GaussianRasterizationSettings,
-GaussianRasterizer``); forwards to fastergs_amd."""
+GaussianRasterizer``, ``SparseGaussianAdam``); forwards to fastergs_amd."""
+from fastergs_amd.optim import SparseGaussianAdam  # noqa: F401
 from fastergs_amd.rasterizer import (  # noqa: F401
     GaussianRasterizationSettings,
     GaussianRasterizer,

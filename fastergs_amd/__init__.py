@@ -13,3 +13,4 @@ from .rasterizer import (  # noqa: F401
     set_default_options,
 )
 from ._lib import SCHEDULE_FAST, SCHEDULE_PLAIN, FasterGSError  # noqa: F401
+from .optim import SparseGaussianAdam  # noqa: F401

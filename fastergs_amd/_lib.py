@@ -50,6 +50,11 @@ class Options(ctypes.Structure):
                 ("tile_sched", ctypes.POINTER(TileSched))]
 
 
+class AdamTensor(ctypes.Structure):
+    _fields_ = [("param", vp), ("grad", vp), ("exp_avg", vp), ("exp_avg_sq", vp),
+                ("M", ctypes.c_int), ("lr", ctypes.c_float), ("eps", ctypes.c_float)]
+
+
 class Batch(ctypes.Structure):
     _fields_ = [("n_views", ctypes.c_int), ("cameras", ctypes.POINTER(Camera))]
 
@@ -133,6 +138,8 @@ SIGNATURES = {
                                                 ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.POINTER(Options),
                                                 ctypes.POINTER(FrameView)]),
+    "fgs_sparse_adam": (ctypes.c_int, [ctypes.POINTER(AdamTensor), ctypes.c_int, vp, ctypes.c_int,
+                                       ctypes.c_float, ctypes.c_float, vp]),
     "fgs_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fgs_profile_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fgs_last_error": (ctypes.c_char_p, []),

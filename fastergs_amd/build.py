@@ -20,7 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfastergs.so")
-SOURCES = ["scan_sort.hip", "preprocess.hip", "render.hip", "backward.hip", "sched.hip", "api.hip"]
+SOURCES = ["scan_sort.hip", "preprocess.hip", "render.hip", "backward.hip", "sched.hip", "optim.hip",
+           "api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

@@ -719,6 +719,28 @@ int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int heig
   return fgs_batch_frame_view_get(gauss_ws, pair_ws, P, &b, R, NB, opt, v);
 }
 
+int fgs_sparse_adam(const fgs_adam_tensor* tensors, int n_tensors, const uint8_t* visible, int N,
+                    float beta1, float beta2, void* stream) {
+  if (n_tensors < 0 || n_tensors > ADAM_MAX_TENSORS || N < 0 || (n_tensors > 0 && !tensors))
+    return fail(FGS_ERR_ARG, "bad Adam tensor list (at most 8 tensors)");
+  if (N == 0 || n_tensors == 0) return FGS_OK;
+  if (!visible) return fail(FGS_ERR_ARG, "visible is NULL");
+  AdamArgs a{};
+  a.n = n_tensors;
+  a.N = N;
+  a.visible = visible;
+  a.b1 = beta1;
+  a.b2 = beta2;
+  for (int t = 0; t < n_tensors; ++t) {
+    const fgs_adam_tensor& d = tensors[t];
+    if (d.M < 1 || !d.param || !d.grad || !d.exp_avg || !d.exp_avg_sq)
+      return fail(FGS_ERR_ARG, "bad Adam tensor");
+    a.t[t] = AdamTensor{d.param, d.grad, d.exp_avg, d.exp_avg_sq, d.M, d.lr, d.eps, 0};
+  }
+  FGS_TRY(launch_sparse_adam(a, (hipStream_t)stream), "sparse Adam");
+  return FGS_OK;
+}
+
 int fgs_mlp_forward(const float* params, const float* x, int64_t n, float* y, int bf16,
                     void* stream) {
   if (n < 0 || (n > 0 && (!params || !x || !y))) return fail(FGS_ERR_ARG, "bad MLP arguments");

@@ -182,6 +182,28 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, 
 hipError_t launch_sched_items(int T, const uint32_t* ranges, int bucket, const float* pred,
                               uint32_t* items, uint32_t* n_items, hipStream_t s);
 
+// optim.hip: visibility-masked Adam over up to ADAM_MAX_TENSORS tensors
+constexpr int ADAM_MAX_TENSORS = 8;
+struct AdamTensor {
+  float* param;
+  const float* grad;
+  float* exp_avg;
+  float* exp_avg_sq;
+  int M;      // parameters per Gaussian
+  float lr;   // bias-compensated step size
+  float eps;
+  int vec;    // set by the launcher: all four arrays 16-byte aligned
+};
+struct AdamArgs {
+  AdamTensor t[ADAM_MAX_TENSORS];
+  int64_t block_off[ADAM_MAX_TENSORS];  // first workgroup of each tensor (launcher)
+  int n;
+  int N;
+  const uint8_t* visible;
+  float b1, b2;
+};
+hipError_t launch_sparse_adam(AdamArgs& a, hipStream_t s);
+
 // backward.hip
 struct BwdArgs {
   int W, H, grid_x, T, bucket, abs_grad;

@@ -252,6 +252,24 @@ int fgs_batch_densify_stats(int P, int n_views, const int32_t* radii, const floa
 int fgs_batch_frame_view_get(void* gauss_ws, void* pair_ws, int P, const fgs_batch* batch,
                              int64_t R, int64_t NB, const fgs_options* opt, fgs_frame_view* view);
 
+/* Replaces _C.ops.adamUpdate (pybind.cpp:341-366, adam.cu:9-67) as used by
+ * SparseGaussianAdam.step (__init__.py:267-298), for up to 8 parameter
+ * tensors in one launch: for every Gaussian g < N with visible[g] != 0 and
+ * each of its M parameters, m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+ * p += -lr m / (sqrt(v) + eps); other Gaussians are left untouched. lr is
+ * the caller's bias-compensated step size. */
+typedef struct fgs_adam_tensor {
+  float* param;       /* N x M */
+  const float* grad;  /* N x M */
+  float* exp_avg;     /* N x M */
+  float* exp_avg_sq;  /* N x M */
+  int M;
+  float lr;
+  float eps;
+} fgs_adam_tensor;
+int fgs_sparse_adam(const fgs_adam_tensor* tensors, int n_tensors, const uint8_t* visible, int N,
+                    float beta1, float beta2, void* stream);
+
 /* Per-kernel timing for measurement runs: while enabled, kernel launches are
  * bracketed by HIP events on their stream; fgs_profile_read synchronises and
  * writes "name total_ms launches\n" lines (aggregated since the last enable)
