@@ -132,6 +132,7 @@ def main():
 
     graph_fps = graph_render_fps(settings, params, sched, args)
     adam = sparse_adam_step(rast, params)
+    ssim = fused_ssim_step(cam)
     loop = render_loop(args) if args.render_loop else None
     torch.cuda.empty_cache()
     mv = multiview_step(args) if args.multiview else None
@@ -167,6 +168,7 @@ def main():
         "roofline": roofline,
         "stages": stages,
         "sparse_adam": adam,
+        "fused_ssim": ssim,
         "render_loop": loop,
         "multiview": mv,
         "learned_scheduler": sched_eval,
@@ -278,6 +280,32 @@ def measured_traffic(kernel, args):
     with open(path) as f:
         k = json.load(f)["kernels"].get(kernel)
     return None if k is None else k["traffic_bytes"]
+
+
+def fused_ssim_step(cam, reps=20):
+    """Fused SSIM loss forward + backward on one 3 x H x W image of the bench
+    camera's size (the loss term of the training loops). Minimum traffic per
+    pass: forward reads 2 images and writes 4 maps, backward reads 6 maps and
+    writes 1 (4 B per value)."""
+    import torch
+    from fastergs_amd.ssim import fused_ssim
+    g = torch.Generator(device="cuda").manual_seed(0)
+    a = torch.rand(1, 3, cam.height, cam.width, device="cuda", generator=g).requires_grad_(True)
+    b = torch.rand(1, 3, cam.height, cam.width, device="cuda", generator=g)
+    fused_ssim(a, b).backward()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        a.grad = None
+        fused_ssim(a, b).backward()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    n = a.numel()
+    gbs = (n * 4 * (2 + 4 + 6 + 1)) / (ms * 1e-3) / 1e9
+    return {"image": [3, cam.height, cam.width], "ms_fwd_bwd": round(ms, 4),
+            "min_traffic_gbs": round(gbs, 1)}
 
 
 def sparse_adam_step(rast, params, reps=20):

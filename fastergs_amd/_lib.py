@@ -140,6 +140,11 @@ SIGNATURES = {
                                                 ctypes.POINTER(FrameView)]),
     "fgs_sparse_adam": (ctypes.c_int, [ctypes.POINTER(AdamTensor), ctypes.c_int, vp, ctypes.c_int,
                                        ctypes.c_float, ctypes.c_float, vp]),
+    "fgs_ssim_forward": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, vp, vp, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp,
+                                        vp]),
+    "fgs_ssim_backward": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, vp, vp]),
     "fgs_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fgs_profile_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fgs_last_error": (ctypes.c_char_p, []),

@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfastergs.so")
 SOURCES = ["scan_sort.hip", "preprocess.hip", "render.hip", "backward.hip", "sched.hip", "optim.hip",
-           "api.hip"]
+           "ssim.hip", "api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

@@ -741,6 +741,36 @@ int fgs_sparse_adam(const fgs_adam_tensor* tensors, int n_tensors, const uint8_t
   return FGS_OK;
 }
 
+int fgs_ssim_forward(float C1, float C2, const float* img1, const float* img2, int B, int CH,
+                     int H, int W, float* ssim_map, float* dm_dmu1, float* dm_dsigma1_sq,
+                     float* dm_dsigma12, void* stream) {
+  if (B < 0 || CH < 0 || H < 0 || W < 0) return fail(FGS_ERR_ARG, "bad SSIM shape");
+  if ((int64_t)B * CH * H * W == 0) return FGS_OK;
+  if (!img1 || !img2 || !ssim_map) return fail(FGS_ERR_ARG, "NULL SSIM argument");
+  if ((dm_dmu1 == nullptr) != (dm_dsigma1_sq == nullptr) ||
+      (dm_dmu1 == nullptr) != (dm_dsigma12 == nullptr))
+    return fail(FGS_ERR_ARG, "give all three SSIM derivative maps or none");
+  if (H > 65535 * 16 || B > 65535) return fail(FGS_ERR_ARG, "SSIM image too large");
+  FGS_TRY(launch_ssim_fwd(B, CH, H, W, C1, C2, img1, img2, ssim_map, dm_dmu1, dm_dsigma1_sq,
+                          dm_dsigma12, (hipStream_t)stream),
+          "SSIM forward");
+  return FGS_OK;
+}
+
+int fgs_ssim_backward(const float* img1, const float* img2, const float* dL_dmap,
+                      const float* dm_dmu1, const float* dm_dsigma1_sq, const float* dm_dsigma12,
+                      int B, int CH, int H, int W, float* dL_dimg1, void* stream) {
+  if (B < 0 || CH < 0 || H < 0 || W < 0) return fail(FGS_ERR_ARG, "bad SSIM shape");
+  if ((int64_t)B * CH * H * W == 0) return FGS_OK;
+  if (!img1 || !img2 || !dL_dmap || !dm_dmu1 || !dm_dsigma1_sq || !dm_dsigma12 || !dL_dimg1)
+    return fail(FGS_ERR_ARG, "NULL SSIM argument");
+  if (H > 65535 * 16 || B > 65535) return fail(FGS_ERR_ARG, "SSIM image too large");
+  FGS_TRY(launch_ssim_bwd(B, CH, H, W, img1, img2, dL_dmap, dm_dmu1, dm_dsigma1_sq, dm_dsigma12,
+                          dL_dimg1, (hipStream_t)stream),
+          "SSIM backward");
+  return FGS_OK;
+}
+
 int fgs_mlp_forward(const float* params, const float* x, int64_t n, float* y, int bf16,
                     void* stream) {
   if (n < 0 || (n > 0 && (!params || !x || !y))) return fail(FGS_ERR_ARG, "bad MLP arguments");

@@ -204,6 +204,14 @@ struct AdamArgs {
 };
 hipError_t launch_sparse_adam(AdamArgs& a, hipStream_t s);
 
+// ssim.hip
+hipError_t launch_ssim_fwd(int B, int CH, int H, int W, float C1, float C2, const float* img1,
+                           const float* img2, float* map, float* dmu1, float* ds1, float* ds12,
+                           hipStream_t s);
+hipError_t launch_ssim_bwd(int B, int CH, int H, int W, const float* img1, const float* img2,
+                           const float* dmap, const float* dmu1, const float* ds1,
+                           const float* ds12, float* dimg1, hipStream_t s);
+
 // backward.hip
 struct BwdArgs {
   int W, H, grid_x, T, bucket, abs_grad;

@@ -270,6 +270,18 @@ typedef struct fgs_adam_tensor {
 int fgs_sparse_adam(const fgs_adam_tensor* tensors, int n_tensors, const uint8_t* visible, int N,
                     float beta1, float beta2, void* stream);
 
+/* Fused SSIM of the training loss (fused-ssim: ssim.cu:186-441). Images are
+ * B x CH x H x W float32. fgs_ssim_forward writes the SSIM map and, when
+ * dm_dmu1 is not NULL (training), the partial derivatives dm/dmu1,
+ * dm/dsigma1^2, dm/dsigma12 (all B x CH x H x W). fgs_ssim_backward turns
+ * dL/dmap into dL/dimg1. 11-tap Gaussian window (sigma 1.5), zero padding. */
+int fgs_ssim_forward(float C1, float C2, const float* img1, const float* img2, int B, int CH,
+                     int H, int W, float* ssim_map, float* dm_dmu1, float* dm_dsigma1_sq,
+                     float* dm_dsigma12, void* stream);
+int fgs_ssim_backward(const float* img1, const float* img2, const float* dL_dmap,
+                      const float* dm_dmu1, const float* dm_dsigma1_sq, const float* dm_dsigma12,
+                      int B, int CH, int H, int W, float* dL_dimg1, void* stream);
+
 /* Per-kernel timing for measurement runs: while enabled, kernel launches are
  * bracketed by HIP events on their stream; fgs_profile_read synchronises and
  * writes "name total_ms launches\n" lines (aggregated since the last enable)
