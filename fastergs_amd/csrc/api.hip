@@ -719,6 +719,30 @@ int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int heig
   return fgs_batch_frame_view_get(gauss_ws, pair_ws, P, &b, R, NB, opt, v);
 }
 
+int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                    int64_t NB, const fgs_options* opt, float* accum_weights,
+                    int32_t* accum_count, float* accum_max_count, void* stream) {
+  int rc;
+  if ((rc = check_opts(opt))) return rc;
+  if (P < 0 || width <= 0 || height <= 0 || R < 0) return fail(FGS_ERR_ARG, "bad frame size");
+  if (P == 0) return FGS_OK;
+  if (!gauss_ws || !pair_ws || !accum_weights || !accum_count || !accum_max_count)
+    return fail(FGS_ERR_ARG, "NULL argument");
+  const int T = tiles_of(width, height);
+  const GaussWs gw = carve_gauss(gauss_ws, P, T);
+  const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
+  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, fast);
+  const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);
+  hipStream_t s = (hipStream_t)stream;
+  FGS_TRY(fill_f32(accum_weights, 0.0f, P, s), "zero weights");
+  FGS_TRY(fill_u32((uint32_t*)accum_count, 0u, P, s), "zero counts");
+  FGS_TRY(fill_f32(accum_max_count, 0.0f, P, s), "zero max counts");
+  FGS_TRY(launch_render_simp(width, height, T, gw.ranges, vals, gw.splat, accum_weights,
+                             accum_count, accum_max_count, s),
+          "render_simp");
+  return FGS_OK;
+}
+
 int fgs_sparse_adam(const fgs_adam_tensor* tensors, int n_tensors, const uint8_t* visible, int N,
                     float beta1, float beta2, void* stream) {
   if (n_tensors < 0 || n_tensors > ADAM_MAX_TENSORS || N < 0 || (n_tensors > 0 && !tensors))

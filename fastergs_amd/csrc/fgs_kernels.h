@@ -167,6 +167,10 @@ struct CompArgs {
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);
 
+hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
+                              const float* splat, float* wsum, int32_t* wcount, float* maxcount,
+                              hipStream_t s);
+
 // sched.hip
 hipError_t launch_tile_features(int T, const uint32_t* ranges, const uint32_t* vals,
                                 const float* geom, const float* splat, float* feat,

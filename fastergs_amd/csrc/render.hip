@@ -416,4 +416,101 @@ hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ----------------------------------------------------------- render_simp
+// Importance accumulation over a rendered frame's lists (render_simp.cu:36-301):
+// one 256-thread workgroup per tile, one pixel per thread, splats staged 256
+// at a time in LDS. Pixels blend front to back without stopping on their
+// own; after each pair of splats (a leading single splat when the list
+// length is odd) the workgroup stops once all 256 pixels of the tile -- those
+// outside the image keep T = 1 -- have T < 1e-4. Per splat and wave the
+// weights alpha*T (DPP sum) and the number of pixels with alpha >= 1/255
+// (ballot) go to the splat with one atomic each; per pixel the splat of
+// largest weight (the first of equals) gets +1 at the end.
+__global__ __launch_bounds__(256) void k_render_simp(int W, int H, int grid_x,
+                                                     const uint32_t* __restrict__ ranges,
+                                                     const uint32_t* __restrict__ vals,
+                                                     const float* __restrict__ splat,
+                                                     float* __restrict__ wsum,
+                                                     int32_t* __restrict__ wcount,
+                                                     float* __restrict__ maxcount) {
+  __shared__ float4 sA[256];
+  __shared__ float2 sB[256];
+  __shared__ uint32_t sId[256];
+  const int tile = blockIdx.x;
+  const int tx = tile % grid_x, ty = tile / grid_x;
+  const int t = threadIdx.x;
+  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
+  const bool inside = px < W && py < H;
+  const float fx = (float)px, fy = (float)py;
+  const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
+  const uint32_t len = end - start;
+  float T = 1.0f, wmax = 0.0f;
+  uint32_t imax = 0;
+  bool flag = false;
+  const int lane = lane_id();
+  bool done = false;
+  for (uint32_t base = 0; base < len && !done; base += 256) {
+    const uint32_t n = min(256u, len - base);
+    __syncthreads();
+    if (t < (int)n) {
+      const uint32_t id = vals[start + base + t];
+      const float4* r = (const float4*)(splat + 12 * (size_t)id);
+      const float4 r1 = r[1];
+      sA[t] = r[0];
+      sB[t] = make_float2(r1.x, r1.y);
+      sId[t] = id;
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t id = sId[j];
+      float w = 0.0f;
+      bool hit = false;
+      if (inside) {
+        const float4 A = sA[j];
+        const float2 B = sB[j];
+        const float a = splat_alpha(A.x, A.y, A.z, A.w, B.x, B.y, fx, fy);
+        if (a != 0.0f) {  // alpha >= 1/255
+          w = a * T;
+          T = T - w;
+          hit = true;
+          if (wmax < w) {
+            wmax = w;
+            imax = id;
+            flag = true;
+          }
+        }
+      }
+      const uint64_t hits = __ballot(hit);
+      if (hits) {
+        const float ws = wave_sum(w);
+        if (lane == 0) {
+          atomic_add_f(&wsum[id], ws);
+          atomicAdd(&wcount[id], (int32_t)__popcll(hits));
+        }
+      }
+      // pairs follow the leading single of an odd-length list: position pos
+      // ends a pair when it is odd (even length) or even and >= 2 (odd length)
+      const uint32_t pos = base + j;
+      const bool pair_end = (len & 1u) ? (pos >= 2u && (pos & 1u) == 0u) : ((pos & 1u) == 1u);
+      if (pair_end) {
+        if (__syncthreads_count(T < T_CUT_F) == 256) {
+          done = true;
+          break;
+        }
+      }
+    }
+  }
+  if (inside && flag) atomic_add_f(&maxcount[imax], 1.0f);
+}
+
+hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
+                              const float* splat, float* wsum, int32_t* wcount, float* maxcount,
+                              hipStream_t s) {
+  if (T > 0)
+    FGS_TIMED("render_simp", s,
+              (k_render_simp<<<T, 256, 0, s>>>(W, H, (W + TILE_W - 1) / TILE_W, ranges, vals, splat,
+                                                wsum, wcount, maxcount)));
+  return hipGetLastError();
+}
+
 }  // namespace fgs

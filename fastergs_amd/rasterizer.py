@@ -209,6 +209,18 @@ class Frame:
                                       ctypes.c_void_p(_stream(self.device))),
               "fgs_densify_stats")
 
+    def render_simp(self):
+        """Importance statistics over this frame's lists (render_simp):
+        (accum_weights (P,) float, accum_count (P,) int32, accum_max_count (P,) float)."""
+        dev, P = self.device, self.P
+        w = torch.empty(P, dtype=torch.float32, device=dev)
+        c = torch.empty(P, dtype=torch.int32, device=dev)
+        m = torch.empty(P, dtype=torch.float32, device=dev)
+        check(lib().fgs_render_simp(_ptr(self.gws), _ptr(self.pws), P, self.W, self.H, self.R,
+                                    self.NB, ctypes.byref(self.copts), _ptr(w), _ptr(c), _ptr(m),
+                                    ctypes.c_void_p(_stream(dev))), "fgs_render_simp")
+        return w, c, m
+
     # -- inspection -----------------------------------------------------------
     def view_arrays(self) -> dict:
         """Torch views of the intermediate arrays (valid until the frame dies)."""
@@ -330,3 +342,23 @@ class GaussianRasterizer(nn.Module):
             xyz_gradient_accum_abs if xyz_gradient_accum_abs is not None else empty(),
             max_radii if max_radii is not None else empty(),
             denom if denom is not None else empty())
+
+    @torch.no_grad()
+    def render_simp(self, means3D, means2D, opacities, culling=None, dc=None, shs=None,
+                    colors_precomp=None, scales=None, rotations=None, cov3D_precomp=None):
+        """Forward render plus the render_simp importance statistics, with the
+        argument order of the Mini-Splatting2 backends' ``render_simp``
+        (diff_gaussian_rasterization_ms_2/__init__.py:416-455; the FasterGS
+        backend binds the same operation as _C.ops.render_simp, pybind.cpp:180-208).
+        ``means2D`` is accepted for that signature and unused. Returns
+        (color, radii, accum_weights, accum_count, accum_max_count)."""
+        if (shs is None and colors_precomp is None) or (shs is not None and colors_precomp is not None):
+            raise ValueError("Please provide excatly one of either SHs or precomputed colors!")
+        if cov3D_precomp is not None and cov3D_precomp.numel() > 0:
+            raise ValueError("cov3D_precomp is not supported: pass scales and rotations")
+        if scales is None or rotations is None:
+            raise ValueError("scales and rotations are required")
+        fr = Frame(self.raster_settings, means3D, opacities, scales, rotations, dc=dc, shs=shs,
+                   colors_precomp=colors_precomp, culling=culling).forward()
+        w, c, m = fr.render_simp()
+        return fr.out_color, fr.radii, w, c, m

@@ -179,6 +179,17 @@ int fgs_backward(const fgs_camera* cam, const fgs_gaussians* g, const fgs_option
 int fgs_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* xyz_grad_accum,
                       float* xyz_grad_accum_abs, float* max_radii, float* denom, void* stream);
 
+/* Replaces _C.ops.render_simp (pybind.cpp:180-208, render_simp.cu:36-311)
+ * on a rendered frame's lists (either schedule, one view): per Gaussian the
+ * sum of blending weights alpha*T over the pixels (accum_weights, P float),
+ * the number of pixels with alpha >= 1/255 (accum_count, P int32) and the
+ * number of pixels for which it has the largest weight (accum_max_count, P
+ * float). Pixels do not stop on their own; a tile stops after a pair of
+ * splats once all its 256 pixels have T < 1e-4. Outputs are overwritten. */
+int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                    int64_t NB, const fgs_options* opt, float* accum_weights,
+                    int32_t* accum_count, float* accum_max_count, void* stream);
+
 /* Views of the workspaces' arrays (host-side pointer arithmetic only). */
 int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
                        int64_t NB, const fgs_options* opt, fgs_frame_view* view);

@@ -540,6 +540,66 @@ long long get_named(State<R>* st, const char* name, void* dst) {
   return -1;
 }
 
+// Importance accumulation of render_simp (render_simp.cu:36-301) over the
+// sorted lists of a forward: per tile every pixel blends front to back
+// without stopping on its own; after each pair of splats (a leading single
+// splat when the list length is odd) the tile stops once all 256 of its
+// pixels -- those outside the image keep T = 1 -- have T < 1e-4. Per splat:
+// the sum of the weights alpha*T and the number of pixels with alpha >=
+// 1/255; per pixel: +1 for the splat of largest weight (the first of equals).
+template <typename R>
+void render_simp(const State<R>* st, double* wsum, int32_t* wcount, double* maxcount) {
+  const int P = st->P;
+  std::fill(wsum, wsum + P, 0.0);
+  std::fill(wcount, wcount + P, 0);
+  std::fill(maxcount, maxcount + P, 0.0);
+  for (int t = 0; t < st->T; ++t) {
+    const int tx = t % st->gx, ty = t / st->gx;
+    const uint32_t s = st->ranges[2 * t], e = st->ranges[2 * t + 1];
+    R T[fgs::TILE_PIX], wmax[fgs::TILE_PIX];
+    uint32_t imax[fgs::TILE_PIX];
+    bool flag[fgs::TILE_PIX];
+    for (int p = 0; p < fgs::TILE_PIX; ++p) {
+      T[p] = R(1);
+      wmax[p] = R(0);
+      imax[p] = 0;
+      flag[p] = false;
+    }
+    auto shade = [&](uint32_t k) {
+      const uint32_t id = st->vals[k];
+      const R* sp = &st->splat[9 * (size_t)id];
+      for (int p = 0; p < fgs::TILE_PIX; ++p) {
+        const int px = tx * fgs::TILE_W + (p % fgs::TILE_W), py = ty * fgs::TILE_H + p / fgs::TILE_W;
+        if (px >= st->W || py >= st->H) continue;
+        const R a = fgs::splat_alpha(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], (R)px, (R)py);
+        if (a == R(0)) continue;  // alpha < 1/255
+        const R w = a * T[p];
+        T[p] = T[p] - w;
+        wsum[id] += (double)w;
+        wcount[id] += 1;
+        if (wmax[p] < w) {
+          wmax[p] = w;
+          imax[p] = id;
+          flag[p] = true;
+        }
+      }
+    };
+    uint32_t k = s;
+    if ((e - s) % 2 == 1) shade(k++);
+    for (; k < e; k += 2) {
+      shade(k);
+      shade(k + 1);
+      bool done = true;
+      for (int p = 0; p < fgs::TILE_PIX; ++p) done = done && T[p] < R(fgs::T_CUT_F);
+      if (done) break;
+    }
+    for (int p = 0; p < fgs::TILE_PIX; ++p) {
+      const int px = tx * fgs::TILE_W + (p % fgs::TILE_W), py = ty * fgs::TILE_H + p / fgs::TILE_W;
+      if (px < st->W && py < st->H && flag[p]) maxcount[imax[p]] += 1.0;
+    }
+  }
+}
+
 template <typename R>
 void add_stats(State<R>* st, R* accum, R* accum_abs, R* max_radii, R* denom) {
   for (int i = 0; i < st->P; ++i) {
@@ -555,6 +615,11 @@ void add_stats(State<R>* st, R* accum, R* accum_abs, R* max_radii, R* denom) {
 }  // namespace
 
 extern "C" {
+
+void orc_render_simp_f32(void* h, double* wsum, int32_t* wcount, double* maxcount) {
+  render_simp((const State<float>*)h, wsum, wcount, maxcount);
+}
+
 
 // ---- float32 oracle
 void* orc_create_f32(int P, const float* means, const float* opac, const float* scales,

@@ -49,6 +49,7 @@ def lib():
         L.orc_tile_hit_f32.argtypes = [i32, i32] + [ctypes.c_float] * 6
         L.orc_tile_hit_f32.restype = i32
         L.orc_num_threads.restype = i32
+        L.orc_render_simp_f32.argtypes = [vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -142,6 +143,15 @@ class Oracle:
 
     def stats(self, accum, accum_abs, max_radii, denom):
         self.L.orc_stats_f32(self.h, _ptr(accum), _ptr(accum_abs), _ptr(max_radii), _ptr(denom))
+
+    def render_simp(self):
+        """(accum weights (double), accum counts (int32), max counts (double)) per Gaussian."""
+        P = self.P
+        w = np.zeros(P, np.float64)
+        c = np.zeros(P, np.int32)
+        m = np.zeros(P, np.float64)
+        self.L.orc_render_simp_f32(self.h, _ptr(w), _ptr(c), _ptr(m))
+        return w, c, m
 
     def grads(self) -> dict:
         P = self.P
