@@ -148,6 +148,9 @@ SIGNATURES = {
     "fgs_render_simp": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(Options),
                                        vp, vp, vp, vp]),
+    "fgs_render_depth": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(Options)]
+                         + [vp] * 11),
     "fgs_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fgs_profile_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fgs_last_error": (ctypes.c_char_p, []),

@@ -743,6 +743,31 @@ int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height,
   return FGS_OK;
 }
 
+int fgs_render_depth(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                     int64_t NB, const fgs_options* opt, const float* means3D,
+                     const float* scales, const float* rotations, const float* projmatrix,
+                     const float* campos, float* out_pts, float* out_depth, float* accum_alpha,
+                     int32_t* gidx, float* discriminants, void* stream) {
+  int rc;
+  if ((rc = check_opts(opt))) return rc;
+  if (P < 0 || width <= 0 || height <= 0 || R < 0) return fail(FGS_ERR_ARG, "bad frame size");
+  if (!gauss_ws || !pair_ws || !projmatrix || !campos || !out_pts || !out_depth ||
+      !accum_alpha || !gidx || !discriminants)
+    return fail(FGS_ERR_ARG, "NULL argument");
+  if (P > 0 && (!means3D || !scales || !rotations))
+    return fail(FGS_ERR_ARG, "render_depth needs means3D, scales and rotations");
+  const int T = tiles_of(width, height);
+  const GaussWs gw = carve_gauss(gauss_ws, P, T);
+  const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
+  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, fast);
+  const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);
+  FGS_TRY(launch_render_depth(width, height, T, gw.ranges, vals, gw.splat, means3D, scales,
+                              rotations, projmatrix, campos, out_pts, out_depth, accum_alpha,
+                              gidx, discriminants, (hipStream_t)stream),
+          "render_depth");
+  return FGS_OK;
+}
+
 int fgs_sparse_adam(const fgs_adam_tensor* tensors, int n_tensors, const uint8_t* visible, int N,
                     float beta1, float beta2, void* stream) {
   if (n_tensors < 0 || n_tensors > ADAM_MAX_TENSORS || N < 0 || (n_tensors > 0 && !tensors))

@@ -170,6 +170,11 @@ hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);
 hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
                               const float* splat, float* wsum, int32_t* wcount, float* maxcount,
                               hipStream_t s);
+hipError_t launch_render_depth(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
+                               const float* splat, const float* means3D, const float* scales,
+                               const float* rotations, const float* proj, const float* campos,
+                               float* out_pts, float* out_depth, float* accum_alpha,
+                               int32_t* gidx, float* disc, hipStream_t s);
 
 // sched.hip
 hipError_t launch_tile_features(int T, const uint32_t* ranges, const uint32_t* vals,

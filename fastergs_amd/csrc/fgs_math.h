@@ -801,4 +801,151 @@ FGS_HD void gauss_bwd(int i, const GaussRef<R>& G, const CamRef<R>& cam, int cla
 // pattern orders like the value (preprocess.cu:618-620).
 FGS_HD uint32_t depth_bits(float z) { return f2u(z); }
 
+// ------------------------------------------------------------ render_depth
+// Inverse of a 4x4 matrix (column-major) by cofactor expansion; zeros when
+// singular. Used once per tile for the pixel rays of render_depth.
+template <typename R>
+FGS_HD void inverse4(const R* m, R* inv) {
+  R c[16];
+  c[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+         m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  c[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+         m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  c[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+         m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  c[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+          m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  c[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+         m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  c[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+         m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  c[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+         m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  c[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+          m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  c[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+         m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  c[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+         m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  c[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+          m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  c[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+          m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  c[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+         m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  c[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+         m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  c[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+          m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  c[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+          m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  const R det = m[0] * c[0] + m[1] * c[4] + m[2] * c[8] + m[3] * c[12];
+  const R id = det != R(0) ? R(1) / det : R(0);
+  for (int k = 0; k < 16; ++k) inv[k] = c[k] * id;
+}
+
+// Camera ray of pixel (px, py) as render_depth builds it
+// (render_depth.cu:158-186): the pixel's NDC point (scaled by 1.0000001, at
+// the normalised depth (100 - 100 * 0.01) / (100 - 0.01) of a 0.01 / 100
+// frustum) mapped through the inverse projection without a perspective
+// divide, minus the camera centre. ndir is the unit direction, len the
+// length before normalisation.
+template <typename R>
+FGS_HD void depth_ray(const R* proj_inv, const R* campos, int px, int py, int W, int H, R ndir[3],
+                      R& len) {
+  const R nx = (R(2) * (R)px + R(1)) / (R)W - R(1);
+  const R ny = (R(2) * (R)py + R(1)) / (R)H - R(1);
+  const R hx = (R)((double)nx * 1.0000001), hy = (R)((double)ny * 1.0000001);
+  const R hz = (R)((100.0 - 100.0 * 0.01) / (100.0 - 0.01));
+  R o[3];
+  xform_point(proj_inv, hx, hy, hz, o);
+  const R d0 = o[0] - campos[0], d1 = o[1] - campos[1], d2 = o[2] - campos[2];
+  len = sqrt_(d0 * d0 + d1 * d1 + d2 * d2);
+  ndir[0] = d0 / len;
+  ndir[1] = d1 / len;
+  ndir[2] = d2 / len;
+}
+
+// Ray against the 3-sigma ellipsoid of a Gaussian (render_depth.cu:220-256),
+// in the frame rotated by the given (unnormalised) quaternion: the quadratic
+// a t^2 + b t + c in the parameter t along the unit direction. The parts
+// that depend on the Gaussian and the ray origin only (rotation, rotated
+// origin, 3 sigma, c) are set up once per Gaussian; ellipsoid_hit adds the
+// direction and returns the midpoint t_mid = -b / 2 / a and the
+// discriminant b^2 - 4ac.
+template <typename R>
+struct Ellipsoid {
+  R Rm[9];  // quat_rot of the quaternion; applied as sum_c Rm[3c + r] v[c]
+  R ro[3];  // rotated (origin - mean)
+  R s3[3];  // 3 * scale
+  R c;
+};
+
+template <typename R>
+FGS_HD void ellipsoid_setup(const R* mean, const R* scale, const R* q, const R* origin,
+                            Ellipsoid<R>& E) {
+  quat_rot(q, E.Rm);
+  const R tmp[3] = {origin[0] - mean[0], origin[1] - mean[1], origin[2] - mean[2]};
+  for (int r = 0; r < 3; ++r)
+    E.ro[r] = E.Rm[r] * tmp[0] + E.Rm[3 + r] * tmp[1] + E.Rm[6 + r] * tmp[2];
+  R ct[3];
+  for (int k = 0; k < 3; ++k) {
+    E.s3[k] = scale[k] * R(3);
+    ct[k] = E.ro[k] / E.s3[k] * E.ro[k] / E.s3[k];
+  }
+  E.c = ct[0] + ct[1] + ct[2] - R(1);
+}
+
+template <typename R>
+FGS_HD void ellipsoid_hit(const Ellipsoid<R>& E, const R* ndir, R& t_mid, R& disc) {
+  R at[3], bt[3];
+  for (int k = 0; k < 3; ++k) {
+    const R rd = E.Rm[k] * ndir[0] + E.Rm[3 + k] * ndir[1] + E.Rm[6 + k] * ndir[2];
+    const R q = rd / E.s3[k];
+    at[k] = q * rd / E.s3[k];
+    bt[k] = q * E.ro[k] / E.s3[k];
+  }
+  const R a = at[0] + at[1] + at[2];
+  const R b = R(2) * (bt[0] + bt[1] + bt[2]);
+  disc = b * b - R(4) * a * E.c;
+  t_mid = -b / R(2) / a;
+}
+
+// One pixel of render_depth (render_depth.cu:207-288), updated with one
+// splat of its list that has alpha >= 1/255: a splat that would bring T
+// below 1e-4 ends the pixel; a hit behind the camera (depth < 0) is skipped
+// without changing T; otherwise the splat with the largest weight alpha*T
+// so far (the first of equals) sets depth, discriminant, index and the
+// point origin + t_mid * ndir, and T becomes T (1 - alpha).
+template <typename R>
+struct DepthPix {
+  R T = R(1), wmax = R(0), depth = R(0), disc = R(0);
+  R pt[3] = {R(0), R(0), R(0)};
+  int32_t idx = 0;
+  bool done = false;
+};
+
+template <typename R>
+FGS_HD void depth_pix_step(DepthPix<R>& D, R alpha, uint32_t id, const Ellipsoid<R>& E,
+                           const R* origin, const R* ndir, R len) {
+  const R test_T = D.T * (R(1) - alpha);
+  if (test_T < R(T_CUT_F)) {
+    D.done = true;
+    return;
+  }
+  R t_mid, disc;
+  ellipsoid_hit(E, ndir, t_mid, disc);
+  const R depth = t_mid / len;
+  if (depth < R(0)) return;
+  const R w = alpha * D.T;
+  if (D.wmax < w) {
+    D.wmax = w;
+    D.depth = depth;
+    D.disc = disc;
+    D.idx = (int32_t)id;
+    for (int k = 0; k < 3; ++k) D.pt[k] = origin[k] + t_mid * ndir[k];
+  }
+  D.T = test_T;
+}
+
 }  // namespace fgs

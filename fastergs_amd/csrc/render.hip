@@ -513,4 +513,97 @@ hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const
   return hipGetLastError();
 }
 
+// ----------------------------------------------------------- render_depth
+// Per-pixel depth over a rendered frame's lists (render_depth.cu:92-290):
+// one 256-thread workgroup per tile, one pixel per thread. Each thread
+// builds its pixel's camera ray (inverse projection, once per thread);
+// splats are staged 256 at a time in LDS together with the per-Gaussian
+// part of the ray/ellipsoid quadratic (rotation, rotated camera offset,
+// 3 sigma, constant term), which depends on the Gaussian and the camera
+// only. A pixel stops when T (1 - alpha) < 1e-4; the workgroup stops once
+// all its pixels have stopped. Every pixel inside the image is written.
+__global__ __launch_bounds__(256) void k_render_depth(
+    int W, int H, int grid_x, const uint32_t* __restrict__ ranges,
+    const uint32_t* __restrict__ vals, const float* __restrict__ splat,
+    const float* __restrict__ means3D, const float* __restrict__ scales,
+    const float* __restrict__ rotations, const float* __restrict__ proj,
+    const float* __restrict__ campos, float* __restrict__ out_pts, float* __restrict__ out_depth,
+    float* __restrict__ accum_alpha, int32_t* __restrict__ gidx, float* __restrict__ disc) {
+  __shared__ float4 sA[256];
+  __shared__ float2 sB[256];
+  __shared__ uint32_t sId[256];
+  __shared__ Ellipsoid<float> sE[256];
+  const int tile = blockIdx.x;
+  const int tx = tile % grid_x, ty = tile / grid_x;
+  const int t = threadIdx.x;
+  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
+  const bool inside = px < W && py < H;
+  const float fx = (float)px, fy = (float)py;
+  const float o[3] = {campos[0], campos[1], campos[2]};
+  float nd[3], len;
+  {
+    float m[16], inv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = proj[k];
+    inverse4(m, inv);
+    depth_ray(inv, o, px, py, W, H, nd, len);
+  }
+  const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
+  const uint32_t len_list = end - start;
+  DepthPix<float> D;
+  D.done = !inside;
+  for (uint32_t base = 0; base < len_list; base += 256) {
+    if (__syncthreads_count(D.done) == 256) break;
+    const uint32_t n = min(256u, len_list - base);
+    if (t < (int)n) {
+      const uint32_t id = vals[start + base + t];
+      const float4* r = (const float4*)(splat + 12 * (size_t)id);
+      const float4 r1 = r[1];
+      sA[t] = r[0];
+      sB[t] = make_float2(r1.x, r1.y);
+      sId[t] = id;
+      const float mean[3] = {means3D[3 * (size_t)id], means3D[3 * (size_t)id + 1],
+                             means3D[3 * (size_t)id + 2]};
+      const float sc[3] = {scales[3 * (size_t)id], scales[3 * (size_t)id + 1],
+                           scales[3 * (size_t)id + 2]};
+      const float q[4] = {rotations[4 * (size_t)id], rotations[4 * (size_t)id + 1],
+                          rotations[4 * (size_t)id + 2], rotations[4 * (size_t)id + 3]};
+      Ellipsoid<float> E;
+      ellipsoid_setup(mean, sc, q, o, E);
+      sE[t] = E;
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < n && !D.done; ++j) {
+      const float4 A = sA[j];
+      const float2 B = sB[j];
+      const float a = splat_alpha(A.x, A.y, A.z, A.w, B.x, B.y, fx, fy);
+      if (a == 0.0f) continue;  // alpha < 1/255
+      depth_pix_step(D, a, sId[j], sE[j], o, nd, len);
+    }
+  }
+  if (inside) {
+    const size_t HW = (size_t)W * H, pix = (size_t)py * W + px;
+    out_pts[pix] = D.pt[0];
+    out_pts[HW + pix] = D.pt[1];
+    out_pts[2 * HW + pix] = D.pt[2];
+    out_depth[pix] = D.depth;
+    accum_alpha[pix] = D.T;
+    gidx[pix] = D.idx;
+    disc[pix] = D.disc;
+  }
+}
+
+hipError_t launch_render_depth(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
+                               const float* splat, const float* means3D, const float* scales,
+                               const float* rotations, const float* proj, const float* campos,
+                               float* out_pts, float* out_depth, float* accum_alpha,
+                               int32_t* gidx, float* disc, hipStream_t s) {
+  if (T > 0)
+    FGS_TIMED("render_depth", s,
+              (k_render_depth<<<T, 256, 0, s>>>(W, H, (W + TILE_W - 1) / TILE_W, ranges, vals,
+                                                 splat, means3D, scales, rotations, proj, campos,
+                                                 out_pts, out_depth, accum_alpha, gidx, disc)));
+  return hipGetLastError();
+}
+
 }  // namespace fgs

@@ -221,6 +221,23 @@ class Frame:
                                     ctypes.c_void_p(_stream(dev))), "fgs_render_simp")
         return w, c, m
 
+    def render_depth(self) -> dict:
+        """Per-pixel depth over this frame's lists (render_depth): out_pts
+        (3,H,W), rendered_depth (1,H,W), accum_alpha (1,H,W), gidx (1,H,W)
+        int32 and discriminants (1,H,W)."""
+        dev, H, W = self.device, self.H, self.W
+        e = lambda c, dt=torch.float32: torch.empty((c, H, W), dtype=dt, device=dev)
+        out = {"out_pts": e(3), "rendered_depth": e(1), "accum_alpha": e(1),
+               "gidx": e(1, torch.int32), "discriminants": e(1)}
+        check(lib().fgs_render_depth(_ptr(self.gws), _ptr(self.pws), self.P, W, H, self.R,
+                                     self.NB, ctypes.byref(self.copts), _ptr(self.means3D),
+                                     _ptr(self.scales), _ptr(self.rotations), _ptr(self.proj),
+                                     _ptr(self.campos), _ptr(out["out_pts"]),
+                                     _ptr(out["rendered_depth"]), _ptr(out["accum_alpha"]),
+                                     _ptr(out["gidx"]), _ptr(out["discriminants"]),
+                                     ctypes.c_void_p(_stream(dev))), "fgs_render_depth")
+        return out
+
     # -- inspection -----------------------------------------------------------
     def view_arrays(self) -> dict:
         """Torch views of the intermediate arrays (valid until the frame dies)."""
@@ -362,3 +379,26 @@ class GaussianRasterizer(nn.Module):
                    colors_precomp=colors_precomp, culling=culling).forward()
         w, c, m = fr.render_simp()
         return fr.out_color, fr.radii, w, c, m
+
+    @torch.no_grad()
+    def render_depth(self, means3D, means2D, opacities, culling=None, dc=None, shs=None,
+                     colors_precomp=None, scales=None, rotations=None, cov3D_precomp=None):
+        """Forward render plus the per-pixel depth of render_depth, with the
+        argument order and result dict of the Mini-Splatting2 backends'
+        ``render_depth`` (diff_gaussian_rasterization_ms_2/__init__.py:457-497,
+        :223-265; the FasterGS backend binds the same kernel as
+        _C.ops.render_depth, pybind.cpp:134-172). ``means2D`` is accepted for
+        that signature and unused. Returns {"render", "out_pts",
+        "rendered_depth", "discriminants", "gidx", "accum_alpha", "radii"}."""
+        if (shs is None and colors_precomp is None) or (shs is not None and colors_precomp is not None):
+            raise ValueError("Please provide excatly one of either SHs or precomputed colors!")
+        if cov3D_precomp is not None and cov3D_precomp.numel() > 0:
+            raise ValueError("cov3D_precomp is not supported: pass scales and rotations")
+        if scales is None or rotations is None:
+            raise ValueError("scales and rotations are required")
+        fr = Frame(self.raster_settings, means3D, opacities, scales, rotations, dc=dc, shs=shs,
+                   colors_precomp=colors_precomp, culling=culling).forward()
+        res = fr.render_depth()
+        res["render"] = fr.out_color
+        res["radii"] = fr.radii
+        return res

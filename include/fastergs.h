@@ -190,6 +190,23 @@ int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height,
                     int64_t NB, const fgs_options* opt, float* accum_weights,
                     int32_t* accum_count, float* accum_max_count, void* stream);
 
+/* Replaces _C.ops.render_depth (pybind.cpp:134-172, render_depth.cu:92-380)
+ * on a rendered frame's lists (either schedule, one view). Per pixel the
+ * camera ray through the pixel centre (inverse of projmatrix, the 16 floats
+ * of fgs_camera.projmatrix; campos 3 floats) walks the tile's list front to
+ * back until T (1 - alpha) < 1e-4; for the splat of largest weight alpha*T
+ * among those whose 3-sigma ellipsoid (means3D, scales, rotations as given,
+ * P x 3 / P x 3 / P x 4) has its chord midpoint in front of the camera:
+ * out_pts (3 x H x W) the midpoint, out_depth (H x W) its ray parameter
+ * over the ray's length, gidx (H x W int32) the Gaussian, discriminants
+ * (H x W) the quadratic's discriminant; accum_alpha (H x W) the pixel's
+ * final T. Every pixel is written (zeros and T = 1 where nothing was hit). */
+int fgs_render_depth(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                     int64_t NB, const fgs_options* opt, const float* means3D,
+                     const float* scales, const float* rotations, const float* projmatrix,
+                     const float* campos, float* out_pts, float* out_depth, float* accum_alpha,
+                     int32_t* gidx, float* discriminants, void* stream);
+
 /* Views of the workspaces' arrays (host-side pointer arithmetic only). */
 int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
                        int64_t NB, const fgs_options* opt, fgs_frame_view* view);

@@ -600,6 +600,51 @@ void render_simp(const State<R>* st, double* wsum, int32_t* wcount, double* maxc
   }
 }
 
+// Per-pixel depth of render_depth (render_depth.cu:92-290) over the sorted
+// lists of a forward: each pixel's camera ray (through the inverse
+// projection) walks its tile's list front to back until T would drop below
+// 1e-4; among splats hit in front of the camera the one of largest blending
+// weight gives the ray parameter of the midpoint of its 3-sigma ellipsoid
+// (depth = t_mid / |ray|), the 3D point, the discriminant and its index.
+// Outputs are written for every pixel inside the image.
+template <typename R>
+void render_depth(const State<R>* st, R* out_pts, R* out_depth, R* accum_alpha, int32_t* gidx,
+                  R* disc) {
+  const int W = st->W, H = st->H, P = st->P;
+  const size_t HW = (size_t)W * H;
+  R inv[16];
+  fgs::inverse4(st->proj.data(), inv);
+  const R* o = st->campos.data();
+  std::vector<fgs::Ellipsoid<R>> ell(P);
+  for (int i = 0; i < P; ++i)
+    fgs::ellipsoid_setup(&st->means[3 * (size_t)i], &st->scales[3 * (size_t)i],
+                         &st->rots[4 * (size_t)i], o, ell[i]);
+  for (int t = 0; t < st->T; ++t) {
+    const int tx = t % st->gx, ty = t / st->gx;
+    const uint32_t s = st->ranges[2 * t], e = st->ranges[2 * t + 1];
+    for (int p = 0; p < fgs::TILE_PIX; ++p) {
+      const int px = tx * fgs::TILE_W + (p % fgs::TILE_W), py = ty * fgs::TILE_H + p / fgs::TILE_W;
+      if (px >= W || py >= H) continue;
+      R nd[3], len;
+      fgs::depth_ray(inv, o, px, py, W, H, nd, len);
+      fgs::DepthPix<R> D;
+      for (uint32_t k = s; k < e && !D.done; ++k) {
+        const uint32_t id = st->vals[k];
+        const R* sp = &st->splat[9 * (size_t)id];
+        const R a = fgs::splat_alpha(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], (R)px, (R)py);
+        if (a == R(0)) continue;  // alpha < 1/255
+        fgs::depth_pix_step(D, a, id, ell[id], o, nd, len);
+      }
+      const size_t pix = (size_t)py * W + px;
+      for (int c = 0; c < 3; ++c) out_pts[c * HW + pix] = D.pt[c];
+      out_depth[pix] = D.depth;
+      accum_alpha[pix] = D.T;
+      gidx[pix] = D.idx;
+      disc[pix] = D.disc;
+    }
+  }
+}
+
 template <typename R>
 void add_stats(State<R>* st, R* accum, R* accum_abs, R* max_radii, R* denom) {
   for (int i = 0; i < st->P; ++i) {
@@ -616,6 +661,30 @@ void add_stats(State<R>* st, R* accum, R* accum_abs, R* max_radii, R* denom) {
 
 extern "C" {
 
+// float64 instances of render_depth's geometry, for pinning it on its own.
+void orc_inverse4_f64(const double* m, double* inv) { fgs::inverse4(m, inv); }
+void orc_depth_ray_f64(const double* proj, const double* campos, int px, int py, int W, int H,
+                       double* out4) {
+  double inv[16];
+  fgs::inverse4(proj, inv);
+  fgs::depth_ray(inv, campos, px, py, W, H, out4, out4[3]);
+}
+void orc_depth_ray_f32(const float* proj, const float* campos, int px, int py, int W, int H,
+                       float* out4) {
+  float inv[16];
+  fgs::inverse4(proj, inv);
+  fgs::depth_ray(inv, campos, px, py, W, H, out4, out4[3]);
+}
+void orc_ellipsoid_f64(const double* mean, const double* scale, const double* q,
+                       const double* origin, const double* ndir, double* out2) {
+  fgs::Ellipsoid<double> E;
+  fgs::ellipsoid_setup(mean, scale, q, origin, E);
+  fgs::ellipsoid_hit(E, ndir, out2[0], out2[1]);
+}
+void orc_render_depth_f32(void* h, float* out_pts, float* out_depth, float* accum_alpha,
+                          int32_t* gidx, float* disc) {
+  render_depth((const State<float>*)h, out_pts, out_depth, accum_alpha, gidx, disc);
+}
 void orc_render_simp_f32(void* h, double* wsum, int32_t* wcount, double* maxcount) {
   render_simp((const State<float>*)h, wsum, wcount, maxcount);
 }

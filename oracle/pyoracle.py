@@ -50,6 +50,11 @@ def lib():
         L.orc_tile_hit_f32.restype = i32
         L.orc_num_threads.restype = i32
         L.orc_render_simp_f32.argtypes = [vp, vp, vp, vp]
+        L.orc_render_depth_f32.argtypes = [vp] * 6
+        L.orc_inverse4_f64.argtypes = [vp, vp]
+        L.orc_depth_ray_f64.argtypes = [vp, vp, i32, i32, i32, i32, vp]
+        L.orc_depth_ray_f32.argtypes = [vp, vp, i32, i32, i32, i32, vp]
+        L.orc_ellipsoid_f64.argtypes = [vp] * 6
         _lib = L
     return _lib
 
@@ -152,6 +157,19 @@ class Oracle:
         m = np.zeros(P, np.float64)
         self.L.orc_render_simp_f32(self.h, _ptr(w), _ptr(c), _ptr(m))
         return w, c, m
+
+    def render_depth(self):
+        """render_depth outputs: points (3,H,W), depth, final T, index (int32) and
+        discriminant (H,W each); pixels are zero until written."""
+        H, W = self.H, self.W
+        pts = np.zeros((3, H, W), np.float32)
+        depth = np.zeros((H, W), np.float32)
+        alpha = np.zeros((H, W), np.float32)
+        gidx = np.zeros((H, W), np.int32)
+        disc = np.zeros((H, W), np.float32)
+        self.L.orc_render_depth_f32(self.h, _ptr(pts), _ptr(depth), _ptr(alpha), _ptr(gidx),
+                                    _ptr(disc))
+        return pts, depth, alpha, gidx, disc
 
     def grads(self) -> dict:
         P = self.P
