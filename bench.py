@@ -133,6 +133,7 @@ def main():
     graph_fps = graph_render_fps(settings, params, sched, args)
     adam = sparse_adam_step(rast, params)
     ssim = fused_ssim_step(cam)
+    lists = list_passes_step(settings, params, args)
     loop = render_loop(args) if args.render_loop else None
     torch.cuda.empty_cache()
     mv = multiview_step(args) if args.multiview else None
@@ -169,6 +170,7 @@ def main():
         "stages": stages,
         "sparse_adam": adam,
         "fused_ssim": ssim,
+        "list_passes": lists,
         "render_loop": loop,
         "multiview": mv,
         "learned_scheduler": sched_eval,
@@ -306,6 +308,33 @@ def fused_ssim_step(cam, reps=20):
     gbs = (n * 4 * (2 + 4 + 6 + 1)) / (ms * 1e-3) / 1e9
     return {"image": [3, cam.height, cam.width], "ms_fwd_bwd": round(ms, 4),
             "min_traffic_gbs": round(gbs, 1)}
+
+
+def list_passes_step(settings, params, args, reps=10):
+    """render_simp and render_depth over the bench frame's lists (the
+    importance-score and depth-reinitialisation passes of the Mini-Splatting
+    training loops), ms per call on the fast schedule's frame."""
+    import torch
+    from fastergs_amd._lib import SCHEDULE_FAST
+    from fastergs_amd.rasterizer import Frame, RasterOptions
+    with torch.no_grad():
+        fr = Frame(settings, params["means3D"].detach(), params["opacities"].detach(),
+                   params["scales"].detach(), params["rotations"].detach(),
+                   dc=params["dc"].detach(), shs=params["shs"].detach(),
+                   options=RasterOptions(schedule=SCHEDULE_FAST,
+                                         bucket_size=args.bucket)).forward()
+        out = {"pairs": fr.R}
+        for name, fn in (("render_simp", fr.render_simp), ("render_depth", fr.render_depth)):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            out[name + "_ms"] = round(e0.elapsed_time(e1) / reps, 4)
+    return out
 
 
 def sparse_adam_step(rast, params, reps=20):

@@ -421,11 +421,19 @@ hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
 // one 256-thread workgroup per tile, one pixel per thread, splats staged 256
 // at a time in LDS. Pixels blend front to back without stopping on their
 // own; after each pair of splats (a leading single splat when the list
-// length is odd) the workgroup stops once all 256 pixels of the tile -- those
-// outside the image keep T = 1 -- have T < 1e-4. Per splat and wave the
-// weights alpha*T (DPP sum) and the number of pixels with alpha >= 1/255
-// (ballot) go to the splat with one atomic each; per pixel the splat of
-// largest weight (the first of equals) gets +1 at the end.
+// length is odd) the tile stops once all 256 pixels of the tile -- those
+// outside the image keep T = 1 -- have T < 1e-4. Per splat the weights
+// alpha*T and the number of pixels with alpha >= 1/255 are summed; per pixel
+// the splat of largest weight (the first of equals) gets +1 at the end.
+//
+// No barrier per pair: T only decreases, so each wave notes the first pair
+// end at which all its 64 pixels are below 1e-4, and the tile's stop point
+// is the largest of the four. Waves run through the whole batch; the
+// per-splat sums (DPP wave sum, ballot count) go to LDS with LDS atomics and
+// are flushed to memory once per batch, only for splats up to the stop
+// point. Past the stop point every weight is < 1e-4 while each pixel's
+// largest weight is >= 1/255 (its first contributor meets T = 1), so the
+// per-pixel maxima need no such cut.
 __global__ __launch_bounds__(256) void k_render_simp(int W, int H, int grid_x,
                                                      const uint32_t* __restrict__ ranges,
                                                      const uint32_t* __restrict__ vals,
@@ -436,9 +444,13 @@ __global__ __launch_bounds__(256) void k_render_simp(int W, int H, int grid_x,
   __shared__ float4 sA[256];
   __shared__ float2 sB[256];
   __shared__ uint32_t sId[256];
+  __shared__ float sW[256];
+  __shared__ uint32_t sC[256];
+  __shared__ uint32_t sStop[4];
   const int tile = blockIdx.x;
   const int tx = tile % grid_x, ty = tile / grid_x;
   const int t = threadIdx.x;
+  const int wave = t >> 6;
   const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
@@ -448,10 +460,9 @@ __global__ __launch_bounds__(256) void k_render_simp(int W, int H, int grid_x,
   uint32_t imax = 0;
   bool flag = false;
   const int lane = lane_id();
-  bool done = false;
-  for (uint32_t base = 0; base < len && !done; base += 256) {
+  uint32_t wave_stop = 0xffffffffu;  // first pair end with all 64 pixels below the cut
+  for (uint32_t base = 0; base < len; base += 256) {
     const uint32_t n = min(256u, len - base);
-    __syncthreads();
     if (t < (int)n) {
       const uint32_t id = vals[start + base + t];
       const float4* r = (const float4*)(splat + 12 * (size_t)id);
@@ -460,9 +471,10 @@ __global__ __launch_bounds__(256) void k_render_simp(int W, int H, int grid_x,
       sB[t] = make_float2(r1.x, r1.y);
       sId[t] = id;
     }
+    sW[t] = 0.0f;
+    sC[t] = 0u;
     __syncthreads();
     for (uint32_t j = 0; j < n; ++j) {
-      const uint32_t id = sId[j];
       float w = 0.0f;
       bool hit = false;
       if (inside) {
@@ -475,7 +487,7 @@ __global__ __launch_bounds__(256) void k_render_simp(int W, int H, int grid_x,
           hit = true;
           if (wmax < w) {
             wmax = w;
-            imax = id;
+            imax = sId[j];
             flag = true;
           }
         }
@@ -484,23 +496,37 @@ __global__ __launch_bounds__(256) void k_render_simp(int W, int H, int grid_x,
       if (hits) {
         const float ws = wave_sum(w);
         if (lane == 0) {
-          atomic_add_f(&wsum[id], ws);
-          atomicAdd(&wcount[id], (int32_t)__popcll(hits));
+          atomicAdd(&sW[j], ws);
+          atomicAdd(&sC[j], (uint32_t)__popcll(hits));
         }
       }
       // pairs follow the leading single of an odd-length list: position pos
       // ends a pair when it is odd (even length) or even and >= 2 (odd length)
       const uint32_t pos = base + j;
       const bool pair_end = (len & 1u) ? (pos >= 2u && (pos & 1u) == 0u) : ((pos & 1u) == 1u);
-      if (pair_end) {
-        if (__syncthreads_count(T < T_CUT_F) == 256) {
-          done = true;
-          break;
-        }
-      }
+      if (pair_end && wave_stop == 0xffffffffu && __ballot(T < T_CUT_F) == ~0ull) wave_stop = pos;
     }
+    if (lane == 0) sStop[wave] = wave_stop;
+    __syncthreads();
+    const uint32_t stop = max(max(sStop[0], sStop[1]), max(sStop[2], sStop[3]));
+    if (t < (int)n && base + t <= stop && sC[t] != 0u) {
+      atomic_add_f(&wsum[sId[t]], sW[t]);
+      atomicAdd(&wcount[sId[t]], (int32_t)sC[t]);
+    }
+    if (stop != 0xffffffffu) break;
+    __syncthreads();
   }
-  if (inside && flag) atomic_add_f(&maxcount[imax], 1.0f);
+  // one atomic per distinct maximum in the wave (neighbouring pixels mostly
+  // share it; the counts are integers, so the float sums stay exact)
+  const bool has = inside && flag;
+  uint64_t pending = __ballot(has);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)imax, leader);
+    const uint64_t same = __ballot(has && imax == k) & pending;
+    if (lane == leader) atomic_add_f(&maxcount[k], (float)__popcll(same));
+    pending &= ~same;
+  }
 }
 
 hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
