@@ -204,10 +204,19 @@ def kernel_units(name, P, M, V, R, T, E, fast):
         # opacity 4, colour 12, mean3D 12, cov3D 24, scale 12, rotation 16
         "gauss_bwd_geom": ("hbm", P * (48 + 4 + 4 + 12 + 12 + 16 + 4 + 16)
                            + P * (16 + 16 + 4 + 12 + 12 + 24 + 12 + 16)),
-        # key-only scatter of the tile sort: read + write 8 B per pair; the
+        # key-only pass of the tile sort: read + write 8 B per pair; the
         # last pass writes the 4 B list entry and the 4 B tile id instead
+        "onesweep_k": ("hbm", 16 * R),
+        "onesweep_k_last": ("hbm", 16 * R),
         "radix_scatter_k": ("hbm", 16 * R),
         "radix_scatter_k_last": ("hbm", 16 * R),
+        # tile sort: 16-bit tile ids with 32-bit records; the histogram
+        # reads the 2 B ids, each pass reads and writes 6 B per pair
+        "tile_hist": ("hbm", 2 * R),
+        "tile_scatter": ("hbm", 12 * R),
+        # depth sort of the records: 4 B key + 4 B index, read + write
+        "depth_onesweep": ("hbm", 16 * P),
+        "depth_hist_all": ("hbm", 4 * P),
         # sorted packed keys -> lists (4 B) and ranges
         "unpack_ranges": ("hbm", 12 * R + 8 * T),
         # ~70 flops per active (pixel, splat) in the reverse step (fgs_math.h pix_bwd)
@@ -216,6 +225,9 @@ def kernel_units(name, P, M, V, R, T, E, fast):
     }
     # 64-bit key + 32-bit value, read + write: the pair sort (plain) or the
     # Gaussian depth sort (fast); histograms read the 8 B keys
+    table["onesweep_kv"] = ("hbm", 24 * (P if fast else R))
+    table["sort_hist_kv"] = ("hbm", 8 * (P if fast else R))
+    table["sort_hist_k"] = ("hbm", 8 * R)
     table["radix_scatter_kv"] = ("hbm", 24 * (P if fast else R))
     table["radix_hist_kv"] = ("hbm", 8 * (P if fast else R))
     table["radix_hist_k"] = ("hbm", 8 * R)
@@ -449,10 +461,19 @@ def render_loop(args, frames=120):
                    p["rotations"], dc=p["dc"], shs=p["shs"], options=opts).forward()
         prof = _lib.profile_read()
         _lib.profile_enable(False)
-        passes = sum(prof[n][1] for n in ("radix_scatter_k", "radix_scatter_k_last") if n in prof)
-        sort_ms += sum(prof[n][0] for n in ("radix_hist_k", "radix_scatter_k",
-                                            "radix_scatter_k_last") if n in prof)
-        sort_bytes += passes * 24 * fr.R
+        tile_sort = ("tile_hist", "tile_scatter", "tile_hist_all", "tile_onesweep")
+        if any(n in prof for n in tile_sort):
+            # 16-bit tile ids + 32-bit records: 2 B histogram read, 12 B per pass
+            passes = sum(prof[n][1] for n in ("tile_scatter", "tile_onesweep") if n in prof)
+            hists = sum(prof[n][1] for n in ("tile_hist", "tile_hist_all") if n in prof)
+            sort_ms += sum(prof[n][0] for n in tile_sort if n in prof)
+            sort_bytes += (2 * hists + 12 * passes) * fr.R
+        else:
+            radix = ("sort_hist_k", "onesweep_k", "onesweep_k_last", "radix_hist_k",
+                     "radix_scatter_k", "radix_scatter_k_last")
+            passes = sum(prof[n][1] for n in radix[1:3] + radix[4:] if n in prof)
+            sort_ms += sum(prof[n][0] for n in radix if n in prof)
+            sort_bytes += (8 + passes * 16) * fr.R
         Rs.append(fr.R)
         del fr
     torch.cuda.empty_cache()

@@ -275,7 +275,7 @@ size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
                         const fgs_options* opt) {
   (void)P;
   const bool ck = opt && opt->schedule == FGS_SCHEDULE_FAST;
-  return carve_pair(nullptr, R, NB, (int64_t)width * height, ck).total;
+  return carve_pair(nullptr, R, NB, (int64_t)width * height, tiles_of(width, height), ck).total;
 }
 
 // raw sums (P x 12) and the split SH pass's mean term (P x 4)
@@ -294,7 +294,7 @@ size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t 
   (void)opt;
   Batch b;
   if (make_batch(batch, b) != FGS_OK) return 0;
-  return carve_pair(nullptr, R, NB, b.HW, true).total;
+  return carve_pair(nullptr, R, NB, b.HW, b.T, true).total;
 }
 
 size_t fgs_batch_grad_ws_size(int P, int n_views) {
@@ -323,7 +323,9 @@ int fgs_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t*
   if (ws_bytes < sort_ws_bytes(n)) return fail(FGS_ERR_CAPACITY, "sort workspace too small");
   hipStream_t s = (hipStream_t)stream;
   bool alt = false;
-  FGS_TRY(sort_pairs(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, alt), "sort");
+  FGS_TRY(sort_kv<uint64_t>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, alt,
+                            nullptr, nullptr, SORT_PAIRS),
+          "sort");
   if (alt) {
     FGS_TRY(hipMemcpyAsync(keys, keys_alt, n * 8, hipMemcpyDeviceToDevice, s), "sort copy");
     if (vals) FGS_TRY(hipMemcpyAsync(vals, vals_alt, n * 4, hipMemcpyDeviceToDevice, s), "sort copy");
@@ -377,13 +379,14 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   FGS_TRY(scan_u32(w.lflag, w.lpos, PV, w.counts + 3, w.scratch, s), "large scan");
   FGS_TRY(launch_compact_large((int)PV, w.lflag, w.lpos, w.large_ids, s), "large compact");
   FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
-                             w.counts, w.pcount, nullptr, nullptr, 0u, nullptr, s),
+                             w.counts, w.pcount, nullptr, nullptr, 0u, nullptr, nullptr, false, s),
           "large tiles");
   // records in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by (global) tile alone yields the canonical per-tile
   // order of every view
   bool alt = false;
-  FGS_TRY(sort_pairs(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, PV, 0, 32, w.scratch, s, alt),
+  FGS_TRY(sort_kv<uint32_t>(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, PV, 0, 32, w.scratch,
+                            s, alt, nullptr, nullptr, SORT_DEPTH),
           "depth sort");
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
   FGS_TRY(launch_gather_scatter(true, (int)PV, w.dsort_v0, w.pcount, w.rcount, s), "gather");
@@ -425,7 +428,7 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   const GaussWs gw = carve_gauss(gauss_ws, (int64_t)g->P * b.V, T);
   if (!gauss_ws || gauss_ws_bytes < gw.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
   if (!pair_ws || pair_ws_bytes < pw.total) return fail(FGS_ERR_CAPACITY, "pair workspace too small");
   hipStream_t s = (hipStream_t)stream;
   const fgs_camera* cam = &b.cams[0];
@@ -462,20 +465,26 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
     // the device, so nothing here waits for the host.
     FGS_TRY(fill_u32(gw.lcur, 0u, (int64_t)g->P * b.V, s), "zero cursors");
     FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.poff, (uint32_t)R,
-                              pw.k0, s),
+                              pw.t0, pw.r0, pw.tile16, s),
             "emit");
     FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts, nullptr,
-                               gw.poff, gw.lcur, (uint32_t)R, pw.k0, s),
+                               gw.poff, gw.lcur, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s),
             "emit large");
-    if (R > 1) {
-      // the last pass writes the lists (low halves) and tile ids (high halves)
+    // stable sort of (tile id, record) by tile id: the last pass writes the
+    // per-tile lists; ranges by binary search over the sorted tile ids
+    {
+      const int tb = tile_bits(T - 1) > 0 ? tile_bits(T - 1) : 1;
       bool alt = false;
-      FGS_TRY(sort_pairs(pw.k0, nullptr, pw.k1, nullptr, R, 32, 32 + tile_bits(T), pw.scratch, s,
-                         alt, pw.v0, pw.tiles, gw.counts),
-              "tile sort");
-      FGS_TRY(launch_tile_ranges(R, gw.counts, pw.tiles, gw.ranges, T, s), "ranges");
-    } else {
-      FGS_TRY(launch_unpack_ranges(R, gw.counts, pw.k0, pw.v0, gw.ranges, T, s), "lists and ranges");
+      if (pw.tile16)
+        FGS_TRY(sort_kv<uint16_t>((uint16_t*)pw.t0, pw.r0, (uint16_t*)pw.t1, pw.r1, R, 0, tb,
+                                  pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES),
+                "tile sort");
+      else
+        FGS_TRY(sort_kv<uint32_t>((uint32_t*)pw.t0, pw.r0, (uint32_t*)pw.t1, pw.r1, R, 0, tb,
+                                  pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES),
+                "tile sort");
+      FGS_TRY(launch_tile_ranges(R, gw.counts, alt ? pw.t1 : pw.t0, pw.tile16, gw.ranges, T, s),
+              "ranges");
     }
     FGS_TRY(launch_bucket_counts(T, gw.ranges, opt->bucket_size, gw.bcount, gw.maxc, s),
             "bucket counts");
@@ -512,8 +521,8 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
                                  pw.k0, pw.v0, s),
           "duplicate");
   bool alt = false;
-  FGS_TRY(sort_pairs(pw.k0, pw.v0, pw.k1, pw.v1, R, 0, 32 + tile_bits(T), pw.scratch, s, alt,
-                     nullptr, nullptr, gw.counts),
+  FGS_TRY(sort_kv<uint64_t>(pw.k0, pw.v0, pw.k1, pw.v1, R, 0, 32 + tile_bits(T), pw.scratch, s,
+                            alt, gw.counts, nullptr, SORT_PAIRS),
           "sort");
   if (R > 0 && alt != plain_sorted_in_alt(T)) return fail(FGS_ERR_HIP, "sort parity");
   const uint64_t* keys = alt ? pw.k1 : pw.k0;
@@ -541,7 +550,7 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   const int64_t PV = (int64_t)g->P * b.V;
   const GaussWs gw = carve_gauss(gauss_ws, PV, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
   hipStream_t s = (hipStream_t)stream;
   float* raw = (float*)grad_ws;
   FGS_TRY(fill_f32(raw, 0.0f, PV * RAW_STRIDE, s), "zero raw");
@@ -629,7 +638,7 @@ int fgs_batch_frame_view_get(void* gauss_ws, void* pair_ws, int P, const fgs_bat
   const int T = b.T;
   const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
   *v = fgs_frame_view{};
   v->geom = gw.geom;
   v->splat = gw.splat;
@@ -731,7 +740,7 @@ int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height,
   const int T = tiles_of(width, height);
   const GaussWs gw = carve_gauss(gauss_ws, P, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast);
   const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);
   hipStream_t s = (hipStream_t)stream;
   FGS_TRY(fill_f32(accum_weights, 0.0f, P, s), "zero weights");
@@ -759,7 +768,7 @@ int fgs_render_depth(void* gauss_ws, void* pair_ws, int P, int width, int height
   const int T = tiles_of(width, height);
   const GaussWs gw = carve_gauss(gauss_ws, P, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast);
   const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);
   FGS_TRY(launch_render_depth(width, height, T, gw.ranges, vals, gw.splat, means3D, scales,
                               rotations, projmatrix, campos, out_pts, out_depth, accum_alpha,

@@ -70,15 +70,18 @@ __device__ __forceinline__ int view_of_tile(const ViewRec* views, int V, uint32_
 // scan_sort.hip
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
                     hipStream_t s);
-// last_lo / last_hi (key-only sorts): the final pass writes the low / high
-// 32 bits of the sorted keys there instead of a key array. n_dev: when
-// given, the element count is min(*n_dev, n) read on the device (n is then
-// the capacity the grid is sized for); no host synchronisation either way.
-hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
-                      int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
-                      uint32_t* last_lo = nullptr, uint32_t* last_hi = nullptr,
-                      const uint32_t* n_dev = nullptr);
-
+// Stable LSD radix sort of (key, value) pairs on key bits [begin_bit,
+// end_bit) (keys of 16, 32 or 64 bits; vals may be NULL for key-only sorts).
+// Passes alternate between the arrays and their *_alt twins; in_alt tells
+// where the keys ended up. last_vals: when given, the final pass writes the
+// values there. n_dev: when given, the element count is min(*n_dev, n) read
+// on the device (n is then the capacity the grids are sized for); no host
+// synchronisation either way.
+enum SortKind { SORT_PAIRS = 0, SORT_DEPTH = 1, SORT_TILES = 2 };
+template <typename K>
+hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
+                   int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
+                   const uint32_t* n_dev, uint32_t* last_vals, SortKind kind);
 // fill.hip-style helpers (render.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 hipError_t fill_f32(float* p, float v, int64_t n, hipStream_t s);
@@ -95,7 +98,7 @@ struct PreArgs {
   int P;
   int split_tiles;
   uint32_t* lflag;     // fast: 1 for Gaussians whose rect is walked by whole workgroups
-  uint64_t* dsort_k;   // fast: depth keys for the Gaussian depth sort
+  uint32_t* dsort_k;   // fast: depth keys for the Gaussian depth sort
   uint32_t* dsort_v;   // fast: Gaussian indices for the depth sort
   uint32_t* counts;
   const ViewRec* views;  // fast: V views (records are V x P)
@@ -108,14 +111,16 @@ hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
                                   uint32_t cap, uint64_t* keys, uint32_t* vals, hipStream_t s);
-// FAST emission over V views of P Gaussians (records v * P + i); pair
-// values are record indices, tile ids global.
+// FAST emission over V views of P Gaussians (records v * P + i) as (tile
+// id, record) into two arrays: tile ids are global, 16-bit when t16 (at
+// most 65536 tiles), else 32-bit.
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t* lcur, uint32_t cap,
-                              uint64_t* packed, hipStream_t s);
+                              const uint32_t* poff, uint32_t* lcur, uint32_t cap, void* tkeys,
+                              uint32_t* recs, bool t16, hipStream_t s);
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
-                             const uint32_t* poff, uint32_t cap, uint64_t* packed, hipStream_t s);
+                             const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
+                             bool t16, hipStream_t s);
 hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lpos,
                                 uint32_t* large_ids, hipStream_t s);
 hipError_t launch_gather_scatter(bool gather, int P, const uint32_t* order, const uint32_t* src,
@@ -128,9 +133,7 @@ hipError_t launch_identify_ranges(int64_t R, const uint32_t* n_dev, const uint64
 hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, const uint32_t* boff,
                                 int bucket, uint32_t* bucket_ranges, uint32_t* bucket_tile,
                                 hipStream_t s);
-hipError_t launch_unpack_ranges(int64_t R, const uint32_t* n_dev, const uint64_t* keys,
-                                uint32_t* vals, uint32_t* ranges, int T, hipStream_t s);
-hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const uint32_t* tiles,
+hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tiles, bool t16,
                               uint32_t* ranges, int T, hipStream_t s);
 hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
                                 uint32_t* maxc, hipStream_t s);

@@ -39,12 +39,6 @@ inline size_t scan_ws_bytes(int64_t n) {
   const size_t nb = (size_t)((n + SCAN_TILE - 1) / SCAN_TILE);
   return align_up((nb + 1) * sizeof(uint32_t));
 }
-inline size_t sort_ws_bytes(int64_t n) {
-  const size_t nb = (size_t)((n + SORT_TILE - 1) / SORT_TILE);
-  const size_t hist = 256 * (nb > 0 ? nb : 1);
-  return align_up(hist * sizeof(uint32_t)) + scan_ws_bytes((int64_t)hist) + ALIGN;
-}
-
 struct Carver {
   char* base;
   size_t off = 0;
@@ -57,14 +51,52 @@ struct Carver {
   }
 };
 
+// Onesweep radix sort (inputs up to ONESWEEP_MAX_KEYS): digit histograms of
+// up to SORT_MAX_PASSES passes, tile counters + fault flag, and two status
+// arrays of 256 words per SORT_TILE keys (alternating between passes).
+constexpr int SORT_MAX_PASSES = 8;
+constexpr int64_t ONESWEEP_MAX_KEYS = int64_t(1) << 22;
+struct SortWs {
+  uint32_t* hist;
+  uint32_t* counters;
+  uint32_t* status[2];
+};
+template <typename C>
+inline SortWs carve_sort(C& c, int64_t nb) {
+  SortWs w;
+  w.hist = c.template take<uint32_t>(SORT_MAX_PASSES * 256);
+  w.counters = c.template take<uint32_t>(SORT_MAX_PASSES + 1);
+  w.status[0] = c.template take<uint32_t>((size_t)nb * 256);
+  w.status[1] = c.template take<uint32_t>((size_t)nb * 256);
+  return w;
+}
+
+inline size_t sort_ws_bytes(int64_t n) {
+  const size_t nb = (size_t)((n + SORT_TILE - 1) / SORT_TILE);
+  const size_t rows = nb > 0 ? nb : 1;
+  // classic passes: digit-major histogram + its scan
+  const size_t hist = 256 * rows;
+  const size_t classic = align_up(hist * sizeof(uint32_t)) + scan_ws_bytes((int64_t)hist);
+  // onesweep passes
+  const size_t onesweep = align_up(SORT_MAX_PASSES * 256 * sizeof(uint32_t)) +
+                          align_up((SORT_MAX_PASSES + 1) * sizeof(uint32_t)) +
+                          2 * align_up(rows * 256 * sizeof(uint32_t));
+  return (classic > onesweep ? classic : onesweep) + ALIGN;
+}
+
+inline SortWs carve_sort(void* ws, int64_t nb) {
+  Carver c(ws);
+  return carve_sort(c, nb);
+}
+
 struct GaussWs {
   float* geom;
   float* splat;
   uint32_t* dkey;
   uint32_t* pcount;
   uint32_t* poff;
-  uint64_t* dsort_k0;  // fast: depth keys, sorted in place
-  uint64_t* dsort_k1;
+  uint32_t* dsort_k0;  // fast: depth keys, sorted in place
+  uint32_t* dsort_k1;
   uint32_t* dsort_v0;  // fast: Gaussian index by depth rank after the sort
   uint32_t* dsort_v1;
   uint32_t* lflag;     // fast: rect walked by whole workgroups (1) or lane-packed (0)
@@ -101,8 +133,8 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   w.dkey = c.take<uint32_t>(p);
   w.pcount = c.take<uint32_t>(p);
   w.poff = c.take<uint32_t>(p + 1);
-  w.dsort_k0 = c.take<uint64_t>(p);
-  w.dsort_k1 = c.take<uint64_t>(p);
+  w.dsort_k0 = c.take<uint32_t>(p);
+  w.dsort_k1 = c.take<uint32_t>(p);
   w.dsort_v0 = c.take<uint32_t>(p);
   w.dsort_v1 = c.take<uint32_t>(p);
   w.lflag = c.take<uint32_t>(p);
@@ -133,11 +165,15 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
 }
 
 struct PairWs {
-  uint64_t* k0;  // plain: keys; fast: packed tile<<32 | index
+  uint64_t* k0;  // plain: keys (tile << 32 | depth bits)
   uint64_t* k1;
-  uint32_t* v0;  // sorted per-tile Gaussian lists (fast), values (plain)
+  uint32_t* v0;  // plain: values; fast: the sorted per-tile Gaussian lists
   uint32_t* v1;
-  uint32_t* tiles;  // fast: tile id of each sorted pair
+  void* t0;      // fast: tile id of each pair (16-bit up to 65536 tiles, else 32-bit)
+  void* t1;
+  uint32_t* r0;  // fast: record of each pair, emission order
+  uint32_t* r1;
+  bool tile16;   // fast: tile ids are 16-bit
   void* scratch;
   size_t scratch_bytes;
   uint32_t* bucket_ranges;
@@ -147,16 +183,26 @@ struct PairWs {
   size_t total;
 };
 
-inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int64_t HW, bool fast) {
+// T: tiles of all views (the width of the fast schedule's tile ids).
+inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int64_t HW, int64_t T, bool fast) {
   Carver c(base);
-  PairWs w;
+  PairWs w{};
   const size_t r = (size_t)(R > 0 ? R : 1);
   const size_t nb = (size_t)(NB > 0 ? NB : 1);
-  w.k0 = c.take<uint64_t>(r);
-  w.k1 = c.take<uint64_t>(r);
-  w.v0 = c.take<uint32_t>(r);
-  w.v1 = fast ? nullptr : c.take<uint32_t>(r);
-  w.tiles = fast ? c.take<uint32_t>(r) : nullptr;
+  if (fast) {
+    w.tile16 = T <= 65536;
+    const size_t tb = w.tile16 ? 2 : 4;
+    w.t0 = c.take<char>(r * tb);
+    w.t1 = c.take<char>(r * tb);
+    w.r0 = c.take<uint32_t>(r);
+    w.r1 = c.take<uint32_t>(r);
+    w.v0 = c.take<uint32_t>(r);
+  } else {
+    w.k0 = c.take<uint64_t>(r);
+    w.k1 = c.take<uint64_t>(r);
+    w.v0 = c.take<uint32_t>(r);
+    w.v1 = c.take<uint32_t>(r);
+  }
   w.scratch_bytes = sort_ws_bytes((int64_t)r);
   w.scratch = c.take<char>(w.scratch_bytes);
   if (fast) {

@@ -144,6 +144,16 @@ struct WaveTiles {
   uint32_t incl[WAVE], area[WAVE], cnt[WAVE];
 };
 
+// One emitted pair: its (global) tile id, 16- or 32-bit, and its record.
+__device__ __forceinline__ void store_pair(void* tkeys, uint32_t* recs, bool t16, uint32_t pos,
+                                           uint32_t tile, uint32_t rec) {
+  if (t16)
+    ((uint16_t*)tkeys)[pos] = (uint16_t)tile;
+  else
+    ((uint32_t*)tkeys)[pos] = tile;
+  recs[pos] = rec;
+}
+
 // Walk the concatenated small rects of one wave 64 candidate tiles per step
 // (lane-packed); count (EMIT=false) or emit (EMIT=true) the pairs that pass
 // the exact tile test. Within a Gaussian's block of the pair array the order
@@ -151,7 +161,8 @@ struct WaveTiles {
 template <bool EMIT>
 __device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, uint32_t gbase, int gx,
                                            uint32_t toff, const uint32_t* __restrict__ poff,
-                                           uint32_t cap, uint64_t* __restrict__ packed) {
+                                           uint32_t cap, void* __restrict__ tkeys,
+                                           uint32_t* __restrict__ recs, bool t16) {
   const int lane = lane_id();
   for (uint32_t base = 0; base < S; base += WAVE) {
     const uint32_t f = base + lane;
@@ -171,7 +182,7 @@ __device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, uint32_t gb
         if (EMIT) {
           const uint32_t g = gbase + (uint32_t)j;
           const uint32_t pos = poff[g] + k;
-          if (pos < cap) packed[pos] = ((uint64_t)(toff + (uint32_t)(ty * gx + tx)) << 32) | g;
+          if (pos < cap) store_pair(tkeys, recs, t16, pos, toff + (uint32_t)(ty * gx + tx), g);
         }
       }
     }
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a) {
   stage_small(L, lane, small, o.x, o.y, o.conic, o.power, o.rect[0], o.rect[1], o.rect[2],
               o.rect[3], S);
   __syncthreads();
-  walk_small<false>(L, S, 0u, cam.grid_x, 0u, nullptr, 0u, nullptr);
+  walk_small<false>(L, S, 0u, cam.grid_x, 0u, nullptr, 0u, nullptr, nullptr, false);
   __syncthreads();
   if (i < a.P) a.pcount[ri] = L.cnt[lane];
 }
@@ -308,7 +319,7 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
                                                      uint32_t* pcount,
                                                      const uint32_t* __restrict__ poff,
                                                      uint32_t* lcur, uint32_t cap,
-                                                     uint64_t* packed) {
+                                                     void* tkeys, uint32_t* recs, int t16) {
   const uint32_t NL = counts[3];
   const int lane = lane_id();
   const int w = threadIdx.x / WAVE;
@@ -336,7 +347,7 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
         slot0 = __shfl(slot0, 0, WAVE);
         const uint32_t pos = poff[g] + slot0 + lanes_below(m);
         if (hit && pos < cap)
-          packed[pos] = ((uint64_t)(toff + (uint32_t)((r.r1 + ty) * gx + r.r0 + tx)) << 32) | g;
+          store_pair(tkeys, recs, t16 != 0, pos, toff + (uint32_t)((r.r1 + ty) * gx + r.r0 + tx), g);
       }
       l += 4 * WAVE;
       tx += sx;
@@ -353,17 +364,19 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
                               const uint32_t* poff, uint32_t* lcur, uint32_t cap,
-                              uint64_t* packed, hipStream_t s) {
+                              void* tkeys, uint32_t* recs, bool t16, hipStream_t s) {
   if (P <= 0) return hipSuccess;
   const int blocks = 2048;  // persistent workgroups; each exits past the device-side count
   if (emit)
     FGS_TIMED("large_tiles_emit", s,
               (k_large_tiles<true><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
-                                                          pcount, poff, lcur, cap, packed)));
+                                                          pcount, poff, lcur, cap, tkeys, recs,
+                                                          t16 ? 1 : 0)));
   else
     FGS_TIMED("large_tiles_count", s,
               (k_large_tiles<false><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
-                                                           pcount, poff, lcur, cap, packed)));
+                                                           pcount, poff, lcur, cap, tkeys, recs,
+                                                           t16 ? 1 : 0)));
   return hipGetLastError();
 }
 
@@ -372,7 +385,8 @@ hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, con
 __global__ __launch_bounds__(256) void k_emit_small(int P, const ViewRec* __restrict__ views,
                                                     const float* __restrict__ geom, int split,
                                                     const uint32_t* __restrict__ poff, uint32_t cap,
-                                                    uint64_t* __restrict__ packed) {
+                                                    void* __restrict__ tkeys,
+                                                    uint32_t* __restrict__ recs, int t16) {
   __shared__ WaveTiles lds[4];
   const int v = blockIdx.y;
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -389,15 +403,17 @@ __global__ __launch_bounds__(256) void k_emit_small(int P, const ViewRec* __rest
   uint32_t S;
   stage_small(L, lane, small, r.x, r.y, r.conic, r.power, r.r0, r.r1, r.r2, r.r3, S);
   __syncthreads();
-  walk_small<true>(L, S, gbase, views[v].grid_x, views[v].tile_off, poff, cap, packed);
+  walk_small<true>(L, S, gbase, views[v].grid_x, views[v].tile_off, poff, cap, tkeys, recs,
+                   t16 != 0);
 }
 
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
-                             const uint32_t* poff, uint32_t cap, uint64_t* packed, hipStream_t s) {
+                             const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
+                             bool t16, hipStream_t s) {
   if (P > 0)
     FGS_TIMED("emit_small", s,
-              (k_emit_small<<<dim3((P + 255) / 256, (unsigned)V), 256, 0, s>>>(P, views, geom, split,
-                                                                               poff, cap, packed)));
+              (k_emit_small<<<dim3((P + 255) / 256, (unsigned)V), 256, 0, s>>>(
+                  P, views, geom, split, poff, cap, tkeys, recs, t16 ? 1 : 0)));
   return hipGetLastError();
 }
 

@@ -100,57 +100,41 @@ hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, cons
   return hipGetLastError();
 }
 
-// Sorted packed keys -> per-tile lists and tile ranges in one pass.
-__global__ void k_unpack_ranges(int64_t R_cap, const uint32_t* __restrict__ n_dev,
-                                const uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                uint32_t* __restrict__ ranges) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t R = dev_count(n_dev, R_cap);
-  if (idx >= R) return;
-  const uint64_t key = keys[idx];
-  vals[idx] = (uint32_t)key;
-  const uint32_t cur = (uint32_t)(key >> 32);
-  if (idx == 0 || (uint32_t)(keys[idx - 1] >> 32) != cur) ranges[2 * cur] = (uint32_t)idx;
-  if (idx == R - 1 || (uint32_t)(keys[idx + 1] >> 32) != cur) ranges[2 * cur + 1] = (uint32_t)(idx + 1);
-}
-hipError_t launch_unpack_ranges(int64_t R, const uint32_t* n_dev, const uint64_t* keys,
-                                uint32_t* vals, uint32_t* ranges, int T, hipStream_t s) {
-  hipError_t e = fill_u32(ranges, 0u, 2 * (int64_t)T, s);
-  if (e != hipSuccess) return e;
-  if (R > 0)
-    FGS_TIMED("unpack_ranges", s,
-              (k_unpack_ranges<<<(unsigned)((R + 255) / 256), 256, 0, s>>>(R, n_dev, keys, vals,
-                                                                           ranges)));
-  return hipGetLastError();
-}
-
-// Ranges from the sorted tile ids by binary search (one thread per tile):
-// ranges[t] = [lower_bound(t), lower_bound(t + 1)).
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t v) {
+// Ranges from the sorted tile ids (16- or 32-bit) by binary search, one
+// thread per tile: ranges[t] = [lower_bound(t), lower_bound(t + 1)).
+template <typename TK>
+__device__ __forceinline__ uint32_t lower_bound_tile(const TK* a, uint32_t n, uint32_t v) {
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1; else hi = mid;
+    if ((uint32_t)a[mid] < v) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
+template <typename TK>
 __global__ void k_tile_ranges(int64_t R_cap, const uint32_t* __restrict__ n_dev,
-                              const uint32_t* __restrict__ tiles, uint32_t* __restrict__ ranges,
-                              int T) {
+                              const TK* __restrict__ tiles, uint32_t* __restrict__ ranges, int T) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= T) return;
   const uint32_t R = (uint32_t)dev_count(n_dev, R_cap);
-  const uint32_t lo = lower_bound_u32(tiles, R, (uint32_t)t);
-  const uint32_t hi = lower_bound_u32(tiles, R, (uint32_t)t + 1);
+  const uint32_t lo = lower_bound_tile(tiles, R, (uint32_t)t);
+  const uint32_t hi = lower_bound_tile(tiles, R, (uint32_t)t + 1);
   // empty tiles keep [0, 0) like identifyTileRanges (render.cu:406-411)
   ranges[2 * t] = lo < hi ? lo : 0u;
   ranges[2 * t + 1] = lo < hi ? hi : 0u;
 }
-hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const uint32_t* tiles,
+hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tiles, bool t16,
                               uint32_t* ranges, int T, hipStream_t s) {
-  if (T > 0)
+  if (T <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((T + 255) / 256);
+  if (t16)
     FGS_TIMED("tile_ranges", s,
-              (k_tile_ranges<<<(T + 255) / 256, 256, 0, s>>>(R, n_dev, tiles, ranges, T)));
+              (k_tile_ranges<uint16_t><<<grid, 256, 0, s>>>(R, n_dev, (const uint16_t*)tiles,
+                                                            ranges, T)));
+  else
+    FGS_TIMED("tile_ranges", s,
+              (k_tile_ranges<uint32_t><<<grid, 256, 0, s>>>(R, n_dev, (const uint32_t*)tiles,
+                                                            ranges, T)));
   return hipGetLastError();
 }
 

@@ -1,17 +1,20 @@
-// Device-wide exclusive scan and stable LSD radix sort for gfx950.
+// Device-wide exclusive scan, stable LSD radix sorts and the stable counting
+// sort by tile for gfx950.
 //
 // Scan: reduce-then-scan in three launches (per-tile sums, one-workgroup scan
-// of the sums, per-tile scan with the carried offset). Used for the pair
-// offsets (replacing the reference's warp-aggregated atomics, which give a
-// nondeterministic pair order), per-tile bucket offsets (render.cu:420) and
-// the radix histogram.
+// of the sums, per-tile scan with the carried offset), one launch up to 8192
+// entries. Used for the pair offsets (replacing the reference's
+// warp-aggregated atomics, which give a nondeterministic pair order), the
+// large-rect compaction and per-tile bucket offsets (render.cu:420).
 //
-// Radix sort: 8-bit digits, three launches per pass (per-tile digit
-// histogram in LDS, scan of the digit-major histogram, stable scatter). The
-// scatter ranks equal digits inside a wave64 with eight ballots (a wave-level
-// multisplit) and across the four waves of a workgroup through LDS, so the
-// sort is stable. It replaces cub::DeviceRadixSort::SortPairs on bits
-// [0, 32 + msb(tiles)) (FGS/csrc/cuda_rasterizer/sort.cu:28-44).
+// Radix sort of (key, u32 value) pairs with 16-, 32- or 64-bit keys
+// (replaces cub::DeviceRadixSort::SortPairs, FGS/csrc/cuda_rasterizer/
+// sort.cu:28-44): LSD, 8-bit digits, stable. Up to 4M keys every pass is one
+// onesweep launch after one histogram read for all passes; above that each
+// pass is three launches (per-tile digit histogram, scan, scatter). Keys are
+// as narrow as the data allows: the FAST tile sort moves 16-bit tile ids
+// with 32-bit record values (6 B per pair per pass instead of a packed
+// 8 B key, 2 B per pair for the histograms).
 #include <hip/hip_runtime.h>
 
 #include "fgs_device.h"
@@ -118,156 +121,457 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ radix
-__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
+// ------------------------------------------------------- radix, classic
+// Digit layout: ceil(bits / 8) passes with the key bits spread evenly over
+// them (13 tile-id bits -> 7 + 6), which keeps each digit's run within a
+// 4096-key tile long enough for full-line stores.
+__host__ __device__ inline int sort_passes(int begin_bit, int end_bit) {
+  return (end_bit - begin_bit + 7) / 8;
+}
+__host__ __device__ inline void pass_digits(int begin_bit, int end_bit, int p, int& shift,
+                                            int& bits) {
+  const int total = end_bit - begin_bit, np = (total + 7) / 8;
+  const int w = total / np, extra = total % np;
+  shift = begin_bit + p * w + (p < extra ? p : extra);
+  bits = w + (p < extra ? 1 : 0);
+}
+
+// Up to 16 / sizeof(K) consecutive keys from e (one 16-byte load when all
+// of them are below n and the start is 16-byte aligned, else one by one).
+template <typename K>
+struct KeyVec {
+  static constexpr int N = 16 / sizeof(K);
+  union {
+    uint4 v;
+    K k[N];
+  };
+};
+template <typename K>
+__device__ __forceinline__ int load_keys(const K* __restrict__ keys, int64_t e, int64_t n,
+                                         KeyVec<K>& out) {
+  constexpr int N = KeyVec<K>::N;
+  if (e + N <= n && ((uintptr_t)(keys + e) & 15) == 0) {
+    out.v = *(const uint4*)(keys + e);
+    return N;
+  }
+  int c = 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    out.k[j] = e + j < n ? keys[e + j] : (K)0;
+    c += e + j < n ? 1 : 0;
+  }
+  return c;
+}
+
+template <typename K>
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const K* __restrict__ keys,
                                                              int64_t n_cap,
                                                              const uint32_t* __restrict__ n_dev,
                                                              int shift, uint32_t mask,
                                                              uint32_t* __restrict__ hist, int nb) {
+  constexpr int N = KeyVec<K>::N;
   __shared__ uint32_t cnt[256];
   const int64_t n = dev_count(n_dev, n_cap);
   cnt[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-#pragma unroll 4
-  for (int r = 0; r < SORT_ITEMS; ++r) {
-    const int64_t i = base + r * SORT_THREADS + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & mask], 1u);
+  for (int c = threadIdx.x; c < SORT_TILE / N; c += SORT_THREADS) {
+    const int64_t e = base + (int64_t)c * N;
+    if (e >= n) break;
+    KeyVec<K> kv;
+    const int m = load_keys(keys, e, n, kv);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (j < m) atomicAdd(&cnt[(uint32_t)(kv.k[j] >> shift) & mask], 1u);
   }
   __syncthreads();
   hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Stable scatter of one 4096-element tile: the block first sorts its
-// elements by digit in LDS (stable ranks from wave ballots + per-wave digit
-// counts), then writes each digit's run contiguously to its global offset,
-// so stores are coalesced runs instead of one element per digit per wave.
-// SPLIT (key-only sorts): write the low and high 32-bit halves of each key
-// to two arrays instead of the key (used by the last pass of the tile sort to
-// emit the per-tile lists and the tile ids directly).
-template <bool VALS, bool SPLIT = false>
-__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
-    uint32_t mask, const uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ lo_out = nullptr,
-    uint32_t* __restrict__ hi_out = nullptr) {
+// Stable ranking of one 4096-key tile in LDS: wave w owns the keys
+// [1024 w, 1024 w + 1024) as 16 rows of 64; equal digits in a row are ranked
+// with ballots over the digit bits, and each wave keeps its own running
+// count per digit in LDS (no barrier per row). Afterwards, per digit, the
+// waves' counts become prefixes and the tile's digit offsets an exclusive
+// scan; (wave, row, lane) order is index order, so the ranks are stable.
+// Returns (after a barrier) with wcnt[w][d] = keys of digit d in waves < w,
+// bstart[d] = the tile's keys of digits < d, and rk[r] = rank of key r
+// within its (wave, digit).
+template <typename K>
+__device__ __forceinline__ void rank_tile(const K (&k)[SORT_ITEMS], int nvalid, int shift, int bits,
+                                          uint32_t (&wcnt)[SORT_THREADS / WAVE][256],
+                                          uint32_t* bstart, uint32_t (&rk)[SORT_ITEMS],
+                                          uint32_t& count) {
   constexpr int NW = SORT_THREADS / WAVE;
-  __shared__ uint64_t sk[SORT_TILE];
-  __shared__ uint32_t sv[VALS ? SORT_TILE : 1];
-  __shared__ uint32_t gofs[256];
-  __shared__ uint32_t bstart[256];
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[NW][256];
-  const int t = threadIdx.x;
-  const int w = t / WAVE;
-  const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-  const int64_t n = dev_count(n_dev, n_cap);
-  if (base >= n) return;  // whole workgroup: past the device-side count
-  const int nvalid = (int)min((int64_t)SORT_TILE, n - base);
-  uint64_t k[SORT_ITEMS];
-  uint32_t v[SORT_ITEMS];
+  constexpr int WITEMS = SORT_TILE / NW;
+  const int t = threadIdx.x, w = t / WAVE, lane = lane_id();
+  const uint32_t mask = (1u << bits) - 1u;
 #pragma unroll
   for (int r = 0; r < SORT_ITEMS; ++r) {
-    const int e = r * SORT_THREADS + t;
-    k[r] = e < nvalid ? kin[base + e] : 0ull;
-    v[r] = (VALS && e < nvalid) ? vin[base + e] : 0u;
-  }
-  run[t] = 0;
-#pragma unroll
-  for (int q = 0; q < NW; ++q) wcnt[q][t] = 0;
-  gofs[t] = hist[(size_t)t * nb + blockIdx.x];
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r)
-    if (r * SORT_THREADS + t < nvalid) atomicAdd(&run[(uint32_t)(k[r] >> shift) & mask], 1u);
-  __syncthreads();
-  uint32_t tot;
-  const uint32_t my = run[t];
-  const uint32_t ex = block_excl_scan<SORT_THREADS>(my, tot);
-  bstart[t] = ex;
-  run[t] = 0;
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r) {
-    const bool valid = r * SORT_THREADS + t < nvalid;
+    const bool valid = w * WITEMS + r * WAVE + lane < nvalid;
     const uint32_t d = (uint32_t)(k[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < bits; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
-    const uint32_t wrank = lanes_below(peers);
-    if (valid && wrank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = bstart[d] + run[d] + wrank;
+    const uint32_t below = lanes_below(peers);
+    const uint32_t before = valid ? wcnt[w][d] : 0u;
+    rk[r] = before + below;
+    if (valid && below == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  uint32_t c = 0;
 #pragma unroll
-      for (int q = 0; q < NW; ++q) pos += (q < w) ? wcnt[q][d] : 0u;
+  for (int q = 0; q < NW; ++q) {
+    const uint32_t x = wcnt[q][t];
+    wcnt[q][t] = c;
+    c += x;
+  }
+  uint32_t tot;
+  bstart[t] = block_excl_scan<SORT_THREADS>(c, tot);
+  count = c;
+  __syncthreads();
+}
+
+// Loads this wave's 16 rows (keys and values) of the tile at base.
+template <typename K, bool VALS>
+__device__ __forceinline__ void load_tile(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                          int64_t base, int nvalid, K (&k)[SORT_ITEMS],
+                                          uint32_t (&v)[SORT_ITEMS]) {
+  constexpr int WITEMS = SORT_TILE / (SORT_THREADS / WAVE);
+  const int w = threadIdx.x / WAVE, lane = lane_id();
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    const int e = w * WITEMS + r * WAVE + lane;
+    k[r] = e < nvalid ? kin[base + e] : (K)0;
+    v[r] = (VALS && e < nvalid) ? vin[base + e] : 0u;
+  }
+}
+
+// Places the ranked keys (and values) in LDS in digit order, then writes
+// each digit's run contiguously from gofs[d].
+template <typename K, bool VALS>
+__device__ __forceinline__ void place_and_write(const K (&k)[SORT_ITEMS], const uint32_t (&v)[SORT_ITEMS],
+                                                const uint32_t (&rk)[SORT_ITEMS], int nvalid,
+                                                int shift, uint32_t mask,
+                                                uint32_t (&wcnt)[SORT_THREADS / WAVE][256],
+                                                const uint32_t* bstart, const uint32_t* gofs,
+                                                K* sk, uint32_t* sv, K* __restrict__ kout,
+                                                uint32_t* __restrict__ vout) {
+  constexpr int WITEMS = SORT_TILE / (SORT_THREADS / WAVE);
+  const int t = threadIdx.x, w = t / WAVE, lane = lane_id();
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    if (w * WITEMS + r * WAVE + lane < nvalid) {
+      const uint32_t d = (uint32_t)(k[r] >> shift) & mask;
+      const uint32_t pos = bstart[d] + wcnt[w][d] + rk[r];
       sk[pos] = k[r];
       if (VALS) sv[pos] = v[r];
     }
-    __syncthreads();
-    uint32_t add = 0;
-#pragma unroll
-    for (int q = 0; q < NW; ++q) {
-      add += wcnt[q][t];
-      wcnt[q][t] = 0;
-    }
-    run[t] += add;
-    __syncthreads();
   }
+  __syncthreads();
   for (int e = t; e < nvalid; e += SORT_THREADS) {
-    const uint64_t key = sk[e];
+    const K key = sk[e];
     const uint32_t d = (uint32_t)(key >> shift) & mask;
     const uint32_t dst = gofs[d] + (uint32_t)e - bstart[d];
-    if (SPLIT) {
-      lo_out[dst] = (uint32_t)key;
-      hi_out[dst] = (uint32_t)(key >> 32);
-    } else {
-      kout[dst] = key;
-    }
+    kout[dst] = key;
     if (VALS) vout[dst] = sv[e];
   }
 }
 
-hipError_t sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
-                      int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
-                      bool& in_alt, uint32_t* last_lo, uint32_t* last_hi, const uint32_t* n_dev) {
-  in_alt = false;
-  // with a device-side count the passes always run (the count may be 0 or 1)
-  if ((n <= 1 && !n_dev) || n <= 0 || end_bit <= begin_bit) return hipSuccess;
+// Workgroup b runs on XCD b % 8 (round-robin dispatch over the 8 XCDs):
+// tile of workgroup b such that each XCD takes a contiguous range of tiles,
+// so the digit runs that consecutive tiles write next to each other meet in
+// the same L2 and leave it as whole lines. The grid is 8 * ceil(nb / 8).
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+  const int64_t q = (nb + 7) / 8;
+  return (b % 8) * q + b / 8;
+}
+inline unsigned xcd_grid(int64_t nb) { return (unsigned)(8 * ((nb + 7) / 8)); }
+
+template <typename K, bool VALS>
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
+    const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
+    uint32_t* __restrict__ vout, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
+    int bits, const uint32_t* __restrict__ hist, int nb) {
+  constexpr int NW = SORT_THREADS / WAVE;
+  __shared__ K sk[SORT_TILE];
+  __shared__ uint32_t sv[VALS ? SORT_TILE : 1];
+  __shared__ uint32_t wcnt[NW][256];
+  __shared__ uint32_t bstart[256];
+  __shared__ uint32_t gofs[256];
+  const int t = threadIdx.x;
+  const int64_t tile = xcd_tile(blockIdx.x, nb);
+  const int64_t base = tile * SORT_TILE;
+  const int64_t n = dev_count(n_dev, n_cap);
+  if (tile >= nb || base >= n) return;  // whole workgroup: past the tiles / device-side count
+  const int nvalid = (int)min((int64_t)SORT_TILE, n - base);
+#pragma unroll
+  for (int q = 0; q < NW; ++q) wcnt[q][t] = 0u;
+  gofs[t] = hist[(size_t)t * nb + tile];
+  K k[SORT_ITEMS];
+  uint32_t v[SORT_ITEMS], rk[SORT_ITEMS], cnt;
+  load_tile<K, VALS>(kin, vin, base, nvalid, k, v);
+  __syncthreads();
+  rank_tile<K>(k, nvalid, shift, bits, wcnt, bstart, rk, cnt);
+  place_and_write<K, VALS>(k, v, rk, nvalid, shift, (1u << bits) - 1u, wcnt, bstart, gofs, sk, sv,
+                           kout, vout);
+}
+
+// Stage names for the per-kernel timing, by sort: pairs (PLAIN), depth
+// (FAST records), tiles (FAST lists).
+struct SortNames {
+  const char *hist, *scatter, *hist_all, *onesweep;
+};
+static const SortNames k_sort_names[3] = {
+    {"radix_hist_kv", "radix_scatter_kv", "sort_hist_kv", "onesweep_kv"},
+    {"depth_hist", "depth_scatter", "depth_hist_all", "depth_onesweep"},
+    {"tile_hist", "tile_scatter", "tile_hist_all", "tile_onesweep"},
+};
+
+template <typename K>
+static hipError_t sort_kv_classic(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
+                                  int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
+                                  bool& in_alt, const uint32_t* n_dev, uint32_t* last_vals,
+                                  const SortNames& nm) {
   const int nb = (int)((n + SORT_TILE - 1) / SORT_TILE);
   uint32_t* hist = (uint32_t*)ws;
   void* scan_ws = (char*)ws + align_up((size_t)256 * nb * sizeof(uint32_t));
-  uint64_t *ka = keys, *kb = keys_alt;
+  K *ka = keys, *kb = keys_alt;
   uint32_t *va = vals, *vb = vals_alt;
-  for (int shift = begin_bit; shift < end_bit; shift += 8) {
-    const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
+  const int npass = sort_passes(begin_bit, end_bit);
+  for (int p = 0; p < npass; ++p) {
+    int shift, bits;
+    pass_digits(begin_bit, end_bit, p, shift, bits);
     const uint32_t mask = (1u << bits) - 1u;
-    FGS_TIMED(vals ? "radix_hist_kv" : "radix_hist_k", s,
-              (k_radix_hist<<<nb, SORT_THREADS, 0, s>>>(ka, n, n_dev, shift, mask, hist, nb)));
+    const bool last = p == npass - 1;
+    uint32_t* vout = (last && last_vals) ? last_vals : vb;
+    FGS_TIMED(nm.hist, s,
+              (k_radix_hist<K><<<nb, SORT_THREADS, 0, s>>>(ka, n, n_dev, shift, mask, hist, nb)));
     hipError_t e = scan_u32(hist, hist, (int64_t)256 * nb, nullptr, scan_ws, s);
     if (e != hipSuccess) return e;
-    const bool last = shift + 8 >= end_bit;
     if (vals)
-      FGS_TIMED("radix_scatter_kv", s,
-                (k_radix_scatter<true><<<nb, SORT_THREADS, 0, s>>>(ka, va, kb, vb, n, n_dev, shift,
-                                                                    mask, hist, nb)));
-    else if (last && last_lo)
-      FGS_TIMED("radix_scatter_k_last", s,
-                (k_radix_scatter<false, true><<<nb, SORT_THREADS, 0, s>>>(
-                    ka, nullptr, kb, nullptr, n, n_dev, shift, mask, hist, nb, last_lo, last_hi)));
+      FGS_TIMED(nm.scatter, s,
+                (k_radix_scatter<K, true><<<xcd_grid(nb), SORT_THREADS, 0, s>>>(
+                    ka, va, kb, vout, n, n_dev, shift, bits, hist, nb)));
     else
-      FGS_TIMED("radix_scatter_k", s,
-                (k_radix_scatter<false><<<nb, SORT_THREADS, 0, s>>>(ka, nullptr, kb, nullptr, n,
-                                                                     n_dev, shift, mask, hist, nb)));
-    uint64_t* tk = ka; ka = kb; kb = tk;
+      FGS_TIMED(nm.scatter, s,
+                (k_radix_scatter<K, false><<<xcd_grid(nb), SORT_THREADS, 0, s>>>(
+                    ka, nullptr, kb, nullptr, n, n_dev, shift, bits, hist, nb)));
+    K* tk = ka; ka = kb; kb = tk;
     uint32_t* tv = va; va = vb; vb = tv;
     in_alt = !in_alt;
   }
   return hipGetLastError();
 }
+
+// ------------------------------------------------------ radix, onesweep
+// Onesweep-style passes for inputs up to ONESWEEP_MAX_KEYS (where the
+// three-launch passes are launch- and latency-bound):
+//   k_sort_prepare   zeroes the digit histograms and tile counters;
+//   k_sort_hist      one read of the keys builds the histograms of all passes
+//                    (LDS counts, one global atomic per digit and workgroup);
+//   k_onesweep       one launch per pass: a workgroup takes the next 4096-key
+//                    tile from a counter, ranks it (rank_tile), publishes its
+//                    digit counts and looks back over the earlier tiles'
+//                    published counts for its global offsets (decoupled
+//                    look-back), then writes each digit's run contiguously.
+// Tiles are taken in launch order, so every tile a workgroup waits on
+// belongs to a workgroup that started earlier and is resident; the wait is
+// additionally bounded (a fault flag is raised instead of hanging).
+//
+// Status words: bits 31..30 = 0 (not yet), 1 (tile count), 2 (inclusive
+// prefix); bits 29..0 the count (n < 2^30).
+constexpr uint32_t ST_AGG = 1u << 30, ST_PRE = 2u << 30, ST_CNT = (1u << 30) - 1u;
+constexpr uint32_t SPIN_LIMIT = 1u << 22;
+
+__global__ __launch_bounds__(256) void k_sort_prepare(uint32_t* __restrict__ hist,
+                                                      uint32_t* __restrict__ counters) {
+  for (int i = threadIdx.x; i < SORT_MAX_PASSES * 256; i += 256) hist[i] = 0u;
+  if (threadIdx.x < SORT_MAX_PASSES) counters[threadIdx.x] = 0u;
+}
+
+// Histograms of all passes in one read (per_block sort tiles per workgroup);
+// also clears the first pass's status rows of the tiles it covers.
+template <typename K>
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const K* __restrict__ keys,
+                                                            int64_t n_cap,
+                                                            const uint32_t* __restrict__ n_dev,
+                                                            int begin_bit, int end_bit,
+                                                            uint32_t* __restrict__ hist,
+                                                            uint32_t* __restrict__ status0,
+                                                            int64_t nb, int per_block) {
+  __shared__ uint32_t cnt[SORT_MAX_PASSES][256];
+  const int npass = sort_passes(begin_bit, end_bit);
+  const int t = threadIdx.x;
+  for (int p = 0; p < npass; ++p) cnt[p][t] = 0u;
+  const int64_t tile0 = (int64_t)blockIdx.x * per_block;
+  for (int64_t tl = tile0; tl < min(nb, tile0 + per_block); ++tl) status0[tl * 256 + t] = 0u;
+  __syncthreads();
+  const int64_t n = dev_count(n_dev, n_cap);
+  const int64_t e0 = tile0 * SORT_TILE, e1 = min(n, e0 + (int64_t)per_block * SORT_TILE);
+  constexpr int N = KeyVec<K>::N;
+  for (int64_t e = e0 + (int64_t)t * N; e < e1; e += (int64_t)SORT_THREADS * N) {
+    KeyVec<K> kv;
+    const int m = load_keys(keys, e, e1, kv);
+    for (int p = 0; p < npass; ++p) {
+      int sh, bits;
+      pass_digits(begin_bit, end_bit, p, sh, bits);
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        if (j < m) atomicAdd(&cnt[p][(uint32_t)(kv.k[j] >> sh) & ((1u << bits) - 1u)], 1u);
+    }
+  }
+  __syncthreads();
+  for (int p = 0; p < npass; ++p)
+    if (cnt[p][t]) atomicAdd(&hist[p * 256 + t], cnt[p][t]);
+}
+
+__device__ __forceinline__ uint32_t status_load(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename K, bool VALS>
+__global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
+    const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
+    uint32_t* __restrict__ vout, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
+    int bits, const uint32_t* __restrict__ hist, uint32_t* __restrict__ counter,
+    uint32_t* __restrict__ status, uint32_t* __restrict__ status_next,
+    uint32_t* __restrict__ fault) {
+  constexpr int NW = SORT_THREADS / WAVE;
+  __shared__ K sk[SORT_TILE];
+  __shared__ uint32_t sv[VALS ? SORT_TILE : 1];
+  __shared__ uint32_t wcnt[NW][256];
+  __shared__ uint32_t bstart[256];
+  __shared__ uint32_t gofs[256];
+  __shared__ uint32_t s_tile;
+  const int t = threadIdx.x;
+  const uint32_t mask = (1u << bits) - 1u;
+  if (t == 0) s_tile = atomicAdd(counter, 1u);
+#pragma unroll
+  for (int q = 0; q < NW; ++q) wcnt[q][t] = 0u;
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t n = dev_count(n_dev, n_cap);
+  const int64_t base = tile * SORT_TILE;
+  if (base >= n) return;  // whole workgroup; no later tile waits on this one
+  if (status_next) status_next[tile * 256 + t] = 0u;
+  const int nvalid = (int)min((int64_t)SORT_TILE, n - base);
+  K k[SORT_ITEMS];
+  uint32_t v[SORT_ITEMS], rk[SORT_ITEMS], c;
+  load_tile<K, VALS>(kin, vin, base, nvalid, k, v);
+  rank_tile<K>(k, nvalid, shift, bits, wcnt, bstart, rk, c);
+  // global offset of digit t: digits below it plus this digit in earlier tiles
+  uint32_t gd;
+  {
+    const uint32_t hv = (uint32_t)t <= mask ? hist[t] : 0u;
+    uint32_t htot;
+    gd = block_excl_scan<SORT_THREADS>(hv, htot);
+  }
+  uint32_t* my = status + tile * 256 + t;
+  uint32_t excl = 0;
+  if (tile == 0) {
+    status_store(my, ST_PRE | c);
+  } else {
+    status_store(my, ST_AGG | c);
+    int64_t p = tile - 1;
+    uint32_t spins = 0;
+    while (p >= 0) {
+      const uint32_t st = status_load(status + p * 256 + t);
+      const uint32_t flag = st & ~ST_CNT;
+      if (flag == 0u) {
+        if (++spins > SPIN_LIMIT) {  // cannot happen with in-order tiles; never hang
+          atomicOr(fault, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += st & ST_CNT;
+      if (flag == ST_PRE) break;
+      --p;
+    }
+    status_store(my, ST_PRE | (excl + c));
+  }
+  gofs[t] = gd + excl;
+  __syncthreads();
+  place_and_write<K, VALS>(k, v, rk, nvalid, shift, mask, wcnt, bstart, gofs, sk, sv, kout, vout);
+}
+
+template <typename K>
+static hipError_t sort_kv_onesweep(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
+                                   int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
+                                   bool& in_alt, const uint32_t* n_dev, uint32_t* last_vals,
+                                   const SortNames& nm) {
+  const int64_t nb = (n + SORT_TILE - 1) / SORT_TILE;
+  const SortWs w = carve_sort(ws, nb);
+  const int npass = sort_passes(begin_bit, end_bit);
+  k_sort_prepare<<<1, 256, 0, s>>>(w.hist, w.counters);
+  // at least ~1024 workgroups for the histogram read when the input allows
+  const int per_block = (int)(nb / 1024 > 1 ? nb / 1024 : 1);
+  const unsigned hb = (unsigned)((nb + per_block - 1) / per_block);
+  FGS_TIMED(nm.hist_all, s,
+            (k_sort_hist<K><<<hb, SORT_THREADS, 0, s>>>(keys, n, n_dev, begin_bit, end_bit,
+                                                         w.hist, w.status[0], nb, per_block)));
+  K *ka = keys, *kb = keys_alt;
+  uint32_t *va = vals, *vb = vals_alt;
+  for (int p = 0; p < npass; ++p) {
+    int shift, bits;
+    pass_digits(begin_bit, end_bit, p, shift, bits);
+    const bool last = p == npass - 1;
+    uint32_t* vout = (last && last_vals) ? last_vals : vb;
+    uint32_t* st = w.status[p & 1];
+    uint32_t* st_next = last ? nullptr : w.status[(p + 1) & 1];
+    const uint32_t* h = w.hist + p * 256;
+    uint32_t* ctr = w.counters + p;
+    uint32_t* fault = w.counters + SORT_MAX_PASSES;
+    if (vals)
+      FGS_TIMED(nm.onesweep, s,
+                (k_onesweep<K, true><<<(unsigned)nb, SORT_THREADS, 0, s>>>(
+                    ka, va, kb, vout, n, n_dev, shift, bits, h, ctr, st, st_next, fault)));
+    else
+      FGS_TIMED(nm.onesweep, s,
+                (k_onesweep<K, false><<<(unsigned)nb, SORT_THREADS, 0, s>>>(
+                    ka, nullptr, kb, nullptr, n, n_dev, shift, bits, h, ctr, st, st_next, fault)));
+    K* tk = ka; ka = kb; kb = tk;
+    uint32_t* tv = va; va = vb; vb = tv;
+    in_alt = !in_alt;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- dispatch
+template <typename K>
+hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
+                   int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
+                   const uint32_t* n_dev, uint32_t* last_vals, SortKind kind) {
+  in_alt = false;
+  // with a device-side count the passes always run (the count may be 0 or 1)
+  if ((n <= 1 && !n_dev) || n <= 0 || end_bit <= begin_bit) return hipSuccess;
+  if (n > (int64_t)ST_CNT || end_bit - begin_bit > 8 * SORT_MAX_PASSES ||
+      end_bit > (int)(8 * sizeof(K)))
+    return hipErrorInvalidValue;
+  const SortNames& nm = k_sort_names[kind];
+  if (n <= ONESWEEP_MAX_KEYS)
+    return sort_kv_onesweep<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s,
+                               in_alt, n_dev, last_vals, nm);
+  return sort_kv_classic<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, in_alt,
+                            n_dev, last_vals, nm);
+}
+
+template hipError_t sort_kv<uint16_t>(uint16_t*, uint32_t*, uint16_t*, uint32_t*, int64_t, int,
+                                      int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
+                                      SortKind);
+template hipError_t sort_kv<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, int64_t, int,
+                                      int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
+                                      SortKind);
+template hipError_t sort_kv<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, int64_t, int,
+                                      int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
+                                      SortKind);
 
 }  // namespace fgs

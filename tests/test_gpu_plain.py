@@ -108,9 +108,11 @@ def test_backward_parity(case, abs_grad):
     check_backward(got, orc, dimg, abs_grad)
 
 
-def _sort_case(n, bits, seed, with_vals=True):
+def _sort_case(n, bits, seed, with_vals=True, ws=None, distinct=None):
     rng = np.random.default_rng(seed)
-    keys = rng.integers(0, 2 ** bits, n, dtype=np.uint64)
+    keys = rng.integers(0, 2 ** bits, n, dtype=np.uint64, endpoint=False)
+    if distinct is not None:  # few distinct digits: long runs per digit
+        keys = rng.choice(keys[:distinct], n)
     keys[rng.random(n) < 0.3] = keys[0]  # many equal keys: stability matters
     vals = np.arange(n, dtype=np.uint32)
     k = torch.from_numpy(keys.view(np.int64)).cuda()
@@ -118,7 +120,8 @@ def _sort_case(n, bits, seed, with_vals=True):
     v = torch.from_numpy(vals.view(np.int32)).cuda() if with_vals else None
     va = torch.empty_like(v) if with_vals else None
     L = lib()
-    ws = torch.empty(L.fgs_sort_ws_size(n), dtype=torch.uint8, device="cuda")
+    if ws is None:
+        ws = torch.empty(L.fgs_sort_ws_size(n), dtype=torch.uint8, device="cuda")
     p = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
     check(L.fgs_sort_pairs(p(k), p(v), p(ka), p(va), n, 0, bits, p(ws), ws.numel(),
                            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "sort")
@@ -135,6 +138,21 @@ def test_radix_sort_stable(n, bits):
 
 def test_radix_sort_keys_only():
     _sort_case(70001, 40, seed=3, with_vals=False)
+
+
+def test_radix_sort_all_passes_and_few_digits():
+    _sort_case(3_000_001, 64, seed=11)                 # eight passes
+    _sort_case(4_500_000, 45, seed=13)                 # above 4M keys: three-launch passes
+    _sort_case(2_000_000, 48, seed=12, distinct=5)      # a handful of digits per pass
+
+
+def test_radix_sort_workspace_reuse():
+    # the look-back status rows are cleared by the sort itself: a second call
+    # on the same workspace (smaller input) must not see the first one's state
+    ws = torch.empty(lib().fgs_sort_ws_size(300_000), dtype=torch.uint8, device="cuda")
+    _sort_case(300_000, 45, seed=21, ws=ws)
+    _sort_case(180_000, 45, seed=22, ws=ws)
+    _sort_case(300_000, 13, seed=23, ws=ws)
 
 
 @pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 5_000_000])
