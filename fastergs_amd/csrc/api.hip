@@ -379,7 +379,7 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   FGS_TRY(scan_u32(w.lflag, w.lpos, PV, w.counts + 3, w.scratch, s), "large scan");
   FGS_TRY(launch_compact_large((int)PV, w.lflag, w.lpos, w.large_ids, s), "large compact");
   FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
-                             w.counts, w.pcount, nullptr, nullptr, 0u, nullptr, nullptr, false, s),
+                             w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
           "large tiles");
   // records in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by (global) tile alone yields the canonical per-tile
@@ -463,12 +463,11 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
     // emission in depth order, tile-only stable sort, lists, ranges, buckets.
     // R is the pair capacity; the pair count itself (counts[0]) stays on
     // the device, so nothing here waits for the host.
-    FGS_TRY(fill_u32(gw.lcur, 0u, (int64_t)g->P * b.V, s), "zero cursors");
     FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.poff, (uint32_t)R,
                               pw.t0, pw.r0, pw.tile16, s),
             "emit");
     FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts, nullptr,
-                               gw.poff, gw.lcur, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s),
+                               gw.poff, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s),
             "emit large");
     // stable sort of (tile id, record) by tile id: the last pass writes the
     // per-tile lists; ranges by binary search over the sorted tile ids

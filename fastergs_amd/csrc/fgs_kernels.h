@@ -116,8 +116,8 @@ hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey
 // most 65536 tiles), else 32-bit.
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t* lcur, uint32_t cap, void* tkeys,
-                              uint32_t* recs, bool t16, hipStream_t s);
+                              const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
+                              bool t16, hipStream_t s);
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
                              const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
                              bool t16, hipStream_t s);

@@ -429,7 +429,8 @@ FGS_HD bool seg_hits(R a, R b, R c, R d, R l, R r) {
   const R delta = b * b - R(4) * a * c;
   const R t1 = (l - d) * (R(2) * a) + b;
   const R t2 = (r - d) * (R(2) * a) + b;
-  return delta >= R(0) && (t1 <= R(0) || t1 * t1 <= delta) && (t2 >= R(0) || t2 * t2 <= delta);
+  // every term evaluated (no short circuit): the same value, branch-free on the GPU
+  return (delta >= R(0)) & ((t1 <= R(0)) | (t1 * t1 <= delta)) & ((t2 >= R(0)) | (t2 * t2 <= delta));
 }
 
 // Exact tile test of preprocess.cu:179-218 for tile (tx,ty): true when the
@@ -439,12 +440,70 @@ template <typename R>
 FGS_HD bool tile_hit(int tx, int ty, R x, R y, const R conic[3], R power) {
   const int x0 = tx * TILE_W, y0 = ty * TILE_H;
   const int x1 = x0 + TILE_W - 1, y1 = y0 + TILE_H - 1;
-  if (x >= (R)x0 && x <= (R)x1 && y >= (R)y0 && y <= (R)y1) return true;
+  const bool inside = (x >= (R)x0) & (x <= (R)x1) & (y >= (R)y0) & (y <= (R)y1);
   const R w = R(2) * power;
   const R dx = (x * R(2) < (R)(x0 + x1)) ? x - (R)x0 : x - (R)x1;
-  if (seg_hits(conic[2], R(-2) * conic[1] * dx, conic[0] * dx * dx - w, y, (R)y0, (R)y1)) return true;
+  const bool h1 = seg_hits(conic[2], R(-2) * conic[1] * dx, conic[0] * dx * dx - w, y, (R)y0, (R)y1);
   const R dy = (y * R(2) < (R)(y0 + y1)) ? y - (R)y0 : y - (R)y1;
-  return seg_hits(conic[0], R(-2) * conic[1] * dy, conic[2] * dy * dy - w, x, (R)x0, (R)x1);
+  const bool h2 = seg_hits(conic[0], R(-2) * conic[1] * dy, conic[2] * dy * dy - w, x, (R)x0, (R)x1);
+  return inside | h1 | h2;
+}
+
+// tile_hit factored for walks over many tiles of one ellipse: the terms
+// that depend on the tile column only (TileCol) or on the tile row only
+// (TileRow) are formed once, and tile_hit_terms finishes the test with the
+// remaining per-tile operations. Every float operation has the operands and
+// order of tile_hit, so the result is identical.
+struct TileCol {
+  float x0f, x1f;  // pixel-centre box columns
+  bool xin;        // x0 <= x <= x1
+  float b1, d1;    // nearer vertical edge: b and discriminant of seg_hits
+  float l2, r2;    // x0 - x, x1 - x (horizontal-edge segment ends)
+};
+struct TileRow {
+  float l1, r1;    // y0 - y, y1 - y (vertical-edge segment ends)
+  bool yin;        // y0 <= y <= y1
+  float b2, d2;    // nearer horizontal edge: b and discriminant of seg_hits
+};
+FGS_HD TileCol tile_col(int tx, float x, const float conic[3], float power) {
+  TileCol c;
+  const int x0 = tx * TILE_W, x1 = x0 + TILE_W - 1;
+  c.x0f = (float)x0;
+  c.x1f = (float)x1;
+  c.xin = (x >= c.x0f) & (x <= c.x1f);
+  const float w = 2.0f * power;
+  const float dx = (x * 2.0f < (float)(x0 + x1)) ? x - c.x0f : x - c.x1f;
+  c.b1 = -2.0f * conic[1] * dx;
+  const float cc = conic[0] * dx * dx - w;
+  c.d1 = c.b1 * c.b1 - 4.0f * conic[2] * cc;
+  c.l2 = c.x0f - x;
+  c.r2 = c.x1f - x;
+  return c;
+}
+FGS_HD TileRow tile_row(int ty, float y, const float conic[3], float power) {
+  TileRow r;
+  const int y0 = ty * TILE_H, y1 = y0 + TILE_H - 1;
+  const float y0f = (float)y0, y1f = (float)y1;
+  r.yin = (y >= y0f) & (y <= y1f);
+  const float w = 2.0f * power;
+  const float dy = (y * 2.0f < (float)(y0 + y1)) ? y - y0f : y - y1f;
+  r.b2 = -2.0f * conic[1] * dy;
+  const float cc = conic[2] * dy * dy - w;
+  r.d2 = r.b2 * r.b2 - 4.0f * conic[0] * cc;
+  r.l1 = y0f - y;
+  r.r1 = y1f - y;
+  return r;
+}
+// The end tests of seg_hits for a segment [l, r] relative to the centre.
+FGS_HD bool seg_ends(float two_a, float b, float delta, float l, float r) {
+  const float t1 = l * two_a + b;
+  const float t2 = r * two_a + b;
+  return (delta >= 0.0f) & ((t1 <= 0.0f) | (t1 * t1 <= delta)) & ((t2 >= 0.0f) | (t2 * t2 <= delta));
+}
+FGS_HD bool tile_hit_terms(const TileCol& c, const TileRow& r, const float conic[3]) {
+  const bool h1 = seg_ends(2.0f * conic[2], c.b1, c.d1, r.l1, r.r1);
+  const bool h2 = seg_ends(2.0f * conic[0], r.b2, r.d2, c.l2, c.r2);
+  return (c.xin & r.yin) | h1 | h2;
 }
 
 // ------------------------------------------------------------ per pixel

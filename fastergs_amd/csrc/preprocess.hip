@@ -307,10 +307,24 @@ hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Large rects: one workgroup per Gaussian of the compact list (grid-stride
-// over the device-side count); its four waves split the rect's tiles, 64
-// per wave step, 256 per workgroup step. One atomic per wave for the count;
-// emission reserves slots in the Gaussian's block with one atomic per step.
+// Large rects: one wave per Gaussian of the compact list (grid-stride over
+// the device-side count). Each lane keeps one tile column (its tile_col
+// terms formed once); a rect of width w <= 64 is walked 64 / w rows per
+// step, a wider one 64 columns at a time, row by row. Counting adds the
+// hits to the Gaussian's pair count; emission places the k-th hit of the
+// walk at poff + k (a running ballot prefix, no atomics).
+template <bool EMIT>
+__device__ __forceinline__ void large_step(bool hit, int tx, int ty, uint32_t base, uint32_t& run,
+                                           int gx, uint32_t toff, uint32_t g, uint32_t cap,
+                                           void* tkeys, uint32_t* recs, int t16) {
+  const uint64_t m = __ballot(hit);
+  if (EMIT) {
+    const uint32_t pos = base + run + lanes_below(m);
+    if (hit && pos < cap) store_pair(tkeys, recs, t16 != 0, pos, toff + (uint32_t)(ty * gx + tx), g);
+  }
+  run += (uint32_t)__popcll(m);
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec* __restrict__ views,
                                                      const float* __restrict__ geom,
@@ -318,64 +332,61 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
                                                      const uint32_t* __restrict__ counts,
                                                      uint32_t* pcount,
                                                      const uint32_t* __restrict__ poff,
-                                                     uint32_t* lcur, uint32_t cap,
-                                                     void* tkeys, uint32_t* recs, int t16) {
+                                                     uint32_t cap, void* tkeys, uint32_t* recs,
+                                                     int t16) {
   const uint32_t NL = counts[3];
   const int lane = lane_id();
-  const int w = threadIdx.x / WAVE;
-  for (uint32_t j = blockIdx.x; j < NL; j += gridDim.x) {
+  const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
+  for (uint32_t j = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; j < NL; j += nwaves) {
     const uint32_t g = large_ids[j];  // record index v * P + i
     const int v = V > 1 ? (int)(g / (uint32_t)P) : 0;
     const int gx = views[v].grid_x;
     const uint32_t toff = views[v].tile_off;
     const GeomRow r = load_geom(geom, g);
+    const uint32_t base = EMIT ? poff[g] : 0u;
     const int wr = r.r2 - r.r0;
-    const int area = wr * (r.r3 - r.r1);
-    int l = w * WAVE + lane;
-    int ty = l / wr;
-    int tx = l - ty * wr;
-    const int sy = 4 * WAVE / wr, sx = 4 * WAVE - sy * wr;
-    uint32_t mine = 0;
-    for (int k = w * WAVE; k < area; k += 4 * WAVE) {
-      const bool hit = l < area && tile_hit(r.r0 + tx, r.r1 + ty, r.x, r.y, r.conic, r.power);
-      const uint64_t m = __ballot(hit);
-      if (!EMIT) {
-        mine += (uint32_t)__popcll(m);
-      } else if (m) {
-        uint32_t slot0 = 0;
-        if (lane == 0) slot0 = atomicAdd(&lcur[g], (uint32_t)__popcll(m));
-        slot0 = __shfl(slot0, 0, WAVE);
-        const uint32_t pos = poff[g] + slot0 + lanes_below(m);
-        if (hit && pos < cap)
-          store_pair(tkeys, recs, t16 != 0, pos, toff + (uint32_t)((r.r1 + ty) * gx + r.r0 + tx), g);
+    uint32_t run = 0;
+    if (wr <= WAVE) {
+      const int k = WAVE / wr;  // rows per step
+      const int rs = lane / wr;
+      const int tx = r.r0 + (lane - rs * wr);
+      const TileCol c = tile_col(tx, r.x, r.conic, r.power);
+      for (int row0 = r.r1; row0 < r.r3; row0 += k) {
+        const int ty = row0 + rs;
+        const TileRow rw = tile_row(ty, r.y, r.conic, r.power);
+        const bool hit = rs < k && ty < r.r3 && tile_hit_terms(c, rw, r.conic);
+        large_step<EMIT>(hit, tx, ty, base, run, gx, toff, g, cap, tkeys, recs, t16);
       }
-      l += 4 * WAVE;
-      tx += sx;
-      ty += sy;
-      if (tx >= wr) {
-        tx -= wr;
-        ty += 1;
+    } else {
+      for (int c0 = r.r0; c0 < r.r2; c0 += WAVE) {
+        const int tx = c0 + lane;
+        const TileCol c = tile_col(tx, r.x, r.conic, r.power);
+        for (int ty = r.r1; ty < r.r3; ++ty) {
+          const TileRow rw = tile_row(ty, r.y, r.conic, r.power);
+          const bool hit = tx < r.r2 && tile_hit_terms(c, rw, r.conic);
+          large_step<EMIT>(hit, tx, ty, base, run, gx, toff, g, cap, tkeys, recs, t16);
+        }
       }
     }
-    if (!EMIT && lane == 0 && mine) atomicAdd(&pcount[g], mine);
+    if (!EMIT && lane == 0 && run) atomicAdd(&pcount[g], run);
   }
 }
 
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t* lcur, uint32_t cap,
-                              void* tkeys, uint32_t* recs, bool t16, hipStream_t s) {
+                              const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
+                              bool t16, hipStream_t s) {
   if (P <= 0) return hipSuccess;
-  const int blocks = 2048;  // persistent workgroups; each exits past the device-side count
+  const int blocks = 2048;  // persistent: 8192 waves, each exits past the device-side count
   if (emit)
     FGS_TIMED("large_tiles_emit", s,
               (k_large_tiles<true><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
-                                                          pcount, poff, lcur, cap, tkeys, recs,
+                                                          pcount, poff, cap, tkeys, recs,
                                                           t16 ? 1 : 0)));
   else
     FGS_TIMED("large_tiles_count", s,
               (k_large_tiles<false><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
-                                                           pcount, poff, lcur, cap, tkeys, recs,
+                                                           pcount, poff, cap, tkeys, recs,
                                                            t16 ? 1 : 0)));
   return hipGetLastError();
 }

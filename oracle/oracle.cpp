@@ -740,6 +740,26 @@ int orc_tile_hit_f32(int tx, int ty, float x, float y, float ca, float cb, float
   const float conic[3] = {ca, cb, cc};
   return fgs::tile_hit(tx, ty, x, y, conic, power) ? 1 : 0;
 }
+// Checks tile_hit's factored form (tile_col / tile_row / tile_hit_terms,
+// used by the GPU's walks over large rects) against tile_hit on every tile
+// of a gx x gy grid for one ellipse; returns the number of disagreements
+// and the number of hits.
+int orc_tile_terms_check(float x, float y, float ca, float cb, float cc, float power, int gx,
+                         int gy, int* hits) {
+  const float conic[3] = {ca, cb, cc};
+  int bad = 0, h = 0;
+  for (int ty = 0; ty < gy; ++ty) {
+    const fgs::TileRow r = fgs::tile_row(ty, y, conic, power);
+    for (int tx = 0; tx < gx; ++tx) {
+      const bool a = fgs::tile_hit(tx, ty, x, y, conic, power);
+      const bool b = fgs::tile_hit_terms(fgs::tile_col(tx, x, conic, power), r, conic);
+      bad += a != b;
+      h += a;
+    }
+  }
+  *hits = h;
+  return bad;
+}
 int orc_num_threads() { return omp_get_max_threads(); }
 
 }  // extern "C"

@@ -49,6 +49,8 @@ def lib():
         L.orc_tile_hit_f32.argtypes = [i32, i32] + [ctypes.c_float] * 6
         L.orc_tile_hit_f32.restype = i32
         L.orc_num_threads.restype = i32
+        L.orc_tile_terms_check.argtypes = [ctypes.c_float] * 6 + [i32, i32, vp]
+        L.orc_tile_terms_check.restype = i32
         L.orc_render_simp_f32.argtypes = [vp, vp, vp, vp]
         L.orc_render_depth_f32.argtypes = [vp] * 6
         L.orc_inverse4_f64.argtypes = [vp, vp]

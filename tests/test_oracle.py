@@ -141,6 +141,31 @@ def test_tile_set_brute_force():
     assert checked > 5000 and ambiguous < checked * 1e-3
 
 
+def test_factored_tile_test_equals_tile_hit():
+    """The per-column / per-row factoring of the exact tile test (used by the
+    GPU's walk over large rects) gives tile_hit's answer on every tile, for
+    ellipses from round to needle-thin at every orientation, centred inside
+    and outside the grid, small and large."""
+    import ctypes
+    L = pyoracle.lib()
+    rng = np.random.default_rng(17)
+    hits = ctypes.c_int(0)
+    total_hits = 0
+    for _ in range(3000):
+        s1 = float(np.exp(rng.uniform(np.log(0.3), np.log(600.0))))
+        s2 = float(s1 * np.exp(rng.uniform(np.log(0.001), 0.0)))
+        th = rng.uniform(0, np.pi)
+        R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+        con = np.linalg.inv(R @ np.diag([s1 * s1, s2 * s2]) @ R.T + 0.3 * np.eye(2))
+        power = float(np.log(256.0 * rng.uniform(0.005, 1.0)))
+        x, y = (float(v) for v in rng.uniform(-300, 1600, 2))
+        bad = L.orc_tile_terms_check(x, y, con[0, 0], con[0, 1], con[1, 1], power, 100, 70,
+                                     ctypes.byref(hits))
+        assert bad == 0, (x, y, con, power)
+        total_hits += hits.value
+    assert total_hits > 100000
+
+
 @pytest.mark.parametrize("bucket", [16, 32, 128])
 def test_sort_ranges_buckets_invariants(bucket):
     scene, cam = scenes.workload("small", n=20000)
