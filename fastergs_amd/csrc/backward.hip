@@ -372,8 +372,16 @@ __global__ __launch_bounds__(GBWDV_THREADS) void k_gauss_bwd_views(GaussBwdArgs 
 // coefficient gradients, and the rows are stored back with float4 stores.
 // Lane i also writes ddc and the mean-gradient term sh_dm of its Gaussian.
 constexpr int SHB_THREADS = WAVE;
+// Rows staged as float4 with every load in flight before the first LDS
+// store: the group's loads are unconditional (indices clamped into the
+// group) and the LDS buffer is padded to SHB_QMAX x 64 float4 so that the
+// stores need no predicate either.
+constexpr int SHB_QMAX = (3 * 15 * SHB_THREADS / 4 + SHB_THREADS - 1) / SHB_THREADS;  // 12
+constexpr size_t SHB_SHMEM = (size_t)SHB_QMAX * SHB_THREADS * sizeof(float4);
+
 __global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
-  extern __shared__ float rows[];  // SHB_THREADS x 3M
+  extern __shared__ float4 rows4[];  // SHB_THREADS x 3M floats (padded)
+  float* rows = (float*)rows4;
   const int lane = threadIdx.x;
   const int g0 = blockIdx.x * SHB_THREADS;
   const int i = g0 + lane;
@@ -382,29 +390,37 @@ __global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
   const int nb = min(SHB_THREADS, a.P - g0);
   const float* src = a.G.shs + (size_t)RM * g0;
   float* dst = a.dsh + (size_t)RM * g0;
+  // per-Gaussian inputs first, so their loads overlap the staging
+  const bool in = i < a.P;
+  const size_t ic = in ? (size_t)i : 0;
+  const bool vis = in && a.radii[ic] > 0;
+  const float* pm = a.G.means + 3 * ic;
+  const float px = pm[0], py = pm[1], pz = pm[2];
+  const float col[3] = {a.raw[RAW_STRIDE * ic + RAW_COL0], a.raw[RAW_STRIDE * ic + RAW_COL1],
+                        a.raw[RAW_STRIDE * ic + RAW_COL2]};
+  const int clampbits = (int)f2u(a.splat[12 * ic + 11]);
   // full groups of 64 rows start 16-byte aligned when the arrays do
-  const bool vec = M > 0 && nb == SHB_THREADS && RM * SHB_THREADS % 4 == 0 &&
+  const bool vec = M > 0 && M <= 15 && nb == SHB_THREADS && RM * SHB_THREADS % 4 == 0 &&
                    (((uintptr_t)a.G.shs | (uintptr_t)a.dsh) & 15) == 0;
+  const int nq = RM * SHB_THREADS / 4;
   if (vec) {
-    for (int j = lane; j < RM * SHB_THREADS / 4; j += SHB_THREADS)
-      ((float4*)rows)[j] = ((const float4*)src)[j];
+    float4 hv[SHB_QMAX];
+#pragma unroll
+    for (int u = 0; u < SHB_QMAX; ++u)
+      hv[u] = ((const float4*)src)[min(lane + u * SHB_THREADS, nq - 1)];
+#pragma unroll
+    for (int u = 0; u < SHB_QMAX; ++u) rows4[lane + u * SHB_THREADS] = hv[u];
   } else {
     for (int j = lane; j < RM * nb; j += SHB_THREADS) rows[j] = src[j];
   }
   __syncthreads();
-  if (i < a.P) {
+  if (in) {
     float* row = rows + RM * lane;
-    const bool vis = a.radii[i] > 0;
     float dm[3] = {0.0f, 0.0f, 0.0f}, ddc[3] = {0.0f, 0.0f, 0.0f};
     if (vis) {
-      const float* pm = a.G.means + 3 * (size_t)i;
-      const float ox = pm[0] - a.cam.campos[0], oy = pm[1] - a.cam.campos[1],
-                  oz = pm[2] - a.cam.campos[2];
-      const float col[3] = {a.raw[RAW_STRIDE * (size_t)i + RAW_COL0],
-                            a.raw[RAW_STRIDE * (size_t)i + RAW_COL1],
-                            a.raw[RAW_STRIDE * (size_t)i + RAW_COL2]};
+      const float ox = px - a.cam.campos[0], oy = py - a.cam.campos[1], oz = pz - a.cam.campos[2];
       float dRGB[3];
-      sh_drgb((int)f2u(a.splat[12 * (size_t)i + 11]), col, dRGB);
+      sh_drgb(clampbits, col, dRGB);
       const float len = sqrt_(ox * ox + oy * oy + oz * oz);
       const float x = ox / len, y = oy / len, z = oz / len;
       float bas[16], db[16][3];
@@ -434,8 +450,12 @@ __global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
   }
   __syncthreads();
   if (vec) {
-    for (int j = lane; j < RM * SHB_THREADS / 4; j += SHB_THREADS)
-      ((float4*)dst)[j] = ((const float4*)rows)[j];
+    float4 ov[SHB_QMAX];
+#pragma unroll
+    for (int u = 0; u < SHB_QMAX; ++u) ov[u] = rows4[lane + u * SHB_THREADS];
+#pragma unroll
+    for (int u = 0; u < SHB_QMAX; ++u)
+      if (lane + u * SHB_THREADS < nq) ((float4*)dst)[lane + u * SHB_THREADS] = ov[u];
   } else {
     for (int j = lane; j < RM * nb; j += SHB_THREADS) dst[j] = rows[j];
   }
@@ -495,7 +515,8 @@ hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
   const bool colour_sh = a.G.colors == nullptr && a.G.dc != nullptr;
   if (a.sh_dm != nullptr && (a.views == nullptr || a.V <= 1) && M <= 15) {
     if (colour_sh) {
-      const size_t shmem = (size_t)SHB_THREADS * 3 * M * sizeof(float);
+      const size_t need = (size_t)SHB_THREADS * 3 * M * sizeof(float);
+      const size_t shmem = need > SHB_SHMEM ? need : SHB_SHMEM;
       FGS_TIMED("sh_bwd", s,
                 (k_sh_bwd<<<(unsigned)((a.P + SHB_THREADS - 1) / SHB_THREADS), SHB_THREADS,
                              shmem, s>>>(a)));

@@ -253,35 +253,56 @@ __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a) {
 // into its splat record (r, g, b, clamp bits). One wave64 per workgroup; the
 // dc / SH rows of its 64 Gaussians are staged in LDS with coalesced loads (a
 // thread's own 192-byte row would otherwise be 48 scattered loads per wave).
+// LDS layout of k_sh_color: dc rows padded to 64 float4, then the rest rows
+// padded to SHC_QMAX x 64 float4, so that the staging stores need no
+// predicate (which would let the compiler sink each load into its own
+// predicated branch and serialise the loads).
+constexpr int SHC_DC_F4 = PRE_THREADS;                                       // 64 float4
+constexpr int SHC_QMAX = (3 * 15 * PRE_THREADS / 4 + PRE_THREADS - 1) / PRE_THREADS;  // 12
+constexpr size_t SHC_SHMEM = (size_t)(SHC_DC_F4 + SHC_QMAX * PRE_THREADS) * sizeof(float4);
+
 __global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
-  extern __shared__ float rows[];  // 64 x 3 dc, then 64 x 3M rest
+  extern __shared__ float4 rows4[];
+  float* rows = (float*)rows4;
   const int lane = threadIdx.x;
   const int g0 = blockIdx.x * PRE_THREADS;
   const int i = g0 + lane;
   const int M = a.cam.max_coeffs;
-  float* sdc = rows;
-  float* ssh = rows + 3 * PRE_THREADS;
+  float* sdc = rows;                    // 64 x 3 dc
+  float* ssh = rows + 4 * SHC_DC_F4;    // 64 x 3M rest
   const int nb = min(PRE_THREADS, a.P - g0);
   const float* gdc = a.G.dc + 3 * (size_t)g0;
   const float* gsh = a.G.shs + (size_t)3 * M * g0;
+  // the mean is independent of the staging: start its load first
+  float pm[3] = {0.0f, 0.0f, 0.0f};
+  if (i < a.P) {
+    pm[0] = a.G.means[3 * (size_t)i];
+    pm[1] = a.G.means[3 * (size_t)i + 1];
+    pm[2] = a.G.means[3 * (size_t)i + 2];
+  }
   // full groups of 64 rows start 16-byte aligned when the arrays do
-  // (3 * 64 and 3 * M * 64 floats are multiples of 4): copy as float4
-  const bool vec = nb == PRE_THREADS &&
+  // (3 * 64 and 3 * M * 64 floats are multiples of 4): copy as float4 with
+  // every load of the group in flight before the first LDS store (indices
+  // clamped into the group; the padding takes the surplus stores)
+  const bool vec = nb == PRE_THREADS && M <= 15 &&
                    (((uintptr_t)a.G.dc | (uintptr_t)(M > 0 ? a.G.shs : a.G.dc)) & 15) == 0;
   if (vec) {
+    const int nq = 3 * M * PRE_THREADS / 4;
     const float4* d4 = (const float4*)gdc;
-    float4* s4 = (float4*)sdc;
-    if (lane < 3 * PRE_THREADS / 4) s4[lane] = d4[lane];
-    const float4* h4 = (const float4*)gsh;
-    float4* t4 = (float4*)ssh;
-    for (int k = lane; k < 3 * M * PRE_THREADS / 4; k += PRE_THREADS) t4[k] = h4[k];
+    const float4* h4 = M > 0 ? (const float4*)gsh : d4;
+    const float4 dv = d4[min(lane, 3 * PRE_THREADS / 4 - 1)];
+    float4 hv[SHC_QMAX];
+#pragma unroll
+    for (int u = 0; u < SHC_QMAX; ++u) hv[u] = h4[min(lane + u * PRE_THREADS, max(nq - 1, 0))];
+    rows4[lane] = dv;
+#pragma unroll
+    for (int u = 0; u < SHC_QMAX; ++u) rows4[SHC_DC_F4 + lane + u * PRE_THREADS] = hv[u];
   } else {
     for (int k = lane; k < 3 * nb; k += PRE_THREADS) sdc[k] = gdc[k];
     for (int k = lane; k < 3 * M * nb; k += PRE_THREADS) ssh[k] = gsh[k];
   }
   __syncthreads();
   if (i >= a.P) return;
-  const float* pm = a.G.means + 3 * (size_t)i;
   // the coefficients are staged once for all views of the batch
   for (int v = 0; v < a.V; ++v) {
     const size_t ri = (size_t)v * a.P + i;
@@ -301,7 +322,8 @@ hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s) {
   FGS_TIMED("preprocess_fast", s,
             (k_preprocess_fast<<<dim3(blocks, (unsigned)a.V), PRE_THREADS, 0, s>>>(a)));
   if (a.G.colors == nullptr && a.G.dc != nullptr) {
-    const size_t shmem = (size_t)PRE_THREADS * 3 * (1 + a.cam.max_coeffs) * sizeof(float);
+    const size_t rows = (size_t)(4 * SHC_DC_F4 + PRE_THREADS * 3 * a.cam.max_coeffs) * sizeof(float);
+    const size_t shmem = rows > SHC_SHMEM ? rows : SHC_SHMEM;
     FGS_TIMED("sh_color", s, (k_sh_color<<<blocks, PRE_THREADS, shmem, s>>>(a)));
   }
   return hipGetLastError();
