@@ -71,8 +71,11 @@ inline SortWs carve_sort(C& c, int64_t nb) {
   return w;
 }
 
+// Smallest tile (keys per workgroup) any radix pass uses: the workspace
+// holds per-tile rows for that many tiles.
+constexpr int SORT_MIN_TILE = 2048;
 inline size_t sort_ws_bytes(int64_t n) {
-  const size_t nb = (size_t)((n + SORT_TILE - 1) / SORT_TILE);
+  const size_t nb = (size_t)((n + SORT_MIN_TILE - 1) / SORT_MIN_TILE);
   const size_t rows = nb > 0 ? nb : 1;
   // classic passes: digit-major histogram + its scan
   const size_t hist = 256 * rows;

@@ -122,6 +122,16 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 }
 
 // ------------------------------------------------------- radix, classic
+// Keys per thread (4096-key tiles for every key width: measured on the
+// 21M-pair tile sort and the 1M-record depth sort, 8192-key tiles halve
+// the occupancy and 2048-key tiles double the histogram and look-back work,
+// both slower).
+template <typename K>
+struct SortTiling {
+  static constexpr int ITEMS = 16;
+  static constexpr int TILE = SORT_THREADS * ITEMS;
+  static_assert(TILE >= SORT_MIN_TILE, "workspace rows are sized for SORT_MIN_TILE");
+};
 // Digit layout: ceil(bits / 8) passes with the key bits spread evenly over
 // them (13 tile-id bits -> 7 + 6), which keeps each digit's run within a
 // 4096-key tile long enough for full-line stores.
@@ -174,8 +184,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const K* __restrict
   const int64_t n = dev_count(n_dev, n_cap);
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-  for (int c = threadIdx.x; c < SORT_TILE / N; c += SORT_THREADS) {
+  const int64_t base = (int64_t)blockIdx.x * SortTiling<K>::TILE;
+  for (int c = threadIdx.x; c < SortTiling<K>::TILE / N; c += SORT_THREADS) {
     const int64_t e = base + (int64_t)c * N;
     if (e >= n) break;
     KeyVec<K> kv;
@@ -198,16 +208,16 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const K* __restrict
 // bstart[d] = the tile's keys of digits < d, and rk[r] = rank of key r
 // within its (wave, digit).
 template <typename K>
-__device__ __forceinline__ void rank_tile(const K (&k)[SORT_ITEMS], int nvalid, int shift, int bits,
+__device__ __forceinline__ void rank_tile(const K (&k)[SortTiling<K>::ITEMS], int nvalid, int shift, int bits,
                                           uint32_t (&wcnt)[SORT_THREADS / WAVE][256],
-                                          uint32_t* bstart, uint32_t (&rk)[SORT_ITEMS],
+                                          uint32_t* bstart, uint32_t (&rk)[SortTiling<K>::ITEMS],
                                           uint32_t& count) {
   constexpr int NW = SORT_THREADS / WAVE;
-  constexpr int WITEMS = SORT_TILE / NW;
+  constexpr int WITEMS = SortTiling<K>::TILE / NW;
   const int t = threadIdx.x, w = t / WAVE, lane = lane_id();
   const uint32_t mask = (1u << bits) - 1u;
 #pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r) {
+  for (int r = 0; r < SortTiling<K>::ITEMS; ++r) {
     const bool valid = w * WITEMS + r * WAVE + lane < nvalid;
     const uint32_t d = (uint32_t)(k[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
@@ -238,12 +248,12 @@ __device__ __forceinline__ void rank_tile(const K (&k)[SORT_ITEMS], int nvalid, 
 // Loads this wave's 16 rows (keys and values) of the tile at base.
 template <typename K, bool VALS>
 __device__ __forceinline__ void load_tile(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                          int64_t base, int nvalid, K (&k)[SORT_ITEMS],
-                                          uint32_t (&v)[SORT_ITEMS]) {
-  constexpr int WITEMS = SORT_TILE / (SORT_THREADS / WAVE);
+                                          int64_t base, int nvalid, K (&k)[SortTiling<K>::ITEMS],
+                                          uint32_t (&v)[SortTiling<K>::ITEMS]) {
+  constexpr int WITEMS = SortTiling<K>::TILE / (SORT_THREADS / WAVE);
   const int w = threadIdx.x / WAVE, lane = lane_id();
 #pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r) {
+  for (int r = 0; r < SortTiling<K>::ITEMS; ++r) {
     const int e = w * WITEMS + r * WAVE + lane;
     k[r] = e < nvalid ? kin[base + e] : (K)0;
     v[r] = (VALS && e < nvalid) ? vin[base + e] : 0u;
@@ -253,17 +263,17 @@ __device__ __forceinline__ void load_tile(const K* __restrict__ kin, const uint3
 // Places the ranked keys (and values) in LDS in digit order, then writes
 // each digit's run contiguously from gofs[d].
 template <typename K, bool VALS>
-__device__ __forceinline__ void place_and_write(const K (&k)[SORT_ITEMS], const uint32_t (&v)[SORT_ITEMS],
-                                                const uint32_t (&rk)[SORT_ITEMS], int nvalid,
+__device__ __forceinline__ void place_and_write(const K (&k)[SortTiling<K>::ITEMS], const uint32_t (&v)[SortTiling<K>::ITEMS],
+                                                const uint32_t (&rk)[SortTiling<K>::ITEMS], int nvalid,
                                                 int shift, uint32_t mask,
                                                 uint32_t (&wcnt)[SORT_THREADS / WAVE][256],
                                                 const uint32_t* bstart, const uint32_t* gofs,
                                                 K* sk, uint32_t* sv, K* __restrict__ kout,
                                                 uint32_t* __restrict__ vout) {
-  constexpr int WITEMS = SORT_TILE / (SORT_THREADS / WAVE);
+  constexpr int WITEMS = SortTiling<K>::TILE / (SORT_THREADS / WAVE);
   const int t = threadIdx.x, w = t / WAVE, lane = lane_id();
 #pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r) {
+  for (int r = 0; r < SortTiling<K>::ITEMS; ++r) {
     if (w * WITEMS + r * WAVE + lane < nvalid) {
       const uint32_t d = (uint32_t)(k[r] >> shift) & mask;
       const uint32_t pos = bstart[d] + wcnt[w][d] + rk[r];
@@ -297,22 +307,22 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     uint32_t* __restrict__ vout, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
     int bits, const uint32_t* __restrict__ hist, int nb) {
   constexpr int NW = SORT_THREADS / WAVE;
-  __shared__ K sk[SORT_TILE];
-  __shared__ uint32_t sv[VALS ? SORT_TILE : 1];
+  __shared__ K sk[SortTiling<K>::TILE];
+  __shared__ uint32_t sv[VALS ? SortTiling<K>::TILE : 1];
   __shared__ uint32_t wcnt[NW][256];
   __shared__ uint32_t bstart[256];
   __shared__ uint32_t gofs[256];
   const int t = threadIdx.x;
   const int64_t tile = xcd_tile(blockIdx.x, nb);
-  const int64_t base = tile * SORT_TILE;
+  const int64_t base = tile * SortTiling<K>::TILE;
   const int64_t n = dev_count(n_dev, n_cap);
   if (tile >= nb || base >= n) return;  // whole workgroup: past the tiles / device-side count
-  const int nvalid = (int)min((int64_t)SORT_TILE, n - base);
+  const int nvalid = (int)min((int64_t)SortTiling<K>::TILE, n - base);
 #pragma unroll
   for (int q = 0; q < NW; ++q) wcnt[q][t] = 0u;
   gofs[t] = hist[(size_t)t * nb + tile];
-  K k[SORT_ITEMS];
-  uint32_t v[SORT_ITEMS], rk[SORT_ITEMS], cnt;
+  K k[SortTiling<K>::ITEMS];
+  uint32_t v[SortTiling<K>::ITEMS], rk[SortTiling<K>::ITEMS], cnt;
   load_tile<K, VALS>(kin, vin, base, nvalid, k, v);
   __syncthreads();
   rank_tile<K>(k, nvalid, shift, bits, wcnt, bstart, rk, cnt);
@@ -336,7 +346,7 @@ static hipError_t sort_kv_classic(K* keys, uint32_t* vals, K* keys_alt, uint32_t
                                   int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
                                   bool& in_alt, const uint32_t* n_dev, uint32_t* last_vals,
                                   const SortNames& nm) {
-  const int nb = (int)((n + SORT_TILE - 1) / SORT_TILE);
+  const int nb = (int)((n + SortTiling<K>::TILE - 1) / SortTiling<K>::TILE);
   uint32_t* hist = (uint32_t*)ws;
   void* scan_ws = (char*)ws + align_up((size_t)256 * nb * sizeof(uint32_t));
   K *ka = keys, *kb = keys_alt;
@@ -411,7 +421,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const K* __restrict_
   for (int64_t tl = tile0; tl < min(nb, tile0 + per_block); ++tl) status0[tl * 256 + t] = 0u;
   __syncthreads();
   const int64_t n = dev_count(n_dev, n_cap);
-  const int64_t e0 = tile0 * SORT_TILE, e1 = min(n, e0 + (int64_t)per_block * SORT_TILE);
+  const int64_t e0 = tile0 * SortTiling<K>::TILE, e1 = min(n, e0 + (int64_t)per_block * SortTiling<K>::TILE);
   constexpr int N = KeyVec<K>::N;
   for (int64_t e = e0 + (int64_t)t * N; e < e1; e += (int64_t)SORT_THREADS * N) {
     KeyVec<K> kv;
@@ -444,8 +454,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
     uint32_t* __restrict__ status, uint32_t* __restrict__ status_next,
     uint32_t* __restrict__ fault) {
   constexpr int NW = SORT_THREADS / WAVE;
-  __shared__ K sk[SORT_TILE];
-  __shared__ uint32_t sv[VALS ? SORT_TILE : 1];
+  __shared__ K sk[SortTiling<K>::TILE];
+  __shared__ uint32_t sv[VALS ? SortTiling<K>::TILE : 1];
   __shared__ uint32_t wcnt[NW][256];
   __shared__ uint32_t bstart[256];
   __shared__ uint32_t gofs[256];
@@ -458,12 +468,12 @@ __global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
   __syncthreads();
   const int64_t tile = s_tile;
   const int64_t n = dev_count(n_dev, n_cap);
-  const int64_t base = tile * SORT_TILE;
+  const int64_t base = tile * SortTiling<K>::TILE;
   if (base >= n) return;  // whole workgroup; no later tile waits on this one
   if (status_next) status_next[tile * 256 + t] = 0u;
-  const int nvalid = (int)min((int64_t)SORT_TILE, n - base);
-  K k[SORT_ITEMS];
-  uint32_t v[SORT_ITEMS], rk[SORT_ITEMS], c;
+  const int nvalid = (int)min((int64_t)SortTiling<K>::TILE, n - base);
+  K k[SortTiling<K>::ITEMS];
+  uint32_t v[SortTiling<K>::ITEMS], rk[SortTiling<K>::ITEMS], c;
   load_tile<K, VALS>(kin, vin, base, nvalid, k, v);
   rank_tile<K>(k, nvalid, shift, bits, wcnt, bstart, rk, c);
   // global offset of digit t: digits below it plus this digit in earlier tiles
@@ -508,7 +518,7 @@ static hipError_t sort_kv_onesweep(K* keys, uint32_t* vals, K* keys_alt, uint32_
                                    int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
                                    bool& in_alt, const uint32_t* n_dev, uint32_t* last_vals,
                                    const SortNames& nm) {
-  const int64_t nb = (n + SORT_TILE - 1) / SORT_TILE;
+  const int64_t nb = (n + SortTiling<K>::TILE - 1) / SortTiling<K>::TILE;
   const SortWs w = carve_sort(ws, nb);
   const int npass = sort_passes(begin_bit, end_bit);
   k_sort_prepare<<<1, 256, 0, s>>>(w.hist, w.counters);

@@ -44,6 +44,10 @@ class RasterOptions:
     bucket_size: int = 32
     split_tiles: int = 64
     tile_sched: object = None
+    # render with pair / bucket capacities remembered from earlier frames of
+    # the same shape instead of waiting for this frame's counts (the counts
+    # are checked while the render runs; an overflow renders again exactly)
+    speculative: bool = True
 
     def to_c(self, abs_grad: bool = True) -> Options:
         o = Options(self.schedule, self.bucket_size, self.split_tiles, 1 if abs_grad else 0, None)
@@ -97,6 +101,20 @@ def _stream(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
+# Largest pair count / bucket bound seen per frame shape (plus headroom):
+# the capacity a speculative forward renders with.
+_capacity_hints: dict = {}
+_side_streams: dict = {}
+
+
+def _side_stream(device) -> torch.cuda.Stream:
+    """A default-priority stream per device for reading a frame's counts."""
+    s = _side_streams.get(device)
+    if s is None:
+        s = _side_streams[device] = torch.cuda.Stream(device)
+    return s
+
+
 class Frame:
     """One forward evaluation: inputs, workspaces and outputs.
 
@@ -147,10 +165,11 @@ class Frame:
         self.gauss = Gaussians(P, self.M, _ptr(self.means3D), _ptr(self.opacities),
                                _ptr(self.scales), _ptr(self.rotations), _ptr(self.dc),
                                _ptr(self.shs), _ptr(self.colors), _ptr(self.culling))
-        self.R = self.NB = 0
+        self.R = self.NB = 0          # this frame's pair count and bucket-table bound
+        self.R_cap = self.NB_cap = 0  # capacities the pair workspace is laid out for
 
     # -- forward stages ---------------------------------------------------
-    def preprocess(self):
+    def preprocess(self, sync: bool = True):
         L, dev = lib(), self.device
         self.radii = torch.zeros(self.P, dtype=torch.int32, device=dev)
         n = L.fgs_gauss_ws_size(self.P, self.W, self.H, ctypes.byref(self.copts))
@@ -158,28 +177,65 @@ class Frame:
         check(L.fgs_preprocess(ctypes.byref(self.cam), ctypes.byref(self.gauss),
                                ctypes.byref(self.copts), _ptr(self.gws), n, _ptr(self.radii),
                                ctypes.c_void_p(_stream(dev))), "fgs_preprocess")
-        counts = (ctypes.c_int64 * 2)()
-        check(L.fgs_read_counts(_ptr(self.gws), self.P, self.W, self.H, ctypes.byref(self.copts),
-                                counts, ctypes.c_void_p(_stream(dev))), "fgs_read_counts")
-        self.R, self.NB = int(counts[0]), int(counts[1])
+        if sync:
+            self.read_counts(torch.cuda.current_stream(dev))
+            self.R_cap, self.NB_cap = self.R, self.NB
         return self
+
+    def read_counts(self, stream):
+        """Pair count R and bucket-table capacity NB of the preprocessed frame
+        (waits on `stream` only)."""
+        counts = (ctypes.c_int64 * 2)()
+        check(lib().fgs_read_counts(_ptr(self.gws), self.P, self.W, self.H,
+                                    ctypes.byref(self.copts), counts,
+                                    ctypes.c_void_p(stream.cuda_stream)), "fgs_read_counts")
+        self.R, self.NB = int(counts[0]), int(counts[1])
 
     def render(self):
         L, dev = lib(), self.device
-        n = L.fgs_pair_ws_size(self.P, self.W, self.H, self.R, self.NB, ctypes.byref(self.copts))
+        n = L.fgs_pair_ws_size(self.P, self.W, self.H, self.R_cap, self.NB_cap,
+                               ctypes.byref(self.copts))
         self.pws = torch.empty(n, dtype=torch.uint8, device=dev)
         self.out_color = torch.empty((3, self.H, self.W), dtype=torch.float32, device=dev)
         self.T_final = torch.empty((self.H, self.W, 1), dtype=torch.float32, device=dev)
         self.last_contributor = torch.empty((self.H, self.W, 1), dtype=torch.int32, device=dev)
         check(L.fgs_render(ctypes.byref(self.cam), ctypes.byref(self.gauss),
                            ctypes.byref(self.copts), _ptr(self.gws), self.gws.numel(),
-                           _ptr(self.pws), n, self.R, self.NB, _ptr(self.out_color),
+                           _ptr(self.pws), n, self.R_cap, self.NB_cap, _ptr(self.out_color),
                            _ptr(self.T_final), _ptr(self.last_contributor),
                            ctypes.c_void_p(_stream(dev))), "fgs_render")
         return self
 
     def forward(self):
-        return self.preprocess().render()
+        """Preprocess and render. With speculative options and a capacity
+        remembered for frames of this shape, the render is launched at once
+        with that capacity, and the frame's counts are read on a side stream
+        that waits for the preprocess only, so the host check overlaps the
+        render; a frame that exceeds the capacity is rendered again with its
+        exact counts (the outputs never depend on the capacity)."""
+        key = (self.P, self.W, self.H, self.opts.schedule, self.opts.bucket_size, self.device)
+        hint = _capacity_hints.get(key) if self.opts.speculative else None
+        if hint is None or self.P == 0:
+            self.preprocess(sync=True).render()
+        else:
+            dev = self.device
+            self.preprocess(sync=False)
+            main = torch.cuda.current_stream(dev)
+            done = torch.cuda.Event()
+            done.record(main)
+            side = _side_stream(dev)
+            side.wait_event(done)
+            self.R_cap, self.NB_cap = hint
+            self.render()
+            self.read_counts(side)
+            if self.R > self.R_cap or self.NB > self.NB_cap:
+                self.R_cap, self.NB_cap = self.R, self.NB
+                self.render()
+        # remember the largest counts seen for this shape, with headroom
+        old = _capacity_hints.get(key, (0, 0))
+        _capacity_hints[key] = (max(old[0], self.R + self.R // 8 + 1024),
+                                max(old[1], self.NB + self.NB // 8 + 64))
+        return self
 
     # -- backward -----------------------------------------------------------
     def backward(self, grad_out_color: torch.Tensor) -> dict:
@@ -196,8 +252,8 @@ class Frame:
                                                 "rotations")])
         gws = torch.empty(L.fgs_grad_ws_size(P), dtype=torch.uint8, device=dev)
         check(L.fgs_backward(ctypes.byref(self.cam), ctypes.byref(self.gauss),
-                             ctypes.byref(self.copts), _ptr(self.gws), _ptr(self.pws), self.R,
-                             self.NB, _ptr(self.radii), _ptr(self.T_final),
+                             ctypes.byref(self.copts), _ptr(self.gws), _ptr(self.pws), self.R_cap,
+                             self.NB_cap, _ptr(self.radii), _ptr(self.T_final),
                              _ptr(self.last_contributor), _ptr(g), _ptr(gws),
                              ctypes.byref(grads), ctypes.c_void_p(_stream(dev))), "fgs_backward")
         out["raw"] = gws[:P * 48].view(torch.float32).reshape(P, 12)
@@ -216,8 +272,8 @@ class Frame:
         w = torch.empty(P, dtype=torch.float32, device=dev)
         c = torch.empty(P, dtype=torch.int32, device=dev)
         m = torch.empty(P, dtype=torch.float32, device=dev)
-        check(lib().fgs_render_simp(_ptr(self.gws), _ptr(self.pws), P, self.W, self.H, self.R,
-                                    self.NB, ctypes.byref(self.copts), _ptr(w), _ptr(c), _ptr(m),
+        check(lib().fgs_render_simp(_ptr(self.gws), _ptr(self.pws), P, self.W, self.H, self.R_cap,
+                                    self.NB_cap, ctypes.byref(self.copts), _ptr(w), _ptr(c), _ptr(m),
                                     ctypes.c_void_p(_stream(dev))), "fgs_render_simp")
         return w, c, m
 
@@ -229,8 +285,8 @@ class Frame:
         e = lambda c, dt=torch.float32: torch.empty((c, H, W), dtype=dt, device=dev)
         out = {"out_pts": e(3), "rendered_depth": e(1), "accum_alpha": e(1),
                "gidx": e(1, torch.int32), "discriminants": e(1)}
-        check(lib().fgs_render_depth(_ptr(self.gws), _ptr(self.pws), self.P, W, H, self.R,
-                                     self.NB, ctypes.byref(self.copts), _ptr(self.means3D),
+        check(lib().fgs_render_depth(_ptr(self.gws), _ptr(self.pws), self.P, W, H, self.R_cap,
+                                     self.NB_cap, ctypes.byref(self.copts), _ptr(self.means3D),
                                      _ptr(self.scales), _ptr(self.rotations), _ptr(self.proj),
                                      _ptr(self.campos), _ptr(out["out_pts"]),
                                      _ptr(out["rendered_depth"]), _ptr(out["accum_alpha"]),
@@ -243,7 +299,8 @@ class Frame:
         """Torch views of the intermediate arrays (valid until the frame dies)."""
         v = FrameView()
         check(lib().fgs_frame_view_get(_ptr(self.gws), _ptr(getattr(self, "pws", None)), self.P,
-                                       self.W, self.H, self.R, self.NB, ctypes.byref(self.copts),
+                                       self.W, self.H, self.R_cap, self.NB_cap,
+                                       ctypes.byref(self.copts),
                                        ctypes.byref(v)), "fgs_frame_view_get")
         T = ((self.W + 15) // 16) * ((self.H + 15) // 16)
         spec = {

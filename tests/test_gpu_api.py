@@ -158,3 +158,40 @@ def test_degree0_dc_gradient(schedule):
     np.testing.assert_allclose(ddc[vis].cpu().numpy(), (C0 * dcol[vis]).cpu().numpy(),
                                rtol=1e-6, atol=1e-12)
     assert (ddc[~vis] == 0).all()
+
+
+@pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "fast"])
+def test_speculative_capacity_matches_exact(schedule):
+    """A forward rendered with a remembered capacity -- larger than needed,
+    or too small so that it renders again -- gives the same image, radii and
+    gradients as the exact (synchronising) forward."""
+    from fastergs_amd import rasterizer as rz
+    from fastergs_amd.rasterizer import Frame, RasterOptions
+    scene, cam = scenes.uniform_scene(20000, seed=9), scenes.look_at_camera((0.1, 0.2, -4.0), 300, 220)
+    d = to_dev(scene)
+    dimg = torch.from_numpy(np.random.default_rng(3).normal(size=(3, 220, 300)).astype(np.float32)).cuda()
+
+    def run(speculative, hint=None):
+        opts = RasterOptions(schedule=schedule, speculative=speculative)
+        key = (scene["means3D"].shape[0], 300, 220, schedule, 32, d["means3D"].device)
+        rz._capacity_hints.pop(key, None)
+        if hint is not None:
+            rz._capacity_hints[key] = hint
+        fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+                   dc=d["dc"], shs=d["shs"], options=opts).forward()
+        g = fr.backward(dimg)
+        torch.cuda.synchronize()
+        return fr, {k: v.cpu().numpy() for k, v in g.items()}
+
+    ref, gref = run(False)
+    for hint in [(ref.R * 2 + 5000, ref.NB * 2 + 100), (ref.R // 3, ref.NB // 3), (ref.R, ref.NB)]:
+        fr, g = run(True, hint)
+        assert (fr.R, fr.NB) == (ref.R, ref.NB)
+        assert fr.R_cap >= fr.R and fr.NB_cap >= fr.NB
+        np.testing.assert_array_equal(fr.out_color.cpu().numpy(), ref.out_color.cpu().numpy())
+        np.testing.assert_array_equal(fr.radii.cpu().numpy(), ref.radii.cpu().numpy())
+        np.testing.assert_array_equal(fr.view_arrays()["sorted_vals"].cpu().numpy(),
+                                      ref.view_arrays()["sorted_vals"].cpu().numpy())
+        for k in ("opacities", "dc", "shs"):
+            grad_close(g[k], gref[k], rtol=1e-4, name=k)  # float atomics: run-to-run order
+    assert rz._capacity_hints  # the forward remembers the shape's counts
