@@ -389,10 +389,9 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
                             s, alt, nullptr, nullptr, SORT_DEPTH),
           "depth sort");
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
-  FGS_TRY(launch_gather_scatter(true, (int)PV, w.dsort_v0, w.pcount, w.rcount, s), "gather");
-  FGS_TRY(scan_u32(w.rcount, w.roff, PV, w.counts + 0, w.scratch, s), "pair scan");
-  FGS_TRY(launch_gather_scatter(false, (int)PV, w.dsort_v0, w.roff, w.poff, s), "scatter");
-  FGS_TRY(launch_nb_capacity(w.counts, opt->bucket_size, b.T, s), "bucket capacity");
+  FGS_TRY(scan_pair_offsets(w.pcount, w.dsort_v0, PV, w.rcount, w.poff, w.counts,
+                            opt->bucket_size, b.T, w.scratch, s),
+          "pair offsets");
   return FGS_OK;
 }
 
@@ -482,11 +481,10 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
         FGS_TRY(sort_kv<uint32_t>((uint32_t*)pw.t0, pw.r0, (uint32_t*)pw.t1, pw.r1, R, 0, tb,
                                   pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES),
                 "tile sort");
-      FGS_TRY(launch_tile_ranges(R, gw.counts, alt ? pw.t1 : pw.t0, pw.tile16, gw.ranges, T, s),
-              "ranges");
+      FGS_TRY(launch_tile_ranges(R, gw.counts, alt ? pw.t1 : pw.t0, pw.tile16, gw.ranges, T,
+                                 opt->bucket_size, gw.bcount, gw.maxc, s),
+              "ranges and bucket counts");
     }
-    FGS_TRY(launch_bucket_counts(T, gw.ranges, opt->bucket_size, gw.bcount, gw.maxc, s),
-            "bucket counts");
     FGS_TRY(scan_u32(gw.bcount, gw.boff, T, gw.boff + T, gw.scratch, s), "bucket scan");
     FGS_TRY(launch_bucket_tables(T, NB, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
                                  pw.bucket_tile, s),

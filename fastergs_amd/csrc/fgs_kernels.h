@@ -70,6 +70,12 @@ __device__ __forceinline__ int view_of_tile(const ViewRec* views, int V, uint32_
 // scan_sort.hip
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
                     hipStream_t s);
+// Pair offsets of the FAST preprocess: off[order[r]] = exclusive prefix of
+// count[order[r]] over r (records in depth order), counts[0] = the total,
+// counts[1] = ceil(total / bucket) + T; gathered is n words of scratch.
+hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64_t n,
+                             uint32_t* gathered, uint32_t* off, uint32_t* counts, int bucket,
+                             int T, void* ws, hipStream_t s);
 // Stable LSD radix sort of (key, value) pairs on key bits [begin_bit,
 // end_bit) (keys of 16, 32 or 64 bits; vals may be NULL for key-only sorts).
 // Passes alternate between the arrays and their *_alt twins; in_alt tells
@@ -123,8 +129,6 @@ hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* ge
                              bool t16, hipStream_t s);
 hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lpos,
                                 uint32_t* large_ids, hipStream_t s);
-hipError_t launch_gather_scatter(bool gather, int P, const uint32_t* order, const uint32_t* src,
-                                 uint32_t* dst, hipStream_t s);
 
 // render.hip
 // Range kernels take the pair count as (capacity R, device count n_dev).
@@ -133,13 +137,13 @@ hipError_t launch_identify_ranges(int64_t R, const uint32_t* n_dev, const uint64
 hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, const uint32_t* boff,
                                 int bucket, uint32_t* bucket_ranges, uint32_t* bucket_tile,
                                 hipStream_t s);
+// Ranges from the sorted tile ids, plus per-tile bucket counts and cleared
+// furthest-contributor slots (the FAST schedule's bucket_counts).
 hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tiles, bool t16,
-                              uint32_t* ranges, int T, hipStream_t s);
-hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
-                                uint32_t* maxc, hipStream_t s);
+                              uint32_t* ranges, int T, int bucket, uint32_t* bcount,
+                              uint32_t* maxc, hipStream_t s);
 hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
                                 hipStream_t s);
-hipError_t launch_nb_capacity(uint32_t* counts, int bucket, int T, hipStream_t s);
 struct CompArgs {
   int W, H, grid_x, T, bucket;
   const uint32_t* ranges;

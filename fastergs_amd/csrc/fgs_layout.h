@@ -106,7 +106,6 @@ struct GaussWs {
   uint32_t* lpos;      // fast: exclusive prefix of lflag
   uint32_t* large_ids; // fast: flagged Gaussians in index order
   uint32_t* rcount;    // fast: pair counts in depth order
-  uint32_t* roff;      // fast: their exclusive prefix
   uint32_t* ranges;    // T x 2
   uint32_t* boff;      // T + 1: exclusive prefix of per-tile bucket counts, [T] = NB
   uint32_t* bcount;    // T + 1 scratch (bucket counts, then active bucket counts)
@@ -143,7 +142,6 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   w.lpos = c.take<uint32_t>(p + 1);
   w.large_ids = c.take<uint32_t>(p);
   w.rcount = c.take<uint32_t>(p);
-  w.roff = c.take<uint32_t>(p + 1);
   w.ranges = c.take<uint32_t>(2 * t);
   w.boff = c.take<uint32_t>(t + 1);
   w.bcount = c.take<uint32_t>(t + 1);

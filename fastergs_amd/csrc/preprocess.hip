@@ -498,26 +498,5 @@ hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lp
   return hipGetLastError();
 }
 
-// Pair counts gathered in depth order (rank r -> Gaussian order[r]).
-__global__ void k_gather_u32(int P, const uint32_t* __restrict__ order,
-                             const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r < P) dst[r] = src[order[r]];
-}
-// Offsets scattered back to Gaussian order.
-__global__ void k_scatter_u32(int P, const uint32_t* __restrict__ order,
-                              const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r < P) dst[order[r]] = src[r];
-}
-hipError_t launch_gather_scatter(bool gather, int P, const uint32_t* order, const uint32_t* src,
-                                 uint32_t* dst, hipStream_t s) {
-  if (P <= 0) return hipSuccess;
-  if (gather)
-    k_gather_u32<<<(P + 255) / 256, 256, 0, s>>>(P, order, src, dst);
-  else
-    k_scatter_u32<<<(P + 255) / 256, 256, 0, s>>>(P, order, src, dst);
-  return hipGetLastError();
-}
 
 }  // namespace fgs

@@ -55,21 +55,6 @@ hipError_t launch_identify_ranges(int64_t R, const uint32_t* n_dev, const uint64
   return hipGetLastError();
 }
 
-// Also clears the per-tile max contributor that compositing combines with
-// atomicMax when a tile is split into row bands.
-__global__ void k_bucket_counts(int T, const uint32_t* __restrict__ ranges, int bucket,
-                                uint32_t* __restrict__ bcount, uint32_t* __restrict__ maxc) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= T) return;
-  const uint32_t len = ranges[2 * t + 1] - ranges[2 * t];
-  bcount[t] = (len + bucket - 1) / bucket;
-  if (maxc) maxc[t] = 0u;
-}
-hipError_t launch_bucket_counts(int T, const uint32_t* ranges, int bucket, uint32_t* bcount,
-                                uint32_t* maxc, hipStream_t s) {
-  if (T > 0) k_bucket_counts<<<(T + 255) / 256, 256, 0, s>>>(T, ranges, bucket, bcount, maxc);
-  return hipGetLastError();
-}
 
 // boff = exclusive prefix of bucket counts (boff[t] = first bucket of tile
 // t, boff[T] = NB). One thread per bucket: its tile is the last t with
@@ -111,9 +96,14 @@ __device__ __forceinline__ uint32_t lower_bound_tile(const TK* a, uint32_t n, ui
   }
   return lo;
 }
+// Also writes each tile's bucket count ceil(len / bucket) and clears its
+// furthest-contributor slot, which compositing combines with atomicMax when
+// a tile is split into row bands.
 template <typename TK>
 __global__ void k_tile_ranges(int64_t R_cap, const uint32_t* __restrict__ n_dev,
-                              const TK* __restrict__ tiles, uint32_t* __restrict__ ranges, int T) {
+                              const TK* __restrict__ tiles, uint32_t* __restrict__ ranges, int T,
+                              int bucket, uint32_t* __restrict__ bcount,
+                              uint32_t* __restrict__ maxc) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= T) return;
   const uint32_t R = (uint32_t)dev_count(n_dev, R_cap);
@@ -122,19 +112,22 @@ __global__ void k_tile_ranges(int64_t R_cap, const uint32_t* __restrict__ n_dev,
   // empty tiles keep [0, 0) like identifyTileRanges (render.cu:406-411)
   ranges[2 * t] = lo < hi ? lo : 0u;
   ranges[2 * t + 1] = lo < hi ? hi : 0u;
+  bcount[t] = (hi - lo + bucket - 1) / bucket;
+  maxc[t] = 0u;
 }
 hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tiles, bool t16,
-                              uint32_t* ranges, int T, hipStream_t s) {
+                              uint32_t* ranges, int T, int bucket, uint32_t* bcount,
+                              uint32_t* maxc, hipStream_t s) {
   if (T <= 0) return hipSuccess;
   const unsigned grid = (unsigned)((T + 255) / 256);
   if (t16)
     FGS_TIMED("tile_ranges", s,
               (k_tile_ranges<uint16_t><<<grid, 256, 0, s>>>(R, n_dev, (const uint16_t*)tiles,
-                                                            ranges, T)));
+                                                            ranges, T, bucket, bcount, maxc)));
   else
     FGS_TIMED("tile_ranges", s,
               (k_tile_ranges<uint32_t><<<grid, 256, 0, s>>>(R, n_dev, (const uint32_t*)tiles,
-                                                            ranges, T)));
+                                                            ranges, T, bucket, bcount, maxc)));
   return hipGetLastError();
 }
 
@@ -151,15 +144,6 @@ hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_
   return hipGetLastError();
 }
 
-// Capacity for the bucket tables before the per-tile counts exist:
-// sum_t ceil(len_t / B) <= ceil(R / B) + T.
-__global__ void k_nb_capacity(uint32_t* counts, int bucket, int T) {
-  if (threadIdx.x == 0) counts[1] = (counts[0] + bucket - 1) / bucket + (uint32_t)T;
-}
-hipError_t launch_nb_capacity(uint32_t* counts, int bucket, int T, hipStream_t s) {
-  k_nb_capacity<<<1, 64, 0, s>>>(counts, bucket, T);
-  return hipGetLastError();
-}
 
 // ------------------------------------------------------ PLAIN compositing
 __global__ __launch_bounds__(256) void k_composite_plain(CompArgs a) {

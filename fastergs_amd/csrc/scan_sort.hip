@@ -121,6 +121,115 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
   return hipGetLastError();
 }
 
+// ------------------------------------------------------ pair offsets
+// Exclusive scan of the records' pair counts taken in depth order
+// (count[order[r]]), written back in record order (off[order[r]]), with
+// the total pair count and the bucket-table bound ceil(total / B) + T: the
+// gather, scan, scatter and bound of the FAST preprocess in three launches
+// (one up to 8192 records).
+__global__ __launch_bounds__(SCAN_THREADS) void k_pscan_reduce(const uint32_t* __restrict__ count,
+                                                               const uint32_t* __restrict__ order,
+                                                               int64_t n,
+                                                               uint32_t* __restrict__ gathered,
+                                                               uint32_t* __restrict__ partial) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t i = base + k * SCAN_THREADS + threadIdx.x;
+    if (i < n) {
+      const uint32_t v = count[order[i]];
+      gathered[i] = v;
+      s += v;
+    }
+  }
+  uint32_t tot;
+  block_excl_scan<SCAN_THREADS>(s, tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_pscan_partials(uint32_t* __restrict__ partial, int nb,
+                                                         uint32_t* __restrict__ counts, int bucket,
+                                                         int T) {
+  uint32_t carry = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    const int i = base + threadIdx.x;
+    const uint32_t v = i < nb ? partial[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<1024>(v, tot);
+    if (i < nb) partial[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    counts[0] = carry;
+    counts[1] = (carry + bucket - 1) / bucket + (uint32_t)T;
+  }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_pscan_apply(const uint32_t* __restrict__ gathered,
+                                                              const uint32_t* __restrict__ order,
+                                                              int64_t n,
+                                                              const uint32_t* __restrict__ partial,
+                                                              uint32_t* __restrict__ off) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = (base + k < n) ? gathered[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan<SCAN_THREADS>(s, tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) off[order[base + k]] = run;
+    run += v[k];
+  }
+}
+
+__global__ __launch_bounds__(SCAN1_THREADS) void k_pscan_single(const uint32_t* __restrict__ count,
+                                                                const uint32_t* __restrict__ order,
+                                                                int64_t n,
+                                                                uint32_t* __restrict__ off,
+                                                                uint32_t* __restrict__ counts,
+                                                                int bucket, int T) {
+  const int64_t b = (int64_t)threadIdx.x * SCAN1_ITEMS;
+  uint32_t v[SCAN1_ITEMS];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN1_ITEMS; ++k) {
+    v[k] = b + k < n ? count[order[b + k]] : 0u;
+    sum += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan<SCAN1_THREADS>(sum, tot);
+#pragma unroll
+  for (int k = 0; k < SCAN1_ITEMS; ++k) {
+    if (b + k < n) off[order[b + k]] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) {
+    counts[0] = tot;
+    counts[1] = (tot + bucket - 1) / bucket + (uint32_t)T;
+  }
+}
+
+hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64_t n,
+                             uint32_t* gathered, uint32_t* off, uint32_t* counts, int bucket,
+                             int T, void* ws, hipStream_t s) {
+  if (n <= SCAN1_MAX) {
+    k_pscan_single<<<1, SCAN1_THREADS, 0, s>>>(count, order, n, off, counts, bucket, T);
+    return hipGetLastError();
+  }
+  const int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
+  uint32_t* partial = (uint32_t*)ws;
+  k_pscan_reduce<<<nb, SCAN_THREADS, 0, s>>>(count, order, n, gathered, partial);
+  k_pscan_partials<<<1, 1024, 0, s>>>(partial, nb, counts, bucket, T);
+  k_pscan_apply<<<nb, SCAN_THREADS, 0, s>>>(gathered, order, n, partial, off);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------- radix, classic
 // Keys per thread (4096-key tiles for every key width: measured on the
 // 21M-pair tile sort and the 1M-record depth sort, 8192-key tiles halve

@@ -88,7 +88,7 @@ class BatchFrame:
 
     def preprocess(self):
         L = lib()
-        self.radii = torch.zeros(self.V, self.P, dtype=torch.int32, device=self.device)
+        self.radii = torch.empty(self.V, self.P, dtype=torch.int32, device=self.device)  # every entry written
         n = L.fgs_batch_gauss_ws_size(self.P, ctypes.byref(self.batch), ctypes.byref(self.copts))
         self.gws = torch.empty(n, dtype=torch.uint8, device=self.device)
         check(L.fgs_batch_preprocess(ctypes.byref(self.batch), ctypes.byref(self.gauss),

@@ -171,7 +171,7 @@ class Frame:
     # -- forward stages ---------------------------------------------------
     def preprocess(self, sync: bool = True):
         L, dev = lib(), self.device
-        self.radii = torch.zeros(self.P, dtype=torch.int32, device=dev)
+        self.radii = torch.empty(self.P, dtype=torch.int32, device=dev)  # every entry written
         n = L.fgs_gauss_ws_size(self.P, self.W, self.H, ctypes.byref(self.copts))
         self.gws = torch.empty(n, dtype=torch.uint8, device=dev)
         check(L.fgs_preprocess(ctypes.byref(self.cam), ctypes.byref(self.gauss),
