@@ -376,8 +376,7 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   if (g->P == 0) return FGS_OK;
   FGS_TRY(launch_preprocess_fast(a, s), "preprocess");
   // large rects, compacted (count in counts[3]) and walked by whole workgroups
-  FGS_TRY(scan_u32(w.lflag, w.lpos, PV, w.counts + 3, w.scratch, s), "large scan");
-  FGS_TRY(launch_compact_large((int)PV, w.lflag, w.lpos, w.large_ids, s), "large compact");
+  FGS_TRY(compact_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.scratch, s), "large compact");
   FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
                              w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
           "large tiles");

@@ -76,6 +76,10 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64_t n,
                              uint32_t* gathered, uint32_t* off, uint32_t* counts, int bucket,
                              int T, void* ws, hipStream_t s);
+// ids = the indices i < n with flag[i] (0 / 1) set, ascending; *count = their
+// number (on the device).
+hipError_t compact_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t* count, void* ws,
+                         hipStream_t s);
 // Stable LSD radix sort of (key, value) pairs on key bits [begin_bit,
 // end_bit) (keys of 16, 32 or 64 bits; vals may be NULL for key-only sorts).
 // Passes alternate between the arrays and their *_alt twins; in_alt tells
@@ -127,8 +131,6 @@ hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, con
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
                              const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
                              bool t16, hipStream_t s);
-hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lpos,
-                                uint32_t* large_ids, hipStream_t s);
 
 // render.hip
 // Range kernels take the pair count as (capacity R, device count n_dev).

@@ -103,7 +103,6 @@ struct GaussWs {
   uint32_t* dsort_v0;  // fast: Gaussian index by depth rank after the sort
   uint32_t* dsort_v1;
   uint32_t* lflag;     // fast: rect walked by whole workgroups (1) or lane-packed (0)
-  uint32_t* lpos;      // fast: exclusive prefix of lflag
   uint32_t* large_ids; // fast: flagged Gaussians in index order
   uint32_t* rcount;    // fast: pair counts in depth order
   uint32_t* ranges;    // T x 2
@@ -139,7 +138,6 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   w.dsort_v0 = c.take<uint32_t>(p);
   w.dsort_v1 = c.take<uint32_t>(p);
   w.lflag = c.take<uint32_t>(p);
-  w.lpos = c.take<uint32_t>(p + 1);
   w.large_ids = c.take<uint32_t>(p);
   w.rcount = c.take<uint32_t>(p);
   w.ranges = c.take<uint32_t>(2 * t);

@@ -485,18 +485,4 @@ hipError_t launch_views_setup(const ViewSetup& src, ViewRec* out, hipStream_t s)
   return hipGetLastError();
 }
 
-// large_ids[lpos[i]] = i for every flagged Gaussian (lpos = exclusive scan of
-// the flags, so the list is in index order).
-__global__ void k_compact_large(int P, const uint32_t* __restrict__ lflag,
-                                const uint32_t* __restrict__ lpos, uint32_t* __restrict__ large_ids) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < P && lflag[i]) large_ids[lpos[i]] = (uint32_t)i;
-}
-hipError_t launch_compact_large(int P, const uint32_t* lflag, const uint32_t* lpos,
-                                uint32_t* large_ids, hipStream_t s) {
-  if (P > 0) k_compact_large<<<(P + 255) / 256, 256, 0, s>>>(P, lflag, lpos, large_ids);
-  return hipGetLastError();
-}
-
-
 }  // namespace fgs

@@ -230,6 +230,67 @@ hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------ compaction
+// Indices i with flag[i] != 0, in index order, and their number: a scan
+// whose last launch writes the indices directly (one launch up to 8192).
+__global__ __launch_bounds__(SCAN_THREADS) void k_compact_apply(const uint32_t* __restrict__ flag,
+                                                                int64_t n,
+                                                                const uint32_t* __restrict__ partial,
+                                                                uint32_t* __restrict__ ids) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = (base + k < n && flag[base + k]) ? 1u : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan<SCAN_THREADS>(s, tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (v[k]) ids[run] = (uint32_t)(base + k);
+    run += v[k];
+  }
+}
+
+__global__ __launch_bounds__(SCAN1_THREADS) void k_compact_single(const uint32_t* __restrict__ flag,
+                                                                  int64_t n,
+                                                                  uint32_t* __restrict__ ids,
+                                                                  uint32_t* __restrict__ count) {
+  const int64_t b = (int64_t)threadIdx.x * SCAN1_ITEMS;
+  uint32_t v[SCAN1_ITEMS];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN1_ITEMS; ++k) {
+    v[k] = (b + k < n && flag[b + k]) ? 1u : 0u;
+    sum += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan<SCAN1_THREADS>(sum, tot);
+#pragma unroll
+  for (int k = 0; k < SCAN1_ITEMS; ++k) {
+    if (v[k]) ids[run] = (uint32_t)(b + k);
+    run += v[k];
+  }
+  if (threadIdx.x == 0) *count = tot;
+}
+
+hipError_t compact_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t* count, void* ws,
+                         hipStream_t s) {
+  if (n <= SCAN1_MAX) {
+    k_compact_single<<<1, SCAN1_THREADS, 0, s>>>(flag, n, ids, count);
+    return hipGetLastError();
+  }
+  const int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
+  uint32_t* partial = (uint32_t*)ws;
+  // flags are 0 / 1, so the plain per-tile sums and their scan apply
+  k_scan_reduce<<<nb, SCAN_THREADS, 0, s>>>(flag, n, partial);
+  k_scan_partials<<<1, 1024, 0, s>>>(partial, nb, count);
+  k_compact_apply<<<nb, SCAN_THREADS, 0, s>>>(flag, n, partial, ids);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------- radix, classic
 // Keys per thread (4096-key tiles for every key width: measured on the
 // 21M-pair tile sort and the 1M-record depth sort, 8192-key tiles halve
