@@ -603,15 +603,18 @@ FGS_HD void pix_bwd(R gx, R gy, R ca, R cb, R cc, R w, R r, R g, R b, R px, R py
   acc[RAW_COL2] = fma_(wgt, dp2, acc[RAW_COL2]);
   dLda = fma_(-tb, inv, dLda);  // tb = T_final * (bg . dL/dpixel), per pixel
   const R dLdp = dLda * (a * R(LN2_F));
-  const R mx = dLdp * (R(2) * ca * dx + cb * dy);
-  const R my = dLdp * (R(2) * cc * dy + cb * dx);
+  // d power / d(gx, gy) = (2 ca dx + cb dy, 2 cc dy + cb dx) and d power /
+  // d(ca, cb, cc) = (dx^2, dx dy, dy^2), through u = dLdp dx, v = dLdp dy
+  const R u = dLdp * dx, v = dLdp * dy;
+  const R mx = fma_(R(2) * ca, u, cb * v);
+  const R my = fma_(R(2) * cc, v, cb * u);
   acc[RAW_MX] += mx;
   acc[RAW_MY] += my;
   acc[RAW_AX] += abs_(mx);
   acc[RAW_AY] += abs_(my);
-  acc[RAW_KXX] = fma_(dLdp, dx * dx, acc[RAW_KXX]);
-  acc[RAW_KXY] = fma_(dLdp, dx * dy, acc[RAW_KXY]);
-  acc[RAW_KYY] = fma_(dLdp, dy * dy, acc[RAW_KYY]);
+  acc[RAW_KXX] = fma_(u, dx, acc[RAW_KXX]);
+  acc[RAW_KXY] = fma_(u, dy, acc[RAW_KXY]);
+  acc[RAW_KYY] = fma_(v, dy, acc[RAW_KYY]);
   acc[RAW_POW] += dLdp;
 }
 
