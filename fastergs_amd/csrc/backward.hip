@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
   const float dp2 = inside ? a.dimg[2 * HW + g] : 0.0f;
   const float tb = Tf * (a.bg[0] * dp0 + a.bg[1] * dp1 + a.bg[2] * dp2);
   const float fx = (float)px, fy = (float)py;
-  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+  float Q = tb;  // empty residual: background only
   for (int top = mc; top > 0; top -= 256) {
     const int lo = top > 256 ? top - 256 : 0;
     const int n = top - lo;
@@ -113,8 +113,8 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
       for (int f = 0; f < RAW_N; ++f) acc[f] = 0.0f;
       if (act) {
         const float4 A = sA[j], Bv = sB[j];
-        pix_bwd(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, sC[j], fx, fy, dp0, dp1, dp2, tb, T,
-                r0, r1, r2, acc);
+        pix_bwd(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, sC[j], fx, fy, dp0, dp1, dp2, T, Q,
+                acc);
       }
       wave_flush<NF>(acc, a.raw, sId[j]);
     }
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
     const float* dimg = a.dimg + 3 * poff;
     const int lt = tile - (int)vr.tile_off;
     const int tx = lt % gx, ty = lt / gx;
-    float T[4], r0[4], r1[4], r2[4], d0[4], d1[4], d2[4], tb[4], fx[4], fy[4];
+    float T[4], Q[4], d0[4], d1[4], d2[4], fx[4], fy[4];
     bool act[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -180,19 +180,16 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
       const size_t gl = inside ? (size_t)py * W + px : 0;  // pixel in the view
       const size_t g = poff + gl;                            // pixel in the batch
       act[k] = inside && lb * (uint32_t)a.bucket < (uint32_t)a.contrib[g];
-      T[k] = 1.0f; r0[k] = r1[k] = r2[k] = 0.0f; d0[k] = d1[k] = d2[k] = 0.0f;
-      tb[k] = 0.0f;
+      T[k] = 1.0f; Q[k] = 0.0f; d0[k] = d1[k] = d2[k] = 0.0f;
       if (act[k]) {
         const float4 ck = ((const float4*)a.acc)[bq + ly * TILE_W + lx];
         const float4 pc = ((const float4*)a.pixc)[g];
         T[k] = ck.w;
-        r0[k] = pc.x - ck.x;
-        r1[k] = pc.y - ck.y;
-        r2[k] = pc.z - ck.z;
         d0[k] = dimg[gl];
         d1[k] = dimg[HW + gl];
         d2[k] = dimg[2 * HW + gl];
-        tb[k] = a.tfinal[g] * (vr.bg[0] * d0[k] + vr.bg[1] * d1[k] + vr.bg[2] * d2[k]);
+        const float tb = a.tfinal[g] * (vr.bg[0] * d0[k] + vr.bg[1] * d1[k] + vr.bg[2] * d2[k]);
+        Q[k] = residual_dot(pc.x - ck.x, pc.y - ck.y, pc.z - ck.z, d0[k], d1[k], d2[k], tb);
       }
     }
     uint32_t id = a.vals[s + n - 1];
@@ -219,7 +216,7 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
       for (int q = 0; q < 4; ++q) {
         if (act[q])
           pix_bwd<float, FastRcp>(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q], fy[q],
-                                  d0[q], d1[q], d2[q], tb[q], T[q], r0[q], r1[q], r2[q], acc);
+                                  d0[q], d1[q], d2[q], T[q], Q[q], acc);
       }
       wave_flush<NF>(acc, a.raw, id);
       id = idn;

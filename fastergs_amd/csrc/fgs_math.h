@@ -572,8 +572,11 @@ enum RawField {
 constexpr int RAW_STRIDE = 12;
 
 // One reverse step of the backward for one pixel (backward.cu:549-632):
-// recovers T before the splat, updates the residual colour and accumulates
-// the raw sums into acc[RAW_N].
+// recovers T before the splat and accumulates the raw sums into acc[RAW_N].
+// The pixel's residual colour (what later splats and the background add)
+// enters dL/dalpha only through its dot product with dL/dpixel, so the state
+// is the scalar Q = residual . dL/dpixel + T_final (bg . dL/dpixel):
+//   dL/dalpha = T (colour . dL/dpixel) - Q / (1 - alpha),  Q += w (colour . dL/dpixel).
 // Reciprocal of the per-pixel backward: IEEE division by default; the GPU
 // backward passes a one-instruction approximate reciprocal (the result only
 // enters gradient sums, which are compared within a tolerance).
@@ -582,26 +585,28 @@ struct IeeeRcp {
   FGS_HD R operator()(R x) const { return R(1) / x; }
 };
 
+// Initial Q of a reverse walk that starts from a checkpoint with residual
+// colour (r0, r1, r2); tb = T_final (bg . dL/dpixel).
+template <typename R>
+FGS_HD R residual_dot(R r0, R r1, R r2, R dp0, R dp1, R dp2, R tb) {
+  return fma_(r2, dp2, fma_(r1, dp1, r0 * dp0)) + tb;
+}
+
 template <typename R, typename Rcp = IeeeRcp>
 FGS_HD void pix_bwd(R gx, R gy, R ca, R cb, R cc, R w, R r, R g, R b, R px, R py,
-                    R dp0, R dp1, R dp2, R tb, R& T, R& res0, R& res1,
-                    R& res2, R* acc) {
+                    R dp0, R dp1, R dp2, R& T, R& Q, R* acc) {
   const R dx = gx - px, dy = gy - py;
   const R p = fma_(cb * dx, dy, fma_(cc * dy, dy, fma_(ca * dx, dx, w)));
   const R a = alpha_of(p);
   const R inv = Rcp()(R(1) - a);
   T = min_(T * inv, R(1));
   const R wgt = a * T;
-  R dLda = (T * r - res0 * inv) * dp0;
-  dLda = fma_(T * g - res1 * inv, dp1, dLda);
-  dLda = fma_(T * b - res2 * inv, dp2, dLda);
-  res0 = fma_(wgt, r, res0);
-  res1 = fma_(wgt, g, res1);
-  res2 = fma_(wgt, b, res2);
+  const R cd = fma_(b, dp2, fma_(g, dp1, r * dp0));
+  const R dLda = fma_(-inv, Q, T * cd);
+  Q = fma_(wgt, cd, Q);
   acc[RAW_COL0] = fma_(wgt, dp0, acc[RAW_COL0]);
   acc[RAW_COL1] = fma_(wgt, dp1, acc[RAW_COL1]);
   acc[RAW_COL2] = fma_(wgt, dp2, acc[RAW_COL2]);
-  dLda = fma_(-tb, inv, dLda);  // tb = T_final * (bg . dL/dpixel), per pixel
   const R dLdp = dLda * (a * R(LN2_F));
   // d power / d(gx, gy) = (2 ca dx + cb dy, 2 cc dy + cb dx) and d power /
   // d(ca, cb, cc) = (dx^2, dx dy, dy^2), through u = dLdp dx, v = dLdp dy

@@ -336,12 +336,12 @@ void backward(State<R>& st, const R* dimg, int accum = 0) {
         const R Tf = st.tfinal[g];
         const R tb = Tf * (bg0 * dp0 + bg1 * dp1 + bg2 * dp2);
         const uint32_t cnt = (uint32_t)st.contrib[g];
-        auto step = [&](uint32_t k, R& T, R& r0, R& r1, R& r2) {
+        auto step = [&](uint32_t k, R& T, R& Q) {
           const R* sp = &st.splat[9 * (size_t)st.vals[k]];
           R acc[fgs::RAW_N];
           for (int f = 0; f < fgs::RAW_N; ++f) acc[f] = R(0);
           fgs::pix_bwd(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], sp[6], sp[7], sp[8], fx, fy, dp0,
-                       dp1, dp2, tb, T, r0, r1, r2, acc);
+                       dp1, dp2, T, Q, acc);
           double* row = &rows[k * fgs::RAW_N];
           if (accum == 0) {
             for (int f = 0; f < fgs::RAW_N; ++f) row[f] += (double)acc[f];
@@ -355,14 +355,15 @@ void backward(State<R>& st, const R* dimg, int accum = 0) {
             if (s - tile_start >= cnt) break;  // backward.cu:663-664
             const size_t q = (size_t)b * fgs::TILE_PIX + lp;
             R T = st.acc_t[q];
-            R r0 = st.pixc[3 * g] - st.acc_c[3 * q];
-            R r1 = st.pixc[3 * g + 1] - st.acc_c[3 * q + 1];
-            R r2 = st.pixc[3 * g + 2] - st.acc_c[3 * q + 2];
-            for (uint32_t k = e; k-- > s;) step(k, T, r0, r1, r2);
+            const R r0 = st.pixc[3 * g] - st.acc_c[3 * q];
+            const R r1 = st.pixc[3 * g + 1] - st.acc_c[3 * q + 1];
+            const R r2 = st.pixc[3 * g + 2] - st.acc_c[3 * q + 2];
+            R Q = fgs::residual_dot(r0, r1, r2, dp0, dp1, dp2, tb);
+            for (uint32_t k = e; k-- > s;) step(k, T, Q);
           }
         } else {
-          R T = Tf, r0 = R(0), r1 = R(0), r2 = R(0);
-          for (uint32_t k = tile_start + cnt; k-- > tile_start;) step(k, T, r0, r1, r2);
+          R T = Tf, Q = tb;
+          for (uint32_t k = tile_start + cnt; k-- > tile_start;) step(k, T, Q);
         }
       }
   }
