@@ -45,6 +45,10 @@ def parse():
                          "FETCH_SIZE / WRITE_SIZE runs); prints no JSON")
     ap.add_argument("--sched-eval", type=int, default=1,
                     help="also train the tile cost model online and score it against the heuristic")
+    ap.add_argument("--ply", default=None,
+                    help="also replay a trained 3DGS model file (point_cloud.ply) ...")
+    ap.add_argument("--cameras", default=None,
+                    help="... over the views of its cameras.json (forward FPS and fwd+bwd ms)")
     return ap.parse_args()
 
 
@@ -139,6 +143,8 @@ def main():
     mv = multiview_step(args) if args.multiview else None
     torch.cuda.empty_cache()
     sched_eval = learned_schedule_eval(args) if args.sched_eval else None
+    torch.cuda.empty_cache()
+    replay = scene_replay(args) if args.ply else None
 
     cpu = None
     if args.cpu_baseline:
@@ -174,6 +180,7 @@ def main():
         "render_loop": loop,
         "multiview": mv,
         "learned_scheduler": sched_eval,
+        "scene_replay": replay,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -483,6 +490,63 @@ def render_loop(args, frames=120):
             "pairs_sampled": Rs, "pair_capacity": R_cap, "bucket_capacity": NB_cap,
             "sort_gbs": round(sort_bytes / (sort_ms / 1e3) / 1e9, 1),
             "sort_frac_hbm": round(sort_bytes / (sort_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def scene_replay(args, max_views=200):
+    """Replay of a trained model file over its cameras: per view one eager
+    forward (timed) and one forward + L2 loss + backward (timed), HIP events
+    on the current stream; means over the views."""
+    import torch
+    from fastergs_amd import plyio
+    from fastergs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+    dev = torch.device("cuda:0")
+    scene = plyio.load_ply(args.ply)
+    cams = (plyio.load_cameras_json(args.cameras) if args.cameras
+            else scenes_orbit(scene))[:max_views]
+    deg = scene.pop("sh_degree")
+    p = {k: torch.from_numpy(v).to(dev).requires_grad_(True) for k, v in scene.items()}
+    fwd, step = [], []
+    for c in cams:
+        st = GaussianRasterizationSettings(
+            image_height=c.height, image_width=c.width, tanfovx=c.tanfovx, tanfovy=c.tanfovy,
+            bg=torch.zeros(3, device=dev), scale_modifier=1.0,
+            viewmatrix=torch.from_numpy(c.viewmatrix).to(dev),
+            projmatrix=torch.from_numpy(c.projmatrix).to(dev), sh_degree=deg,
+            campos=torch.from_numpy(c.campos).to(dev), mult=1.0, prefiltered=False, debug=False)
+        rast = GaussianRasterizer(st)
+        kw = dict(means3D=p["means3D"], opacities=p["opacities"], dc=p["dc"], shs=p["shs"],
+                  scales=p["scales"], rotations=p["rotations"])
+        for timed_list, train in ((fwd, False), (step, True), (fwd, False), (step, True)):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            if train:
+                img, _ = rast(**kw)
+                (img ** 2).mean().backward()
+            else:
+                with torch.no_grad():
+                    rast(**kw)
+            e1.record()
+            torch.cuda.synchronize()
+            timed_list.append(e0.elapsed_time(e1))
+    fwd_ms = float(np.mean(fwd[1::2]))  # second of each pair (first one warms up the view)
+    step_ms = float(np.mean(step[1::2]))
+    return {"ply": os.path.basename(args.ply), "gaussians": int(p["means3D"].shape[0]),
+            "sh_degree": deg, "views": len(cams), "fwd_ms": round(fwd_ms, 4),
+            "fps": round(1000.0 / fwd_ms, 2), "fwd_bwd_ms": round(step_ms, 4)}
+
+
+def scenes_orbit(scene, frames=60):
+    """Orbit cameras around a scene's centre when no cameras.json is given."""
+    from fastergs_amd import scenes
+    xyz = scene["means3D"]
+    centre = np.median(xyz, axis=0)
+    radius = 2.0 * float(np.percentile(np.linalg.norm(xyz - centre, axis=1), 90))
+    cams = []
+    for k in range(frames):
+        a = 2.0 * np.pi * k / frames
+        pos = centre + radius * np.array([np.sin(a), 0.2, -np.cos(a)])
+        cams.append(scenes.look_at_camera(pos, 1280, 840, target=tuple(centre)))
+    return cams
 
 
 def multiview_step(args, views=16, steps=5):
