@@ -36,6 +36,66 @@ int hip_check(hipError_t e, const char* where) {
     if (rc__ != FGS_OK) return rc__;            \
   } while (0)
 
+// Side streams for stages that do not depend on each other. Per host thread
+// and device: two non-blocking streams of default priority and a few events,
+// created on first use outside stream capture (a call made during capture
+// before that runs every stage on the caller's stream). fork(i, e) makes
+// side stream i wait for the work queued so far on the main stream; join(i,
+// e) makes the main stream wait for side stream i. Under capture the side
+// streams join the same graph through these events; every fork is joined
+// before the entry point returns.
+constexpr int SIDE_STREAMS = 2, SIDE_EVENTS = 8, SIDE_MAX_DEVICES = 64;
+struct SideCtx {
+  hipStream_t st[SIDE_STREAMS] = {};
+  hipEvent_t ev[SIDE_EVENTS] = {};
+  bool ready = false, failed = false;
+};
+thread_local SideCtx t_side[SIDE_MAX_DEVICES];
+
+SideCtx* side_ctx(hipStream_t s) {
+  int dev = -1;
+  if (hipStreamGetDevice(s, &dev) != hipSuccess || dev < 0 || dev >= SIDE_MAX_DEVICES)
+    return nullptr;
+  SideCtx& c = t_side[dev];
+  if (c.ready) return &c;
+  if (c.failed) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return nullptr;
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+  bool ok = cur == dev || hipSetDevice(dev) == hipSuccess;
+  for (int i = 0; ok && i < SIDE_STREAMS; ++i)
+    ok = hipStreamCreateWithFlags(&c.st[i], hipStreamNonBlocking) == hipSuccess;
+  for (int i = 0; ok && i < SIDE_EVENTS; ++i)
+    ok = hipEventCreateWithFlags(&c.ev[i], hipEventDisableTiming) == hipSuccess;
+  if (cur != dev) (void)hipSetDevice(cur);
+  (void)hipGetLastError();
+  if (!ok) {
+    c.failed = true;
+    return nullptr;
+  }
+  c.ready = true;
+  return &c;
+}
+
+struct Fork {
+  SideCtx* c;
+  hipStream_t main;
+  explicit Fork(hipStream_t s) : c(side_ctx(s)), main(s) {}
+  hipStream_t side(int i) const { return c ? c->st[i] : main; }
+  hipError_t fork(int i, int e) const {
+    if (!c) return hipSuccess;
+    hipError_t r = hipEventRecord(c->ev[e], main);
+    return r != hipSuccess ? r : hipStreamWaitEvent(c->st[i], c->ev[e], 0);
+  }
+  hipError_t join(int i, int e) const {
+    if (!c) return hipSuccess;
+    hipError_t r = hipEventRecord(c->ev[e], c->st[i]);
+    return r != hipSuccess ? r : hipStreamWaitEvent(main, c->ev[e], 0);
+  }
+};
+
 int grid_x_of(int W) { return (W + TILE_W - 1) / TILE_W; }
 int grid_y_of(int H) { return (H + TILE_H - 1) / TILE_H; }
 int tiles_of(int W, int H) { return grid_x_of(W) * grid_y_of(H); }
@@ -346,7 +406,6 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   if (!gauss_ws || gauss_ws_bytes < w.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
   if (g->P > 0 && !radii) return fail(FGS_ERR_ARG, "radii is NULL");
   hipStream_t s = (hipStream_t)stream;
-  FGS_TRY(fill_u32(w.counts, 0u, 16, s), "fill counts");
   const fgs_camera* cam = &b.cams[0];
   PreArgs a;
   a.G = make_gauss(g);
@@ -365,6 +424,7 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   a.views = (const ViewRec*)w.views;
   a.V = b.V;
   if (opt->schedule == FGS_SCHEDULE_PLAIN) {
+    FGS_TRY(fill_u32(w.counts, 0u, GAUSS_SYNC_WORDS, s), "fill counts");
     if (g->P == 0) return FGS_OK;
     FGS_TRY(launch_preprocess_plain(a, s), "preprocess");
     FGS_TRY(scan_u32(w.pcount, w.poff, g->P, w.counts + 0, w.scratch, s), "pair scan");
@@ -372,40 +432,52 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   }
   // FAST: the view table, then per-Gaussian math + lane-packed exact counts
   // of small rects for every view
-  FGS_TRY(launch_views_setup(b.setup, (ViewRec*)w.views, s), "views");
+  FGS_TRY(launch_views_setup(b.setup, (ViewRec*)w.views, w.counts, GAUSS_SYNC_WORDS, s),
+          "views");
   if (g->P == 0) return FGS_OK;
   FGS_TRY(launch_preprocess_fast(a, s), "preprocess");
-  // large rects, compacted (count in counts[3]) and walked by whole workgroups
-  FGS_TRY(compact_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.scratch, s), "large compact");
-  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
-                             w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
-          "large tiles");
+  // Three independent chains after the geometry pass: the SH colours (read
+  // only by compositing) and the depth sort of the records on side streams,
+  // the large-rect compaction and tile counts on the main stream.
+  const Fork f(s);
+  FGS_TRY(f.fork(0, 0), "fork");
+  FGS_TRY(launch_sh_color(a, f.side(0)), "colour");
+  FGS_TRY(f.fork(1, 1), "fork");
   // records in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by (global) tile alone yields the canonical per-tile
   // order of every view
   bool alt = false;
   FGS_TRY(sort_kv<uint32_t>(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, PV, 0, 32, w.scratch,
-                            s, alt, nullptr, nullptr, SORT_DEPTH),
+                            f.side(1), alt, nullptr, nullptr, SORT_DEPTH, w.dsort_sync),
           "depth sort");
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
+  // large rects, compacted (count in counts[3]) and walked by whole waves
+  FGS_TRY(compact_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large compact");
+  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
+                             w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
+          "large tiles");
+  FGS_TRY(f.join(1, 2), "join");
   FGS_TRY(scan_pair_offsets(w.pcount, w.dsort_v0, PV, w.rcount, w.poff, w.counts,
                             opt->bucket_size, b.T, w.scratch, s),
           "pair offsets");
+  FGS_TRY(f.join(0, 3), "join");
   return FGS_OK;
 }
 
 int fgs_batch_read_counts(const void* gauss_ws, int P, const fgs_batch* batch,
                           const fgs_options* opt, int64_t counts_host[2], void* stream) {
-  (void)opt;
   Batch b;
   int rc;
   if ((rc = make_batch(batch, b))) return rc;
   if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
   const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), (int64_t)P * b.V, b.T);
-  uint32_t c[2] = {0, 0};  // [0] pairs, [1] bucket-table capacity
+  // [0] pairs, [1] bucket-table capacity, up to the depth sort's fault flag
+  uint32_t c[DSORT_FAULT + 1];
   hipStream_t s = (hipStream_t)stream;
   FGS_TRY(hipMemcpyAsync(c, w.counts, sizeof(c), hipMemcpyDeviceToHost, s), "read counts");
   FGS_TRY(hipStreamSynchronize(s), "read counts sync");
+  if (opt && opt->schedule == FGS_SCHEDULE_FAST && c[DSORT_FAULT])
+    return fail(FGS_ERR_HIP, "a look-back wait of the preprocess exceeded its bound");
   counts_host[0] = c[0];
   counts_host[1] = c[1];
   return FGS_OK;
@@ -461,12 +533,16 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
     // emission in depth order, tile-only stable sort, lists, ranges, buckets.
     // R is the pair capacity; the pair count itself (counts[0]) stays on
     // the device, so nothing here waits for the host.
+    // small and large rects write disjoint pair blocks: side by side
+    const Fork f(s);
+    FGS_TRY(f.fork(0, 4), "fork");
     FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.poff, (uint32_t)R,
-                              pw.t0, pw.r0, pw.tile16, s),
+                              pw.t0, pw.r0, pw.tile16, f.side(0)),
             "emit");
     FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts, nullptr,
                                gw.poff, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s),
             "emit large");
+    FGS_TRY(f.join(0, 5), "join");
     // stable sort of (tile id, record) by tile id: the last pass writes the
     // per-tile lists; ranges by binary search over the sorted tile ids
     {
@@ -677,13 +753,15 @@ int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_opti
 
 int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fgs_options* opt,
                     int64_t counts_host[2], void* stream) {
-  (void)opt;
   if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
   const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height));
-  uint32_t c[2] = {0, 0};  // [0] pairs, [1] bucket-table capacity
+  // [0] pairs, [1] bucket-table capacity, up to the depth sort's fault flag
+  uint32_t c[DSORT_FAULT + 1];
   hipStream_t s = (hipStream_t)stream;
   FGS_TRY(hipMemcpyAsync(c, w.counts, sizeof(c), hipMemcpyDeviceToHost, s), "read counts");
   FGS_TRY(hipStreamSynchronize(s), "read counts sync");
+  if (opt && opt->schedule == FGS_SCHEDULE_FAST && c[DSORT_FAULT])
+    return fail(FGS_ERR_HIP, "a look-back wait of the preprocess exceeded its bound");
   counts_host[0] = c[0];
   counts_host[1] = c[1];
   return FGS_OK;

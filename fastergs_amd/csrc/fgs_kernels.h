@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fgs_layout.h"
 #include "fgs_math.h"
 
 namespace fgs {
@@ -38,8 +39,10 @@ struct ViewSetup {
   int n;
 };
 // Copies the camera data into the device table (kernel arguments only, so
-// a captured graph re-reads the camera buffers on every replay).
-hipError_t launch_views_setup(const ViewSetup& src, ViewRec* out, hipStream_t s);
+// a captured graph re-reads the camera buffers on every replay) and clears
+// the frame's counters (zero_n words at zero).
+hipError_t launch_views_setup(const ViewSetup& src, ViewRec* out, uint32_t* zero, int64_t zero_n,
+                              hipStream_t s);
 
 // Camera of view r for the shared per-Gaussian math.
 __device__ __forceinline__ CamRef<float> view_cam(const CamRef<float>& base, const ViewRec& r) {
@@ -82,6 +85,8 @@ hipError_t compact_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_
                          hipStream_t s);
 // Stable LSD radix sort of (key, value) pairs on key bits [begin_bit,
 // end_bit) (keys of 16, 32 or 64 bits; vals may be NULL for key-only sorts).
+// sync: when given, SORT_SYNC_WORDS words that are zero on entry, used for
+// the onesweep state instead of clearing it in a launch of its own.
 // Passes alternate between the arrays and their *_alt twins; in_alt tells
 // where the keys ended up. last_vals: when given, the final pass writes the
 // values there. n_dev: when given, the element count is min(*n_dev, n) read
@@ -91,7 +96,8 @@ enum SortKind { SORT_PAIRS = 0, SORT_DEPTH = 1, SORT_TILES = 2 };
 template <typename K>
 hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
                    int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
-                   const uint32_t* n_dev, uint32_t* last_vals, SortKind kind);
+                   const uint32_t* n_dev, uint32_t* last_vals, SortKind kind,
+                   uint32_t* sync = nullptr);
 // fill.hip-style helpers (render.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 hipError_t fill_f32(float* p, float v, int64_t n, hipStream_t s);
@@ -116,6 +122,9 @@ struct PreArgs {
 };
 hipError_t launch_preprocess_plain(const PreArgs& a, hipStream_t s);
 hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
+// SH colour of every valid record (after launch_preprocess_fast; only the
+// compositing pass reads its output). No-op with precomputed colours.
+hipError_t launch_sh_color(const PreArgs& a, hipStream_t s);
 // Pair emission writes only slots below cap (the pair capacity); a frame
 // whose pair count exceeds it is detected on the host from the count.
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,

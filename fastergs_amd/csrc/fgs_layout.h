@@ -55,6 +55,9 @@ struct Carver {
 // up to SORT_MAX_PASSES passes, tile counters + fault flag, and two status
 // arrays of 256 words per SORT_TILE keys (alternating between passes).
 constexpr int SORT_MAX_PASSES = 8;
+// Zero-initialised state of one onesweep sort (digit histograms, tile
+// counters, fault flag) that a caller may clear ahead of the sort.
+constexpr int SORT_SYNC_WORDS = SORT_MAX_PASSES * 256 + SORT_MAX_PASSES + 1;
 constexpr int64_t ONESWEEP_MAX_KEYS = int64_t(1) << 22;
 struct SortWs {
   uint32_t* hist;
@@ -92,6 +95,11 @@ inline SortWs carve_sort(void* ws, int64_t nb) {
   return carve_sort(c, nb);
 }
 
+constexpr int GAUSS_SYNC_WORDS = 16 + SORT_SYNC_WORDS;
+// The depth sort's fault flag (last word of its state, counted from
+// counts), set when a look-back wait exceeds its bound.
+constexpr int DSORT_FAULT = 16 + SORT_SYNC_WORDS - 1;
+
 struct GaussWs {
   float* geom;
   float* splat;
@@ -104,18 +112,20 @@ struct GaussWs {
   uint32_t* dsort_v1;
   uint32_t* lflag;     // fast: rect walked by whole workgroups (1) or lane-packed (0)
   uint32_t* large_ids; // fast: flagged Gaussians in index order
-  uint32_t* rcount;    // fast: pair counts in depth order
   uint32_t* ranges;    // T x 2
   uint32_t* boff;      // T + 1: exclusive prefix of per-tile bucket counts, [T] = NB
   uint32_t* bcount;    // T + 1 scratch (bucket counts, then active bucket counts)
   uint32_t* actoff;    // T + 1: exclusive prefix of per-tile active buckets, [T] = total
   uint32_t* maxc;      // T
+  uint32_t* rcount;    // fast: pair counts in depth order
   uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects, [4] work items
+  uint32_t* dsort_sync;  // fast: depth sort state (SORT_SYNC_WORDS), right after counts
   float* feat;         // T x 8 tile features (scheduled compositing)
   float* pred;         // T x 2 cost-model outputs (predicted log2 cost, split logit)
   uint32_t* items;     // 2T compositing work items (tile << 2 | part)
   uint32_t* cycles;    // 2T per-item timing log (100 MHz ticks)
   void* views;         // view table (FAST)
+  void* cscratch;      // fast: large-rect compaction (runs beside the depth sort)
   void* scratch;
   size_t scratch_bytes;
   size_t total;
@@ -139,13 +149,16 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   w.dsort_v1 = c.take<uint32_t>(p);
   w.lflag = c.take<uint32_t>(p);
   w.large_ids = c.take<uint32_t>(p);
-  w.rcount = c.take<uint32_t>(p);
   w.ranges = c.take<uint32_t>(2 * t);
   w.boff = c.take<uint32_t>(t + 1);
   w.bcount = c.take<uint32_t>(t + 1);
   w.actoff = c.take<uint32_t>(t + 1);
   w.maxc = c.take<uint32_t>(t);
-  w.counts = c.take<uint32_t>(16);
+  w.rcount = c.take<uint32_t>(p);
+  // counts and the depth sort's state form one region, cleared together at
+  // the start of a frame (GAUSS_SYNC_WORDS)
+  w.counts = c.take<uint32_t>(GAUSS_SYNC_WORDS);
+  w.dsort_sync = w.counts + 16;
   w.feat = c.take<float>(8 * t);
   w.pred = c.take<float>(2 * t);
   w.items = c.take<uint32_t>(2 * t);
@@ -157,6 +170,7 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   if (s3 > sb) sb = s3;
   w.scratch_bytes = sb;
   w.scratch = c.take<char>(sb);
+  w.cscratch = c.take<char>(scan_ws_bytes((int64_t)p));
   w.total = c.off;
   return w;
 }

@@ -321,11 +321,15 @@ hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s) {
   const unsigned blocks = (a.P + PRE_THREADS - 1) / PRE_THREADS;
   FGS_TIMED("preprocess_fast", s,
             (k_preprocess_fast<<<dim3(blocks, (unsigned)a.V), PRE_THREADS, 0, s>>>(a)));
-  if (a.G.colors == nullptr && a.G.dc != nullptr) {
-    const size_t rows = (size_t)(4 * SHC_DC_F4 + PRE_THREADS * 3 * a.cam.max_coeffs) * sizeof(float);
-    const size_t shmem = rows > SHC_SHMEM ? rows : SHC_SHMEM;
-    FGS_TIMED("sh_color", s, (k_sh_color<<<blocks, PRE_THREADS, shmem, s>>>(a)));
-  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sh_color(const PreArgs& a, hipStream_t s) {
+  if (a.P <= 0 || a.G.colors != nullptr || a.G.dc == nullptr) return hipSuccess;
+  const unsigned blocks = (a.P + PRE_THREADS - 1) / PRE_THREADS;
+  const size_t rows = (size_t)(4 * SHC_DC_F4 + PRE_THREADS * 3 * a.cam.max_coeffs) * sizeof(float);
+  const size_t shmem = rows > SHC_SHMEM ? rows : SHC_SHMEM;
+  FGS_TIMED("sh_color", s, (k_sh_color<<<blocks, PRE_THREADS, shmem, s>>>(a)));
   return hipGetLastError();
 }
 
@@ -451,9 +455,11 @@ hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* ge
 }
 
 // View table from kernel arguments (see ViewSetup).
-__global__ void k_views_setup(ViewSetup src, ViewRec* out) {
+__global__ void k_views_setup(ViewSetup src, ViewRec* out, uint32_t* zero, int64_t zero_n) {
   const int v = blockIdx.x;
   const int t = threadIdx.x;
+  for (int64_t i = (int64_t)v * blockDim.x + t; i < zero_n; i += (int64_t)gridDim.x * blockDim.x)
+    zero[i] = 0u;
   if (v >= src.n) return;
   const ViewSrc& s = src.v[v];
   ViewRec& r = out[v];
@@ -480,8 +486,13 @@ __global__ void k_views_setup(ViewSetup src, ViewRec* out) {
   }
 }
 
-hipError_t launch_views_setup(const ViewSetup& src, ViewRec* out, hipStream_t s) {
-  if (src.n > 0) k_views_setup<<<src.n, 64, 0, s>>>(src, out);
+hipError_t launch_views_setup(const ViewSetup& src, ViewRec* out, uint32_t* zero, int64_t zero_n,
+                              hipStream_t s) {
+  // enough workgroups that each clears at most ~8 words per thread
+  int64_t blocks = (zero_n + 2047) / 2048;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < src.n) blocks = src.n;
+  if (blocks > 0) k_views_setup<<<(unsigned)blocks, 256, 0, s>>>(src, out, zero, zero_n);
   return hipGetLastError();
 }
 

@@ -566,6 +566,8 @@ static hipError_t sort_kv_classic(K* keys, uint32_t* vals, K* keys_alt, uint32_t
 // prefix); bits 29..0 the count (n < 2^30).
 constexpr uint32_t ST_AGG = 1u << 30, ST_PRE = 2u << 30, ST_CNT = (1u << 30) - 1u;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
+constexpr int64_t HIST_ALL_BLOCKS = 1024;
+constexpr int LOOKBACK = 8;  // status words read per look-back step (in flight together)
 
 __global__ __launch_bounds__(256) void k_sort_prepare(uint32_t* __restrict__ hist,
                                                       uint32_t* __restrict__ counters) {
@@ -593,15 +595,21 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const K* __restrict_
   const int64_t n = dev_count(n_dev, n_cap);
   const int64_t e0 = tile0 * SortTiling<K>::TILE, e1 = min(n, e0 + (int64_t)per_block * SortTiling<K>::TILE);
   constexpr int N = KeyVec<K>::N;
-  for (int64_t e = e0 + (int64_t)t * N; e < e1; e += (int64_t)SORT_THREADS * N) {
-    KeyVec<K> kv;
-    const int m = load_keys(keys, e, e1, kv);
+  constexpr int U = 4;  // vector loads in flight per thread
+  constexpr int64_t STEP = (int64_t)SORT_THREADS * N;
+  for (int64_t e = e0 + (int64_t)t * N; e < e1; e += U * STEP) {
+    KeyVec<K> kv[U];
+    int m[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) m[u] = load_keys(keys, e + u * STEP, e1, kv[u]);
     for (int p = 0; p < npass; ++p) {
       int sh, bits;
       pass_digits(begin_bit, end_bit, p, sh, bits);
 #pragma unroll
-      for (int j = 0; j < N; ++j)
-        if (j < m) atomicAdd(&cnt[p][(uint32_t)(kv.k[j] >> sh) & ((1u << bits) - 1u)], 1u);
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+          if (j < m[u]) atomicAdd(&cnt[p][(uint32_t)(kv[u].k[j] >> sh) & ((1u << bits) - 1u)], 1u);
     }
   }
   __syncthreads();
@@ -659,22 +667,38 @@ __global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
     status_store(my, ST_PRE | c);
   } else {
     status_store(my, ST_AGG | c);
+    // Walk back over the earlier tiles' words of digit t, LOOKBACK at a time
+    // (independent loads, one latency per step): add counts up to the first
+    // inclusive prefix; a word not yet published ends the step and is read
+    // again in the next one.
     int64_t p = tile - 1;
     uint32_t spins = 0;
-    while (p >= 0) {
-      const uint32_t st = status_load(status + p * 256 + t);
-      const uint32_t flag = st & ~ST_CNT;
-      if (flag == 0u) {
+    bool done = false;
+    while (!done) {
+      uint32_t st[LOOKBACK];
+#pragma unroll
+      for (int j = 0; j < LOOKBACK; ++j)
+        st[j] = p - j >= 0 ? status_load(status + (p - j) * 256 + t) : 0u;
+      int used = 0;
+      bool stop = false;
+#pragma unroll
+      for (int j = 0; j < LOOKBACK; ++j) {
+        const uint32_t flag = st[j] & ~ST_CNT;
+        if (!stop && flag != 0u) {
+          excl += st[j] & ST_CNT;
+          if (flag == ST_PRE) done = true;
+          ++used;
+        }
+        stop = stop || flag == 0u || flag == ST_PRE;
+      }
+      p -= used;
+      if (!done && used == 0) {
         if (++spins > SPIN_LIMIT) {  // cannot happen with in-order tiles; never hang
           atomicOr(fault, 1u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      excl += st & ST_CNT;
-      if (flag == ST_PRE) break;
-      --p;
     }
     status_store(my, ST_PRE | (excl + c));
   }
@@ -687,13 +711,20 @@ template <typename K>
 static hipError_t sort_kv_onesweep(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
                                    int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
                                    bool& in_alt, const uint32_t* n_dev, uint32_t* last_vals,
-                                   const SortNames& nm) {
+                                   const SortNames& nm, uint32_t* sync) {
   const int64_t nb = (n + SortTiling<K>::TILE - 1) / SortTiling<K>::TILE;
-  const SortWs w = carve_sort(ws, nb);
+  SortWs w = carve_sort(ws, nb);
   const int npass = sort_passes(begin_bit, end_bit);
-  k_sort_prepare<<<1, 256, 0, s>>>(w.hist, w.counters);
-  // at least ~1024 workgroups for the histogram read when the input allows
-  const int per_block = (int)(nb / 1024 > 1 ? nb / 1024 : 1);
+  if (sync) {  // cleared by the caller
+    w.hist = sync;
+    w.counters = sync + SORT_MAX_PASSES * 256;
+  } else {
+    k_sort_prepare<<<1, 256, 0, s>>>(w.hist, w.counters);
+  }
+  // one sort tile per workgroup up to HIST_ALL_BLOCKS workgroups (measured
+  // at 1M keys: fewer, longer workgroups are slower although they issue
+  // fewer same-address atomics)
+  const int per_block = (int)((nb + HIST_ALL_BLOCKS - 1) / HIST_ALL_BLOCKS);
   const unsigned hb = (unsigned)((nb + per_block - 1) / per_block);
   FGS_TIMED(nm.hist_all, s,
             (k_sort_hist<K><<<hb, SORT_THREADS, 0, s>>>(keys, n, n_dev, begin_bit, end_bit,
@@ -729,7 +760,8 @@ static hipError_t sort_kv_onesweep(K* keys, uint32_t* vals, K* keys_alt, uint32_
 template <typename K>
 hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
                    int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
-                   const uint32_t* n_dev, uint32_t* last_vals, SortKind kind) {
+                   const uint32_t* n_dev, uint32_t* last_vals, SortKind kind,
+                   uint32_t* sync) {
   in_alt = false;
   // with a device-side count the passes always run (the count may be 0 or 1)
   if ((n <= 1 && !n_dev) || n <= 0 || end_bit <= begin_bit) return hipSuccess;
@@ -739,19 +771,19 @@ hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int
   const SortNames& nm = k_sort_names[kind];
   if (n <= ONESWEEP_MAX_KEYS)
     return sort_kv_onesweep<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s,
-                               in_alt, n_dev, last_vals, nm);
+                               in_alt, n_dev, last_vals, nm, sync);
   return sort_kv_classic<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, in_alt,
                             n_dev, last_vals, nm);
 }
 
 template hipError_t sort_kv<uint16_t>(uint16_t*, uint32_t*, uint16_t*, uint32_t*, int64_t, int,
                                       int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
-                                      SortKind);
+                                      SortKind, uint32_t*);
 template hipError_t sort_kv<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, int64_t, int,
                                       int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
-                                      SortKind);
+                                      SortKind, uint32_t*);
 template hipError_t sort_kv<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, int64_t, int,
                                       int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
-                                      SortKind);
+                                      SortKind, uint32_t*);
 
 }  // namespace fgs

@@ -195,11 +195,15 @@ class _RasterizeViews(torch.autograd.Function):
                       (means3D, dc, shs, colors_precomp, opacities, scales, rotations)]
         ctx.stats = stats
         ctx.mark_non_differentiable(frame.radii)
+        ctx.set_materialize_grads(False)  # no zero-filled gradient for the radii
         return frame.out_color, frame.radii
 
     @staticmethod
     def backward(ctx, grad_flat, _grad_radii):
         frame: BatchFrame = ctx.frame
+        if grad_flat is None:  # the images took no part in the loss
+            ctx.frame = None
+            return (None,) * 9
         g = frame.backward(grad_flat)
         accum, accum_abs, max_radii, denom = ctx.stats
         if accum is not None and accum.numel() != 0:

@@ -365,11 +365,16 @@ class _RasterizeGaussians(torch.autograd.Function):
                        cov3Ds_precomp)]
         ctx.stats = (xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom)
         ctx.mark_non_differentiable(frame.radii)
+        # no zero-filled gradient for the radii (and none for an unused image)
+        ctx.set_materialize_grads(False)
         return frame.out_color, frame.radii
 
     @staticmethod
     def backward(ctx, grad_out_color, _grad_radii):
         frame: Frame = ctx.frame
+        if grad_out_color is None:  # the image took no part in the loss
+            ctx.frame = None
+            return (None,) * 14
         g = frame.backward(grad_out_color)
         accum, accum_abs, max_radii, denom = ctx.stats
         if accum is not None and accum.numel() != 0:  # pybind.cpp:324
