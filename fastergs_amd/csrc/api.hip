@@ -625,7 +625,10 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
   hipStream_t s = (hipStream_t)stream;
   float* raw = (float*)grad_ws;
-  FGS_TRY(fill_f32(raw, 0.0f, PV * RAW_STRIDE, s), "zero raw");
+  // the raw sums are cleared beside the active-bucket scan
+  const Fork f(s);
+  FGS_TRY(f.fork(0, 6), "fork");
+  FGS_TRY(fill_f32(raw, 0.0f, PV * RAW_STRIDE, f.side(0)), "zero raw");
   BwdArgs bw;
   bw.W = cam->width;
   bw.H = cam->height;
@@ -654,9 +657,11 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
     bw.vals = pw.v0;
     FGS_TRY(launch_active_counts(T, gw.maxc, opt->bucket_size, gw.bcount, s), "active buckets");
     FGS_TRY(scan_u32(gw.bcount, gw.actoff, T, gw.actoff + T, gw.scratch, s), "active scan");
+    FGS_TRY(f.join(0, 7), "join");
     FGS_TRY(launch_backward_fast(bw, s), "raster backward");
   } else {
     bw.vals = plain_sorted_in_alt(T) ? pw.v1 : pw.v0;
+    FGS_TRY(f.join(0, 7), "join");
     FGS_TRY(launch_backward_plain(bw, s), "raster backward");
   }
   GaussBwdArgs q;

@@ -368,6 +368,26 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const K* __restrict
   hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Register form of a tile's keys: 16-bit keys sit in the low half of a
+// 32-bit register whose high half receives the key's rank (ranks are below
+// 4096), which saves a register per item; wider keys keep a separate rank
+// array.
+template <typename K>
+struct KeyReg {
+  using T = K;
+  static constexpr bool PACK = false;
+};
+template <>
+struct KeyReg<uint16_t> {
+  using T = uint32_t;
+  static constexpr bool PACK = true;
+};
+template <typename K>
+__device__ __forceinline__ uint32_t reg_digit(typename KeyReg<K>::T k, int shift, uint32_t mask) {
+  if constexpr (KeyReg<K>::PACK) return ((k & 0xffffu) >> shift) & mask;
+  else return (uint32_t)(k >> shift) & mask;
+}
+
 // Stable ranking of one 4096-key tile in LDS: wave w owns the keys
 // [1024 w, 1024 w + 1024) as 16 rows of 64; equal digits in a row are ranked
 // with ballots over the digit bits, and each wave keeps its own running
@@ -377,28 +397,35 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const K* __restrict
 // Returns (after a barrier) with wcnt[w][d] = keys of digit d in waves < w,
 // bstart[d] = the tile's keys of digits < d, and rk[r] = rank of key r
 // within its (wave, digit).
-template <typename K>
-__device__ __forceinline__ void rank_tile(const K (&k)[SortTiling<K>::ITEMS], int nvalid, int shift, int bits,
+template <typename K, int BITS>
+__device__ __forceinline__ void rank_tile(typename KeyReg<K>::T (&k)[SortTiling<K>::ITEMS], int nvalid, int shift,
                                           uint32_t (&wcnt)[SORT_THREADS / WAVE][256],
                                           uint32_t* bstart, uint32_t (&rk)[SortTiling<K>::ITEMS],
                                           uint32_t& count) {
   constexpr int NW = SORT_THREADS / WAVE;
   constexpr int WITEMS = SortTiling<K>::TILE / NW;
   const int t = threadIdx.x, w = t / WAVE, lane = lane_id();
-  const uint32_t mask = (1u << bits) - 1u;
+  constexpr uint32_t mask = (1u << BITS) - 1u;
 #pragma unroll
   for (int r = 0; r < SortTiling<K>::ITEMS; ++r) {
     const bool valid = w * WITEMS + r * WAVE + lane < nvalid;
-    const uint32_t d = (uint32_t)(k[r] >> shift) & mask;
-    uint64_t peers = __ballot(valid);
-    for (int b = 0; b < bits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
+    const uint32_t d = reg_digit<K>(k[r], shift, mask);
+    // lanes with the same digit: per bit, the ballot of the bit where this
+    // lane's bit is set, its complement where it is clear
+    const uint64_t vm = __ballot(valid);
+    uint32_t plo = (uint32_t)vm, phi = (uint32_t)(vm >> 32);
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+      const uint32_t sb = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);  // all ones when set
+      const uint64_t m = __ballot(sb != 0u);
+      plo &= ~((uint32_t)m ^ sb);
+      phi &= ~((uint32_t)(m >> 32) ^ sb);
     }
+    const uint64_t peers = ((uint64_t)phi << 32) | plo;
     const uint32_t below = lanes_below(peers);
-    const uint32_t before = valid ? wcnt[w][d] : 0u;
-    rk[r] = before + below;
+    const uint32_t before = wcnt[w][d];  // d is in range for every lane
+    if constexpr (KeyReg<K>::PACK) k[r] |= (before + below) << 16;
+    else rk[r] = before + below;
     if (valid && below == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
   }
   __syncthreads();
@@ -418,22 +445,24 @@ __device__ __forceinline__ void rank_tile(const K (&k)[SortTiling<K>::ITEMS], in
 // Loads this wave's 16 rows (keys and values) of the tile at base.
 template <typename K, bool VALS>
 __device__ __forceinline__ void load_tile(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                          int64_t base, int nvalid, K (&k)[SortTiling<K>::ITEMS],
+                                          int64_t base, int nvalid, typename KeyReg<K>::T (&k)[SortTiling<K>::ITEMS],
                                           uint32_t (&v)[SortTiling<K>::ITEMS]) {
   constexpr int WITEMS = SortTiling<K>::TILE / (SORT_THREADS / WAVE);
   const int w = threadIdx.x / WAVE, lane = lane_id();
+  // every load issued unconditionally (index clamped to the last valid
+  // key; such items are masked by nvalid afterwards)
 #pragma unroll
   for (int r = 0; r < SortTiling<K>::ITEMS; ++r) {
-    const int e = w * WITEMS + r * WAVE + lane;
-    k[r] = e < nvalid ? kin[base + e] : (K)0;
-    v[r] = (VALS && e < nvalid) ? vin[base + e] : 0u;
+    const int e = min(w * WITEMS + r * WAVE + lane, nvalid - 1);
+    k[r] = (typename KeyReg<K>::T)kin[base + e];
+    v[r] = VALS ? vin[base + e] : 0u;
   }
 }
 
 // Places the ranked keys (and values) in LDS in digit order, then writes
 // each digit's run contiguously from gofs[d].
 template <typename K, bool VALS>
-__device__ __forceinline__ void place_and_write(const K (&k)[SortTiling<K>::ITEMS], const uint32_t (&v)[SortTiling<K>::ITEMS],
+__device__ __forceinline__ void place_and_write(const typename KeyReg<K>::T (&k)[SortTiling<K>::ITEMS], const uint32_t (&v)[SortTiling<K>::ITEMS],
                                                 const uint32_t (&rk)[SortTiling<K>::ITEMS], int nvalid,
                                                 int shift, uint32_t mask,
                                                 uint32_t (&wcnt)[SORT_THREADS / WAVE][256],
@@ -445,9 +474,12 @@ __device__ __forceinline__ void place_and_write(const K (&k)[SortTiling<K>::ITEM
 #pragma unroll
   for (int r = 0; r < SortTiling<K>::ITEMS; ++r) {
     if (w * WITEMS + r * WAVE + lane < nvalid) {
-      const uint32_t d = (uint32_t)(k[r] >> shift) & mask;
-      const uint32_t pos = bstart[d] + wcnt[w][d] + rk[r];
-      sk[pos] = k[r];
+      const uint32_t d = reg_digit<K>(k[r], shift, mask);
+      uint32_t rank;
+      if constexpr (KeyReg<K>::PACK) rank = k[r] >> 16;
+      else rank = rk[r];
+      const uint32_t pos = bstart[d] + wcnt[w][d] + rank;
+      sk[pos] = (K)k[r];
       if (VALS) sv[pos] = v[r];
     }
   }
@@ -471,11 +503,11 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 }
 inline unsigned xcd_grid(int64_t nb) { return (unsigned)(8 * ((nb + 7) / 8)); }
 
-template <typename K, bool VALS>
+template <typename K, bool VALS, int BITS>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
-    int bits, const uint32_t* __restrict__ hist, int nb) {
+    const uint32_t* __restrict__ hist, int nb) {
   constexpr int NW = SORT_THREADS / WAVE;
   __shared__ K sk[SortTiling<K>::TILE];
   __shared__ uint32_t sv[VALS ? SortTiling<K>::TILE : 1];
@@ -491,12 +523,12 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
 #pragma unroll
   for (int q = 0; q < NW; ++q) wcnt[q][t] = 0u;
   gofs[t] = hist[(size_t)t * nb + tile];
-  K k[SortTiling<K>::ITEMS];
+  typename KeyReg<K>::T k[SortTiling<K>::ITEMS];
   uint32_t v[SortTiling<K>::ITEMS], rk[SortTiling<K>::ITEMS], cnt;
   load_tile<K, VALS>(kin, vin, base, nvalid, k, v);
   __syncthreads();
-  rank_tile<K>(k, nvalid, shift, bits, wcnt, bstart, rk, cnt);
-  place_and_write<K, VALS>(k, v, rk, nvalid, shift, (1u << bits) - 1u, wcnt, bstart, gofs, sk, sv,
+  rank_tile<K, BITS>(k, nvalid, shift, wcnt, bstart, rk, cnt);
+  place_and_write<K, VALS>(k, v, rk, nvalid, shift, (1u << BITS) - 1u, wcnt, bstart, gofs, sk, sv,
                            kout, vout);
 }
 
@@ -510,6 +542,21 @@ static const SortNames k_sort_names[3] = {
     {"depth_hist", "depth_scatter", "depth_hist_all", "depth_onesweep"},
     {"tile_hist", "tile_scatter", "tile_hist_all", "tile_onesweep"},
 };
+
+// Runs BODY with the constexpr int B_ = bits (1..8), the digit width the
+// ranking kernels are compiled for.
+#define FGS_WITH_BITS(bits, BODY)                   \
+  switch (bits) {                                   \
+    case 1: { constexpr int B_ = 1; BODY; } break;  \
+    case 2: { constexpr int B_ = 2; BODY; } break;  \
+    case 3: { constexpr int B_ = 3; BODY; } break;  \
+    case 4: { constexpr int B_ = 4; BODY; } break;  \
+    case 5: { constexpr int B_ = 5; BODY; } break;  \
+    case 6: { constexpr int B_ = 6; BODY; } break;  \
+    case 7: { constexpr int B_ = 7; BODY; } break;  \
+    case 8: { constexpr int B_ = 8; BODY; } break;  \
+    default: return hipErrorInvalidValue;           \
+  }
 
 template <typename K>
 static hipError_t sort_kv_classic(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
@@ -533,13 +580,13 @@ static hipError_t sort_kv_classic(K* keys, uint32_t* vals, K* keys_alt, uint32_t
     hipError_t e = scan_u32(hist, hist, (int64_t)256 * nb, nullptr, scan_ws, s);
     if (e != hipSuccess) return e;
     if (vals)
-      FGS_TIMED(nm.scatter, s,
-                (k_radix_scatter<K, true><<<xcd_grid(nb), SORT_THREADS, 0, s>>>(
-                    ka, va, kb, vout, n, n_dev, shift, bits, hist, nb)));
+      FGS_WITH_BITS(bits, FGS_TIMED(nm.scatter, s,
+                                    (k_radix_scatter<K, true, B_><<<xcd_grid(nb), SORT_THREADS, 0, s>>>(
+                                        ka, va, kb, vout, n, n_dev, shift, hist, nb))))
     else
-      FGS_TIMED(nm.scatter, s,
-                (k_radix_scatter<K, false><<<xcd_grid(nb), SORT_THREADS, 0, s>>>(
-                    ka, nullptr, kb, nullptr, n, n_dev, shift, bits, hist, nb)));
+      FGS_WITH_BITS(bits, FGS_TIMED(nm.scatter, s,
+                                    (k_radix_scatter<K, false, B_><<<xcd_grid(nb), SORT_THREADS, 0, s>>>(
+                                        ka, nullptr, kb, nullptr, n, n_dev, shift, hist, nb))))
     K* tk = ka; ka = kb; kb = tk;
     uint32_t* tv = va; va = vb; vb = tv;
     in_alt = !in_alt;
@@ -624,11 +671,11 @@ __device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename K, bool VALS>
+template <typename K, bool VALS, int BITS>
 __global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
-    int bits, const uint32_t* __restrict__ hist, uint32_t* __restrict__ counter,
+    const uint32_t* __restrict__ hist, uint32_t* __restrict__ counter,
     uint32_t* __restrict__ status, uint32_t* __restrict__ status_next,
     uint32_t* __restrict__ fault) {
   constexpr int NW = SORT_THREADS / WAVE;
@@ -639,7 +686,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
   __shared__ uint32_t gofs[256];
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
-  const uint32_t mask = (1u << bits) - 1u;
+  constexpr uint32_t mask = (1u << BITS) - 1u;
   if (t == 0) s_tile = atomicAdd(counter, 1u);
 #pragma unroll
   for (int q = 0; q < NW; ++q) wcnt[q][t] = 0u;
@@ -650,10 +697,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
   if (base >= n) return;  // whole workgroup; no later tile waits on this one
   if (status_next) status_next[tile * 256 + t] = 0u;
   const int nvalid = (int)min((int64_t)SortTiling<K>::TILE, n - base);
-  K k[SortTiling<K>::ITEMS];
+  typename KeyReg<K>::T k[SortTiling<K>::ITEMS];
   uint32_t v[SortTiling<K>::ITEMS], rk[SortTiling<K>::ITEMS], c;
   load_tile<K, VALS>(kin, vin, base, nvalid, k, v);
-  rank_tile<K>(k, nvalid, shift, bits, wcnt, bstart, rk, c);
+  rank_tile<K, BITS>(k, nvalid, shift, wcnt, bstart, rk, c);
   // global offset of digit t: digits below it plus this digit in earlier tiles
   uint32_t gd;
   {
@@ -742,13 +789,15 @@ static hipError_t sort_kv_onesweep(K* keys, uint32_t* vals, K* keys_alt, uint32_
     uint32_t* ctr = w.counters + p;
     uint32_t* fault = w.counters + SORT_MAX_PASSES;
     if (vals)
-      FGS_TIMED(nm.onesweep, s,
-                (k_onesweep<K, true><<<(unsigned)nb, SORT_THREADS, 0, s>>>(
-                    ka, va, kb, vout, n, n_dev, shift, bits, h, ctr, st, st_next, fault)));
+      FGS_WITH_BITS(bits, FGS_TIMED(nm.onesweep, s,
+                                    (k_onesweep<K, true, B_><<<(unsigned)nb, SORT_THREADS, 0, s>>>(
+                                        ka, va, kb, vout, n, n_dev, shift, h, ctr, st, st_next,
+                                        fault))))
     else
-      FGS_TIMED(nm.onesweep, s,
-                (k_onesweep<K, false><<<(unsigned)nb, SORT_THREADS, 0, s>>>(
-                    ka, nullptr, kb, nullptr, n, n_dev, shift, bits, h, ctr, st, st_next, fault)));
+      FGS_WITH_BITS(bits, FGS_TIMED(nm.onesweep, s,
+                                    (k_onesweep<K, false, B_><<<(unsigned)nb, SORT_THREADS, 0, s>>>(
+                                        ka, nullptr, kb, nullptr, n, n_dev, shift, h, ctr, st,
+                                        st_next, fault))))
     K* tk = ka; ka = kb; kb = tk;
     uint32_t* tv = va; va = vb; vb = tv;
     in_alt = !in_alt;
