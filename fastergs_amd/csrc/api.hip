@@ -89,10 +89,13 @@ struct Fork {
     hipError_t r = hipEventRecord(c->ev[e], main);
     return r != hipSuccess ? r : hipStreamWaitEvent(c->st[i], c->ev[e], 0);
   }
+  // mark(i, e): event e at the current end of side stream i; wait(e): the
+  // main stream waits for it; join = mark + wait.
+  hipError_t mark(int i, int e) const { return c ? hipEventRecord(c->ev[e], c->st[i]) : hipSuccess; }
+  hipError_t wait(int e) const { return c ? hipStreamWaitEvent(main, c->ev[e], 0) : hipSuccess; }
   hipError_t join(int i, int e) const {
-    if (!c) return hipSuccess;
-    hipError_t r = hipEventRecord(c->ev[e], c->st[i]);
-    return r != hipSuccess ? r : hipStreamWaitEvent(main, c->ev[e], 0);
+    hipError_t r = mark(i, e);
+    return r != hipSuccess ? r : wait(e);
   }
 };
 
@@ -393,23 +396,12 @@ int fgs_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t*
   return FGS_OK;
 }
 
-int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const fgs_options* opt,
-                         void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream) {
-  Batch b;
-  int rc;
-  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt)) ||
-      (rc = check_gauss(g, &b.cams[0])))
-    return rc;
-  const int64_t PV = (int64_t)g->P * b.V;
-  if (PV >= ((int64_t)1 << 31)) return fail(FGS_ERR_ARG, "P x n_views must be < 2^31");
-  const GaussWs w = carve_gauss(gauss_ws, PV, b.T);
-  if (!gauss_ws || gauss_ws_bytes < w.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
-  if (g->P > 0 && !radii) return fail(FGS_ERR_ARG, "radii is NULL");
-  hipStream_t s = (hipStream_t)stream;
-  const fgs_camera* cam = &b.cams[0];
+// Arguments of the per-Gaussian passes over a carved gauss workspace.
+PreArgs pre_args(const Batch& b, const fgs_gaussians* g, const fgs_options* opt, const GaussWs& w,
+                 int32_t* radii) {
   PreArgs a;
   a.G = make_gauss(g);
-  a.cam = make_cam(cam, g);
+  a.cam = make_cam(&b.cams[0], g);
   a.geom = w.geom;
   a.splat = w.splat;
   a.dkey = w.dkey;
@@ -423,6 +415,23 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   a.counts = w.counts;
   a.views = (const ViewRec*)w.views;
   a.V = b.V;
+  return a;
+}
+
+int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const fgs_options* opt,
+                         void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream) {
+  Batch b;
+  int rc;
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt)) ||
+      (rc = check_gauss(g, &b.cams[0])))
+    return rc;
+  const int64_t PV = (int64_t)g->P * b.V;
+  if (PV >= ((int64_t)1 << 31)) return fail(FGS_ERR_ARG, "P x n_views must be < 2^31");
+  const GaussWs w = carve_gauss(gauss_ws, PV, b.T);
+  if (!gauss_ws || gauss_ws_bytes < w.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
+  if (g->P > 0 && !radii) return fail(FGS_ERR_ARG, "radii is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  const PreArgs a = pre_args(b, g, opt, w, radii);
   if (opt->schedule == FGS_SCHEDULE_PLAIN) {
     FGS_TRY(fill_u32(w.counts, 0u, GAUSS_SYNC_WORDS, s), "fill counts");
     if (g->P == 0) return FGS_OK;
@@ -436,13 +445,17 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
           "views");
   if (g->P == 0) return FGS_OK;
   FGS_TRY(launch_preprocess_fast(a, s), "preprocess");
-  // Three independent chains after the geometry pass: the SH colours (read
-  // only by compositing) and the depth sort of the records on side streams,
-  // the large-rect compaction and tile counts on the main stream.
+  // Independent chains after the geometry pass: on the main stream the
+  // large-rect compaction and tile counts (the longest; queued first), on a
+  // side stream the depth sort of the records, then the SH colours (read
+  // only by compositing). The pair offsets wait for the depth sort only.
   const Fork f(s);
-  FGS_TRY(f.fork(0, 0), "fork");
-  FGS_TRY(launch_sh_color(a, f.side(0)), "colour");
   FGS_TRY(f.fork(1, 1), "fork");
+  // large rects, compacted (count in counts[3]) and walked by whole waves
+  FGS_TRY(compact_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large compact");
+  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
+                             w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
+          "large tiles");
   // records in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by (global) tile alone yields the canonical per-tile
   // order of every view
@@ -451,16 +464,13 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
                             f.side(1), alt, nullptr, nullptr, SORT_DEPTH, w.dsort_sync),
           "depth sort");
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
-  // large rects, compacted (count in counts[3]) and walked by whole waves
-  FGS_TRY(compact_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large compact");
-  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
-                             w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
-          "large tiles");
-  FGS_TRY(f.join(1, 2), "join");
+  FGS_TRY(f.mark(1, 2), "mark");
+  FGS_TRY(launch_sh_color(a, f.side(1)), "colour");
+  FGS_TRY(f.wait(2), "join");
   FGS_TRY(scan_pair_offsets(w.pcount, w.dsort_v0, PV, w.rcount, w.poff, w.counts,
                             opt->bucket_size, b.T, w.scratch, s),
           "pair offsets");
-  FGS_TRY(f.join(0, 3), "join");
+  FGS_TRY(f.join(1, 3), "join");
   return FGS_OK;
 }
 

@@ -141,7 +141,7 @@ size_t fgs_grad_ws_size(int P);
 
 /* Replaces faster::preprocess (preprocess.cu:928-962, pybind.cpp:26-76):
  * projection, covariances, SH colour, tile rects, exact tile/ellipse pair
- * counts and the per-tile pair histogram. Writes every entry of radii (P int32). */
+ * counts and the pair offsets. Writes every entry of radii (P int32). */
 int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
                    void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream);
 
