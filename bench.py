@@ -600,7 +600,7 @@ def multiview_step(args, views=16, steps=5):
                                          None)
             ((img - t) ** 2).mean().backward()
 
-    tuner = Autotuner(buckets=(32, 64), splits=(32, 64, 128), reps=2)
+    tuner = Autotuner(buckets=(16, 32, 64), splits=(32, 64, 128), reps=2)
     best, table = tuner.tune(batched)
     times = {}
     for name, fn in (("batched", batched), ("sequential", sequential)):

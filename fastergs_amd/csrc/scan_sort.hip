@@ -1,17 +1,18 @@
-// Device-wide exclusive scan, stable LSD radix sorts and the stable counting
-// sort by tile for gfx950.
+// Device-wide exclusive scan and stable LSD radix sorts for gfx950.
 //
-// Scan: reduce-then-scan in three launches (per-tile sums, one-workgroup scan
-// of the sums, per-tile scan with the carried offset), one launch up to 8192
+// Scan: reduce-then-scan in two launches (per-tile sums; per-tile scan
+// whose carried offset each workgroup adds up from the tile sums before it),
+// a third launch scanning the sums beyond 4096 tiles, one launch up to 8192
 // entries. Used for the pair offsets (replacing the reference's
 // warp-aggregated atomics, which give a nondeterministic pair order), the
 // large-rect compaction and per-tile bucket offsets (render.cu:420).
 //
 // Radix sort of (key, u32 value) pairs with 16-, 32- or 64-bit keys
 // (replaces cub::DeviceRadixSort::SortPairs, FGS/csrc/cuda_rasterizer/
-// sort.cu:28-44): LSD, 8-bit digits, stable. Up to 4M keys every pass is one
+// sort.cu:28-44): LSD, digits of up to 8 bits spread evenly, stable; the
+// ranking is compiled per digit width. Up to 4M keys every pass is one
 // onesweep launch after one histogram read for all passes; above that each
-// pass is three launches (per-tile digit histogram, scan, scatter). Keys are
+// pass is a per-tile digit histogram, its scan and a scatter. Keys are
 // as narrow as the data allows: the FAST tile sort moves 16-bit tile ids
 // with 32-bit record values (6 B per pair per pass instead of a packed
 // 8 B key, 2 B per pair for the histograms).
@@ -56,10 +57,27 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ p
   }
 }
 
+// Up to PARTIALS_FUSED_MAX tiles, the apply launch adds up the tile sums
+// before its own tile itself (a strided sum and one block reduction)
+// instead of a separate launch scanning them.
+constexpr int PARTIALS_FUSED_MAX = 4096;
+__device__ __forceinline__ uint32_t partials_before(const uint32_t* __restrict__ partial, int b) {
+  uint32_t s = 0;
+  for (int j = threadIdx.x; j < b; j += SCAN_THREADS) s += partial[j];
+  uint32_t tot;
+  block_excl_scan<SCAN_THREADS>(s, tot);
+  return tot;
+}
+
+// FUSED: partial holds the tile sums (k_scan_partials not run); the last
+// workgroup writes the total. Else partial holds their exclusive scan.
+template <bool FUSED>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* in, uint32_t* out,
                                                              int64_t n,
-                                                             const uint32_t* __restrict__ partial) {
+                                                             const uint32_t* __restrict__ partial,
+                                                             uint32_t* __restrict__ total) {
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  const uint32_t carry = FUSED ? partials_before(partial, blockIdx.x) : partial[blockIdx.x];
   uint32_t v[SCAN_ITEMS];
   uint32_t s = 0;
 #pragma unroll
@@ -68,7 +86,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* in,
     s += v[k];
   }
   uint32_t tot;
-  uint32_t run = block_excl_scan<SCAN_THREADS>(s, tot) + partial[blockIdx.x];
+  uint32_t run = block_excl_scan<SCAN_THREADS>(s, tot) + carry;
+  if (FUSED && total && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total = carry + tot;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < n) out[base + k] = run;
@@ -115,9 +134,13 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
   }
   const int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
   uint32_t* partial = (uint32_t*)ws;
-  if (nb > 0) k_scan_reduce<<<nb, SCAN_THREADS, 0, s>>>(in, n, partial);
-  k_scan_partials<<<1, 1024, 0, s>>>(partial, nb, total);
-  if (nb > 0) k_scan_apply<<<nb, SCAN_THREADS, 0, s>>>(in, out, n, partial);
+  k_scan_reduce<<<nb, SCAN_THREADS, 0, s>>>(in, n, partial);
+  if (nb <= PARTIALS_FUSED_MAX) {
+    k_scan_apply<true><<<nb, SCAN_THREADS, 0, s>>>(in, out, n, partial, total);
+  } else {
+    k_scan_partials<<<1, 1024, 0, s>>>(partial, nb, total);
+    k_scan_apply<false><<<nb, SCAN_THREADS, 0, s>>>(in, out, n, partial, nullptr);
+  }
   return hipGetLastError();
 }
 
@@ -125,7 +148,7 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 // Exclusive scan of the records' pair counts taken in depth order
 // (count[order[r]]), written back in record order (off[order[r]]), with
 // the total pair count and the bucket-table bound ceil(total / B) + T: the
-// gather, scan, scatter and bound of the FAST preprocess in three launches
+// gather, scan, scatter and bound of the FAST preprocess in two launches
 // (one up to 8192 records).
 __global__ __launch_bounds__(SCAN_THREADS) void k_pscan_reduce(const uint32_t* __restrict__ count,
                                                                const uint32_t* __restrict__ order,
@@ -166,12 +189,18 @@ __global__ __launch_bounds__(1024) void k_pscan_partials(uint32_t* __restrict__ 
   }
 }
 
+// FUSED: partial holds the tile sums; the last workgroup writes the total
+// and the bucket-table bound. Else partial holds their exclusive scan.
+template <bool FUSED>
 __global__ __launch_bounds__(SCAN_THREADS) void k_pscan_apply(const uint32_t* __restrict__ gathered,
                                                               const uint32_t* __restrict__ order,
                                                               int64_t n,
                                                               const uint32_t* __restrict__ partial,
-                                                              uint32_t* __restrict__ off) {
+                                                              uint32_t* __restrict__ off,
+                                                              uint32_t* __restrict__ counts,
+                                                              int bucket, int T) {
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  const uint32_t carry = FUSED ? partials_before(partial, blockIdx.x) : partial[blockIdx.x];
   uint32_t v[SCAN_ITEMS];
   uint32_t s = 0;
 #pragma unroll
@@ -180,7 +209,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_pscan_apply(const uint32_t* __
     s += v[k];
   }
   uint32_t tot;
-  uint32_t run = block_excl_scan<SCAN_THREADS>(s, tot) + partial[blockIdx.x];
+  uint32_t run = block_excl_scan<SCAN_THREADS>(s, tot) + carry;
+  if (FUSED && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) {
+    const uint32_t all = carry + tot;
+    counts[0] = all;
+    counts[1] = (all + bucket - 1) / bucket + (uint32_t)T;
+  }
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < n) off[order[base + k]] = run;
@@ -225,19 +259,30 @@ hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64
   const int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
   uint32_t* partial = (uint32_t*)ws;
   k_pscan_reduce<<<nb, SCAN_THREADS, 0, s>>>(count, order, n, gathered, partial);
-  k_pscan_partials<<<1, 1024, 0, s>>>(partial, nb, counts, bucket, T);
-  k_pscan_apply<<<nb, SCAN_THREADS, 0, s>>>(gathered, order, n, partial, off);
+  if (nb <= PARTIALS_FUSED_MAX) {
+    k_pscan_apply<true><<<nb, SCAN_THREADS, 0, s>>>(gathered, order, n, partial, off, counts,
+                                                   bucket, T);
+  } else {
+    k_pscan_partials<<<1, 1024, 0, s>>>(partial, nb, counts, bucket, T);
+    k_pscan_apply<false><<<nb, SCAN_THREADS, 0, s>>>(gathered, order, n, partial, off, counts,
+                                                    bucket, T);
+  }
   return hipGetLastError();
 }
 
 // ------------------------------------------------------------ compaction
 // Indices i with flag[i] != 0, in index order, and their number: a scan
-// whose last launch writes the indices directly (one launch up to 8192).
+// whose last launch writes the indices directly (one launch up to 8192,
+// two beyond).
+// FUSED: partial holds the tile sums; the last workgroup writes the count.
+template <bool FUSED>
 __global__ __launch_bounds__(SCAN_THREADS) void k_compact_apply(const uint32_t* __restrict__ flag,
                                                                 int64_t n,
                                                                 const uint32_t* __restrict__ partial,
-                                                                uint32_t* __restrict__ ids) {
+                                                                uint32_t* __restrict__ ids,
+                                                                uint32_t* __restrict__ count) {
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  const uint32_t carry = FUSED ? partials_before(partial, blockIdx.x) : partial[blockIdx.x];
   uint32_t v[SCAN_ITEMS];
   uint32_t s = 0;
 #pragma unroll
@@ -246,7 +291,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact_apply(const uint32_t* 
     s += v[k];
   }
   uint32_t tot;
-  uint32_t run = block_excl_scan<SCAN_THREADS>(s, tot) + partial[blockIdx.x];
+  uint32_t run = block_excl_scan<SCAN_THREADS>(s, tot) + carry;
+  if (FUSED && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *count = carry + tot;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (v[k]) ids[run] = (uint32_t)(base + k);
@@ -286,8 +332,12 @@ hipError_t compact_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_
   uint32_t* partial = (uint32_t*)ws;
   // flags are 0 / 1, so the plain per-tile sums and their scan apply
   k_scan_reduce<<<nb, SCAN_THREADS, 0, s>>>(flag, n, partial);
-  k_scan_partials<<<1, 1024, 0, s>>>(partial, nb, count);
-  k_compact_apply<<<nb, SCAN_THREADS, 0, s>>>(flag, n, partial, ids);
+  if (nb <= PARTIALS_FUSED_MAX) {
+    k_compact_apply<true><<<nb, SCAN_THREADS, 0, s>>>(flag, n, partial, ids, count);
+  } else {
+    k_scan_partials<<<1, 1024, 0, s>>>(partial, nb, count);
+    k_compact_apply<false><<<nb, SCAN_THREADS, 0, s>>>(flag, n, partial, ids, count);
+  }
   return hipGetLastError();
 }
 
