@@ -445,17 +445,13 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
           "views");
   if (g->P == 0) return FGS_OK;
   FGS_TRY(launch_preprocess_fast(a, s), "preprocess");
-  // Independent chains after the geometry pass: on the main stream the
-  // large-rect compaction and tile counts (the longest; queued first), on a
-  // side stream the depth sort of the records, then the SH colours (read
-  // only by compositing). The pair offsets wait for the depth sort only.
+  // Independent chains after the geometry pass: on a side stream the depth
+  // sort of the records (queued first: its workgroups take their places
+  // before the tile counts fill the GPU), then the SH colours (read only by
+  // compositing); on the main stream the large-rect compaction and tile
+  // counts. The pair offsets wait for the depth sort only.
   const Fork f(s);
   FGS_TRY(f.fork(1, 1), "fork");
-  // large rects, compacted (count in counts[3]) and walked by whole waves
-  FGS_TRY(compact_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large compact");
-  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
-                             w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
-          "large tiles");
   // records in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by (global) tile alone yields the canonical per-tile
   // order of every view
@@ -466,6 +462,11 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
   FGS_TRY(f.mark(1, 2), "mark");
   FGS_TRY(launch_sh_color(a, f.side(1)), "colour");
+  // large rects, compacted (count in counts[3]) and walked by whole waves
+  FGS_TRY(compact_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large compact");
+  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
+                             w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
+          "large tiles");
   FGS_TRY(f.wait(2), "join");
   FGS_TRY(scan_pair_offsets(w.pcount, w.dsort_v0, PV, w.rcount, w.poff, w.counts,
                             opt->bucket_size, b.T, w.scratch, s),
