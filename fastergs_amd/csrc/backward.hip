@@ -141,7 +141,7 @@ hipError_t launch_backward_plain(const BwdArgs& a, hipStream_t s) {
 // pixel contributions are summed in registers, then over the wave with DPP,
 // and one atomic instruction adds the 9 (or 11) raw sums.
 template <bool ABS>
-__global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
+__global__ __launch_bounds__(256, 6) void k_backward_fast(BwdArgs a) {
   constexpr int NF = ABS ? RAW_N : RAW_POW + 1;
   const uint32_t A = a.actoff[a.T];
   const int lane = lane_id();
@@ -168,20 +168,22 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
     const float* dimg = a.dimg + 3 * poff;
     const int lt = tile - (int)vr.tile_off;
     const int tx = lt % gx, ty = lt / gx;
+    // Every pixel runs the same branch-free step: an inactive pixel is
+    // placed 1e7 pixels away (power far below the cut-off, alpha 0, every
+    // contribution 0) with zero dL/dpixel.
     float T[4], Q[4], d0[4], d1[4], d2[4], fx[4], fy[4];
-    bool act[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int lx = qx + (k & 1), ly = qy + (k >> 1);
       const int px = tx * TILE_W + lx, py = ty * TILE_H + ly;
-      fx[k] = (float)px;
       fy[k] = (float)py;
       const bool inside = px < W && py < H;
       const size_t gl = inside ? (size_t)py * W + px : 0;  // pixel in the view
       const size_t g = poff + gl;                            // pixel in the batch
-      act[k] = inside && lb * (uint32_t)a.bucket < (uint32_t)a.contrib[g];
+      const bool act = inside && lb * (uint32_t)a.bucket < (uint32_t)a.contrib[g];
+      fx[k] = act ? (float)px : 1.0e7f;
       T[k] = 1.0f; Q[k] = 0.0f; d0[k] = d1[k] = d2[k] = 0.0f;
-      if (act[k]) {
+      if (act) {
         const float4 ck = ((const float4*)a.acc)[bq + ly * TILE_W + lx];
         const float4 pc = ((const float4*)a.pixc)[g];
         T[k] = ck.w;
@@ -209,15 +211,14 @@ __global__ __launch_bounds__(256) void k_backward_fast(BwdArgs a) {
       const float4 A1 = rn[0], n1 = rn[1], n2 = rn[2];
       const float4 B1 = make_float4(n1.x, n1.y, n2.x, n2.y);
       const float C1 = rn[2].z;
+      // -0 start values: the first additions fold away (x + -0 = x)
       float acc[RAW_N];
 #pragma unroll
-      for (int f = 0; f < RAW_N; ++f) acc[f] = 0.0f;
+      for (int f = 0; f < RAW_N; ++f) acc[f] = -0.0f;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (act[q])
-          pix_bwd<float, FastRcp>(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q], fy[q],
-                                  d0[q], d1[q], d2[q], T[q], Q[q], acc);
-      }
+      for (int q = 0; q < 4; ++q)
+        pix_bwd<float, FastRcp, ABS>(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q],
+                                     fy[q], d0[q], d1[q], d2[q], T[q], Q[q], acc);
       wave_flush<NF>(acc, a.raw, id);
       id = idn;
       A0 = A1;

@@ -592,7 +592,8 @@ FGS_HD R residual_dot(R r0, R r1, R r2, R dp0, R dp1, R dp2, R tb) {
   return fma_(r2, dp2, fma_(r1, dp1, r0 * dp0)) + tb;
 }
 
-template <typename R, typename Rcp = IeeeRcp>
+// ABS = false leaves acc[RAW_AX], acc[RAW_AY] untouched.
+template <typename R, typename Rcp = IeeeRcp, bool ABS = true>
 FGS_HD void pix_bwd(R gx, R gy, R ca, R cb, R cc, R w, R r, R g, R b, R px, R py,
                     R dp0, R dp1, R dp2, R& T, R& Q, R* acc) {
   const R dx = gx - px, dy = gy - py;
@@ -615,8 +616,10 @@ FGS_HD void pix_bwd(R gx, R gy, R ca, R cb, R cc, R w, R r, R g, R b, R px, R py
   const R my = fma_(R(2) * cc, v, cb * u);
   acc[RAW_MX] += mx;
   acc[RAW_MY] += my;
-  acc[RAW_AX] += abs_(mx);
-  acc[RAW_AY] += abs_(my);
+  if (ABS) {
+    acc[RAW_AX] += abs_(mx);
+    acc[RAW_AY] += abs_(my);
+  }
   acc[RAW_KXX] = fma_(u, dx, acc[RAW_KXX]);
   acc[RAW_KXY] = fma_(u, dy, acc[RAW_KXY]);
   acc[RAW_KYY] = fma_(v, dy, acc[RAW_KYY]);
