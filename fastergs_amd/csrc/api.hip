@@ -44,7 +44,7 @@ int hip_check(hipError_t e, const char* where) {
 // e) makes the main stream wait for side stream i. Under capture the side
 // streams join the same graph through these events; every fork is joined
 // before the entry point returns.
-constexpr int SIDE_STREAMS = 2, SIDE_EVENTS = 8, SIDE_MAX_DEVICES = 64;
+constexpr int SIDE_STREAMS = 2, SIDE_EVENTS = 10, SIDE_MAX_DEVICES = 64;
 struct SideCtx {
   hipStream_t st[SIDE_STREAMS] = {};
   hipEvent_t ev[SIDE_EVENTS] = {};
@@ -572,8 +572,11 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
               "ranges and bucket counts");
     }
     FGS_TRY(scan_u32(gw.bcount, gw.boff, T, gw.boff + T, gw.scratch, s), "bucket scan");
+    // bucket -> (tile, list range) tables (kept for the frame view; the
+    // passes here index buckets through boff) beside compositing
+    FGS_TRY(f.fork(0, 8), "fork");
     FGS_TRY(launch_bucket_tables(T, NB, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
-                                 pw.bucket_tile, s),
+                                 pw.bucket_tile, f.side(0)),
             "bucket tables");
     c.vals = pw.v0;
     if (const fgs_tile_sched* ts = opt->tile_sched) {
@@ -596,6 +599,7 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
       if (ts->log_cycles) c.cycles = gw.cycles;
     }
     FGS_TRY(launch_composite_fast(c, s), "composite");
+    FGS_TRY(f.join(0, 9), "join");
     return FGS_OK;
   }
 
