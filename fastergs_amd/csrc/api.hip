@@ -568,10 +568,10 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
                                   pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES),
                 "tile sort");
       FGS_TRY(launch_tile_ranges(R, gw.counts, alt ? pw.t1 : pw.t0, pw.tile16, gw.ranges, T,
-                                 opt->bucket_size, gw.bcount, gw.maxc, s),
-              "ranges and bucket counts");
+                                 opt->bucket_size, gw.bcount, gw.maxc, gw.boff,
+                                 gw.counts + COUNT_ARRIVE_RANGES, s),
+              "ranges and bucket offsets");
     }
-    FGS_TRY(scan_u32(gw.bcount, gw.boff, T, gw.boff + T, gw.scratch, s), "bucket scan");
     // bucket -> (tile, list range) tables (kept for the frame view; the
     // passes here index buckets through boff) beside compositing
     FGS_TRY(f.fork(0, 8), "fork");

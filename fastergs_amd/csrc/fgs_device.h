@@ -89,6 +89,55 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
 
 __device__ __forceinline__ void atomic_add_f(float* p, float v) { unsafeAtomicAdd(p, v); }
 
+// Words handed from every workgroup of a grid to its last one travel
+// through agent-scope atomics (performed where all XCDs see them; a fence
+// here would write back the whole L2 of the XCD, which is slow after large
+// writes). publish_u32 swaps the word in and waits until it is done;
+// read_published_u32 reads such a word.
+__device__ __forceinline__ void publish_u32(uint32_t* p, uint32_t v) {
+  uint32_t old = __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  asm volatile("" : "+v"(old));  // the swap has returned, so it has been performed
+}
+__device__ __forceinline__ uint32_t read_published_u32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// True (in every thread) in the last workgroup of the grid to call it, once
+// every workgroup has; the words every workgroup published before its call
+// are then visible to read_published_u32. The counter is zero on entry and
+// reset by the last workgroup, so the next launch can use it again. Every
+// workgroup of the grid must call it once.
+__device__ __forceinline__ bool last_workgroup(uint32_t* counter) {
+  __shared__ uint32_t s_last;
+  __syncthreads();  // every thread's publishes have been performed
+  if (threadIdx.x == 0) {
+    const uint32_t n = gridDim.x * gridDim.y * gridDim.z;
+    const uint32_t prev = atomicAdd(counter, 1u);
+    s_last = prev == n - 1 ? 1u : 0u;
+    if (s_last) atomicExch(counter, 0u);
+  }
+  __syncthreads();
+  return s_last != 0u;
+}
+
+// Exclusive prefix of f(i) over i < n into out[0..n), total into out[n], by
+// one workgroup of NT threads (each sums a contiguous chunk).
+template <int NT, class F>
+__device__ __forceinline__ void block_scan_into(int n, F f, uint32_t* out) {
+  const int per = (n + NT - 1) / NT;
+  const int i0 = min((int)threadIdx.x * per, n), i1 = min(i0 + per, n);
+  uint32_t s = 0;
+  for (int i = i0; i < i1; ++i) s += f(i);
+  uint32_t tot;
+  uint32_t run = block_excl_scan<NT>(s, tot);
+  for (int i = i0; i < i1; ++i) {
+    const uint32_t v = f(i);
+    out[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) out[n] = tot;
+}
+
 // Element count of a stage whose size is produced on the device: the value
 // at n_dev (when given) bounded by the host-side capacity cap the launch
 // grid and the buffers were sized for.

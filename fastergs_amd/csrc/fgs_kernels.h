@@ -150,9 +150,11 @@ hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, cons
                                 hipStream_t s);
 // Ranges from the sorted tile ids, plus per-tile bucket counts and cleared
 // furthest-contributor slots (the FAST schedule's bucket_counts).
+// Ranges, bucket counts and (scanned by the last workgroup) the bucket
+// offsets boff; arrive: a zero counter word the launch leaves at zero.
 hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tiles, bool t16,
                               uint32_t* ranges, int T, int bucket, uint32_t* bcount,
-                              uint32_t* maxc, hipStream_t s);
+                              uint32_t* maxc, uint32_t* boff, uint32_t* arrive, hipStream_t s);
 hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
                                 hipStream_t s);
 struct CompArgs {

@@ -96,6 +96,8 @@ inline SortWs carve_sort(void* ws, int64_t nb) {
 }
 
 constexpr int GAUSS_SYNC_WORDS = 16 + SORT_SYNC_WORDS;
+// counts words the last-workgroup launches count arrivals in
+constexpr int COUNT_ARRIVE_RANGES = 6;
 // The depth sort's fault flag (last word of its state, counted from
 // counts), set when a look-back wait exceeds its bound.
 constexpr int DSORT_FAULT = 16 + SORT_SYNC_WORDS - 1;
@@ -118,7 +120,8 @@ struct GaussWs {
   uint32_t* actoff;    // T + 1: exclusive prefix of per-tile active buckets, [T] = total
   uint32_t* maxc;      // T
   uint32_t* rcount;    // fast: pair counts in depth order
-  uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects, [4] work items
+  uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects, [4] work items,
+                       // [6] arrival counter (COUNT_ARRIVE_RANGES)
   uint32_t* dsort_sync;  // fast: depth sort state (SORT_SYNC_WORDS), right after counts
   float* feat;         // T x 8 tile features (scheduled compositing)
   float* pred;         // T x 2 cost-model outputs (predicted log2 cost, split logit)

@@ -98,36 +98,51 @@ __device__ __forceinline__ uint32_t lower_bound_tile(const TK* a, uint32_t n, ui
 }
 // Also writes each tile's bucket count ceil(len / bucket) and clears its
 // furthest-contributor slot, which compositing combines with atomicMax when
-// a tile is split into row bands.
+// a tile is split into row bands; the last workgroup to finish scans the
+// bucket counts into boff (T + 1 entries, total at [T]). Workgroup b looks
+// up the lower bounds of tiles 255 b .. 255 b + 255 (one search per
+// thread) and writes the ranges of the first 255 of them.
+constexpr int RANGES_PER_BLOCK = 255;
 template <typename TK>
-__global__ void k_tile_ranges(int64_t R_cap, const uint32_t* __restrict__ n_dev,
-                              const TK* __restrict__ tiles, uint32_t* __restrict__ ranges, int T,
-                              int bucket, uint32_t* __restrict__ bcount,
-                              uint32_t* __restrict__ maxc) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= T) return;
+__global__ __launch_bounds__(256) void k_tile_ranges(int64_t R_cap,
+                                                     const uint32_t* __restrict__ n_dev,
+                                                     const TK* __restrict__ tiles,
+                                                     uint32_t* __restrict__ ranges, int T,
+                                                     int bucket, uint32_t* __restrict__ bcount,
+                                                     uint32_t* __restrict__ maxc,
+                                                     uint32_t* __restrict__ boff,
+                                                     uint32_t* __restrict__ arrive) {
+  __shared__ uint32_t s_lo[256];
+  const int t = blockIdx.x * RANGES_PER_BLOCK + threadIdx.x;
   const uint32_t R = (uint32_t)dev_count(n_dev, R_cap);
-  const uint32_t lo = lower_bound_tile(tiles, R, (uint32_t)t);
-  const uint32_t hi = lower_bound_tile(tiles, R, (uint32_t)t + 1);
-  // empty tiles keep [0, 0) like identifyTileRanges (render.cu:406-411)
-  ranges[2 * t] = lo < hi ? lo : 0u;
-  ranges[2 * t + 1] = lo < hi ? hi : 0u;
-  bcount[t] = (hi - lo + bucket - 1) / bucket;
-  maxc[t] = 0u;
+  if (t <= T) s_lo[threadIdx.x] = lower_bound_tile(tiles, R, (uint32_t)t);
+  __syncthreads();
+  if (threadIdx.x < RANGES_PER_BLOCK && t < T) {
+    const uint32_t lo = s_lo[threadIdx.x], hi = s_lo[threadIdx.x + 1];
+    // empty tiles keep [0, 0) like identifyTileRanges (render.cu:406-411)
+    ranges[2 * t] = lo < hi ? lo : 0u;
+    ranges[2 * t + 1] = lo < hi ? hi : 0u;
+    publish_u32(&bcount[t], (hi - lo + bucket - 1) / bucket);
+    maxc[t] = 0u;
+  }
+  if (last_workgroup(arrive))
+    block_scan_into<256>(T, [&](int i) { return read_published_u32(&bcount[i]); }, boff);
 }
 hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tiles, bool t16,
                               uint32_t* ranges, int T, int bucket, uint32_t* bcount,
-                              uint32_t* maxc, hipStream_t s) {
+                              uint32_t* maxc, uint32_t* boff, uint32_t* arrive, hipStream_t s) {
   if (T <= 0) return hipSuccess;
-  const unsigned grid = (unsigned)((T + 255) / 256);
+  const unsigned grid = (unsigned)((T + RANGES_PER_BLOCK - 1) / RANGES_PER_BLOCK);
   if (t16)
     FGS_TIMED("tile_ranges", s,
               (k_tile_ranges<uint16_t><<<grid, 256, 0, s>>>(R, n_dev, (const uint16_t*)tiles,
-                                                            ranges, T, bucket, bcount, maxc)));
+                                                            ranges, T, bucket, bcount, maxc,
+                                                            boff, arrive)));
   else
     FGS_TIMED("tile_ranges", s,
               (k_tile_ranges<uint32_t><<<grid, 256, 0, s>>>(R, n_dev, (const uint32_t*)tiles,
-                                                            ranges, T, bucket, bcount, maxc)));
+                                                            ranges, T, bucket, bcount, maxc,
+                                                            boff, arrive)));
   return hipGetLastError();
 }
 
@@ -365,7 +380,7 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   __syncthreads();
   if (t == 0) {
     const uint32_t mx = (uint32_t)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-    // the two bands of a split tile combine into the entry k_bucket_counts cleared
+    // the two bands of a split tile combine into the entry k_tile_ranges cleared
     if (band)
       atomicMax(&a.maxc[tile], mx);
     else
