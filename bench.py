@@ -88,7 +88,7 @@ def main():
                           dc=params["dc"], shs=params["shs"], scales=params["scales"],
                           rotations=params["rotations"], xyz_gradient_accum=accum,
                           max_radii=max_radii, denom=denom)
-        loss = ((img - target) ** 2).mean()
+        loss = torch.nn.functional.mse_loss(img, target)  # L2 loss, one fused reduction
         loss.backward()
         return img
 
@@ -521,7 +521,7 @@ def scene_replay(args, max_views=200):
             e0.record()
             if train:
                 img, _ = rast(**kw)
-                (img ** 2).mean().backward()
+                torch.nn.functional.mse_loss(img, torch.zeros_like(img)).backward()
             else:
                 with torch.no_grad():
                     rast(**kw)
@@ -585,7 +585,7 @@ def multiview_step(args, views=16, steps=5):
         imgs, _ = rasterize_views(settings, params["means3D"], params["opacities"],
                                   dc=params["dc"], shs=params["shs"], scales=params["scales"],
                                   rotations=params["rotations"], options=opts)
-        sum(((img - t) ** 2).mean() for img, t in zip(imgs, targets)).backward()
+        sum(torch.nn.functional.mse_loss(img, t) for img, t in zip(imgs, targets)).backward()
 
     def sequential(opts):
         for t in params.values():
@@ -598,7 +598,7 @@ def multiview_step(args, views=16, steps=5):
                                          params["opacities"], params["scales"],
                                          params["rotations"], None, st, None, None, None, None,
                                          None)
-            ((img - t) ** 2).mean().backward()
+            torch.nn.functional.mse_loss(img, t).backward()
 
     tuner = Autotuner(buckets=(16, 32, 64), splits=(32, 64, 128), reps=2)
     best, table = tuner.tune(batched)
