@@ -441,10 +441,9 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   }
   // FAST: the view table, then per-Gaussian math + lane-packed exact counts
   // of small rects for every view
-  FGS_TRY(launch_views_setup(b.setup, (ViewRec*)w.views, w.counts, GAUSS_SYNC_WORDS, s),
-          "views");
+  FGS_TRY(launch_preprocess_fast(a, b.setup, (ViewRec*)w.views, w.counts, GAUSS_SYNC_WORDS, s),
+          "preprocess");
   if (g->P == 0) return FGS_OK;
-  FGS_TRY(launch_preprocess_fast(a, s), "preprocess");
   // Independent chains after the geometry pass: on a side stream the depth
   // sort of the records (queued first: its workgroups take their places
   // before the tile counts fill the GPU), then the SH colours (read only by

@@ -60,6 +60,22 @@ __device__ __forceinline__ CamRef<float> view_cam(const CamRef<float>& base, con
   c.grid_y = r.grid_y;
   return c;
 }
+// Camera of view source s (caller's buffers) for the shared per-Gaussian math.
+__device__ __forceinline__ CamRef<float> src_cam(const CamRef<float>& base, const ViewSrc& s) {
+  CamRef<float> c = base;
+  c.view = s.view;
+  c.proj = s.proj;
+  c.campos = s.campos;
+  c.tanfovx = s.tanfovx;
+  c.tanfovy = s.tanfovy;
+  c.focal_x = s.focal_x;
+  c.focal_y = s.focal_y;
+  c.W = s.W;
+  c.H = s.H;
+  c.grid_x = (s.W + TILE_W - 1) / TILE_W;
+  c.grid_y = (s.H + TILE_H - 1) / TILE_H;
+  return c;
+}
 // View owning global tile t (binary search over the views' first tiles).
 __device__ __forceinline__ int view_of_tile(const ViewRec* views, int V, uint32_t t) {
   int lo = 0, hi = V;
@@ -121,7 +137,11 @@ struct PreArgs {
   int V;
 };
 hipError_t launch_preprocess_plain(const PreArgs& a, hipStream_t s);
-hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s);
+// Geometry pass of the FAST schedule; also writes the view table from vs
+// and clears zero_n (< 2^31) words at zero. With P = 0 only the table and
+// the clearing are done.
+hipError_t launch_preprocess_fast(const PreArgs& a, const ViewSetup& vs, ViewRec* table,
+                                  uint32_t* zero, int zero_n, hipStream_t s);
 // SH colour of every valid record (after launch_preprocess_fast; only the
 // compositing pass reads its output). No-op with precomputed colours.
 hipError_t launch_sh_color(const PreArgs& a, hipStream_t s);

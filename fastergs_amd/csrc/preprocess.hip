@@ -214,10 +214,21 @@ __device__ __forceinline__ void stage_small(WaveTiles& L, int lane, bool small, 
 // Only 3 KB of LDS per wave64 workgroup, so occupancy is set by registers.
 constexpr int PRE_THREADS = WAVE;
 // Grid (ceil(P / 64), V): blockIdx.y is the view, records v * P + i.
-__global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a) {
+__device__ __forceinline__ void write_view(const ViewSetup& src, ViewRec* out, int v, int t);
+
+// The cameras come from the kernel arguments (the caller's device
+// buffers); workgroup (0, 0) also writes the view table the later passes
+// read and clears the frame's counters (zero_n words).
+__global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a, ViewSetup vs,
+                                                                 ViewRec* table, uint32_t* zero,
+                                                                 int zero_n) {
   __shared__ WaveTiles L;
   const int v = blockIdx.y;
-  const CamRef<float> cam = view_cam(a.cam, a.views[v]);
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    for (int k = threadIdx.x; k < zero_n; k += PRE_THREADS) zero[k] = 0u;
+    for (int u = 0; u < vs.n; ++u) write_view(vs, table, u, threadIdx.x);
+  }
+  const CamRef<float> cam = src_cam(a.cam, vs.v[v]);
   const int i = blockIdx.x * PRE_THREADS + threadIdx.x;
   const int lane = threadIdx.x;
   const size_t ri = (size_t)v * a.P + i;
@@ -316,11 +327,13 @@ __global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
   }
 }
 
-hipError_t launch_preprocess_fast(const PreArgs& a, hipStream_t s) {
-  if (a.P <= 0) return hipSuccess;
+hipError_t launch_preprocess_fast(const PreArgs& a, const ViewSetup& vs, ViewRec* table,
+                                  uint32_t* zero, int zero_n, hipStream_t s) {
+  if (a.P <= 0) return launch_views_setup(vs, table, zero, zero_n, s);
   const unsigned blocks = (a.P + PRE_THREADS - 1) / PRE_THREADS;
   FGS_TIMED("preprocess_fast", s,
-            (k_preprocess_fast<<<dim3(blocks, (unsigned)a.V), PRE_THREADS, 0, s>>>(a)));
+            (k_preprocess_fast<<<dim3(blocks, (unsigned)a.V), PRE_THREADS, 0, s>>>(a, vs, table,
+                                                                                zero, zero_n)));
   return hipGetLastError();
 }
 
@@ -455,12 +468,8 @@ hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* ge
 }
 
 // View table from kernel arguments (see ViewSetup).
-__global__ void k_views_setup(ViewSetup src, ViewRec* out, uint32_t* zero, int64_t zero_n) {
-  const int v = blockIdx.x;
-  const int t = threadIdx.x;
-  for (int64_t i = (int64_t)v * blockDim.x + t; i < zero_n; i += (int64_t)gridDim.x * blockDim.x)
-    zero[i] = 0u;
-  if (v >= src.n) return;
+// Thread t (< 20) of the copy of view v into the table.
+__device__ __forceinline__ void write_view(const ViewSetup& src, ViewRec* out, int v, int t) {
   const ViewSrc& s = src.v[v];
   ViewRec& r = out[v];
   if (t < 16) {
@@ -484,6 +493,14 @@ __global__ void k_views_setup(ViewSetup src, ViewRec* out, uint32_t* zero, int64
     r.pix_off = s.pix_off;
     r.pad0 = r.pad1 = 0u;
   }
+}
+
+__global__ void k_views_setup(ViewSetup src, ViewRec* out, uint32_t* zero, int64_t zero_n) {
+  const int v = blockIdx.x;
+  const int t = threadIdx.x;
+  for (int64_t i = (int64_t)v * blockDim.x + t; i < zero_n; i += (int64_t)gridDim.x * blockDim.x)
+    zero[i] = 0u;
+  if (v < src.n) write_view(src, out, v, t);
 }
 
 hipError_t launch_views_setup(const ViewSetup& src, ViewRec* out, uint32_t* zero, int64_t zero_n,
