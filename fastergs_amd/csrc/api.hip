@@ -426,7 +426,8 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
       (rc = check_gauss(g, &b.cams[0])))
     return rc;
   const int64_t PV = (int64_t)g->P * b.V;
-  if (PV >= ((int64_t)1 << 31)) return fail(FGS_ERR_ARG, "P x n_views must be < 2^31");
+  // band list entries are record * LARGE_BANDS + band in 32 bits
+  if (PV >= ((int64_t)1 << 29)) return fail(FGS_ERR_ARG, "P x n_views must be < 2^29");
   const GaussWs w = carve_gauss(gauss_ws, PV, b.T);
   if (!gauss_ws || gauss_ws_bytes < w.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
   if (g->P > 0 && !radii) return fail(FGS_ERR_ARG, "radii is NULL");
@@ -461,10 +462,11 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
   FGS_TRY(f.mark(1, 2), "mark");
   FGS_TRY(launch_sh_color(a, f.side(1)), "colour");
-  // large rects, compacted (count in counts[3]) and walked by whole waves
-  FGS_TRY(compact_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large compact");
+  // large rects, one list entry per row band (count in counts[3]), each band
+  // walked by a whole wave
+  FGS_TRY(expand_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large bands");
   FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
-                             w.counts, w.pcount, nullptr, 0u, nullptr, nullptr, false, s),
+                             w.counts, w.pcount, nullptr, w.lband, 0u, nullptr, nullptr, false, s),
           "large tiles");
   FGS_TRY(f.wait(2), "join");
   FGS_TRY(scan_pair_offsets(w.pcount, w.dsort_v0, PV, w.rcount, w.poff, w.counts,
@@ -546,11 +548,12 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
     // small and large rects write disjoint pair blocks: side by side
     const Fork f(s);
     FGS_TRY(f.fork(0, 4), "fork");
-    FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.poff, (uint32_t)R,
+    FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.dsort_v0, gw.poff,
+                              (uint32_t)R,
                               pw.t0, pw.r0, pw.tile16, f.side(0)),
             "emit");
     FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts, nullptr,
-                               gw.poff, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s),
+                               gw.poff, gw.lband, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s),
             "emit large");
     FGS_TRY(f.join(0, 5), "join");
     // stable sort of (tile id, record) by tile id: the last pass writes the

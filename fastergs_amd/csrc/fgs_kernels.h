@@ -95,10 +95,10 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64_t n,
                              uint32_t* gathered, uint32_t* off, uint32_t* counts, int bucket,
                              int T, void* ws, hipStream_t s);
-// ids = the indices i < n with flag[i] (0 / 1) set, ascending; *count = their
-// number (on the device).
-hipError_t compact_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t* count, void* ws,
-                         hipStream_t s);
+// ids = i * LARGE_BANDS + b for every index i < n and b < flag[i] (flags
+// 0..LARGE_BANDS), ascending; *count = their number (on the device).
+hipError_t expand_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t* count, void* ws,
+                        hipStream_t s);
 // Stable LSD radix sort of (key, value) pairs on key bits [begin_bit,
 // end_bit) (keys of 16, 32 or 64 bits; vals may be NULL for key-only sorts).
 // sync: when given, SORT_SYNC_WORDS words that are zero on entry, used for
@@ -129,7 +129,7 @@ struct PreArgs {
   int32_t* radii;
   int P;
   int split_tiles;
-  uint32_t* lflag;     // fast: 1 for Gaussians whose rect is walked by whole workgroups
+  uint32_t* lflag;     // fast: row bands of a large rect (walked by whole waves), 0 if small
   uint32_t* dsort_k;   // fast: depth keys for the Gaussian depth sort
   uint32_t* dsort_v;   // fast: Gaussian indices for the depth sort
   uint32_t* counts;
@@ -155,11 +155,11 @@ hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey
 // most 65536 tiles), else 32-bit.
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
-                              bool t16, hipStream_t s);
+                              const uint32_t* poff, uint32_t* lband, uint32_t cap, void* tkeys,
+                              uint32_t* recs, bool t16, hipStream_t s);
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
-                             const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
-                             bool t16, hipStream_t s);
+                             const uint32_t* order, const uint32_t* poff, uint32_t cap,
+                             void* tkeys, uint32_t* recs, bool t16, hipStream_t s);
 
 // render.hip
 // Range kernels take the pair count as (capacity R, device count n_dev).

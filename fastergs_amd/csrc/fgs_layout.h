@@ -95,6 +95,11 @@ inline SortWs carve_sort(void* ws, int64_t nb) {
   return carve_sort(c, nb);
 }
 
+// Large rects are walked in up to LARGE_BANDS row bands of about
+// LARGE_BAND_TILES tiles each, one wave per band.
+constexpr int LARGE_BANDS = 8;
+constexpr int LARGE_BAND_TILES = 512;
+
 constexpr int GAUSS_SYNC_WORDS = 16 + SORT_SYNC_WORDS;
 // counts words the last-workgroup launches count arrivals in
 constexpr int COUNT_ARRIVE_RANGES = 6;
@@ -112,8 +117,9 @@ struct GaussWs {
   uint32_t* dsort_k1;
   uint32_t* dsort_v0;  // fast: Gaussian index by depth rank after the sort
   uint32_t* dsort_v1;
-  uint32_t* lflag;     // fast: rect walked by whole workgroups (1) or lane-packed (0)
-  uint32_t* large_ids; // fast: flagged Gaussians in index order
+  uint32_t* lflag;     // fast: row bands of a large rect (walked by whole waves), 0 if small
+  uint32_t* large_ids; // fast: row bands of large rects (record * LARGE_BANDS + band), in order
+  uint32_t* lband;     // fast: hits of each listed band
   uint32_t* ranges;    // T x 2
   uint32_t* boff;      // T + 1: exclusive prefix of per-tile bucket counts, [T] = NB
   uint32_t* bcount;    // T + 1 scratch (bucket counts, then active bucket counts)
@@ -151,7 +157,8 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   w.dsort_v0 = c.take<uint32_t>(p);
   w.dsort_v1 = c.take<uint32_t>(p);
   w.lflag = c.take<uint32_t>(p);
-  w.large_ids = c.take<uint32_t>(p);
+  w.large_ids = c.take<uint32_t>((size_t)LARGE_BANDS * p);
+  w.lband = c.take<uint32_t>((size_t)LARGE_BANDS * p);
   w.ranges = c.take<uint32_t>(2 * t);
   w.boff = c.take<uint32_t>(t + 1);
   w.bcount = c.take<uint32_t>(t + 1);

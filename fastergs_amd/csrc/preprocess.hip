@@ -143,6 +143,28 @@ struct WaveTiles {
   int r0[WAVE], r1[WAVE], w[WAVE];
   uint32_t incl[WAVE], area[WAVE], cnt[WAVE];
 };
+// Emission also stages each Gaussian's record, first pair slot, view tile
+// offset and tile-grid width.
+struct WaveEmit {
+  uint32_t rec[WAVE], pos[WAVE], toff[WAVE];
+  int gx[WAVE];
+};
+
+// Row bands of a large rect of wr x hr tiles: up to LARGE_BANDS bands of
+// `rows` whole tile rows, about LARGE_BAND_TILES tiles each (none for an
+// empty rect).
+__device__ __forceinline__ void large_bands(int wr, int hr, int& rows, int& nbands) {
+  if (wr <= 0 || hr <= 0) {
+    rows = 1;
+    nbands = 0;
+    return;
+  }
+  const int area = wr * hr;
+  int want = (area + LARGE_BAND_TILES - 1) / LARGE_BAND_TILES;
+  want = want < 1 ? 1 : (want > LARGE_BANDS ? LARGE_BANDS : want);
+  rows = (hr + want - 1) / want;
+  nbands = (hr + rows - 1) / rows;
+}
 
 // One emitted pair: its (global) tile id, 16- or 32-bit, and its record.
 __device__ __forceinline__ void store_pair(void* tkeys, uint32_t* recs, bool t16, uint32_t pos,
@@ -155,12 +177,12 @@ __device__ __forceinline__ void store_pair(void* tkeys, uint32_t* recs, bool t16
 }
 
 // Walk the concatenated small rects of one wave 64 candidate tiles per step
-// (lane-packed); count (EMIT=false) or emit (EMIT=true) the pairs that pass
-// the exact tile test. Within a Gaussian's block of the pair array the order
-// is free: its pairs are in distinct tiles and the tile sort is stable.
+// (lane-packed); count (EMIT=false) or emit (EMIT=true, E staged) the pairs
+// that pass the exact tile test. Within a Gaussian's block of the pair array
+// the order is free: its pairs are in distinct tiles and the tile sort is
+// stable.
 template <bool EMIT>
-__device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, uint32_t gbase, int gx,
-                                           uint32_t toff, const uint32_t* __restrict__ poff,
+__device__ __forceinline__ void walk_small(WaveTiles& L, const WaveEmit* E, uint32_t S,
                                            uint32_t cap, void* __restrict__ tkeys,
                                            uint32_t* __restrict__ recs, bool t16) {
   const int lane = lane_id();
@@ -180,9 +202,10 @@ __device__ __forceinline__ void walk_small(WaveTiles& L, uint32_t S, uint32_t gb
       if (tile_hit(tx, ty, L.x[j], L.y[j], conic, L.pw[j])) {
         const uint32_t k = atomicAdd(&L.cnt[j], 1u);
         if (EMIT) {
-          const uint32_t g = gbase + (uint32_t)j;
-          const uint32_t pos = poff[g] + k;
-          if (pos < cap) store_pair(tkeys, recs, t16, pos, toff + (uint32_t)(ty * gx + tx), g);
+          const uint32_t pos = E->pos[j] + k;
+          if (pos < cap)
+            store_pair(tkeys, recs, t16, pos, E->toff[j] + (uint32_t)(ty * E->gx[j] + tx),
+                       E->rec[j]);
         }
       }
     }
@@ -249,13 +272,18 @@ __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a, View
   }
   const int area = o.valid ? (o.rect[2] - o.rect[0]) * (o.rect[3] - o.rect[1]) : 0;
   const bool small = o.valid && area < a.split_tiles;
-  // large rects: flagged here, compacted by a scan (no same-address atomics)
-  if (i < a.P) a.lflag[ri] = (o.valid && !small) ? 1u : 0u;
+  // large rects: their band counts here, expanded into one list entry per
+  // band by a scan (no same-address atomics)
+  if (i < a.P) {
+    int rows, nbands = 0;
+    if (o.valid && !small) large_bands(o.rect[2] - o.rect[0], o.rect[3] - o.rect[1], rows, nbands);
+    a.lflag[ri] = (uint32_t)nbands;
+  }
   uint32_t S;
   stage_small(L, lane, small, o.x, o.y, o.conic, o.power, o.rect[0], o.rect[1], o.rect[2],
               o.rect[3], S);
   __syncthreads();
-  walk_small<false>(L, S, 0u, cam.grid_x, 0u, nullptr, 0u, nullptr, nullptr, false);
+  walk_small<false>(L, nullptr, S, 0u, nullptr, nullptr, false);
   __syncthreads();
   if (i < a.P) a.pcount[ri] = L.cnt[lane];
 }
@@ -346,12 +374,20 @@ hipError_t launch_sh_color(const PreArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Large rects: one wave per Gaussian of the compact list (grid-stride over
-// the device-side count). Each lane keeps one tile column (its tile_col
-// terms formed once); a rect of width w <= 64 is walked 64 / w rows per
-// step, a wider one 64 columns at a time, row by row. Counting adds the
-// hits to the Gaussian's pair count; emission places the k-th hit of the
-// walk at poff + k (a running ballot prefix, no atomics).
+// Large rects: each rect is cut into up to LARGE_BANDS bands of whole tile
+// rows (about LARGE_BAND_TILES tiles per band; large_bands), and the band
+// list (entry = record * LARGE_BANDS + band, the bands of a record next to
+// each other) is walked one band per wave, grid-stride over the device-side
+// count. A full-screen rect is then 8 waves of about 7 rows instead of one
+// wave of every row, which keeps the slowest wave short. Each lane keeps one
+// tile column (its tile_col terms formed once); a band of width w <= 64 is
+// walked 64 / w rows per step, a wider one 64 columns at a time, row by row.
+// Counting stores the band's hits in lband and adds them to the Gaussian's
+// pair count; emission places the k-th hit of band b at poff + (hits of the
+// record's bands < b) + k (a running ballot prefix, no atomics). The order
+// of one Gaussian's pairs within its slot range does not matter: the tile
+// sort that follows is stable and each Gaussian appears at most once per
+// tile.
 template <bool EMIT>
 __device__ __forceinline__ void large_step(bool hit, int tx, int ty, uint32_t base, uint32_t& run,
                                            int gx, uint32_t toff, uint32_t g, uint32_t cap,
@@ -371,99 +407,131 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
                                                      const uint32_t* __restrict__ counts,
                                                      uint32_t* pcount,
                                                      const uint32_t* __restrict__ poff,
+                                                     uint32_t* lband,
                                                      uint32_t cap, void* tkeys, uint32_t* recs,
                                                      int t16) {
-  const uint32_t NL = counts[3];
+  const uint32_t NT = counts[3];
   const int lane = lane_id();
   const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
-  for (uint32_t j = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; j < NL; j += nwaves) {
-    const uint32_t g = large_ids[j];  // record index v * P + i
+  for (uint32_t q = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; q < NT; q += nwaves) {
+    const uint32_t id = large_ids[q];
+    const uint32_t g = id / LARGE_BANDS;  // record index v * P + i
+    const int band = (int)(id % LARGE_BANDS);
+    const GeomRow r = load_geom(geom, g);
+    const int wr = r.r2 - r.r0;
+    int rows, nbands;
+    large_bands(wr, r.r3 - r.r1, rows, nbands);  // band < nbands for a listed band
+    (void)nbands;
+    const int y0 = r.r1 + band * rows;
+    const int y1 = min(y0 + rows, r.r3);
     const int v = V > 1 ? (int)(g / (uint32_t)P) : 0;
     const int gx = views[v].grid_x;
     const uint32_t toff = views[v].tile_off;
-    const GeomRow r = load_geom(geom, g);
-    const uint32_t base = EMIT ? poff[g] : 0u;
-    const int wr = r.r2 - r.r0;
+    uint32_t base = 0u;
+    if (EMIT) {
+      base = poff[g];
+      for (int k = 1; k <= band; ++k) base += lband[q - k];
+    }
     uint32_t run = 0;
     if (wr <= WAVE) {
       const int k = WAVE / wr;  // rows per step
       const int rs = lane / wr;
       const int tx = r.r0 + (lane - rs * wr);
       const TileCol c = tile_col(tx, r.x, r.conic, r.power);
-      for (int row0 = r.r1; row0 < r.r3; row0 += k) {
+      for (int row0 = y0; row0 < y1; row0 += k) {
         const int ty = row0 + rs;
         const TileRow rw = tile_row(ty, r.y, r.conic, r.power);
-        const bool hit = rs < k && ty < r.r3 && tile_hit_terms(c, rw, r.conic);
+        const bool hit = rs < k && ty < y1 && tile_hit_terms(c, rw, r.conic);
         large_step<EMIT>(hit, tx, ty, base, run, gx, toff, g, cap, tkeys, recs, t16);
       }
     } else {
       for (int c0 = r.r0; c0 < r.r2; c0 += WAVE) {
         const int tx = c0 + lane;
         const TileCol c = tile_col(tx, r.x, r.conic, r.power);
-        for (int ty = r.r1; ty < r.r3; ++ty) {
+        for (int ty = y0; ty < y1; ++ty) {
           const TileRow rw = tile_row(ty, r.y, r.conic, r.power);
           const bool hit = tx < r.r2 && tile_hit_terms(c, rw, r.conic);
           large_step<EMIT>(hit, tx, ty, base, run, gx, toff, g, cap, tkeys, recs, t16);
         }
       }
     }
-    if (!EMIT && lane == 0 && run) atomicAdd(&pcount[g], run);
+    if (!EMIT && lane == 0) {
+      lband[q] = run;
+      if (run) atomicAdd(&pcount[g], run);
+    }
   }
 }
 
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
-                              bool t16, hipStream_t s) {
+                              const uint32_t* poff, uint32_t* lband, uint32_t cap, void* tkeys,
+                              uint32_t* recs, bool t16, hipStream_t s) {
   if (P <= 0) return hipSuccess;
-  const int blocks = 2048;  // persistent: 8192 waves, each exits past the device-side count
+  const int blocks = 4096;  // 16384 waves, grid-stride; each exits past the device-side count
   if (emit)
     FGS_TIMED("large_tiles_emit", s,
               (k_large_tiles<true><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
-                                                          pcount, poff, cap, tkeys, recs,
+                                                          pcount, poff, lband, cap, tkeys, recs,
                                                           t16 ? 1 : 0)));
   else
     FGS_TIMED("large_tiles_count", s,
               (k_large_tiles<false><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
-                                                           pcount, poff, cap, tkeys, recs,
+                                                           pcount, poff, lband, cap, tkeys, recs,
                                                            t16 ? 1 : 0)));
   return hipGetLastError();
 }
 
-// Grid (ceil(P / 256), V): blockIdx.y is the view; each wave walks the small
-// rects of 64 consecutive Gaussians of one view.
-__global__ __launch_bounds__(256) void k_emit_small(int P, const ViewRec* __restrict__ views,
+// Emission of the small rects in depth-rank order: wave w walks the records
+// of ranks [64 w, 64 w + 64) (order = records by depth rank). Pair slots are
+// assigned in that order, so a wave's pairs fill one contiguous stretch of
+// the pair array (apart from the slots of large rects among them, which
+// k_large_tiles fills) and its stores combine in L2.
+__global__ __launch_bounds__(256) void k_emit_small(int P, int NR, const ViewRec* __restrict__ views,
                                                     const float* __restrict__ geom, int split,
+                                                    const uint32_t* __restrict__ order,
                                                     const uint32_t* __restrict__ poff, uint32_t cap,
                                                     void* __restrict__ tkeys,
                                                     uint32_t* __restrict__ recs, int t16) {
   __shared__ WaveTiles lds[4];
-  const int v = blockIdx.y;
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  __shared__ WaveEmit lde[4];
+  const int rank = blockIdx.x * 256 + threadIdx.x;
   const int lane = lane_id();
   WaveTiles& L = lds[threadIdx.x / WAVE];
-  const uint32_t gbase = (uint32_t)((size_t)v * P) + blockIdx.x * 256 + (threadIdx.x / WAVE) * WAVE;
+  WaveEmit& E = lde[threadIdx.x / WAVE];
   GeomRow r{};
   bool small = false;
-  if (i < P) {
-    r = load_geom(geom, (size_t)v * P + i);
+  uint32_t g = 0u, pos = 0u, toff = 0u;
+  int gx = 1;
+  if (rank < NR) {
+    g = order[rank];
+    r = load_geom(geom, g);
     const int area = (r.r2 - r.r0) * (r.r3 - r.r1);
     small = area > 0 && area < split;
+    if (small) {
+      const int v = NR > P ? (int)(g / (uint32_t)P) : 0;
+      pos = poff[g];
+      gx = views[v].grid_x;
+      toff = views[v].tile_off;
+    }
   }
   uint32_t S;
   stage_small(L, lane, small, r.x, r.y, r.conic, r.power, r.r0, r.r1, r.r2, r.r3, S);
+  E.rec[lane] = g;
+  E.pos[lane] = pos;
+  E.toff[lane] = toff;
+  E.gx[lane] = gx;
   __syncthreads();
-  walk_small<true>(L, S, gbase, views[v].grid_x, views[v].tile_off, poff, cap, tkeys, recs,
-                   t16 != 0);
+  walk_small<true>(L, &E, S, cap, tkeys, recs, t16 != 0);
 }
 
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
-                             const uint32_t* poff, uint32_t cap, void* tkeys, uint32_t* recs,
-                             bool t16, hipStream_t s) {
-  if (P > 0)
+                             const uint32_t* order, const uint32_t* poff, uint32_t cap,
+                             void* tkeys, uint32_t* recs, bool t16, hipStream_t s) {
+  const int64_t NR = (int64_t)P * V;
+  if (NR > 0)
     FGS_TIMED("emit_small", s,
-              (k_emit_small<<<dim3((P + 255) / 256, (unsigned)V), 256, 0, s>>>(
-                  P, views, geom, split, poff, cap, tkeys, recs, t16 ? 1 : 0)));
+              (k_emit_small<<<(unsigned)((NR + 255) / 256), 256, 0, s>>>(
+                  P, (int)NR, views, geom, split, order, poff, cap, tkeys, recs, t16 ? 1 : 0)));
   return hipGetLastError();
 }
 

@@ -270,9 +270,10 @@ hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------ compaction
-// Indices i with flag[i] != 0, in index order, and their number: a scan
-// whose last launch writes the indices directly (one launch up to 8192,
+// ------------------------------------------------------------ expansion
+// Entry i with flag[i] = c > 0 (c <= LARGE_BANDS) becomes the c consecutive
+// ids i * LARGE_BANDS + b, b < c, in index order; count = total ids. A scan
+// whose last launch writes the ids directly (one launch up to 8192 entries,
 // two beyond).
 // FUSED: partial holds the tile sums; the last workgroup writes the count.
 template <bool FUSED>
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact_apply(const uint32_t* 
   uint32_t s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    v[k] = (base + k < n && flag[base + k]) ? 1u : 0u;
+    v[k] = base + k < n ? flag[base + k] : 0u;
     s += v[k];
   }
   uint32_t tot;
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact_apply(const uint32_t* 
   if (FUSED && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *count = carry + tot;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    if (v[k]) ids[run] = (uint32_t)(base + k);
+    for (uint32_t b = 0; b < v[k]; ++b) ids[run + b] = (uint32_t)(base + k) * LARGE_BANDS + b;
     run += v[k];
   }
 }
@@ -304,25 +305,25 @@ __global__ __launch_bounds__(SCAN1_THREADS) void k_compact_single(const uint32_t
                                                                   int64_t n,
                                                                   uint32_t* __restrict__ ids,
                                                                   uint32_t* __restrict__ count) {
-  const int64_t b = (int64_t)threadIdx.x * SCAN1_ITEMS;
+  const int64_t b0 = (int64_t)threadIdx.x * SCAN1_ITEMS;
   uint32_t v[SCAN1_ITEMS];
   uint32_t sum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN1_ITEMS; ++k) {
-    v[k] = (b + k < n && flag[b + k]) ? 1u : 0u;
+    v[k] = b0 + k < n ? flag[b0 + k] : 0u;
     sum += v[k];
   }
   uint32_t tot;
   uint32_t run = block_excl_scan<SCAN1_THREADS>(sum, tot);
 #pragma unroll
   for (int k = 0; k < SCAN1_ITEMS; ++k) {
-    if (v[k]) ids[run] = (uint32_t)(b + k);
+    for (uint32_t b = 0; b < v[k]; ++b) ids[run + b] = (uint32_t)(b0 + k) * LARGE_BANDS + b;
     run += v[k];
   }
   if (threadIdx.x == 0) *count = tot;
 }
 
-hipError_t compact_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t* count, void* ws,
+hipError_t expand_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t* count, void* ws,
                          hipStream_t s) {
   if (n <= SCAN1_MAX) {
     k_compact_single<<<1, SCAN1_THREADS, 0, s>>>(flag, n, ids, count);
@@ -330,7 +331,7 @@ hipError_t compact_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_
   }
   const int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
   uint32_t* partial = (uint32_t*)ws;
-  // flags are 0 / 1, so the plain per-tile sums and their scan apply
+  // the plain per-tile sums of the flags and their scan apply
   k_scan_reduce<<<nb, SCAN_THREADS, 0, s>>>(flag, n, partial);
   if (nb <= PARTIALS_FUSED_MAX) {
     k_compact_apply<true><<<nb, SCAN_THREADS, 0, s>>>(flag, n, partial, ids, count);
