@@ -670,7 +670,7 @@ def learned_schedule_eval(args, train_views=60, eval_views=40, n=None):
         del fr
         losses.append(trainer.train(20))
     train_s = time.perf_counter() - t0
-    names = ("composite_fast", "sched_items", "tile_features", "mlp_infer")
+    names = ("composite_fast", "sched_items", "tile_features", "mlp_infer", "tile_predict")
     scheds = {"raster": TileSchedule.raster(), "heuristic": TileSchedule.heuristic(split_len),
               "learned": TileSchedule.learned(model),
               "learned_ordered": TileSchedule.learned(model, order=True)}

@@ -583,10 +583,13 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
     c.vals = pw.v0;
     if (const fgs_tile_sched* ts = opt->tile_sched) {
       const bool learned = ts->mode == FGS_TILES_LEARNED;
-      if (learned || ts->features)
+      if (learned)
+        FGS_TRY(launch_tile_predict(T, gw.ranges, pw.v0, gw.geom, gw.splat, ts->mlp_params,
+                                    gw.feat, gw.pred, s),
+                "cost model");
+      else if (ts->features)
         FGS_TRY(launch_tile_features(T, gw.ranges, pw.v0, gw.geom, gw.splat, gw.feat, s),
                 "tile features");
-      if (learned) FGS_TRY(launch_mlp_infer(ts->mlp_params, gw.feat, T, gw.pred, true, s), "cost model");
       if (learned && ts->order) {
         FGS_TRY(launch_sched_items(T, gw.ranges, opt->bucket_size, gw.pred, gw.items,
                                    gw.counts + 4, s),

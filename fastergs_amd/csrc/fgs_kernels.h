@@ -220,6 +220,12 @@ hipError_t launch_render_depth(int W, int H, int T, const uint32_t* ranges, cons
 hipError_t launch_tile_features(int T, const uint32_t* ranges, const uint32_t* vals,
                                 const float* geom, const float* splat, float* feat,
                                 hipStream_t s);
+// Tile features (into feat) and the cost model's bf16 prediction (into
+// pred) in one launch: what launch_tile_features + launch_mlp_infer(bf16)
+// compute, to float rounding of the feature sums.
+hipError_t launch_tile_predict(int T, const uint32_t* ranges, const uint32_t* vals,
+                               const float* geom, const float* splat, const float* params,
+                               float* feat, float* pred, hipStream_t s);
 hipError_t launch_mlp_infer(const float* params, const float* x, int64_t n, float* y, bool bf16,
                             hipStream_t s);
 size_t mlp_grad_ws_bytes(int64_t n);

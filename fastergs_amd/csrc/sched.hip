@@ -175,6 +175,140 @@ hipError_t launch_mlp_infer(const float* params, const float* x, int64_t n, floa
   return hipGetLastError();
 }
 
+// Features and bf16 inference of the learned schedule in one launch: one
+// wave per tile (grid-stride), 16 waves per workgroup, so that a frame's
+// tiles are all in flight at once and the dependent gathers of their
+// features (range -> list entries -> records) overlap. The workgroup stages
+// the bf16-rounded weights once in LDS (W2 rows padded to 65 floats, so lane
+// j reading row j meets no bank conflicts) and each lane then keeps hidden
+// unit j's weights in registers, as k_mlp_infer does. The features are those of k_tile_features (two sample entries per
+// lane; sums in a different order, so equal to float rounding), the network
+// is evaluated with k_mlp_infer's arithmetic (same rounding points and
+// accumulation order).
+constexpr int PRED_THREADS = 1024;
+constexpr int PRED_BLOCKS_MAX = 1024;
+constexpr int W2_PAD = MLP_HID + 1;
+
+__global__ __launch_bounds__(PRED_THREADS) void k_tile_predict(
+    int T, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ vals,
+    const float* __restrict__ geom, const float* __restrict__ splat,
+    const float* __restrict__ params, float* __restrict__ feat, float* __restrict__ pred) {
+  __shared__ float sW2[MLP_HID * W2_PAD];
+  __shared__ float sW1[MLP_HID * (MLP_IN + 1)];
+  __shared__ float sB[MLP_HID * 2 + MLP_OUT * MLP_HID + MLP_OUT];  // b1 | b2 | w3 | b3
+  const int t = threadIdx.x;
+  for (int k = t; k < MLP_HID * MLP_HID; k += PRED_THREADS)
+    sW2[(k / MLP_HID) * W2_PAD + (k % MLP_HID)] = bf16_round(params[MLP_W2 + k]);
+  for (int k = t; k < MLP_HID * MLP_IN; k += PRED_THREADS)
+    sW1[(k / MLP_IN) * (MLP_IN + 1) + (k % MLP_IN)] = bf16_round(params[MLP_W1 + k]);
+  for (int k = t; k < MLP_HID; k += PRED_THREADS) {
+    sB[k] = bf16_round(params[MLP_B1 + k]);
+    sB[MLP_HID + k] = bf16_round(params[MLP_B2 + k]);
+  }
+  for (int k = t; k < MLP_OUT * MLP_HID + MLP_OUT; k += PRED_THREADS)
+    sB[2 * MLP_HID + k] = bf16_round(params[MLP_W3 + k]);  // w3 rows, then b3
+  __syncthreads();
+  const int lane = t & (WAVE - 1);
+  float w1[MLP_IN], w2[MLP_HID];
+#pragma unroll
+  for (int k = 0; k < MLP_IN; ++k) w1[k] = sW1[lane * (MLP_IN + 1) + k];
+#pragma unroll
+  for (int k = 0; k < MLP_HID; ++k) w2[k] = sW2[lane * W2_PAD + k];
+  const float b1 = sB[lane], b2 = sB[MLP_HID + lane];
+  const float w30 = sB[2 * MLP_HID + lane], w31 = sB[2 * MLP_HID + MLP_HID + lane];
+  const float b30 = sB[2 * MLP_HID + 2 * MLP_HID], b31 = sB[2 * MLP_HID + 2 * MLP_HID + 1];
+  const int nwaves = gridDim.x * (PRED_THREADS / WAVE);
+  for (int tile = blockIdx.x * (PRED_THREADS / WAVE) + t / WAVE; tile < T; tile += nwaves) {
+    const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
+    const uint32_t len = end - start;
+    const int n = (int)min(len, (uint32_t)FEAT_SAMPLE);
+    float la[2], d[2];
+    int bin[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = lane + h * WAVE;
+      la[h] = 0.0f;
+      d[h] = 0.0f;
+      bin[h] = -1;
+      if (e < n) {
+        const uint32_t g = vals[start + e];
+        const float* gr = geom + 8 * (size_t)g;
+        const uint32_t rmin = f2u(gr[6]), rmax = f2u(gr[7]);
+        const int area = (int)((rmax & 0xffffu) - (rmin & 0xffffu)) * (int)((rmax >> 16) - (rmin >> 16));
+        la[h] = log2f((float)max(area, 1));
+        const float* sp = splat + 12 * (size_t)g;
+        bin[h] = min((int)(exp2f(sp[5]) * 16.0f), 15);
+        d[h] = sp[6];
+      }
+    }
+    const bool v0 = lane < n, v1 = lane + WAVE < n;
+    const float S = wave_sum(la[0] + la[1]);
+    float M = fmaxf(la[0], la[1]);
+    float dn = fminf(v0 ? d[0] : 3.0e38f, v1 ? d[1] : 3.0e38f);
+    float dx = fmaxf(d[0], d[1]);
+    for (int o = 32; o > 0; o >>= 1) {
+      M = fmaxf(M, __shfl_xor(M, o, WAVE));
+      dn = fminf(dn, __shfl_xor(dn, o, WAVE));
+      dx = fmaxf(dx, __shfl_xor(dx, o, WAVE));
+    }
+    float x[MLP_IN];
+    x[0] = log2f(1.0f + (float)len);
+    x[1] = n > 0 ? S / (float)n : 0.0f;
+    x[2] = M;
+    // opacity quantiles from the 16-bin histogram (ballot counts)
+    const float qs[4] = {0.25f, 0.5f, 0.75f, 1.0f};
+    uint32_t cum = 0, hb = 0;
+    {
+      int b = 0;
+      hb = (uint32_t)(__popcll(__ballot(bin[0] == 0)) + __popcll(__ballot(bin[1] == 0)));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t need = n > 0 ? (uint32_t)ceilf(qs[k] * (float)n) : 0u;
+        while (b < 15 && cum + hb < need) {
+          cum += hb;
+          ++b;
+          hb = (uint32_t)(__popcll(__ballot(bin[0] == b)) + __popcll(__ballot(bin[1] == b)));
+        }
+        x[3 + k] = n > 0 ? ((float)b + 0.5f) / 16.0f : 0.0f;
+      }
+    }
+    x[7] = (n > 0 && dn > 0.0f) ? log2f(dx / dn) : 0.0f;
+    if (feat && lane < MLP_IN) {
+      float fv = x[0];
+#pragma unroll
+      for (int k = 1; k < MLP_IN; ++k) fv = lane == k ? x[k] : fv;
+      feat[8 * (size_t)tile + lane] = fv;
+    }
+    // the network (k_mlp_infer<true>)
+    float z1 = b1;
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) z1 = fmaf(w1[k], bf16_round(x[k]), z1);
+    const float h1 = bf16_round(fmaxf(z1, 0.0f));
+    float z2 = b2;
+#pragma unroll
+    for (int k = 0; k < MLP_HID; ++k)
+      z2 = fmaf(w2[k], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), k)), z2);
+    const float h2 = bf16_round(fmaxf(z2, 0.0f));
+    const float o0 = wave_sum(w30 * h2), o1 = wave_sum(w31 * h2);
+    if (lane == 0) {
+      pred[2 * tile] = o0 + b30;
+      pred[2 * tile + 1] = o1 + b31;
+    }
+  }
+}
+
+hipError_t launch_tile_predict(int T, const uint32_t* ranges, const uint32_t* vals,
+                               const float* geom, const float* splat, const float* params,
+                               float* feat, float* pred, hipStream_t s) {
+  if (T <= 0) return hipSuccess;
+  const int want = (T + PRED_THREADS / WAVE - 1) / (PRED_THREADS / WAVE);  // a tile per wave
+  const unsigned blocks = (unsigned)(want > PRED_BLOCKS_MAX ? PRED_BLOCKS_MAX : want);
+  FGS_TIMED("tile_predict", s,
+            (k_tile_predict<<<blocks, PRED_THREADS, 0, s>>>(T, ranges, vals, geom, splat, params,
+                                                            feat, pred)));
+  return hipGetLastError();
+}
+
 // Training rows: fp32 forward, loss and backward to the pre-activation
 // gradients, one thread per row in one-wave workgroups (the row's h1 and gz2
 // live in LDS, column-major per wave, so no per-thread arrays spill). Per row

@@ -119,6 +119,26 @@ def test_tile_features(scene_cam):
     assert checked > 50
 
 
+def test_fused_features_and_prediction(scene_cam):
+    """The learned schedule computes the tile features and the bf16 cost
+    prediction in one launch: its features equal the separate feature pass
+    to float rounding of the sums, and its prediction is the standalone bf16
+    inference applied to those same features, bit for bit."""
+    d, cam = scene_cam
+    ref = _frame(d, cam, TileSchedule.raster(features=True))
+    fref = ref.view_arrays()["tile_features"].cpu().numpy()
+    for seed in (0, 7):
+        model = CostModel(seed=seed)
+        fr = _frame(d, cam, TileSchedule.learned(model))
+        v = fr.view_arrays()
+        T = v["ranges"].numel() // 2
+        feat = v["tile_features"].reshape(T, 8)
+        np.testing.assert_allclose(feat.cpu().numpy(), fref.reshape(T, 8), rtol=1e-5, atol=1e-5)
+        pred = v["tile_pred"].cpu().numpy().reshape(T, 2)
+        alone = model.forward(feat, bf16=True).cpu().numpy()
+        np.testing.assert_array_equal(pred.view(np.uint32), alone.view(np.uint32))
+
+
 def _items(fr):
     v = fr.view_arrays()
     n = int(v["counts"][4].item())
