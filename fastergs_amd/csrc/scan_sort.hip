@@ -416,7 +416,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const K* __restrict
       if (j < m) atomicAdd(&cnt[(uint32_t)(kv.k[j] >> shift) & mask], 1u);
   }
   __syncthreads();
-  hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+  // digit-major rows of the pass's (mask + 1) digits only
+  if (threadIdx.x <= mask) hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
 // Register form of a tile's keys: 16-bit keys sit in the low half of a
@@ -573,7 +574,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
   const int nvalid = (int)min((int64_t)SortTiling<K>::TILE, n - base);
 #pragma unroll
   for (int q = 0; q < NW; ++q) wcnt[q][t] = 0u;
-  gofs[t] = hist[(size_t)t * nb + tile];
+  gofs[t] = t < (1 << BITS) ? hist[(size_t)t * nb + tile] : 0u;  // rows of the pass's digits
   typename KeyReg<K>::T k[SortTiling<K>::ITEMS];
   uint32_t v[SortTiling<K>::ITEMS], rk[SortTiling<K>::ITEMS], cnt;
   load_tile<K, VALS>(kin, vin, base, nvalid, k, v);
@@ -628,7 +629,7 @@ static hipError_t sort_kv_classic(K* keys, uint32_t* vals, K* keys_alt, uint32_t
     uint32_t* vout = (last && last_vals) ? last_vals : vb;
     FGS_TIMED(nm.hist, s,
               (k_radix_hist<K><<<nb, SORT_THREADS, 0, s>>>(ka, n, n_dev, shift, mask, hist, nb)));
-    hipError_t e = scan_u32(hist, hist, (int64_t)256 * nb, nullptr, scan_ws, s);
+    hipError_t e = scan_u32(hist, hist, (int64_t)(mask + 1) * nb, nullptr, scan_ws, s);
     if (e != hipSuccess) return e;
     if (vals)
       FGS_WITH_BITS(bits, FGS_TIMED(nm.scatter, s,
