@@ -253,7 +253,8 @@ int fgs_adam_update(float* params, const float* grad, float* exp_avg, float* exp
  * (dopacity, dcolor, dmean3D, dcov3D, ddc, dsh, dscale, drot) is the sum
  * over the views, added in view order; fgs_batch_densify_stats adds the
  * views' statistics in view order. Each single-view entry point above is
- * the batch of its one camera. */
+ * the batch of its one camera. P x n_views must stay below 2^29 (FGS_ERR_ARG
+ * otherwise). */
 typedef struct fgs_batch {
   int n_views;
   const fgs_camera* cameras; /* host array of n_views cameras */
