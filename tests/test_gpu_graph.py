@@ -95,3 +95,100 @@ def test_capacity_overflow_reported_and_grown():
     assert rend.check()
     ref = _eager(d, cam, SCHEDULE_FAST)
     np.testing.assert_array_equal(img.cpu().numpy(), ref.out_color.cpu().numpy())
+
+
+# ---------------------------------------------------------------------------
+# Captured training steps (forward, L2 loss gradient, backward, statistics)
+
+def _views(k, sizes):
+    cams = scenes.orbit_cameras(6, 64, 64, radius=3.0)
+    out = []
+    for j, (w, h) in enumerate(sizes):
+        c = cams[(k + 2 * j) % 6]
+        out.append(scenes.look_at_camera(tuple(c.campos), w, h))
+    return out
+
+
+def _eager_step(d, cams, targets, stats):
+    from fastergs_amd.multiview import BatchFrame
+    fr = BatchFrame([settings_for(c) for c in cams], d["means3D"], d["opacities"], d["scales"],
+                    d["rotations"], dc=d["dc"], shs=d["shs"], abs_grad=True).forward()
+    scale = torch.cat([torch.full((3 * c.height * c.width,), 2.0 / (3 * c.height * c.width),
+                                  device="cuda") for c in cams])
+    dimg = (fr.out_color - torch.cat([t.reshape(-1) for t in targets])) * scale
+    g = fr.backward(dimg)
+    if stats is not None:
+        fr.densify_stats(g["mean2D"], *stats)
+    torch.cuda.synchronize()
+    return fr, g
+
+
+@pytest.mark.parametrize("sizes", [[(320, 240)], [(200, 150), (320, 240), (160, 96)]],
+                         ids=["one_view", "three_views"])
+def test_captured_training_step_matches_eager(sizes):
+    """Replays of a captured step for changing cameras and targets: images
+    and radii bit for bit, gradients and densification statistics to float
+    atomics' reassociation, with eager work and host synchronisation between
+    replays."""
+    from fastergs_amd.graph import CapturedTrainingStep
+    from tests.helpers import grad_close
+    print(f"\n[graph] captured step, views {sizes}", flush=True)
+    d = to_dev(_scene())
+    P = d["means3D"].shape[0]
+    z = lambda *s: torch.zeros(*s, device="cuda")
+    stats = (z(P, 1), z(P, 1), z(P), z(P, 1))
+    cams0 = _views(0, sizes)
+    step = CapturedTrainingStep([settings_for(c) for c in cams0], d["means3D"], d["opacities"],
+                                d["scales"], d["rotations"], dc=d["dc"], shs=d["shs"],
+                                xyz_gradient_accum=stats[0], xyz_gradient_accum_abs=stats[1],
+                                max_radii=stats[2], denom=stats[3], headroom=1.5)
+    ref_stats = tuple(t.clone() for t in stats)
+    for k in (0, 1, 3, 0):
+        cams = _views(k, sizes)
+        step.set_cameras([_cam_tensors(c) for c in cams])
+        tg = [torch.from_numpy(scenes.target_image(c.width, c.height, seed=k)).cuda() for c in cams]
+        for t, src in zip(step.targets(), tg):
+            t.copy_(src)
+        g = {n: t.clone() for n, t in step.step().items()}
+        img = step.out_color.clone()
+        radii = step.radii.clone()
+        torch.cuda.synchronize()
+        fr, ref = _eager_step(d, cams, tg, ref_stats)  # eager work between replays
+        np.testing.assert_array_equal(img.cpu().numpy(), fr.out_color.cpu().numpy())
+        np.testing.assert_array_equal(radii.cpu().numpy(), fr.radii.cpu().numpy())
+        for n in ("opacities", "dc", "shs", "mean2D", "conic"):
+            grad_close(g[n].cpu().numpy(), ref[n].cpu().numpy(), rtol=1e-4, name=n)
+        for n in ("means3D", "scales", "rotations"):
+            a, b = g[n].cpu().numpy().ravel(), ref[n].cpu().numpy().ravel()
+            assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b), n
+    assert step.check()
+    np.testing.assert_array_equal(stats[3].cpu().numpy(), ref_stats[3].cpu().numpy())
+    np.testing.assert_array_equal(stats[2].cpu().numpy(), ref_stats[2].cpu().numpy())
+    for a, b in ((stats[0], ref_stats[0]), (stats[1], ref_stats[1])):
+        a, b = a.cpu().numpy().ravel(), b.cpu().numpy().ravel()
+        assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b)
+
+
+def test_captured_training_step_capacity():
+    """A capacity below the pair count is reported, and a grown capture
+    matches the eager step again."""
+    from fastergs_amd.graph import CapturedTrainingStep
+    d = to_dev(_scene())
+    near = scenes.look_at_camera((0.0, 0.0, -2.0), W, H)
+    far = scenes.look_at_camera((0.0, 0.0, -6.0), W, H)
+    step = CapturedTrainingStep([settings_for(far)], d["means3D"], d["opacities"], d["scales"],
+                                d["rotations"], dc=d["dc"], shs=d["shs"], headroom=1.0)
+    step.set_cameras([_cam_tensors(near)])
+    step.step()
+    with pytest.raises(CapacityError):
+        step.check()
+    step.step()
+    assert not step.check(grow=True)
+    tg = torch.from_numpy(scenes.target_image(W, H, seed=2)).cuda()
+    step.targets()[0].copy_(tg)
+    g = {n: t.clone() for n, t in step.step().items()}
+    assert step.check()
+    fr, ref = _eager_step(d, [near], [tg], None)
+    np.testing.assert_array_equal(step.out_color.cpu().numpy(), fr.out_color.cpu().numpy())
+    a, b = g["opacities"].cpu().numpy().ravel(), ref["opacities"].cpu().numpy().ravel()
+    assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b)
