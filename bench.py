@@ -135,6 +135,13 @@ def main():
         plain_ms, _ = timed(SCHEDULE_PLAIN, max(5, args.steps // 2), 2)
 
     graph_fps = graph_render_fps(settings, params, sched, args)
+    step_graph_ms = None
+    if sched == SCHEDULE_FAST:
+        for t in params.values():
+            t.grad = None
+        step_graph_ms = captured_step_ms([settings], params, [target],
+                                         RasterOptions(schedule=sched, bucket_size=args.bucket),
+                                         steps=args.steps, stats=True)
     adam = sparse_adam_step(rast, params)
     ssim = fused_ssim_step(cam)
     lists = list_passes_step(settings, params, args)
@@ -170,6 +177,7 @@ def main():
         "fwd_ms": round(fwd_ms, 4),
         "render_fps_graph": graph_fps,
         "wall_ms_per_step": round(wall, 4),
+        "ms_per_step_graph": None if step_graph_ms is None else round(step_graph_ms, 4),
         "plain_ms_per_step": None if plain_ms is None else round(plain_ms, 4),
         "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms, 3),
         "roofline": roofline,
@@ -402,6 +410,40 @@ def _time_replays(rend, camt, reps):
     return e0.elapsed_time(e1)
 
 
+def captured_step_ms(settings_list, params, targets, opts, steps=10, stats=False):
+    """ms per replay of a training step captured in one HIP graph
+    (CapturedTrainingStep: forward, L2 loss gradient, backward, and with
+    stats the densification statistics) for the given views and targets."""
+    import torch
+    from fastergs_amd.graph import CapturedTrainingStep
+    p = {k: v.detach() for k, v in params.items()}
+    P = p["means3D"].shape[0]
+    kw = {}
+    if stats:
+        dev = p["means3D"].device
+        kw = dict(xyz_gradient_accum=torch.zeros(P, 1, device=dev),
+                  max_radii=torch.zeros(P, device=dev), denom=torch.zeros(P, 1, device=dev))
+    step = CapturedTrainingStep(settings_list, p["means3D"], p["opacities"], p["scales"],
+                                p["rotations"], dc=p["dc"], shs=p["shs"], options=opts,
+                                headroom=1.1, **kw)
+    for t, src in zip(step.targets(), targets):
+        t.copy_(src.reshape(t.shape))
+    for _ in range(3):
+        step.step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step.step()
+    e1.record()
+    torch.cuda.synchronize()
+    step.check()
+    ms = e0.elapsed_time(e1) / steps
+    del step
+    torch.cuda.empty_cache()
+    return ms
+
+
 def graph_render_fps(settings, params, sched, args):
     """Forward-only FPS of the heavy workload with the frame captured in one HIP graph."""
     import torch
@@ -613,12 +655,17 @@ def multiview_step(args, views=16, steps=5):
         e1.record()
         torch.cuda.synchronize()
         times[name] = e0.elapsed_time(e1) / steps
+    for t in params.values():
+        t.grad = None
+    torch.cuda.empty_cache()
+    times["captured"] = captured_step_ms(settings, params, targets, best, steps=steps)
     del params
     return {"workload": f"multiview: {P} Gaussians, {views} views "
                         f"({sum(c.width * c.height for c in cams)} pixels, sizes "
                         f"{sorted(set((c.width, c.height) for c in cams))}), fwd+bwd per step",
             "ms_per_step_batched": round(times["batched"], 3),
             "ms_per_step_sequential": round(times["sequential"], 3),
+            "ms_per_step_batched_graph": round(times["captured"], 3),
             "views_per_s": round(views * 1000.0 / times["batched"], 1),
             "speedup_vs_sequential": round(times["sequential"] / times["batched"], 3),
             "autotuned": {"bucket_size": best.bucket_size, "split_tiles": best.split_tiles},
