@@ -100,6 +100,9 @@ SIGNATURES = {
     "fgs_frame_view_get": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int64, ctypes.c_int64,
                                           ctypes.POINTER(Options), ctypes.POINTER(FrameView)]),
+    "fgs_bucket_tables": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(Options),
+                                         vp]),
     "fgs_sort_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
     "fgs_sort_pairs": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int,
                                       ctypes.c_int, vp, ctypes.c_size_t, vp]),
@@ -138,6 +141,9 @@ SIGNATURES = {
                                                 ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.POINTER(Options),
                                                 ctypes.POINTER(FrameView)]),
+    "fgs_batch_bucket_tables": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.POINTER(Batch),
+                                               ctypes.c_int64, ctypes.c_int64,
+                                               ctypes.POINTER(Options), vp]),
     "fgs_sparse_adam": (ctypes.c_int, [ctypes.POINTER(AdamTensor), ctypes.c_int, vp, ctypes.c_int,
                                        ctypes.c_float, ctypes.c_float, vp]),
     "fgs_ssim_forward": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, vp, vp, ctypes.c_int,

@@ -574,12 +574,9 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
                                  gw.counts + COUNT_ARRIVE_RANGES, s),
               "ranges and bucket offsets");
     }
-    // bucket -> (tile, list range) tables (kept for the frame view; the
-    // passes here index buckets through boff) beside compositing
-    FGS_TRY(f.fork(0, 8), "fork");
-    FGS_TRY(launch_bucket_tables(T, NB, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
-                                 pw.bucket_tile, f.side(0)),
-            "bucket tables");
+    // the bucket -> (tile, list range) tables are not built here: every pass
+    // indexes buckets through boff; fgs_batch_bucket_tables builds them on
+    // demand
     c.vals = pw.v0;
     if (const fgs_tile_sched* ts = opt->tile_sched) {
       const bool learned = ts->mode == FGS_TILES_LEARNED;
@@ -604,7 +601,6 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
       if (ts->log_cycles) c.cycles = gw.cycles;
     }
     FGS_TRY(launch_composite_fast(c, s), "composite");
-    FGS_TRY(f.join(0, 9), "join");
     return FGS_OK;
   }
 
@@ -814,6 +810,34 @@ int fgs_densify_stats(int P, const int32_t* radii, const float* dmean2D, float* 
                       float* xyz_grad_accum_abs, float* max_radii, float* denom, void* stream) {
   return fgs_batch_densify_stats(P, 1, radii, dmean2D, xyz_grad_accum, xyz_grad_accum_abs,
                                  max_radii, denom, stream);
+}
+
+int fgs_batch_bucket_tables(void* gauss_ws, void* pair_ws, int P, const fgs_batch* batch,
+                            int64_t R, int64_t NB, const fgs_options* opt, void* stream) {
+  Batch b;
+  int rc;
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b))) return rc;
+  if (opt->schedule != FGS_SCHEDULE_FAST) return fail(FGS_ERR_ARG, "bucket tables need the FAST schedule");
+  if (!gauss_ws || !pair_ws) return fail(FGS_ERR_ARG, "NULL workspace");
+  if (P < 0 || R < 0 || NB < 0) return fail(FGS_ERR_ARG, "bad frame size");
+  if (P == 0 || NB == 0) return FGS_OK;
+  const int T = b.T;
+  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, true);
+  FGS_TRY(launch_bucket_tables(T, NB, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
+                               pw.bucket_tile, (hipStream_t)stream),
+          "bucket tables");
+  return FGS_OK;
+}
+
+int fgs_bucket_tables(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                      int64_t NB, const fgs_options* opt, void* stream) {
+  fgs_camera c{};
+  c.width = width;
+  c.height = height;
+  c.tanfovx = c.tanfovy = 1.0f;
+  const fgs_batch b{1, &c};
+  return fgs_batch_bucket_tables(gauss_ws, pair_ws, P, &b, R, NB, opt, stream);
 }
 
 int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,

@@ -158,6 +158,12 @@ class BatchFrame:
 
     def view_arrays(self) -> dict:
         """Copies of the batch's intermediate arrays (lists, ranges, buckets)."""
+        if getattr(self, "pws", None) is not None:
+            # the bucket tables are built on demand (no pass of the step reads them)
+            check(lib().fgs_batch_bucket_tables(_ptr(self.gws), _ptr(self.pws), self.P,
+                                                ctypes.byref(self.batch), self.R, self.NB,
+                                                ctypes.byref(self.copts), self._s()),
+                  "fgs_batch_bucket_tables")
         v = FrameView()
         check(lib().fgs_batch_frame_view_get(_ptr(self.gws), _ptr(getattr(self, "pws", None)),
                                              self.P, ctypes.byref(self.batch), self.R, self.NB,

@@ -297,6 +297,12 @@ class Frame:
     # -- inspection -----------------------------------------------------------
     def view_arrays(self) -> dict:
         """Torch views of the intermediate arrays (valid until the frame dies)."""
+        if self.opts.schedule == _lib.SCHEDULE_FAST and getattr(self, "pws", None) is not None:
+            # the bucket tables are built on demand (no pass of the step reads them)
+            check(lib().fgs_bucket_tables(_ptr(self.gws), _ptr(self.pws), self.P, self.W, self.H,
+                                          self.R_cap, self.NB_cap, ctypes.byref(self.copts),
+                                          ctypes.c_void_p(_stream(self.device))),
+                  "fgs_bucket_tables")
         v = FrameView()
         check(lib().fgs_frame_view_get(_ptr(self.gws), _ptr(getattr(self, "pws", None)), self.P,
                                        self.W, self.H, self.R_cap, self.NB_cap,

@@ -207,6 +207,15 @@ int fgs_render_depth(void* gauss_ws, void* pair_ws, int P, int width, int height
                      const float* campos, float* out_pts, float* out_depth, float* accum_alpha,
                      int32_t* gidx, float* discriminants, void* stream);
 
+/* Bucket -> (list range, tile) tables of a rendered FAST frame, built on
+ * demand into the pair workspace and read through the frame view (replaces
+ * the tables of perBucketRange, FGS/csrc/cuda_rasterizer/render.cu:25-49,
+ * which the reference's backward reads; here every pass indexes buckets
+ * through the per-tile bucket offsets, so no pass of a training step needs
+ * them). R and NB as given to fgs_render. */
+int fgs_bucket_tables(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
+                      int64_t NB, const fgs_options* opt, void* stream);
+
 /* Views of the workspaces' arrays (host-side pointer arithmetic only). */
 int fgs_frame_view_get(void* gauss_ws, void* pair_ws, int P, int width, int height, int64_t R,
                        int64_t NB, const fgs_options* opt, fgs_frame_view* view);
@@ -278,6 +287,9 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
 int fgs_batch_densify_stats(int P, int n_views, const int32_t* radii, const float* dmean2D,
                             float* xyz_grad_accum, float* xyz_grad_accum_abs, float* max_radii,
                             float* denom, void* stream);
+/* fgs_bucket_tables of a batch of views. */
+int fgs_batch_bucket_tables(void* gauss_ws, void* pair_ws, int P, const fgs_batch* batch,
+                            int64_t R, int64_t NB, const fgs_options* opt, void* stream);
 int fgs_batch_frame_view_get(void* gauss_ws, void* pair_ws, int P, const fgs_batch* batch,
                              int64_t R, int64_t NB, const fgs_options* opt, fgs_frame_view* view);
 
