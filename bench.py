@@ -215,10 +215,11 @@ def kernel_units(name, P, M, V, R, T, E, fast):
         # out dSH, ddc, mean term 16
         "sh_bwd": ("hbm", P * (12 + 12 + 4 + 4 + 12 * M) + P * (12 * M + 12 + 16)),
         # split adjoint, geometric pass: in raw sums 48, opacity, radius, means,
-        # scales, rotations, clamp, SH mean term 16; out mean2D 16, conic 16,
-        # opacity 4, colour 12, mean3D 12, cov3D 24, scale 12, rotation 16
+        # scales, rotations, clamp, SH mean term 16; out mean2D 16, opacity 4,
+        # mean3D 12, scale 12, rotation 16 (the autograd path asks for no
+        # conic, colour or cov3D gradients)
         "gauss_bwd_geom": ("hbm", P * (48 + 4 + 4 + 12 + 12 + 16 + 4 + 16)
-                           + P * (16 + 16 + 4 + 12 + 12 + 24 + 12 + 16)),
+                           + P * (16 + 4 + 12 + 12 + 16)),
         # key-only pass of the tile sort: read + write 8 B per pair; the
         # last pass writes the 4 B list entry and the 4 B tile id instead
         "onesweep_k": ("hbm", 16 * R),

@@ -266,16 +266,20 @@ __global__ __launch_bounds__(GBWD_THREADS) void k_gauss_bwd(GaussBwdArgs a) {
     finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
     if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
     ((float4*)a.dmean2D)[i] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
-    ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
+    if (a.dconic)
+      ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
     a.dopacity[i] = g2.opacity;
-    a.dcolor[3 * (size_t)i] = g2.color[0];
-    a.dcolor[3 * (size_t)i + 1] = g2.color[1];
-    a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+    if (a.dcolor) {
+      a.dcolor[3 * (size_t)i] = g2.color[0];
+      a.dcolor[3 * (size_t)i + 1] = g2.color[1];
+      a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+    }
     float* row = rows + 3 * M * threadIdx.x;
     float ddc[3] = {0.0f, 0.0f, 0.0f};
+    float tcov[6];  // dcov3D not requested
     GaussGrad<float> out;
     out.dmean3d = a.dmean3D + 3 * (size_t)i;
-    out.dcov3d = a.dcov3D + 6 * (size_t)i;
+    out.dcov3d = a.dcov3D ? a.dcov3D + 6 * (size_t)i : tcov;
     out.dscale = a.dscale + 3 * (size_t)i;
     out.drot = a.drot + 4 * (size_t)i;
     out.ddc = ddc;
@@ -345,7 +349,8 @@ __global__ __launch_bounds__(GBWDV_THREADS) void k_gauss_bwd_views(GaussBwdArgs 
       finalize_raw(raw, a.G.opac[i], vr.W, vr.H, g2);
       if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
       ((float4*)a.dmean2D)[ri] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
-      ((float4*)a.dconic)[ri] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
+      if (a.dconic)
+        ((float4*)a.dconic)[ri] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
       sop += g2.opacity;
       for (int k = 0; k < 3; ++k) scol[k] += g2.color[k];
       if (a.radii[ri] > 0) {
@@ -372,12 +377,13 @@ __global__ __launch_bounds__(GBWDV_THREADS) void k_gauss_bwd_views(GaussBwdArgs 
     }
     a.dopacity[i] = sop;
     for (int k = 0; k < 3; ++k) {
-      a.dcolor[3 * (size_t)i + k] = scol[k];
+      if (a.dcolor) a.dcolor[3 * (size_t)i + k] = scol[k];
       a.dmean3D[3 * (size_t)i + k] = sm[k];
       a.dscale[3 * (size_t)i + k] = ss[k];
       a.ddc[3 * (size_t)i + k] = sdc[k];
     }
-    for (int k = 0; k < 6; ++k) a.dcov3D[6 * (size_t)i + k] = sc[k];
+    if (a.dcov3D)
+      for (int k = 0; k < 6; ++k) a.dcov3D[6 * (size_t)i + k] = sc[k];
     for (int k = 0; k < 4; ++k) a.drot[4 * (size_t)i + k] = sr[k];
   }
   __syncthreads();
@@ -498,11 +504,14 @@ __global__ __launch_bounds__(256) void k_gauss_bwd_geom(GaussBwdArgs a) {
   finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
   if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
   ((float4*)a.dmean2D)[i] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
-  ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
+  if (a.dconic)
+    ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
   a.dopacity[i] = g2.opacity;
-  a.dcolor[3 * (size_t)i] = g2.color[0];
-  a.dcolor[3 * (size_t)i + 1] = g2.color[1];
-  a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+  if (a.dcolor) {
+    a.dcolor[3 * (size_t)i] = g2.color[0];
+    a.dcolor[3 * (size_t)i + 1] = g2.color[1];
+    a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+  }
   float tm[3], tc[6], ts[3], tr[4], tdc[3];
   GaussGrad<float> out;
   out.dmean3d = tm;
@@ -525,7 +534,8 @@ __global__ __launch_bounds__(256) void k_gauss_bwd_geom(GaussBwdArgs a) {
     a.dmean3D[3 * (size_t)i + k] = tm[k];
     a.dscale[3 * (size_t)i + k] = ts[k];
   }
-  for (int k = 0; k < 6; ++k) a.dcov3D[6 * (size_t)i + k] = tc[k];
+  if (a.dcov3D)
+    for (int k = 0; k < 6; ++k) a.dcov3D[6 * (size_t)i + k] = tc[k];
   ((float4*)a.drot)[i] = make_float4(tr[0], tr[1], tr[2], tr[3]);
 }
 

@@ -165,7 +165,9 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
  * pybind.cpp:211-339). dL_dimage is 3 x H x W. Outputs are fully written
  * (no pre-zeroing needed): dmean2D P x 4, dconic P x 4, dopacity P,
  * dcolor P x 3, dmean3D P x 3, dcov3D P x 6, ddc P x 3, dsh P x max_coeffs x 3,
- * dscale P x 3, drot P x 4. grad_ws is fgs_grad_ws_size(P) bytes. */
+ * dscale P x 3, drot P x 4; dconic, dcolor and dcov3D may be NULL (not
+ * written; the autograd path of the drop-in API returns none of them unless
+ * colours are precomputed). grad_ws is fgs_grad_ws_size(P) bytes. */
 typedef struct fgs_grads {
   float *dmean2D, *dconic, *dopacity, *dcolor, *dmean3D, *dcov3D, *ddc, *dsh, *dscale, *drot;
 } fgs_grads;

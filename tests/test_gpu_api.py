@@ -4,8 +4,8 @@ import pytest
 import torch
 
 from fastergs_amd import scenes
-from fastergs_amd.rasterizer import (GaussianRasterizer, get_default_options,
-                                     set_default_options)
+from fastergs_amd.rasterizer import (Frame, GaussianRasterizer, RasterOptions,
+                                     get_default_options, set_default_options)
 from oracle.pyoracle import Oracle
 from tests.helpers import grad_close, settings_for, to_dev
 
@@ -195,3 +195,25 @@ def test_speculative_capacity_matches_exact(schedule):
         for k in ("opacities", "dc", "shs"):
             grad_close(g[k], gref[k], rtol=1e-4, name=k)  # float atomics: run-to-run order
     assert rz._capacity_hints  # the forward remembers the shape's counts
+
+
+@pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "fast"])
+def test_backward_skips_unrequested_gradients(schedule):
+    """Skipped outputs (conic, colors, cov3D) are not returned and the others
+    equal the full backward's (to float atomics' reassociation)."""
+    scene = scenes.uniform_scene(8000, seed=9)
+    cam = scenes.look_at_camera((0.0, 0.3, -4.0), 200, 160)
+    d = to_dev(scene)
+    fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+               dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule)).forward()
+    gimg = torch.from_numpy(scenes.target_image(200, 160, seed=3)).cuda()
+    full = fr.backward(gimg)
+    part = fr.backward(gimg, skip=("conic", "colors", "cov3D"))
+    assert not {"conic", "colors", "cov3D"} & set(part)
+    for k in ("mean2D", "opacities", "dc", "shs"):
+        grad_close(part[k].cpu().numpy(), full[k].cpu().numpy(), rtol=1e-5, name=k)
+    for k in ("means3D", "scales", "rotations"):  # see helpers.check_backward
+        a, b = part[k].cpu().numpy().ravel(), full[k].cpu().numpy().ravel()
+        assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b), k
+    with pytest.raises(ValueError):
+        fr.backward(gimg, skip=("means3D",))
