@@ -130,18 +130,21 @@ class BatchFrame:
         """Per-view (H, W) views of a concatenated per-pixel buffer."""
         return [flat[o:o + H * W].view(H, W) for o, (H, W) in zip(self.pix_off, self.sizes)]
 
-    def backward(self, grad_flat: torch.Tensor) -> dict:
+    def backward(self, grad_flat: torch.Tensor, skip=()) -> dict:
+        """Gradients of the batch; `skip` may name "conic", "colors" and
+        "cov3D", which are then neither computed into memory nor returned."""
         L, dev, P, V = lib(), self.device, self.P, self.V
+        if not set(skip) <= {"conic", "colors", "cov3D"}:
+            raise ValueError("only conic, colors and cov3D gradients can be skipped")
         g = _f32(grad_flat, dev).reshape(-1)
         e = lambda *s: torch.empty(s, dtype=torch.float32, device=dev)
-        out = {
-            "mean2D": e(V, P, 4), "conic": e(V, P, 2, 2), "opacities": e(P, 1),
-            "colors": e(P, 3), "means3D": e(P, 3), "cov3D": e(P, 6), "dc": e(P, 1, 3),
-            "shs": e(P, self.M, 3), "scales": e(P, 3), "rotations": e(P, 4),
-        }
-        grads = Grads(*[_ptr(out[k]) for k in ("mean2D", "conic", "opacities", "colors",
-                                                "means3D", "cov3D", "dc", "shs", "scales",
-                                                "rotations")])
+        shapes = {"mean2D": (V, P, 4), "conic": (V, P, 2, 2), "opacities": (P, 1),
+                  "colors": (P, 3), "means3D": (P, 3), "cov3D": (P, 6), "dc": (P, 1, 3),
+                  "shs": (P, self.M, 3), "scales": (P, 3), "rotations": (P, 4)}
+        out = {k: e(*sh) for k, sh in shapes.items() if k not in skip}
+        grads = Grads(*[_ptr(out.get(k)) for k in ("mean2D", "conic", "opacities", "colors",
+                                                    "means3D", "cov3D", "dc", "shs", "scales",
+                                                    "rotations")])
         gws = torch.empty(L.fgs_batch_grad_ws_size(P, V), dtype=torch.uint8, device=dev)
         check(L.fgs_batch_backward(ctypes.byref(self.batch), ctypes.byref(self.gauss),
                                    ctypes.byref(self.copts), _ptr(self.gws), _ptr(self.pws),
@@ -210,22 +213,26 @@ class _RasterizeViews(torch.autograd.Function):
         if grad_flat is None:  # the images took no part in the loss
             ctx.frame = None
             return (None,) * 9
-        g = frame.backward(grad_flat)
+        s = ctx.shapes
+        # conic and cov3D gradients are never returned, colours only when precomputed
+        skip = ["conic", "cov3D"]
+        if s[3] is None or s[3].numel() == 0:
+            skip.append("colors")
+        g = frame.backward(grad_flat, skip=skip)
         accum, accum_abs, max_radii, denom = ctx.stats
         if accum is not None and accum.numel() != 0:
             frame.densify_stats(g["mean2D"], accum, accum_abs, max_radii, denom)
         ctx.frame = None
 
         def shaped(t, shape):
-            if shape is None:  # the input was None
+            if shape is None or t is None:  # input None, or not computed
                 return None
             if t.numel() == shape.numel() and t.numel() > 0:
                 return t.reshape(shape)
             return t
 
-        s = ctx.shapes
         return (shaped(g["means3D"], s[0]), shaped(g["dc"], s[1]), shaped(g["shs"], s[2]),
-                shaped(g["colors"], s[3]), shaped(g["opacities"], s[4]),
+                shaped(g.get("colors"), s[3]), shaped(g["opacities"], s[4]),
                 shaped(g["scales"], s[5]), shaped(g["rotations"], s[6]), None, None)
 
 
