@@ -635,7 +635,7 @@ def multiview_step(args, views=16, steps=5):
             t.grad = None
         from fastergs_amd.rasterizer import set_default_options
         set_default_options(schedule=SCHEDULE_FAST, bucket_size=opts.bucket_size,
-                            split_tiles=opts.split_tiles)
+                            split_tiles=opts.split_tiles, tile_sched=opts.tile_sched)
         for st, t in zip(settings, targets):
             img, _ = rasterize_gaussians(params["means3D"], params["dc"], params["shs"], None,
                                          params["opacities"], params["scales"],
@@ -643,7 +643,9 @@ def multiview_step(args, views=16, steps=5):
                                          None)
             torch.nn.functional.mse_loss(img, t).backward()
 
-    tuner = Autotuner(buckets=(16, 32, 64), splits=(32, 64, 128), reps=2)
+    # bucket size (semantic, opted in), split threshold and compositing work
+    # shape (whole 16x16 tiles or 16x8 bands)
+    tuner = Autotuner(buckets=(16, 32, 64), splits=(32, 64, 128), work_splits=(None, 1), reps=2)
     best, table = tuner.tune(batched)
     times = {}
     for name, fn in (("batched", batched), ("sequential", sequential)):
@@ -669,7 +671,9 @@ def multiview_step(args, views=16, steps=5):
             "ms_per_step_batched_graph": round(times["captured"], 3),
             "views_per_s": round(views * 1000.0 / times["batched"], 1),
             "speedup_vs_sequential": round(times["sequential"] / times["batched"], 3),
-            "autotuned": {"bucket_size": best.bucket_size, "split_tiles": best.split_tiles},
+            "autotuned": {"bucket_size": best.bucket_size, "split_tiles": best.split_tiles,
+                          "work_shape": "16x16 tiles" if best.tile_sched is None
+                          else f"16x8 bands from {best.tile_sched.split_len} entries"},
             "autotune_table_ms": table}
 
 

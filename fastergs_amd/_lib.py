@@ -97,6 +97,7 @@ SIGNATURES = {
                                     ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.POINTER(Grads),
                                     vp]),
     "fgs_densify_stats": (ctypes.c_int, [ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]),
+    "fgs_rcp_selftest": (ctypes.c_int, [vp, vp]),
     "fgs_frame_view_get": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int64, ctypes.c_int64,
                                           ctypes.POINTER(Options), ctypes.POINTER(FrameView)]),
@@ -106,6 +107,7 @@ SIGNATURES = {
     "fgs_sort_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
     "fgs_sort_pairs": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int,
                                       ctypes.c_int, vp, ctypes.c_size_t, vp]),
+    "fgs_sort_check": (ctypes.c_int, [vp, ctypes.c_int64, vp]),
     "fgs_scan_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
     "fgs_scan_u32": (ctypes.c_int, [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_size_t, vp]),
     "fgs_mlp_forward": (ctypes.c_int, [vp, vp, ctypes.c_int64, vp, ctypes.c_int, vp]),

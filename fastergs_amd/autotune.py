@@ -1,12 +1,23 @@
 """Autotuner of the FAST schedule's knobs.
 
-The knobs change the work decomposition, never the rendered values:
-``bucket_size`` (splats per bucket: compositing parallelism against
-checkpoint traffic and backward restarts), ``split_tiles`` (rect size from
-which a Gaussian's tile walk is spread over a whole workgroup instead of
-being lane-packed) and the tile schedule of the compositing pass. The tile
-shape is fixed at 16x16 pixels (as in the reference's render_16x16), so it
-is not a knob here.
+Knobs that never change the rendered values or gradients (they only change
+the work decomposition):
+
+* ``split_tiles``: rect size (in tiles) from which a Gaussian's tile walk is
+  spread over row bands of whole waves instead of being lane-packed;
+* the compositing work shape: every 16x16 tile as one work item, or tiles
+  with at least ``split_len`` list entries as two 16x8 row bands (the tile
+  schedule of the compositing pass; ``split_len`` 1 splits every tile).
+
+The list tile itself stays 16x16 pixels (the reference's render_16x16,
+__init__.py:109): the per-tile lists, and with them the bucket boundaries
+at which the T < 1e-4 cut-off is tested, are defined on it.
+
+The bucket size is semantic: the cut-off is tested at bucket ends, so last
+contributors, images (by < 1e-4 of a colour) and gradients depend on it.
+By default the tuner keeps the bucket size of ``base`` (the reference's 128
+unless the caller chose another); passing ``buckets`` makes it a tuned knob
+too, for callers that accept that the result then depends on the choice.
 
 ``Autotuner.tune(step, key)`` times ``step(options)`` -- one full training
 step with those options, e.g. forward + backward of a multi-view batch --
@@ -26,25 +37,40 @@ import torch
 
 from . import _lib
 from .rasterizer import RasterOptions
+from .scheduler import TileSchedule
 
-DEFAULT_BUCKETS = (16, 32, 64)
 DEFAULT_SPLITS = (16, 32, 64, 128, 256)
+# compositing work shapes: None = whole 16x16 tiles; n = 16x8 row bands for
+# tiles with at least n list entries
+DEFAULT_WORK_SPLITS = (None, 1)
+
+
+def _sched(split_len):
+    return None if split_len is None else TileSchedule.heuristic(split_len=split_len)
 
 
 def _describe(o: RasterOptions) -> dict:
-    d = {"bucket_size": o.bucket_size, "split_tiles": o.split_tiles}
     ts = o.tile_sched
-    d["tiles"] = "raster" if ts is None else {0: "raster", 1: f"heuristic({ts.split_len})",
-                                              2: "learned"}[ts.mode]
-    return d
+    return {"bucket_size": o.bucket_size, "split_tiles": o.split_tiles,
+            "band_split_len": None if ts is None else int(ts.split_len)}
+
+
+def _from_description(c: dict) -> RasterOptions:
+    return RasterOptions(schedule=_lib.SCHEDULE_FAST, bucket_size=int(c["bucket_size"]),
+                         split_tiles=int(c["split_tiles"]),
+                         tile_sched=_sched(c.get("band_split_len")))
 
 
 class Autotuner:
-    def __init__(self, buckets=DEFAULT_BUCKETS, splits=DEFAULT_SPLITS, tile_scheds=(None,),
-                 reps: int = 3, cache_path: str | None = None):
+    def __init__(self, splits=DEFAULT_SPLITS, work_splits=DEFAULT_WORK_SPLITS, buckets=None,
+                 base: RasterOptions | None = None, reps: int = 3,
+                 cache_path: str | None = None):
+        base = base or RasterOptions()
+        bucket_list = tuple(buckets) if buckets else (base.bucket_size,)
         self.candidates = [RasterOptions(schedule=_lib.SCHEDULE_FAST, bucket_size=b,
-                                         split_tiles=s, tile_sched=t)
-                           for b, s, t in itertools.product(buckets, splits, tile_scheds)]
+                                         split_tiles=s, tile_sched=_sched(w),
+                                         speculative=base.speculative)
+                           for b, s, w in itertools.product(bucket_list, splits, work_splits)]
         self.reps = reps
         self.cache_path = cache_path
         self.cache: dict = {}
@@ -66,24 +92,24 @@ class Autotuner:
         return statistics.median(times)
 
     def tune(self, step, key: str | None = None):
-        """Returns (best RasterOptions, [(description, ms), ...] sorted)."""
+        """Returns (best RasterOptions, [(description, ms), ...] sorted). A
+        cached choice is returned with every knob it was tuned with."""
         if key is not None and key in self.cache:
             c = self.cache[key]
-            best = RasterOptions(schedule=_lib.SCHEDULE_FAST, bucket_size=c["bucket_size"],
-                                 split_tiles=c["split_tiles"])
-            return best, c.get("table", [])
+            return _from_description(c), c.get("table", [])
         table = []
         for o in self.candidates:
             try:
                 table.append((o, self._time(step, o)))
             except torch.cuda.OutOfMemoryError:  # this decomposition does not fit: skip it
                 torch.cuda.empty_cache()
+        if not table:
+            raise RuntimeError("no candidate fitted in memory")
         table.sort(key=lambda t: t[1])
         best = table[0][0]
         rows = [(_describe(o), round(ms, 4)) for o, ms in table]
         if key is not None:
             self.cache[key] = dict(_describe(best), table=rows)
-            self.cache[key].pop("tiles", None)
             if self.cache_path:
                 with open(self.cache_path, "w") as f:
                     json.dump(self.cache, f, indent=1)

@@ -341,8 +341,10 @@ size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
   return carve_pair(nullptr, R, NB, (int64_t)width * height, tiles_of(width, height), ck).total;
 }
 
-// raw sums (P x 12) and the split SH pass's mean term (P x 4)
-size_t fgs_grad_ws_size(int P) { return align_up((size_t)(P > 0 ? P : 1) * (RAW_STRIDE + 4) * 4); }
+// raw sums (P x 12 doubles) and the split SH pass's mean term (P x 4 floats)
+size_t fgs_grad_ws_size(int P) {
+  return align_up((size_t)(P > 0 ? P : 1) * (RAW_STRIDE * sizeof(double) + 4 * sizeof(float)));
+}
 
 size_t fgs_batch_gauss_ws_size(int P, const fgs_batch* batch, const fgs_options* opt) {
   (void)opt;
@@ -362,7 +364,7 @@ size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t 
 
 size_t fgs_batch_grad_ws_size(int P, int n_views) {
   const size_t p = (size_t)(P > 0 ? P : 1), v = (size_t)(n_views > 0 ? n_views : 1);
-  return align_up(p * v * RAW_STRIDE * 4 + (v == 1 ? p * 4 * 4 : 0));
+  return align_up(p * v * RAW_STRIDE * sizeof(double) + (v == 1 ? p * 4 * sizeof(float) : 0));
 }
 
 size_t fgs_sort_ws_size(int64_t n) { return sort_ws_bytes(n); }
@@ -373,6 +375,21 @@ int fgs_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_d
   if (n < 0 || (n > 0 && (!in || !out))) return fail(FGS_ERR_ARG, "bad scan arguments");
   if (ws_bytes < scan_ws_bytes(n)) return fail(FGS_ERR_CAPACITY, "scan workspace too small");
   FGS_TRY(scan_u32(in, out, n, total_dev, ws, (hipStream_t)stream), "scan");
+  return FGS_OK;
+}
+
+int fgs_sort_check(const void* ws, int64_t n, void* stream) {
+  if (!ws || n < 0) return fail(FGS_ERR_ARG, "bad sort check arguments");
+  if (n <= 1 || n > ONESWEEP_MAX_KEYS) return FGS_OK;  // three-launch passes never wait
+  // the fault word sits after the histograms, before the per-tile rows
+  const SortWs w = carve_sort(const_cast<void*>(ws), 1);
+  uint32_t fault = 0;
+  hipStream_t s = (hipStream_t)stream;
+  FGS_TRY(hipMemcpyAsync(&fault, w.counters + SORT_MAX_PASSES, sizeof(fault),
+                         hipMemcpyDeviceToHost, s),
+          "read sort fault");
+  FGS_TRY(hipStreamSynchronize(s), "sort check sync");
+  if (fault) return fail(FGS_ERR_HIP, "a look-back wait of the sort exceeded its bound");
   return FGS_OK;
 }
 
@@ -490,6 +507,8 @@ int fgs_batch_read_counts(const void* gauss_ws, int P, const fgs_batch* batch,
   FGS_TRY(hipStreamSynchronize(s), "read counts sync");
   if (opt && opt->schedule == FGS_SCHEDULE_FAST && c[DSORT_FAULT])
     return fail(FGS_ERR_HIP, "a look-back wait of the preprocess exceeded its bound");
+  if (c[COUNT_LIST_FAULT])
+    return fail(FGS_ERR_HIP, "a look-back wait of the list sort exceeded its bound");
   counts_host[0] = c[0];
   counts_host[1] = c[1];
   return FGS_OK;
@@ -517,6 +536,7 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   const int W = cam->width, H = cam->height;
   const int gx = grid_x_of(W);
   CompArgs c;
+  c.counts = gw.counts;
   c.W = W;
   c.H = H;
   c.grid_x = gx;
@@ -563,11 +583,13 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
       bool alt = false;
       if (pw.tile16)
         FGS_TRY(sort_kv<uint16_t>((uint16_t*)pw.t0, pw.r0, (uint16_t*)pw.t1, pw.r1, R, 0, tb,
-                                  pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES),
+                                  pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES, nullptr,
+                                  gw.counts + COUNT_LIST_FAULT),
                 "tile sort");
       else
         FGS_TRY(sort_kv<uint32_t>((uint32_t*)pw.t0, pw.r0, (uint32_t*)pw.t1, pw.r1, R, 0, tb,
-                                  pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES),
+                                  pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES, nullptr,
+                                  gw.counts + COUNT_LIST_FAULT),
                 "tile sort");
       FGS_TRY(launch_tile_ranges(R, gw.counts, alt ? pw.t1 : pw.t0, pw.tile16, gw.ranges, T,
                                  opt->bucket_size, gw.bcount, gw.maxc, gw.boff,
@@ -610,7 +632,8 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
           "duplicate");
   bool alt = false;
   FGS_TRY(sort_kv<uint64_t>(pw.k0, pw.v0, pw.k1, pw.v1, R, 0, 32 + tile_bits(T), pw.scratch, s,
-                            alt, gw.counts, nullptr, SORT_PAIRS),
+                            alt, gw.counts, nullptr, SORT_PAIRS, nullptr,
+                            gw.counts + COUNT_LIST_FAULT),
           "sort");
   if (R > 0 && alt != plain_sorted_in_alt(T)) return fail(FGS_ERR_HIP, "sort parity");
   const uint64_t* keys = alt ? pw.k1 : pw.k0;
@@ -640,11 +663,11 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
   hipStream_t s = (hipStream_t)stream;
-  float* raw = (float*)grad_ws;
-  // the raw sums are cleared beside the active-bucket scan
+  double* raw = (double*)grad_ws;
+  // the raw sums are cleared beside the active-bucket scan (0.0 is all-zero bits)
   const Fork f(s);
   FGS_TRY(f.fork(0, 6), "fork");
-  FGS_TRY(fill_f32(raw, 0.0f, PV * RAW_STRIDE, f.side(0)), "zero raw");
+  FGS_TRY(fill_f32((float*)raw, 0.0f, PV * RAW_STRIDE * 2, f.side(0)), "zero raw");
   BwdArgs bw;
   bw.W = cam->width;
   bw.H = cam->height;
@@ -706,8 +729,14 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   q.drot = out->drot;
   q.views = fast ? (const ViewRec*)gw.views : nullptr;
   q.V = b.V;
-  q.sh_dm = b.V == 1 ? raw + (size_t)g->P * RAW_STRIDE : nullptr;
+  q.sh_dm = b.V == 1 ? (float*)(raw + (size_t)g->P * RAW_STRIDE) : nullptr;
   FGS_TRY(launch_gauss_bwd(q, s), "preprocess backward");
+  return FGS_OK;
+}
+
+int fgs_rcp_selftest(unsigned long long* mismatches, void* stream) {
+  if (!mismatches) return fail(FGS_ERR_ARG, "mismatches is NULL");
+  FGS_TRY(launch_rcp_selftest(mismatches, (hipStream_t)stream), "rcp selftest");
   return FGS_OK;
 }
 
@@ -783,6 +812,8 @@ int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fg
   FGS_TRY(hipStreamSynchronize(s), "read counts sync");
   if (opt && opt->schedule == FGS_SCHEDULE_FAST && c[DSORT_FAULT])
     return fail(FGS_ERR_HIP, "a look-back wait of the preprocess exceeded its bound");
+  if (c[COUNT_LIST_FAULT])
+    return fail(FGS_ERR_HIP, "a look-back wait of the list sort exceeded its bound");
   counts_host[0] = c[0];
   counts_host[1] = c[1];
   return FGS_OK;

@@ -3,12 +3,23 @@
 //
 // Per-pixel reverse step: FGS/csrc/cuda_rasterizer/backward.cu:549-632.
 // Per-Gaussian reduction then atomics: backward.cu:634-693 (the reference
-// reduces 11 values over a 32-lane warp; here DPP reductions over wave64).
+// reduces 11 values over a 32-lane warp and adds them with float atomics, in
+// an order that varies run to run; here a wave64 reduction whose float part
+// has a fixed tree and whose cross-quad part and atomics are fp64).
 // Preprocess adjoint: backward.cu:250-539. Statistics: backward.cu:892-918.
 //
-// Raw per-Gaussian sums land in a P x 12 float buffer (fields in
-// fgs_math.h RawField order); the per-Gaussian kernel applies the constant
-// factors and produces the reference's gradient tensors.
+// Raw per-Gaussian sums land in a P x 12 double buffer (fields in
+// fgs_math.h RawField order); the per-Gaussian kernels round them to float,
+// apply the constant factors and produce the reference's gradient tensors.
+//
+// Determinism: every raw sum is sum_j (float) q_j over the lane quads j that
+// the splat's buckets cover, where q_j = (l0 + l1) + (l2 + l3) in float over
+// the quad's four lanes and each lane value is the fused float accumulation
+// of its pixels in a fixed order. The q_j are added in fp64 (tree inside the
+// wave, atomics across waves), whose reassociation error (~1e-16 relative)
+// is far below the float rounding of the result, so the sums -- and every
+// gradient computed from them -- are the same from run to run and equal to
+// the CPU oracle's, which forms the same q_j and adds them in double.
 #include <hip/hip_runtime.h>
 
 #include "fgs_device.h"
@@ -18,28 +29,46 @@
 
 namespace fgs {
 
-// v_rcp_f32 (1 ulp) for 1 / (1 - alpha) in the reverse walk: alpha <= 0.99,
-// so the argument is in [0.01, 1] and never denormal.
-struct FastRcp {
-  __device__ float operator()(float x) const { return __builtin_amdgcn_rcpf(x); }
+// 1 / (1 - alpha) of the reverse walk, correctly rounded: v_rcp_f32 (1 ulp)
+// refined by one Newton step with fused multiply-adds. For alpha <= 0.99 the
+// argument lies in [0.01, 1]; over every float in [2^-7, 1] this equals the
+// IEEE quotient 1.0f / x (checked exhaustively on the device by
+// fgs_rcp_selftest), so the walk recovers the same T as the oracle's
+// division at 3 instructions instead of the ~10 of a general IEEE division.
+struct ExactRcp {
+  __device__ float operator()(float x) const {
+    const float y = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, y, 1.0f);
+    return __builtin_fmaf(e, y, y);
+  }
 };
 
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
 
 // Reduce NF (<= 16) raw fields over the wave and add them to raw[12*id + f]
-// with one atomic instruction. Transposing butterfly: at each step a lane
-// keeps one field of every pair and hands the other to a partner lane that
-// differs in the step's lane bit (xor 1, xor 2 by quad_perm; bit 2 and bit 3
-// by row rotations of 4 and 8, whose partners hold the same fields), so the
-// number of fields per lane halves while the lanes they cover double; after
-// four steps lane l holds field l % 16 summed over its row, and two shuffles
-// add the four rows. About 40 instructions for 9 fields instead of six DPP
-// steps per field.
+// with one fp64 atomic instruction. Transposing butterfly: at each step a
+// lane keeps one field of every pair and hands the other to a partner lane
+// that differs in the step's lane bit (xor 1, xor 2 by quad_perm; bit 2 and
+// bit 3 by row rotations of 4 and 8, whose partners hold the same fields), so
+// the number of fields per lane halves while the lanes they cover double;
+// after four steps lane l holds field l % 16 summed over its row, and two
+// shuffles add the four rows. The first two steps are in float: for every
+// field they form (l0 + l1) + (l2 + l3) over the lanes 4j .. 4j+3 (float
+// addition commutes, so which lane adds is immaterial); the remaining steps
+// convert those quad sums to double.
 template <int NF>
-__device__ __forceinline__ void wave_flush(const float* acc, float* raw, uint32_t id) {
+__device__ __forceinline__ void wave_flush(const float* acc, double* raw, uint32_t id) {
   static_assert(NF <= 16, "at most 16 fields");
   const int l = lane_id();
   const bool b0 = l & 1, b1 = (l >> 1) & 1, b2 = (l >> 2) & 1, b3 = (l >> 3) & 1;
-  float s1[8], s2[4], s3[2];
+  float s1[8];
+  double s2[4], s3[2];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const float a = 2 * m < NF ? acc[2 * m] : 0.0f;
@@ -48,18 +77,49 @@ __device__ __forceinline__ void wave_flush(const float* acc, float* raw, uint32_
   }
 #pragma unroll
   for (int n = 0; n < 4; ++n)
-    s2[n] = 4 * n < NF ? (b1 ? s1[2 * n + 1] : s1[2 * n]) +
-                             dpp_f<0x4e>(b1 ? s1[2 * n] : s1[2 * n + 1])  // quad_perm [2,3,0,1]
-                       : 0.0f;
+    s2[n] = 4 * n < NF ? (double)((b1 ? s1[2 * n + 1] : s1[2 * n]) +
+                                  dpp_f<0x4e>(b1 ? s1[2 * n] : s1[2 * n + 1]))  // quad_perm [2,3,0,1]
+                       : 0.0;
 #pragma unroll
   for (int p = 0; p < 2; ++p)
     s3[p] = 8 * p < NF ? (b2 ? s2[2 * p + 1] : s2[2 * p]) +
-                             dpp_f<0x124>(b2 ? s2[2 * p] : s2[2 * p + 1])  // row_ror:4
-                       : 0.0f;
-  float v = (b3 ? s3[1] : s3[0]) + dpp_f<0x128>(b3 ? s3[0] : s3[1]);  // row_ror:8
+                             dpp_d<0x124>(b2 ? s2[2 * p] : s2[2 * p + 1])  // row_ror:4
+                       : 0.0;
+  double v = (b3 ? s3[1] : s3[0]) + dpp_d<0x128>(b3 ? s3[0] : s3[1]);  // row_ror:8
   v += __shfl_xor(v, 16, WAVE);
   v += __shfl_xor(v, 32, WAVE);
-  if (l < NF) atomic_add_f(raw + (size_t)RAW_STRIDE * id + l, v);
+  if (l < NF) unsafeAtomicAdd(raw + (size_t)RAW_STRIDE * id + l, v);
+}
+
+// Exhaustive check of ExactRcp against the IEEE quotient over [2^-7, 1].
+__global__ __launch_bounds__(256) void k_rcp_selftest(uint32_t lo, uint32_t n,
+                                                      unsigned long long* bad) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float x = __uint_as_float(lo + i);
+  const bool diff = __float_as_uint(ExactRcp()(x)) != __float_as_uint(1.0f / x);
+  const uint64_t m = __ballot(diff);
+  if (m && lane_id() == __builtin_ctzll(m)) atomicAdd(bad, (unsigned long long)__popcll(m));
+}
+
+hipError_t launch_rcp_selftest(unsigned long long* bad, hipStream_t s) {
+  const uint32_t lo = 0x3C000000u, hi = 0x3F800000u;  // 2^-7 .. 1.0
+  const uint32_t n = hi - lo + 1;
+  hipError_t e = fill_u32((uint32_t*)bad, 0u, 2, s);
+  if (e != hipSuccess) return e;
+  k_rcp_selftest<<<(n + 255) / 256, 256, 0, s>>>(lo, n, bad);
+  return hipGetLastError();
+}
+
+// Raw sums of one record, rounded to float.
+__device__ __forceinline__ void load_raw(const double* p, float* raw) {
+  const double2* r2 = (const double2*)p;
+#pragma unroll
+  for (int k = 0; k < RAW_STRIDE / 2; ++k) {
+    const double2 q = r2[k];
+    raw[2 * k] = (float)q.x;
+    raw[2 * k + 1] = (float)q.y;
+  }
 }
 
 // ---------------------------------------------------------- PLAIN backward
@@ -113,8 +173,8 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
       for (int f = 0; f < RAW_N; ++f) acc[f] = 0.0f;
       if (act) {
         const float4 A = sA[j], Bv = sB[j];
-        pix_bwd(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, sC[j], fx, fy, dp0, dp1, dp2, T, Q,
-                acc);
+        pix_bwd<float, ExactRcp, ABS>(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, sC[j], fx, fy,
+                                      dp0, dp1, dp2, T, Q, acc);
       }
       wave_flush<NF>(acc, a.raw, sId[j]);
     }
@@ -217,7 +277,7 @@ __global__ __launch_bounds__(256, 6) void k_backward_fast(BwdArgs a) {
       for (int f = 0; f < RAW_N; ++f) acc[f] = -0.0f;
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        pix_bwd<float, FastRcp, ABS>(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q],
+        pix_bwd<float, ExactRcp, ABS>(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q],
                                      fy[q], d0[q], d1[q], d2[q], T[q], Q[q], acc);
       wave_flush<NF>(acc, a.raw, id);
       id = idn;
@@ -256,12 +316,8 @@ __global__ __launch_bounds__(GBWD_THREADS) void k_gauss_bwd(GaussBwdArgs a) {
       rows[k] = a.G.shs[(size_t)3 * M * g0 + k];
   __syncthreads();
   if (i < a.P) {
-    const float4* r4 = (const float4*)(a.raw + (size_t)RAW_STRIDE * i);
     float raw[RAW_STRIDE];
-    const float4 q0 = r4[0], q1 = r4[1], q2 = r4[2];
-    raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
-    raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
-    raw[8] = q2.x; raw[9] = q2.y; raw[10] = q2.z; raw[11] = q2.w;
+    load_raw(a.raw + (size_t)RAW_STRIDE * i, raw);
     Grad2D<float> g2;
     finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
     if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
@@ -339,12 +395,8 @@ __global__ __launch_bounds__(GBWDV_THREADS) void k_gauss_bwd_views(GaussBwdArgs 
     for (int v = 0; v < a.V; ++v) {
       const ViewRec& vr = a.views[v];
       const size_t ri = (size_t)v * a.P + i;
-      const float4* r4 = (const float4*)(a.raw + (size_t)RAW_STRIDE * ri);
       float raw[RAW_STRIDE];
-      const float4 q0 = r4[0], q1 = r4[1], q2 = r4[2];
-      raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
-      raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
-      raw[8] = q2.x; raw[9] = q2.y; raw[10] = q2.z; raw[11] = q2.w;
+      load_raw(a.raw + (size_t)RAW_STRIDE * ri, raw);
       Grad2D<float> g2;
       finalize_raw(raw, a.G.opac[i], vr.W, vr.H, g2);
       if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
@@ -423,8 +475,9 @@ __global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
   const bool vis = in && a.radii[ic] > 0;
   const float* pm = a.G.means + 3 * ic;
   const float px = pm[0], py = pm[1], pz = pm[2];
-  const float col[3] = {a.raw[RAW_STRIDE * ic + RAW_COL0], a.raw[RAW_STRIDE * ic + RAW_COL1],
-                        a.raw[RAW_STRIDE * ic + RAW_COL2]};
+  const float col[3] = {(float)a.raw[RAW_STRIDE * ic + RAW_COL0],
+                        (float)a.raw[RAW_STRIDE * ic + RAW_COL1],
+                        (float)a.raw[RAW_STRIDE * ic + RAW_COL2]};
   const int clampbits = (int)f2u(a.splat[12 * ic + 11]);
   // full groups of 64 rows start 16-byte aligned when the arrays do
   const bool vec = M > 0 && M <= 15 && nb == SHB_THREADS && RM * SHB_THREADS % 4 == 0 &&
@@ -494,12 +547,8 @@ __global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
 __global__ __launch_bounds__(256) void k_gauss_bwd_geom(GaussBwdArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= a.P) return;
-  const float4* r4 = (const float4*)(a.raw + (size_t)RAW_STRIDE * i);
   float raw[RAW_STRIDE];
-  const float4 q0 = r4[0], q1 = r4[1], q2 = r4[2];
-  raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
-  raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
-  raw[8] = q2.x; raw[9] = q2.y; raw[10] = q2.z; raw[11] = q2.w;
+  load_raw(a.raw + (size_t)RAW_STRIDE * i, raw);
   Grad2D<float> g2;
   finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
   if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;

@@ -107,13 +107,17 @@ hipError_t expand_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t
 // where the keys ended up. last_vals: when given, the final pass writes the
 // values there. n_dev: when given, the element count is min(*n_dev, n) read
 // on the device (n is then the capacity the grids are sized for); no host
-// synchronisation either way.
+// synchronisation either way. The onesweep passes bound their look-back
+// waits (a wait cannot exceed the bound when tiles are claimed in launch
+// order); a wait that does sets a fault word to 1: fault_word when given
+// (the caller clears it), else the last word of the sort's own state
+// (sort_fault_word).
 enum SortKind { SORT_PAIRS = 0, SORT_DEPTH = 1, SORT_TILES = 2 };
 template <typename K>
 hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
                    int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
                    const uint32_t* n_dev, uint32_t* last_vals, SortKind kind,
-                   uint32_t* sync = nullptr);
+                   uint32_t* sync = nullptr, uint32_t* fault_word = nullptr);
 // fill.hip-style helpers (render.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 hipError_t fill_f32(float* p, float v, int64_t n, hipStream_t s);
@@ -179,6 +183,10 @@ hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_
                                 hipStream_t s);
 struct CompArgs {
   int W, H, grid_x, T, bucket;
+  // the frame's counts words: when a sort's look-back bound was exceeded
+  // (counts[COUNT_LIST_FAULT] or counts[DSORT_FAULT] set) the lists are not
+  // valid, and images and T_final are written NaN
+  const uint32_t* counts;
   const uint32_t* ranges;
   const uint32_t* vals;
   const float* splat;
@@ -285,11 +293,14 @@ struct BwdArgs {
   const float* acc;
   const uint32_t* actoff;  // fast: exclusive prefix of active buckets per tile, [T] = total
   const float* dimg;
-  float* raw;
+  double* raw;  // P x RAW_STRIDE (V x P x RAW_STRIDE) raw sums, accumulated in fp64
   const ViewRec* views;  // fast: V views
   int V;
 };
 hipError_t launch_backward_plain(const BwdArgs& a, hipStream_t s);
+// Counts the floats in [2^-7, 1] whose backward reciprocal is not the IEEE
+// quotient (bad: 1 counter, cleared by the launch).
+hipError_t launch_rcp_selftest(unsigned long long* bad, hipStream_t s);
 hipError_t launch_backward_fast(const BwdArgs& a, hipStream_t s);
 // Per-Gaussian adjoint. With views (V >= 1 of them, records v * P + i):
 // dmean2D / dconic are per view (V x P rows), every other gradient is the
@@ -298,7 +309,7 @@ struct GaussBwdArgs {
   GaussRef<float> G;
   CamRef<float> cam;
   int P;
-  const float* raw;
+  const double* raw;
   const int32_t* radii;
   const float* splat;
   float *dmean2D, *dconic, *dopacity, *dcolor, *dmean3D, *dcov3D, *ddc, *dsh, *dscale, *drot;

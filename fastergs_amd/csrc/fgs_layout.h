@@ -103,6 +103,9 @@ constexpr int LARGE_BAND_TILES = 512;
 constexpr int GAUSS_SYNC_WORDS = 16 + SORT_SYNC_WORDS;
 // counts words the last-workgroup launches count arrivals in
 constexpr int COUNT_ARRIVE_RANGES = 6;
+// counts word set when a look-back wait of the render's list sort (tile sort
+// or PLAIN pair sort) exceeded its bound
+constexpr int COUNT_LIST_FAULT = 5;
 // The depth sort's fault flag (last word of its state, counted from
 // counts), set when a look-back wait exceeds its bound.
 constexpr int DSORT_FAULT = 16 + SORT_SYNC_WORDS - 1;
@@ -127,7 +130,8 @@ struct GaussWs {
   uint32_t* maxc;      // T
   uint32_t* rcount;    // fast: pair counts in depth order
   uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects, [4] work items,
-                       // [6] arrival counter (COUNT_ARRIVE_RANGES)
+                       // [5] list-sort fault (COUNT_LIST_FAULT), [6] arrival counter
+                       // (COUNT_ARRIVE_RANGES)
   uint32_t* dsort_sync;  // fast: depth sort state (SORT_SYNC_WORDS), right after counts
   float* feat;         // T x 8 tile features (scheduled compositing)
   float* pred;         // T x 2 cost-model outputs (predicted log2 cost, split logit)

@@ -577,9 +577,8 @@ constexpr int RAW_STRIDE = 12;
 // enters dL/dalpha only through its dot product with dL/dpixel, so the state
 // is the scalar Q = residual . dL/dpixel + T_final (bg . dL/dpixel):
 //   dL/dalpha = T (colour . dL/dpixel) - Q / (1 - alpha),  Q += w (colour . dL/dpixel).
-// Reciprocal of the per-pixel backward: IEEE division by default; the GPU
-// backward passes a one-instruction approximate reciprocal (the result only
-// enters gradient sums, which are compared within a tolerance).
+// Reciprocal of the per-pixel backward: IEEE division (the GPU backward
+// passes a reciprocal that equals it for every argument the walk can see).
 struct IeeeRcp {
   template <typename R>
   FGS_HD R operator()(R x) const { return R(1) / x; }

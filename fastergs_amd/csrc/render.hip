@@ -160,6 +160,13 @@ hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_
 }
 
 
+// True when a sort of this frame gave up a look-back wait (the lists are
+// then not valid; compositing writes NaN so that the frame cannot pass as an
+// image, and fgs_read_counts reports the error).
+__device__ __forceinline__ bool lists_faulted(const uint32_t* counts) {
+  return counts && (counts[COUNT_LIST_FAULT] | counts[DSORT_FAULT]) != 0u;
+}
+
 // ------------------------------------------------------ PLAIN compositing
 __global__ __launch_bounds__(256) void k_composite_plain(CompArgs a) {
   __shared__ float4 sA[256];
@@ -205,6 +212,7 @@ __global__ __launch_bounds__(256) void k_composite_plain(CompArgs a) {
       }
     }
   }
+  if (lists_faulted(a.counts)) T = C0 = __builtin_nanf("");
   if (inside) {
     const size_t HW = (size_t)a.W * a.H;
     const size_t g = (size_t)py * a.W + px;
@@ -364,6 +372,7 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
       }
     }
   }
+  if (lists_faulted(a.counts)) T = C0 = __builtin_nanf("");
   if (inside) {
     const size_t HW = (size_t)W * H;
     const size_t g = (size_t)vr.pix_off + (size_t)py * W + px;

@@ -671,7 +671,7 @@ constexpr int LOOKBACK = 8;  // status words read per look-back step (in flight 
 __global__ __launch_bounds__(256) void k_sort_prepare(uint32_t* __restrict__ hist,
                                                       uint32_t* __restrict__ counters) {
   for (int i = threadIdx.x; i < SORT_MAX_PASSES * 256; i += 256) hist[i] = 0u;
-  if (threadIdx.x < SORT_MAX_PASSES) counters[threadIdx.x] = 0u;
+  if (threadIdx.x <= SORT_MAX_PASSES) counters[threadIdx.x] = 0u;  // tile counters + fault flag
 }
 
 // Histograms of all passes in one read (per_block sort tiles per workgroup);
@@ -810,7 +810,7 @@ template <typename K>
 static hipError_t sort_kv_onesweep(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
                                    int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t s,
                                    bool& in_alt, const uint32_t* n_dev, uint32_t* last_vals,
-                                   const SortNames& nm, uint32_t* sync) {
+                                   const SortNames& nm, uint32_t* sync, uint32_t* fault_word) {
   const int64_t nb = (n + SortTiling<K>::TILE - 1) / SortTiling<K>::TILE;
   SortWs w = carve_sort(ws, nb);
   const int npass = sort_passes(begin_bit, end_bit);
@@ -839,7 +839,7 @@ static hipError_t sort_kv_onesweep(K* keys, uint32_t* vals, K* keys_alt, uint32_
     uint32_t* st_next = last ? nullptr : w.status[(p + 1) & 1];
     const uint32_t* h = w.hist + p * 256;
     uint32_t* ctr = w.counters + p;
-    uint32_t* fault = w.counters + SORT_MAX_PASSES;
+    uint32_t* fault = fault_word ? fault_word : w.counters + SORT_MAX_PASSES;
     if (vals)
       FGS_WITH_BITS(bits, FGS_TIMED(nm.onesweep, s,
                                     (k_onesweep<K, true, B_><<<(unsigned)nb, SORT_THREADS, 0, s>>>(
@@ -862,7 +862,7 @@ template <typename K>
 hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
                    int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
                    const uint32_t* n_dev, uint32_t* last_vals, SortKind kind,
-                   uint32_t* sync) {
+                   uint32_t* sync, uint32_t* fault_word) {
   in_alt = false;
   // with a device-side count the passes always run (the count may be 0 or 1)
   if ((n <= 1 && !n_dev) || n <= 0 || end_bit <= begin_bit) return hipSuccess;
@@ -872,19 +872,19 @@ hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int
   const SortNames& nm = k_sort_names[kind];
   if (n <= ONESWEEP_MAX_KEYS)
     return sort_kv_onesweep<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s,
-                               in_alt, n_dev, last_vals, nm, sync);
+                               in_alt, n_dev, last_vals, nm, sync, fault_word);
   return sort_kv_classic<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, in_alt,
                             n_dev, last_vals, nm);
 }
 
 template hipError_t sort_kv<uint16_t>(uint16_t*, uint32_t*, uint16_t*, uint32_t*, int64_t, int,
                                       int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
-                                      SortKind, uint32_t*);
+                                      SortKind, uint32_t*, uint32_t*);
 template hipError_t sort_kv<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, int64_t, int,
                                       int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
-                                      SortKind, uint32_t*);
+                                      SortKind, uint32_t*, uint32_t*);
 template hipError_t sort_kv<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, int64_t, int,
                                       int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,
-                                      SortKind, uint32_t*);
+                                      SortKind, uint32_t*, uint32_t*);
 
 }  // namespace fgs

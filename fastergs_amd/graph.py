@@ -43,6 +43,19 @@ class CapacityError(RuntimeError):
     """A replayed frame had more pairs or buckets than the captured capacity."""
 
 
+def _in_place(name: str, t: torch.Tensor, dev, per: int) -> torch.Tensor:
+    """A Gaussian tensor that every replay reads in place: it must already be
+    contiguous float32 on the capture's device with `per` values per
+    Gaussian (a converted copy would not see the caller's later updates)."""
+    if not torch.is_tensor(t) or t.dtype != torch.float32 or not t.is_contiguous() or \
+            t.device != dev:
+        raise ValueError(f"{name} must be a contiguous float32 tensor on {dev} "
+                         "(replays read it in place)")
+    if per and t.numel() % per:
+        raise ValueError(f"{name} has {t.numel()} values, not a multiple of {per}")
+    return t
+
+
 class CapturedRenderer:
     """Forward renderer whose whole frame is one graph replay.
 
@@ -63,18 +76,20 @@ class CapturedRenderer:
         self.W, self.H = int(settings.image_width), int(settings.image_height)
         self.T = ((self.W + 15) // 16) * ((self.H + 15) // 16)
         self.headroom = float(headroom)
-        self.means3D = _f32(means3D, dev).reshape(P, 3)
-        self.opacities = _f32(opacities, dev).reshape(P)
-        self.scales = _f32(scales, dev).reshape(P, 3)
-        self.rotations = _f32(rotations, dev).reshape(P, 4)
+        self.means3D = _in_place("means3D", means3D, dev, 3).reshape(P, 3)
+        self.opacities = _in_place("opacities", opacities, dev, 1).reshape(P)
+        self.scales = _in_place("scales", scales, dev, 3).reshape(P, 3)
+        self.rotations = _in_place("rotations", rotations, dev, 4).reshape(P, 4)
         self.colors = self.dc = self.shs = None
         M = 0
         if colors_precomp is not None and colors_precomp.numel() > 0:
-            self.colors = _f32(colors_precomp, dev).reshape(P, 3)
+            self.colors = _in_place("colors_precomp", colors_precomp, dev, 3).reshape(P, 3)
         else:
-            self.dc = _f32(dc, dev).reshape(P, 3)
+            if dc is None:
+                raise ValueError("dc is required when colours come from SH")
+            self.dc = _in_place("dc", dc, dev, 3).reshape(P, 3)
             if shs is not None and shs.numel() > 0:
-                self.shs = _f32(shs, dev).reshape(P, -1, 3)
+                self.shs = _in_place("shs", shs, dev, 3).reshape(P, -1, 3)
                 M = int(self.shs.shape[1])
         # camera buffers the graph reads: view 16 | proj 16 | campos 3
         self.cambuf = torch.zeros(35, dtype=torch.float32, device=dev)
@@ -232,19 +247,20 @@ class CapturedTrainingStep:
         self.V = V = len(settings_list)
         self.headroom = float(headroom)
         # the Gaussian tensors are read in place by every replay
-        for name, t in (("means3D", means3D), ("opacities", opacities), ("scales", scales),
-                        ("rotations", rotations)):
-            if t.dtype != torch.float32 or not t.is_contiguous() or t.device != dev:
-                raise ValueError(f"{name} must be a contiguous float32 tensor on {dev}")
+        for name, t, per in (("means3D", means3D, 3), ("opacities", opacities, 1),
+                             ("scales", scales, 3), ("rotations", rotations, 4)):
+            _in_place(name, t, dev, per)
         self.means3D, self.opacities, self.scales, self.rotations = means3D, opacities, scales, rotations
         self.dc = self.shs = self.colors = None
         self.M = 0
         if colors_precomp is not None and colors_precomp.numel() > 0:
-            self.colors = colors_precomp
+            self.colors = _in_place("colors_precomp", colors_precomp, dev, 3)
         else:
-            self.dc = dc
+            if dc is None:
+                raise ValueError("dc is required when colours come from SH")
+            self.dc = _in_place("dc", dc, dev, 3)
             if shs is not None and shs.numel() > 0:
-                self.shs = shs
+                self.shs = _in_place("shs", shs, dev, 3)
                 self.M = int(shs.reshape(P, -1, 3).shape[1])
         self.sizes = [(int(s.image_height), int(s.image_width)) for s in settings_list]
         self.pix_off = [0]

@@ -151,7 +151,7 @@ class BatchFrame:
                                    self.R, self.NB, _ptr(self.radii), _ptr(self.T_final),
                                    _ptr(self.last_contributor), _ptr(g), _ptr(gws),
                                    ctypes.byref(grads), self._s()), "fgs_batch_backward")
-        out["raw"] = gws[:V * P * 48].view(torch.float32).reshape(V, P, 12)
+        out["raw"] = gws[:V * P * 96].view(torch.float64).reshape(V, P, 12)
         return out
 
     def densify_stats(self, dmean2D, accum, accum_abs, max_radii, denom):

@@ -36,12 +36,17 @@ from ._lib import Camera, FrameView, Gaussians, Grads, Options, check, lib
 class RasterOptions:
     """Schedule knobs (not part of the reference settings tuple).
 
+    The defaults are the reference backend's semantics: the bucketed FAST
+    schedule with 128 splats per bucket (BUCKET_SIZE, rasterizer_imp.h:24),
+    so an unchanged caller gets FasterGS's images, cut-offs and gradients.
+    ``bucket_size`` is semantic: the T < 1e-4 cut-off is tested at bucket
+    ends, so last contributors and gradients depend on it.
     ``tile_sched``: work schedule of the FAST compositing pass, an object
     with a ``to_c()`` returning a ``_lib.TileSched`` (see
     ``fastergs_amd.scheduler``); None = one workgroup per tile in tile order.
     """
-    schedule: int = _lib.SCHEDULE_PLAIN
-    bucket_size: int = 32
+    schedule: int = _lib.SCHEDULE_FAST
+    bucket_size: int = 128
     split_tiles: int = 64
     tile_sched: object = None
     # render with pair / bucket capacities remembered from earlier frames of
@@ -259,7 +264,7 @@ class Frame:
                              self.NB_cap, _ptr(self.radii), _ptr(self.T_final),
                              _ptr(self.last_contributor), _ptr(g), _ptr(gws),
                              ctypes.byref(grads), ctypes.c_void_p(_stream(dev))), "fgs_backward")
-        out["raw"] = gws[:P * 48].view(torch.float32).reshape(P, 12)
+        out["raw"] = gws[:P * 96].view(torch.float64).reshape(P, 12)
         return out
 
     def densify_stats(self, dmean2D, accum, accum_abs, max_radii, denom):

@@ -136,7 +136,9 @@ typedef struct fgs_frame_view {
 size_t fgs_gauss_ws_size(int P, int width, int height, const fgs_options* opt);
 size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
                         const fgs_options* opt);
-/* Raw gradient workspace of the backward: P x 12 floats. */
+/* Gradient workspace of the backward: the raw per-Gaussian sums (P x 12
+ * doubles at its start, fields colour 3, mean2D 2, conic 3, power 1,
+ * |mean2D| 2, readable after fgs_backward) and P x 4 floats of scratch. */
 size_t fgs_grad_ws_size(int P);
 
 /* Replaces faster::preprocess (preprocess.cu:928-962, pybind.cpp:26-76):
@@ -230,11 +232,22 @@ size_t fgs_sort_ws_size(int64_t n);
 int fgs_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                    int64_t n, int begin_bit, int end_bit, void* ws, size_t ws_bytes,
                    void* stream);
+/* Status of the last fgs_sort_pairs on workspace ws with n pairs
+ * (synchronises the stream): FGS_ERR_HIP if one of its bounded look-back
+ * waits gave up (the result is then not sorted), else FGS_OK. The frame-level
+ * sorts report the same condition through fgs_read_counts, and their images
+ * are written NaN. */
+int fgs_sort_check(const void* ws, int64_t n, void* stream);
 /* Exclusive prefix sum of n uint32 (in place allowed); total written to
  * *total_dev. Replaces cub::DeviceScan (render.cu:420-422). */
 size_t fgs_scan_ws_size(int64_t n);
 int fgs_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, void* ws,
                  size_t ws_bytes, void* stream);
+
+/* Self-check of the reverse walk's reciprocal: counts (into *mismatches,
+ * device memory, overwritten) the floats x in [2^-7, 1] for which the
+ * backward's 1/x differs from the IEEE quotient; 0 on a correct device. */
+int fgs_rcp_selftest(unsigned long long* mismatches, void* stream);
 
 /* Cost-model kernels. fgs_mlp_forward: y (n x 2) from x (n x 8), bf16 != 0
  * rounds weights and activations to bfloat16 (fp32 accumulation).

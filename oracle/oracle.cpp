@@ -26,8 +26,10 @@
 // whole list, with the same T<1e-4 test at bucket boundaries, and a
 // per-pixel reverse backward starting from T_final.
 //
-// Per-Gaussian gradient sums are formed per (sorted pair) in double and
-// reduced in sorted-pair order, so results do not depend on thread count.
+// Per-Gaussian gradient sums: float partial sums over fixed groups of
+// pixels (the lane quads of the GPU walk, see backward()), added in double
+// per sorted pair and then in sorted-pair order, so results do not depend on
+// the thread count.
 #include <parallel/algorithm>
 #include <omp.h>
 
@@ -308,75 +310,129 @@ void forward(State<R>& st) {
 template <typename R>
 void gauss_grads(State<R>& st, const float* raw_f);
 
-// accum: 0 = per-pair rows and the per-Gaussian reduction in double (the
-// checker), 1 = rows and reduction in float, pairs in sorted order, 2 = the
-// same with pairs in reverse order, 3 = rows in float and the reduction in
-// double (1 and 2 bound the error of float partial sums plus float atomics,
-// 3 that of float partial sums plus double atomics).
+// Raw per-Gaussian sums of the rasterizer adjoint, formed exactly as the
+// gfx950 reverse walks form them (so GPU and oracle agree to the last bit of
+// the float-rounded sums). A wave of the walk covers a tile's 256 pixels in
+// 64 lanes; per splat each lane accumulates its pixels' terms in float, in a
+// fixed order, into one partial (fused accumulation from -0 / 0), and every
+// four lanes 4j .. 4j+3 form the float quad sum (l0 + l1) + (l2 + l3); the
+// quad sums are added in double. Lane l holds:
+//   FAST  (mode 0): the 2x2 pixels (2 (l % 8) + i, 2 (l / 8) + j), in the
+//                   order (0,0) (1,0) (0,1) (1,1), one wave per bucket;
+//   PLAIN (mode 1): pixel (l % 16, 4 w + l / 16) of wave w, one per lane.
+// The reference adds float warp sums with float atomics (backward.cu:634-693),
+// whose order varies between runs; the double addition of fixed float quad
+// sums is that computation with a deterministic result.
 template <typename R>
-void backward(State<R>& st, const R* dimg, int accum = 0) {
+void backward(State<R>& st, const R* dimg) {
   const size_t HW = (size_t)st.W * st.H;
   const size_t Rn = st.vals.size();
   std::vector<double> rows(Rn * fgs::RAW_N, 0.0);
   const R bg0 = st.bg[0], bg1 = st.bg[1], bg2 = st.bg[2];
+  constexpr int NF = fgs::RAW_N;
 #pragma omp parallel for schedule(dynamic, 1)
   for (int t = 0; t < st.T; ++t) {
     const int tx = t % st.gx, ty = t / st.gx;
     const uint32_t b0 = t == 0 ? 0 : st.bucket_off[t - 1];
     const uint32_t b1 = st.bucket_off[t];
     const uint32_t tile_start = st.ranges[2 * t];
-    for (int ly = 0; ly < fgs::TILE_H; ++ly)
-      for (int lx = 0; lx < fgs::TILE_W; ++lx) {
-        const int px = tx * fgs::TILE_W + lx, py = ty * fgs::TILE_H + ly;
-        if (px >= st.W || py >= st.H) continue;
-        const int lp = ly * fgs::TILE_W + lx;
-        const size_t g = (size_t)py * st.W + px;
-        const R fx = (R)px, fy = (R)py;
-        const R dp0 = dimg[g], dp1 = dimg[HW + g], dp2 = dimg[2 * HW + g];
-        const R Tf = st.tfinal[g];
-        const R tb = Tf * (bg0 * dp0 + bg1 * dp1 + bg2 * dp2);
-        const uint32_t cnt = (uint32_t)st.contrib[g];
-        auto step = [&](uint32_t k, R& T, R& Q) {
-          const R* sp = &st.splat[9 * (size_t)st.vals[k]];
-          R acc[fgs::RAW_N];
-          for (int f = 0; f < fgs::RAW_N; ++f) acc[f] = R(0);
-          fgs::pix_bwd(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], sp[6], sp[7], sp[8], fx, fy, dp0,
-                       dp1, dp2, T, Q, acc);
-          double* row = &rows[k * fgs::RAW_N];
-          if (accum == 0) {
-            for (int f = 0; f < fgs::RAW_N; ++f) row[f] += (double)acc[f];
-          } else {
-            for (int f = 0; f < fgs::RAW_N; ++f) row[f] = (double)((float)row[f] + (float)acc[f]);
+    // per pixel of the tile (index ly * 16 + lx)
+    struct Pix {
+      bool in = false;
+      R fx, fy, dp0, dp1, dp2, tb;
+      uint32_t cnt = 0;
+      size_t g = 0;
+    };
+    Pix pix[fgs::TILE_PIX];
+    for (int lp = 0; lp < fgs::TILE_PIX; ++lp) {
+      const int px = tx * fgs::TILE_W + lp % fgs::TILE_W, py = ty * fgs::TILE_H + lp / fgs::TILE_W;
+      Pix& q = pix[lp];
+      if (px >= st.W || py >= st.H) continue;
+      q.in = true;
+      q.g = (size_t)py * st.W + px;
+      q.fx = (R)px;
+      q.fy = (R)py;
+      q.dp0 = dimg[q.g];
+      q.dp1 = dimg[HW + q.g];
+      q.dp2 = dimg[2 * HW + q.g];
+      q.tb = st.tfinal[q.g] * (bg0 * q.dp0 + bg1 * q.dp1 + bg2 * q.dp2);
+      q.cnt = (uint32_t)st.contrib[q.g];
+    }
+    auto add_quad = [&](uint32_t k, R lane[4][NF]) {
+      double* row = &rows[k * NF];
+      for (int f = 0; f < NF; ++f)
+        row[f] += (double)((lane[0][f] + lane[1][f]) + (lane[2][f] + lane[3][f]));
+    };
+    auto step = [&](uint32_t k, const Pix& q, R& T, R& Q, R* acc) {
+      const R* sp = &st.splat[9 * (size_t)st.vals[k]];
+      fgs::pix_bwd(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], sp[6], sp[7], sp[8], q.fx, q.fy,
+                   q.dp0, q.dp1, q.dp2, T, Q, acc);
+    };
+    if (st.mode == 0) {
+      for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t s = st.bucket_ranges[2 * b], e = st.bucket_ranges[2 * b + 1];
+        for (int j = 0; j < 16; ++j) {
+          int lp[4][4];
+          bool act[4][4];
+          R T[4][4], Q[4][4];
+          bool any = false;
+          for (int i = 0; i < 4; ++i) {
+            const int l = 4 * j + i;
+            for (int m = 0; m < 4; ++m) {
+              const int lx = 2 * (l & 7) + (m & 1), ly = 2 * (l >> 3) + (m >> 1);
+              const int p = ly * fgs::TILE_W + lx;
+              lp[i][m] = p;
+              act[i][m] = pix[p].in && s - tile_start < pix[p].cnt;  // backward.cu:663-664
+              any = any || act[i][m];
+              if (!act[i][m]) continue;
+              const size_t q = (size_t)b * fgs::TILE_PIX + p;
+              const size_t g = pix[p].g;
+              T[i][m] = st.acc_t[q];
+              Q[i][m] = fgs::residual_dot(st.pixc[3 * g] - st.acc_c[3 * q],
+                                          st.pixc[3 * g + 1] - st.acc_c[3 * q + 1],
+                                          st.pixc[3 * g + 2] - st.acc_c[3 * q + 2], pix[p].dp0,
+                                          pix[p].dp1, pix[p].dp2, pix[p].tb);
+            }
           }
-        };
-        if (st.mode == 0) {
-          for (uint32_t b = b0; b < b1; ++b) {
-            const uint32_t s = st.bucket_ranges[2 * b], e = st.bucket_ranges[2 * b + 1];
-            if (s - tile_start >= cnt) break;  // backward.cu:663-664
-            const size_t q = (size_t)b * fgs::TILE_PIX + lp;
-            R T = st.acc_t[q];
-            const R r0 = st.pixc[3 * g] - st.acc_c[3 * q];
-            const R r1 = st.pixc[3 * g + 1] - st.acc_c[3 * q + 1];
-            const R r2 = st.pixc[3 * g + 2] - st.acc_c[3 * q + 2];
-            R Q = fgs::residual_dot(r0, r1, r2, dp0, dp1, dp2, tb);
-            for (uint32_t k = e; k-- > s;) step(k, T, Q);
+          if (!any) continue;
+          for (uint32_t k = e; k-- > s;) {
+            R lane[4][NF];
+            for (int i = 0; i < 4; ++i) {
+              for (int f = 0; f < NF; ++f) lane[i][f] = R(-0.0);
+              for (int m = 0; m < 4; ++m)
+                if (act[i][m]) step(k, pix[lp[i][m]], T[i][m], Q[i][m], lane[i]);
+            }
+            add_quad(k, lane);
           }
-        } else {
-          R T = Tf, Q = tb;
-          for (uint32_t k = tile_start + cnt; k-- > tile_start;) step(k, T, Q);
         }
       }
+    } else {
+      for (int j = 0; j < 64; ++j) {  // pixels 4j .. 4j+3 (one row, four columns)
+        R T[4], Q[4];
+        uint32_t top = 0;
+        for (int i = 0; i < 4; ++i) {
+          const Pix& q = pix[4 * j + i];
+          T[i] = q.in ? st.tfinal[q.g] : R(0);
+          Q[i] = q.in ? q.tb : R(0);
+          if (q.in) top = std::max(top, q.cnt);
+        }
+        for (uint32_t k = tile_start + top; k-- > tile_start;) {
+          R lane[4][NF];
+          for (int i = 0; i < 4; ++i) {
+            for (int f = 0; f < NF; ++f) lane[i][f] = R(0);
+            const Pix& q = pix[4 * j + i];
+            if (q.in && k - tile_start < q.cnt) step(k, q, T[i], Q[i], lane[i]);
+          }
+          add_quad(k, lane);
+        }
+      }
+    }
   }
   st.raw.assign((size_t)st.P * fgs::RAW_N, 0.0);
-  for (size_t q = 0; q < Rn; ++q) {
-    const size_t k = accum == 2 ? Rn - 1 - q : q;
+  for (size_t k = 0; k < Rn; ++k) {
     double* dst = &st.raw[(size_t)st.vals[k] * fgs::RAW_N];
     const double* src = &rows[k * fgs::RAW_N];
-    if (accum == 0 || accum == 3) {
-      for (int f = 0; f < fgs::RAW_N; ++f) dst[f] += src[f];
-    } else {
-      for (int f = 0; f < fgs::RAW_N; ++f) dst[f] = (double)((float)dst[f] + (float)src[f]);
-    }
+    for (int f = 0; f < fgs::RAW_N; ++f) dst[f] += src[f];
   }
 
   gauss_grads(st, nullptr);
@@ -705,9 +761,6 @@ void orc_forward_f32(void* h) { forward(*(State<float>*)h); }
 void orc_backward_f32(void* h, const float* dimg) { backward(*(State<float>*)h, dimg); }
 void orc_gauss_grads_from_raw_f32(void* h, const float* raw12) {
   gauss_grads(*(State<float>*)h, raw12);
-}
-void orc_backward_f32_accum(void* h, const float* dimg, int accum) {
-  backward(*(State<float>*)h, dimg, accum);
 }
 long long orc_get_f32(void* h, const char* name, void* dst) {
   return get_named((State<float>*)h, name, dst);

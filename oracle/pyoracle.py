@@ -40,7 +40,6 @@ def lib():
             g.restype = ctypes.c_longlong
             getattr(L, "orc_destroy_" + suf).argtypes = [vp]
         L.orc_stats_f32.argtypes = [vp] * 5
-        L.orc_backward_f32_accum.argtypes = [vp, vp, i32]
         L.orc_gauss_grads_from_raw_f32.argtypes = [vp, vp]
         L.orc_exp2f.argtypes = [ctypes.c_float]
         L.orc_exp2f.restype = ctypes.c_float
@@ -119,19 +118,16 @@ class Oracle:
         getattr(self.L, "orc_forward_" + self.suf)(self.h)
         return self
 
-    def backward(self, dimg, accum: int = 0):
-        """accum 0: double sums (checker); 1 / 2: float sums in sorted / reverse
-        pair order (float32 oracle only), to bound float summation error."""
+    def backward(self, dimg):
+        """Adjoint of the image for dL/dimage `dimg` (3 x H x W)."""
         d = np.ascontiguousarray(dimg, dtype=self.dtype).reshape(3, self.H, self.W)
-        if accum:
-            self.L.orc_backward_f32_accum(self.h, _ptr(d), int(accum))
-        else:
-            getattr(self.L, "orc_backward_" + self.suf)(self.h, _ptr(d))
+        getattr(self.L, "orc_backward_" + self.suf)(self.h, _ptr(d))
         return self
 
     def gauss_grads_from_raw(self, raw12):
-        """Run only the per-Gaussian adjoint on given raw sums (P x 12 float32)."""
-        r = np.ascontiguousarray(raw12, dtype=np.float32).reshape(self.P, 12)
+        """Run only the per-Gaussian adjoint on given raw sums (P x 12; rounded
+        to float32 first, as the GPU's per-Gaussian kernels round them)."""
+        r = np.ascontiguousarray(np.asarray(raw12).astype(np.float32)).reshape(self.P, 12)
         self.L.orc_gauss_grads_from_raw_f32(self.h, _ptr(r))
         return self.grads()
 

@@ -26,7 +26,7 @@ CASES = {
 def _frame(scene, cam, schedule):
     d = to_dev(scene)
     return Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
-                 dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule)).forward()
+                 dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule, bucket_size=32)).forward()
 
 
 @pytest.mark.parametrize("case", list(CASES))

@@ -30,12 +30,12 @@ def _scene():
 def _renderer(d, cam, schedule, headroom=1.5):
     return CapturedRenderer(settings_for(cam), d["means3D"], d["opacities"], d["scales"],
                             d["rotations"], dc=d["dc"], shs=d["shs"],
-                            options=RasterOptions(schedule=schedule), headroom=headroom)
+                            options=RasterOptions(schedule=schedule, bucket_size=32), headroom=headroom)
 
 
 def _eager(d, cam, schedule):
     fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
-               dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule)).forward()
+               dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule, bucket_size=32)).forward()
     torch.cuda.synchronize()
     return fr
 
@@ -156,11 +156,8 @@ def test_captured_training_step_matches_eager(sizes):
         fr, ref = _eager_step(d, cams, tg, ref_stats)  # eager work between replays
         np.testing.assert_array_equal(img.cpu().numpy(), fr.out_color.cpu().numpy())
         np.testing.assert_array_equal(radii.cpu().numpy(), fr.radii.cpu().numpy())
-        for n in ("opacities", "dc", "shs", "mean2D", "conic"):
+        for n in ("opacities", "dc", "shs", "mean2D", "conic", "means3D", "scales", "rotations"):
             grad_close(g[n].cpu().numpy(), ref[n].cpu().numpy(), rtol=1e-4, name=n)
-        for n in ("means3D", "scales", "rotations"):
-            a, b = g[n].cpu().numpy().ravel(), ref[n].cpu().numpy().ravel()
-            assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b), n
     assert step.check()
     np.testing.assert_array_equal(stats[3].cpu().numpy(), ref_stats[3].cpu().numpy())
     np.testing.assert_array_equal(stats[2].cpu().numpy(), ref_stats[2].cpu().numpy())

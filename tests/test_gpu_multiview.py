@@ -30,7 +30,7 @@ def batch_case():
     scene = scenes.cluster_scene(30000, seed=21, clusters=16, radius=1.5)
     cams = scenes.sphere_cameras(len(SIZES), seed=4, rmin=3.0, rmax=4.5, sizes=SIZES)
     d = to_dev(scene)
-    opts = RasterOptions(schedule=SCHEDULE_FAST)
+    opts = RasterOptions(schedule=SCHEDULE_FAST, bucket_size=32)
     bf = BatchFrame([settings_for(c) for c in cams], d["means3D"], d["opacities"], d["scales"],
                     d["rotations"], dc=d["dc"], shs=d["shs"], options=opts).forward()
     singles = [Frame(settings_for(c), d["means3D"], d["opacities"], d["scales"], d["rotations"],
@@ -89,7 +89,7 @@ def test_backward_matches_single_views(batch_case):
         gs = s.backward(2.0 * (s.out_color - t) / s.out_color.numel())
         grad_close(gb["mean2D"][v].cpu().numpy(), gs["mean2D"].cpu().numpy(), 1e-4, f"mean2D[{v}]")
         grad_close(gb["conic"][v].cpu().numpy(), gs["conic"].cpu().numpy(), 1e-4, f"conic[{v}]")
-        grad_close(gb["raw"][v].cpu().numpy(), gs["raw"].cpu().numpy(), 1e-4, f"raw[{v}]")
+        grad_close(gb["raw"][v].cpu().numpy(), gs["raw"].cpu().numpy(), 1e-10, f"raw[{v}]")
         if sums is None:
             sums = {k: gs[k].double().clone() for k in ("means3D", "dc", "shs", "opacities",
                                                          "scales", "rotations", "cov3D")}
@@ -97,15 +97,8 @@ def test_backward_matches_single_views(batch_case):
             for k in sums:
                 sums[k] += gs[k].double()
     for k, ref in sums.items():
-        # geometric terms amplify float reassociation of the raw sums (see
-        # the single-view parity tests); linear ones stay within 1e-4
-        tol = 1e-4 if k in ("dc", "shs", "opacities") else 1e-2
-        got = gb[k].double().cpu().numpy().ravel()
-        ref = ref.cpu().numpy().ravel()
-        if tol == 1e-4:
-            grad_close(got, ref, tol, k)
-        else:
-            assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref), k
+        # the batch adds the views' parameter gradients in float, view order
+        grad_close(gb[k].double().cpu().numpy().ravel(), ref.cpu().numpy().ravel(), 1e-4, k)
 
 
 def test_densify_stats_match_consecutive_views(batch_case):
@@ -126,7 +119,8 @@ def test_autograd_entry_point(batch_case):
     params = {k: v.clone().requires_grad_(True) for k, v in d.items()}
     imgs, radii = rasterize_views([settings_for(c) for c in cams], params["means3D"],
                                   params["opacities"], dc=params["dc"], shs=params["shs"],
-                                  scales=params["scales"], rotations=params["rotations"])
+                                  scales=params["scales"], rotations=params["rotations"],
+                                  options=bf.opts)
     assert radii.shape == (len(cams), d["means3D"].shape[0])
     for v, s in enumerate(singles):
         np.testing.assert_array_equal(imgs[v].detach().cpu().numpy(), s.out_color.cpu().numpy())
@@ -151,14 +145,23 @@ def test_autotuner_picks_a_candidate(batch_case, tmp_path):
         torch.cuda.synchronize()
 
     cache = str(tmp_path / "tune.json")
-    tuner = Autotuner(buckets=(16, 32), splits=(32, 128), reps=1, cache_path=cache)
+    # bucket size tuned on request (it is semantic); split threshold and the
+    # compositing work shape (whole tiles or 16x8 bands) never change outputs
+    tuner = Autotuner(buckets=(16, 32), splits=(32, 128), work_splits=(None, 1), reps=1,
+                      cache_path=cache)
     best, table = tuner.tune(step, key="k")
-    assert len(table) == 4 and table[0][1] <= table[-1][1]
+    assert len(table) == 8 and table[0][1] <= table[-1][1]
     assert (best.bucket_size, best.split_tiles) in {(16, 32), (16, 128), (32, 32), (32, 128)}
+    shape = lambda o: None if o.tile_sched is None else o.tile_sched.split_len
     n = len(calls)
     again, _ = Autotuner(cache_path=cache).tune(step, key="k")
     assert len(calls) == n  # served from the cache file
-    assert (again.bucket_size, again.split_tiles) == (best.bucket_size, best.split_tiles)
+    assert (again.bucket_size, again.split_tiles, shape(again)) == \
+        (best.bucket_size, best.split_tiles, shape(best))
+    # without buckets the caller's bucket size is kept
+    fixed = Autotuner(splits=(64,), work_splits=(None,), reps=1,
+                      base=RasterOptions(schedule=1, bucket_size=32))
+    assert [c.bucket_size for c in fixed.candidates] == [32]
     # the tuned options render exactly what a frame with those options does
     # (the bucket size is part of the reference's semantics -- the T < 1e-4
     # stop is tested once per bucket -- so images of different bucket sizes

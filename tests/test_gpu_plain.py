@@ -123,8 +123,10 @@ def _sort_case(n, bits, seed, with_vals=True, ws=None, distinct=None):
     if ws is None:
         ws = torch.empty(L.fgs_sort_ws_size(n), dtype=torch.uint8, device="cuda")
     p = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
-    check(L.fgs_sort_pairs(p(k), p(v), p(ka), p(va), n, 0, bits, p(ws), ws.numel(),
-                           ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "sort")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(L.fgs_sort_pairs(p(k), p(v), p(ka), p(va), n, 0, bits, p(ws), ws.numel(), stream),
+          "sort")
+    check(L.fgs_sort_check(p(ws), n, stream), "sort check")  # no look-back wait gave up
     order = np.argsort(keys, kind="stable")
     np.testing.assert_array_equal(k.cpu().numpy().view(np.uint64), keys[order])
     if with_vals:

@@ -30,7 +30,7 @@ def test_render_simp_matches_oracle(case, schedule):
     scene, cam = CASES[case]()
     d = to_dev(scene)
     fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
-               dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule)).forward()
+               dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule, bucket_size=32)).forward()
     w, c, m = fr.render_simp()
     orc = Oracle(scene, cam, bucket=32, mode=0 if schedule else 1).forward()
     rw, rc, rm = orc.render_simp()
