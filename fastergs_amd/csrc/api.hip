@@ -341,9 +341,11 @@ size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
   return carve_pair(nullptr, R, NB, (int64_t)width * height, tiles_of(width, height), ck).total;
 }
 
-// raw sums (P x 12 doubles) and the split SH pass's mean term (P x 4 floats)
+// raw sums (P x 4 doubles + P x 8 floats) and the split SH pass's mean term
+// (P x 4 floats)
 size_t fgs_grad_ws_size(int P) {
-  return align_up((size_t)(P > 0 ? P : 1) * (RAW_STRIDE * sizeof(double) + 4 * sizeof(float)));
+  return align_up((size_t)(P > 0 ? P : 1) *
+                  (RAW_CON_STRIDE * sizeof(double) + (RAW_LIN_STRIDE + 4) * sizeof(float)));
 }
 
 size_t fgs_batch_gauss_ws_size(int P, const fgs_batch* batch, const fgs_options* opt) {
@@ -364,7 +366,8 @@ size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t 
 
 size_t fgs_batch_grad_ws_size(int P, int n_views) {
   const size_t p = (size_t)(P > 0 ? P : 1), v = (size_t)(n_views > 0 ? n_views : 1);
-  return align_up(p * v * RAW_STRIDE * sizeof(double) + (v == 1 ? p * 4 * sizeof(float) : 0));
+  return align_up(p * v * (RAW_CON_STRIDE * sizeof(double) + RAW_LIN_STRIDE * sizeof(float)) +
+                  (v == 1 ? p * 4 * sizeof(float) : 0));
 }
 
 size_t fgs_sort_ws_size(int64_t n) { return sort_ws_bytes(n); }
@@ -663,11 +666,15 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
   hipStream_t s = (hipStream_t)stream;
-  double* raw = (double*)grad_ws;
-  // the raw sums are cleared beside the active-bucket scan (0.0 is all-zero bits)
+  // raw sums: PV conic rows (doubles), then PV rows of the other fields
+  // (floats), then the split SH pass's P x 4 floats; cleared beside the
+  // active-bucket scan (0.0 is all-zero bits)
+  double* raw_con = (double*)grad_ws;
+  float* raw_lin = (float*)(raw_con + (size_t)PV * RAW_CON_STRIDE);
   const Fork f(s);
   FGS_TRY(f.fork(0, 6), "fork");
-  FGS_TRY(fill_f32((float*)raw, 0.0f, PV * RAW_STRIDE * 2, f.side(0)), "zero raw");
+  FGS_TRY(fill_f32((float*)raw_con, 0.0f, PV * (2 * RAW_CON_STRIDE + RAW_LIN_STRIDE), f.side(0)),
+          "zero raw");
   BwdArgs bw;
   bw.W = cam->width;
   bw.H = cam->height;
@@ -689,7 +696,8 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   bw.acc = pw.acc;
   bw.actoff = gw.actoff;
   bw.dimg = dL_dimage;
-  bw.raw = raw;
+  bw.raw_lin = raw_lin;
+  bw.raw_con = raw_con;
   bw.views = (const ViewRec*)gw.views;
   bw.V = b.V;
   if (fast) {
@@ -714,7 +722,8 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
     q.cam.campos = v0->campos;
   }
   q.P = g->P;
-  q.raw = raw;
+  q.raw_lin = raw_lin;
+  q.raw_con = raw_con;
   q.radii = radii;
   q.splat = gw.splat;
   q.dmean2D = out->dmean2D;
@@ -729,7 +738,7 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   q.drot = out->drot;
   q.views = fast ? (const ViewRec*)gw.views : nullptr;
   q.V = b.V;
-  q.sh_dm = b.V == 1 ? (float*)(raw + (size_t)g->P * RAW_STRIDE) : nullptr;
+  q.sh_dm = b.V == 1 ? raw_lin + (size_t)PV * RAW_LIN_STRIDE : nullptr;
   FGS_TRY(launch_gauss_bwd(q, s), "preprocess backward");
   return FGS_OK;
 }

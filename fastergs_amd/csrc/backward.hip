@@ -8,18 +8,23 @@
 // has a fixed tree and whose cross-quad part and atomics are fp64).
 // Preprocess adjoint: backward.cu:250-539. Statistics: backward.cu:892-918.
 //
-// Raw per-Gaussian sums land in a P x 12 double buffer (fields in
-// fgs_math.h RawField order); the per-Gaussian kernels round them to float,
-// apply the constant factors and produce the reference's gradient tensors.
+// Raw per-Gaussian sums land in two arrays of rows (fgs_math.h RawField,
+// raw_lin_index): the three conic sums in double, the eight others in
+// float; the per-Gaussian passes round them to float, apply the constant
+// factors and produce the reference's gradient tensors.
 //
-// Determinism: every raw sum is sum_j (float) q_j over the lane quads j that
-// the splat's buckets cover, where q_j = (l0 + l1) + (l2 + l3) in float over
-// the quad's four lanes and each lane value is the fused float accumulation
-// of its pixels in a fixed order. The q_j are added in fp64 (tree inside the
-// wave, atomics across waves), whose reassociation error (~1e-16 relative)
-// is far below the float rounding of the result, so the sums -- and every
-// gradient computed from them -- are the same from run to run and equal to
-// the CPU oracle's, which forms the same q_j and adds them in double.
+// Summation: per splat, every lane accumulates its pixels' terms in float in
+// a fixed order, and the wave adds its 64 lane values in float along a fixed
+// tree (wave_flush); the wave totals are added across waves with atomics,
+// in fp64 for the conic sums and in float for the others. The conic -> 2D
+// covariance adjoint (inverse of a possibly elongated 2x2 matrix) amplifies
+// the rounding of the conic sums by up to its squared condition number, so
+// those are added order-independently: fp64 reassociation (~1e-16 relative)
+// is far below their float rounding, so they -- and the geometric gradients
+// they feed -- come out the same from run to run and equal to the CPU
+// oracle's, which forms the same float wave totals and adds them in double.
+// The other sums enter their gradients linearly; float atomics move those
+// by ~1e-7 relative.
 #include <hip/hip_runtime.h>
 
 #include "fgs_device.h"
@@ -43,32 +48,28 @@ struct ExactRcp {
   }
 };
 
-template <int CTRL>
-__device__ __forceinline__ double dpp_d(double v) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
-  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-}
-
-// Reduce NF (<= 16) raw fields over the wave and add them to raw[12*id + f]
-// with one fp64 atomic instruction. Transposing butterfly: at each step a
-// lane keeps one field of every pair and hands the other to a partner lane
-// that differs in the step's lane bit (xor 1, xor 2 by quad_perm; bit 2 and
-// bit 3 by row rotations of 4 and 8, whose partners hold the same fields), so
-// the number of fields per lane halves while the lanes they cover double;
-// after four steps lane l holds field l % 16 summed over its row, and two
-// shuffles add the four rows. The first two steps are in float: for every
-// field they form (l0 + l1) + (l2 + l3) over the lanes 4j .. 4j+3 (float
-// addition commutes, so which lane adds is immaterial); the remaining steps
-// convert those quad sums to double.
+// Reduce NF (<= 16) raw fields over the wave and add field f to the raw sums
+// of record id (fp64 atomics for the conic fields, float for the others). Transposing butterfly: at
+// each step a lane keeps one field of every pair and hands the other to a
+// partner lane that differs in the step's lane bit (xor 1, xor 2 by
+// quad_perm; bit 2 and bit 3 by row rotations of 4 and 8, whose partners hold
+// the same fields), so the number of fields per lane halves while the lanes
+// they cover double; after four steps lane l holds field l % 16 summed over
+// its row, and two shuffles add the four rows. About 40 instructions for 9
+// fields instead of six DPP steps per field. The float tree it evaluates is
+// fixed (float addition commutes, so which lane of a pair adds is
+// immaterial): with a_l the lane values, q_j = (a_4j + a_4j+1) + (a_4j+2 +
+// a_4j+3) the lane-quad sums, row r's sum of field f is (q_g + q_g-1) +
+// (q_g-2 + q_g-3) over the quads 4r + (. mod 4) with g = (f / 4) mod 4
+// (row_ror:n reads lane (i - n) mod 16), and the total is (row0 + row1) +
+// (row2 + row3).
 template <int NF>
-__device__ __forceinline__ void wave_flush(const float* acc, double* raw, uint32_t id) {
+__device__ __forceinline__ void wave_flush(const float* acc, float* raw_lin, double* raw_con,
+                                           uint32_t id) {
   static_assert(NF <= 16, "at most 16 fields");
   const int l = lane_id();
   const bool b0 = l & 1, b1 = (l >> 1) & 1, b2 = (l >> 2) & 1, b3 = (l >> 3) & 1;
-  float s1[8];
-  double s2[4], s3[2];
+  float s1[8], s2[4], s3[2];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const float a = 2 * m < NF ? acc[2 * m] : 0.0f;
@@ -77,18 +78,21 @@ __device__ __forceinline__ void wave_flush(const float* acc, double* raw, uint32
   }
 #pragma unroll
   for (int n = 0; n < 4; ++n)
-    s2[n] = 4 * n < NF ? (double)((b1 ? s1[2 * n + 1] : s1[2 * n]) +
-                                  dpp_f<0x4e>(b1 ? s1[2 * n] : s1[2 * n + 1]))  // quad_perm [2,3,0,1]
-                       : 0.0;
+    s2[n] = 4 * n < NF ? (b1 ? s1[2 * n + 1] : s1[2 * n]) +
+                             dpp_f<0x4e>(b1 ? s1[2 * n] : s1[2 * n + 1])  // quad_perm [2,3,0,1]
+                       : 0.0f;
 #pragma unroll
   for (int p = 0; p < 2; ++p)
     s3[p] = 8 * p < NF ? (b2 ? s2[2 * p + 1] : s2[2 * p]) +
-                             dpp_d<0x124>(b2 ? s2[2 * p] : s2[2 * p + 1])  // row_ror:4
-                       : 0.0;
-  double v = (b3 ? s3[1] : s3[0]) + dpp_d<0x128>(b3 ? s3[0] : s3[1]);  // row_ror:8
+                             dpp_f<0x124>(b2 ? s2[2 * p] : s2[2 * p + 1])  // row_ror:4
+                       : 0.0f;
+  float v = (b3 ? s3[1] : s3[0]) + dpp_f<0x128>(b3 ? s3[0] : s3[1]);  // row_ror:8
   v += __shfl_xor(v, 16, WAVE);
   v += __shfl_xor(v, 32, WAVE);
-  if (l < NF) unsafeAtomicAdd(raw + (size_t)RAW_STRIDE * id + l, v);
+  if (l >= RAW_KXX && l <= RAW_KYY)
+    unsafeAtomicAdd(raw_con + (size_t)RAW_CON_STRIDE * id + (l - RAW_KXX), (double)v);
+  else if (l < NF)
+    atomic_add_f(raw_lin + (size_t)RAW_LIN_STRIDE * id + raw_lin_index(l), v);
 }
 
 // Exhaustive check of ExactRcp against the IEEE quotient over [2^-7, 1].
@@ -111,15 +115,27 @@ hipError_t launch_rcp_selftest(unsigned long long* bad, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Raw sums of one record, rounded to float.
-__device__ __forceinline__ void load_raw(const double* p, float* raw) {
-  const double2* r2 = (const double2*)p;
-#pragma unroll
-  for (int k = 0; k < RAW_STRIDE / 2; ++k) {
-    const double2 q = r2[k];
-    raw[2 * k] = (float)q.x;
-    raw[2 * k + 1] = (float)q.y;
-  }
+// Raw sums of one record in RawField order (float; the conic ones rounded
+// from double).
+__device__ __forceinline__ void load_raw(const float* raw_lin, const double* raw_con, size_t i,
+                                         float* raw) {
+  const float4* l4 = (const float4*)(raw_lin + RAW_LIN_STRIDE * i);
+  const double2* c2 = (const double2*)(raw_con + RAW_CON_STRIDE * i);
+  const float4 a = l4[0], b = l4[1];
+  const double2 k01 = c2[0];
+  const double k2 = raw_con[RAW_CON_STRIDE * i + 2];
+  raw[RAW_COL0] = a.x;
+  raw[RAW_COL1] = a.y;
+  raw[RAW_COL2] = a.z;
+  raw[RAW_MX] = a.w;
+  raw[RAW_MY] = b.x;
+  raw[RAW_KXX] = (float)k01.x;
+  raw[RAW_KXY] = (float)k01.y;
+  raw[RAW_KYY] = (float)k2;
+  raw[RAW_POW] = b.y;
+  raw[RAW_AX] = b.z;
+  raw[RAW_AY] = b.w;
+  raw[RAW_STRIDE - 1] = 0.0f;
 }
 
 // ---------------------------------------------------------- PLAIN backward
@@ -176,7 +192,7 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
         pix_bwd<float, ExactRcp, ABS>(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, sC[j], fx, fy,
                                       dp0, dp1, dp2, T, Q, acc);
       }
-      wave_flush<NF>(acc, a.raw, sId[j]);
+      wave_flush<NF>(acc, a.raw_lin, a.raw_con, sId[j]);
     }
   }
 }
@@ -279,7 +295,7 @@ __global__ __launch_bounds__(256, 6) void k_backward_fast(BwdArgs a) {
       for (int q = 0; q < 4; ++q)
         pix_bwd<float, ExactRcp, ABS>(A0.x, A0.y, A0.z, A0.w, B0.x, B0.y, B0.z, B0.w, C0, fx[q],
                                      fy[q], d0[q], d1[q], d2[q], T[q], Q[q], acc);
-      wave_flush<NF>(acc, a.raw, id);
+      wave_flush<NF>(acc, a.raw_lin, a.raw_con, id);
       id = idn;
       A0 = A1;
       B0 = B1;
@@ -317,7 +333,7 @@ __global__ __launch_bounds__(GBWD_THREADS) void k_gauss_bwd(GaussBwdArgs a) {
   __syncthreads();
   if (i < a.P) {
     float raw[RAW_STRIDE];
-    load_raw(a.raw + (size_t)RAW_STRIDE * i, raw);
+    load_raw(a.raw_lin, a.raw_con, (size_t)i, raw);
     Grad2D<float> g2;
     finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
     if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
@@ -396,7 +412,7 @@ __global__ __launch_bounds__(GBWDV_THREADS) void k_gauss_bwd_views(GaussBwdArgs 
       const ViewRec& vr = a.views[v];
       const size_t ri = (size_t)v * a.P + i;
       float raw[RAW_STRIDE];
-      load_raw(a.raw + (size_t)RAW_STRIDE * ri, raw);
+      load_raw(a.raw_lin, a.raw_con, ri, raw);
       Grad2D<float> g2;
       finalize_raw(raw, a.G.opac[i], vr.W, vr.H, g2);
       if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
@@ -475,9 +491,8 @@ __global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
   const bool vis = in && a.radii[ic] > 0;
   const float* pm = a.G.means + 3 * ic;
   const float px = pm[0], py = pm[1], pz = pm[2];
-  const float col[3] = {(float)a.raw[RAW_STRIDE * ic + RAW_COL0],
-                        (float)a.raw[RAW_STRIDE * ic + RAW_COL1],
-                        (float)a.raw[RAW_STRIDE * ic + RAW_COL2]};
+  const float4 c4 = ((const float4*)(a.raw_lin + RAW_LIN_STRIDE * ic))[0];
+  const float col[3] = {c4.x, c4.y, c4.z};
   const int clampbits = (int)f2u(a.splat[12 * ic + 11]);
   // full groups of 64 rows start 16-byte aligned when the arrays do
   const bool vec = M > 0 && M <= 15 && nb == SHB_THREADS && RM * SHB_THREADS % 4 == 0 &&
@@ -548,7 +563,7 @@ __global__ __launch_bounds__(256) void k_gauss_bwd_geom(GaussBwdArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= a.P) return;
   float raw[RAW_STRIDE];
-  load_raw(a.raw + (size_t)RAW_STRIDE * i, raw);
+  load_raw(a.raw_lin, a.raw_con, (size_t)i, raw);
   Grad2D<float> g2;
   finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
   if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;

@@ -293,7 +293,10 @@ struct BwdArgs {
   const float* acc;
   const uint32_t* actoff;  // fast: exclusive prefix of active buckets per tile, [T] = total
   const float* dimg;
-  double* raw;  // P x RAW_STRIDE (V x P x RAW_STRIDE) raw sums, accumulated in fp64
+  // raw sums of the P (V x P) records: float rows of RAW_LIN_STRIDE (every
+  // field but the conic ones) and double rows of RAW_CON_STRIDE (conic)
+  float* raw_lin;
+  double* raw_con;
   const ViewRec* views;  // fast: V views
   int V;
 };
@@ -309,7 +312,8 @@ struct GaussBwdArgs {
   GaussRef<float> G;
   CamRef<float> cam;
   int P;
-  const double* raw;
+  const float* raw_lin;  // as in BwdArgs
+  const double* raw_con;
   const int32_t* radii;
   const float* splat;
   float *dmean2D, *dconic, *dopacity, *dcolor, *dmean3D, *dcov3D, *ddc, *dsh, *dscale, *drot;

@@ -570,6 +570,13 @@ enum RawField {
   RAW_N = 11
 };
 constexpr int RAW_STRIDE = 12;
+// Device layout of the raw sums, two arrays of rows per record: the conic
+// sums RAW_KXX..RAW_KYY in double (padded to 4), summed order-independently
+// because the conic -> 2D covariance adjoint amplifies their rounding, and
+// the other eight fields in float, in RawField order without the conic ones.
+constexpr int RAW_CON_STRIDE = 4;
+constexpr int RAW_LIN_STRIDE = 8;
+FGS_HD constexpr int raw_lin_index(int f) { return f < RAW_KXX ? f : f - 3; }
 
 // One reverse step of the backward for one pixel (backward.cu:549-632):
 // recovers T before the splat and accumulates the raw sums into acc[RAW_N].

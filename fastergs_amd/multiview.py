@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from ._lib import Batch, Camera, FrameView, Gaussians, Grads, check, lib
-from .rasterizer import RasterOptions, _f32, _ptr, _stream
+from .rasterizer import RasterOptions, _Grads, _f32, _ptr, _stream
 
 
 class BatchFrame:
@@ -151,8 +151,7 @@ class BatchFrame:
                                    self.R, self.NB, _ptr(self.radii), _ptr(self.T_final),
                                    _ptr(self.last_contributor), _ptr(g), _ptr(gws),
                                    ctypes.byref(grads), self._s()), "fgs_batch_backward")
-        out["raw"] = gws[:V * P * 96].view(torch.float64).reshape(V, P, 12)
-        return out
+        return _Grads(out, gws, V * P, (V, P, 12))
 
     def densify_stats(self, dmean2D, accum, accum_abs, max_radii, denom):
         check(lib().fgs_batch_densify_stats(self.P, self.V, _ptr(self.radii), _ptr(dmean2D),

@@ -264,8 +264,7 @@ class Frame:
                              self.NB_cap, _ptr(self.radii), _ptr(self.T_final),
                              _ptr(self.last_contributor), _ptr(g), _ptr(gws),
                              ctypes.byref(grads), ctypes.c_void_p(_stream(dev))), "fgs_backward")
-        out["raw"] = gws[:P * 96].view(torch.float64).reshape(P, 12)
-        return out
+        return _Grads(out, gws, P)
 
     def densify_stats(self, dmean2D, accum, accum_abs, max_radii, denom):
         check(lib().fgs_densify_stats(self.P, _ptr(self.radii), _ptr(dmean2D), _ptr(accum),
@@ -336,6 +335,32 @@ class Frame:
                 continue
             res[name] = _tensor_at(addr, n, dt, self)
         return res
+
+
+class _Grads(dict):
+    """Gradient tensors by name; "raw" (the raw per-record sums, see
+    raw_sums) is assembled from the gradient workspace on first access."""
+
+    def __init__(self, grads: dict, grad_ws: torch.Tensor, n: int, shape=None):
+        super().__init__(grads)
+        self._ws, self._n, self._shape = grad_ws, n, shape
+
+    def __missing__(self, key):
+        if key != "raw":
+            raise KeyError(key)
+        raw = raw_sums(self._ws, self._n)
+        self["raw"] = raw if self._shape is None else raw.reshape(self._shape)
+        return self["raw"]
+
+
+def raw_sums(grad_ws: torch.Tensor, n: int) -> torch.Tensor:
+    """The backward's raw per-record sums (n x 12 float64, fgs_math.h RawField
+    order: colour 3, mean2D 2, conic 3, power 1, |mean2D| 2, one zero) from
+    the gradient workspace's conic rows (4 doubles) and float rows of the
+    other eight fields."""
+    con = grad_ws[:n * 32].view(torch.float64).reshape(n, 4)
+    lin = grad_ws[n * 32:n * 64].view(torch.float32).reshape(n, 8).double()
+    return torch.cat([lin[:, :5], con[:, :3], lin[:, 5:], torch.zeros_like(con[:, :1])], 1)
 
 
 def _tensor_at(addr: int, n: int, dtype, frame: Frame) -> torch.Tensor:

@@ -136,9 +136,10 @@ typedef struct fgs_frame_view {
 size_t fgs_gauss_ws_size(int P, int width, int height, const fgs_options* opt);
 size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
                         const fgs_options* opt);
-/* Gradient workspace of the backward: the raw per-Gaussian sums (P x 12
- * doubles at its start, fields colour 3, mean2D 2, conic 3, power 1,
- * |mean2D| 2, readable after fgs_backward) and P x 4 floats of scratch. */
+/* Gradient workspace of the backward. It starts with the raw per-Gaussian
+ * sums, readable after fgs_backward: P rows of 4 doubles (conic sums 3, 1
+ * unused; added order-independently) then P rows of 8 floats (colour 3,
+ * mean2D 2, power 1, |mean2D| 2); then P x 4 floats of scratch. */
 size_t fgs_grad_ws_size(int P);
 
 /* Replaces faster::preprocess (preprocess.cu:928-962, pybind.cpp:26-76):

@@ -311,18 +311,39 @@ template <typename R>
 void gauss_grads(State<R>& st, const float* raw_f);
 
 // Raw per-Gaussian sums of the rasterizer adjoint, formed exactly as the
-// gfx950 reverse walks form them (so GPU and oracle agree to the last bit of
-// the float-rounded sums). A wave of the walk covers a tile's 256 pixels in
-// 64 lanes; per splat each lane accumulates its pixels' terms in float, in a
-// fixed order, into one partial (fused accumulation from -0 / 0), and every
-// four lanes 4j .. 4j+3 form the float quad sum (l0 + l1) + (l2 + l3); the
-// quad sums are added in double. Lane l holds:
-//   FAST  (mode 0): the 2x2 pixels (2 (l % 8) + i, 2 (l / 8) + j), in the
-//                   order (0,0) (1,0) (0,1) (1,1), one wave per bucket;
-//   PLAIN (mode 1): pixel (l % 16, 4 w + l / 16) of wave w, one per lane.
+// gfx950 reverse walks form them, so that GPU and oracle agree to the last
+// bit of the float-rounded sums. A wave of the walk covers 64 lanes; per
+// splat each lane accumulates its pixels' terms in float in a fixed order
+// (fused accumulation from -0 / 0), the wave adds its 64 lane values in float
+// along a fixed tree (wave_total), and the wave totals are added in double.
+//   FAST  (mode 0): one wave per bucket, lane l holds the 2x2 pixels
+//                   (2 (l % 8) + i, 2 (l / 8) + j), in the order (0,0) (1,0)
+//                   (0,1) (1,1);
+//   PLAIN (mode 1): four waves per tile, lane l of wave w holds pixel
+//                   (l % 16, 4 w + l / 16).
 // The reference adds float warp sums with float atomics (backward.cu:634-693),
-// whose order varies between runs; the double addition of fixed float quad
-// sums is that computation with a deterministic result.
+// whose order varies between runs; the double addition of fixed float wave
+// totals is that computation with a deterministic result.
+
+// Float tree of the GPU's wave reduction for field f over lane values a[64]:
+// lane-quad sums q_j = (a_4j + a_4j+1) + (a_4j+2 + a_4j+3); per row of 16
+// lanes (q_4r .. q_4r+3, indices mod 4 below) (q_g + q_g-1) + (q_g-2 + q_g-3)
+// with g = (f / 4) mod 4, the order the transposing butterfly's row
+// rotations produce; then (row0 + row1) + (row2 + row3).
+template <typename R>
+R wave_total(const R (*a)[fgs::RAW_N], int f) {
+  R q[16];
+  for (int j = 0; j < 16; ++j)
+    q[j] = (a[4 * j][f] + a[4 * j + 1][f]) + (a[4 * j + 2][f] + a[4 * j + 3][f]);
+  const int g = (f >> 2) & 3;
+  R row[4];
+  for (int r = 0; r < 4; ++r) {
+    const R* qq = &q[4 * r];
+    row[r] = (qq[g] + qq[(g + 3) & 3]) + (qq[(g + 2) & 3] + qq[(g + 1) & 3]);
+  }
+  return (row[0] + row[1]) + (row[2] + row[3]);
+}
+
 template <typename R>
 void backward(State<R>& st, const R* dimg) {
   const size_t HW = (size_t)st.W * st.H;
@@ -358,72 +379,66 @@ void backward(State<R>& st, const R* dimg) {
       q.tb = st.tfinal[q.g] * (bg0 * q.dp0 + bg1 * q.dp1 + bg2 * q.dp2);
       q.cnt = (uint32_t)st.contrib[q.g];
     }
-    auto add_quad = [&](uint32_t k, R lane[4][NF]) {
+    auto add_wave = [&](uint32_t k, R (*lane)[NF]) {
       double* row = &rows[k * NF];
-      for (int f = 0; f < NF; ++f)
-        row[f] += (double)((lane[0][f] + lane[1][f]) + (lane[2][f] + lane[3][f]));
+      for (int f = 0; f < NF; ++f) row[f] += (double)wave_total(lane, f);
     };
     auto step = [&](uint32_t k, const Pix& q, R& T, R& Q, R* acc) {
       const R* sp = &st.splat[9 * (size_t)st.vals[k]];
       fgs::pix_bwd(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], sp[6], sp[7], sp[8], q.fx, q.fy,
                    q.dp0, q.dp1, q.dp2, T, Q, acc);
     };
+    R lane[64][NF];
     if (st.mode == 0) {
       for (uint32_t b = b0; b < b1; ++b) {
         const uint32_t s = st.bucket_ranges[2 * b], e = st.bucket_ranges[2 * b + 1];
-        for (int j = 0; j < 16; ++j) {
-          int lp[4][4];
-          bool act[4][4];
-          R T[4][4], Q[4][4];
-          bool any = false;
-          for (int i = 0; i < 4; ++i) {
-            const int l = 4 * j + i;
-            for (int m = 0; m < 4; ++m) {
-              const int lx = 2 * (l & 7) + (m & 1), ly = 2 * (l >> 3) + (m >> 1);
-              const int p = ly * fgs::TILE_W + lx;
-              lp[i][m] = p;
-              act[i][m] = pix[p].in && s - tile_start < pix[p].cnt;  // backward.cu:663-664
-              any = any || act[i][m];
-              if (!act[i][m]) continue;
-              const size_t q = (size_t)b * fgs::TILE_PIX + p;
-              const size_t g = pix[p].g;
-              T[i][m] = st.acc_t[q];
-              Q[i][m] = fgs::residual_dot(st.pixc[3 * g] - st.acc_c[3 * q],
-                                          st.pixc[3 * g + 1] - st.acc_c[3 * q + 1],
-                                          st.pixc[3 * g + 2] - st.acc_c[3 * q + 2], pix[p].dp0,
-                                          pix[p].dp1, pix[p].dp2, pix[p].tb);
-            }
+        int lp[64][4];
+        bool act[64][4];
+        R T[64][4], Q[64][4];
+        bool any = false;
+        for (int l = 0; l < 64; ++l)
+          for (int m = 0; m < 4; ++m) {
+            const int lx = 2 * (l & 7) + (m & 1), ly = 2 * (l >> 3) + (m >> 1);
+            const int p = ly * fgs::TILE_W + lx;
+            lp[l][m] = p;
+            act[l][m] = pix[p].in && s - tile_start < pix[p].cnt;  // backward.cu:663-664
+            any = any || act[l][m];
+            if (!act[l][m]) continue;
+            const size_t q = (size_t)b * fgs::TILE_PIX + p;
+            const size_t g = pix[p].g;
+            T[l][m] = st.acc_t[q];
+            Q[l][m] = fgs::residual_dot(st.pixc[3 * g] - st.acc_c[3 * q],
+                                        st.pixc[3 * g + 1] - st.acc_c[3 * q + 1],
+                                        st.pixc[3 * g + 2] - st.acc_c[3 * q + 2], pix[p].dp0,
+                                        pix[p].dp1, pix[p].dp2, pix[p].tb);
           }
-          if (!any) continue;
-          for (uint32_t k = e; k-- > s;) {
-            R lane[4][NF];
-            for (int i = 0; i < 4; ++i) {
-              for (int f = 0; f < NF; ++f) lane[i][f] = R(-0.0);
-              for (int m = 0; m < 4; ++m)
-                if (act[i][m]) step(k, pix[lp[i][m]], T[i][m], Q[i][m], lane[i]);
-            }
-            add_quad(k, lane);
+        if (!any) break;  // later buckets are needed by no pixel either
+        for (uint32_t k = e; k-- > s;) {
+          for (int l = 0; l < 64; ++l) {
+            for (int f = 0; f < NF; ++f) lane[l][f] = R(-0.0);
+            for (int m = 0; m < 4; ++m)
+              if (act[l][m]) step(k, pix[lp[l][m]], T[l][m], Q[l][m], lane[l]);
           }
+          add_wave(k, lane);
         }
       }
     } else {
-      for (int j = 0; j < 64; ++j) {  // pixels 4j .. 4j+3 (one row, four columns)
-        R T[4], Q[4];
+      for (int w = 0; w < 4; ++w) {  // pixels 64 w .. 64 w + 63: rows 4 w .. 4 w + 3
+        R T[64], Q[64];
         uint32_t top = 0;
-        for (int i = 0; i < 4; ++i) {
-          const Pix& q = pix[4 * j + i];
-          T[i] = q.in ? st.tfinal[q.g] : R(0);
-          Q[i] = q.in ? q.tb : R(0);
+        for (int l = 0; l < 64; ++l) {
+          const Pix& q = pix[64 * w + l];
+          T[l] = q.in ? st.tfinal[q.g] : R(0);
+          Q[l] = q.in ? q.tb : R(0);
           if (q.in) top = std::max(top, q.cnt);
         }
         for (uint32_t k = tile_start + top; k-- > tile_start;) {
-          R lane[4][NF];
-          for (int i = 0; i < 4; ++i) {
-            for (int f = 0; f < NF; ++f) lane[i][f] = R(0);
-            const Pix& q = pix[4 * j + i];
-            if (q.in && k - tile_start < q.cnt) step(k, q, T[i], Q[i], lane[i]);
+          for (int l = 0; l < 64; ++l) {
+            for (int f = 0; f < NF; ++f) lane[l][f] = R(0);
+            const Pix& q = pix[64 * w + l];
+            if (q.in && k - tile_start < q.cnt) step(k, q, T[l], Q[l], lane[l]);
           }
-          add_quad(k, lane);
+          add_wave(k, lane);
         }
       }
     }

@@ -8,7 +8,7 @@
 * Outputs do not depend on the schedule: images, T_final, last
   contributors and per-tile max contributors bit-identical across raster,
   heuristic (every tile split / some split) and learned schedules; the
-  backward's raw sums agree to fp64 reassociation.
+  backward's raw sums agree to reassociation of their cross-wave atomics.
 * Online training on logged tile durations lowers the cost-model loss.
 """
 import numpy as np
@@ -20,7 +20,7 @@ from fastergs_amd._lib import SCHEDULE_FAST
 from fastergs_amd.rasterizer import Frame, RasterOptions
 from fastergs_amd.scheduler import CostModel, OnlineTrainer, TileSchedule, tile_targets
 from oracle import mlp
-from tests.helpers import grad_close, settings_for, to_dev
+from tests.helpers import grad_close, raw_close, settings_for, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -191,7 +191,7 @@ def test_outputs_independent_of_schedule(scene_cam):
         v = fr.view_arrays()
         np.testing.assert_array_equal(v["max_contrib"].cpu().numpy(), rv["max_contrib"].cpu().numpy())
         b = fr.backward(g)
-        grad_close(b["raw"].cpu().numpy(), ref_b["raw"].cpu().numpy(), rtol=1e-10, name=name)
+        raw_close(b["raw"].cpu().numpy(), ref_b["raw"].cpu().numpy(), name=name)
 
 
 def test_online_training_lowers_loss(scene_cam):
