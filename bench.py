@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+COMPOSITE_FLOPS = 21  # per (pixel, splat) evaluation
 
 
 def parse():
@@ -36,6 +37,9 @@ def parse():
     ap.add_argument("--bucket", type=int, default=32)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--compare-plain", type=int, default=1)
+    ap.add_argument("--small", type=int, default=1,
+                    help="also time the small workload (100k, 800x600, bucket 32) on both "
+                         "schedules, with its CPU baseline")
     ap.add_argument("--render-loop", type=int, default=1,
                     help="also run the graph-captured 120-frame orbit of the dense workload")
     ap.add_argument("--multiview", type=int, default=1,
@@ -105,8 +109,8 @@ def main():
         print("traffic probe done", flush=True)
         return
 
-    def timed(schedule: int, steps: int, warmup: int, fwd_only=False):
-        set_default_options(schedule=schedule, bucket_size=args.bucket)
+    def timed(schedule: int, steps: int, warmup: int, fwd_only=False, bucket=None):
+        set_default_options(schedule=schedule, bucket_size=args.bucket if bucket is None else bucket)
         for _ in range(warmup):
             step()
         torch.cuda.synchronize()
@@ -133,6 +137,14 @@ def main():
     plain_ms = None
     if args.compare_plain and sched != SCHEDULE_PLAIN:
         plain_ms, _ = timed(SCHEDULE_PLAIN, max(5, args.steps // 2), 2)
+    # the drop-in default: the reference's 128-splat buckets (its semantics)
+    dropin = None
+    if sched == SCHEDULE_FAST and args.bucket != 128:
+        d_ms, _ = timed(SCHEDULE_FAST, args.steps, 3, bucket=128)
+        d_fwd, _ = timed(SCHEDULE_FAST, args.steps, 2, fwd_only=True, bucket=128)
+        dropin = {"bucket_size": 128, "ms_per_step": round(d_ms, 4), "fwd_ms": round(d_fwd, 4),
+                  "render_fps": round(1000.0 / d_fwd, 2)}
+    set_default_options(schedule=sched, bucket_size=args.bucket)
 
     graph_fps = graph_render_fps(settings, params, sched, args)
     step_graph_ms = None
@@ -145,6 +157,7 @@ def main():
     adam = sparse_adam_step(rast, params)
     ssim = fused_ssim_step(cam)
     lists = list_passes_step(settings, params, args)
+    small = small_config(args) if args.small else None
     loop = render_loop(args) if args.render_loop else None
     torch.cuda.empty_cache()
     mv = multiview_step(args) if args.multiview else None
@@ -180,6 +193,8 @@ def main():
         "ms_per_step_graph": None if step_graph_ms is None else round(step_graph_ms, 4),
         "plain_ms_per_step": None if plain_ms is None else round(plain_ms, 4),
         "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms, 3),
+        "dropin_default": dropin,
+        "small_config": small,
         "roofline": roofline,
         "stages": stages,
         "sparse_adam": adam,
@@ -238,6 +253,12 @@ def kernel_units(name, P, M, V, R, T, E, fast):
         # ~70 flops per active (pixel, splat) in the reverse step (fgs_math.h pix_bwd)
         "backward_fast": ("valu", 70 * E),
         "backward_plain": ("valu", 70 * E),
+        # (pixel, splat) evaluations of compositing, 21 flops each (alpha
+        # from the log2-domain conic: 3 fma + exp2 + min/compare; blend: 3 fma
+        # + mul + sub): the plain pass evaluates each pixel up to its last
+        # contributor (E); the fast pass's executed count comes from its
+        # lists (composite_evaluations)
+        "composite_plain": ("valu", COMPOSITE_FLOPS * E),
     }
     # 64-bit key + 32-bit value, read + write: the pair sort (plain) or the
     # Gaussian depth sort (fast); histograms read the 8 B keys
@@ -248,6 +269,20 @@ def kernel_units(name, P, M, V, R, T, E, fast):
     table["radix_hist_kv"] = ("hbm", 8 * (P if fast else R))
     table["radix_hist_k"] = ("hbm", 8 * R)
     return table.get(name)
+
+
+def composite_evaluations(fr, bucket):
+    """(pixel, splat) evaluations k_composite_fast executes: per tile, its
+    four waves composite four buckets a round, for all 256 pixels, until the
+    round holding the bucket of the tile's furthest contributor."""
+    v = fr.view_arrays()
+    rng = v["ranges"].cpu().numpy().view(np.uint32).reshape(-1, 2).astype(np.int64)
+    maxc = v["max_contrib"].cpu().numpy().view(np.uint32).astype(np.int64)
+    length = rng[:, 1] - rng[:, 0]
+    nb = (length + bucket - 1) // bucket
+    last = np.maximum((maxc + bucket - 1) // bucket - 1, 0)  # bucket of the furthest contributor
+    done = np.minimum(nb, 4 * (last // 4 + 1))               # buckets of the rounds run
+    return int(256 * np.minimum(length, done * bucket).sum())
 
 
 def kernel_roofline(step, params, settings, sched, args, P):
@@ -264,6 +299,7 @@ def kernel_roofline(step, params, settings, sched, args, P):
     V = int((fr.radii > 0).sum().item())
     E = int(fr.last_contributor.sum().item())
     M = int(params["shs"].shape[1])
+    executed = composite_evaluations(fr, args.bucket) if sched == 1 else None
     del fr
     steps = 5
     torch.cuda.synchronize()
@@ -280,6 +316,16 @@ def kernel_roofline(step, params, settings, sched, args, P):
         ent = {"ms_per_iter": round(tot_ms / steps, 4), "launches_per_iter": launches / steps,
                "avg_us": round(avg_s * 1e6, 2)}
         units = kernel_units(name, P, M, V, R, T, E, sched == 1)
+        if name == "composite_fast" and executed is not None:
+            # the reference evaluates every pair for all 256 pixels of its
+            # tile (E_f = 256 R); this pass stops at the round in which the
+            # tile's last pixel stopped, so it executes fewer. The roofline
+            # counts the executed evaluations; the reference-equivalent rate
+            # (E_f per launch time) is reported beside it.
+            ent.update({"evals_reference": 256 * R, "evals_executed": executed,
+                        "reference_equiv_tflops":
+                            round(COMPOSITE_FLOPS * 256 * R / avg_s / 1e12, 2)})
+            units = ("valu", COMPOSITE_FLOPS * executed)
         if units is not None and launches % steps == 0:
             bound, per_launch = units
             ach = per_launch / avg_s / (1e9 if bound == "hbm" else 1e12)
@@ -756,14 +802,65 @@ def learned_schedule_eval(args, train_views=60, eval_views=40, n=None):
             "gain_vs_heuristic": round(float(h.sum() / l.sum()), 4)}
 
 
-def cpu_baseline(scene, cam):
+def small_config(args, steps=20, warmup=5):
+    """The small workload (100k Gaussians uniform in a ball, 800x600, bucket
+    32): fwd+bwd ms/iter through the drop-in API on both schedules, and the
+    CPU oracle's fwd+bwd on the same inputs."""
+    import torch
+    from fastergs_amd import scenes
+    from fastergs_amd._lib import SCHEDULE_FAST, SCHEDULE_PLAIN
+    from fastergs_amd.rasterizer import (GaussianRasterizationSettings, GaussianRasterizer,
+                                         set_default_options)
+    dev = torch.device("cuda:0")
+    scene, cam = scenes.workload("small", seed=0)
+    P = scene["means3D"].shape[0]
+    p = {k: torch.from_numpy(v).to(dev).requires_grad_(True) for k, v in scene.items()}
+    st = GaussianRasterizationSettings(
+        image_height=cam.height, image_width=cam.width, tanfovx=cam.tanfovx,
+        tanfovy=cam.tanfovy, bg=torch.zeros(3, device=dev), scale_modifier=1.0,
+        viewmatrix=torch.from_numpy(cam.viewmatrix).to(dev),
+        projmatrix=torch.from_numpy(cam.projmatrix).to(dev), sh_degree=3,
+        campos=torch.from_numpy(cam.campos).to(dev), mult=1.0, prefiltered=False, debug=False)
+    target = torch.from_numpy(scenes.target_image(cam.width, cam.height)).to(dev)
+    rast = GaussianRasterizer(st)
+
+    def step():
+        for t in p.values():
+            t.grad = None
+        img, _ = rast(means3D=p["means3D"], opacities=p["opacities"], dc=p["dc"], shs=p["shs"],
+                      scales=p["scales"], rotations=p["rotations"])
+        torch.nn.functional.mse_loss(img, target).backward()
+
+    res = {"workload": f"small: {P} Gaussians uniform in a ball, {cam.width}x{cam.height}, SH3, "
+                       "bucket 32"}
+    for name, sch in (("fast", SCHEDULE_FAST), ("plain", SCHEDULE_PLAIN)):
+        set_default_options(schedule=sch, bucket_size=32)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        res[f"{name}_ms_per_step"] = round(e0.elapsed_time(e1) / steps, 4)
+    res["speedup_vs_plain"] = round(res["plain_ms_per_step"] / res["fast_ms_per_step"], 3)
+    set_default_options(schedule=SCHEDULE_FAST, bucket_size=args.bucket)
+    del p
+    if args.cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(scene, cam, iters=5)
+    return res
+
+
+def cpu_baseline(scene, cam, iters=3):
     """CPU oracle (C++, OpenMP) fwd+bwd on the same workload, a few iterations."""
     from oracle.pyoracle import Oracle, num_threads
     from fastergs_amd import scenes
     tgt = scenes.target_image(cam.width, cam.height)
     times = []
     t_end = time.perf_counter() + 20.0
-    while len(times) < 3 and (not times or time.perf_counter() < t_end):
+    while len(times) < iters and (not times or time.perf_counter() < t_end):
         t0 = time.perf_counter()
         o = Oracle(scene, cam, bucket=32, mode=0).forward()
         img = o.image()
@@ -771,8 +868,19 @@ def cpu_baseline(scene, cam):
         times.append((time.perf_counter() - t0) * 1e3)
         del o
     return {"value": round(float(np.median(times)), 1), "unit": "ms/iter",
-            "cores": num_threads(), "kind": "port",
+            "cores": num_threads(), "kind": "port", "cpu": cpu_model(),
             "sample": f"{len(times)} full fwd+bwd iterations of the same workload (median)"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 if __name__ == "__main__":

@@ -47,12 +47,13 @@ def raw_close(got, ref, nf: int = 11, name: str = "raw"):
     """Raw per-Gaussian sums (n x >= nf). The conic sums are added
     order-independently (fp64) from the same float wave totals as the
     oracle's: equal to fp64 reassociation (1e-10 relative). The others are
-    added with float atomics across waves: equal to float reassociation
-    (1e-5 relative, with the rms floor of grad_close)."""
+    added with float atomics across waves: equal to float reassociation,
+    whose error scales with the summed terms rather than with a cancelling
+    sum (1e-4 relative, with the rms floor of grad_close)."""
     got = np.asarray(got, dtype=np.float64).reshape(-1, np.shape(got)[-1])
     ref = np.asarray(ref, dtype=np.float64).reshape(-1, np.shape(ref)[-1])
     for f in range(nf):
-        grad_close(got[:, f], ref[:, f], rtol=1e-10 if f in CONIC_FIELDS else 1e-5,
+        grad_close(got[:, f], ref[:, f], rtol=1e-10 if f in CONIC_FIELDS else 1e-4,
                    name=f"{name}[{f}]")
 
 
