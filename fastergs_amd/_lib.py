@@ -108,6 +108,8 @@ SIGNATURES = {
     "fgs_sort_pairs": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int,
                                       ctypes.c_int, vp, ctypes.c_size_t, vp]),
     "fgs_sort_check": (ctypes.c_int, [vp, ctypes.c_int64, vp]),
+    "fgs_sort_pairs_noalloc": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int,
+                                              ctypes.c_int, vp]),
     "fgs_scan_ws_size": (ctypes.c_size_t, [ctypes.c_int64]),
     "fgs_scan_u32": (ctypes.c_int, [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_size_t, vp]),
     "fgs_mlp_forward": (ctypes.c_int, [vp, vp, ctypes.c_int64, vp, ctypes.c_int, vp]),

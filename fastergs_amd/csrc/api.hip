@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -378,6 +379,66 @@ int fgs_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_d
   if (n < 0 || (n > 0 && (!in || !out))) return fail(FGS_ERR_ARG, "bad scan arguments");
   if (ws_bytes < scan_ws_bytes(n)) return fail(FGS_ERR_CAPACITY, "scan workspace too small");
   FGS_TRY(scan_u32(in, out, n, total_dev, ws, (hipStream_t)stream), "scan");
+  return FGS_OK;
+}
+
+// ------------------------------------------------ no-allocation sort
+// fgs_sort_pairs_noalloc keeps its workspace in a file-scope device array
+// (scratch) instead of a caller buffer, for callers that cannot allocate.
+// Each device holds its own copy (loaded with the library, kept until the
+// process ends); host code uses the address hipGetSymbolAddress returns for
+// the device, never the array's name (a host-side stand-in).
+constexpr int64_t NOALLOC_SORT_MAX_N = int64_t(1) << 24;
+constexpr size_t NOALLOC_SORT_BYTES = size_t(8) << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SORT_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The scratch array's address on the stream's device, looked up once per
+// device with that device current.
+int scratch_on(hipStream_t s, void** out) {
+  int dev = -1, cur = -1;
+  FGS_TRY(hipStreamGetDevice(s, &dev), "hipStreamGetDevice");
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(FGS_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    FGS_TRY(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != dev) FGS_TRY(hipSetDevice(dev), "hipSetDevice");
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    FGS_TRY(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(FGS_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FGS_OK;
+}
+}  // namespace
+
+int fgs_sort_pairs_noalloc(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                           int64_t n, int begin_bit, int end_bit, void* stream) {
+  if (n < 0 || begin_bit < 0 || end_bit > 64 || (n > 0 && (!keys || !keys_alt)))
+    return fail(FGS_ERR_ARG, "bad sort arguments");
+  if ((vals == nullptr) != (vals_alt == nullptr)) return fail(FGS_ERR_ARG, "vals/vals_alt mismatch");
+  if (n > NOALLOC_SORT_MAX_N)
+    return fail(FGS_ERR_CAPACITY, "fgs_sort_pairs_noalloc sorts at most 2^24 pairs");
+  if (sort_classic_ws_bytes(n) > NOALLOC_SORT_BYTES)
+    return fail(FGS_ERR_CAPACITY, "sort scratch too small");
+  hipStream_t s = (hipStream_t)stream;
+  void* ws = nullptr;
+  if (int rc = scratch_on(s, &ws)) return rc;
+  bool alt = false;
+  FGS_TRY(sort_kv_passes<uint64_t>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s,
+                                   alt),
+          "sort");
+  if (alt) {
+    FGS_TRY(hipMemcpyAsync(keys, keys_alt, n * 8, hipMemcpyDeviceToDevice, s), "sort copy");
+    if (vals) FGS_TRY(hipMemcpyAsync(vals, vals_alt, n * 4, hipMemcpyDeviceToDevice, s), "sort copy");
+  }
   return FGS_OK;
 }
 

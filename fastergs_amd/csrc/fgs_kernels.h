@@ -118,6 +118,12 @@ hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int
                    int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
                    const uint32_t* n_dev, uint32_t* last_vals, SortKind kind,
                    uint32_t* sync = nullptr, uint32_t* fault_word = nullptr);
+// The same sort with three-launch passes only (histogram, scan, stable
+// scatter per digit; no look-back waits), for any n. Its workspace is
+// sort_classic_ws_bytes(n), every word of it written before it is read.
+template <typename K>
+hipError_t sort_kv_passes(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
+                          int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt);
 // fill.hip-style helpers (render.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 hipError_t fill_f32(float* p, float v, int64_t n, hipStream_t s);

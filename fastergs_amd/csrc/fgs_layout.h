@@ -90,6 +90,14 @@ inline size_t sort_ws_bytes(int64_t n) {
   return (classic > onesweep ? classic : onesweep) + ALIGN;
 }
 
+// Workspace of the three-launch passes alone (SORT_TILE-key tiles):
+// digit-major histogram rows and the scan partials over them.
+inline size_t sort_classic_ws_bytes(int64_t n) {
+  const size_t nb = (size_t)((n + SORT_TILE - 1) / SORT_TILE);
+  const size_t hist = 256 * (nb > 0 ? nb : 1);
+  return align_up(hist * sizeof(uint32_t)) + scan_ws_bytes((int64_t)hist);
+}
+
 inline SortWs carve_sort(void* ws, int64_t nb) {
   Carver c(ws);
   return carve_sort(c, nb);

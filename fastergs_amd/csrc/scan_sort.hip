@@ -349,8 +349,9 @@ hipError_t expand_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t
 // both slower).
 template <typename K>
 struct SortTiling {
-  static constexpr int ITEMS = 16;
+  static constexpr int ITEMS = SORT_ITEMS;
   static constexpr int TILE = SORT_THREADS * ITEMS;
+  static_assert(TILE == SORT_TILE, "workspace sizes assume SORT_TILE-key tiles");
   static_assert(TILE >= SORT_MIN_TILE, "workspace rows are sized for SORT_MIN_TILE");
 };
 // Digit layout: ceil(bits / 8) passes with the key bits spread evenly over
@@ -876,6 +877,20 @@ hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int
   return sort_kv_classic<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, in_alt,
                             n_dev, last_vals, nm);
 }
+
+template <typename K>
+hipError_t sort_kv_passes(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
+                          int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt) {
+  in_alt = false;
+  if (n <= 1 || end_bit <= begin_bit) return hipSuccess;
+  if (n > (int64_t)ST_CNT || end_bit - begin_bit > 8 * SORT_MAX_PASSES ||
+      end_bit > (int)(8 * sizeof(K)))
+    return hipErrorInvalidValue;
+  return sort_kv_classic<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, in_alt,
+                            nullptr, nullptr, k_sort_names[SORT_PAIRS]);
+}
+template hipError_t sort_kv_passes<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, int64_t,
+                                             int, int, void*, hipStream_t, bool&);
 
 template hipError_t sort_kv<uint16_t>(uint16_t*, uint32_t*, uint16_t*, uint32_t*, int64_t, int,
                                       int, void*, hipStream_t, bool&, const uint32_t*, uint32_t*,

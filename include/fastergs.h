@@ -233,6 +233,18 @@ size_t fgs_sort_ws_size(int64_t n);
 int fgs_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                    int64_t n, int begin_bit, int end_bit, void* ws, size_t ws_bytes,
                    void* stream);
+/* fgs_sort_pairs without a workspace argument: the same result, with the
+ * histograms and scan partials in a static device array of the library
+ * (8 MiB, one per device, held until the process ends) instead of a caller
+ * buffer. No counterpart in the reference: for callers that cannot allocate
+ * (code under graph capture, tight serving loops). n <= 2^24 (else
+ * FGS_ERR_CAPACITY). It runs the three-launch passes only (no waits that
+ * could time out); nothing in it allocates. Every call on one device shares
+ * that device's array: calls that overlap on one device (on different
+ * streams or host threads) race. Call it with the stream's device reachable
+ * (the array's address is looked up on that device on its first call). */
+int fgs_sort_pairs_noalloc(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                           int64_t n, int begin_bit, int end_bit, void* stream);
 /* Status of the last fgs_sort_pairs on workspace ws with n pairs
  * (synchronises the stream): FGS_ERR_HIP if one of its bounded look-back
  * waits gave up (the result is then not sorted), else FGS_OK. The frame-level

@@ -66,3 +66,14 @@ def test_drop_in_import_names():
     assert fields == ("image_height", "image_width", "tanfovx", "tanfovy", "bg",
                       "scale_modifier", "viewmatrix", "projmatrix", "sh_degree", "campos", "mult",
                       "prefiltered", "debug")
+
+
+def test_noalloc_sort_rejects_sizes_beyond_its_scratch():
+    """Host-side check, before any device work: more than 2^24 pairs is a
+    capacity error (the pointers are never touched)."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    rc = L.fgs_sort_pairs_noalloc(fake, fake, fake, fake, (1 << 24) + 1, 0, 45, None)
+    assert rc == -3  # FGS_ERR_CAPACITY
+    assert b"2^24" in L.fgs_last_error()
+    assert L.fgs_sort_pairs_noalloc(None, None, None, None, -1, 0, 45, None) == -1
