@@ -271,3 +271,37 @@ def test_list_sort_fault_is_reported(schedule):
     assert torch.isnan(fr.out_color).all() and torch.isnan(fr.T_final).all()
     with pytest.raises(FasterGSError, match="list sort"):
         fr.read_counts(torch.cuda.current_stream())
+
+
+@pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "fast"])
+def test_densify_stats_closed_form(schedule):
+    """fgs_densify_stats adds, for every Gaussian with radius > 0, the norm of
+    its mean2D gradient (x, y) and of the |.| sums (z, w), one to denom, and
+    takes the max with its radius (backward.cu:892-918), in place on the
+    caller's values; culled Gaussians keep theirs."""
+    scene = scenes.uniform_scene(20000, seed=13)
+    scene["opacities"][:300] = 0.002  # culled
+    cam = scenes.look_at_camera((0.3, 0.0, -4.0), 320, 240)
+    d = to_dev(scene)
+    fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+               dc=d["dc"], shs=d["shs"],
+               options=RasterOptions(schedule=schedule, bucket_size=32)).forward()
+    g = fr.backward(torch.from_numpy(scenes.target_image(320, 240, seed=5)).cuda())
+    P = fr.P
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    init = [torch.rand(P, device="cuda", generator=gen) for _ in range(4)]
+    acc, acc_abs, mr, den = (t.clone() for t in init)
+    fr.densify_stats(g["mean2D"], acc, acc_abs, mr, den)
+    torch.cuda.synchronize()
+    m = g["mean2D"].double().cpu().numpy()
+    r = fr.radii.cpu().numpy()
+    vis = r > 0
+    assert vis.sum() > 1000 and (~vis).sum() >= 300
+    i0, i1, i2, i3 = (t.cpu().numpy() for t in init)
+    np.testing.assert_allclose(acc.cpu().numpy(), i0 + np.where(vis, np.hypot(m[:, 0], m[:, 1]), 0),
+                               rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(acc_abs.cpu().numpy(),
+                               i1 + np.where(vis, np.hypot(m[:, 2], m[:, 3]), 0), rtol=1e-6,
+                               atol=1e-12)
+    np.testing.assert_array_equal(mr.cpu().numpy(), np.where(vis, np.maximum(i2, r.astype(np.float32)), i2))
+    np.testing.assert_array_equal(den.cpu().numpy(), i3 + vis.astype(np.float32))
