@@ -272,16 +272,18 @@ def kernel_units(name, P, M, V, R, T, E, fast):
 
 
 def composite_evaluations(fr, bucket):
-    """(pixel, splat) evaluations k_composite_fast executes: per tile, its
-    four waves composite four buckets a round, for all 256 pixels, until the
-    round holding the bucket of the tile's furthest contributor."""
+    """(pixel, splat) evaluations k_composite_fast executes on whole tiles:
+    rounds of 1, 2, then 4 buckets, each for all 256 pixels, up to the round
+    holding the bucket of the tile's furthest contributor."""
     v = fr.view_arrays()
     rng = v["ranges"].cpu().numpy().view(np.uint32).reshape(-1, 2).astype(np.int64)
     maxc = v["max_contrib"].cpu().numpy().view(np.uint32).astype(np.int64)
     length = rng[:, 1] - rng[:, 0]
     nb = (length + bucket - 1) // bucket
     last = np.maximum((maxc + bucket - 1) // bucket - 1, 0)  # bucket of the furthest contributor
-    done = np.minimum(nb, 4 * (last // 4 + 1))               # buckets of the rounds run
+    # buckets after the rounds run: 1, 3, 7, 11, ... up to the first above `last`
+    ramp = np.where(last < 1, 1, np.where(last < 3, 3, 3 + 4 * ((last - 3) // 4 + 1)))
+    done = np.minimum(nb, ramp)
     return int(256 * np.minimum(length, done * bucket).sum())
 
 

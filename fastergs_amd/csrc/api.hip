@@ -118,6 +118,10 @@ bool plain_sorted_in_alt(int T) {
   return (passes & 1) != 0;
 }
 
+// Whether a frame keeps what the backward reads (bucket checkpoints and
+// per-pixel colours): not when rendered forward-only.
+bool keeps_ckpt(const fgs_options* o) { return !(o && o->forward_only); }
+
 int check_opts(const fgs_options* o) {
   if (!o) return fail(FGS_ERR_ARG, "options is NULL");
   if (o->schedule != FGS_SCHEDULE_PLAIN && o->schedule != FGS_SCHEDULE_FAST)
@@ -338,8 +342,9 @@ size_t fgs_gauss_ws_size(int P, int width, int height, const fgs_options* opt) {
 size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
                         const fgs_options* opt) {
   (void)P;
-  const bool ck = opt && opt->schedule == FGS_SCHEDULE_FAST;
-  return carve_pair(nullptr, R, NB, (int64_t)width * height, tiles_of(width, height), ck).total;
+  const bool fast = opt && opt->schedule == FGS_SCHEDULE_FAST;
+  return carve_pair(nullptr, R, NB, (int64_t)width * height, tiles_of(width, height), fast,
+                    keeps_ckpt(opt)).total;
 }
 
 // raw sums (P x 4 doubles + P x 8 floats) and the split SH pass's mean term
@@ -362,7 +367,7 @@ size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t 
   (void)opt;
   Batch b;
   if (make_batch(batch, b) != FGS_OK) return 0;
-  return carve_pair(nullptr, R, NB, b.HW, b.T, true).total;
+  return carve_pair(nullptr, R, NB, b.HW, b.T, true, keeps_ckpt(opt)).total;
 }
 
 size_t fgs_batch_grad_ws_size(int P, int n_views) {
@@ -593,7 +598,7 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   const GaussWs gw = carve_gauss(gauss_ws, (int64_t)g->P * b.V, T);
   if (!gauss_ws || gauss_ws_bytes < gw.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast, keeps_ckpt(opt));
   if (!pair_ws || pair_ws_bytes < pw.total) return fail(FGS_ERR_CAPACITY, "pair workspace too small");
   hipStream_t s = (hipStream_t)stream;
   const fgs_camera* cam = &b.cams[0];
@@ -719,13 +724,15 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
     return rc;
   if (!out || !grad_ws || !dL_dimage || !T_final || !last_contributor)
     return fail(FGS_ERR_ARG, "NULL argument");
+  if (opt->forward_only)
+    return fail(FGS_ERR_ARG, "the frame was rendered forward-only (no checkpoints to differentiate)");
   if (g->P == 0) return FGS_OK;
   const fgs_camera* cam = &b.cams[0];
   const int T = b.T;
   const int64_t PV = (int64_t)g->P * b.V;
   const GaussWs gw = carve_gauss(gauss_ws, PV, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast, keeps_ckpt(opt));
   hipStream_t s = (hipStream_t)stream;
   // raw sums: PV conic rows (doubles), then PV rows of the other fields
   // (floats), then the split SH pass's P x 4 floats; cleared beside the
@@ -830,7 +837,7 @@ int fgs_batch_frame_view_get(void* gauss_ws, void* pair_ws, int P, const fgs_bat
   const int T = b.T;
   const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast, keeps_ckpt(opt));
   *v = fgs_frame_view{};
   v->geom = gw.geom;
   v->splat = gw.splat;
@@ -924,7 +931,7 @@ int fgs_batch_bucket_tables(void* gauss_ws, void* pair_ws, int P, const fgs_batc
   if (P == 0 || NB == 0) return FGS_OK;
   const int T = b.T;
   const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T);
-  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, true);
+  const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, true, keeps_ckpt(opt));
   FGS_TRY(launch_bucket_tables(T, NB, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
                                pw.bucket_tile, (hipStream_t)stream),
           "bucket tables");
@@ -964,7 +971,7 @@ int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height,
   const int T = tiles_of(width, height);
   const GaussWs gw = carve_gauss(gauss_ws, P, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast, keeps_ckpt(opt));
   const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);
   hipStream_t s = (hipStream_t)stream;
   FGS_TRY(fill_f32(accum_weights, 0.0f, P, s), "zero weights");
@@ -992,7 +999,7 @@ int fgs_render_depth(void* gauss_ws, void* pair_ws, int P, int width, int height
   const int T = tiles_of(width, height);
   const GaussWs gw = carve_gauss(gauss_ws, P, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
-  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast);
+  const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast, keeps_ckpt(opt));
   const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);
   FGS_TRY(launch_render_depth(width, height, T, gw.ranges, vals, gw.splat, means3D, scales,
                               rotations, projmatrix, campos, out_pts, out_depth, accum_alpha,

@@ -217,7 +217,10 @@ struct PairWs {
 };
 
 // T: tiles of all views (the width of the fast schedule's tile ids).
-inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int64_t HW, int64_t T, bool fast) {
+// ckpt: the bucket checkpoints (FAST) and per-pixel colours the backward
+// reads; a forward-only frame has neither.
+inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int64_t HW, int64_t T, bool fast,
+                         bool ckpt = true) {
   Carver c(base);
   PairWs w{};
   const size_t r = (size_t)(R > 0 ? R : 1);
@@ -241,13 +244,13 @@ inline PairWs carve_pair(void* base, int64_t R, int64_t NB, int64_t HW, int64_t 
   if (fast) {
     w.bucket_ranges = c.take<uint32_t>(2 * nb);
     w.bucket_tile = c.take<uint32_t>(nb);
-    w.acc = c.take<float>(nb * 256 * 4);
+    w.acc = ckpt ? c.take<float>(nb * 256 * 4) : nullptr;
   } else {
     w.bucket_ranges = nullptr;
     w.bucket_tile = nullptr;
     w.acc = nullptr;
   }
-  w.pixc = c.take<float>((size_t)(HW > 0 ? HW : 1) * 4);
+  w.pixc = ckpt ? c.take<float>((size_t)(HW > 0 ? HW : 1) * 4) : nullptr;
   w.total = c.off;
   return w;
 }

@@ -72,6 +72,7 @@ class CapturedRenderer:
         self.device = dev
         self.opts = options or RasterOptions(schedule=_lib.SCHEDULE_FAST)
         self.copts = self.opts.to_c(False)
+        self.copts.forward_only = 1  # rendering only: no bucket checkpoints
         self.P = P = int(means3D.shape[0])
         self.W, self.H = int(settings.image_width), int(settings.image_height)
         self.T = ((self.W + 15) // 16) * ((self.H + 15) // 16)

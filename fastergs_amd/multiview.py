@@ -33,7 +33,7 @@ class BatchFrame:
 
     def __init__(self, settings_list, means3D, opacities, scales, rotations, dc=None, shs=None,
                  colors_precomp=None, options: RasterOptions | None = None,
-                 abs_grad: bool = True):
+                 abs_grad: bool = True, forward_only: bool = False):
         if not 1 <= len(settings_list) <= _lib.MAX_VIEWS:
             raise ValueError(f"a batch holds 1 to {_lib.MAX_VIEWS} views")
         dev = means3D.device
@@ -42,6 +42,8 @@ class BatchFrame:
         if self.opts.schedule != _lib.SCHEDULE_FAST:
             raise ValueError("batches of views use the FAST schedule")
         self.copts = self.opts.to_c(abs_grad)
+        self.forward_only = bool(forward_only)  # no bucket checkpoints: not differentiable
+        self.copts.forward_only = 1 if forward_only else 0
         self.P = P = int(means3D.shape[0])
         self.V = V = len(settings_list)
         self.means3D = _f32(means3D, dev).reshape(P, 3)
@@ -134,6 +136,8 @@ class BatchFrame:
         """Gradients of the batch; `skip` may name "conic", "colors" and
         "cov3D", which are then neither computed into memory nor returned."""
         L, dev, P, V = lib(), self.device, self.P, self.V
+        if self.forward_only:
+            raise ValueError("the batch was rendered forward-only")
         if not set(skip) <= {"conic", "colors", "cov3D"}:
             raise ValueError("only conic, colors and cov3D gradients can be skipped")
         g = _f32(grad_flat, dev).reshape(-1)
@@ -196,8 +200,8 @@ class _RasterizeViews(torch.autograd.Function):
         settings_list, options = frame_args
         want_abs = stats[1] is not None and stats[1].numel() > 0
         frame = BatchFrame(settings_list, means3D, opacities, scales, rotations, dc=dc, shs=shs,
-                           colors_precomp=colors_precomp, options=options,
-                           abs_grad=want_abs).forward()
+                           colors_precomp=colors_precomp, options=options, abs_grad=want_abs,
+                           forward_only=not any(ctx.needs_input_grad[:7])).forward()
         ctx.frame = frame
         ctx.shapes = [t.shape if torch.is_tensor(t) else None for t in
                       (means3D, dc, shs, colors_precomp, opacities, scales, rotations)]

@@ -129,13 +129,17 @@ class Frame:
 
     def __init__(self, settings: GaussianRasterizationSettings, means3D, opacities, scales,
                  rotations, dc=None, shs=None, colors_precomp=None, culling=None,
-                 options: RasterOptions | None = None, abs_grad: bool = True):
+                 options: RasterOptions | None = None, abs_grad: bool = True,
+                 forward_only: bool = False):
         if not means3D.is_cuda:
             raise ValueError("means3D must be a GPU tensor")
         dev = means3D.device
         self.device = dev
         self.opts = options or _default_options
         self.copts = self.opts.to_c(abs_grad)
+        # a frame that will not be differentiated keeps no bucket checkpoints
+        self.forward_only = bool(forward_only)
+        self.copts.forward_only = 1 if forward_only else 0
         self.P = P = int(means3D.shape[0])
         self.W, self.H = int(settings.image_width), int(settings.image_height)
         self.settings = settings
@@ -249,6 +253,8 @@ class Frame:
         L, dev, P = lib(), self.device, self.P
         if not set(skip) <= {"conic", "colors", "cov3D"}:
             raise ValueError("only conic, colors and cov3D gradients can be skipped")
+        if self.forward_only:
+            raise ValueError("the frame was rendered forward-only")
         g = _f32(grad_out_color, dev).reshape(3, self.H, self.W)
         e = lambda *s: torch.empty(s, dtype=torch.float32, device=dev)
         shapes = {"mean2D": (P, 4), "conic": (P, 2, 2), "opacities": (P, 1), "colors": (P, 3),
@@ -396,8 +402,11 @@ class _RasterizeGaussians(torch.autograd.Function):
                                                              rotations.numel() == 0)):
             raise ValueError("scales and rotations are required")
         want_abs = xyz_gradient_accum_abs is not None and xyz_gradient_accum_abs.numel() > 0
+        # no input asks for a gradient (e.g. under torch.no_grad): nothing to keep for a backward
+        fwd_only = not any(ctx.needs_input_grad[:8])
         frame = Frame(raster_settings, means3D, opacities, scales, rotations, dc=dc, shs=shs,
-                      colors_precomp=colors_precomp, culling=culling, abs_grad=want_abs).forward()
+                      colors_precomp=colors_precomp, culling=culling, abs_grad=want_abs,
+                      forward_only=fwd_only).forward()
         ctx.frame = frame
         ctx.shapes = [t.shape if torch.is_tensor(t) else None for t in
                       (means3D, dc, shs, colors_precomp, opacities, scales, rotations,
@@ -482,7 +491,7 @@ class GaussianRasterizer(nn.Module):
         if scales is None or rotations is None:
             raise ValueError("scales and rotations are required")
         fr = Frame(self.raster_settings, means3D, opacities, scales, rotations, dc=dc, shs=shs,
-                   colors_precomp=colors_precomp, culling=culling).forward()
+                   colors_precomp=colors_precomp, culling=culling, forward_only=True).forward()
         w, c, m = fr.render_simp()
         return fr.out_color, fr.radii, w, c, m
 
@@ -503,7 +512,7 @@ class GaussianRasterizer(nn.Module):
         if scales is None or rotations is None:
             raise ValueError("scales and rotations are required")
         fr = Frame(self.raster_settings, means3D, opacities, scales, rotations, dc=dc, shs=shs,
-                   colors_precomp=colors_precomp, culling=culling).forward()
+                   colors_precomp=colors_precomp, culling=culling, forward_only=True).forward()
         res = fr.render_depth()
         res["render"] = fr.out_color
         res["radii"] = fr.radii

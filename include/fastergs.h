@@ -102,6 +102,10 @@ typedef struct fgs_options {
   int split_tiles;  /* fast preprocess: rects with >= this many tiles are split */
   int abs_grad;     /* backward accumulates |dL/dmean2D| terms (needed for xyz_gradient_accum_abs) */
   const fgs_tile_sched* tile_sched; /* FAST compositing schedule; NULL = RASTER */
+  int forward_only; /* the frame will not be differentiated: compositing stores no bucket
+                       checkpoints or per-pixel colours and the pair workspace holds none
+                       (pass the same value to every call on the frame; fgs_backward then
+                       returns FGS_ERR_ARG) */
 } fgs_options;
 
 /* Device pointers into the two workspaces, for inspection by tests and

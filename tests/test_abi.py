@@ -77,3 +77,16 @@ def test_noalloc_sort_rejects_sizes_beyond_its_scratch():
     assert rc == -3  # FGS_ERR_CAPACITY
     assert b"2^24" in L.fgs_last_error()
     assert L.fgs_sort_pairs_noalloc(None, None, None, None, -1, 0, 45, None) == -1
+
+
+def test_forward_only_pair_workspace_is_smaller():
+    """A forward-only frame's pair workspace holds no bucket checkpoints
+    (NB x 256 x 16 B) and no per-pixel colours (H x W x 16 B)."""
+    L = _lib.lib()
+    full = _lib.Options(1, 32, 64, 1)
+    fo = _lib.Options(1, 32, 64, 1)
+    fo.forward_only = 1
+    R, NB, W, H = 500000, 20000, 640, 480
+    a = L.fgs_pair_ws_size(1000, W, H, R, NB, ctypes.byref(full))
+    b = L.fgs_pair_ws_size(1000, W, H, R, NB, ctypes.byref(fo))
+    assert a - b >= NB * 256 * 16 + W * H * 16

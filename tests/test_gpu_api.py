@@ -190,16 +190,15 @@ def test_speculative_capacity_matches_exact(schedule):
         np.testing.assert_array_equal(fr.view_arrays()["sorted_vals"].cpu().numpy(),
                                       ref.view_arrays()["sorted_vals"].cpu().numpy())
         for k in ("opacities", "dc", "shs", "means3D", "scales", "rotations"):
-            # deterministic raw sums: only fp64 reassociation of the atomics differs
-            grad_close(g[k], gref[k], rtol=1e-6, name=k)
+            # run-to-run: the float atomics of the linear raw sums reassociate
+            grad_close(g[k], gref[k], rtol=1e-5, name=k)
     assert rz._capacity_hints  # the forward remembers the shape's counts
 
 
 @pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "fast"])
 def test_backward_skips_unrequested_gradients(schedule):
     """Skipped outputs (conic, colors, cov3D) are not returned and the others
-    equal the full backward's (the raw sums are deterministic up to fp64
-    reassociation of the atomics)."""
+    equal the full backward's to the reassociation of the float atomics."""
     scene = scenes.uniform_scene(8000, seed=9)
     cam = scenes.look_at_camera((0.0, 0.3, -4.0), 200, 160)
     d = to_dev(scene)
@@ -209,8 +208,9 @@ def test_backward_skips_unrequested_gradients(schedule):
     full = fr.backward(gimg)
     part = fr.backward(gimg, skip=("conic", "colors", "cov3D"))
     assert not {"conic", "colors", "cov3D"} & set(part)
+    # float atomics of the colour, mean2D and power sums reassociate run to run
     for k in ("mean2D", "opacities", "dc", "shs", "means3D", "scales", "rotations"):
-        grad_close(part[k].cpu().numpy(), full[k].cpu().numpy(), rtol=1e-6, name=k)
+        grad_close(part[k].cpu().numpy(), full[k].cpu().numpy(), rtol=1e-5, name=k)
     with pytest.raises(ValueError):
         fr.backward(gimg, skip=("means3D",))
 
@@ -305,3 +305,35 @@ def test_densify_stats_closed_form(schedule):
                                atol=1e-12)
     np.testing.assert_array_equal(mr.cpu().numpy(), np.where(vis, np.maximum(i2, r.astype(np.float32)), i2))
     np.testing.assert_array_equal(den.cpu().numpy(), i3 + vis.astype(np.float32))
+
+
+@pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "fast"])
+def test_forward_only_frames(schedule):
+    """A frame rendered forward-only (no input needs a gradient, e.g. under
+    torch.no_grad) keeps no bucket checkpoints or per-pixel colours, yet its
+    image, T_final, contributors and lists equal a full frame's; it cannot be
+    differentiated."""
+    scene = scenes.cluster_scene(30000, seed=21)
+    cam = scenes.look_at_camera((0.1, 0.2, -4.0), 352, 288)
+    d = to_dev(scene)
+    opts = RasterOptions(schedule=schedule, bucket_size=32)
+    full = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+                 dc=d["dc"], shs=d["shs"], options=opts).forward()
+    fo = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+               dc=d["dc"], shs=d["shs"], options=opts, forward_only=True).forward()
+    torch.cuda.synchronize()
+    for k in ("out_color", "T_final", "last_contributor", "radii"):
+        assert torch.equal(getattr(fo, k), getattr(full, k)), k
+    a, b = fo.view_arrays(), full.view_arrays()
+    assert torch.equal(a["sorted_vals"], b["sorted_vals"]) and torch.equal(a["ranges"], b["ranges"])
+    assert "acc_c" not in a and "pixel_colors" not in a
+    assert fo.pws.numel() < full.pws.numel()
+    with pytest.raises(ValueError, match="forward-only"):
+        fo.backward(torch.ones_like(fo.out_color))
+    # the drop-in API: under no_grad the forward is forward-only, same image
+    set_default_options(schedule=schedule, bucket_size=32)
+    ras = GaussianRasterizer(settings_for(cam))
+    with torch.no_grad():
+        img, _ = ras(means3D=d["means3D"], opacities=d["opacities"], dc=d["dc"], shs=d["shs"],
+                     scales=d["scales"], rotations=d["rotations"])
+    assert torch.equal(img, full.out_color)
