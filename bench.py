@@ -154,6 +154,7 @@ def main():
         step_graph_ms = captured_step_ms([settings], params, [target],
                                          RasterOptions(schedule=sched, bucket_size=args.bucket),
                                          steps=args.steps, stats=True)
+    fma_tflops = fma_peak()
     adam = sparse_adam_step(rast, params)
     ssim = fused_ssim_step(cam)
     lists = list_passes_step(settings, params, args)
@@ -196,6 +197,7 @@ def main():
         "dropin_default": dropin,
         "small_config": small,
         "roofline": roofline,
+        "fp32_fma_attainable_tflops": fma_tflops,
         "stages": stages,
         "sparse_adam": adam,
         "fused_ssim": ssim,
@@ -358,6 +360,26 @@ def measured_traffic(kernel, args):
     with open(path) as f:
         k = json.load(f)["kernels"].get(kernel)
     return None if k is None else k["traffic_bytes"]
+
+
+def fma_peak(blocks=4096, iters=4096, reps=5):
+    """Attainable non-packed FP32 FMA rate of this GPU (fgs_fma_probe: 8
+    independent FMA chains per thread), TFLOP/s, beside the published peak."""
+    import ctypes
+    import torch
+    from fastergs_amd._lib import check, lib
+    out = torch.empty(blocks * 256, device="cuda")
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = ctypes.c_void_p(out.data_ptr())
+    check(lib().fgs_fma_probe(p, blocks, iters, s), "fgs_fma_probe")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        check(lib().fgs_fma_probe(p, blocks, iters, s), "fgs_fma_probe")
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / reps / 1e3
+    return round(16.0 * iters * blocks * 256 / sec / 1e12, 2)
 
 
 def fused_ssim_step(cam, reps=20):
@@ -849,6 +871,13 @@ def small_config(args, steps=20, warmup=5):
         res[f"{name}_ms_per_step"] = round(e0.elapsed_time(e1) / steps, 4)
     res["speedup_vs_plain"] = round(res["plain_ms_per_step"] / res["fast_ms_per_step"], 3)
     set_default_options(schedule=SCHEDULE_FAST, bucket_size=args.bucket)
+    # the same step replayed from one HIP graph: GPU time without host overhead
+    from fastergs_amd.rasterizer import RasterOptions
+    for t in p.values():
+        t.grad = None
+    res["fast_ms_per_step_graph"] = round(
+        captured_step_ms([st], p, [target], RasterOptions(schedule=SCHEDULE_FAST, bucket_size=32),
+                         steps=steps), 4)
     del p
     if args.cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(scene, cam, iters=5)

@@ -98,6 +98,7 @@ SIGNATURES = {
                                     vp]),
     "fgs_densify_stats": (ctypes.c_int, [ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]),
     "fgs_rcp_selftest": (ctypes.c_int, [vp, vp]),
+    "fgs_fma_probe": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp]),
     "fgs_frame_view_get": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int64, ctypes.c_int64,
                                           ctypes.POINTER(Options), ctypes.POINTER(FrameView)]),

@@ -811,6 +811,12 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   return FGS_OK;
 }
 
+int fgs_fma_probe(float* out, int blocks, int iters, void* stream) {
+  if (!out || blocks < 1 || iters < 1) return fail(FGS_ERR_ARG, "bad probe arguments");
+  FGS_TRY(launch_fma_probe(out, blocks, iters, (hipStream_t)stream), "fma probe");
+  return FGS_OK;
+}
+
 int fgs_rcp_selftest(unsigned long long* mismatches, void* stream) {
   if (!mismatches) return fail(FGS_ERR_ARG, "mismatches is NULL");
   FGS_TRY(launch_rcp_selftest(mismatches, (hipStream_t)stream), "rcp selftest");

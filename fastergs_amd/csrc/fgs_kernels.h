@@ -127,6 +127,8 @@ hipError_t sort_kv_passes(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_a
 // fill.hip-style helpers (render.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 hipError_t fill_f32(float* p, float v, int64_t n, hipStream_t s);
+// 8 x iters FMAs per thread of blocks x 256 threads; out: blocks x 256 floats.
+hipError_t launch_fma_probe(float* out, int blocks, int iters, hipStream_t s);
 
 // preprocess.hip
 struct PreArgs {

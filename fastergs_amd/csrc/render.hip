@@ -30,6 +30,24 @@ hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
   if (n > 0) k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, v, n);
   return hipGetLastError();
 }
+// Attainable non-packed FP32 FMA rate: 8 independent FMA chains per thread,
+// `iters` steps each (for measurement runs; out gets one value per thread so
+// the chains are not removed).
+__global__ __launch_bounds__(256) void k_fma_probe(float* __restrict__ out, int iters, float a,
+                                                   float b) {
+  float x0 = (float)threadIdx.x, x1 = x0 + 1.f, x2 = x0 + 2.f, x3 = x0 + 3.f;
+  float x4 = x0 + 4.f, x5 = x0 + 5.f, x6 = x0 + 6.f, x7 = x0 + 7.f;
+  for (int i = 0; i < iters; ++i) {
+    x0 = fmaf(x0, a, b); x1 = fmaf(x1, a, b); x2 = fmaf(x2, a, b); x3 = fmaf(x3, a, b);
+    x4 = fmaf(x4, a, b); x5 = fmaf(x5, a, b); x6 = fmaf(x6, a, b); x7 = fmaf(x7, a, b);
+  }
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7));
+}
+hipError_t launch_fma_probe(float* out, int blocks, int iters, hipStream_t s) {
+  if (blocks > 0) k_fma_probe<<<blocks, 256, 0, s>>>(out, iters, 0.999f, 0.001f);
+  return hipGetLastError();
+}
+
 hipError_t fill_f32(float* p, float v, int64_t n, hipStream_t s) {
   if (n > 0) k_fill_f32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, v, n);
   return hipGetLastError();

@@ -266,6 +266,12 @@ int fgs_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_d
  * backward's 1/x differs from the IEEE quotient; 0 on a correct device. */
 int fgs_rcp_selftest(unsigned long long* mismatches, void* stream);
 
+/* Measurement probe of the attainable non-packed FP32 FMA rate: blocks x
+ * 256 threads each run 8 independent chains of iters fused multiply-adds
+ * (16 x iters flops per thread); out receives blocks x 256 floats. Time it
+ * with events on the stream. */
+int fgs_fma_probe(float* out, int blocks, int iters, void* stream);
+
 /* Cost-model kernels. fgs_mlp_forward: y (n x 2) from x (n x 8), bf16 != 0
  * rounds weights and activations to bfloat16 (fp32 accumulation).
  * fgs_mlp_loss_grad: loss = mean over rows of (y0 - t0)^2 + split_weight *
