@@ -247,9 +247,10 @@ def kernel_units(name, P, M, V, R, T, E, fast):
         # reads the 2 B ids, each pass reads and writes 6 B per pair
         "tile_hist": ("hbm", 2 * R),
         "tile_scatter": ("hbm", 12 * R),
-        # depth sort of the records: 4 B key + 4 B index, read + write
-        "depth_onesweep": ("hbm", 16 * P),
-        "depth_hist_all": ("hbm", 4 * P),
+        # depth sort of the records: 8 B key (depth | record) + 4 B pair
+        # count, read + write per pass; the histogram reads the keys
+        "depth_onesweep": ("hbm", 24 * P),
+        "depth_hist_all": ("hbm", 8 * P),
         # sorted packed keys -> lists (4 B) and ranges
         "unpack_ranges": ("hbm", 12 * R + 8 * T),
         # ~70 flops per active (pixel, splat) in the reverse step (fgs_math.h pix_bwd)

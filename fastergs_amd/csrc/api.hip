@@ -533,29 +533,31 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   if (g->P == 0) return FGS_OK;
   // Independent chains after the geometry pass: on a side stream the depth
   // sort of the records (queued first: its workgroups take their places
-  // before the tile counts fill the GPU), then the SH colours (read only by
-  // compositing); on the main stream the large-rect compaction and tile
-  // counts. The pair offsets wait for the depth sort only.
+  // before the tile counts fill the GPU); on the main stream the large-rect
+  // compaction and tile counts, then the SH colours (read only by
+  // compositing). The pair offsets wait for both.
   const Fork f(s);
   FGS_TRY(f.fork(1, 1), "fork");
   // records in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by (global) tile alone yields the canonical per-tile
   // order of every view
   bool alt = false;
-  FGS_TRY(sort_kv<uint32_t>(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, PV, 0, 32, w.scratch,
-                            f.side(1), alt, nullptr, nullptr, SORT_DEPTH, w.dsort_sync),
+  FGS_TRY(sort_kv<uint64_t>(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, PV, 32, 64,
+                            w.scratch, f.side(1), alt, nullptr, nullptr, SORT_DEPTH, w.dsort_sync),
           "depth sort");
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
   FGS_TRY(f.mark(1, 2), "mark");
-  FGS_TRY(launch_sh_color(a, f.side(1)), "colour");
   // large rects, one list entry per row band (count in counts[3]), each band
-  // walked by a whole wave
+  // walked by a whole wave; then the SH colours, which fill the CUs the
+  // latency-bound depth sort (a few workgroups per pass) leaves idle
   FGS_TRY(expand_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large bands");
   FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
-                             w.counts, w.pcount, nullptr, w.lband, 0u, nullptr, nullptr, false, s),
+                             w.counts, w.pcount, nullptr, nullptr, w.lband, 0u, nullptr, nullptr,
+                             false, s),
           "large tiles");
+  FGS_TRY(launch_sh_color(a, s), "colour");
   FGS_TRY(f.wait(2), "join");
-  FGS_TRY(scan_pair_offsets(w.pcount, w.dsort_v0, PV, w.rcount, w.poff, w.counts,
+  FGS_TRY(scan_pair_offsets(w.dsort_k0, w.dsort_v0, w.pcount, w.lrank, PV, w.poff, w.counts,
                             opt->bucket_size, b.T, w.scratch, s),
           "pair offsets");
   FGS_TRY(f.join(1, 3), "join");
@@ -637,12 +639,13 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
     // small and large rects write disjoint pair blocks: side by side
     const Fork f(s);
     FGS_TRY(f.fork(0, 4), "fork");
-    FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.dsort_v0, gw.poff,
+    FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.dsort_k0, gw.poff,
                               (uint32_t)R,
                               pw.t0, pw.r0, pw.tile16, f.side(0)),
             "emit");
     FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts, nullptr,
-                               gw.poff, gw.lband, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s),
+                               gw.poff, gw.lrank, gw.lband, (uint32_t)R, pw.t0, pw.r0,
+                               pw.tile16, s),
             "emit large");
     FGS_TRY(f.join(0, 5), "join");
     // stable sort of (tile id, record) by tile id: the last pass writes the

@@ -89,12 +89,16 @@ __device__ __forceinline__ int view_of_tile(const ViewRec* views, int V, uint32_
 // scan_sort.hip
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
                     hipStream_t s);
-// Pair offsets of the FAST preprocess: off[order[r]] = exclusive prefix of
-// count[order[r]] over r (records in depth order), counts[0] = the total,
-// counts[1] = ceil(total / bucket) + T; gathered is n words of scratch.
-hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64_t n,
-                             uint32_t* gathered, uint32_t* off, uint32_t* counts, int bucket,
-                             int T, void* ws, hipStream_t s);
+// Pair offsets of the FAST preprocess by depth rank r, from the depth
+// sort's keys (depth bits << 32 | record) and values (the record's pair
+// count, or PAIRS_LARGE for a large rect: its count is read from count[] and
+// its rank written to lrank[record]): off[r] = exclusive prefix over ranks,
+// counts[0] = the total, counts[1] = ceil(total / bucket) + T. vals is
+// overwritten with the resolved counts.
+constexpr uint32_t PAIRS_LARGE = 0x80000000u;
+hipError_t scan_pair_offsets(const uint64_t* keys, uint32_t* vals, const uint32_t* count,
+                             uint32_t* lrank, int64_t n, uint32_t* off, uint32_t* counts,
+                             int bucket, int T, void* ws, hipStream_t s);
 // ids = i * LARGE_BANDS + b for every index i < n and b < flag[i] (flags
 // 0..LARGE_BANDS), ascending; *count = their number (on the device).
 hipError_t expand_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t* count, void* ws,
@@ -142,8 +146,8 @@ struct PreArgs {
   int P;
   int split_tiles;
   uint32_t* lflag;     // fast: row bands of a large rect (walked by whole waves), 0 if small
-  uint32_t* dsort_k;   // fast: depth keys for the Gaussian depth sort
-  uint32_t* dsort_v;   // fast: Gaussian indices for the depth sort
+  uint64_t* dsort_k;   // fast: depth sort keys, depth bits << 32 | record
+  uint32_t* dsort_v;   // fast: depth sort values, small-rect pair count or PAIRS_LARGE
   uint32_t* counts;
   const ViewRec* views;  // fast: V views (records are V x P)
   int V;
@@ -165,12 +169,15 @@ hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey
 // FAST emission over V views of P Gaussians (records v * P + i) as (tile
 // id, record) into two arrays: tile ids are global, 16-bit when t16 (at
 // most 65536 tiles), else 32-bit.
+// poff: pair offsets by depth rank; emission reads order (the sorted depth
+// keys, record in the low 32 bits) for the small rects and lrank (the rank
+// of each large record) for the large ones.
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t* lband, uint32_t cap, void* tkeys,
-                              uint32_t* recs, bool t16, hipStream_t s);
+                              const uint32_t* poff, const uint32_t* lrank, uint32_t* lband,
+                              uint32_t cap, void* tkeys, uint32_t* recs, bool t16, hipStream_t s);
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
-                             const uint32_t* order, const uint32_t* poff, uint32_t cap,
+                             const uint64_t* order, const uint32_t* poff, uint32_t cap,
                              void* tkeys, uint32_t* recs, bool t16, hipStream_t s);
 
 // render.hip

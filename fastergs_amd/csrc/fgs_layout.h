@@ -123,11 +123,11 @@ struct GaussWs {
   float* splat;
   uint32_t* dkey;
   uint32_t* pcount;
-  uint32_t* poff;
-  uint32_t* dsort_k0;  // fast: depth keys, sorted in place
-  uint32_t* dsort_k1;
-  uint32_t* dsort_v0;  // fast: Gaussian index by depth rank after the sort
-  uint32_t* dsort_v1;
+  uint32_t* poff;      // pair offsets: by record (PLAIN), by depth rank (FAST)
+  uint64_t* dsort_k0;  // fast: depth keys (depth bits << 32 | record), sorted in place:
+  uint64_t* dsort_k1;  //   the records by depth rank
+  uint32_t* dsort_v0;  // fast: small-rect pair count or PAIRS_LARGE, sorted with the keys,
+  uint32_t* dsort_v1;  //   then the resolved counts by depth rank
   uint32_t* lflag;     // fast: row bands of a large rect (walked by whole waves), 0 if small
   uint32_t* large_ids; // fast: row bands of large rects (record * LARGE_BANDS + band), in order
   uint32_t* lband;     // fast: hits of each listed band
@@ -136,7 +136,7 @@ struct GaussWs {
   uint32_t* bcount;    // T + 1 scratch (bucket counts, then active bucket counts)
   uint32_t* actoff;    // T + 1: exclusive prefix of per-tile active buckets, [T] = total
   uint32_t* maxc;      // T
-  uint32_t* rcount;    // fast: pair counts in depth order
+  uint32_t* lrank;     // fast: depth rank of each large-rect record
   uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects, [4] work items,
                        // [5] list-sort fault (COUNT_LIST_FAULT), [6] arrival counter
                        // (COUNT_ARRIVE_RANGES)
@@ -164,8 +164,8 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   w.dkey = c.take<uint32_t>(p);
   w.pcount = c.take<uint32_t>(p);
   w.poff = c.take<uint32_t>(p + 1);
-  w.dsort_k0 = c.take<uint32_t>(p);
-  w.dsort_k1 = c.take<uint32_t>(p);
+  w.dsort_k0 = c.take<uint64_t>(p);
+  w.dsort_k1 = c.take<uint64_t>(p);
   w.dsort_v0 = c.take<uint32_t>(p);
   w.dsort_v1 = c.take<uint32_t>(p);
   w.lflag = c.take<uint32_t>(p);
@@ -176,7 +176,7 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   w.bcount = c.take<uint32_t>(t + 1);
   w.actoff = c.take<uint32_t>(t + 1);
   w.maxc = c.take<uint32_t>(t);
-  w.rcount = c.take<uint32_t>(p);
+  w.lrank = c.take<uint32_t>(p);
   // counts and the depth sort's state form one region, cleared together at
   // the start of a frame (GAUSS_SYNC_WORDS)
   w.counts = c.take<uint32_t>(GAUSS_SYNC_WORDS);

@@ -263,8 +263,8 @@ __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a, View
     a.radii[ri] = o.valid ? o.radius : 0;
     const uint32_t dk = o.valid ? depth_bits(o.depth) : 0xffffffffu;
     a.dkey[ri] = dk;
-    a.dsort_k[ri] = dk;
-    a.dsort_v[ri] = (uint32_t)ri;
+    // sorted on the high half; ties keep record order (the sort is stable)
+    a.dsort_k[ri] = ((uint64_t)dk << 32) | (uint64_t)ri;
     if (o.valid)
       store_records(ri, o, a.geom, a.splat);
     else
@@ -285,7 +285,11 @@ __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a, View
   __syncthreads();
   walk_small<false>(L, nullptr, S, 0u, nullptr, nullptr, false);
   __syncthreads();
-  if (i < a.P) a.pcount[ri] = L.cnt[lane];
+  if (i < a.P) {
+    a.pcount[ri] = L.cnt[lane];
+    // the depth sort carries the small rects' counts; large ones are counted later
+    a.dsort_v[ri] = o.valid && !small ? PAIRS_LARGE : L.cnt[lane];
+  }
 }
 
 // Colour pass: SH degree <= 3 of every valid Gaussian (preprocess.cu:698-707)
@@ -407,6 +411,7 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
                                                      const uint32_t* __restrict__ counts,
                                                      uint32_t* pcount,
                                                      const uint32_t* __restrict__ poff,
+                                                     const uint32_t* __restrict__ lrank,
                                                      uint32_t* lband,
                                                      uint32_t cap, void* tkeys, uint32_t* recs,
                                                      int t16) {
@@ -429,7 +434,7 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
     const uint32_t toff = views[v].tile_off;
     uint32_t base = 0u;
     if (EMIT) {
-      base = poff[g];
+      base = poff[lrank[g]];
       for (int k = 1; k <= band; ++k) base += lband[q - k];
     }
     uint32_t run = 0;
@@ -464,20 +469,20 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
 
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
-                              const uint32_t* poff, uint32_t* lband, uint32_t cap, void* tkeys,
-                              uint32_t* recs, bool t16, hipStream_t s) {
+                              const uint32_t* poff, const uint32_t* lrank, uint32_t* lband,
+                              uint32_t cap, void* tkeys, uint32_t* recs, bool t16, hipStream_t s) {
   if (P <= 0) return hipSuccess;
   const int blocks = 4096;  // 16384 waves, grid-stride; each exits past the device-side count
   if (emit)
     FGS_TIMED("large_tiles_emit", s,
               (k_large_tiles<true><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
-                                                          pcount, poff, lband, cap, tkeys, recs,
-                                                          t16 ? 1 : 0)));
+                                                          pcount, poff, lrank, lband, cap, tkeys,
+                                                          recs, t16 ? 1 : 0)));
   else
     FGS_TIMED("large_tiles_count", s,
               (k_large_tiles<false><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
-                                                           pcount, poff, lband, cap, tkeys, recs,
-                                                           t16 ? 1 : 0)));
+                                                           pcount, poff, lrank, lband, cap, tkeys,
+                                                           recs, t16 ? 1 : 0)));
   return hipGetLastError();
 }
 
@@ -488,7 +493,7 @@ hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, con
 // k_large_tiles fills) and its stores combine in L2.
 __global__ __launch_bounds__(256) void k_emit_small(int P, int NR, const ViewRec* __restrict__ views,
                                                     const float* __restrict__ geom, int split,
-                                                    const uint32_t* __restrict__ order,
+                                                    const uint64_t* __restrict__ order,
                                                     const uint32_t* __restrict__ poff, uint32_t cap,
                                                     void* __restrict__ tkeys,
                                                     uint32_t* __restrict__ recs, int t16) {
@@ -503,13 +508,13 @@ __global__ __launch_bounds__(256) void k_emit_small(int P, int NR, const ViewRec
   uint32_t g = 0u, pos = 0u, toff = 0u;
   int gx = 1;
   if (rank < NR) {
-    g = order[rank];
+    g = (uint32_t)order[rank];
     r = load_geom(geom, g);
     const int area = (r.r2 - r.r0) * (r.r3 - r.r1);
     small = area > 0 && area < split;
     if (small) {
       const int v = NR > P ? (int)(g / (uint32_t)P) : 0;
-      pos = poff[g];
+      pos = poff[rank];
       gx = views[v].grid_x;
       toff = views[v].tile_off;
     }
@@ -525,7 +530,7 @@ __global__ __launch_bounds__(256) void k_emit_small(int P, int NR, const ViewRec
 }
 
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
-                             const uint32_t* order, const uint32_t* poff, uint32_t cap,
+                             const uint64_t* order, const uint32_t* poff, uint32_t cap,
                              void* tkeys, uint32_t* recs, bool t16, hipStream_t s) {
   const int64_t NR = (int64_t)P * V;
   if (NR > 0)

@@ -145,15 +145,32 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 }
 
 // ------------------------------------------------------ pair offsets
-// Exclusive scan of the records' pair counts taken in depth order
-// (count[order[r]]), written back in record order (off[order[r]]), with
-// the total pair count and the bucket-table bound ceil(total / B) + T: the
-// gather, scan, scatter and bound of the FAST preprocess in two launches
-// (one up to 8192 records).
-__global__ __launch_bounds__(SCAN_THREADS) void k_pscan_reduce(const uint32_t* __restrict__ count,
-                                                               const uint32_t* __restrict__ order,
+// Pair offsets of the FAST preprocess by depth rank, from the depth sort's
+// output: keys[r] = depth bits << 32 | record and vals[r] = the record's
+// pair count, or PAIRS_LARGE for a large rect, whose count the large-rect
+// pass added to count[record] and whose rank is recorded in lrank[record].
+// off[r] = exclusive prefix over ranks, counts[0] = the total, counts[1] =
+// ceil(total / bucket) + T. Everything is read and written in rank order
+// except the few large rects' entries. The first pass resolves the counts
+// in place (vals) and sums tiles; the second scans them (one launch up to
+// SCAN1_MAX records).
+__device__ __forceinline__ uint32_t rank_count(const uint64_t* __restrict__ keys, uint32_t* vals,
+                                               const uint32_t* __restrict__ count,
+                                               uint32_t* __restrict__ lrank, int64_t r) {
+  uint32_t v = vals[r];
+  if (v & PAIRS_LARGE) {
+    const uint32_t rec = (uint32_t)keys[r];
+    lrank[rec] = (uint32_t)r;
+    v = count[rec];
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_pscan_reduce(const uint64_t* __restrict__ keys,
+                                                               uint32_t* __restrict__ vals,
+                                                               const uint32_t* __restrict__ count,
+                                                               uint32_t* __restrict__ lrank,
                                                                int64_t n,
-                                                               uint32_t* __restrict__ gathered,
                                                                uint32_t* __restrict__ partial) {
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
   uint32_t s = 0;
@@ -161,8 +178,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_pscan_reduce(const uint32_t* _
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     const int64_t i = base + k * SCAN_THREADS + threadIdx.x;
     if (i < n) {
-      const uint32_t v = count[order[i]];
-      gathered[i] = v;
+      const uint32_t v = rank_count(keys, vals, count, lrank, i);
+      vals[i] = v;
       s += v;
     }
   }
@@ -192,8 +209,7 @@ __global__ __launch_bounds__(1024) void k_pscan_partials(uint32_t* __restrict__ 
 // FUSED: partial holds the tile sums; the last workgroup writes the total
 // and the bucket-table bound. Else partial holds their exclusive scan.
 template <bool FUSED>
-__global__ __launch_bounds__(SCAN_THREADS) void k_pscan_apply(const uint32_t* __restrict__ gathered,
-                                                              const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(SCAN_THREADS) void k_pscan_apply(const uint32_t* __restrict__ vals,
                                                               int64_t n,
                                                               const uint32_t* __restrict__ partial,
                                                               uint32_t* __restrict__ off,
@@ -205,7 +221,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_pscan_apply(const uint32_t* __
   uint32_t s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    v[k] = (base + k < n) ? gathered[base + k] : 0u;
+    v[k] = (base + k < n) ? vals[base + k] : 0u;
     s += v[k];
   }
   uint32_t tot;
@@ -217,13 +233,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_pscan_apply(const uint32_t* __
   }
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    if (base + k < n) off[order[base + k]] = run;
+    if (base + k < n) off[base + k] = run;
     run += v[k];
   }
 }
 
-__global__ __launch_bounds__(SCAN1_THREADS) void k_pscan_single(const uint32_t* __restrict__ count,
-                                                                const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(SCAN1_THREADS) void k_pscan_single(const uint64_t* __restrict__ keys,
+                                                                uint32_t* __restrict__ vals,
+                                                                const uint32_t* __restrict__ count,
+                                                                uint32_t* __restrict__ lrank,
                                                                 int64_t n,
                                                                 uint32_t* __restrict__ off,
                                                                 uint32_t* __restrict__ counts,
@@ -233,14 +251,14 @@ __global__ __launch_bounds__(SCAN1_THREADS) void k_pscan_single(const uint32_t* 
   uint32_t sum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN1_ITEMS; ++k) {
-    v[k] = b + k < n ? count[order[b + k]] : 0u;
+    v[k] = b + k < n ? rank_count(keys, vals, count, lrank, b + k) : 0u;
     sum += v[k];
   }
   uint32_t tot;
   uint32_t run = block_excl_scan<SCAN1_THREADS>(sum, tot);
 #pragma unroll
   for (int k = 0; k < SCAN1_ITEMS; ++k) {
-    if (b + k < n) off[order[b + k]] = run;
+    if (b + k < n) off[b + k] = run;
     run += v[k];
   }
   if (threadIdx.x == 0) {
@@ -249,23 +267,22 @@ __global__ __launch_bounds__(SCAN1_THREADS) void k_pscan_single(const uint32_t* 
   }
 }
 
-hipError_t scan_pair_offsets(const uint32_t* count, const uint32_t* order, int64_t n,
-                             uint32_t* gathered, uint32_t* off, uint32_t* counts, int bucket,
-                             int T, void* ws, hipStream_t s) {
+hipError_t scan_pair_offsets(const uint64_t* keys, uint32_t* vals, const uint32_t* count,
+                             uint32_t* lrank, int64_t n, uint32_t* off, uint32_t* counts,
+                             int bucket, int T, void* ws, hipStream_t s) {
   if (n <= SCAN1_MAX) {
-    k_pscan_single<<<1, SCAN1_THREADS, 0, s>>>(count, order, n, off, counts, bucket, T);
+    k_pscan_single<<<1, SCAN1_THREADS, 0, s>>>(keys, vals, count, lrank, n, off, counts, bucket,
+                                                T);
     return hipGetLastError();
   }
   const int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
   uint32_t* partial = (uint32_t*)ws;
-  k_pscan_reduce<<<nb, SCAN_THREADS, 0, s>>>(count, order, n, gathered, partial);
+  k_pscan_reduce<<<nb, SCAN_THREADS, 0, s>>>(keys, vals, count, lrank, n, partial);
   if (nb <= PARTIALS_FUSED_MAX) {
-    k_pscan_apply<true><<<nb, SCAN_THREADS, 0, s>>>(gathered, order, n, partial, off, counts,
-                                                   bucket, T);
+    k_pscan_apply<true><<<nb, SCAN_THREADS, 0, s>>>(vals, n, partial, off, counts, bucket, T);
   } else {
     k_pscan_partials<<<1, 1024, 0, s>>>(partial, nb, counts, bucket, T);
-    k_pscan_apply<false><<<nb, SCAN_THREADS, 0, s>>>(gathered, order, n, partial, off, counts,
-                                                    bucket, T);
+    k_pscan_apply<false><<<nb, SCAN_THREADS, 0, s>>>(vals, n, partial, off, counts, bucket, T);
   }
   return hipGetLastError();
 }
