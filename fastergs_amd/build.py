@@ -1,4 +1,5 @@
-"""Builds libfastergs.so (the C-ABI library of hand-written gfx950 kernels).
+"""Builds libfastergs.so (the C-ABI library of hand-written gfx950 kernels)
+and _C.so, the PyTorch-ROCm extension of its torch-typed operations.
 
     python -m fastergs_amd.build
 
@@ -41,18 +42,61 @@ def _compile(src: str) -> str:
     return obj
 
 
+# The torch extension is host code only: g++ against torch's headers (the
+# compiler torch was built with), linked to libfastergs.so and torch's HIP
+# libraries; __HIP_PLATFORM_AMD__ selects the AMD platform in the HIP
+# runtime headers that c10/hip includes.
+EXT_SRC = os.path.join(CSRC, "torch_ext.cpp")
+EXT = os.path.join(HERE, "_C.so")
+
+
+def _ext_flags():
+    import sysconfig
+
+    import torch
+    import torch.utils.cpp_extension as ce
+    inc = [*ce.include_paths(), sysconfig.get_paths()["include"], "/opt/rocm/include"]
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cflags = ["-O2", "-std=c++17", "-fPIC", *[f"-I{d}" for d in inc], "-D__HIP_PLATFORM_AMD__",
+              "-DUSE_ROCM", "-DTORCH_EXTENSION_NAME=_C", "-DTORCH_API_INCLUDE_EXTENSION_H",
+              f"-D_GLIBCXX_USE_CXX11_ABI={abi}"]
+    tlib = ce.library_paths()[0]
+    ldflags = [f"-L{tlib}", "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip",
+               "-ltorch_python", f"-L{HERE}", "-lfastergs", "-L/opt/rocm/lib", "-lamdhip64",
+               "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{tlib}"]
+    return cflags, ldflags
+
+
+def _compile_ext(cflags) -> str:
+    obj = os.path.join(OBJ, "torch_ext.o")
+    r = subprocess.run(["g++", *cflags, "-c", EXT_SRC, "-o", obj], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for torch_ext.cpp:\n{r.stderr[-6000:]}")
+    return obj
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
+    cflags, ldflags = _ext_flags()
+    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES) + 1, os.cpu_count() or 4)) as ex:
+        ext_obj = ex.submit(_compile_ext, cflags)
         objs = list(ex.map(_compile, SOURCES))
+        ext_obj = ext_obj.result()
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libfastergs.so",
+           "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    tmp = EXT + ".tmp"
+    r = subprocess.run(["g++", "-shared", "-o", tmp, ext_obj, *ldflags], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"extension link failed:\n{r.stderr[-6000:]}")
+    os.replace(tmp, EXT)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", EXT)
     return LIB
 
 

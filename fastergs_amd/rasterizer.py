@@ -17,7 +17,11 @@ statistics. Differences, all deliberate:
 * argument errors raise ``ValueError`` instead of a bare ``Exception``.
 
 Compute runs in libfastergs.so (hand-written gfx950 kernels) through its C
-ABI; PyTorch only provides device memory and the current stream.
+ABI: a frame's forward and backward go through the PyTorch-ROCm extension
+``fastergs_amd._C`` (torch-typed ops over the ABI, the role of the
+reference's ``_C.ops``); the stage-level and inspection calls of ``Frame``
+use the ABI directly through ctypes. PyTorch only provides device memory
+and the current stream.
 """
 from __future__ import annotations
 
@@ -28,8 +32,10 @@ from typing import NamedTuple
 import torch
 import torch.nn as nn
 
-from . import _lib
+from . import _C, _lib
 from ._lib import Camera, FrameView, Gaussians, Grads, Options, check, lib
+
+_NONE = torch.empty(0)  # an absent optional tensor for the extension's ops
 
 
 @dataclass
@@ -109,15 +115,6 @@ def _stream(device) -> int:
 # Largest pair count / bucket bound seen per frame shape (plus headroom):
 # the capacity a speculative forward renders with.
 _capacity_hints: dict = {}
-_side_streams: dict = {}
-
-
-def _side_stream(device) -> torch.cuda.Stream:
-    """A default-priority stream per device for reading a frame's counts."""
-    s = _side_streams.get(device)
-    if s is None:
-        s = _side_streams[device] = torch.cuda.Stream(device)
-    return s
 
 
 class Frame:
@@ -167,15 +164,44 @@ class Frame:
         self.campos = _f32(settings.campos, dev).reshape(3)
         bg = settings.bg if torch.is_tensor(settings.bg) else torch.tensor(settings.bg)
         self.bg = _f32(bg, dev).reshape(3)
-        self.cam = Camera(self.W, self.H, float(settings.tanfovx), float(settings.tanfovy),
-                          float(settings.scale_modifier), float(settings.mult),
-                          int(settings.sh_degree), _ptr(self.view), _ptr(self.proj),
-                          _ptr(self.campos), _ptr(self.bg))
-        self.gauss = Gaussians(P, self.M, _ptr(self.means3D), _ptr(self.opacities),
-                               _ptr(self.scales), _ptr(self.rotations), _ptr(self.dc),
-                               _ptr(self.shs), _ptr(self.colors), _ptr(self.culling))
         self.R = self.NB = 0          # this frame's pair count and bucket-table bound
         self.R_cap = self.NB_cap = 0  # capacities the pair workspace is laid out for
+        self.pws = None
+
+    # ctypes views of the frame for the stage-level ABI calls
+    @property
+    def cam(self) -> Camera:
+        st = self.settings
+        c = Camera(self.W, self.H, float(st.tanfovx), float(st.tanfovy), float(st.scale_modifier),
+                   float(st.mult), int(st.sh_degree), _ptr(self.view), _ptr(self.proj),
+                   _ptr(self.campos), _ptr(self.bg))
+        return c
+
+    @property
+    def gauss(self) -> Gaussians:
+        return Gaussians(self.P, self.M, _ptr(self.means3D), _ptr(self.opacities),
+                         _ptr(self.scales), _ptr(self.rotations), _ptr(self.dc), _ptr(self.shs),
+                         _ptr(self.colors), _ptr(self.culling))
+
+    def _ext_args(self):
+        """The frame's tensors, settings and options as the extension's ops take them."""
+        st, o = self.settings, self.copts
+        ts = self.opts.tile_sched
+        mlp = _NONE
+        if ts is None:
+            tsl = [-1, 0, 0, 0, 0]
+        else:
+            tsl = [int(ts.mode), int(ts.split_len), int(bool(ts.features)),
+                   int(bool(ts.log_cycles)), int(bool(ts.order))]
+            if getattr(ts, "model", None) is not None:
+                mlp = ts.model.params
+        opts = [int(o.schedule), int(o.bucket_size), int(o.split_tiles), int(o.abs_grad),
+                int(o.forward_only), *tsl]
+        n = lambda t: _NONE if t is None else t
+        return (self.means3D, self.opacities, self.scales, self.rotations, n(self.dc), n(self.shs),
+                n(self.colors), n(self.culling), self.view, self.proj, self.campos, self.bg,
+                self.W, self.H, float(st.tanfovx), float(st.tanfovy), float(st.scale_modifier),
+                float(st.mult), int(st.sh_degree), opts, mlp)
 
     # -- forward stages ---------------------------------------------------
     def preprocess(self, sync: bool = True):
@@ -216,30 +242,18 @@ class Frame:
         return self
 
     def forward(self):
-        """Preprocess and render. With speculative options and a capacity
-        remembered for frames of this shape, the render is launched at once
-        with that capacity, and the frame's counts are read on a side stream
-        that waits for the preprocess only, so the host check overlaps the
-        render; a frame that exceeds the capacity is rendered again with its
-        exact counts (the outputs never depend on the capacity)."""
+        """Preprocess and render (one call of the extension). With speculative
+        options and a capacity remembered for frames of this shape, the render
+        is queued at once with that capacity while the frame's counts are
+        read on a side stream that waits for the preprocess only; a frame that
+        exceeds the capacity is rendered again with its exact counts (the
+        outputs never depend on the capacity)."""
         key = (self.P, self.W, self.H, self.opts.schedule, self.opts.bucket_size, self.device)
         hint = _capacity_hints.get(key) if self.opts.speculative else None
-        if hint is None or self.P == 0:
-            self.preprocess(sync=True).render()
-        else:
-            dev = self.device
-            self.preprocess(sync=False)
-            main = torch.cuda.current_stream(dev)
-            done = torch.cuda.Event()
-            done.record(main)
-            side = _side_stream(dev)
-            side.wait_event(done)
-            self.R_cap, self.NB_cap = hint
-            self.render()
-            self.read_counts(side)
-            if self.R > self.R_cap or self.NB > self.NB_cap:
-                self.R_cap, self.NB_cap = self.R, self.NB
-                self.render()
+        R_hint, NB_hint = hint if hint is not None else (0, 0)
+        (self.radii, self.gws, self.pws, self.out_color, self.T_final, self.last_contributor,
+         c) = _C.ops.forward(*self._ext_args(), int(R_hint), int(NB_hint))
+        self.R, self.NB, self.R_cap, self.NB_cap = (int(x) for x in c)
         # remember the largest counts seen for this shape, with headroom
         old = _capacity_hints.get(key, (0, 0))
         _capacity_hints[key] = (max(old[0], self.R + self.R // 8 + 1024),
@@ -250,33 +264,21 @@ class Frame:
     def backward(self, grad_out_color: torch.Tensor, skip=()) -> dict:
         """Gradients of the frame; `skip` may name "conic", "colors" and
         "cov3D", which are then neither computed into memory nor returned."""
-        L, dev, P = lib(), self.device, self.P
         if not set(skip) <= {"conic", "colors", "cov3D"}:
             raise ValueError("only conic, colors and cov3D gradients can be skipped")
         if self.forward_only:
             raise ValueError("the frame was rendered forward-only")
-        g = _f32(grad_out_color, dev).reshape(3, self.H, self.W)
-        e = lambda *s: torch.empty(s, dtype=torch.float32, device=dev)
-        shapes = {"mean2D": (P, 4), "conic": (P, 2, 2), "opacities": (P, 1), "colors": (P, 3),
-                  "means3D": (P, 3), "cov3D": (P, 6), "dc": (P, 1, 3), "shs": (P, self.M, 3),
-                  "scales": (P, 3), "rotations": (P, 4)}
-        out = {k: e(*sh) for k, sh in shapes.items() if k not in skip}
-        grads = Grads(*[_ptr(out.get(k)) for k in ("mean2D", "conic", "opacities", "colors",
-                                                    "means3D", "cov3D", "dc", "shs", "scales",
-                                                    "rotations")])
-        gws = torch.empty(L.fgs_grad_ws_size(P), dtype=torch.uint8, device=dev)
-        check(L.fgs_backward(ctypes.byref(self.cam), ctypes.byref(self.gauss),
-                             ctypes.byref(self.copts), _ptr(self.gws), _ptr(self.pws), self.R_cap,
-                             self.NB_cap, _ptr(self.radii), _ptr(self.T_final),
-                             _ptr(self.last_contributor), _ptr(g), _ptr(gws),
-                             ctypes.byref(grads), ctypes.c_void_p(_stream(dev))), "fgs_backward")
-        return _Grads(out, gws, P)
+        r = _C.ops.backward(*self._ext_args(), self.gws, self.pws, self.R_cap, self.NB_cap,
+                            self.radii, self.T_final, self.last_contributor, grad_out_color,
+                            "conic" not in skip, "colors" not in skip, "cov3D" not in skip)
+        names = ("mean2D", "conic", "opacities", "colors", "means3D", "cov3D", "dc", "shs",
+                 "scales", "rotations")
+        out = {k: t for k, t in zip(names, r[:10]) if k not in skip}
+        return _Grads(out, r[10], self.P)
 
     def densify_stats(self, dmean2D, accum, accum_abs, max_radii, denom):
-        check(lib().fgs_densify_stats(self.P, _ptr(self.radii), _ptr(dmean2D), _ptr(accum),
-                                      _ptr(accum_abs), _ptr(max_radii), _ptr(denom),
-                                      ctypes.c_void_p(_stream(self.device))),
-              "fgs_densify_stats")
+        n = lambda t: _NONE if t is None else t
+        _C.ops.densify_stats(self.radii, dmean2D, n(accum), n(accum_abs), n(max_radii), n(denom))
 
     def render_simp(self):
         """Importance statistics over this frame's lists (render_simp):

@@ -90,3 +90,16 @@ def test_forward_only_pair_workspace_is_smaller():
     a = L.fgs_pair_ws_size(1000, W, H, R, NB, ctypes.byref(full))
     b = L.fgs_pair_ws_size(1000, W, H, R, NB, ctypes.byref(fo))
     assert a - b >= NB * 256 * 16 + W * H * 16
+
+
+def test_torch_extension_loads_and_shares_the_library():
+    """The PyTorch-ROCm extension imports (without a GPU), exposes its ops,
+    and resolves libfastergs.so to the same in-tree file ctypes loads."""
+    import diff_gaussian_rasterization_faster as d
+    from fastergs_amd import _C
+    assert d._C is _C
+    assert {"forward", "backward", "densify_stats"} <= set(dir(_C.ops))
+    assert _C.build_info() == _lib.lib().fgs_build_info().decode()
+    maps = open("/proc/self/maps").read()
+    libs = {line.split()[-1] for line in maps.splitlines() if line.endswith("libfastergs.so")}
+    assert len(libs) == 1, libs
