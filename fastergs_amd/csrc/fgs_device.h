@@ -55,6 +55,45 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return __builtin_amdgcn_readlane(v, 63);
 }
 
+// Transposing sum of 32 per-lane values over the wave: v[k] of every lane
+// is summed into one lane pair, and lane l returns the total of v[l >> 1]
+// (both lanes of the pair hold it). Each step exchanges half of the values
+// with a partner lane and adds the halves that stay: lane ^ 32 and lane ^ 16
+// with v_permlane32_swap / v_permlane16_swap (gfx950), then lane ^ 15
+// (row_mirror), lane ^ 7 (row_half_mirror) and lane ^ 2 (quad_perm) with a
+// select of the kept half, and a final add over lane ^ 1. The partner masks
+// span all six lane bits, so every total covers all 64 lanes; 70 VALU
+// operations per call instead of 32 wave sums.
+__device__ __forceinline__ float f_of(uint32_t u) { return __builtin_bit_cast(float, u); }
+__device__ __forceinline__ uint32_t u_of(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+__device__ __forceinline__ float wave_transpose_sum32(const float (&v)[32]) {
+  const int l = lane_id();
+  float a[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {  // lanes < 32 keep v[j], lanes >= 32 keep v[j + 16]
+    const auto s = __builtin_amdgcn_permlane32_swap(u_of(v[j]), u_of(v[j + 16]), false, false);
+    a[j] = f_of(s[0]) + f_of(s[1]);
+  }
+  float b[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {  // even rows keep a[j], odd rows a[j + 8]
+    const auto s = __builtin_amdgcn_permlane16_swap(u_of(a[j]), u_of(a[j + 8]), false, false);
+    b[j] = f_of(s[0]) + f_of(s[1]);
+  }
+  const bool h3 = (l & 8) != 0, h2 = (l & 4) != 0, h1 = (l & 2) != 0;
+  float c[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)  // bit 3 clear keeps b[j], set keeps b[j + 4]
+    c[j] = (h3 ? b[j + 4] : b[j]) + dpp_f<0x140>(h3 ? b[j] : b[j + 4]);
+  float d[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)  // bit 2 clear keeps c[j], set keeps c[j + 2]
+    d[j] = (h2 ? c[j + 2] : c[j]) + dpp_f<0x141>(h2 ? c[j] : c[j + 2]);
+  const float e = (h1 ? d[1] : d[0]) + dpp_f<0x4e>(h1 ? d[0] : d[1]);
+  return e + dpp_f<0xb1>(e);
+}
+
 // Inclusive prefix sum over the wave (shuffle based; used by scans only).
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   const int l = lane_id();

@@ -522,9 +522,13 @@ FGS_HD void log2_conic(const R conic[3], R log2op, R& ca, R& cb, R& cc, R& w) {
 // 0.99 and every p < -9 gives alpha < 1/255 -> 0 either way, so the result
 // equals min(0.99, 2^p) thresholded at 1/255 for all p (NaN included, which
 // becomes 0.99 like fminf(0.99, NaN)), and exp2 needs no range checks.
-FGS_HD float alpha_of(float p) {
+// alpha_capped is min(0.99, 2^p) before the threshold (never NaN).
+FGS_HD float alpha_capped(float p) {
   const float pc = fmaxf(fminf(p, 0.0f), -9.0f);
-  const float a = fminf(ALPHA_CAP_F, exp2_core(pc));
+  return fminf(ALPHA_CAP_F, exp2_core(pc));
+}
+FGS_HD float alpha_of(float p) {
+  const float a = alpha_capped(p);
   return a < 1.0f / 255.0f ? 0.0f : a;
 }
 FGS_HD double alpha_of(double p) {
@@ -532,11 +536,16 @@ FGS_HD double alpha_of(double p) {
   return a < 1.0 / 255.0 ? 0.0 : a;
 }
 
+// Exponent of a splat at pixel (px,py): log2 opacity plus the log2-domain
+// quadratic form of the offset.
+template <typename R>
+FGS_HD R splat_power(R gx, R gy, R ca, R cb, R cc, R w, R px, R py) {
+  const R dx = gx - px, dy = gy - py;
+  return fma_(cb * dx, dy, fma_(cc * dy, dy, fma_(ca * dx, dx, w)));
+}
 template <typename R>
 FGS_HD R splat_alpha(R gx, R gy, R ca, R cb, R cc, R w, R px, R py) {
-  const R dx = gx - px, dy = gy - py;
-  const R p = fma_(cb * dx, dy, fma_(cc * dy, dy, fma_(ca * dx, dx, w)));
-  return alpha_of(p);
+  return alpha_of(splat_power(gx, gy, ca, cb, cc, w, px, py));
 }
 
 // Front-to-back blend of one splat (render.cu:112-115).

@@ -454,124 +454,451 @@ hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
 
 // ----------------------------------------------------------- render_simp
 // Importance accumulation over a rendered frame's lists (render_simp.cu:36-301):
-// one 256-thread workgroup per tile, one pixel per thread, splats staged 256
-// at a time in LDS. Pixels blend front to back without stopping on their
-// own; after each pair of splats (a leading single splat when the list
-// length is odd) the tile stops once all 256 pixels of the tile -- those
-// outside the image keep T = 1 -- have T < 1e-4. Per splat the weights
-// alpha*T and the number of pixels with alpha >= 1/255 are summed; per pixel
-// the splat of largest weight (the first of equals) gets +1 at the end.
+// per tile, pixels blend front to back without stopping on their own; after
+// each pair of splats (a leading single splat when the list length is odd)
+// the tile stops once all 256 of its pixels -- those outside the image keep
+// T = 1 -- have T < 1e-4. Per splat the weights alpha*T and the number of
+// pixels with alpha >= 1/255 are summed; per pixel the splat of largest
+// weight (the first of equals) gets +1 at the end.
 //
-// No barrier per pair: T only decreases, so each wave notes the first pair
-// end at which all its 64 pixels are below 1e-4, and the tile's stop point
-// is the largest of the four. Waves run through the whole batch; the
-// per-splat sums (DPP wave sum, ballot count) go to LDS with LDS atomics and
-// are flushed to memory once per batch, only for splats up to the stop
-// point. Past the stop point every weight is < 1e-4 while each pixel's
+// A tile with pixels outside the image therefore never stops and walks its
+// whole list, serially per pixel: such tiles (the last tile row and column
+// when the image is not a multiple of the tile) are cut into pieces of 32
+// pixels that run first in the grid, each with its alphas computed by three
+// waves and the T chains run by a fourth (simp_piece); the full tiles follow,
+// one workgroup each, walking in chunks until the stop (simp_tile).
+constexpr int SIMP_CHUNK = 32;
+constexpr int SIMP_PIECE = 32;
+
+// Work items of one render_simp launch. Tiles with pixels outside the image
+// never stop (those pixels keep T = 1), so their whole lists are walked:
+// their pixels inside the image are cut into pieces of SIMP_PIECE
+// (row-major over the tile's inside rectangle), handled first in the grid
+// by simp_piece. The tiles of the full-tile region follow, one per
+// workgroup.
+struct SimpGrid {
+  int W, H, gx, gy;
+  int hr, wr;   // inside rows of the last tile row / columns of the last column (0: full)
+  int gxf;      // full-width tiles of the last row
+  int nbf, nc, nr;  // pieces per last-row tile, of the corner tile, per last-column tile
+  int n_pieces;
+  int gxi, gyi;     // the full-tile region
+};
+inline SimpGrid simp_grid(int W, int H) {
+  SimpGrid g;
+  g.W = W;
+  g.H = H;
+  g.gx = (W + TILE_W - 1) / TILE_W;
+  g.gy = (H + TILE_H - 1) / TILE_H;
+  g.hr = H % TILE_H;
+  g.wr = W % TILE_W;
+  g.gxf = g.wr ? g.gx - 1 : g.gx;
+  g.nbf = g.hr ? (TILE_W * g.hr + SIMP_PIECE - 1) / SIMP_PIECE : 0;
+  g.nc = (g.hr && g.wr) ? (g.wr * g.hr + SIMP_PIECE - 1) / SIMP_PIECE : 0;
+  g.nr = g.wr ? (g.wr * TILE_H + SIMP_PIECE - 1) / SIMP_PIECE : 0;
+  g.gxi = g.gxf;
+  g.gyi = g.hr ? g.gy - 1 : g.gy;
+  g.n_pieces = g.gxf * g.nbf + g.nc + g.gyi * g.nr;
+  return g;
+}
+
+// First position at which the tile checks its stop at or after position
+// pos: pair ends are the odd positions of an even-length list and the even
+// positions >= 2 of an odd-length one (whose position 0 is the single).
+__device__ __forceinline__ uint32_t simp_pair_end(uint32_t pos, uint32_t len) {
+  return (len & 1u) ? max(2u, pos + (pos & 1u)) : (pos | 1u);
+}
+
+// Blend of one staged chunk for the calling lane's pixel (inside the image):
+// the weights into w[k], T and the running maximum updated (kmax: chunk
+// index of a new maximum, else -1), and the wave's count of pixels hit by
+// splat k into lane 2k of cnt. Entries k >= n of a partial chunk (FULL
+// false) hold stale values: they are read anyway, so the body has no branch,
+// and masked out of the hits. Splats are read from LDS four ahead.
+template <bool FULL>
+__device__ __forceinline__ void simp_chunk(const float4* __restrict__ sA,
+                                           const float2* __restrict__ sB, uint32_t n, float fx,
+                                           float fy, float& T, float& wmax, int& kmax,
+                                           float (&w)[SIMP_CHUNK], float& cnt) {
+  constexpr int D = 4;
+  float4 A[D];
+  float2 Bv[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    A[d] = sA[d];
+    Bv[d] = sB[d];
+  }
+#pragma unroll
+  for (int k = 0; k < SIMP_CHUNK; ++k) {
+    const float4 Ak = A[k % D];
+    const float2 Bk = Bv[k % D];
+    if (k + D < SIMP_CHUNK) {
+      A[k % D] = sA[k + D];
+      Bv[k % D] = sB[k + D];
+    }
+    const float ar = alpha_capped(splat_power(Ak.x, Ak.y, Ak.z, Ak.w, Bk.x, Bk.y, fx, fy));
+    const bool above = !(ar < 1.0f / 255.0f);
+    const bool valid = FULL || (uint32_t)k < n;
+    const uint64_t hitm = valid ? __builtin_amdgcn_ballot_w64(above) : 0ull;
+    const bool hit = above && valid;
+    const float a = hit ? ar : 0.0f;
+    w[k] = a * T;
+    T = T - w[k];
+    if (wmax < w[k]) {
+      wmax = w[k];
+      kmax = k;
+    }
+    const int hits = (int)__builtin_popcountll(hitm);
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(cnt) : "s"(hits), "i"(2 * k));
+  }
+}
+
+// One atomic per distinct maximum in the wave (neighbouring pixels mostly
+// share it; the counts are integers, so the float sums stay exact).
+__device__ __forceinline__ void simp_max_flush(bool has, uint32_t id, float* maxcount) {
+  const int lane = lane_id();
+  uint64_t pending = __builtin_amdgcn_ballot_w64(has);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)id, leader);
+    const uint64_t same = __builtin_amdgcn_ballot_w64(has && id == k) & pending;
+    if (lane == leader) atomic_add_f(&maxcount[k], (float)__popcll(same));
+    pending &= ~same;
+  }
+}
+
+// A tile of the full-tile region: one pixel per thread, the list walked in
+// chunks of SIMP_CHUNK splats with one barrier per chunk. The chunk's splats
+// are staged in LDS (double buffered; the next chunk's loads are in flight
+// while the current one is blended), each lane keeps its pixel's weights of
+// the chunk in registers, and one transposing wave sum per chunk gives the
+// per-splat totals; the hit counts come from ballots. T only decreases, so a
+// wave whose 64 pixels are all below 1e-4 at a chunk end finds the first
+// pair end at which that held by replaying the chunk's T chain (exactly,
+// from the stored weights) in the lanes that crossed in it; the tile's stop
+// is the largest of the four waves' stops, and only splats up to it are
+// added to memory. Past the stop every weight is < 1e-4 while each pixel's
 // largest weight is >= 1/255 (its first contributor meets T = 1), so the
 // per-pixel maxima need no such cut.
-__global__ __launch_bounds__(256) void k_render_simp(int W, int H, int grid_x,
+__device__ __forceinline__ void simp_tile(int tile, const SimpGrid& g,
+                                          const uint32_t* __restrict__ ranges,
+                                          const uint32_t* __restrict__ vals,
+                                          const float* __restrict__ splat, float* __restrict__ wsum,
+                                          int32_t* __restrict__ wcount,
+                                          float* __restrict__ maxcount) {
+  constexpr int CH = SIMP_CHUNK;
+  constexpr uint32_t NONE = 0xffffffffu;
+  __shared__ float4 sA[2][CH];        // gx, gy, ca, cb
+  __shared__ float2 sB[2][CH];        // cc, log2 opacity
+  __shared__ float sW[2][4][CH];      // per wave and splat: weight sum
+  __shared__ uint32_t sC[2][4][CH];   // per wave and splat: pixels hit
+  __shared__ uint32_t sStop[2][4];    // per wave: its stop position, or NONE
+  const int tx = tile % g.gx, ty = tile / g.gx;
+  const int t = threadIdx.x;
+  const int wave = t >> 6, lane = t & (WAVE - 1);
+  const float fx = (float)(tx * TILE_W + (t & 15)), fy = (float)(ty * TILE_H + (t >> 4));
+  const uint32_t start = ranges[2 * tile], len = ranges[2 * tile + 1] - start;
+  // staging: thread t < CH handles position p0 + t of each chunk; ids are
+  // loaded two chunks ahead and records one chunk ahead, so no load waits
+  // on another within a chunk
+  const bool stager = t < CH;
+  auto load_id = [&](uint32_t p0) -> uint32_t {
+    return stager && p0 + (uint32_t)t < len ? vals[start + p0 + t] : 0u;
+  };
+  float4 nA = make_float4(0.f, 0.f, 0.f, 0.f);
+  float2 nB = make_float2(0.f, 0.f);
+  auto load_rec = [&](uint32_t id, uint32_t p0) {
+    if (stager && p0 + (uint32_t)t < len) {
+      const float4* r = (const float4*)(splat + 12 * (size_t)id);
+      const float4 r1 = r[1];
+      nA = r[0];
+      nB = make_float2(r1.x, r1.y);
+    }
+  };
+  uint32_t id0 = load_id(0), id1 = load_id(CH);  // ids of chunks c and c + 1
+  load_rec(id0, 0);
+  if (stager) {
+    sA[0][t] = nA;
+    sB[0][t] = nB;
+  }
+  __syncthreads();
+  float T = 1.0f, wmax = 0.0f;
+  uint32_t imax = 0;       // list position of the largest weight
+  uint32_t my_stop = NONE;  // this wave's stop position, once known
+  uint32_t c = 0;
+  for (uint32_t p0 = 0; p0 < len; p0 += CH, ++c) {
+    const int b = (int)(c & 1u);
+    const uint32_t n = min((uint32_t)CH, len - p0);
+    load_rec(id1, p0 + CH);                  // records of chunk c + 1
+    const uint32_t id2 = load_id(p0 + 2 * CH);  // ids of chunk c + 2
+    float w[CH];
+    float cnt = 0.0f;  // lane 2k: pixels of this wave that splat k hits
+    const float T0 = T;
+    int kmax = -1;
+    if (n == (uint32_t)CH)
+      simp_chunk<true>(sA[b], sB[b], n, fx, fy, T, wmax, kmax, w, cnt);
+    else
+      simp_chunk<false>(sA[b], sB[b], n, fx, fy, T, wmax, kmax, w, cnt);
+    if (kmax >= 0) imax = p0 + (uint32_t)kmax;
+    if (my_stop == NONE && __builtin_amdgcn_ballot_w64(T < T_CUT_F) == ~0ull) {
+      // every pixel is below the cut now; the last one to get there crossed
+      // in this chunk: replay its T chain to the crossing position
+      int k0 = -1;
+      if (T0 >= T_CUT_F) {
+        float Tr = T0;
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          Tr = Tr - w[k];
+          if (k0 < 0 && Tr < T_CUT_F) k0 = k;
+        }
+      }
+      my_stop = simp_pair_end(p0 + (uint32_t)max(wave_max_i(k0), 0), len);
+    }
+    const float ws = wave_transpose_sum32(w);
+    if ((lane & 1) == 0) {
+      sW[b][wave][lane >> 1] = ws;
+      sC[b][wave][lane >> 1] = (uint32_t)__builtin_bit_cast(int, cnt);
+    }
+    if (lane == 0) sStop[b][wave] = my_stop;
+    if (stager && p0 + CH + (uint32_t)t < len) {
+      sA[b ^ 1][t] = nA;
+      sB[b ^ 1][t] = nB;
+    }
+    __syncthreads();
+    const uint32_t stop = max(max(sStop[b][0], sStop[b][1]), max(sStop[b][2], sStop[b][3]));
+    if ((uint32_t)t < n && p0 + (uint32_t)t <= stop) {
+      const uint32_t hits = (sC[b][0][t] + sC[b][1][t]) + (sC[b][2][t] + sC[b][3][t]);
+      if (hits != 0u) {
+        atomic_add_f(&wsum[id0], (sW[b][0][t] + sW[b][1][t]) + (sW[b][2][t] + sW[b][3][t]));
+        atomicAdd(&wcount[id0], (int32_t)hits);
+      }
+    }
+    if (stop < p0 + n) break;
+    id0 = id1;
+    id1 = id2;
+  }
+  const bool has = wmax > 0.0f;
+  simp_max_flush(has, has ? vals[start + imax] : 0u, maxcount);
+}
+
+// A piece of a tile that never stops: up to SIMP_PIECE of its pixels inside
+// the image, the whole list walked. Wave 0 runs the pixels' T chains (lanes
+// below SIMP_PIECE) over alphas that waves 1-3 computed one chunk ahead into
+// LDS; a producing pass evaluates two splats at once (lanes 0-31 the first,
+// 32-63 the second, one pixel per lane), pass g of a chunk (splats 2g, 2g +
+// 1) by wave 1 + g % 3, which also counts the hits. Wave 1 stages the splats
+// two chunks ahead (a ring of three), its loads one chunk earlier. Wave 0
+// sums the weights of a chunk with one transposing wave sum and adds them
+// to memory with the staged counts. A lone wave issues a dependent VALU
+// instruction only every few cycles, so the piece is sized for the chain
+// wave to be the slowest of the four.
+template <bool FULL>
+__device__ __forceinline__ void simp_piece_chain(const float (*__restrict__ al)[SIMP_PIECE],
+                                                 uint32_t n, int p, bool inside, float& T,
+                                                 float& wmax, int& kmax,
+                                                 float (&w)[SIMP_CHUNK]) {
+#pragma unroll
+  for (int k = 0; k < SIMP_CHUNK; ++k) {
+    const float av = al[k][p];  // stale for k >= n: masked
+    const float a = inside && (FULL || (uint32_t)k < n) ? av : 0.0f;
+    w[k] = a * T;
+    T = T - w[k];
+    if (wmax < w[k]) {
+      wmax = w[k];
+      kmax = k;
+    }
+  }
+}
+
+__device__ __forceinline__ void simp_piece(int piece, const SimpGrid& g,
+                                           const uint32_t* __restrict__ ranges,
+                                           const uint32_t* __restrict__ vals,
+                                           const float* __restrict__ splat,
+                                           float* __restrict__ wsum, int32_t* __restrict__ wcount,
+                                           float* __restrict__ maxcount) {
+  constexpr int CH = SIMP_CHUNK, PX = SIMP_PIECE;
+  constexpr int PASSES = CH / 2;               // two splats per producing pass
+  constexpr int PER = (PASSES + 2) / 3;        // passes of wave 1 (the most)
+  // alpha (0 if not hit) per chunk splat and piece pixel; rows CH.. take the
+  // unused last passes of waves 2 and 3, so the stores need no branch
+  __shared__ float sAl[2][CH + 4][PX];
+  __shared__ uint32_t sCnt[2][CH];    // pixels of the piece hit per chunk splat
+  __shared__ float4 qA[3][CH];        // staged splats: gx, gy, ca, cb
+  __shared__ float2 qB[3][CH];        // cc, log2 opacity
+  __shared__ uint32_t qId[3][CH];
+  // the piece: tile, index within it, the tile's inside rectangle
+  int tile, q, iw, ih;
+  {
+    int b = piece;
+    if (b < g.gxf * g.nbf) {
+      tile = (g.gy - 1) * g.gx + b / g.nbf;
+      q = b % g.nbf;
+      iw = TILE_W;
+      ih = g.hr;
+    } else if ((b -= g.gxf * g.nbf) < g.nc) {
+      tile = (g.gy - 1) * g.gx + g.gx - 1;
+      q = b;
+      iw = g.wr;
+      ih = g.hr;
+    } else {
+      b -= g.nc;
+      tile = (b / g.nr) * g.gx + g.gx - 1;
+      q = b % g.nr;
+      iw = g.wr;
+      ih = TILE_H;
+    }
+  }
+  const int tx = tile % g.gx, ty = tile / g.gx;
+  const int t = threadIdx.x;
+  const int wave = t >> 6, lane = t & (WAVE - 1);
+  // this lane's pixel: p of the piece (both halves of a producing wave map
+  // lanes l and l + 32 to the same pixel; in the chain wave only lanes < 32)
+  const int p = lane & (PX - 1);
+  const int i = PX * q + p;
+  const bool in_piece = i < iw * ih;
+  const bool inside = in_piece && (wave != 0 || lane < PX);
+  const float fx = (float)(tx * TILE_W + i % iw), fy = (float)(ty * TILE_H + i / iw);
+  const uint32_t start = ranges[2 * tile], len = ranges[2 * tile + 1] - start;
+  const uint32_t nch = (len + CH - 1) / CH;
+  // stager: wave 1, lanes < CH, position c CH + lane of chunk c. Step c
+  // stages chunk c + 2 from records loaded in step c - 1, loads the records
+  // of chunk c + 3 from ids loaded in step c - 1, and the ids of chunk c + 4.
+  const bool stager = wave == 1 && lane < CH;
+  auto in_list = [&](uint32_t c) { return stager && c < nch && c * CH + (uint32_t)lane < len; };
+  auto load_id = [&](uint32_t c) -> uint32_t {
+    return in_list(c) ? vals[start + c * CH + (uint32_t)lane] : 0u;
+  };
+  struct Rec {
+    float4 A;
+    float2 B;
+    uint32_t id;
+  };
+  auto load_rec = [&](uint32_t c, uint32_t id) -> Rec {
+    Rec r{make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f), id};
+    if (in_list(c)) {
+      const float4* pr = (const float4*)(splat + 12 * (size_t)id);
+      const float4 p1 = pr[1];
+      r.A = pr[0];
+      r.B = make_float2(p1.x, p1.y);
+    }
+    return r;
+  };
+  auto stage = [&](uint32_t c, const Rec& r) {
+    if (in_list(c)) {
+      qA[c % 3][lane] = r.A;
+      qB[c % 3][lane] = r.B;
+      qId[c % 3][lane] = r.id;
+    }
+  };
+  // alphas of chunk c; entries k >= n are computed from stale splats and
+  // never read. Hit counts go to lane 2m + half of `hc` for the wave's m-th
+  // pass and are stored once per chunk.
+  const int half = lane >> 5;
+  const uint64_t inside_mask = __builtin_amdgcn_ballot_w64(inside);
+  auto produce = [&](uint32_t c) {
+    if (c >= nch) return;
+    const uint32_t n = min((uint32_t)CH, len - c * CH);
+    const int r = (int)(c % 3), bb = (int)(c & 1u);
+    const int j0 = wave - 1;
+    float hc = 0.0f;
+    float ar[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int k = min(2 * (j0 + 3 * m) + half, CH - 1);  // rows >= CH read splat CH - 1
+      const float4 A = qA[r][k];
+      const float2 Bv = qB[r][k];
+      ar[m] = alpha_capped(splat_power(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx, fy));
+    }
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int k = 2 * (j0 + 3 * m) + half;  // < CH + 4
+      const bool above = !(ar[m] < 1.0f / 255.0f);
+      sAl[bb][k][p] = above && inside ? ar[m] : 0.0f;
+      const uint64_t hitm = __builtin_amdgcn_ballot_w64(above) & inside_mask;
+      const int lo = (int)__builtin_popcount((uint32_t)hitm);
+      const int hi = (int)__builtin_popcount((uint32_t)(hitm >> 32));
+      asm("v_writelane_b32 %0, %1, %2" : "+v"(hc) : "s"(lo), "i"(2 * m));
+      asm("v_writelane_b32 %0, %1, %2" : "+v"(hc) : "s"(hi), "i"(2 * m + 1));
+    }
+    const int k = 2 * (j0 + 3 * (lane >> 1)) + (lane & 1);
+    if (lane < 2 * PER && k < (int)n) sCnt[bb][k] = (uint32_t)__builtin_bit_cast(int, hc);
+  };
+  Rec R;       // records of chunk c + 2 at the start of step c
+  uint32_t I;  // ids of chunk c + 3 at the start of step c
+  {
+    const uint32_t i0 = load_id(0), i1 = load_id(1), i2 = load_id(2);
+    I = load_id(3);
+    const Rec r0 = load_rec(0, i0), r1 = load_rec(1, i1);
+    R = load_rec(2, i2);
+    stage(0, r0);
+    stage(1, r1);
+  }
+  __syncthreads();
+  if (wave != 0) produce(0);
+  __syncthreads();
+  float T = 1.0f, wmax = 0.0f;
+  uint32_t imax = 0;
+  for (uint32_t c = 0; c < nch; ++c) {
+    if (wave == 0) {
+      const uint32_t p0 = c * CH;
+      const uint32_t n = min((uint32_t)CH, len - p0);
+      const int bb = (int)(c & 1u);
+      float w[CH];
+      int kmax = -1;
+      if (n == (uint32_t)CH)
+        simp_piece_chain<true>(sAl[bb], n, p, inside, T, wmax, kmax, w);
+      else
+        simp_piece_chain<false>(sAl[bb], n, p, inside, T, wmax, kmax, w);
+      if (kmax >= 0) imax = p0 + (uint32_t)kmax;
+      const float ws = wave_transpose_sum32(w);
+      const uint32_t k = (uint32_t)lane >> 1;
+      if ((lane & 1) == 0 && k < n) {
+        const uint32_t hits = sCnt[bb][k];
+        if (hits != 0u) {
+          const uint32_t id = qId[c % 3][k];
+          atomic_add_f(&wsum[id], ws);
+          atomicAdd(&wcount[id], (int32_t)hits);
+        }
+      }
+    } else {
+      produce(c + 1);
+      stage(c + 2, R);
+      R = load_rec(c + 3, I);
+      I = load_id(c + 4);
+    }
+    __syncthreads();
+  }
+  if (wave == 0) {
+    const bool has = inside && wmax > 0.0f;
+    simp_max_flush(has, has ? vals[start + imax] : 0u, maxcount);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_render_simp(SimpGrid g,
                                                      const uint32_t* __restrict__ ranges,
                                                      const uint32_t* __restrict__ vals,
                                                      const float* __restrict__ splat,
                                                      float* __restrict__ wsum,
                                                      int32_t* __restrict__ wcount,
                                                      float* __restrict__ maxcount) {
-  __shared__ float4 sA[256];
-  __shared__ float2 sB[256];
-  __shared__ uint32_t sId[256];
-  __shared__ float sW[256];
-  __shared__ uint32_t sC[256];
-  __shared__ uint32_t sStop[4];
-  const int tile = blockIdx.x;
-  const int tx = tile % grid_x, ty = tile / grid_x;
-  const int t = threadIdx.x;
-  const int wave = t >> 6;
-  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
-  const bool inside = px < W && py < H;
-  const float fx = (float)px, fy = (float)py;
-  const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
-  const uint32_t len = end - start;
-  float T = 1.0f, wmax = 0.0f;
-  uint32_t imax = 0;
-  bool flag = false;
-  const int lane = lane_id();
-  uint32_t wave_stop = 0xffffffffu;  // first pair end with all 64 pixels below the cut
-  for (uint32_t base = 0; base < len; base += 256) {
-    const uint32_t n = min(256u, len - base);
-    if (t < (int)n) {
-      const uint32_t id = vals[start + base + t];
-      const float4* r = (const float4*)(splat + 12 * (size_t)id);
-      const float4 r1 = r[1];
-      sA[t] = r[0];
-      sB[t] = make_float2(r1.x, r1.y);
-      sId[t] = id;
-    }
-    sW[t] = 0.0f;
-    sC[t] = 0u;
-    __syncthreads();
-    for (uint32_t j = 0; j < n; ++j) {
-      float w = 0.0f;
-      bool hit = false;
-      if (inside) {
-        const float4 A = sA[j];
-        const float2 B = sB[j];
-        const float a = splat_alpha(A.x, A.y, A.z, A.w, B.x, B.y, fx, fy);
-        if (a != 0.0f) {  // alpha >= 1/255
-          w = a * T;
-          T = T - w;
-          hit = true;
-          if (wmax < w) {
-            wmax = w;
-            imax = sId[j];
-            flag = true;
-          }
-        }
-      }
-      const uint64_t hits = __ballot(hit);
-      if (hits) {
-        const float ws = wave_sum(w);
-        if (lane == 0) {
-          atomicAdd(&sW[j], ws);
-          atomicAdd(&sC[j], (uint32_t)__popcll(hits));
-        }
-      }
-      // pairs follow the leading single of an odd-length list: position pos
-      // ends a pair when it is odd (even length) or even and >= 2 (odd length)
-      const uint32_t pos = base + j;
-      const bool pair_end = (len & 1u) ? (pos >= 2u && (pos & 1u) == 0u) : ((pos & 1u) == 1u);
-      if (pair_end && wave_stop == 0xffffffffu && __ballot(T < T_CUT_F) == ~0ull) wave_stop = pos;
-    }
-    if (lane == 0) sStop[wave] = wave_stop;
-    __syncthreads();
-    const uint32_t stop = max(max(sStop[0], sStop[1]), max(sStop[2], sStop[3]));
-    if (t < (int)n && base + t <= stop && sC[t] != 0u) {
-      atomic_add_f(&wsum[sId[t]], sW[t]);
-      atomicAdd(&wcount[sId[t]], (int32_t)sC[t]);
-    }
-    if (stop != 0xffffffffu) break;
-    __syncthreads();
-  }
-  // one atomic per distinct maximum in the wave (neighbouring pixels mostly
-  // share it; the counts are integers, so the float sums stay exact)
-  const bool has = inside && flag;
-  uint64_t pending = __ballot(has);
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)imax, leader);
-    const uint64_t same = __ballot(has && imax == k) & pending;
-    if (lane == leader) atomic_add_f(&maxcount[k], (float)__popcll(same));
-    pending &= ~same;
+  const int b = (int)blockIdx.x;
+  if (b < g.n_pieces) {
+    simp_piece(b, g, ranges, vals, splat, wsum, wcount, maxcount);
+  } else {
+    const int i = b - g.n_pieces;
+    simp_tile((i / g.gxi) * g.gx + i % g.gxi, g, ranges, vals, splat, wsum, wcount, maxcount);
   }
 }
 
 hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
                               const float* splat, float* wsum, int32_t* wcount, float* maxcount,
                               hipStream_t s) {
-  if (T > 0)
+  if (T <= 0) return hipSuccess;
+  const SimpGrid g = simp_grid(W, H);
+  const int grid = g.n_pieces + g.gxi * g.gyi;
+  if (grid > 0)
     FGS_TIMED("render_simp", s,
-              (k_render_simp<<<T, 256, 0, s>>>(W, H, (W + TILE_W - 1) / TILE_W, ranges, vals, splat,
-                                                wsum, wcount, maxcount)));
+              (k_render_simp<<<grid, 256, 0, s>>>(g, ranges, vals, splat, wsum, wcount, maxcount)));
   return hipGetLastError();
 }
 

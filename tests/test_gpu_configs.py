@@ -9,8 +9,9 @@ seeded inputs:
 * heavy   1M Gaussians (2% large splats), 1280x840: 21M pairs, above the
           reference's 2^24 pair cap (__init__.py:21-22, :118-119) and above the
           4M keys where the tile sort switches to three-launch passes; FAST
-          forward and backward as above, PLAIN forward, and the drop-in
-          default (FAST, bucket 128) forward;
+          forward and backward as above, render_simp and render_depth over
+          its lists, PLAIN forward, and the drop-in default (FAST, bucket
+          128) forward;
 * dense   3M Gaussians, 1920x1080 (125M pairs): an orbit frame against the
           oracle, and graph replays of orbit frames equal to eager frames;
 * multiview  2M Gaussians, 16 views of uneven sizes in one batch: every view
@@ -94,6 +95,21 @@ def test_heavy_1M_1280x840_fast(heavy):
     assert fr.R > (1 << 24) and fr.R > (1 << 22)  # above the reference's cap and 4M keys
     orc = Oracle(scene, cam, bucket=32, mode=0).forward()
     assert_forward_equal(fr, orc)
+    # the list passes over the same lists: render_simp (counts and maxima
+    # exact, weight sums within 1e-5) and render_depth (bit for bit); the
+    # last tile row (840 = 52.5 tiles) never stops and walks whole lists
+    w, c, m = fr.render_simp()
+    rw, rc, rm = orc.render_simp()
+    np.testing.assert_array_equal(c.cpu().numpy(), rc)
+    np.testing.assert_array_equal(m.cpu().numpy(), rm.astype(np.float32))
+    grad_close(w.cpu().numpy(), rw, rtol=1e-5, name="accum weights")
+    assert rc.sum() > 0 and 0 < rm.sum() <= cam.width * cam.height
+    res = fr.render_depth()
+    pts, depth, alpha, gidx, disc = orc.render_depth()
+    np.testing.assert_array_equal(res["gidx"][0].cpu().numpy(), gidx)
+    np.testing.assert_array_equal(res["rendered_depth"][0].cpu().numpy(), depth)
+    np.testing.assert_array_equal(res["accum_alpha"][0].cpu().numpy(), alpha)
+    del res, w, c, m
     dimg = _l2_grad(orc, cam)
     got = fr.backward(torch.from_numpy(dimg).cuda())
     torch.cuda.synchronize()
