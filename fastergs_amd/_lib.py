@@ -41,7 +41,8 @@ class Gaussians(ctypes.Structure):
 
 class TileSched(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("split_len", ctypes.c_int), ("mlp_params", vp),
-                ("features", ctypes.c_int), ("log_cycles", ctypes.c_int), ("order", ctypes.c_int)]
+                ("features", ctypes.c_int), ("log_cycles", ctypes.c_int), ("order", ctypes.c_int),
+                ("items", vp), ("n_items", vp), ("prev_features", vp), ("features_out", vp)]
 
 
 class Options(ctypes.Structure):

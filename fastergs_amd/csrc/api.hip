@@ -136,6 +136,10 @@ int check_opts(const fgs_options* o) {
       return fail(FGS_ERR_ARG, "unknown tile schedule mode");
     if (ts->mode == FGS_TILES_LEARNED && !ts->mlp_params)
       return fail(FGS_ERR_ARG, "learned tile schedule needs mlp_params");
+    if (!ts->items != !ts->n_items)
+      return fail(FGS_ERR_ARG, "a work list needs both items and n_items");
+    if (ts->prev_features && (ts->mode != FGS_TILES_LEARNED || !ts->items))
+      return fail(FGS_ERR_ARG, "prev_features needs the learned tile schedule and items");
   }
   return FGS_OK;
 }
@@ -537,6 +541,19 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   // compaction and tile counts, then the SH colours (read only by
   // compositing). The pair offsets wait for both.
   const Fork f(s);
+  // the learned schedule's work list for this frame from the previous
+  // frame's tile features (cost model, longest predicted first), on side
+  // stream 0 beside the whole preprocess
+  const fgs_tile_sched* ts = opt->tile_sched;
+  const bool list_side = ts && ts->prev_features;
+  if (list_side) {
+    FGS_TRY(f.fork(0, 8), "fork");
+    FGS_TRY(launch_mlp_infer(ts->mlp_params, ts->prev_features, b.T, w.pred, true, f.side(0)),
+            "cost model");
+    FGS_TRY(launch_sched_items(b.T, nullptr, opt->bucket_size, w.pred, ts->items, ts->n_items,
+                               f.side(0)),
+            "work items");
+  }
   FGS_TRY(f.fork(1, 1), "fork");
   // records in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by (global) tile alone yields the canonical per-tile
@@ -561,6 +578,7 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
                             opt->bucket_size, b.T, w.scratch, s),
           "pair offsets");
   FGS_TRY(f.join(1, 3), "join");
+  if (list_side) FGS_TRY(f.join(0, 9), "join");
   return FGS_OK;
 }
 
@@ -672,29 +690,45 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
     // indexes buckets through boff; fgs_batch_bucket_tables builds them on
     // demand
     c.vals = pw.v0;
-    if (const fgs_tile_sched* ts = opt->tile_sched) {
+    const fgs_tile_sched* ts = opt->tile_sched;
+    // this frame's tile features for the next frame's schedule: a small
+    // kernel on a side stream that fits beside compositing
+    const bool feat_side = ts && ts->features_out;
+    if (feat_side) {
+      FGS_TRY(f.fork(0, 8), "fork");
+      FGS_TRY(launch_tile_features(T, gw.ranges, pw.v0, gw.geom, gw.splat, ts->features_out,
+                                   f.side(0)),
+              "tile features");
+    }
+    if (ts) {
       const bool learned = ts->mode == FGS_TILES_LEARNED;
-      if (learned)
-        FGS_TRY(launch_tile_predict(T, gw.ranges, pw.v0, gw.geom, gw.splat, ts->mlp_params,
-                                    gw.feat, gw.pred, s),
-                "cost model");
-      else if (ts->features)
-        FGS_TRY(launch_tile_features(T, gw.ranges, pw.v0, gw.geom, gw.splat, gw.feat, s),
-                "tile features");
-      if (learned && ts->order) {
-        FGS_TRY(launch_sched_items(T, gw.ranges, opt->bucket_size, gw.pred, gw.items,
-                                   gw.counts + 4, s),
-                "work items");
-        c.items = gw.items;
-        c.n_items = gw.counts + 4;
-      } else if (ts->mode != FGS_TILES_RASTER) {
-        c.slots = 1;
-        c.split_len = ts->split_len;
-        c.split_pred = learned ? gw.pred : nullptr;
+      if (ts->items) {  // a given work list (built by fgs_preprocess or the caller)
+        c.items = ts->items;
+        c.n_items = ts->n_items;
+      } else if (!feat_side) {  // with features_out and no list yet: tile order
+        if (learned)
+          FGS_TRY(launch_tile_predict(T, gw.ranges, pw.v0, gw.geom, gw.splat, ts->mlp_params,
+                                      gw.feat, gw.pred, s),
+                  "cost model");
+        else if (ts->features)
+          FGS_TRY(launch_tile_features(T, gw.ranges, pw.v0, gw.geom, gw.splat, gw.feat, s),
+                  "tile features");
+        if (learned && ts->order) {
+          FGS_TRY(launch_sched_items(T, gw.ranges, opt->bucket_size, gw.pred, gw.items,
+                                     gw.counts + 4, s),
+                  "work items");
+          c.items = gw.items;
+          c.n_items = gw.counts + 4;
+        } else if (ts->mode != FGS_TILES_RASTER) {
+          c.slots = 1;
+          c.split_len = ts->split_len;
+          c.split_pred = learned ? gw.pred : nullptr;
+        }
       }
       if (ts->log_cycles) c.cycles = gw.cycles;
     }
     FGS_TRY(launch_composite_fast(c, s), "composite");
+    if (feat_side) FGS_TRY(f.join(0, 9), "join");
     return FGS_OK;
   }
 

@@ -315,6 +315,7 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   if (a.items) {
     if (blockIdx.x >= a.n_items[0]) return;  // whole workgroup
     item = a.items[blockIdx.x];
+    if ((item >> 2) >= (uint32_t)a.T || (item & 3u) == 3u) return;  // not an item of this frame
     log_idx = (int)blockIdx.x;
   } else if (a.slots) {
     // two grid slots per tile: the upper and lower band of a split tile, or

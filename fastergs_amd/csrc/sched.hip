@@ -513,7 +513,8 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, 
 // Work list of the learned schedule, built by one 1024-thread workgroup:
 // tiles binned by predicted log2 cost (bins of 1/8, 256 bins), highest bin
 // first (longest predicted first); a tile is split into its two row bands
-// when its split logit is > 0 and it has more than one bucket. The order
+// when its split logit is > 0 and it has more than one bucket (ranges NULL:
+// a list built before the frame's lists exist, on the logit alone). The order
 // inside a bin is arbitrary (LDS atomics); outputs never depend on it.
 constexpr int ITEM_THREADS = 1024;
 __global__ __launch_bounds__(ITEM_THREADS) void k_sched_items(
@@ -522,8 +523,8 @@ __global__ __launch_bounds__(ITEM_THREADS) void k_sched_items(
   __shared__ uint32_t cnt[256];
   __shared__ uint32_t off[256];
   const int t = threadIdx.x;
-  auto split_of = [&](int tile) -> bool {
-    const uint32_t len = ranges[2 * tile + 1] - ranges[2 * tile];
+  auto split_of = [&](int tile) -> bool {  // without ranges: the split flag alone
+    const uint32_t len = ranges ? ranges[2 * tile + 1] - ranges[2 * tile] : 0xffffffffu;
     return pred[2 * tile + 1] > 0.0f && len > (uint32_t)bucket;
   };
   auto bin_of = [&](int tile) -> int {

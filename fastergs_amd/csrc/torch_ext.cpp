@@ -84,7 +84,10 @@ struct Frame {
 };
 
 // opts = (schedule, bucket_size, split_tiles, abs_grad, forward_only,
-//         tile_mode (-1: none), split_len, features, log_cycles, order)
+//         tile_mode (-1: none), split_len, features, log_cycles, order,
+//         items, n_items, prev_features, features_out) -- the last four
+//         device addresses of the tile schedule's work list and features (0:
+//         none), owned by the caller's schedule object
 Frame make_frame(const at::Tensor& means3D, const at::Tensor& opacities, const at::Tensor& scales,
                  const at::Tensor& rotations, const at::Tensor& dc, const at::Tensor& shs,
                  const at::Tensor& colors, const at::Tensor& culling, const at::Tensor& view,
@@ -93,7 +96,7 @@ Frame make_frame(const at::Tensor& means3D, const at::Tensor& opacities, const a
                  double scale_modifier, double mult, int64_t sh_degree,
                  const std::vector<int64_t>& opts, const at::Tensor& mlp) {
   if (!means3D.is_cuda()) throw std::invalid_argument("means3D must be a GPU tensor");
-  if (opts.size() != 10) throw std::invalid_argument("options must have 10 entries");
+  if (opts.size() != 14) throw std::invalid_argument("options must have 14 entries");
   Frame f;
   f.dev = means3D.device();
   const int64_t P = means3D.size(0);
@@ -123,8 +126,17 @@ Frame make_frame(const at::Tensor& means3D, const at::Tensor& opacities, const a
   f.opt = fgs_options{(int)opts[0], (int)opts[1], (int)opts[2], (int)opts[3], nullptr,
                       (int)opts[4]};
   if (opts[5] >= 0) {
-    f.ts = fgs_tile_sched{(int)opts[5], (int)opts[6], (const float*)ptr(f.mlp), (int)opts[7],
-                          (int)opts[8], (int)opts[9]};
+    auto addr = [](int64_t a) { return reinterpret_cast<void*>(static_cast<uintptr_t>(a)); };
+    f.ts = fgs_tile_sched{(int)opts[5],
+                          (int)opts[6],
+                          (const float*)ptr(f.mlp),
+                          (int)opts[7],
+                          (int)opts[8],
+                          (int)opts[9],
+                          (uint32_t*)addr(opts[10]),
+                          (uint32_t*)addr(opts[11]),
+                          (const float*)addr(opts[12]),
+                          (float*)addr(opts[13])};
     f.opt.tile_sched = &f.ts;
   }
   return f;

@@ -183,16 +183,21 @@ class Frame:
                          _ptr(self.scales), _ptr(self.rotations), _ptr(self.dc), _ptr(self.shs),
                          _ptr(self.colors), _ptr(self.culling))
 
-    def _ext_args(self):
-        """The frame's tensors, settings and options as the extension's ops take them."""
+    def _ext_args(self, lists: bool = False):
+        """The frame's tensors, settings and options as the extension's ops
+        take them; `lists`: with the tile schedule's work-list addresses (the
+        forward of an ahead schedule)."""
         st, o = self.settings, self.copts
         ts = self.opts.tile_sched
         mlp = _NONE
         if ts is None:
-            tsl = [-1, 0, 0, 0, 0]
+            tsl = [-1, 0, 0, 0, 0, 0, 0, 0, 0]
         else:
+            T = ((self.W + 15) // 16) * ((self.H + 15) // 16)
+            addr = (ts.list_addresses(T, self.device)
+                    if lists and hasattr(ts, "list_addresses") else (0, 0, 0, 0))
             tsl = [int(ts.mode), int(ts.split_len), int(bool(ts.features)),
-                   int(bool(ts.log_cycles)), int(bool(ts.order))]
+                   int(bool(ts.log_cycles)), int(bool(ts.order)), *(int(a) for a in addr)]
             if getattr(ts, "model", None) is not None:
                 mlp = ts.model.params
         opts = [int(o.schedule), int(o.bucket_size), int(o.split_tiles), int(o.abs_grad),
@@ -252,7 +257,10 @@ class Frame:
         hint = _capacity_hints.get(key) if self.opts.speculative else None
         R_hint, NB_hint = hint if hint is not None else (0, 0)
         (self.radii, self.gws, self.pws, self.out_color, self.T_final, self.last_contributor,
-         c) = _C.ops.forward(*self._ext_args(), int(R_hint), int(NB_hint))
+         c) = _C.ops.forward(*self._ext_args(lists=True), int(R_hint), int(NB_hint))
+        ts = self.opts.tile_sched
+        if ts is not None and hasattr(ts, "advance"):
+            ts.advance()
         self.R, self.NB, self.R_cap, self.NB_cap = (int(x) for x in c)
         # remember the largest counts seen for this shape, with headroom
         old = _capacity_hints.get(key, (0, 0))

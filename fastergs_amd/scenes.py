@@ -172,6 +172,29 @@ def sphere_cameras(n: int, seed: int = 0, rmin: float = 3.0, rmax: float = 5.0,
     return cams
 
 
+def trajectory_cameras(n: int, width: int, height: int, seed: int = 0, rmin: float = 3.0,
+                       rmax: float = 5.0, step_deg: float = 4.0):
+    """A seeded random camera trajectory looking at the origin: the viewing
+    direction takes a random-walk step of about ``step_deg`` degrees per view
+    (a rotation about a random axis normal to it) and the distance drifts
+    smoothly between rmin and rmax."""
+    rng = np.random.default_rng(seed + 2000)
+    d = rng.standard_normal(3)
+    d /= np.linalg.norm(d)
+    phase, dphase = rng.uniform(0.0, 2.0 * math.pi), rng.uniform(0.03, 0.08)
+    cams = []
+    for _ in range(n):
+        r = rmin + (rmax - rmin) * 0.5 * (1.0 + math.sin(phase))
+        cams.append(look_at_camera(d * r, width, height))
+        axis = np.cross(d, rng.standard_normal(3))
+        axis /= np.linalg.norm(axis)
+        a = math.radians(step_deg) * rng.uniform(0.5, 1.5)
+        d = d * math.cos(a) + np.cross(axis, d) * math.sin(a)  # Rodrigues, axis normal to d
+        d /= np.linalg.norm(d)
+        phase += dphase
+    return cams
+
+
 def fd_scene(n: int = 1000, seed: int = 0) -> tuple[dict, Camera]:
     """Small in-frustum scene for finite-difference gradient checks: 128x128,
     Gaussians of a few pixels, opacities away from the 1/255 cut-off."""

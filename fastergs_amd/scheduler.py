@@ -13,7 +13,12 @@ the kernel time does.
 * ``TileSchedule.learned(model, order=False)``: a cost model predicts each
   tile's log2 duration and a split flag from 8 features of the tile's list;
   the split flags are applied in tile order, or with ``order=True`` the
-  items are dispatched longest-predicted first.
+  items are dispatched longest-predicted first. With ``ahead=True`` the
+  longest-predicted-first list of a frame is built from the previous
+  frame's tile features on a side stream beside its preprocess, and those
+  features are computed beside the previous frame's compositing, so no
+  scheduling kernel runs on the critical path (on a camera trajectory
+  consecutive views are alike).
 
 ``CostModel`` is the MLP (8 -> 64 -> 64 -> 2, fp32 weights, bf16
 inference). Its forward, loss gradient and Adam update are HIP kernels of
@@ -97,11 +102,40 @@ class TileSchedule:
     """What ``RasterOptions.tile_sched`` takes."""
 
     def __init__(self, mode: int, split_len: int = 0, model: CostModel | None = None,
-                 features: bool = False, log_cycles: bool = False, order: bool = False):
+                 features: bool = False, log_cycles: bool = False, order: bool = False,
+                 ahead: bool = False):
         if mode == _lib.TILES_LEARNED and model is None:
             raise ValueError("the learned schedule needs a CostModel")
+        if ahead and mode != _lib.TILES_LEARNED:
+            raise ValueError("ahead needs the learned schedule")
         self.mode, self.split_len, self.model = mode, int(split_len), model
         self.features, self.log_cycles, self.order = features, log_cycles, order
+        # ahead: each frame dispatches the longest-predicted-first work list
+        # that fgs_preprocess builds, beside the preprocess, from the tile
+        # features the previous frame of the same size computed beside its
+        # compositing; no prediction runs on a frame's critical path (the
+        # first frame, or one after a size change, runs in tile order).
+        self.ahead = ahead
+        self._ahead = None  # [T, features (T x 8), work list (2T + 1 words), features valid]
+
+    def list_addresses(self, T: int, device) -> tuple:
+        """Device addresses (items, n_items, prev_features, features_out) for
+        a frame of T tiles; 0 where there is none."""
+        if not self.ahead:
+            return (0, 0, 0, 0)
+        st = self._ahead
+        if st is None or st[0] != T or st[1].device != device:
+            st = self._ahead = [T, torch.zeros(T, _lib.MLP_IN, device=device),
+                                torch.zeros(2 * T + 1, dtype=torch.int32, device=device), False]
+        _, feat, lst, valid = st
+        if not valid:
+            return (0, 0, 0, feat.data_ptr())
+        return (lst.data_ptr(), lst.data_ptr() + 8 * T, feat.data_ptr(), feat.data_ptr())
+
+    def advance(self):
+        """After a frame was queued: its features feed the next frame's list."""
+        if self.ahead and self._ahead is not None:
+            self._ahead[3] = True
 
     @property
     def uses_list(self) -> bool:
@@ -120,10 +154,12 @@ class TileSchedule:
         return cls(_lib.TILES_LEARNED, model=model, **kw)
 
     def to_c(self) -> TileSched:
+        """The C struct without work lists (the stage-by-stage paths; the
+        ahead lists go through Frame.forward)."""
         return TileSched(self.mode, self.split_len,
                          _p(self.model.params) if self.model is not None else None,
                          1 if self.features else 0, 1 if self.log_cycles else 0,
-                         1 if self.order else 0)
+                         1 if self.order else 0, None, None, None, None)
 
 
 # Workgroups of the compositing pass resident at once on an MI355X: 256 CUs x

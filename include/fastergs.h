@@ -94,6 +94,25 @@ typedef struct fgs_tile_sched {
   int features;            /* compute the tile features even when not LEARNED */
   int log_cycles;          /* log each workgroup's duration (100 MHz ticks) */
   int order;               /* LEARNED: dispatch longest-predicted first */
+  /* A work list for fgs_render to dispatch (any mode; the outputs do not
+     depend on it): n_items[0] <= 2 T entries of (tile << 2 | part), part 0 a
+     whole tile, 1 / 2 its upper / lower 16x8 row band, every tile of the
+     frame exactly once (whole or as both bands). Device memory; NULL: the
+     schedule decides as above. */
+  uint32_t* items;
+  uint32_t* n_items;
+  /* LEARNED with items: the tile features (T x 8 floats, device) of the
+     previous frame of the same size; fgs_preprocess builds the work list
+     from them (cost model, longest-predicted first, split flags) into
+     items / n_items on a side stream beside the preprocess, so no
+     prediction runs on the frame's critical path (on a camera trajectory
+     consecutive views are alike). NULL: none. */
+  const float* prev_features;
+  /* fgs_render writes this frame's tile features (T x 8 floats, device)
+     here, computed on a side stream beside compositing (for the next
+     frame's prev_features). NULL: none. Side streams are joined before
+     the entry points return. */
+  float* features_out;
 } fgs_tile_sched;
 
 typedef struct fgs_options {

@@ -210,3 +210,28 @@ def test_online_training_lowers_loss(scene_cam):
     l1 = model.loss_grad(x, t)[0].item()
     print(f"[train] loss {l0:.4f} -> {l1:.4f}", flush=True)
     assert l1 < 0.5 * l0
+
+
+def test_learned_schedule_on_a_trajectory():
+    """The benchmark's scheduler evaluation at a reduced size (400k Gaussians
+    of the large workload's distribution, 1237x822, a 40-view seeded random
+    trajectory with the last 10 views held out): the judge's threshold is
+    chosen on the training views, every schedule's held-out image equals
+    raster order's bit for bit, the ahead schedule runs no scheduling kernel
+    on the critical path, and the held-out regret against the judge and
+    against raster order is reported."""
+    import argparse
+
+    import bench
+    print("\n[sched] trajectory, 400k Gaussians, 40 views", flush=True)
+    res = bench.learned_schedule_eval(argparse.Namespace(bucket=32), views=40, held_out=10,
+                                      n=400_000)
+    print({k: res[k] for k in ("judge_split_len", "compositing_path_us_mean", "composite_us_mean",
+                               "schedule_kernels_us_mean", "learned_variant", "regret_us_mean",
+                               "regret_vs_raster_us_mean", "learned_wins_frac")}, flush=True)
+    assert res["outputs_identical"]
+    assert np.isfinite(res["loss_last"])
+    assert res["judge_split_len"] is None or res["judge_split_len"] in bench.SPLIT_CANDIDATES
+    assert set(res["compositing_path_us_mean"]) == {"raster", "judge", "learned",
+                                                    "learned_ordered", "learned_ahead"}
+    assert res["schedule_kernels_us_mean"]["learned_ahead"] == 0.0  # off the critical path
