@@ -535,11 +535,14 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   FGS_TRY(launch_preprocess_fast(a, b.setup, (ViewRec*)w.views, w.counts, GAUSS_SYNC_WORDS, s),
           "preprocess");
   if (g->P == 0) return FGS_OK;
-  // Independent chains after the geometry pass: on a side stream the depth
-  // sort of the records (queued first: its workgroups take their places
-  // before the tile counts fill the GPU); on the main stream the large-rect
-  // compaction and tile counts, then the SH colours (read only by
-  // compositing). The pair offsets wait for both.
+  // Independent chains after the geometry pass: on a side stream the
+  // large-rect compaction and tile counts, then the SH colours (read only by
+  // compositing), which fill the GPU that the depth sort's last,
+  // latency-bound passes leave idle; on the main stream the depth sort of
+  // the records, the longer chain (a cross-stream wait costs the waiting
+  // stream 10-15 us, so the longer chain stays on the main stream and the
+  // side stream has finished when it is joined). The pair offsets wait for
+  // the tile counts only; the colours are joined at the end.
   const Fork f(s);
   // the learned schedule's work list for this frame from the previous
   // frame's tile features (cost model, longest predicted first), on side
@@ -555,29 +558,29 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
             "work items");
   }
   FGS_TRY(f.fork(1, 1), "fork");
+  // large rects, one list entry per row band (count in counts[3]), each band
+  // walked by a whole wave
+  FGS_TRY(expand_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, f.side(1)),
+          "large bands");
+  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
+                             w.counts, w.pcount, nullptr, nullptr, w.lband, 0u, nullptr, nullptr,
+                             false, f.side(1)),
+          "large tiles");
+  FGS_TRY(f.mark(1, 2), "mark");
+  FGS_TRY(launch_sh_color(a, f.side(1)), "colour");
   // records in (depth, index) order; pair offsets assigned in that order so
   // that a stable sort by (global) tile alone yields the canonical per-tile
   // order of every view
   bool alt = false;
   FGS_TRY(sort_kv<uint64_t>(w.dsort_k0, w.dsort_v0, w.dsort_k1, w.dsort_v1, PV, 32, 64,
-                            w.scratch, f.side(1), alt, nullptr, nullptr, SORT_DEPTH, w.dsort_sync),
+                            w.scratch, s, alt, nullptr, nullptr, SORT_DEPTH, w.dsort_sync),
           "depth sort");
   if (alt) return fail(FGS_ERR_HIP, "depth sort parity");
-  FGS_TRY(f.mark(1, 2), "mark");
-  // large rects, one list entry per row band (count in counts[3]), each band
-  // walked by a whole wave; then the SH colours, which fill the CUs the
-  // latency-bound depth sort (a few workgroups per pass) leaves idle
-  FGS_TRY(expand_flags(w.lflag, PV, w.large_ids, w.counts + 3, w.cscratch, s), "large bands");
-  FGS_TRY(launch_large_tiles(false, g->P, b.V, (const ViewRec*)w.views, w.geom, w.large_ids,
-                             w.counts, w.pcount, nullptr, nullptr, w.lband, 0u, nullptr, nullptr,
-                             false, s),
-          "large tiles");
-  FGS_TRY(launch_sh_color(a, s), "colour");
   FGS_TRY(f.wait(2), "join");
   FGS_TRY(scan_pair_offsets(w.dsort_k0, w.dsort_v0, w.pcount, w.lrank, PV, w.poff, w.counts,
                             opt->bucket_size, b.T, w.scratch, s),
           "pair offsets");
-  FGS_TRY(f.join(1, 3), "join");
+  FGS_TRY(f.join(1, 3), "join");  // the colours
   if (list_side) FGS_TRY(f.join(0, 9), "join");
   return FGS_OK;
 }
