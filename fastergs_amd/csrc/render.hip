@@ -256,58 +256,20 @@ hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s) {
 // One 256-thread workgroup (4 wave64) per work item: a whole 16x16 tile, or
 // one of its two 16x8 row bands when the tile schedule splits it (pixels
 // are independent, so a band does exactly the per-pixel work of the whole
-// tile for its rows). The item's buckets are composited in rounds; every
-// bucket from T = 1 (render.cu:130-275), its splats staged in LDS. The
-// partials (T_b, C_b) of a round's buckets are then merged in order per
-// pixel (render.cu:329-362), the checkpoint after each bucket is stored for
-// the backward (unless the frame is forward-only), and a pixel stops after
-// the first bucket that leaves T < 1e-4. The workgroup stops at the first
-// round in which every pixel of the item has stopped, so buckets past the
-// cutoff (which the reference renders and then discards) are not evaluated.
-//
-// Round widths ramp up, 1, 2, then 4 buckets for a whole tile (2, then 4 for
-// a band): in a round of K buckets, 4 / K waves share each bucket, each
-// taking a part of the item's pixels (1, 2 or 4 per lane), so a tile whose
-// pixels all stop within the first buckets evaluates little beyond them,
-// while later rounds keep four buckets in flight. Every pixel's partial of a
-// bucket is the same sequential blend wherever it is computed, so outputs do
-// not depend on the round widths, the item order or the splitting.
-
-// Sequential blend of n staged splats from T = 1 for the NPX pixels of a
-// lane: (x0 + (k & 1), y0 + (k >> 1)) for k < NPX; partials into out[lp0 +
-// (k & 1) + TILE_W (k >> 1)].
-template <int NPX>
-__device__ __forceinline__ void bucket_pass(const float4* sA, const float4* sB, const float* sC,
-                                            int n, float fx0, float fy0, float4* out, int lp0) {
-  const float fx1 = fx0 + 1.0f, fy1 = fy0 + 1.0f;
-  float T0 = 1.f, T1 = 1.f, T2 = 1.f, T3 = 1.f;
-  float c00 = 0.f, c01 = 0.f, c02 = 0.f, c10 = 0.f, c11 = 0.f, c12 = 0.f;
-  float c20 = 0.f, c21 = 0.f, c22 = 0.f, c30 = 0.f, c31 = 0.f, c32 = 0.f;
-  for (int k = 0; k < n; ++k) {
-    const float4 A = sA[k], Bv = sB[k];
-    const float cb = sC[k];
-    blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx0, fy0), Bv.z, Bv.w, cb, T0, c00, c01, c02);
-    if (NPX >= 2)
-      blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx1, fy0), Bv.z, Bv.w, cb, T1, c10, c11, c12);
-    if (NPX == 4) {
-      blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx0, fy1), Bv.z, Bv.w, cb, T2, c20, c21, c22);
-      blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx1, fy1), Bv.z, Bv.w, cb, T3, c30, c31, c32);
-    }
-  }
-  out[lp0] = make_float4(c00, c01, c02, T0);
-  if (NPX >= 2) out[lp0 + 1] = make_float4(c10, c11, c12, T1);
-  if (NPX == 4) {
-    out[lp0 + TILE_W] = make_float4(c20, c21, c22, T2);
-    out[lp0 + TILE_W + 1] = make_float4(c30, c31, c32, T3);
-  }
-}
-
-template <int BMAX>
+// tile for its rows, with the upper 128 threads staging only). One pixel
+// per thread; splats staged 256 at a time in LDS. Every bucket is blended
+// from T = 1 (render.cu:130-275) and merged into the pixel's running (T, C)
+// in bucket order (render.cu:329-362); the checkpoint after each bucket is
+// stored for the backward (unless the frame is forward-only), and a pixel
+// stops after the first bucket that leaves T < 1e-4. A wave whose 64 pixels
+// have all stopped skips the rest of the list, and the workgroup leaves once
+// all its pixels have: buckets past the cut-off (which the reference renders
+// and then discards) are not evaluated. The outputs do not depend on the
+// item order or the splitting.
 __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
-  __shared__ float4 sA[4][BMAX];
-  __shared__ float4 sB[4][BMAX];
-  __shared__ float sC[4][BMAX];
-  __shared__ float4 sP[4][TILE_PIX];
+  __shared__ float4 sA[256];
+  __shared__ float4 sB[256];
+  __shared__ float sC[256];
   __shared__ int wmax[4];
   const uint64_t t_start = a.cycles ? wall_clock64() : 0;
   uint32_t item;
@@ -349,71 +311,53 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   const int tx = lt % gx, ty = lt / gx;
   const int t = threadIdx.x;
   const int w = t / WAVE, lane = t & (WAVE - 1);
-  // merge / output pixel of this thread (lp: index inside the tile)
+  // this thread's pixel (a band's upper 128 threads have none)
   const int mrow = row0 + (t >> 4);
   const int lp = mrow * TILE_W + (t & 15);
   const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + mrow;
-  const bool merger = (t >> 4) < nrows;
-  const bool inside = merger && px < W && py < H;
+  const bool mine = (t >> 4) < nrows;
+  const bool inside = mine && px < W && py < H;
+  const float fx = (float)px, fy = (float)py;
   const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
   const int B = a.bucket;
-  const int nb = (int)((end - start + B - 1) / B);
   const uint32_t b0 = a.boff[tile];
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
   int cnt = 0;
   bool done = !inside;
-  int first = 0;  // first bucket of the round
-  for (int round = 0; first < nb; ++round) {
+  for (uint32_t base = start; base < end; base += 256) {
     if (__syncthreads_count(!done) == 0) break;
-    // buckets this round (K) and waves per bucket (G = 4 / K)
-    const int K = band ? (round == 0 ? 2 : 4) : (round == 0 ? 1 : (round == 1 ? 2 : 4));
-    const int G = 4 / K;
-    const int q = w / G, sub = w % G;  // this wave's bucket in the round, its share
-    const int bl = first + q;
-    const uint32_t s = start + (uint32_t)bl * B;
-    const int n = bl < nb ? (int)min((uint32_t)B, end - s) : 0;
-    for (int k = sub * WAVE + lane; k < n; k += G * WAVE) {
-      const float4* r = (const float4*)(a.splat + 12 * (size_t)a.vals[s + k]);
+    const uint32_t k = base + (uint32_t)t;
+    if (k < end) {
+      const float4* r = (const float4*)(a.splat + 12 * (size_t)a.vals[k]);
       const float4 r1 = r[1], r2 = r[2];
-      sA[q][k] = r[0];
-      sB[q][k] = make_float4(r1.x, r1.y, r2.x, r2.y);  // cc, log2 op, r, g
-      sC[q][k] = r2.z;
-    }
-    __syncthreads();
-    if (n > 0) {
-      // this wave's pixels: 256 / G of the tile (128 / G of a band)
-      const int pxl = band ? 128 / G : 256 / G;
-      if (pxl == 64) {  // one per lane: 4 rows of 16
-        const int qx = lane & 15, qy = (band ? row0 : 0) + 4 * sub + (lane >> 4);
-        bucket_pass<1>(sA[q], sB[q], sC[q], n, (float)(tx * TILE_W + qx),
-                       (float)(ty * TILE_H + qy), sP[q], qy * TILE_W + qx);
-      } else if (pxl == 128) {  // two per lane: 8 rows of 16 in 2x1 pairs
-        const int qx = 2 * (lane & 7), qy = (band ? row0 : 8 * sub) + (lane >> 3);
-        bucket_pass<2>(sA[q], sB[q], sC[q], n, (float)(tx * TILE_W + qx),
-                       (float)(ty * TILE_H + qy), sP[q], qy * TILE_W + qx);
-      } else {  // four per lane: the tile in 2x2 quads
-        const int qx = 2 * (lane & 7), qy = 2 * (lane >> 3);
-        bucket_pass<4>(sA[q], sB[q], sC[q], n, (float)(tx * TILE_W + qx),
-                       (float)(ty * TILE_H + qy), sP[q], qy * TILE_W + qx);
-      }
+      sA[t] = r[0];
+      sB[t] = make_float4(r1.x, r1.y, r2.x, r2.y);  // cc, log2 op, r, g
+      sC[t] = r2.z;
     }
     __syncthreads();
     if (!done) {
-      for (int j = 0; j < K; ++j) {
-        const int bq = first + j;
-        if (bq >= nb) break;
-        const float4 v = sP[j][lp];
-        merge(v.w, v.x, v.y, v.z, T, C0, C1, C2);
+      const int n = (int)min(256u, end - base);
+      // bucket index of the batch's first bucket (256 is a multiple of B)
+      const uint32_t bq0 = (base - start) / (uint32_t)B;
+      for (int j = 0, q = 0; j < n; j += B, ++q) {
+        const int e = min(j + B, n);
+        float Tb = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+        for (int i = j; i < e; ++i) {
+          const float4 A = sA[i], Bv = sB[i];
+          blend(splat_alpha(A.x, A.y, A.z, A.w, Bv.x, Bv.y, fx, fy), Bv.z, Bv.w, sC[i], Tb, c0,
+                c1, c2);
+        }
+        merge(Tb, c0, c1, c2, T, C0, C1, C2);
         if (a.acc)
-          ((float4*)a.acc)[(size_t)(b0 + bq) * TILE_PIX + lp] = make_float4(C0, C1, C2, T);
-        cnt += (int)min((uint32_t)B, end - (start + (uint32_t)bq * B));
+          ((float4*)a.acc)[(size_t)(b0 + bq0 + (uint32_t)q) * TILE_PIX + lp] =
+              make_float4(C0, C1, C2, T);
+        cnt += e - j;
         if (T < T_CUT_F) {
           done = true;
           break;
         }
       }
     }
-    first += K;
   }
   if (lists_faulted(a.counts)) T = C0 = __builtin_nanf("");
   if (inside) {
@@ -427,7 +371,7 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
     a.contrib[g] = cnt;
     if (a.pixc) ((float4*)a.pixc)[g] = make_float4(C0, C1, C2, 0.0f);
   }
-  const int m = wave_max_i(cnt);
+  const int m = wave_max_i(mine ? cnt : 0);
   if (lane == 0) wmax[w] = m;
   __syncthreads();
   if (t == 0) {
@@ -443,13 +387,9 @@ __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
 
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
   if (a.T <= 0) return hipSuccess;
+  if (a.bucket > 256 || 256 % a.bucket != 0) return hipErrorInvalidValue;
   const unsigned grid = (a.items || a.slots) ? 2u * (unsigned)a.T : (unsigned)a.T;
-  if (a.bucket <= 64)
-    FGS_TIMED("composite_fast", s, (k_composite_fast<64><<<grid, 256, 0, s>>>(a)));
-  else if (a.bucket <= 128)
-    FGS_TIMED("composite_fast", s, (k_composite_fast<128><<<grid, 256, 0, s>>>(a)));
-  else
-    FGS_TIMED("composite_fast", s, (k_composite_fast<256><<<grid, 256, 0, s>>>(a)));
+  FGS_TIMED("composite_fast", s, (k_composite_fast<<<grid, 256, 0, s>>>(a)));
   return hipGetLastError();
 }
 
