@@ -1,6 +1,6 @@
 """ctypes binding of libfastergs.so (declarations in include/fastergs.h).
 
-The library must be built in-tree (``python -m fastergs_amd.build``);
+The library must be built in-tree (``python fastergs_amd/build.py``);
 importing the rasterizer without it raises immediately -- there is no CPU
 fallback.
 """
@@ -182,7 +182,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(
-                f"{LIB_PATH} is missing: build it with `python -m fastergs_amd.build` "
+                f"{LIB_PATH} is missing: build it with `python fastergs_amd/build.py` "
                 "(the rasterizer has no CPU fallback)")
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():

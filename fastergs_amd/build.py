@@ -1,7 +1,7 @@
 """Builds libfastergs.so (the C-ABI library of hand-written gfx950 kernels)
 and _C.so, the PyTorch-ROCm extension of its torch-typed operations.
 
-    python -m fastergs_amd.build
+    python fastergs_amd/build.py
 
 Every HIP translation unit is compiled for gfx950 only, with
   -ffp-contract=off                         no implicit FMA contraction (bit-identity
