@@ -706,7 +706,8 @@ def multiview_step(args, views=16, steps=5):
             t.grad = None
         from fastergs_amd.rasterizer import set_default_options
         set_default_options(schedule=SCHEDULE_FAST, bucket_size=opts.bucket_size,
-                            split_tiles=opts.split_tiles, tile_sched=opts.tile_sched)
+                            split_tiles=opts.split_tiles, tile_sched=opts.tile_sched,
+                            composite_queue=opts.composite_queue)
         for st, t in zip(settings, targets):
             img, _ = rasterize_gaussians(params["means3D"], params["dc"], params["shs"], None,
                                          params["opacities"], params["scales"],
@@ -744,7 +745,8 @@ def multiview_step(args, views=16, steps=5):
             "speedup_vs_sequential": round(times["sequential"] / times["batched"], 3),
             "autotuned": {"bucket_size": best.bucket_size, "split_tiles": best.split_tiles,
                           "work_shape": "16x16 tiles" if best.tile_sched is None
-                          else f"16x8 bands from {best.tile_sched.split_len} entries"},
+                          else f"16x8 bands from {best.tile_sched.split_len} entries",
+                          "composite_queue": bool(best.composite_queue)},
             "autotune_table_ms": table}
 
 

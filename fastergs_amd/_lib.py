@@ -48,7 +48,8 @@ class TileSched(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int), ("bucket_size", ctypes.c_int),
                 ("split_tiles", ctypes.c_int), ("abs_grad", ctypes.c_int),
-                ("tile_sched", ctypes.POINTER(TileSched)), ("forward_only", ctypes.c_int)]
+                ("tile_sched", ctypes.POINTER(TileSched)), ("forward_only", ctypes.c_int),
+                ("composite_queue", ctypes.c_int)]
 
 
 class AdamTensor(ctypes.Structure):

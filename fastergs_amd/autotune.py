@@ -7,7 +7,10 @@ the work decomposition):
   spread over row bands of whole waves instead of being lane-packed;
 * the compositing work shape: every 16x16 tile as one work item, or tiles
   with at least ``split_len`` list entries as two 16x8 row bands (the tile
-  schedule of the compositing pass; ``split_len`` 1 splits every tile).
+  schedule of the compositing pass; ``split_len`` 1 splits every tile);
+* the compositing launch: one workgroup per work item, or a persistent grid
+  taking the work items of every view from per-XCD work-queue counters
+  (``composite_queue``).
 
 The list tile itself stays 16x16 pixels (the reference's render_16x16,
 __init__.py:109): the per-tile lists, and with them the bucket boundaries
@@ -52,25 +55,28 @@ def _sched(split_len):
 def _describe(o: RasterOptions) -> dict:
     ts = o.tile_sched
     return {"bucket_size": o.bucket_size, "split_tiles": o.split_tiles,
-            "band_split_len": None if ts is None else int(ts.split_len)}
+            "band_split_len": None if ts is None else int(ts.split_len),
+            "composite_queue": bool(o.composite_queue)}
 
 
 def _from_description(c: dict) -> RasterOptions:
     return RasterOptions(schedule=_lib.SCHEDULE_FAST, bucket_size=int(c["bucket_size"]),
                          split_tiles=int(c["split_tiles"]),
-                         tile_sched=_sched(c.get("band_split_len")))
+                         tile_sched=_sched(c.get("band_split_len")),
+                         composite_queue=bool(c.get("composite_queue", False)))
 
 
 class Autotuner:
     def __init__(self, splits=DEFAULT_SPLITS, work_splits=DEFAULT_WORK_SPLITS, buckets=None,
                  base: RasterOptions | None = None, reps: int = 3,
-                 cache_path: str | None = None):
+                 cache_path: str | None = None, queues=(False, True)):
         base = base or RasterOptions()
         bucket_list = tuple(buckets) if buckets else (base.bucket_size,)
         self.candidates = [RasterOptions(schedule=_lib.SCHEDULE_FAST, bucket_size=b,
                                          split_tiles=s, tile_sched=_sched(w),
-                                         speculative=base.speculative)
-                           for b, s, w in itertools.product(bucket_list, splits, work_splits)]
+                                         composite_queue=q, speculative=base.speculative)
+                           for b, s, w, q in itertools.product(bucket_list, splits, work_splits,
+                                                               queues)]
         self.reps = reps
         self.cache_path = cache_path
         self.cache: dict = {}

@@ -650,6 +650,7 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   c.split_len = 0;
   c.split_pred = nullptr;
   c.cycles = nullptr;
+  c.queue = fast && opt->composite_queue ? gw.counts + COMPOSITE_QUEUE_OFF : nullptr;
   c.views = (const ViewRec*)gw.views;
   c.V = b.V;
   if (fast) {

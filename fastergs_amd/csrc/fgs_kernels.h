@@ -224,6 +224,9 @@ struct CompArgs {
   int split_len;
   const float* split_pred;
   uint32_t* cycles;
+  // fast: the work-queue counter of the persistent form (zero at launch, left
+  // zero), or NULL for one workgroup per grid index
+  uint32_t* queue;
   const ViewRec* views;  // fast: tiles / pixels of V views (W, H, grid_x, bg per view)
   int V;
 };

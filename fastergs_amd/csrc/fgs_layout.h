@@ -108,7 +108,13 @@ inline SortWs carve_sort(void* ws, int64_t nb) {
 constexpr int LARGE_BANDS = 8;
 constexpr int LARGE_BAND_TILES = 512;
 
-constexpr int GAUSS_SYNC_WORDS = 16 + SORT_SYNC_WORDS;
+// Counters of the persistent compositing grid's work queue: one per XCD
+// group of workgroups, each in its own 256-byte line, after the depth sort's
+// state (zeroed with it at the start of a frame, left zero by compositing).
+constexpr int COMPOSITE_QUEUE_GROUPS = 8;
+constexpr int COMPOSITE_QUEUE_STRIDE = 64;
+constexpr int COMPOSITE_QUEUE_OFF = (16 + SORT_SYNC_WORDS + 63) / 64 * 64;
+constexpr int GAUSS_SYNC_WORDS = COMPOSITE_QUEUE_OFF + COMPOSITE_QUEUE_GROUPS * COMPOSITE_QUEUE_STRIDE;
 // counts words the last-workgroup launches count arrivals in
 constexpr int COUNT_ARRIVE_RANGES = 6;
 // counts word set when a look-back wait of the render's list sort (tile sort
@@ -139,7 +145,8 @@ struct GaussWs {
   uint32_t* lrank;     // fast: depth rank of each large-rect record
   uint32_t* counts;    // [0] R, [1] NB capacity, [2] NB, [3] large rects, [4] work items,
                        // [5] list-sort fault (COUNT_LIST_FAULT), [6] arrival counter
-                       // (COUNT_ARRIVE_RANGES)
+                       // (COUNT_ARRIVE_RANGES); compositing's work-queue
+                       // counters from COMPOSITE_QUEUE_OFF
   uint32_t* dsort_sync;  // fast: depth sort state (SORT_SYNC_WORDS), right after counts
   float* feat;         // T x 8 tile features (scheduled compositing)
   float* pred;         // T x 2 cost-model outputs (predicted log2 cost, split logit)

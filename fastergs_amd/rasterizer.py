@@ -55,6 +55,9 @@ class RasterOptions:
     bucket_size: int = 128
     split_tiles: int = 64
     tile_sched: object = None
+    # FAST compositing as a persistent grid taking work items from a device
+    # counter (same outputs as one workgroup per item)
+    composite_queue: bool = False
     # render with pair / bucket capacities remembered from earlier frames of
     # the same shape instead of waiting for this frame's counts (the counts
     # are checked while the render runs; an overflow renders again exactly)
@@ -62,6 +65,7 @@ class RasterOptions:
 
     def to_c(self, abs_grad: bool = True) -> Options:
         o = Options(self.schedule, self.bucket_size, self.split_tiles, 1 if abs_grad else 0, None)
+        o.composite_queue = 1 if self.composite_queue else 0
         if self.tile_sched is not None:
             ts = self.tile_sched.to_c()
             o.tile_sched = ctypes.pointer(ts)
@@ -201,7 +205,7 @@ class Frame:
             if getattr(ts, "model", None) is not None:
                 mlp = ts.model.params
         opts = [int(o.schedule), int(o.bucket_size), int(o.split_tiles), int(o.abs_grad),
-                int(o.forward_only), *tsl]
+                int(o.forward_only), *tsl, int(o.composite_queue)]
         n = lambda t: _NONE if t is None else t
         return (self.means3D, self.opacities, self.scales, self.rotations, n(self.dc), n(self.shs),
                 n(self.colors), n(self.culling), self.view, self.proj, self.campos, self.bg,

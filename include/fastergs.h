@@ -125,6 +125,9 @@ typedef struct fgs_options {
                        checkpoints or per-pixel colours and the pair workspace holds none
                        (pass the same value to every call on the frame; fgs_backward then
                        returns FGS_ERR_ARG) */
+  int composite_queue; /* FAST compositing as a persistent grid (the workgroups resident at
+                          once) taking work items from a device counter, instead of one
+                          workgroup per item; the outputs are the same */
 } fgs_options;
 
 /* Device pointers into the two workspaces, for inspection by tests and

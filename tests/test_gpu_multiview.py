@@ -53,6 +53,32 @@ def test_forward_matches_single_views(batch_case):
         np.testing.assert_array_equal(bf.radii[v].cpu().numpy(), s.radii.cpu().numpy())
 
 
+def test_queue_compositing_matches(batch_case):
+    """The batch composited by the persistent work-queue grid (work items of
+    all views taken from one counter) gives the same images, T_final, last
+    contributors and raw sums as one workgroup per tile, twice in a row."""
+    print("\n[multiview] queue", flush=True)
+    scene, cams, d, bf, singles = batch_case
+    g = torch.cat([torch.from_numpy(scenes.target_image(c.width, c.height, seed=v)).cuda()
+                   .reshape(-1) for v, c in enumerate(cams)])
+    ref = bf.backward(g)
+    for rep in range(2):
+        opts = RasterOptions(schedule=SCHEDULE_FAST, bucket_size=32, composite_queue=True)
+        q = BatchFrame([settings_for(c) for c in cams], d["means3D"], d["opacities"],
+                       d["scales"], d["rotations"], dc=d["dc"], shs=d["shs"],
+                       options=opts).forward()
+        torch.cuda.synchronize()
+        for a, b in zip(q.images(), bf.images()):
+            np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+        np.testing.assert_array_equal(q.T_final.cpu().numpy(), bf.T_final.cpu().numpy())
+        np.testing.assert_array_equal(q.last_contributor.cpu().numpy(),
+                                      bf.last_contributor.cpu().numpy())
+        qb = q.backward(g)
+        for v in range(len(cams)):
+            raw_close(qb["raw"][v].cpu().numpy(), ref["raw"][v].cpu().numpy(),
+                      name=f"queue {rep} raw[{v}]")
+
+
 def test_lists_match_single_views(batch_case):
     scene, cams, d, bf, singles = batch_case
     bv = bf.view_arrays()

@@ -7,7 +7,8 @@
   heuristic list is in tile order.
 * Outputs do not depend on the schedule: images, T_final, last
   contributors and per-tile max contributors bit-identical across raster,
-  heuristic (every tile split / some split) and learned schedules; the
+  heuristic (every tile split / some split) and learned schedules, each run
+  by one workgroup per work item and by the persistent work-queue grid; the
   backward's raw sums agree to reassociation of their cross-wave atomics.
 * Online training on logged tile durations lowers the cost-model loss.
 """
@@ -66,11 +67,11 @@ def test_adam_kernel():
         np.testing.assert_allclose(model.params.cpu().numpy(), p, rtol=1e-5, atol=1e-7)
 
 
-def _frame(d, cam, sched, bucket=32):
+def _frame(d, cam, sched, bucket=32, queue=False):
     fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
                dc=d["dc"], shs=d["shs"],
                options=RasterOptions(schedule=SCHEDULE_FAST, bucket_size=bucket,
-                                     tile_sched=sched)).forward()
+                                     tile_sched=sched, composite_queue=queue)).forward()
     torch.cuda.synchronize()
     return fr
 
@@ -176,14 +177,20 @@ def test_outputs_independent_of_schedule(scene_cam):
     ref_b = ref.backward(g)
     rv = ref.view_arrays()
     model = CostModel(seed=2)
-    for name, sched in [("raster_logged", TileSchedule.raster(log_cycles=True, features=True)),
-                        ("split_all", TileSchedule.heuristic(split_len=1, log_cycles=True)),
-                        ("split_some", TileSchedule.heuristic(split_len=300)),
-                        ("learned", TileSchedule.learned(model, log_cycles=True)),
-                        ("learned_order", TileSchedule.learned(model, order=True,
-                                                               log_cycles=True))]:
-        print(f"[sched] {name}", flush=True)
-        fr = _frame(d, cam, sched)
+    scheds = [("raster_logged", lambda: TileSchedule.raster(log_cycles=True, features=True)),
+              ("split_all", lambda: TileSchedule.heuristic(split_len=1, log_cycles=True)),
+              ("split_some", lambda: TileSchedule.heuristic(split_len=300)),
+              ("learned", lambda: TileSchedule.learned(model, log_cycles=True)),
+              ("learned_order", lambda: TileSchedule.learned(model, order=True,
+                                                             log_cycles=True))]
+    # every schedule both as one workgroup per work item and as the persistent
+    # work-queue grid; the raster schedule's queue form twice in a row (the
+    # counter is left at zero for the next launch)
+    runs = [(n, f, False) for n, f in scheds] + [(n, f, True) for n, f in scheds]
+    runs += [("none", lambda: None, True), ("none_again", lambda: None, True)]
+    for name, mk, queue in runs:
+        print(f"[sched] {name} queue={queue}", flush=True)
+        fr = _frame(d, cam, mk(), queue=queue)
         np.testing.assert_array_equal(fr.out_color.cpu().numpy(), ref.out_color.cpu().numpy())
         np.testing.assert_array_equal(fr.T_final.cpu().numpy(), ref.T_final.cpu().numpy())
         np.testing.assert_array_equal(fr.last_contributor.cpu().numpy(),
