@@ -109,8 +109,10 @@ def main():
         print("traffic probe done", flush=True)
         return
 
-    def timed(schedule: int, steps: int, warmup: int, fwd_only=False, bucket=None):
-        set_default_options(schedule=schedule, bucket_size=args.bucket if bucket is None else bucket)
+    def timed(schedule: int, steps: int, warmup: int, fwd_only=False, bucket=None, tile_w=16,
+              queue=False):
+        set_default_options(schedule=schedule, bucket_size=args.bucket if bucket is None else bucket,
+                            tile_w=tile_w, composite_queue=queue)
         for _ in range(warmup):
             step()
         torch.cuda.synchronize()
@@ -144,7 +146,14 @@ def main():
         d_fwd, _ = timed(SCHEDULE_FAST, args.steps, 2, fwd_only=True, bucket=128)
         dropin = {"bucket_size": 128, "ms_per_step": round(d_ms, 4), "fwd_ms": round(d_fwd, 4),
                   "render_fps": round(1000.0 / d_fwd, 2)}
-    set_default_options(schedule=sched, bucket_size=args.bucket)
+    # the tile shape (semantic like the bucket size) and the compositing
+    # launch form (work-queue grid) at the bench's bucket size
+    variants = None
+    if sched == SCHEDULE_FAST:
+        variants = {f"tile_{tw}x{256 // tw}": round(timed(sched, args.steps, 3, tile_w=tw)[0], 4)
+                    for tw in (8, 16, 32)}
+        variants["composite_queue"] = round(timed(sched, args.steps, 3, queue=True)[0], 4)
+    set_default_options(schedule=sched, bucket_size=args.bucket, tile_w=16, composite_queue=False)
 
     graph_fps = graph_render_fps(settings, params, sched, args)
     step_graph_ms = None
@@ -195,6 +204,7 @@ def main():
         "plain_ms_per_step": None if plain_ms is None else round(plain_ms, 4),
         "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms, 3),
         "dropin_default": dropin,
+        "variants_ms_per_step": variants,
         "small_config": small,
         "roofline": roofline,
         "fp32_fma_attainable_tflops": fma_tflops,
@@ -707,7 +717,7 @@ def multiview_step(args, views=16, steps=5):
         from fastergs_amd.rasterizer import set_default_options
         set_default_options(schedule=SCHEDULE_FAST, bucket_size=opts.bucket_size,
                             split_tiles=opts.split_tiles, tile_sched=opts.tile_sched,
-                            composite_queue=opts.composite_queue)
+                            composite_queue=opts.composite_queue, tile_w=opts.tile_w)
         for st, t in zip(settings, targets):
             img, _ = rasterize_gaussians(params["means3D"], params["dc"], params["shs"], None,
                                          params["opacities"], params["scales"],
@@ -717,7 +727,8 @@ def multiview_step(args, views=16, steps=5):
 
     # bucket size (semantic, opted in), split threshold and compositing work
     # shape (whole 16x16 tiles or 16x8 bands)
-    tuner = Autotuner(buckets=(16, 32, 64), splits=(32, 64, 128), work_splits=(None, 1), reps=2)
+    tuner = Autotuner(buckets=(16, 32, 64), splits=(32, 64, 128), work_splits=(None, 1), reps=2,
+                      tile_ws=(8, 16, 32))
     best, table = tuner.tune(batched)
     times = {}
     for name, fn in (("batched", batched), ("sequential", sequential)):
@@ -743,7 +754,8 @@ def multiview_step(args, views=16, steps=5):
             "ms_per_step_batched_graph": round(times["captured"], 3),
             "views_per_s": round(views * 1000.0 / times["batched"], 1),
             "speedup_vs_sequential": round(times["sequential"] / times["batched"], 3),
-            "autotuned": {"bucket_size": best.bucket_size, "split_tiles": best.split_tiles,
+            "autotuned": {"bucket_size": best.bucket_size, "tile_w": best.tile_w,
+                          "split_tiles": best.split_tiles,
                           "work_shape": "16x16 tiles" if best.tile_sched is None
                           else f"16x8 bands from {best.tile_sched.split_len} entries",
                           "composite_queue": bool(best.composite_queue)},

@@ -45,11 +45,18 @@ class TileSched(ctypes.Structure):
                 ("items", vp), ("n_items", vp), ("prev_features", vp), ("features_out", vp)]
 
 
+def tiles_of(width: int, height: int, tile_w: int = 16) -> int:
+    """Tiles of a width x height image on tile_w x (256 / tile_w) tiles."""
+    tw = tile_w or 16
+    th = 256 // tw
+    return ((width + tw - 1) // tw) * ((height + th - 1) // th)
+
+
 class Options(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int), ("bucket_size", ctypes.c_int),
                 ("split_tiles", ctypes.c_int), ("abs_grad", ctypes.c_int),
                 ("tile_sched", ctypes.POINTER(TileSched)), ("forward_only", ctypes.c_int),
-                ("composite_queue", ctypes.c_int)]
+                ("composite_queue", ctypes.c_int), ("tile_w", ctypes.c_int)]
 
 
 class AdamTensor(ctypes.Structure):

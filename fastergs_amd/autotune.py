@@ -12,15 +12,18 @@ the work decomposition):
   taking the work items of every view from per-XCD work-queue counters
   (``composite_queue``).
 
-The list tile itself stays 16x16 pixels (the reference's render_16x16,
-__init__.py:109): the per-tile lists, and with them the bucket boundaries
-at which the T < 1e-4 cut-off is tested, are defined on it.
+Semantic knobs, tuned only on request:
 
-The bucket size is semantic: the cut-off is tested at bucket ends, so last
-contributors, images (by < 1e-4 of a colour) and gradients depend on it.
-By default the tuner keeps the bucket size of ``base`` (the reference's 128
-unless the caller chose another); passing ``buckets`` makes it a tuned knob
-too, for callers that accept that the result then depends on the choice.
+* the bucket size: the cut-off is tested at bucket ends, so last
+  contributors, images (by < 1e-4 of a colour) and gradients depend on it;
+* the tile shape (``tile_w``: 8 x 32, 16 x 16 or 32 x 8 pixels; the
+  reference hard-codes 16 x 16, render_16x16, __init__.py:109): the per-tile
+  lists, and with them the bucket boundaries, are defined on the tile.
+
+By default the tuner keeps the bucket size and tile shape of ``base`` (the
+reference's 128 and 16 x 16 unless the caller chose others); passing
+``buckets`` or ``tile_ws`` makes them tuned knobs too, for callers that
+accept that the result then depends on the choice.
 
 ``Autotuner.tune(step, key)`` times ``step(options)`` -- one full training
 step with those options, e.g. forward + backward of a multi-view batch --
@@ -54,7 +57,7 @@ def _sched(split_len):
 
 def _describe(o: RasterOptions) -> dict:
     ts = o.tile_sched
-    return {"bucket_size": o.bucket_size, "split_tiles": o.split_tiles,
+    return {"bucket_size": o.bucket_size, "tile_w": int(o.tile_w), "split_tiles": o.split_tiles,
             "band_split_len": None if ts is None else int(ts.split_len),
             "composite_queue": bool(o.composite_queue)}
 
@@ -63,20 +66,22 @@ def _from_description(c: dict) -> RasterOptions:
     return RasterOptions(schedule=_lib.SCHEDULE_FAST, bucket_size=int(c["bucket_size"]),
                          split_tiles=int(c["split_tiles"]),
                          tile_sched=_sched(c.get("band_split_len")),
-                         composite_queue=bool(c.get("composite_queue", False)))
+                         composite_queue=bool(c.get("composite_queue", False)),
+                         tile_w=int(c.get("tile_w", 16)))
 
 
 class Autotuner:
     def __init__(self, splits=DEFAULT_SPLITS, work_splits=DEFAULT_WORK_SPLITS, buckets=None,
                  base: RasterOptions | None = None, reps: int = 3,
-                 cache_path: str | None = None, queues=(False, True)):
+                 cache_path: str | None = None, queues=(False, True), tile_ws=None):
         base = base or RasterOptions()
         bucket_list = tuple(buckets) if buckets else (base.bucket_size,)
-        self.candidates = [RasterOptions(schedule=_lib.SCHEDULE_FAST, bucket_size=b,
+        tile_list = tuple(tile_ws) if tile_ws else (base.tile_w,)
+        self.candidates = [RasterOptions(schedule=_lib.SCHEDULE_FAST, bucket_size=b, tile_w=tw,
                                          split_tiles=s, tile_sched=_sched(w),
                                          composite_queue=q, speculative=base.speculative)
-                           for b, s, w, q in itertools.product(bucket_list, splits, work_splits,
-                                                               queues)]
+                           for b, tw, s, w, q in itertools.product(bucket_list, tile_list, splits,
+                                                                   work_splits, queues)]
         self.reps = reps
         self.cache_path = cache_path
         self.cache: dict = {}

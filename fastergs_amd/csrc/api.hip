@@ -100,9 +100,11 @@ struct Fork {
   }
 };
 
-int grid_x_of(int W) { return (W + TILE_W - 1) / TILE_W; }
-int grid_y_of(int H) { return (H + TILE_H - 1) / TILE_H; }
-int tiles_of(int W, int H) { return grid_x_of(W) * grid_y_of(H); }
+// Tile width of the options (0: the reference's 16; tiles hold 256 pixels).
+int tile_w_of(const fgs_options* o) { return o && o->tile_w ? o->tile_w : TILE_W_REF; }
+int grid_x_of(int W, int tw) { return grid_dim(W, tw); }
+int grid_y_of(int H, int tw) { return grid_dim(H, tile_h_of(tw)); }
+int tiles_of(int W, int H, int tw) { return grid_x_of(W, tw) * grid_y_of(H, tw); }
 
 // Number of bits needed for tile ids (sort.cu:9-24 computes the same).
 int tile_bits(int T) {
@@ -131,6 +133,8 @@ int check_opts(const fgs_options* o) {
     return fail(FGS_ERR_ARG, "bucket_size must be 16, 32, 64, 128 or 256");
   if (o->split_tiles < 1 || o->split_tiles > (1 << 24))
     return fail(FGS_ERR_ARG, "split_tiles must be in [1, 2^24]");
+  if (o->tile_w != 0 && !tile_w_ok(o->tile_w))
+    return fail(FGS_ERR_ARG, "tile_w must be 8, 16 or 32 (0: 16)");
   if (const fgs_tile_sched* ts = o->tile_sched) {
     if (ts->mode < FGS_TILES_RASTER || ts->mode > FGS_TILES_LEARNED)
       return fail(FGS_ERR_ARG, "unknown tile schedule mode");
@@ -144,10 +148,10 @@ int check_opts(const fgs_options* o) {
   return FGS_OK;
 }
 
-int check_cam(const fgs_camera* c) {
+int check_cam(const fgs_camera* c, int tw) {
   if (!c) return fail(FGS_ERR_ARG, "camera is NULL");
   if (c->width <= 0 || c->height <= 0) return fail(FGS_ERR_ARG, "image size must be positive");
-  if (grid_x_of(c->width) > 65535 || grid_y_of(c->height) > 65535)
+  if (grid_x_of(c->width, tw) > 65535 || grid_y_of(c->height, tw) > 65535)
     return fail(FGS_ERR_ARG, "image too large");
   if (!c->viewmatrix || !c->projmatrix || !c->campos || !c->bg)
     return fail(FGS_ERR_ARG, "camera matrices must be device pointers");
@@ -170,7 +174,7 @@ int check_gauss(const fgs_gaussians* g, const fgs_camera* c) {
   return FGS_OK;
 }
 
-CamRef<float> make_cam(const fgs_camera* c, const fgs_gaussians* g) {
+CamRef<float> make_cam(const fgs_camera* c, const fgs_gaussians* g, int tw) {
   CamRef<float> r;
   r.view = c->viewmatrix;
   r.proj = c->projmatrix;
@@ -183,8 +187,10 @@ CamRef<float> make_cam(const fgs_camera* c, const fgs_gaussians* g) {
   r.mult = c->mult;
   r.W = c->width;
   r.H = c->height;
-  r.grid_x = grid_x_of(c->width);
-  r.grid_y = grid_y_of(c->height);
+  r.grid_x = grid_x_of(c->width, tw);
+  r.grid_y = grid_y_of(c->height, tw);
+  r.tile_w = tw;
+  r.tile_h = tile_h_of(tw);
   r.deg = c->sh_degree;
   r.max_coeffs = (g->colors_precomp || !g->shs) ? 0 : g->max_coeffs;
   return r;
@@ -195,27 +201,31 @@ struct Batch {
   int V = 0;
   const fgs_camera* cams = nullptr;
   int T = 0;        // tiles of all views
+  int tw = TILE_W_REF;  // tile width (one shape for every view)
   int64_t HW = 0;   // pixels of all views
   ViewSetup setup;  // kernel-argument description of the view table
 };
 
-// make_batch without options only sizes the batch (workspace queries, views).
-int make_batch(const fgs_batch* batch, Batch& b, const fgs_options* opt = nullptr) {
+// With check false the batch is only sized (workspace queries, views); the
+// options give the tile shape either way (NULL: 16 x 16).
+int make_batch(const fgs_batch* batch, Batch& b, const fgs_options* opt, bool check) {
   static_assert(sizeof(ViewRec) <= VIEW_REC_BYTES, "view record too large");
   static_assert(MAX_VIEWS <= VIEW_TABLE_VIEWS, "view table too small");
   if (!batch || !batch->cameras) return fail(FGS_ERR_ARG, "batch is NULL");
   if (batch->n_views < 1 || batch->n_views > MAX_VIEWS)
     return fail(FGS_ERR_ARG, "n_views must be in [1, 32]");
-  if (opt && opt->schedule == FGS_SCHEDULE_PLAIN && batch->n_views != 1)
+  if (check && opt->schedule == FGS_SCHEDULE_PLAIN && batch->n_views != 1)
     return fail(FGS_ERR_ARG, "the PLAIN schedule renders one view per call");
   b.V = batch->n_views;
+  b.tw = tile_w_of(opt);
+  if (!tile_w_ok(b.tw)) return fail(FGS_ERR_ARG, "tile_w must be 8, 16 or 32 (0: 16)");
   b.cams = batch->cameras;
   int64_t T = 0, HW = 0;
   b.setup.n = b.V;
   for (int v = 0; v < b.V; ++v) {
     const fgs_camera* c = &b.cams[v];
-    if (opt) {
-      int rc = check_cam(c);
+    if (check) {
+      int rc = check_cam(c, b.tw);
       if (rc) return rc;
       if (v > 0 && (c->sh_degree != b.cams[0].sh_degree ||
                     c->scale_modifier != b.cams[0].scale_modifier || c->mult != b.cams[0].mult))
@@ -236,7 +246,9 @@ int make_batch(const fgs_batch* batch, Batch& b, const fgs_options* opt = nullpt
     src.H = c->height;
     src.tile_off = (uint32_t)T;
     src.pix_off = (uint32_t)HW;
-    T += tiles_of(c->width, c->height);
+    src.tile_w = b.tw;
+    src.tile_h = tile_h_of(b.tw);
+    T += tiles_of(c->width, c->height, b.tw);
     HW += (int64_t)c->width * c->height;
   }
   if (T >= (1 << 24) || HW >= (int64_t)1 << 32) return fail(FGS_ERR_ARG, "batch too large");
@@ -339,15 +351,14 @@ const char* fgs_build_info(void) {
 }
 
 size_t fgs_gauss_ws_size(int P, int width, int height, const fgs_options* opt) {
-  (void)opt;
-  return carve_gauss(nullptr, P, tiles_of(width, height)).total;
+  return carve_gauss(nullptr, P, tiles_of(width, height, tile_w_of(opt))).total;
 }
 
 size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
                         const fgs_options* opt) {
   (void)P;
   const bool fast = opt && opt->schedule == FGS_SCHEDULE_FAST;
-  return carve_pair(nullptr, R, NB, (int64_t)width * height, tiles_of(width, height), fast,
+  return carve_pair(nullptr, R, NB, (int64_t)width * height, tiles_of(width, height, tile_w_of(opt)), fast,
                     keeps_ckpt(opt)).total;
 }
 
@@ -359,18 +370,16 @@ size_t fgs_grad_ws_size(int P) {
 }
 
 size_t fgs_batch_gauss_ws_size(int P, const fgs_batch* batch, const fgs_options* opt) {
-  (void)opt;
   Batch b;
-  if (make_batch(batch, b) != FGS_OK) return 0;
+  if (make_batch(batch, b, opt, false) != FGS_OK) return 0;
   return carve_gauss(nullptr, (int64_t)P * b.V, b.T).total;
 }
 
 size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t NB,
                               const fgs_options* opt) {
   (void)P;
-  (void)opt;
   Batch b;
-  if (make_batch(batch, b) != FGS_OK) return 0;
+  if (make_batch(batch, b, opt, false) != FGS_OK) return 0;
   return carve_pair(nullptr, R, NB, b.HW, b.T, true, keeps_ckpt(opt)).total;
 }
 
@@ -491,7 +500,7 @@ PreArgs pre_args(const Batch& b, const fgs_gaussians* g, const fgs_options* opt,
                  int32_t* radii) {
   PreArgs a;
   a.G = make_gauss(g);
-  a.cam = make_cam(&b.cams[0], g);
+  a.cam = make_cam(&b.cams[0], g, b.tw);
   a.geom = w.geom;
   a.splat = w.splat;
   a.dkey = w.dkey;
@@ -512,7 +521,7 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
                          void* gauss_ws, size_t gauss_ws_bytes, int32_t* radii, void* stream) {
   Batch b;
   int rc;
-  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt)) ||
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt, true)) ||
       (rc = check_gauss(g, &b.cams[0])))
     return rc;
   const int64_t PV = (int64_t)g->P * b.V;
@@ -589,7 +598,7 @@ int fgs_batch_read_counts(const void* gauss_ws, int P, const fgs_batch* batch,
                           const fgs_options* opt, int64_t counts_host[2], void* stream) {
   Batch b;
   int rc;
-  if ((rc = make_batch(batch, b))) return rc;
+  if ((rc = make_batch(batch, b, opt, false))) return rc;
   if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
   const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), (int64_t)P * b.V, b.T);
   // [0] pairs, [1] bucket-table capacity, up to the depth sort's fault flag
@@ -612,7 +621,7 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
                      int32_t* last_contributor, void* stream) {
   Batch b;
   int rc;
-  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt)) ||
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt, true)) ||
       (rc = check_gauss(g, &b.cams[0])))
     return rc;
   if (!out_color || !T_final || !last_contributor) return fail(FGS_ERR_ARG, "NULL output");
@@ -626,12 +635,13 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   hipStream_t s = (hipStream_t)stream;
   const fgs_camera* cam = &b.cams[0];
   const int W = cam->width, H = cam->height;
-  const int gx = grid_x_of(W);
+  const int gx = grid_x_of(W, b.tw);
   CompArgs c;
   c.counts = gw.counts;
   c.W = W;
   c.H = H;
   c.grid_x = gx;
+  c.tile_w = b.tw;
   c.T = T;
   c.bucket = opt->bucket_size;
   c.ranges = gw.ranges;
@@ -737,7 +747,7 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   }
 
   // PLAIN: duplicate with keys, 64-bit radix sort, ranges, composite.
-  FGS_TRY(launch_duplicate_plain(g->P, gw.geom, gw.dkey, gw.pcount, gw.poff, gx, (uint32_t)R,
+  FGS_TRY(launch_duplicate_plain(g->P, gw.geom, gw.dkey, gw.pcount, gw.poff, gx, b.tw, (uint32_t)R,
                                  pw.k0, pw.v0, s),
           "duplicate");
   bool alt = false;
@@ -760,7 +770,7 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
                        void* stream) {
   Batch b;
   int rc;
-  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt)) ||
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt, true)) ||
       (rc = check_gauss(g, &b.cams[0])))
     return rc;
   if (!out || !grad_ws || !dL_dimage || !T_final || !last_contributor)
@@ -787,7 +797,8 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   BwdArgs bw;
   bw.W = cam->width;
   bw.H = cam->height;
-  bw.grid_x = grid_x_of(cam->width);
+  bw.grid_x = grid_x_of(cam->width, b.tw);
+  bw.tile_w = b.tw;
   bw.T = T;
   bw.bucket = opt->bucket_size;
   bw.abs_grad = opt->abs_grad;
@@ -822,7 +833,7 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   }
   GaussBwdArgs q;
   q.G = make_gauss(g);
-  q.cam = make_cam(cam, g);
+  q.cam = make_cam(cam, g, b.tw);
   if (fast) {
     // the camera the forward used: the view table in the workspace
     const ViewRec* v0 = (const ViewRec*)gw.views;
@@ -879,7 +890,7 @@ int fgs_batch_frame_view_get(void* gauss_ws, void* pair_ws, int P, const fgs_bat
                              int64_t R, int64_t NB, const fgs_options* opt, fgs_frame_view* v) {
   Batch b;
   int rc;
-  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b))) return rc;
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt, false))) return rc;
   if (!v) return fail(FGS_ERR_ARG, "view is NULL");
   const int T = b.T;
   const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T);
@@ -928,7 +939,7 @@ int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_opti
 int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fgs_options* opt,
                     int64_t counts_host[2], void* stream) {
   if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
-  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height));
+  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height, tile_w_of(opt)));
   // [0] pairs, [1] bucket-table capacity, up to the depth sort's fault flag
   uint32_t c[DSORT_FAULT + 1];
   hipStream_t s = (hipStream_t)stream;
@@ -971,7 +982,7 @@ int fgs_batch_bucket_tables(void* gauss_ws, void* pair_ws, int P, const fgs_batc
                             int64_t R, int64_t NB, const fgs_options* opt, void* stream) {
   Batch b;
   int rc;
-  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b))) return rc;
+  if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt, false))) return rc;
   if (opt->schedule != FGS_SCHEDULE_FAST) return fail(FGS_ERR_ARG, "bucket tables need the FAST schedule");
   if (!gauss_ws || !pair_ws) return fail(FGS_ERR_ARG, "NULL workspace");
   if (P < 0 || R < 0 || NB < 0) return fail(FGS_ERR_ARG, "bad frame size");
@@ -1015,7 +1026,7 @@ int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height,
   if (P == 0) return FGS_OK;
   if (!gauss_ws || !pair_ws || !accum_weights || !accum_count || !accum_max_count)
     return fail(FGS_ERR_ARG, "NULL argument");
-  const int T = tiles_of(width, height);
+  const int T = tiles_of(width, height, tile_w_of(opt));
   const GaussWs gw = carve_gauss(gauss_ws, P, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast, keeps_ckpt(opt));
@@ -1024,7 +1035,7 @@ int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height,
   FGS_TRY(fill_f32(accum_weights, 0.0f, P, s), "zero weights");
   FGS_TRY(fill_u32((uint32_t*)accum_count, 0u, P, s), "zero counts");
   FGS_TRY(fill_f32(accum_max_count, 0.0f, P, s), "zero max counts");
-  FGS_TRY(launch_render_simp(width, height, T, gw.ranges, vals, gw.splat, accum_weights,
+  FGS_TRY(launch_render_simp(width, height, tile_w_of(opt), T, gw.ranges, vals, gw.splat, accum_weights,
                              accum_count, accum_max_count, s),
           "render_simp");
   return FGS_OK;
@@ -1043,12 +1054,12 @@ int fgs_render_depth(void* gauss_ws, void* pair_ws, int P, int width, int height
     return fail(FGS_ERR_ARG, "NULL argument");
   if (P > 0 && (!means3D || !scales || !rotations))
     return fail(FGS_ERR_ARG, "render_depth needs means3D, scales and rotations");
-  const int T = tiles_of(width, height);
+  const int T = tiles_of(width, height, tile_w_of(opt));
   const GaussWs gw = carve_gauss(gauss_ws, P, T);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast, keeps_ckpt(opt));
   const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);
-  FGS_TRY(launch_render_depth(width, height, T, gw.ranges, vals, gw.splat, means3D, scales,
+  FGS_TRY(launch_render_depth(width, height, tile_w_of(opt), T, gw.ranges, vals, gw.splat, means3D, scales,
                               rotations, projmatrix, campos, out_pts, out_depth, accum_alpha,
                               gidx, discriminants, (hipStream_t)stream),
           "render_depth");

@@ -152,7 +152,8 @@ __global__ __launch_bounds__(256) void k_backward_plain(BwdArgs a) {
   const int tile = blockIdx.x;
   const int tx = tile % a.grid_x, ty = tile / a.grid_x;
   const int t = threadIdx.x;
-  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
+  const int tw = a.tile_w;
+  const int px = tx * tw + (t & (tw - 1)), py = ty * tile_h_of(tw) + (t >> tile_w_log2(tw));
   const bool inside = px < a.W && py < a.H;
   const uint32_t start = a.ranges[2 * tile];
   const int mc = (int)a.maxc[tile];
@@ -210,7 +211,8 @@ hipError_t launch_backward_plain(const BwdArgs& a, hipStream_t s) {
 // One wave64 per needed bucket (backward.cu:695-878): the buckets of each
 // tile below its furthest contributor form a compact task list (prefix over
 // tiles, `actoff`), taken by persistent waves in a grid-stride loop. A wave
-// covers all 256 pixels of the tile (4 per lane), restarts each pixel from
+// covers all 256 pixels of the tile (a 2x2 quad per lane, the quads in
+// row-major order over the tile's tw/2 x th/2 grid), restarts each pixel from
 // the checkpoint stored after this bucket (T, C) and walks the bucket's
 // splats in reverse; the splat records are wave-uniform scalar loads, the
 // next one prefetched while the current one is processed. Per splat the 4
@@ -223,7 +225,6 @@ __global__ __launch_bounds__(256, 6) void k_backward_fast(BwdArgs a) {
   const int lane = lane_id();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / WAVE);
   const uint32_t nwaves = gridDim.x * 4;
-  const int qx = 2 * (lane & 7), qy = 2 * (lane >> 3);
   for (uint32_t u = wave; u < A; u += nwaves) {
     int lo = 0, hi = a.T;  // actoff[lo] <= u < actoff[hi]
     while (hi - lo > 1) {
@@ -238,7 +239,8 @@ __global__ __launch_bounds__(256, 6) void k_backward_fast(BwdArgs a) {
     const size_t bq = (size_t)(a.boff[tile] + lb) * TILE_PIX;
     // the tile's view: size, tile grid, background and image offsets
     const ViewRec& vr = a.views[a.V > 1 ? view_of_tile(a.views, a.V, (uint32_t)tile) : 0];
-    const int W = vr.W, H = vr.H, gx = vr.grid_x;
+    const int W = vr.W, H = vr.H, gx = vr.grid_x, tw = vr.tile_w, th = vr.tile_h;
+    const int qx = 2 * (lane & (tw / 2 - 1)), qy = 2 * (lane >> (tile_w_log2(tw) - 1));
     const size_t HW = (size_t)W * H;
     const size_t poff = vr.pix_off;
     const float* dimg = a.dimg + 3 * poff;
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256, 6) void k_backward_fast(BwdArgs a) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int lx = qx + (k & 1), ly = qy + (k >> 1);
-      const int px = tx * TILE_W + lx, py = ty * TILE_H + ly;
+      const int px = tx * tw + lx, py = ty * th + ly;
       fy[k] = (float)py;
       const bool inside = px < W && py < H;
       const size_t gl = inside ? (size_t)py * W + px : 0;  // pixel in the view
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(256, 6) void k_backward_fast(BwdArgs a) {
       fx[k] = act ? (float)px : 1.0e7f;
       T[k] = 1.0f; Q[k] = 0.0f; d0[k] = d1[k] = d2[k] = 0.0f;
       if (act) {
-        const float4 ck = ((const float4*)a.acc)[bq + ly * TILE_W + lx];
+        const float4 ck = ((const float4*)a.acc)[bq + ly * tw + lx];
         const float4 pc = ((const float4*)a.pixc)[g];
         T[k] = ck.w;
         d0[k] = dimg[gl];

@@ -23,7 +23,8 @@ struct ViewRec {
   float bg[4];
   float tanfovx, tanfovy, focal_x, focal_y;
   int W, H, grid_x, grid_y;
-  uint32_t tile_off, pix_off, pad0, pad1;
+  uint32_t tile_off, pix_off;
+  int tile_w, tile_h;  // tile shape (tile_w * tile_h = TILE_PIX)
 };
 struct ViewSrc {     // host-side description; matrices are device pointers
   const float* view;
@@ -33,6 +34,7 @@ struct ViewSrc {     // host-side description; matrices are device pointers
   float tanfovx, tanfovy, focal_x, focal_y;
   int W, H;
   uint32_t tile_off, pix_off;
+  int tile_w, tile_h;
 };
 struct ViewSetup {
   ViewSrc v[MAX_VIEWS];
@@ -58,6 +60,8 @@ __device__ __forceinline__ CamRef<float> view_cam(const CamRef<float>& base, con
   c.H = r.H;
   c.grid_x = r.grid_x;
   c.grid_y = r.grid_y;
+  c.tile_w = r.tile_w;
+  c.tile_h = r.tile_h;
   return c;
 }
 // Camera of view source s (caller's buffers) for the shared per-Gaussian math.
@@ -72,8 +76,10 @@ __device__ __forceinline__ CamRef<float> src_cam(const CamRef<float>& base, cons
   c.focal_y = s.focal_y;
   c.W = s.W;
   c.H = s.H;
-  c.grid_x = (s.W + TILE_W - 1) / TILE_W;
-  c.grid_y = (s.H + TILE_H - 1) / TILE_H;
+  c.grid_x = grid_dim(s.W, s.tile_w);
+  c.grid_y = grid_dim(s.H, s.tile_h);
+  c.tile_w = s.tile_w;
+  c.tile_h = s.tile_h;
   return c;
 }
 // View owning global tile t (binary search over the views' first tiles).
@@ -165,7 +171,8 @@ hipError_t launch_sh_color(const PreArgs& a, hipStream_t s);
 // whose pair count exceeds it is detected on the host from the count.
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
-                                  uint32_t cap, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                                  int tw, uint32_t cap, uint64_t* keys, uint32_t* vals,
+                                  hipStream_t s);
 // FAST emission over V views of P Gaussians (records v * P + i) as (tile
 // id, record) into two arrays: tile ids are global, 16-bit when t16 (at
 // most 65536 tiles), else 32-bit.
@@ -198,6 +205,7 @@ hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_
                                 hipStream_t s);
 struct CompArgs {
   int W, H, grid_x, T, bucket;
+  int tile_w;  // tile shape of the PLAIN kernel (FAST reads it from the view records)
   // the frame's counts words: when a sort's look-back bound was exceeded
   // (counts[COUNT_LIST_FAULT] or counts[DSORT_FAULT] set) the lists are not
   // valid, and images and T_final are written NaN
@@ -233,10 +241,12 @@ struct CompArgs {
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);
 
-hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
+hipError_t launch_render_simp(int W, int H, int tw, int T, const uint32_t* ranges,
+                              const uint32_t* vals,
                               const float* splat, float* wsum, int32_t* wcount, float* maxcount,
                               hipStream_t s);
-hipError_t launch_render_depth(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
+hipError_t launch_render_depth(int W, int H, int tw, int T, const uint32_t* ranges,
+                               const uint32_t* vals,
                                const float* splat, const float* means3D, const float* scales,
                                const float* rotations, const float* proj, const float* campos,
                                float* out_pts, float* out_depth, float* accum_alpha,
@@ -296,6 +306,7 @@ hipError_t launch_ssim_bwd(int B, int CH, int H, int W, const float* img1, const
 // backward.hip
 struct BwdArgs {
   int W, H, grid_x, T, bucket, abs_grad;
+  int tile_w;  // tile shape of the PLAIN kernel (FAST reads it from the view records)
   int64_t NB;
   const uint32_t* ranges;
   const uint32_t* vals;

@@ -36,9 +36,16 @@
 namespace fgs {
 
 // ---------------------------------------------------------------- constants
-constexpr int TILE_W = 16;
-constexpr int TILE_H = 16;
-constexpr int TILE_PIX = TILE_W * TILE_H;
+// Tiles hold TILE_PIX = 256 pixels in one of three shapes, tile_w x
+// (256 / tile_w) with tile_w 8, 16 or 32; 16 x 16 is the reference's
+// (BLOCK_X x BLOCK_Y, render_16x16, __init__.py:109). The shape is a
+// property of the camera (CamRef) and of each view record.
+constexpr int TILE_PIX = 256;
+constexpr int TILE_W_REF = 16;
+FGS_HD bool tile_w_ok(int tw) { return tw == 8 || tw == 16 || tw == 32; }
+FGS_HD int tile_h_of(int tw) { return TILE_PIX / tw; }
+FGS_HD int tile_w_log2(int tw) { return tw == 8 ? 3 : (tw == 32 ? 5 : 4); }
+FGS_HD int grid_dim(int n, int tile) { return (n + tile - 1) / tile; }
 constexpr float LOG2E_F = 1.4426950216293334961f;   // rasterizer_imp.h:43
 constexpr float LN2_F = 0.69314718055f;             // rasterizer_imp.h:44
 constexpr float ALPHA_CAP_F = 0.99f;                // render.cu:109
@@ -162,6 +169,7 @@ struct CamRef {
   const R* campos;  // 3
   R tanfovx, tanfovy, focal_x, focal_y, scale_mod, mult;
   int W, H, grid_x, grid_y, deg, max_coeffs;
+  int tile_w, tile_h;  // tile shape in pixels (tile_w * tile_h = TILE_PIX)
 };
 
 template <typename R>
@@ -339,10 +347,10 @@ FGS_HD void sh_color(int i, const GaussRef<R>& G, const CamRef<R>& cam, R rgb[3]
 }
 
 // Tile rect of a splat centred at (x,y) with integer half-extent (w,h)
-// (rasterizer_imp.h:532-541).
+// (rasterizer_imp.h:532-541) on tw x th tiles.
 template <typename R>
-FGS_HD void tile_rect(R x, R y, int w, int h, int gx, int gy, int rect[4]) {
-  const R bx = R(TILE_W), by = R(TILE_H);
+FGS_HD void tile_rect(R x, R y, int w, int h, int gx, int gy, int tw, int th, int rect[4]) {
+  const R bx = R(tw), by = R(th);
   int v;
   v = trunc_i((x - (R)w) / bx); rect[0] = v < 0 ? 0 : (v > gx ? gx : v);
   v = trunc_i((y - (R)h) / by); rect[1] = v < 0 ? 0 : (v > gy ? gy : v);
@@ -399,7 +407,7 @@ FGS_HD void preprocess_gauss(int i, const GaussRef<R>& G, const CamRef<R>& cam, 
   const int hw = trunc_i(R(SQRT2_F) * sqrt_(a * power) + R(1));
   const int hh = trunc_i(R(SQRT2_F) * sqrt_(c * power) + R(1));
   const R px = ndc2pix(ndx, cam.W), py = ndc2pix(ndy, cam.H);
-  tile_rect(px, py, hw, hh, cam.grid_x, cam.grid_y, o.rect);
+  tile_rect(px, py, hw, hh, cam.grid_x, cam.grid_y, cam.tile_w, cam.tile_h, o.rect);
   const int tc = (o.rect[2] - o.rect[0]) * (o.rect[3] - o.rect[1]);
   if (tc <= 0) return;
 
@@ -433,13 +441,13 @@ FGS_HD bool seg_hits(R a, R b, R c, R d, R l, R r) {
   return (delta >= R(0)) & ((t1 <= R(0)) | (t1 * t1 <= delta)) & ((t2 >= R(0)) | (t2 * t2 <= delta));
 }
 
-// Exact tile test of preprocess.cu:179-218 for tile (tx,ty): true when the
-// pixel-centre box of the tile contains the centre or the ellipse meets the
-// nearer vertical edge or the nearer horizontal edge.
+// Exact tile test of preprocess.cu:179-218 for tile (tx,ty) of tw x th
+// tiles: true when the pixel-centre box of the tile contains the centre or
+// the ellipse meets the nearer vertical edge or the nearer horizontal edge.
 template <typename R>
-FGS_HD bool tile_hit(int tx, int ty, R x, R y, const R conic[3], R power) {
-  const int x0 = tx * TILE_W, y0 = ty * TILE_H;
-  const int x1 = x0 + TILE_W - 1, y1 = y0 + TILE_H - 1;
+FGS_HD bool tile_hit(int tx, int ty, int tw, int th, R x, R y, const R conic[3], R power) {
+  const int x0 = tx * tw, y0 = ty * th;
+  const int x1 = x0 + tw - 1, y1 = y0 + th - 1;
   const bool inside = (x >= (R)x0) & (x <= (R)x1) & (y >= (R)y0) & (y <= (R)y1);
   const R w = R(2) * power;
   const R dx = (x * R(2) < (R)(x0 + x1)) ? x - (R)x0 : x - (R)x1;
@@ -465,9 +473,9 @@ struct TileRow {
   bool yin;        // y0 <= y <= y1
   float b2, d2;    // nearer horizontal edge: b and discriminant of seg_hits
 };
-FGS_HD TileCol tile_col(int tx, float x, const float conic[3], float power) {
+FGS_HD TileCol tile_col(int tx, int tw, float x, const float conic[3], float power) {
   TileCol c;
-  const int x0 = tx * TILE_W, x1 = x0 + TILE_W - 1;
+  const int x0 = tx * tw, x1 = x0 + tw - 1;
   c.x0f = (float)x0;
   c.x1f = (float)x1;
   c.xin = (x >= c.x0f) & (x <= c.x1f);
@@ -480,9 +488,9 @@ FGS_HD TileCol tile_col(int tx, float x, const float conic[3], float power) {
   c.r2 = c.x1f - x;
   return c;
 }
-FGS_HD TileRow tile_row(int ty, float y, const float conic[3], float power) {
+FGS_HD TileRow tile_row(int ty, int th, float y, const float conic[3], float power) {
   TileRow r;
-  const int y0 = ty * TILE_H, y1 = y0 + TILE_H - 1;
+  const int y0 = ty * th, y1 = y0 + th - 1;
   const float y0f = (float)y0, y1f = (float)y1;
   r.yin = (y >= y0f) & (y <= y1f);
   const float w = 2.0f * power;

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_preprocess_plain(PreArgs a) {
   uint32_t n = 0;
   for (int ty = o.rect[1]; ty < o.rect[3]; ++ty)
     for (int tx = o.rect[0]; tx < o.rect[2]; ++tx)
-      n += tile_hit(tx, ty, o.x, o.y, o.conic, o.power) ? 1u : 0u;
+      n += tile_hit(tx, ty, a.cam.tile_w, a.cam.tile_h, o.x, o.y, o.conic, o.power) ? 1u : 0u;
   a.pcount[i] = n;
   a.dkey[i] = depth_bits(o.depth);
   store_records(i, o, a.geom, a.splat);
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(256) void k_duplicate_plain(int P, const float* __r
                                                          const uint32_t* __restrict__ dkey,
                                                          const uint32_t* __restrict__ pcount,
                                                          const uint32_t* __restrict__ poff,
-                                                         int grid_x, uint32_t cap,
+                                                         int grid_x, int tw, int th,
+                                                         uint32_t cap,
                                                          uint64_t* __restrict__ keys,
                                                          uint32_t* __restrict__ vals) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(256) void k_duplicate_plain(int P, const float* __r
   const uint64_t d = dkey[i];
   for (int ty = g.r1; ty < g.r3; ++ty)
     for (int tx = g.r0; tx < g.r2; ++tx)
-      if (tile_hit(tx, ty, g.x, g.y, g.conic, g.power)) {
+      if (tile_hit(tx, ty, tw, th, g.x, g.y, g.conic, g.power)) {
         if (k < cap) {
           keys[k] = ((uint64_t)(uint32_t)(ty * grid_x + tx) << 32) | d;
           vals[k] = (uint32_t)i;
@@ -126,11 +127,13 @@ __global__ __launch_bounds__(256) void k_duplicate_plain(int P, const float* __r
 
 hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey,
                                   const uint32_t* pcount, const uint32_t* poff, int grid_x,
-                                  uint32_t cap, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+                                  int tw, uint32_t cap, uint64_t* keys, uint32_t* vals,
+                                  hipStream_t s) {
   if (P > 0)
     FGS_TIMED("duplicate_plain", s,
               (k_duplicate_plain<<<(P + 255) / 256, 256, 0, s>>>(P, geom, dkey, pcount, poff, grid_x,
-                                                                 cap, keys, vals)));
+                                                                 tw, tile_h_of(tw), cap, keys,
+                                                                 vals)));
   return hipGetLastError();
 }
 
@@ -178,13 +181,14 @@ __device__ __forceinline__ void store_pair(void* tkeys, uint32_t* recs, bool t16
 
 // Walk the concatenated small rects of one wave 64 candidate tiles per step
 // (lane-packed); count (EMIT=false) or emit (EMIT=true, E staged) the pairs
-// that pass the exact tile test. Within a Gaussian's block of the pair array
-// the order is free: its pairs are in distinct tiles and the tile sort is
-// stable.
+// that pass the exact tile test on tw x th tiles. Within a Gaussian's block
+// of the pair array the order is free: its pairs are in distinct tiles and
+// the tile sort is stable.
 template <bool EMIT>
 __device__ __forceinline__ void walk_small(WaveTiles& L, const WaveEmit* E, uint32_t S,
                                            uint32_t cap, void* __restrict__ tkeys,
-                                           uint32_t* __restrict__ recs, bool t16) {
+                                           uint32_t* __restrict__ recs, bool t16, int tw,
+                                           int th) {
   const int lane = lane_id();
   for (uint32_t base = 0; base < S; base += WAVE) {
     const uint32_t f = base + lane;
@@ -199,7 +203,7 @@ __device__ __forceinline__ void walk_small(WaveTiles& L, const WaveEmit* E, uint
       const int q = (int)((float)l / (float)wj);  // exact for l < 2^24
       const int tx = L.r0[j] + (l - q * wj), ty = L.r1[j] + q;
       const float conic[3] = {L.c0[j], L.c1[j], L.c2[j]};
-      if (tile_hit(tx, ty, L.x[j], L.y[j], conic, L.pw[j])) {
+      if (tile_hit(tx, ty, tw, th, L.x[j], L.y[j], conic, L.pw[j])) {
         const uint32_t k = atomicAdd(&L.cnt[j], 1u);
         if (EMIT) {
           const uint32_t pos = E->pos[j] + k;
@@ -283,7 +287,7 @@ __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a, View
   stage_small(L, lane, small, o.x, o.y, o.conic, o.power, o.rect[0], o.rect[1], o.rect[2],
               o.rect[3], S);
   __syncthreads();
-  walk_small<false>(L, nullptr, S, 0u, nullptr, nullptr, false);
+  walk_small<false>(L, nullptr, S, 0u, nullptr, nullptr, false, cam.tile_w, cam.tile_h);
   __syncthreads();
   if (i < a.P) {
     a.pcount[ri] = L.cnt[lane];
@@ -430,7 +434,7 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
     const int y0 = r.r1 + band * rows;
     const int y1 = min(y0 + rows, r.r3);
     const int v = V > 1 ? (int)(g / (uint32_t)P) : 0;
-    const int gx = views[v].grid_x;
+    const int gx = views[v].grid_x, tw = views[v].tile_w, th = views[v].tile_h;
     const uint32_t toff = views[v].tile_off;
     uint32_t base = 0u;
     if (EMIT) {
@@ -442,19 +446,19 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
       const int k = WAVE / wr;  // rows per step
       const int rs = lane / wr;
       const int tx = r.r0 + (lane - rs * wr);
-      const TileCol c = tile_col(tx, r.x, r.conic, r.power);
+      const TileCol c = tile_col(tx, tw, r.x, r.conic, r.power);
       for (int row0 = y0; row0 < y1; row0 += k) {
         const int ty = row0 + rs;
-        const TileRow rw = tile_row(ty, r.y, r.conic, r.power);
+        const TileRow rw = tile_row(ty, th, r.y, r.conic, r.power);
         const bool hit = rs < k && ty < y1 && tile_hit_terms(c, rw, r.conic);
         large_step<EMIT>(hit, tx, ty, base, run, gx, toff, g, cap, tkeys, recs, t16);
       }
     } else {
       for (int c0 = r.r0; c0 < r.r2; c0 += WAVE) {
         const int tx = c0 + lane;
-        const TileCol c = tile_col(tx, r.x, r.conic, r.power);
+        const TileCol c = tile_col(tx, tw, r.x, r.conic, r.power);
         for (int ty = y0; ty < y1; ++ty) {
-          const TileRow rw = tile_row(ty, r.y, r.conic, r.power);
+          const TileRow rw = tile_row(ty, th, r.y, r.conic, r.power);
           const bool hit = tx < r.r2 && tile_hit_terms(c, rw, r.conic);
           large_step<EMIT>(hit, tx, ty, base, run, gx, toff, g, cap, tkeys, recs, t16);
         }
@@ -526,7 +530,8 @@ __global__ __launch_bounds__(256) void k_emit_small(int P, int NR, const ViewRec
   E.toff[lane] = toff;
   E.gx[lane] = gx;
   __syncthreads();
-  walk_small<true>(L, &E, S, cap, tkeys, recs, t16 != 0);
+  // one tile shape for every view of a batch
+  walk_small<true>(L, &E, S, cap, tkeys, recs, t16 != 0, views[0].tile_w, views[0].tile_h);
 }
 
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
@@ -560,11 +565,12 @@ __device__ __forceinline__ void write_view(const ViewSetup& src, ViewRec* out, i
     r.focal_y = s.focal_y;
     r.W = s.W;
     r.H = s.H;
-    r.grid_x = (s.W + TILE_W - 1) / TILE_W;
-    r.grid_y = (s.H + TILE_H - 1) / TILE_H;
+    r.grid_x = grid_dim(s.W, s.tile_w);
+    r.grid_y = grid_dim(s.H, s.tile_h);
     r.tile_off = s.tile_off;
     r.pix_off = s.pix_off;
-    r.pad0 = r.pad1 = 0u;
+    r.tile_w = s.tile_w;
+    r.tile_h = s.tile_h;
   }
 }
 

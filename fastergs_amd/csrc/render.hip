@@ -194,7 +194,8 @@ __global__ __launch_bounds__(256) void k_composite_plain(CompArgs a) {
   const int tile = blockIdx.x;
   const int tx = tile % a.grid_x, ty = tile / a.grid_x;
   const int t = threadIdx.x;
-  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
+  const int tw = a.tile_w, twl = tile_w_log2(tw);
+  const int px = tx * tw + (t & (tw - 1)), py = ty * tile_h_of(tw) + (t >> twl);
   const bool inside = px < a.W && py < a.H;
   const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
   const float fx = (float)px, fy = (float)py;
@@ -310,21 +311,23 @@ struct CompShared {
 __device__ __forceinline__ void composite_item(const CompArgs& a, CompShared& sh, uint32_t item,
                                                int log_idx, uint64_t t_start) {
   const int tile = (int)(item >> 2), part = (int)(item & 3u);
-  const bool band = part != 0;
-  const int row0 = part == 2 ? TILE_H / 2 : 0;
-  const int nrows = band ? TILE_H / 2 : TILE_H;
-  // the tile's view: its size, tile grid, background and image offsets
+  // the tile's view: its size, tile grid and shape, background and image
+  // offsets
   const ViewRec& vr = a.views[a.V > 1 ? view_of_tile(a.views, a.V, (uint32_t)tile) : 0];
-  const int W = vr.W, H = vr.H, gx = vr.grid_x;
+  const int W = vr.W, H = vr.H, gx = vr.grid_x, tw = vr.tile_w, th = vr.tile_h;
+  const int twl = tile_w_log2(tw);
+  const bool band = part != 0;
+  const int row0 = part == 2 ? th / 2 : 0;
+  const int nrows = band ? th / 2 : th;
   const int lt = tile - (int)vr.tile_off;
   const int tx = lt % gx, ty = lt / gx;
   const int t = threadIdx.x;
   const int w = t / WAVE, lane = t & (WAVE - 1);
   // this thread's pixel (a band's upper 128 threads have none)
-  const int mrow = row0 + (t >> 4);
-  const int lp = mrow * TILE_W + (t & 15);
-  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + mrow;
-  const bool mine = (t >> 4) < nrows;
+  const int mrow = row0 + (t >> twl);
+  const int lp = mrow * tw + (t & (tw - 1));
+  const int px = tx * tw + (t & (tw - 1)), py = ty * th + mrow;
+  const bool mine = (t >> twl) < nrows;
   const bool inside = mine && px < W && py < H;
   const float fx = (float)px, fy = (float)py;
   const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
@@ -505,24 +508,27 @@ constexpr int SIMP_PIECE = 32;
 // workgroup.
 struct SimpGrid {
   int W, H, gx, gy;
+  int tw, th;   // tile shape
   int hr, wr;   // inside rows of the last tile row / columns of the last column (0: full)
   int gxf;      // full-width tiles of the last row
   int nbf, nc, nr;  // pieces per last-row tile, of the corner tile, per last-column tile
   int n_pieces;
   int gxi, gyi;     // the full-tile region
 };
-inline SimpGrid simp_grid(int W, int H) {
+inline SimpGrid simp_grid(int W, int H, int tw) {
   SimpGrid g;
   g.W = W;
   g.H = H;
-  g.gx = (W + TILE_W - 1) / TILE_W;
-  g.gy = (H + TILE_H - 1) / TILE_H;
-  g.hr = H % TILE_H;
-  g.wr = W % TILE_W;
+  g.tw = tw;
+  g.th = tile_h_of(tw);
+  g.gx = grid_dim(W, g.tw);
+  g.gy = grid_dim(H, g.th);
+  g.hr = H % g.th;
+  g.wr = W % g.tw;
   g.gxf = g.wr ? g.gx - 1 : g.gx;
-  g.nbf = g.hr ? (TILE_W * g.hr + SIMP_PIECE - 1) / SIMP_PIECE : 0;
+  g.nbf = g.hr ? (g.tw * g.hr + SIMP_PIECE - 1) / SIMP_PIECE : 0;
   g.nc = (g.hr && g.wr) ? (g.wr * g.hr + SIMP_PIECE - 1) / SIMP_PIECE : 0;
-  g.nr = g.wr ? (g.wr * TILE_H + SIMP_PIECE - 1) / SIMP_PIECE : 0;
+  g.nr = g.wr ? (g.wr * g.th + SIMP_PIECE - 1) / SIMP_PIECE : 0;
   g.gxi = g.gxf;
   g.gyi = g.hr ? g.gy - 1 : g.gy;
   g.n_pieces = g.gxf * g.nbf + g.nc + g.gyi * g.nr;
@@ -623,7 +629,8 @@ __device__ __forceinline__ void simp_tile(int tile, const SimpGrid& g,
   const int tx = tile % g.gx, ty = tile / g.gx;
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & (WAVE - 1);
-  const float fx = (float)(tx * TILE_W + (t & 15)), fy = (float)(ty * TILE_H + (t >> 4));
+  const float fx = (float)(tx * g.tw + (t & (g.tw - 1))),
+              fy = (float)(ty * g.th + (t >> tile_w_log2(g.tw)));
   const uint32_t start = ranges[2 * tile], len = ranges[2 * tile + 1] - start;
   // staging: thread t < CH handles position p0 + t of each chunk; ids are
   // loaded two chunks ahead and records one chunk ahead, so no load waits
@@ -760,7 +767,7 @@ __device__ __forceinline__ void simp_piece(int piece, const SimpGrid& g,
     if (b < g.gxf * g.nbf) {
       tile = (g.gy - 1) * g.gx + b / g.nbf;
       q = b % g.nbf;
-      iw = TILE_W;
+      iw = g.tw;
       ih = g.hr;
     } else if ((b -= g.gxf * g.nbf) < g.nc) {
       tile = (g.gy - 1) * g.gx + g.gx - 1;
@@ -772,7 +779,7 @@ __device__ __forceinline__ void simp_piece(int piece, const SimpGrid& g,
       tile = (b / g.nr) * g.gx + g.gx - 1;
       q = b % g.nr;
       iw = g.wr;
-      ih = TILE_H;
+      ih = g.th;
     }
   }
   const int tx = tile % g.gx, ty = tile / g.gx;
@@ -784,7 +791,7 @@ __device__ __forceinline__ void simp_piece(int piece, const SimpGrid& g,
   const int i = PX * q + p;
   const bool in_piece = i < iw * ih;
   const bool inside = in_piece && (wave != 0 || lane < PX);
-  const float fx = (float)(tx * TILE_W + i % iw), fy = (float)(ty * TILE_H + i / iw);
+  const float fx = (float)(tx * g.tw + i % iw), fy = (float)(ty * g.th + i / iw);
   const uint32_t start = ranges[2 * tile], len = ranges[2 * tile + 1] - start;
   const uint32_t nch = (len + CH - 1) / CH;
   // stager: wave 1, lanes < CH, position c CH + lane of chunk c. Step c
@@ -917,11 +924,11 @@ __global__ __launch_bounds__(256) void k_render_simp(SimpGrid g,
   }
 }
 
-hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
-                              const float* splat, float* wsum, int32_t* wcount, float* maxcount,
-                              hipStream_t s) {
+hipError_t launch_render_simp(int W, int H, int tw, int T, const uint32_t* ranges,
+                              const uint32_t* vals, const float* splat, float* wsum,
+                              int32_t* wcount, float* maxcount, hipStream_t s) {
   if (T <= 0) return hipSuccess;
-  const SimpGrid g = simp_grid(W, H);
+  const SimpGrid g = simp_grid(W, H, tw);
   const int grid = g.n_pieces + g.gxi * g.gyi;
   if (grid > 0)
     FGS_TIMED("render_simp", s,
@@ -939,7 +946,7 @@ hipError_t launch_render_simp(int W, int H, int T, const uint32_t* ranges, const
 // only. A pixel stops when T (1 - alpha) < 1e-4; the workgroup stops once
 // all its pixels have stopped. Every pixel inside the image is written.
 __global__ __launch_bounds__(256) void k_render_depth(
-    int W, int H, int grid_x, const uint32_t* __restrict__ ranges,
+    int W, int H, int grid_x, int tw, const uint32_t* __restrict__ ranges,
     const uint32_t* __restrict__ vals, const float* __restrict__ splat,
     const float* __restrict__ means3D, const float* __restrict__ scales,
     const float* __restrict__ rotations, const float* __restrict__ proj,
@@ -952,7 +959,7 @@ __global__ __launch_bounds__(256) void k_render_depth(
   const int tile = blockIdx.x;
   const int tx = tile % grid_x, ty = tile / grid_x;
   const int t = threadIdx.x;
-  const int px = tx * TILE_W + (t & 15), py = ty * TILE_H + (t >> 4);
+  const int px = tx * tw + (t & (tw - 1)), py = ty * tile_h_of(tw) + (t >> tile_w_log2(tw));
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
   const float o[3] = {campos[0], campos[1], campos[2]};
@@ -1009,14 +1016,15 @@ __global__ __launch_bounds__(256) void k_render_depth(
   }
 }
 
-hipError_t launch_render_depth(int W, int H, int T, const uint32_t* ranges, const uint32_t* vals,
+hipError_t launch_render_depth(int W, int H, int tw, int T, const uint32_t* ranges,
+                               const uint32_t* vals,
                                const float* splat, const float* means3D, const float* scales,
                                const float* rotations, const float* proj, const float* campos,
                                float* out_pts, float* out_depth, float* accum_alpha,
                                int32_t* gidx, float* disc, hipStream_t s) {
   if (T > 0)
     FGS_TIMED("render_depth", s,
-              (k_render_depth<<<T, 256, 0, s>>>(W, H, (W + TILE_W - 1) / TILE_W, ranges, vals,
+              (k_render_depth<<<T, 256, 0, s>>>(W, H, grid_dim(W, tw), tw, ranges, vals,
                                                  splat, means3D, scales, rotations, proj, campos,
                                                  out_pts, out_depth, accum_alpha, gidx, disc)));
   return hipGetLastError();

@@ -75,7 +75,7 @@ class CapturedRenderer:
         self.copts.forward_only = 1  # rendering only: no bucket checkpoints
         self.P = P = int(means3D.shape[0])
         self.W, self.H = int(settings.image_width), int(settings.image_height)
-        self.T = ((self.W + 15) // 16) * ((self.H + 15) // 16)
+        self.T = _lib.tiles_of(self.W, self.H, self.opts.tile_w)
         self.headroom = float(headroom)
         self.means3D = _in_place("means3D", means3D, dev, 3).reshape(P, 3)
         self.opacities = _in_place("opacities", opacities, dev, 1).reshape(P)
@@ -268,7 +268,7 @@ class CapturedTrainingStep:
         for H, W in self.sizes:
             self.pix_off.append(self.pix_off[-1] + H * W)
         HW = self.pix_off[-1]
-        self.T = sum(((W + 15) // 16) * ((H + 15) // 16) for H, W in self.sizes)
+        self.T = sum(_lib.tiles_of(W, H, self.opts.tile_w) for H, W in self.sizes)
         # per-view camera buffers the graph reads: view 16 | proj 16 | campos 3 | bg 3
         self.cambuf = torch.zeros(V, 40, dtype=torch.float32, device=dev)
         cams = (Camera * V)()

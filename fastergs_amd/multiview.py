@@ -175,7 +175,7 @@ class BatchFrame:
                                              self.P, ctypes.byref(self.batch), self.R, self.NB,
                                              ctypes.byref(self.copts), ctypes.byref(v)),
               "fgs_batch_frame_view_get")
-        T = sum(((W + 15) // 16) * ((H + 15) // 16) for H, W in self.sizes)
+        T = sum(_lib.tiles_of(W, H, self.opts.tile_w) for H, W in self.sizes)
         PV = self.P * self.V
         spec = {"splat": (torch.float32, PV * 12), "ranges": (torch.int32, 2 * T),
                 "max_contrib": (torch.int32, T), "counts": (torch.int32, 16),

@@ -58,6 +58,10 @@ class RasterOptions:
     # FAST compositing as a persistent grid taking work items from a device
     # counter (same outputs as one workgroup per item)
     composite_queue: bool = False
+    # tile width in pixels (tiles of 256 pixels: 8 x 32, 16 x 16 or 32 x 8);
+    # 16 is the reference's 16 x 16. Semantic in FAST like bucket_size: the
+    # per-tile lists and bucket boundaries are defined on the tile
+    tile_w: int = 16
     # render with pair / bucket capacities remembered from earlier frames of
     # the same shape instead of waiting for this frame's counts (the counts
     # are checked while the render runs; an overflow renders again exactly)
@@ -66,6 +70,7 @@ class RasterOptions:
     def to_c(self, abs_grad: bool = True) -> Options:
         o = Options(self.schedule, self.bucket_size, self.split_tiles, 1 if abs_grad else 0, None)
         o.composite_queue = 1 if self.composite_queue else 0
+        o.tile_w = int(self.tile_w)
         if self.tile_sched is not None:
             ts = self.tile_sched.to_c()
             o.tile_sched = ctypes.pointer(ts)
@@ -197,7 +202,7 @@ class Frame:
         if ts is None:
             tsl = [-1, 0, 0, 0, 0, 0, 0, 0, 0]
         else:
-            T = ((self.W + 15) // 16) * ((self.H + 15) // 16)
+            T = _lib.tiles_of(self.W, self.H, self.opts.tile_w)
             addr = (ts.list_addresses(T, self.device)
                     if lists and hasattr(ts, "list_addresses") else (0, 0, 0, 0))
             tsl = [int(ts.mode), int(ts.split_len), int(bool(ts.features)),
@@ -205,7 +210,7 @@ class Frame:
             if getattr(ts, "model", None) is not None:
                 mlp = ts.model.params
         opts = [int(o.schedule), int(o.bucket_size), int(o.split_tiles), int(o.abs_grad),
-                int(o.forward_only), *tsl, int(o.composite_queue)]
+                int(o.forward_only), *tsl, int(o.composite_queue), int(o.tile_w)]
         n = lambda t: _NONE if t is None else t
         return (self.means3D, self.opacities, self.scales, self.rotations, n(self.dc), n(self.shs),
                 n(self.colors), n(self.culling), self.view, self.proj, self.campos, self.bg,
@@ -335,7 +340,7 @@ class Frame:
                                        self.W, self.H, self.R_cap, self.NB_cap,
                                        ctypes.byref(self.copts),
                                        ctypes.byref(v)), "fgs_frame_view_get")
-        T = ((self.W + 15) // 16) * ((self.H + 15) // 16)
+        T = _lib.tiles_of(self.W, self.H, self.opts.tile_w)
         spec = {
             "geom": (torch.float32, self.P * 8), "splat": (torch.float32, self.P * 12),
             "pair_count": (torch.int32, self.P), "tile_count": (torch.int32, T),

@@ -128,6 +128,12 @@ typedef struct fgs_options {
   int composite_queue; /* FAST compositing as a persistent grid (the workgroups resident at
                           once) taking work items from a device counter, instead of one
                           workgroup per item; the outputs are the same */
+  int tile_w;       /* tile shape: tiles of 256 pixels, tile_w x (256 / tile_w) with tile_w 8,
+                       16 or 32; 0 = 16, the reference's 16 x 16 (BLOCK_X x BLOCK_Y). Every
+                       call on a frame takes the same value. The per-tile lists, and with them
+                       the FAST bucket boundaries (where the T < 1e-4 cut-off is tested) and
+                       render_simp's per-tile stop, are defined on the tile; the PLAIN images
+                       do not depend on it. T (tiles) = ceil(W / tile_w) * ceil(H / tile_h). */
 } fgs_options;
 
 /* Device pointers into the two workspaces, for inspection by tests and

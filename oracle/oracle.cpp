@@ -49,6 +49,7 @@ using fgs::GaussRef;
 template <typename R>
 struct State {
   int P = 0, W = 0, H = 0, gx = 0, gy = 0, T = 0, B = 32, mode = 0, deg = 0, M = 0;
+  int tw = 16, th = 16;  // tile shape (tw x th = 256 pixels)
   R tanfovx, tanfovy, focal_x, focal_y, scale_mod, mult;
   std::vector<R> view, proj, campos, bg;
   std::vector<R> means, opac, scales, rots, dc, shs, colors;
@@ -93,6 +94,8 @@ struct State {
     c.H = H;
     c.grid_x = gx;
     c.grid_y = gy;
+    c.tile_w = tw;
+    c.tile_h = th;
     c.deg = deg;
     c.max_coeffs = M;
     return c;
@@ -133,7 +136,7 @@ void preprocess(State<R>& st) {
     uint32_t n = 0;
     for (int ty = o.rect[1]; ty < o.rect[3]; ++ty)
       for (int tx = o.rect[0]; tx < o.rect[2]; ++tx)
-        n += fgs::tile_hit(tx, ty, o.x, o.y, o.conic, o.power) ? 1u : 0u;
+        n += fgs::tile_hit(tx, ty, st.tw, st.th, o.x, o.y, o.conic, o.power) ? 1u : 0u;
     st.pair_count[i] = n;
   }
 }
@@ -157,7 +160,7 @@ void emit_and_sort(State<R>& st) {
     const float dz = (float)o.depth;
     for (int ty = o.rect[1]; ty < o.rect[3]; ++ty)
       for (int tx = o.rect[0]; tx < o.rect[2]; ++tx)
-        if (fgs::tile_hit(tx, ty, o.x, o.y, o.conic, o.power)) {
+        if (fgs::tile_hit(tx, ty, st.tw, st.th, o.x, o.y, o.conic, o.power)) {
           const uint64_t tile = (uint64_t)(ty * st.gx + tx);
           pairs[k].key = (tile << 32) | fgs::depth_bits(dz);
           pairs[k].val = (uint32_t)i;
@@ -250,11 +253,11 @@ void composite(State<R>& st) {
     const uint32_t b0 = t == 0 ? 0 : st.bucket_off[t - 1];
     const uint32_t b1 = st.bucket_off[t];
     uint32_t mc = 0;
-    for (int ly = 0; ly < fgs::TILE_H; ++ly)
-      for (int lx = 0; lx < fgs::TILE_W; ++lx) {
-        const int px = tx * fgs::TILE_W + lx, py = ty * fgs::TILE_H + ly;
+    for (int ly = 0; ly < st.th; ++ly)
+      for (int lx = 0; lx < st.tw; ++lx) {
+        const int px = tx * st.tw + lx, py = ty * st.th + ly;
         if (px >= st.W || py >= st.H) continue;
-        const int lp = ly * fgs::TILE_W + lx;
+        const int lp = ly * st.tw + lx;
         const R fx = (R)px, fy = (R)py;
         R T = R(1), C0 = R(0), C1 = R(0), C2 = R(0);
         int32_t cnt = 0;
@@ -357,7 +360,7 @@ void backward(State<R>& st, const R* dimg) {
     const uint32_t b0 = t == 0 ? 0 : st.bucket_off[t - 1];
     const uint32_t b1 = st.bucket_off[t];
     const uint32_t tile_start = st.ranges[2 * t];
-    // per pixel of the tile (index ly * 16 + lx)
+    // per pixel of the tile (index ly * tw + lx)
     struct Pix {
       bool in = false;
       R fx, fy, dp0, dp1, dp2, tb;
@@ -366,7 +369,7 @@ void backward(State<R>& st, const R* dimg) {
     };
     Pix pix[fgs::TILE_PIX];
     for (int lp = 0; lp < fgs::TILE_PIX; ++lp) {
-      const int px = tx * fgs::TILE_W + lp % fgs::TILE_W, py = ty * fgs::TILE_H + lp / fgs::TILE_W;
+      const int px = tx * st.tw + lp % st.tw, py = ty * st.th + lp / st.tw;
       Pix& q = pix[lp];
       if (px >= st.W || py >= st.H) continue;
       q.in = true;
@@ -398,8 +401,10 @@ void backward(State<R>& st, const R* dimg) {
         bool any = false;
         for (int l = 0; l < 64; ++l)
           for (int m = 0; m < 4; ++m) {
-            const int lx = 2 * (l & 7) + (m & 1), ly = 2 * (l >> 3) + (m >> 1);
-            const int p = ly * fgs::TILE_W + lx;
+            // lane l's 2x2 quad: quads in row-major order over the tile's
+            // tw/2 x th/2 grid
+            const int lx = 2 * (l % (st.tw / 2)) + (m & 1), ly = 2 * (l / (st.tw / 2)) + (m >> 1);
+            const int p = ly * st.tw + lx;
             lp[l][m] = p;
             act[l][m] = pix[p].in && s - tile_start < pix[p].cnt;  // backward.cu:663-664
             any = any || act[l][m];
@@ -423,7 +428,7 @@ void backward(State<R>& st, const R* dimg) {
         }
       }
     } else {
-      for (int w = 0; w < 4; ++w) {  // pixels 64 w .. 64 w + 63: rows 4 w .. 4 w + 3
+      for (int w = 0; w < 4; ++w) {  // pixels 64 w .. 64 w + 63 (row-major)
         R T[64], Q[64];
         uint32_t top = 0;
         for (int l = 0; l < 64; ++l) {
@@ -503,13 +508,16 @@ template <typename R>
 State<R>* create(int P, const R* means, const R* opac, const R* scales, const R* rots, const R* dc,
                  const R* shs, int M, const R* colors, const uint8_t* cull, const R* view,
                  const R* proj, const R* campos, const R* bg, double tanfovx, double tanfovy,
-                 double scale_mod, double mult, int W, int H, int deg, int bucket, int mode) {
+                 double scale_mod, double mult, int W, int H, int deg, int bucket, int mode,
+                 int tile_w) {
   auto* st = new State<R>();
   st->P = P;
   st->W = W;
   st->H = H;
-  st->gx = (W + fgs::TILE_W - 1) / fgs::TILE_W;
-  st->gy = (H + fgs::TILE_H - 1) / fgs::TILE_H;
+  st->tw = tile_w ? tile_w : fgs::TILE_W_REF;
+  st->th = fgs::tile_h_of(st->tw);
+  st->gx = fgs::grid_dim(W, st->tw);
+  st->gy = fgs::grid_dim(H, st->th);
   st->T = st->gx * st->gy;
   st->B = bucket;
   st->mode = mode;
@@ -641,7 +649,7 @@ void render_simp(const State<R>* st, double* wsum, int32_t* wcount, double* maxc
       const uint32_t id = st->vals[k];
       const R* sp = &st->splat[9 * (size_t)id];
       for (int p = 0; p < fgs::TILE_PIX; ++p) {
-        const int px = tx * fgs::TILE_W + (p % fgs::TILE_W), py = ty * fgs::TILE_H + p / fgs::TILE_W;
+        const int px = tx * st->tw + (p % st->tw), py = ty * st->th + p / st->tw;
         if (px >= st->W || py >= st->H) continue;
         const R a = fgs::splat_alpha(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], (R)px, (R)py);
         if (a == R(0)) continue;  // alpha < 1/255
@@ -666,7 +674,7 @@ void render_simp(const State<R>* st, double* wsum, int32_t* wcount, double* maxc
       if (done) break;
     }
     for (int p = 0; p < fgs::TILE_PIX; ++p) {
-      const int px = tx * fgs::TILE_W + (p % fgs::TILE_W), py = ty * fgs::TILE_H + p / fgs::TILE_W;
+      const int px = tx * st->tw + (p % st->tw), py = ty * st->th + p / st->tw;
       if (px < st->W && py < st->H && flag[p]) maxcount[imax[p]] += 1.0;
     }
   }
@@ -695,7 +703,7 @@ void render_depth(const State<R>* st, R* out_pts, R* out_depth, R* accum_alpha, 
     const int tx = t % st->gx, ty = t / st->gx;
     const uint32_t s = st->ranges[2 * t], e = st->ranges[2 * t + 1];
     for (int p = 0; p < fgs::TILE_PIX; ++p) {
-      const int px = tx * fgs::TILE_W + (p % fgs::TILE_W), py = ty * fgs::TILE_H + p / fgs::TILE_W;
+      const int px = tx * st->tw + (p % st->tw), py = ty * st->th + p / st->tw;
       if (px >= W || py >= H) continue;
       R nd[3], len;
       fgs::depth_ray(inv, o, px, py, W, H, nd, len);
@@ -768,9 +776,9 @@ void* orc_create_f32(int P, const float* means, const float* opac, const float* 
                      const float* colors, const uint8_t* cull, const float* view,
                      const float* proj, const float* campos, const float* bg, double tanfovx,
                      double tanfovy, double scale_mod, double mult, int W, int H, int deg,
-                     int bucket, int mode) {
+                     int bucket, int mode, int tile_w) {
   return create<float>(P, means, opac, scales, rots, dc, shs, M, colors, cull, view, proj, campos,
-                       bg, tanfovx, tanfovy, scale_mod, mult, W, H, deg, bucket, mode);
+                       bg, tanfovx, tanfovy, scale_mod, mult, W, H, deg, bucket, mode, tile_w);
 }
 void orc_forward_f32(void* h) { forward(*(State<float>*)h); }
 void orc_backward_f32(void* h, const float* dimg) { backward(*(State<float>*)h, dimg); }
@@ -791,9 +799,10 @@ void* orc_create_f64(int P, const double* means, const double* opac, const doubl
                      const double* colors, const uint8_t* cull, const double* view,
                      const double* proj, const double* campos, const double* bg, double tanfovx,
                      double tanfovy, double scale_mod, double mult, int W, int H, int deg,
-                     int bucket, int mode) {
+                     int bucket, int mode, int tile_w) {
   return create<double>(P, means, opac, scales, rots, dc, shs, M, colors, cull, view, proj,
-                        campos, bg, tanfovx, tanfovy, scale_mod, mult, W, H, deg, bucket, mode);
+                        campos, bg, tanfovx, tanfovy, scale_mod, mult, W, H, deg, bucket, mode,
+                        tile_w);
 }
 void orc_forward_f64(void* h) { forward(*(State<double>*)h); }
 void orc_backward_f64(void* h, const double* dimg) { backward(*(State<double>*)h, dimg); }
@@ -805,23 +814,25 @@ void orc_destroy_f64(void* h) { delete (State<double>*)h; }
 // scalar helpers exposed for unit tests of the shared arithmetic
 float orc_exp2f(float x) { return fgs::exp2_(x); }
 float orc_log2f(float x) { return fgs::log2_(x); }
-int orc_tile_hit_f32(int tx, int ty, float x, float y, float ca, float cb, float cc, float power) {
+int orc_tile_hit_f32(int tx, int ty, float x, float y, float ca, float cb, float cc, float power,
+                     int tw) {
   const float conic[3] = {ca, cb, cc};
-  return fgs::tile_hit(tx, ty, x, y, conic, power) ? 1 : 0;
+  return fgs::tile_hit(tx, ty, tw, fgs::tile_h_of(tw), x, y, conic, power) ? 1 : 0;
 }
 // Checks tile_hit's factored form (tile_col / tile_row / tile_hit_terms,
 // used by the GPU's walks over large rects) against tile_hit on every tile
 // of a gx x gy grid for one ellipse; returns the number of disagreements
 // and the number of hits.
 int orc_tile_terms_check(float x, float y, float ca, float cb, float cc, float power, int gx,
-                         int gy, int* hits) {
+                         int gy, int* hits, int tw) {
   const float conic[3] = {ca, cb, cc};
+  const int th = fgs::tile_h_of(tw);
   int bad = 0, h = 0;
   for (int ty = 0; ty < gy; ++ty) {
-    const fgs::TileRow r = fgs::tile_row(ty, y, conic, power);
+    const fgs::TileRow r = fgs::tile_row(ty, th, y, conic, power);
     for (int tx = 0; tx < gx; ++tx) {
-      const bool a = fgs::tile_hit(tx, ty, x, y, conic, power);
-      const bool b = fgs::tile_hit_terms(fgs::tile_col(tx, x, conic, power), r, conic);
+      const bool a = fgs::tile_hit(tx, ty, tw, th, x, y, conic, power);
+      const bool b = fgs::tile_hit_terms(fgs::tile_col(tx, tw, x, conic, power), r, conic);
       bad += a != b;
       h += a;
     }

@@ -32,7 +32,7 @@ def lib():
         for suf in ("f32", "f64"):
             create = getattr(L, "orc_create_" + suf)
             create.restype = vp
-            create.argtypes = [i32] + [vp] * 6 + [i32] + [vp] * 6 + [f64] * 4 + [i32] * 5
+            create.argtypes = [i32] + [vp] * 6 + [i32] + [vp] * 6 + [f64] * 4 + [i32] * 6
             getattr(L, "orc_forward_" + suf).argtypes = [vp]
             getattr(L, "orc_backward_" + suf).argtypes = [vp, vp]
             g = getattr(L, "orc_get_" + suf)
@@ -45,10 +45,10 @@ def lib():
         L.orc_exp2f.restype = ctypes.c_float
         L.orc_log2f.argtypes = [ctypes.c_float]
         L.orc_log2f.restype = ctypes.c_float
-        L.orc_tile_hit_f32.argtypes = [i32, i32] + [ctypes.c_float] * 6
+        L.orc_tile_hit_f32.argtypes = [i32, i32] + [ctypes.c_float] * 6 + [i32]
         L.orc_tile_hit_f32.restype = i32
         L.orc_num_threads.restype = i32
-        L.orc_tile_terms_check.argtypes = [ctypes.c_float] * 6 + [i32, i32, vp]
+        L.orc_tile_terms_check.argtypes = [ctypes.c_float] * 6 + [i32, i32, vp, i32]
         L.orc_tile_terms_check.restype = i32
         L.orc_render_simp_f32.argtypes = [vp, vp, vp, vp]
         L.orc_render_depth_f32.argtypes = [vp] * 6
@@ -76,11 +76,13 @@ class Oracle:
 
     mode 0 = bucketed compositing (reference semantics), 1 = sequential
     (plain path). dtype float32 (bit-exact checker) or float64 (FD checks).
+    tile_w: tiles of tile_w x (256 / tile_w) pixels (16: the reference's).
     """
 
     def __init__(self, scene: dict, cam, bg=(0.0, 0.0, 0.0), sh_degree: int = 3,
                  bucket: int = 32, mode: int = 0, dtype=np.float32, scale_modifier: float = 1.0,
-                 mult: float = 1.0, colors=None, culling=None, use_sh: bool = True):
+                 mult: float = 1.0, colors=None, culling=None, use_sh: bool = True,
+                 tile_w: int = 16):
         self.L = lib()
         self.dtype = np.dtype(dtype)
         self.suf = "f32" if self.dtype == np.float32 else "f64"
@@ -105,14 +107,15 @@ class Oracle:
         bgv = c(np.asarray(bg)).reshape(3)
         self._keep = [means, opac, scales, rots, dc, shs, col, cull, view, proj, campos, bgv]
         self.W, self.H = cam.width, cam.height
-        self.gx = (self.W + 15) // 16
-        self.gy = (self.H + 15) // 16
+        self.tile_w, self.tile_h = tile_w, 256 // tile_w
+        self.gx = (self.W + self.tile_w - 1) // self.tile_w
+        self.gy = (self.H + self.tile_h - 1) // self.tile_h
         self.bucket = bucket
         self.h = getattr(self.L, "orc_create_" + self.suf)(
             P, _ptr(means), _ptr(opac), _ptr(scales), _ptr(rots), _ptr(dc), _ptr(shs), M,
             _ptr(col), _ptr(cull), _ptr(view), _ptr(proj), _ptr(campos), _ptr(bgv),
             float(cam.tanfovx), float(cam.tanfovy), float(scale_modifier), float(mult),
-            self.W, self.H, sh_degree, bucket, mode)
+            self.W, self.H, sh_degree, bucket, mode, tile_w)
 
     def forward(self):
         getattr(self.L, "orc_forward_" + self.suf)(self.h)
@@ -199,8 +202,9 @@ def log2f(x: float) -> float:
     return lib().orc_log2f(x)
 
 
-def tile_hit(tx, ty, x, y, conic, power) -> bool:
-    return bool(lib().orc_tile_hit_f32(tx, ty, x, y, conic[0], conic[1], conic[2], power))
+def tile_hit(tx, ty, x, y, conic, power, tile_w: int = 16) -> bool:
+    return bool(lib().orc_tile_hit_f32(tx, ty, x, y, conic[0], conic[1], conic[2], power,
+                                       tile_w))
 
 
 def num_threads() -> int:

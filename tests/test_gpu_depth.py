@@ -23,19 +23,23 @@ CASES = {
 }
 
 
-def _frame(scene, cam, schedule):
+def _frame(scene, cam, schedule, tile_w=16):
     d = to_dev(scene)
     return Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
-                 dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule, bucket_size=32)).forward()
+                 dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule, bucket_size=32,
+                                                                 tile_w=tile_w)).forward()
 
 
-@pytest.mark.parametrize("case", list(CASES))
+SHAPED = [(c, 16) for c in CASES] + [("clusters_40k_410x250", 8), ("clusters_40k_410x250", 32)]
+
+
+@pytest.mark.parametrize("case,tile_w", SHAPED, ids=[f"{c}-tile{w}" for c, w in SHAPED])
 @pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "fast"])
-def test_render_depth_matches_oracle(case, schedule):
-    print(f"\n[depth] {case} schedule {schedule}", flush=True)
+def test_render_depth_matches_oracle(case, tile_w, schedule):
+    print(f"\n[depth] {case} tile_w {tile_w} schedule {schedule}", flush=True)
     scene, cam = CASES[case]()
-    res = _frame(scene, cam, schedule).render_depth()
-    orc = Oracle(scene, cam, bucket=32, mode=0 if schedule else 1).forward()
+    res = _frame(scene, cam, schedule, tile_w).render_depth()
+    orc = Oracle(scene, cam, bucket=32, mode=0 if schedule else 1, tile_w=tile_w).forward()
     pts, depth, alpha, gidx, disc = orc.render_depth()
     np.testing.assert_array_equal(res["gidx"][0].cpu().numpy(), gidx)
     np.testing.assert_array_equal(res["accum_alpha"][0].cpu().numpy(), alpha)

@@ -4,7 +4,9 @@ Bit-exact: records, per-tile lists (and the 64-bit keys rebuilt from them),
 tile ranges, bucket offsets / ranges / owner tiles, per-bucket checkpoints up
 to each pixel's cutoff, images, T_final, last contributors. Gradients within
 1e-4 (relative). The split threshold of the packed/split tile walk and the
-bucket size must not change the lists.
+bucket size must not change the lists. Every check runs on 16 x 16 tiles (the
+reference's) and, for the heavy-tailed case, on 8 x 32 and 32 x 8 tiles
+against the oracle on the same tile shape.
 """
 import numpy as np
 import pytest
@@ -26,22 +28,27 @@ CASES = {
 }
 
 
-def run_fast(scene, cam, bucket=32, split=64):
+def run_fast(scene, cam, bucket=32, split=64, tile_w=16):
     d = to_dev(scene)
     fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
                dc=d["dc"], shs=d["shs"],
                options=RasterOptions(schedule=SCHEDULE_FAST, bucket_size=bucket,
-                                     split_tiles=split)).forward()
+                                     split_tiles=split, tile_w=tile_w)).forward()
     torch.cuda.synchronize()
     return fr
 
 
-@pytest.fixture(scope="module", params=list(CASES))
+PARAMS = [(c, 16) for c in CASES] + [("heavy_tailed_60k_640x420", 8),
+                                     ("heavy_tailed_60k_640x420", 32)]
+
+
+@pytest.fixture(scope="module", params=PARAMS, ids=[f"{c}-tile{w}" for c, w in PARAMS])
 def case(request):
-    print(f"\n[case] {request.param}", flush=True)
-    scene, cam = CASES[request.param]()
-    fr = run_fast(scene, cam)
-    orc = Oracle(scene, cam, bucket=32, mode=0).forward()
+    name, tile_w = request.param
+    print(f"\n[case] {name} tile_w {tile_w}", flush=True)
+    scene, cam = CASES[name]()
+    fr = run_fast(scene, cam, tile_w=tile_w)
+    orc = Oracle(scene, cam, bucket=32, mode=0, tile_w=tile_w).forward()
     return scene, cam, fr, orc
 
 
@@ -97,11 +104,11 @@ def test_image_and_checkpoints(case):
     br = orc.get("bucket_ranges").reshape(-1, 2).astype(np.int64)
     bt = orc.get("bucket_tile").astype(np.int64)
     rng = orc.get("ranges").reshape(-1, 2).astype(np.int64)
-    gx = orc.gx
+    gx, tw, th = orc.gx, orc.tile_w, orc.tile_h
     lp = np.arange(256)
     tx, ty = bt % gx, bt // gx
-    px = (tx[:, None] * 16 + (lp % 16)[None, :])
-    py = (ty[:, None] * 16 + (lp // 16)[None, :])
+    px = (tx[:, None] * tw + (lp % tw)[None, :])
+    py = (ty[:, None] * th + (lp // tw)[None, :])
     inside = (px < orc.W) & (py < orc.H)
     c = np.zeros_like(px)
     c[inside] = contrib.reshape(orc.H, orc.W)[py[inside], px[inside]]

@@ -79,6 +79,32 @@ def test_queue_compositing_matches(batch_case):
                       name=f"queue {rep} raw[{v}]")
 
 
+def test_tile_shape_batch_matches_single_views(batch_case):
+    """On 8 x 32 tiles the batch equals single-view frames on 8 x 32 tiles
+    (images, T_final, last contributors, raw sums)."""
+    print("\n[multiview] tile_w 8", flush=True)
+    scene, cams, d, bf, singles = batch_case
+    opts = RasterOptions(schedule=SCHEDULE_FAST, bucket_size=32, tile_w=8)
+    b8 = BatchFrame([settings_for(c) for c in cams], d["means3D"], d["opacities"], d["scales"],
+                    d["rotations"], dc=d["dc"], shs=d["shs"], options=opts).forward()
+    s8 = [Frame(settings_for(c), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+                dc=d["dc"], shs=d["shs"], options=opts).forward() for c in cams]
+    torch.cuda.synchronize()
+    tf, lc = b8.per_view(b8.T_final), b8.per_view(b8.last_contributor)
+    g = torch.cat([torch.from_numpy(scenes.target_image(c.width, c.height, seed=v)).cuda()
+                   .reshape(-1) for v, c in enumerate(cams)])
+    gb = b8.backward(g)
+    off = 0
+    for v, (img, s) in enumerate(zip(b8.images(), s8)):
+        np.testing.assert_array_equal(img.cpu().numpy(), s.out_color.cpu().numpy())
+        np.testing.assert_array_equal(tf[v].cpu().numpy(), s.T_final[..., 0].cpu().numpy())
+        np.testing.assert_array_equal(lc[v].cpu().numpy(), s.last_contributor[..., 0].cpu().numpy())
+        n = img.numel()
+        gs = s.backward(g[off:off + n].reshape(img.shape))
+        off += n
+        raw_close(gb["raw"][v].cpu().numpy(), gs["raw"].cpu().numpy(), name=f"tile8 raw[{v}]")
+
+
 def test_lists_match_single_views(batch_case):
     scene, cams, d, bf, singles = batch_case
     bv = bf.view_arrays()
@@ -173,21 +199,25 @@ def test_autotuner_picks_a_candidate(batch_case, tmp_path):
     cache = str(tmp_path / "tune.json")
     # bucket size tuned on request (it is semantic); split threshold and the
     # compositing work shape (whole tiles or 16x8 bands) never change outputs
+    # and the tile shape on request (semantic as well); the launch form of
+    # compositing (one workgroup per item or the work-queue grid) never does
     tuner = Autotuner(buckets=(16, 32), splits=(32, 128), work_splits=(None, 1), reps=1,
-                      cache_path=cache)
+                      cache_path=cache, tile_ws=(16, 32))
     best, table = tuner.tune(step, key="k")
-    assert len(table) == 8 and table[0][1] <= table[-1][1]
+    assert len(table) == 32 and table[0][1] <= table[-1][1]
     assert (best.bucket_size, best.split_tiles) in {(16, 32), (16, 128), (32, 32), (32, 128)}
+    assert best.tile_w in (16, 32)
     shape = lambda o: None if o.tile_sched is None else o.tile_sched.split_len
     n = len(calls)
     again, _ = Autotuner(cache_path=cache).tune(step, key="k")
     assert len(calls) == n  # served from the cache file
-    assert (again.bucket_size, again.split_tiles, shape(again)) == \
-        (best.bucket_size, best.split_tiles, shape(best))
+    assert (again.bucket_size, again.split_tiles, shape(again), again.tile_w,
+            again.composite_queue) == \
+        (best.bucket_size, best.split_tiles, shape(best), best.tile_w, best.composite_queue)
     # without buckets the caller's bucket size is kept
-    fixed = Autotuner(splits=(64,), work_splits=(None,), reps=1,
-                      base=RasterOptions(schedule=1, bucket_size=32))
-    assert [c.bucket_size for c in fixed.candidates] == [32]
+    fixed = Autotuner(splits=(64,), work_splits=(None,), reps=1, queues=(False,),
+                      base=RasterOptions(schedule=1, bucket_size=32, tile_w=8))
+    assert [(c.bucket_size, c.tile_w) for c in fixed.candidates] == [(32, 8)]
     # the tuned options render exactly what a frame with those options does
     # (the bucket size is part of the reference's semantics -- the T < 1e-4
     # stop is tested once per bucket -- so images of different bucket sizes

@@ -23,16 +23,20 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("case", list(CASES))
+SHAPED = [(c, 16) for c in CASES] + [("clusters_40k_410x250", 8), ("clusters_40k_410x250", 32)]
+
+
+@pytest.mark.parametrize("case,tile_w", SHAPED, ids=[f"{c}-tile{w}" for c, w in SHAPED])
 @pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "fast"])
-def test_render_simp_matches_oracle(case, schedule):
-    print(f"\n[simp] {case} schedule {schedule}", flush=True)
+def test_render_simp_matches_oracle(case, tile_w, schedule):
+    print(f"\n[simp] {case} tile_w {tile_w} schedule {schedule}", flush=True)
     scene, cam = CASES[case]()
     d = to_dev(scene)
     fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
-               dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule, bucket_size=32)).forward()
+               dc=d["dc"], shs=d["shs"], options=RasterOptions(schedule=schedule, bucket_size=32,
+                                                               tile_w=tile_w)).forward()
     w, c, m = fr.render_simp()
-    orc = Oracle(scene, cam, bucket=32, mode=0 if schedule else 1).forward()
+    orc = Oracle(scene, cam, bucket=32, mode=0 if schedule else 1, tile_w=tile_w).forward()
     rw, rc, rm = orc.render_simp()
     np.testing.assert_array_equal(c.cpu().numpy(), rc)
     np.testing.assert_array_equal(m.cpu().numpy(), rm.astype(np.float32))

@@ -107,10 +107,15 @@ def _min_quad_on_box(cx, cy, A, B, C, x0, x1, y0, y1):
     return best
 
 
-def test_tile_set_brute_force():
+@pytest.mark.parametrize("tile_w", [16, 8, 32])
+def test_tile_set_brute_force(tile_w):
+    """Every Gaussian's tiles are exactly the tiles (of each tile shape) whose
+    pixel-centre box meets the ellipse {q <= 2 power}, by a minimisation of
+    the quadratic form over each box."""
+    tw, th = tile_w, 256 // tile_w
     scene = scenes.uniform_scene(3000, seed=3, log_mu=-3.5, log_sigma=0.8)
     cam = scenes.look_at_camera((0.2, 0.1, -4.0), 320, 240)
-    o = Oracle(scene, cam).forward()
+    o = Oracle(scene, cam, tile_w=tile_w).forward()
     pj = o.get("proj").reshape(-1, 16)
     keys, vals = o.get("keys"), o.get("vals")
     tiles = (keys >> np.uint64(32)).astype(np.int64)
@@ -126,19 +131,20 @@ def test_tile_set_brute_force():
         r0, r1, r2, r3 = (int(v) for v in pj[i, 1:5])
         got = hit.get(int(i), set())
         assert all(r0 <= t % gx < r2 and r1 <= t // gx < r3 for t in got)  # subset of rect
-        cx_t, cy_t = int(x // 16), int(y // 16)
+        cx_t, cy_t = int(x // tw), int(y // th)
         if 0 <= x < o.W and 0 <= y < o.H and r0 <= cx_t < r2 and r1 <= cy_t < r3:
             assert cy_t * gx + cx_t in got  # centre tile always present
         for ty in range(r1, r3):
             for tx in range(r0, r2):
-                q = _min_quad_on_box(x, y, A, B, C, 16 * tx, 16 * tx + 15, 16 * ty, 16 * ty + 15)
+                q = _min_quad_on_box(x, y, A, B, C, tw * tx, tw * tx + tw - 1, th * ty,
+                                     th * ty + th - 1)
                 thr = 2 * power
                 if abs(q - thr) <= 1e-4 * thr:
                     ambiguous += 1
                     continue
                 checked += 1
                 assert ((ty * gx + tx) in got) == (q <= thr), (i, tx, ty, q, thr)
-    assert checked > 5000 and ambiguous < checked * 1e-3
+    assert checked > 4000 and ambiguous < checked * 1e-3
 
 
 def test_factored_tile_test_equals_tile_hit():
@@ -159,10 +165,11 @@ def test_factored_tile_test_equals_tile_hit():
         con = np.linalg.inv(R @ np.diag([s1 * s1, s2 * s2]) @ R.T + 0.3 * np.eye(2))
         power = float(np.log(256.0 * rng.uniform(0.005, 1.0)))
         x, y = (float(v) for v in rng.uniform(-300, 1600, 2))
-        bad = L.orc_tile_terms_check(x, y, con[0, 0], con[0, 1], con[1, 1], power, 100, 70,
-                                     ctypes.byref(hits))
-        assert bad == 0, (x, y, con, power)
-        total_hits += hits.value
+        for tw in (16, 8, 32):
+            bad = L.orc_tile_terms_check(x, y, con[0, 0], con[0, 1], con[1, 1], power,
+                                         1600 // tw, 1120 * tw // 256, ctypes.byref(hits), tw)
+            assert bad == 0, (x, y, con, power, tw)
+            total_hits += hits.value
     assert total_hits > 100000
 
 
