@@ -240,6 +240,7 @@ __device__ __forceinline__ void stage_small(WaveTiles& L, int lane, bool small, 
 // (colour comes from k_sh_color), then the wave's lane-packed tile walk.
 // Only 3 KB of LDS per wave64 workgroup, so occupancy is set by registers.
 constexpr int PRE_THREADS = WAVE;
+
 // Grid (ceil(P / 64), V): blockIdx.y is the view, records v * P + i.
 __device__ __forceinline__ void write_view(const ViewSetup& src, ViewRec* out, int v, int t);
 
@@ -307,12 +308,16 @@ __global__ __launch_bounds__(PRE_THREADS) void k_preprocess_fast(PreArgs a, View
 constexpr int SHC_DC_F4 = PRE_THREADS;                                       // 64 float4
 constexpr int SHC_QMAX = (3 * 15 * PRE_THREADS / 4 + PRE_THREADS - 1) / PRE_THREADS;  // 12
 constexpr size_t SHC_SHMEM = (size_t)(SHC_DC_F4 + SHC_QMAX * PRE_THREADS) * sizeof(float4);
+// 6 per CU of the 256 (measured at 1M Gaussians, 1280x840: 2, 3, 4, 6, 8
+// per CU and an unbounded grid give 0.996, 0.966, 0.957, 0.950, 0.968 and
+// 0.971 ms per training step)
+constexpr unsigned SHC_BLOCKS = 256 * 6;
 
-__global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
-  extern __shared__ float4 rows4[];
+// One group of 64 Gaussians of k_sh_color.
+__device__ __forceinline__ void sh_color_group(const PreArgs& a, float4* rows4, int grp) {
   float* rows = (float*)rows4;
   const int lane = threadIdx.x;
-  const int g0 = blockIdx.x * PRE_THREADS;
+  const int g0 = grp * PRE_THREADS;
   const int i = g0 + lane;
   const int M = a.cam.max_coeffs;
   float* sdc = rows;                    // 64 x 3 dc
@@ -363,6 +368,19 @@ __global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
   }
 }
 
+// A grid of at most SHC_BLOCKS workgroups (each takes groups of 64 Gaussians
+// in a grid-stride loop): the pass runs beside the depth sort, whose
+// latency-bound passes need a workgroup slot and 55 KB of LDS on every CU; a
+// full grid of this pass would hold every slot until it ends.
+__global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
+  extern __shared__ float4 rows4[];
+  const int ngroups = (a.P + PRE_THREADS - 1) / PRE_THREADS;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    if (grp != (int)blockIdx.x) __syncthreads();  // the previous group's rows are read
+    sh_color_group(a, rows4, grp);
+  }
+}
+
 hipError_t launch_preprocess_fast(const PreArgs& a, const ViewSetup& vs, ViewRec* table,
                                   uint32_t* zero, int zero_n, hipStream_t s) {
   if (a.P <= 0) return launch_views_setup(vs, table, zero, zero_n, s);
@@ -375,7 +393,8 @@ hipError_t launch_preprocess_fast(const PreArgs& a, const ViewSetup& vs, ViewRec
 
 hipError_t launch_sh_color(const PreArgs& a, hipStream_t s) {
   if (a.P <= 0 || a.G.colors != nullptr || a.G.dc == nullptr) return hipSuccess;
-  const unsigned blocks = (a.P + PRE_THREADS - 1) / PRE_THREADS;
+  const unsigned groups = (a.P + PRE_THREADS - 1) / PRE_THREADS;
+  const unsigned blocks = groups < SHC_BLOCKS ? groups : SHC_BLOCKS;
   const size_t rows = (size_t)(4 * SHC_DC_F4 + PRE_THREADS * 3 * a.cam.max_coeffs) * sizeof(float);
   const size_t shmem = rows > SHC_SHMEM ? rows : SHC_SHMEM;
   FGS_TIMED("sh_color", s, (k_sh_color<<<blocks, PRE_THREADS, shmem, s>>>(a)));
@@ -476,7 +495,10 @@ hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, con
                               const uint32_t* poff, const uint32_t* lrank, uint32_t* lband,
                               uint32_t cap, void* tkeys, uint32_t* recs, bool t16, hipStream_t s) {
   if (P <= 0) return hipSuccess;
-  const int blocks = 4096;  // 16384 waves, grid-stride; each exits past the device-side count
+  // 16384 waves, grid-stride; each exits past the device-side count (fewer
+  // for the count, which runs beside the depth sort, measured slower: the
+  // pair offsets wait for it)
+  const int blocks = 4096;
   if (emit)
     FGS_TIMED("large_tiles_emit", s,
               (k_large_tiles<true><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
