@@ -231,8 +231,9 @@ def kernel_units(name, P, M, V, R, T, E, fast):
         # geometry only: in means, opacity, scales, rotations; out geom 32,
         # splat 48, depth key 4, depth-sort key 8 + index 4, large flag 4, count 4, radius 4
         "preprocess_fast": ("hbm", P * (12 + 4 + 12 + 16) + P * (32 + 48 + 4 + 8 + 4 + 4 + 4 + 4)),
-        # colour pass: in dc + rest SH, means, depth key; out rgb + clamp 16
-        "sh_color": ("hbm", P * (sh + 12 + 4) + V * 16),
+        # colour pass: in dc + rest SH, means, depth key; out rgb + clamp 16 and
+        # (frames that will be differentiated) the 3x3 SH Jacobian 36
+        "sh_color": ("hbm", P * (sh + 12 + 4) + V * 16 + P * 36),
         "preprocess_plain": ("hbm", P * (12 + 4 + 12 + 16 + sh) + P * (32 + 48 + 4 + 4 + 4)),
         # in: means, scales, rotations, opacity, rest SH, raw sums 48, radius, clamp;
         # out: mean2D 16, conic 16, opacity 4, colour 12, mean3D 12, cov3D 24, dc 12, SH, scale 12, rot 16
@@ -247,6 +248,11 @@ def kernel_units(name, P, M, V, R, T, E, fast):
         # conic, colour or cov3D gradients)
         "gauss_bwd_geom": ("hbm", P * (48 + 4 + 4 + 12 + 12 + 16 + 4 + 16)
                            + P * (16 + 4 + 12 + 12 + 16)),
+        # adjoint with the forward's SH Jacobians: in raw sums 48, opacity,
+        # radius, means, scales, rotations, clamp, Jacobian 36; out mean2D 16,
+        # opacity 4, mean3D 12, scale 12, rotation 16, dc 12, rest SH
+        "gauss_bwd_jac": ("hbm", P * (48 + 4 + 4 + 12 + 12 + 16 + 4 + 36)
+                          + P * (16 + 4 + 12 + 12 + 16 + 12 + 12 * M)),
         # key-only pass of the tile sort: read + write 8 B per pair; the
         # last pass writes the 4 B list entry and the 4 B tile id instead
         "onesweep_k": ("hbm", 16 * R),

@@ -351,7 +351,7 @@ const char* fgs_build_info(void) {
 }
 
 size_t fgs_gauss_ws_size(int P, int width, int height, const fgs_options* opt) {
-  return carve_gauss(nullptr, P, tiles_of(width, height, tile_w_of(opt))).total;
+  return carve_gauss(nullptr, P, tiles_of(width, height, tile_w_of(opt)), true).total;
 }
 
 size_t fgs_pair_ws_size(int P, int width, int height, int64_t R, int64_t NB,
@@ -372,7 +372,7 @@ size_t fgs_grad_ws_size(int P) {
 size_t fgs_batch_gauss_ws_size(int P, const fgs_batch* batch, const fgs_options* opt) {
   Batch b;
   if (make_batch(batch, b, opt, false) != FGS_OK) return 0;
-  return carve_gauss(nullptr, (int64_t)P * b.V, b.T).total;
+  return carve_gauss(nullptr, (int64_t)P * b.V, b.T, b.V == 1).total;
 }
 
 size_t fgs_batch_pair_ws_size(int P, const fgs_batch* batch, int64_t R, int64_t NB,
@@ -514,6 +514,9 @@ PreArgs pre_args(const Batch& b, const fgs_gaussians* g, const fgs_options* opt,
   a.counts = w.counts;
   a.views = (const ViewRec*)w.views;
   a.V = b.V;
+  // single FAST views that will be differentiated keep the SH Jacobians
+  a.shjac = b.V == 1 && opt->schedule == FGS_SCHEDULE_FAST && !opt->forward_only ? w.shjac
+                                                                                  : nullptr;
   return a;
 }
 
@@ -527,7 +530,7 @@ int fgs_batch_preprocess(const fgs_batch* batch, const fgs_gaussians* g, const f
   const int64_t PV = (int64_t)g->P * b.V;
   // band list entries are record * LARGE_BANDS + band in 32 bits
   if (PV >= ((int64_t)1 << 29)) return fail(FGS_ERR_ARG, "P x n_views must be < 2^29");
-  const GaussWs w = carve_gauss(gauss_ws, PV, b.T);
+  const GaussWs w = carve_gauss(gauss_ws, PV, b.T, b.V == 1);
   if (!gauss_ws || gauss_ws_bytes < w.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
   if (g->P > 0 && !radii) return fail(FGS_ERR_ARG, "radii is NULL");
   hipStream_t s = (hipStream_t)stream;
@@ -600,7 +603,7 @@ int fgs_batch_read_counts(const void* gauss_ws, int P, const fgs_batch* batch,
   int rc;
   if ((rc = make_batch(batch, b, opt, false))) return rc;
   if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
-  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), (int64_t)P * b.V, b.T);
+  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), (int64_t)P * b.V, b.T, b.V == 1);
   // [0] pairs, [1] bucket-table capacity, up to the depth sort's fault flag
   uint32_t c[DSORT_FAULT + 1];
   hipStream_t s = (hipStream_t)stream;
@@ -627,7 +630,7 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   if (!out_color || !T_final || !last_contributor) return fail(FGS_ERR_ARG, "NULL output");
   if (R < 0 || NB < 0 || R >= (int64_t)0xffffffffll) return fail(FGS_ERR_ARG, "bad R / NB");
   const int T = b.T;
-  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)g->P * b.V, T);
+  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)g->P * b.V, T, b.V == 1);
   if (!gauss_ws || gauss_ws_bytes < gw.total) return fail(FGS_ERR_CAPACITY, "gauss workspace too small");
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast, keeps_ckpt(opt));
@@ -781,7 +784,7 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   const fgs_camera* cam = &b.cams[0];
   const int T = b.T;
   const int64_t PV = (int64_t)g->P * b.V;
-  const GaussWs gw = carve_gauss(gauss_ws, PV, T);
+  const GaussWs gw = carve_gauss(gauss_ws, PV, T, b.V == 1);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast, keeps_ckpt(opt));
   hipStream_t s = (hipStream_t)stream;
@@ -859,6 +862,7 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   q.views = fast ? (const ViewRec*)gw.views : nullptr;
   q.V = b.V;
   q.sh_dm = b.V == 1 ? raw_lin + (size_t)PV * RAW_LIN_STRIDE : nullptr;
+  q.shjac = b.V == 1 && fast ? gw.shjac : nullptr;
   FGS_TRY(launch_gauss_bwd(q, s), "preprocess backward");
   return FGS_OK;
 }
@@ -893,7 +897,7 @@ int fgs_batch_frame_view_get(void* gauss_ws, void* pair_ws, int P, const fgs_bat
   if ((rc = check_opts(opt)) || (rc = make_batch(batch, b, opt, false))) return rc;
   if (!v) return fail(FGS_ERR_ARG, "view is NULL");
   const int T = b.T;
-  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T);
+  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T, b.V == 1);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, fast, keeps_ckpt(opt));
   *v = fgs_frame_view{};
@@ -939,7 +943,7 @@ int fgs_preprocess(const fgs_camera* cam, const fgs_gaussians* g, const fgs_opti
 int fgs_read_counts(const void* gauss_ws, int P, int width, int height, const fgs_options* opt,
                     int64_t counts_host[2], void* stream) {
   if (!gauss_ws || !counts_host) return fail(FGS_ERR_ARG, "NULL argument");
-  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height, tile_w_of(opt)));
+  const GaussWs w = carve_gauss(const_cast<void*>(gauss_ws), P, tiles_of(width, height, tile_w_of(opt)), true);
   // [0] pairs, [1] bucket-table capacity, up to the depth sort's fault flag
   uint32_t c[DSORT_FAULT + 1];
   hipStream_t s = (hipStream_t)stream;
@@ -988,7 +992,7 @@ int fgs_batch_bucket_tables(void* gauss_ws, void* pair_ws, int P, const fgs_batc
   if (P < 0 || R < 0 || NB < 0) return fail(FGS_ERR_ARG, "bad frame size");
   if (P == 0 || NB == 0) return FGS_OK;
   const int T = b.T;
-  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T);
+  const GaussWs gw = carve_gauss(gauss_ws, (int64_t)P * b.V, T, b.V == 1);
   const PairWs pw = carve_pair(pair_ws, R, NB, b.HW, T, true, keeps_ckpt(opt));
   FGS_TRY(launch_bucket_tables(T, NB, gw.ranges, gw.boff, opt->bucket_size, pw.bucket_ranges,
                                pw.bucket_tile, (hipStream_t)stream),
@@ -1027,7 +1031,7 @@ int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height,
   if (!gauss_ws || !pair_ws || !accum_weights || !accum_count || !accum_max_count)
     return fail(FGS_ERR_ARG, "NULL argument");
   const int T = tiles_of(width, height, tile_w_of(opt));
-  const GaussWs gw = carve_gauss(gauss_ws, P, T);
+  const GaussWs gw = carve_gauss(gauss_ws, P, T, true);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast, keeps_ckpt(opt));
   const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);
@@ -1055,7 +1059,7 @@ int fgs_render_depth(void* gauss_ws, void* pair_ws, int P, int width, int height
   if (P > 0 && (!means3D || !scales || !rotations))
     return fail(FGS_ERR_ARG, "render_depth needs means3D, scales and rotations");
   const int T = tiles_of(width, height, tile_w_of(opt));
-  const GaussWs gw = carve_gauss(gauss_ws, P, T);
+  const GaussWs gw = carve_gauss(gauss_ws, P, T, true);
   const bool fast = opt->schedule == FGS_SCHEDULE_FAST;
   const PairWs pw = carve_pair(pair_ws, R, NB, (int64_t)width * height, T, fast, keeps_ckpt(opt));
   const uint32_t* vals = fast ? pw.v0 : (plain_sorted_in_alt(T) ? pw.v1 : pw.v0);

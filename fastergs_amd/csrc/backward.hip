@@ -520,22 +520,15 @@ __global__ __launch_bounds__(SHB_THREADS) void k_sh_bwd(GaussBwdArgs a) {
       sh_drgb(clampbits, col, dRGB);
       const float len = sqrt_(ox * ox + oy * oy + oz * oz);
       const float x = ox / len, y = oy / len, z = oz / len;
-      float bas[16], db[16][3];
+      float bas[16], J[3][3], ddir[3];
       sh_basis(x, y, z, bas);
-      sh_dbasis(x, y, z, db);
       const int n = sh_count(a.cam.deg);
-      float ddir[3] = {0.0f, 0.0f, 0.0f};
+      sh_jacobian(x, y, z, row, a.cam.deg, J);
+      sh_ddir(J, dRGB, ddir);
 #pragma unroll
-      for (int k = 1; k < 16; ++k) {
-        if (k < n) {
-          const float cdot = row[3 * (k - 1)] * dRGB[0] + row[3 * (k - 1) + 1] * dRGB[1] +
-                             row[3 * (k - 1) + 2] * dRGB[2];
-          ddir[0] += db[k][0] * cdot;
-          ddir[1] += db[k][1] * cdot;
-          ddir[2] += db[k][2] * cdot;
+      for (int k = 1; k < 16; ++k)
+        if (k < n)
           for (int ch = 0; ch < 3; ++ch) row[3 * (k - 1) + ch] = bas[k] * dRGB[ch];
-        }
-      }
       for (int ch = 0; ch < 3; ++ch) ddc[ch] = bas[0] * dRGB[ch];
       for (int j = 3 * (n - 1); j < RM; ++j) row[j] = 0.0f;
       sh_dmean(ox, oy, oz, ddir, dm);
@@ -605,11 +598,125 @@ __global__ __launch_bounds__(256) void k_gauss_bwd_geom(GaussBwdArgs a) {
   ((float4*)a.drot)[i] = make_float4(tr[0], tr[1], tr[2], tr[3]);
 }
 
+// Adjoint of one view when the forward's colour pass stored the SH
+// Jacobians: the geometry adjoint of k_gauss_bwd_geom and the SH part in one
+// pass without reading the coefficients (their gradients bas_k dRGB need
+// only the direction; the mean term takes dL/d(direction) from the
+// Jacobian). One wave64 per workgroup, one Gaussian per lane; each lane
+// writes its coefficient-gradient row into LDS and the rows leave with
+// coalesced float4 stores.
+__global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
+  extern __shared__ float4 rows4[];  // SHB_THREADS x 3M floats (padded)
+  float* rows = (float*)rows4;
+  const int lane = threadIdx.x;
+  const int g0 = blockIdx.x * SHB_THREADS;
+  const int i = g0 + lane;
+  const int M = a.cam.max_coeffs;
+  const int RM = 3 * M;
+  const bool sh = a.dsh != nullptr && M > 0;
+  const bool colour_sh = a.G.colors == nullptr && a.G.dc != nullptr;
+  const int nb = min(SHB_THREADS, a.P - g0);
+  if (i < a.P) {
+    float raw[RAW_STRIDE];
+    load_raw(a.raw_lin, a.raw_con, (size_t)i, raw);
+    Grad2D<float> g2;
+    finalize_raw(raw, a.G.opac[i], a.cam.W, a.cam.H, g2);
+    if (raw[RAW_POW] == 0.0f) g2.opacity = 0.0f;
+    ((float4*)a.dmean2D)[i] = make_float4(g2.mean2d[0], g2.mean2d[1], g2.mean2d[2], g2.mean2d[3]);
+    if (a.dconic)
+      ((float4*)a.dconic)[i] = make_float4(g2.conic[0], g2.conic[1], g2.conic[2], g2.conic[3]);
+    a.dopacity[i] = g2.opacity;
+    if (a.dcolor) {
+      a.dcolor[3 * (size_t)i] = g2.color[0];
+      a.dcolor[3 * (size_t)i + 1] = g2.color[1];
+      a.dcolor[3 * (size_t)i + 2] = g2.color[2];
+    }
+    float* row = rows + RM * lane;
+    float tm[3], tc[6], ts[3], tr[4], ddc[3] = {0.0f, 0.0f, 0.0f};
+    int written = 0;  // leading coefficient-gradient entries of the row
+    if (a.radii[i] > 0) {
+      const int clamp = (int)f2u(a.splat[12 * (size_t)i + 11]);
+      float dms[3] = {0.0f, 0.0f, 0.0f};
+      if (colour_sh) {
+        const float* pm = a.G.means + 3 * (size_t)i;
+        const float ox = pm[0] - a.cam.campos[0], oy = pm[1] - a.cam.campos[1],
+                    oz = pm[2] - a.cam.campos[2];
+        const float len = sqrt_(ox * ox + oy * oy + oz * oz);
+        float bas[16], dRGB[3], J[3][3], ddir[3];
+        sh_basis(ox / len, oy / len, oz / len, bas);
+        sh_drgb(clamp, g2.color, dRGB);
+        const float4* jr = (const float4*)(a.shjac + 12 * (size_t)i);
+        const float4 j0 = jr[0], j1 = jr[1], j2 = jr[2];
+        J[0][0] = j0.x; J[0][1] = j0.y; J[0][2] = j0.z;
+        J[1][0] = j1.x; J[1][1] = j1.y; J[1][2] = j1.z;
+        J[2][0] = j2.x; J[2][1] = j2.y; J[2][2] = j2.z;
+        sh_ddir(J, dRGB, ddir);
+        for (int ch = 0; ch < 3; ++ch) ddc[ch] = bas[0] * dRGB[ch];
+        if (sh) {
+          const int n = sh_count(a.cam.deg);
+#pragma unroll
+          for (int k = 1; k < 16; ++k)
+            if (k < n)
+              for (int ch = 0; ch < 3; ++ch) row[3 * (k - 1) + ch] = bas[k] * dRGB[ch];
+          written = 3 * (n - 1);
+        }
+        sh_dmean(ox, oy, oz, ddir, dms);
+      }
+      GaussGrad<float> out;
+      out.dmean3d = tm;
+      out.dcov3d = tc;
+      out.dscale = ts;
+      out.drot = tr;
+      out.ddc = nullptr;
+      out.dsh = nullptr;
+      gauss_bwd(i, a.G, a.cam, clamp, g2, out, (const float*)nullptr, dms);
+    } else {
+      for (int k = 0; k < 3; ++k) tm[k] = ts[k] = 0.0f;
+      for (int k = 0; k < 6; ++k) tc[k] = 0.0f;
+      for (int k = 0; k < 4; ++k) tr[k] = 0.0f;
+    }
+    if (sh)
+      for (int k = written; k < RM; ++k) row[k] = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+      a.dmean3D[3 * (size_t)i + k] = tm[k];
+      a.dscale[3 * (size_t)i + k] = ts[k];
+      if (a.ddc) a.ddc[3 * (size_t)i + k] = ddc[k];
+    }
+    if (a.dcov3D)
+      for (int k = 0; k < 6; ++k) a.dcov3D[6 * (size_t)i + k] = tc[k];
+    ((float4*)a.drot)[i] = make_float4(tr[0], tr[1], tr[2], tr[3]);
+  }
+  if (!sh) return;
+  __syncthreads();
+  float* dst = a.dsh + (size_t)RM * g0;
+  const int nq = RM * SHB_THREADS / 4;
+  const bool vec = M <= 15 && nb == SHB_THREADS && RM * SHB_THREADS % 4 == 0 &&
+                   ((uintptr_t)a.dsh & 15) == 0;
+  if (vec) {
+    float4 ov[SHB_QMAX];
+#pragma unroll
+    for (int u = 0; u < SHB_QMAX; ++u) ov[u] = rows4[lane + u * SHB_THREADS];
+#pragma unroll
+    for (int u = 0; u < SHB_QMAX; ++u)
+      if (lane + u * SHB_THREADS < nq) ((float4*)dst)[lane + u * SHB_THREADS] = ov[u];
+  } else {
+    for (int j = lane; j < RM * nb; j += SHB_THREADS) dst[j] = rows[j];
+  }
+}
+
 hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
   if (a.P <= 0) return hipSuccess;
   const int M = a.cam.max_coeffs;
   const bool sh = a.dsh != nullptr && M > 0;
   const bool colour_sh = a.G.colors == nullptr && a.G.dc != nullptr;
+  if (a.shjac != nullptr && (a.views == nullptr || a.V <= 1) && M <= 15) {
+    const size_t need = (size_t)SHB_THREADS * 3 * M * sizeof(float);
+    const size_t shmem = sh ? (need > SHB_SHMEM ? need : SHB_SHMEM) : 0;
+    FGS_TIMED("gauss_bwd_jac", s,
+              (k_gauss_bwd_jac<<<(unsigned)((a.P + SHB_THREADS - 1) / SHB_THREADS), SHB_THREADS,
+                                 shmem, s>>>(a)));
+    return hipGetLastError();
+  }
   if (a.sh_dm != nullptr && (a.views == nullptr || a.V <= 1) && M <= 15) {
     if (colour_sh) {
       const size_t need = (size_t)SHB_THREADS * 3 * M * sizeof(float);

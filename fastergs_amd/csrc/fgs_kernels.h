@@ -157,6 +157,8 @@ struct PreArgs {
   uint32_t* counts;
   const ViewRec* views;  // fast: V views (records are V x P)
   int V;
+  float* shjac;        // fast, one view: the colour pass stores each Gaussian's SH Jacobian
+                       // (P x 12 floats: rows J[ch][0..2], 0) for the backward; nullptr: not
 };
 hipError_t launch_preprocess_plain(const PreArgs& a, hipStream_t s);
 // Geometry pass of the FAST schedule; also writes the view table from vs
@@ -349,6 +351,9 @@ struct GaussBwdArgs {
   const ViewRec* views;
   int V;
   float* sh_dm;  // one view: P x 4 scratch for the split SH pass (nullptr: fused kernel)
+  // one view: the SH Jacobians the forward's colour pass stored (P x 12
+  // floats), used instead of the coefficients (nullptr: none)
+  const float* shjac;
 };
 hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s);
 // Densification statistics of V views (radii / dmean2D rows v * P + i) added

@@ -148,6 +148,7 @@ struct GaussWs {
                        // (COUNT_ARRIVE_RANGES); compositing's work-queue
                        // counters from COMPOSITE_QUEUE_OFF
   uint32_t* dsort_sync;  // fast: depth sort state (SORT_SYNC_WORDS), right after counts
+  float* shjac;        // fast, one view: P x 12 SH Jacobians (colour pass -> backward)
   float* feat;         // T x 8 tile features (scheduled compositing)
   float* pred;         // T x 2 cost-model outputs (predicted log2 cost, split logit)
   uint32_t* items;     // 2T compositing work items (tile << 2 | part)
@@ -159,8 +160,9 @@ struct GaussWs {
   size_t total;
 };
 
-// P: records (views x Gaussians), T: tiles of all views.
-inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
+// P: records (views x Gaussians), T: tiles of all views; jac: room for the
+// SH Jacobians (single views).
+inline GaussWs carve_gauss(void* base, int64_t P, int64_t T, bool jac) {
   Carver c(base);
   GaussWs w;
   w.views = c.take<char>(VIEW_TABLE_VIEWS * VIEW_REC_BYTES);
@@ -188,6 +190,7 @@ inline GaussWs carve_gauss(void* base, int64_t P, int64_t T) {
   // the start of a frame (GAUSS_SYNC_WORDS)
   w.counts = c.take<uint32_t>(GAUSS_SYNC_WORDS);
   w.dsort_sync = w.counts + 16;
+  w.shjac = jac ? c.take<float>(12 * p) : nullptr;
   w.feat = c.take<float>(8 * t);
   w.pred = c.take<float>(2 * t);
   w.items = c.take<uint32_t>(2 * t);

@@ -692,31 +692,80 @@ struct GaussGrad {
   R* dsh;      // max_coeffs x 3
 };
 
-// Derivatives of the 16 real SH basis functions w.r.t. the unit direction
-// (x, y, z) (backward.cu:121-245; rows of unused degrees are ignored).
+// Derivative of real SH basis function k (1..15) w.r.t. the unit direction
+// (x, y, z) (backward.cu:121-245); k is a compile-time constant in the
+// unrolled loop of sh_jacobian, so each call folds to its own expressions.
 template <typename R>
-FGS_HD void sh_dbasis(R x, R y, R z, R db[16][3]) {
-  const R xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-  const R t[16][3] = {
-      {R(0), R(0), R(0)},
-      {R(0), -R(SH_C1), R(0)},
-      {R(0), R(0), R(SH_C1)},
-      {-R(SH_C1), R(0), R(0)},
-      {R(SH_C2_0) * y, R(SH_C2_0) * x, R(0)},
-      {R(0), R(SH_C2_1) * z, R(SH_C2_1) * y},
-      {R(-2) * R(SH_C2_2) * x, R(-2) * R(SH_C2_2) * y, R(4) * R(SH_C2_2) * z},
-      {R(SH_C2_3) * z, R(0), R(SH_C2_3) * x},
-      {R(2) * R(SH_C2_4) * x, R(-2) * R(SH_C2_4) * y, R(0)},
-      {R(6) * R(SH_C3_0) * xy, R(SH_C3_0) * (R(3) * xx - R(3) * yy), R(0)},
-      {R(SH_C3_1) * yz, R(SH_C3_1) * xz, R(SH_C3_1) * xy},
-      {R(-2) * R(SH_C3_2) * xy, R(SH_C3_2) * (R(4) * zz - xx - R(3) * yy), R(8) * R(SH_C3_2) * yz},
-      {R(-6) * R(SH_C3_3) * xz, R(-6) * R(SH_C3_3) * yz,
-       R(SH_C3_3) * (R(6) * zz - R(3) * xx - R(3) * yy)},
-      {R(SH_C3_4) * (R(4) * zz - R(3) * xx - yy), R(-2) * R(SH_C3_4) * xy, R(8) * R(SH_C3_4) * xz},
-      {R(2) * R(SH_C3_5) * xz, R(-2) * R(SH_C3_5) * yz, R(SH_C3_5) * (xx - yy)},
-      {R(SH_C3_6) * (R(3) * xx - R(3) * yy), R(-6) * R(SH_C3_6) * xy, R(0)}};
-  for (int k = 0; k < 16; ++k)
-    for (int c = 0; c < 3; ++c) db[k][c] = t[k][c];
+FGS_HD void sh_dbasis_k(int k, R x, R y, R z, R d[3]) {
+  switch (k) {
+    case 1: d[0] = R(0); d[1] = -R(SH_C1); d[2] = R(0); break;
+    case 2: d[0] = R(0); d[1] = R(0); d[2] = R(SH_C1); break;
+    case 3: d[0] = -R(SH_C1); d[1] = R(0); d[2] = R(0); break;
+    case 4: d[0] = R(SH_C2_0) * y; d[1] = R(SH_C2_0) * x; d[2] = R(0); break;
+    case 5: d[0] = R(0); d[1] = R(SH_C2_1) * z; d[2] = R(SH_C2_1) * y; break;
+    case 6:
+      d[0] = R(-2) * R(SH_C2_2) * x; d[1] = R(-2) * R(SH_C2_2) * y; d[2] = R(4) * R(SH_C2_2) * z;
+      break;
+    case 7: d[0] = R(SH_C2_3) * z; d[1] = R(0); d[2] = R(SH_C2_3) * x; break;
+    case 8: d[0] = R(2) * R(SH_C2_4) * x; d[1] = R(-2) * R(SH_C2_4) * y; d[2] = R(0); break;
+    case 9:
+      d[0] = R(6) * R(SH_C3_0) * (x * y); d[1] = R(SH_C3_0) * (R(3) * (x * x) - R(3) * (y * y));
+      d[2] = R(0);
+      break;
+    case 10:
+      d[0] = R(SH_C3_1) * (y * z); d[1] = R(SH_C3_1) * (x * z); d[2] = R(SH_C3_1) * (x * y);
+      break;
+    case 11:
+      d[0] = R(-2) * R(SH_C3_2) * (x * y);
+      d[1] = R(SH_C3_2) * (R(4) * (z * z) - (x * x) - R(3) * (y * y));
+      d[2] = R(8) * R(SH_C3_2) * (y * z);
+      break;
+    case 12:
+      d[0] = R(-6) * R(SH_C3_3) * (x * z); d[1] = R(-6) * R(SH_C3_3) * (y * z);
+      d[2] = R(SH_C3_3) * (R(6) * (z * z) - R(3) * (x * x) - R(3) * (y * y));
+      break;
+    case 13:
+      d[0] = R(SH_C3_4) * (R(4) * (z * z) - R(3) * (x * x) - (y * y));
+      d[1] = R(-2) * R(SH_C3_4) * (x * y); d[2] = R(8) * R(SH_C3_4) * (x * z);
+      break;
+    case 14:
+      d[0] = R(2) * R(SH_C3_5) * (x * z); d[1] = R(-2) * R(SH_C3_5) * (y * z);
+      d[2] = R(SH_C3_5) * ((x * x) - (y * y));
+      break;
+    default:
+      d[0] = R(SH_C3_6) * (R(3) * (x * x) - R(3) * (y * y)); d[1] = R(-6) * R(SH_C3_6) * (x * y);
+      d[2] = R(0);
+      break;
+  }
+}
+
+// Jacobian of the SH colour w.r.t. the unit view direction (x, y, z):
+// J[ch][j] = sum over k = 1 .. n-1 in order of sh_k[ch] d basis_k / d dir_j
+// (the reference forms the same three colour vectors dRGBdx, dRGBdy, dRGBdz,
+// backward.cu:121-245). The forward's colour pass stores it for the
+// backward, which then needs no coefficient reads for the mean gradient.
+template <typename R>
+FGS_HD void sh_jacobian(R x, R y, R z, const R* sh, int deg, R J[3][3]) {
+  const int n = sh_count(deg);
+  for (int ch = 0; ch < 3; ++ch)
+    for (int j = 0; j < 3; ++j) J[ch][j] = R(0);
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int k = 1; k < 16; ++k)
+    if (k < n) {
+      R d[3];
+      sh_dbasis_k(k, x, y, z, d);
+      for (int ch = 0; ch < 3; ++ch) {
+        const R c = sh[3 * (k - 1) + ch];
+        for (int j = 0; j < 3; ++j) J[ch][j] = J[ch][j] + c * d[j];
+      }
+    }
+}
+// dL/d(direction) from the Jacobian: ddir_j = sum_ch dRGB[ch] J[ch][j].
+template <typename R>
+FGS_HD void sh_ddir(const R J[3][3], const R dRGB[3], R ddir[3]) {
+  for (int j = 0; j < 3; ++j) ddir[j] = dRGB[0] * J[0][j] + dRGB[1] * J[1][j] + dRGB[2] * J[2][j];
 }
 
 // Colour gradient seen by the SH coefficients: zero on clamped channels.
@@ -829,22 +878,10 @@ FGS_HD void gauss_bwd(int i, const GaussRef<R>& G, const CamRef<R>& cam, int cla
     R dRGB[3];
     sh_drgb(clamp, g2.color, dRGB);
     const int n = sh_count(cam.deg);
-    R db[16][3];
-    sh_dbasis(x, y, z, db);
     const R* sh = sh_row ? sh_row : G.shs + (size_t)i * cam.max_coeffs * 3;
-    R ddir[3] = {R(0), R(0), R(0)};
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
-    for (int k = 1; k < 16; ++k) {
-      if (k < n) {
-        const R cdot = sh[3 * (k - 1)] * dRGB[0] + sh[3 * (k - 1) + 1] * dRGB[1] +
-                       sh[3 * (k - 1) + 2] * dRGB[2];
-        ddir[0] += db[k][0] * cdot;
-        ddir[1] += db[k][1] * cdot;
-        ddir[2] += db[k][2] * cdot;
-      }
-    }
+    R J[3][3], ddir[3];
+    sh_jacobian(x, y, z, sh, cam.deg, J);
+    sh_ddir(J, dRGB, ddir);
     // SH coefficient gradients (written after sh has been read, so sh and dsh
     // may share storage)
     for (int ch = 0; ch < 3; ++ch) out.ddc[ch] = bas[0] * dRGB[ch];

@@ -313,7 +313,8 @@ constexpr size_t SHC_SHMEM = (size_t)(SHC_DC_F4 + SHC_QMAX * PRE_THREADS) * size
 // 0.971 ms per training step)
 constexpr unsigned SHC_BLOCKS = 256 * 6;
 
-// One group of 64 Gaussians of k_sh_color.
+// One group of 64 Gaussians of k_sh_color; JAC: also the SH Jacobians.
+template <bool JAC>
 __device__ __forceinline__ void sh_color_group(const PreArgs& a, float4* rows4, int grp) {
   float* rows = (float*)rows4;
   const int lane = threadIdx.x;
@@ -362,9 +363,18 @@ __device__ __forceinline__ void sh_color_group(const PreArgs& a, float4* rows4, 
     const float* cp = a.views[v].campos;
     float rgb[3];
     int clamp;
-    sh_color_rows(pm[0] - cp[0], pm[1] - cp[1], pm[2] - cp[2], sdc + 3 * lane,
-                  ssh + 3 * M * lane, a.cam.deg, rgb, clamp);
+    const float ox = pm[0] - cp[0], oy = pm[1] - cp[1], oz = pm[2] - cp[2];
+    sh_color_rows(ox, oy, oz, sdc + 3 * lane, ssh + 3 * M * lane, a.cam.deg, rgb, clamp);
     ((float4*)(a.splat + 12 * ri))[2] = make_float4(rgb[0], rgb[1], rgb[2], u2f((uint32_t)clamp));
+    if (JAC) {  // one view: the Jacobian for the backward
+      const float len = sqrt_(ox * ox + oy * oy + oz * oz);
+      float J[3][3];
+      sh_jacobian(ox / len, oy / len, oz / len, ssh + 3 * M * lane, a.cam.deg, J);
+      float4* jo = (float4*)(a.shjac + 12 * (size_t)i);
+      jo[0] = make_float4(J[0][0], J[0][1], J[0][2], 0.0f);
+      jo[1] = make_float4(J[1][0], J[1][1], J[1][2], 0.0f);
+      jo[2] = make_float4(J[2][0], J[2][1], J[2][2], 0.0f);
+    }
   }
 }
 
@@ -372,12 +382,16 @@ __device__ __forceinline__ void sh_color_group(const PreArgs& a, float4* rows4, 
 // in a grid-stride loop): the pass runs beside the depth sort, whose
 // latency-bound passes need a workgroup slot and 55 KB of LDS on every CU; a
 // full grid of this pass would hold every slot until it ends.
+// JAC (frames of one view that will be differentiated): the Jacobians are
+// stored too (a separate instance: their registers would lower the
+// occupancy of the plain colour pass).
+template <bool JAC>
 __global__ __launch_bounds__(PRE_THREADS) void k_sh_color(PreArgs a) {
   extern __shared__ float4 rows4[];
   const int ngroups = (a.P + PRE_THREADS - 1) / PRE_THREADS;
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     if (grp != (int)blockIdx.x) __syncthreads();  // the previous group's rows are read
-    sh_color_group(a, rows4, grp);
+    sh_color_group<JAC>(a, rows4, grp);
   }
 }
 
@@ -397,7 +411,10 @@ hipError_t launch_sh_color(const PreArgs& a, hipStream_t s) {
   const unsigned blocks = groups < SHC_BLOCKS ? groups : SHC_BLOCKS;
   const size_t rows = (size_t)(4 * SHC_DC_F4 + PRE_THREADS * 3 * a.cam.max_coeffs) * sizeof(float);
   const size_t shmem = rows > SHC_SHMEM ? rows : SHC_SHMEM;
-  FGS_TIMED("sh_color", s, (k_sh_color<<<blocks, PRE_THREADS, shmem, s>>>(a)));
+  if (a.shjac)
+    FGS_TIMED("sh_color", s, (k_sh_color<true><<<blocks, PRE_THREADS, shmem, s>>>(a)));
+  else
+    FGS_TIMED("sh_color", s, (k_sh_color<false><<<blocks, PRE_THREADS, shmem, s>>>(a)));
   return hipGetLastError();
 }
 

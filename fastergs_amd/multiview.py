@@ -197,11 +197,12 @@ class _RasterizeViews(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means3D, dc, shs, colors_precomp, opacities, scales, rotations, frame_args,
                 stats):
-        settings_list, options = frame_args
+        settings_list, options, grad_enabled = frame_args
         want_abs = stats[1] is not None and stats[1].numel() > 0
         frame = BatchFrame(settings_list, means3D, opacities, scales, rotations, dc=dc, shs=shs,
                            colors_precomp=colors_precomp, options=options, abs_grad=want_abs,
-                           forward_only=not any(ctx.needs_input_grad[:7])).forward()
+                           forward_only=not grad_enabled or
+                           not any(ctx.needs_input_grad[:7])).forward()
         ctx.frame = frame
         ctx.shapes = [t.shape if torch.is_tensor(t) else None for t in
                       (means3D, dc, shs, colors_precomp, opacities, scales, rotations)]
@@ -251,7 +252,9 @@ def rasterize_views(settings_list, means3D, opacities, dc=None, shs=None, colors
     opts = options or RasterOptions(schedule=_lib.SCHEDULE_FAST)
     stats = (xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom)
     flat, radii = _RasterizeViews.apply(means3D, dc, shs, colors_precomp, opacities, scales,
-                                        rotations, (list(settings_list), opts), stats)
+                                        rotations,
+                                        (list(settings_list), opts, torch.is_grad_enabled()),
+                                        stats)
     out, o = [], 0
     for st in settings_list:
         H, W = int(st.image_height), int(st.image_width)

@@ -404,16 +404,18 @@ def _tensor_at(addr: int, n: int, dtype, frame: Frame) -> torch.Tensor:
 def rasterize_gaussians(means3D, dc, shs, colors_precomp, opacities, scales, rotations,
                         cov3Ds_precomp, raster_settings, culling, xyz_gradient_accum,
                         xyz_gradient_accum_abs, max_radii, denom):
+    # grad mode as the caller has it (autograd runs forward() with it off)
     return _RasterizeGaussians.apply(means3D, dc, shs, colors_precomp, opacities, scales,
                                      rotations, cov3Ds_precomp, raster_settings, culling,
-                                     xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom)
+                                     xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom,
+                                     torch.is_grad_enabled())
 
 
 class _RasterizeGaussians(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means3D, dc, shs, colors_precomp, opacities, scales, rotations,
                 cov3Ds_precomp, raster_settings, culling, xyz_gradient_accum,
-                xyz_gradient_accum_abs, max_radii, denom):
+                xyz_gradient_accum_abs, max_radii, denom, grad_enabled=True):
         if cov3Ds_precomp is not None and cov3Ds_precomp.numel() > 0:
             raise ValueError("cov3D_precomp is not supported: pass scales and rotations")
         P = int(means3D.shape[0])
@@ -421,8 +423,10 @@ class _RasterizeGaussians(torch.autograd.Function):
                                                              rotations.numel() == 0)):
             raise ValueError("scales and rotations are required")
         want_abs = xyz_gradient_accum_abs is not None and xyz_gradient_accum_abs.numel() > 0
-        # no input asks for a gradient (e.g. under torch.no_grad): nothing to keep for a backward
-        fwd_only = not any(ctx.needs_input_grad[:8])
+        # called under torch.no_grad, or no input asks for a gradient: nothing
+        # to keep for a backward (needs_input_grad reports requires_grad even
+        # under no_grad)
+        fwd_only = not grad_enabled or not any(ctx.needs_input_grad[:8])
         frame = Frame(raster_settings, means3D, opacities, scales, rotations, dc=dc, shs=shs,
                       colors_precomp=colors_precomp, culling=culling, abs_grad=want_abs,
                       forward_only=fwd_only).forward()
@@ -441,7 +445,7 @@ class _RasterizeGaussians(torch.autograd.Function):
         frame: Frame = ctx.frame
         if grad_out_color is None:  # the image took no part in the loss
             ctx.frame = None
-            return (None,) * 14
+            return (None,) * 15
         s = ctx.shapes
         # the conic and cov3D gradients are never returned (cov3D_precomp is
         # rejected), the colour gradient only for precomputed colours
@@ -464,7 +468,7 @@ class _RasterizeGaussians(torch.autograd.Function):
         return (shaped(g["means3D"], s[0]), shaped(g["dc"], s[1]), shaped(g["shs"], s[2]),
                 shaped(g.get("colors"), s[3]), shaped(g["opacities"], s[4]),
                 shaped(g["scales"], s[5]), shaped(g["rotations"], s[6]),
-                None, None, None, None, None, None, None)
+                None, None, None, None, None, None, None, None)
 
 
 class GaussianRasterizer(nn.Module):

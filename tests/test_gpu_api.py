@@ -330,10 +330,29 @@ def test_forward_only_frames(schedule):
     assert fo.pws.numel() < full.pws.numel()
     with pytest.raises(ValueError, match="forward-only"):
         fo.backward(torch.ones_like(fo.out_color))
-    # the drop-in API: under no_grad the forward is forward-only, same image
-    set_default_options(schedule=schedule, bucket_size=32)
-    ras = GaussianRasterizer(settings_for(cam))
-    with torch.no_grad():
-        img, _ = ras(means3D=d["means3D"], opacities=d["opacities"], dc=d["dc"], shs=d["shs"],
-                     scales=d["scales"], rotations=d["rotations"])
-    assert torch.equal(img, full.out_color)
+    # the drop-in API: under no_grad the forward is forward-only even for
+    # inputs that require gradients (autograd reports them as needing one),
+    # with the same image; with grad mode on it is a full frame
+    import fastergs_amd.rasterizer as rz
+    seen = []
+
+    class Spy(rz.Frame):
+        def __init__(self, *a, **k):
+            seen.append(bool(k.get("forward_only", False)))
+            super().__init__(*a, **k)
+
+    orig = rz.Frame
+    rz.Frame = Spy
+    try:
+        set_default_options(schedule=schedule, bucket_size=32)
+        ras = GaussianRasterizer(settings_for(cam))
+        p = _params(scene)
+        with torch.no_grad():
+            img, _ = ras(means3D=p["means3D"], opacities=p["opacities"], dc=p["dc"],
+                         shs=p["shs"], scales=p["scales"], rotations=p["rotations"])
+        img2, _ = ras(means3D=p["means3D"], opacities=p["opacities"], dc=p["dc"], shs=p["shs"],
+                      scales=p["scales"], rotations=p["rotations"])
+    finally:
+        rz.Frame = orig
+    assert seen == [True, False]
+    assert torch.equal(img, full.out_color) and torch.equal(img2.detach(), full.out_color)
