@@ -81,7 +81,7 @@ class FrameView(ctypes.Structure):
 class Grads(ctypes.Structure):
     _fields_ = [(n, vp) for n in (
         "dmean2D", "dconic", "dopacity", "dcolor", "dmean3D", "dcov3D", "ddc", "dsh", "dscale",
-        "drot")]
+        "drot", "xyz_grad_accum", "xyz_grad_accum_abs", "max_radii", "denom")]
 
 
 # name -> (restype, argtypes); the list is the ABI surface checked by tests.

@@ -863,6 +863,10 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   q.V = b.V;
   q.sh_dm = b.V == 1 ? raw_lin + (size_t)PV * RAW_LIN_STRIDE : nullptr;
   q.shjac = b.V == 1 && fast ? gw.shjac : nullptr;
+  q.accum = out->xyz_grad_accum;
+  q.accum_abs = out->xyz_grad_accum_abs;
+  q.max_radii = out->max_radii;
+  q.denom = out->denom;
   FGS_TRY(launch_gauss_bwd(q, s), "preprocess backward");
   return FGS_OK;
 }

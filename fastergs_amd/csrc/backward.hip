@@ -634,19 +634,28 @@ __global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
     float* row = rows + RM * lane;
     float tm[3], tc[6], ts[3], tr[4], ddc[3] = {0.0f, 0.0f, 0.0f};
     int written = 0;  // leading coefficient-gradient entries of the row
-    if (a.radii[i] > 0) {
-      const int clamp = (int)f2u(a.splat[12 * (size_t)i + 11]);
+    const int radius = a.radii[i];
+    if (radius > 0) {
+      // densification statistics (backward.cu:892-918, as k_densify_stats)
+      if (a.denom) a.denom[i] += 1.0f;
+      if (a.max_radii) a.max_radii[i] = fmaxf(a.max_radii[i], (float)radius);
+      if (a.accum)
+        a.accum[i] += sqrtf(g2.mean2d[0] * g2.mean2d[0] + g2.mean2d[1] * g2.mean2d[1]);
+      if (a.accum_abs)
+        a.accum_abs[i] += sqrtf(g2.mean2d[2] * g2.mean2d[2] + g2.mean2d[3] * g2.mean2d[3]);
+      int clamp = 0;
       float dms[3] = {0.0f, 0.0f, 0.0f};
       if (colour_sh) {
         const float* pm = a.G.means + 3 * (size_t)i;
         const float ox = pm[0] - a.cam.campos[0], oy = pm[1] - a.cam.campos[1],
                     oz = pm[2] - a.cam.campos[2];
         const float len = sqrt_(ox * ox + oy * oy + oz * oz);
+        const float4* jr = (const float4*)(a.shjac + 12 * (size_t)i);
+        const float4 j0 = jr[0], j1 = jr[1], j2 = jr[2];
+        clamp = (int)f2u(j0.w);
         float bas[16], dRGB[3], J[3][3], ddir[3];
         sh_basis(ox / len, oy / len, oz / len, bas);
         sh_drgb(clamp, g2.color, dRGB);
-        const float4* jr = (const float4*)(a.shjac + 12 * (size_t)i);
-        const float4 j0 = jr[0], j1 = jr[1], j2 = jr[2];
         J[0][0] = j0.x; J[0][1] = j0.y; J[0][2] = j0.z;
         J[1][0] = j1.x; J[1][1] = j1.y; J[1][2] = j1.z;
         J[2][0] = j2.x; J[2][1] = j2.y; J[2][2] = j2.z;
@@ -704,8 +713,10 @@ __global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
   }
 }
 
-hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
-  if (a.P <= 0) return hipSuccess;
+// The adjoint pass for a's shape; stats_done: it also added the
+// densification statistics.
+static hipError_t launch_gauss_adjoint(const GaussBwdArgs& a, hipStream_t s, bool& stats_done) {
+  stats_done = false;
   const int M = a.cam.max_coeffs;
   const bool sh = a.dsh != nullptr && M > 0;
   const bool colour_sh = a.G.colors == nullptr && a.G.dc != nullptr;
@@ -715,6 +726,7 @@ hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
     FGS_TIMED("gauss_bwd_jac", s,
               (k_gauss_bwd_jac<<<(unsigned)((a.P + SHB_THREADS - 1) / SHB_THREADS), SHB_THREADS,
                                  shmem, s>>>(a)));
+    stats_done = true;
     return hipGetLastError();
   }
   if (a.sh_dm != nullptr && (a.views == nullptr || a.V <= 1) && M <= 15) {
@@ -743,6 +755,16 @@ hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
   FGS_TIMED("gauss_bwd", s,
             (k_gauss_bwd<<<(a.P + GBWD_THREADS - 1) / GBWD_THREADS, GBWD_THREADS, shmem, s>>>(a)));
   return hipGetLastError();
+}
+
+hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s) {
+  if (a.P <= 0) return hipSuccess;
+  bool stats_done = false;
+  hipError_t e = launch_gauss_adjoint(a, s, stats_done);
+  if (e == hipSuccess && !stats_done && (a.accum || a.accum_abs || a.max_radii || a.denom))
+    e = launch_densify_stats(a.P, a.views != nullptr && a.V > 1 ? a.V : 1, a.radii, a.dmean2D,
+                             a.accum, a.accum_abs, a.max_radii, a.denom, s);
+  return e;
 }
 
 __global__ __launch_bounds__(256) void k_densify_stats(int P, int V,

@@ -352,8 +352,11 @@ struct GaussBwdArgs {
   int V;
   float* sh_dm;  // one view: P x 4 scratch for the split SH pass (nullptr: fused kernel)
   // one view: the SH Jacobians the forward's colour pass stored (P x 12
-  // floats), used instead of the coefficients (nullptr: none)
+  // floats: rows J[ch][0..2] and, in the first row's fourth word, the
+  // colour clamp bits), used instead of the coefficients (nullptr: none)
   const float* shjac;
+  // optional densification statistics added by the adjoint pass (one view)
+  float *accum, *accum_abs, *max_radii, *denom;
 };
 hipError_t launch_gauss_bwd(const GaussBwdArgs& a, hipStream_t s);
 // Densification statistics of V views (radii / dmean2D rows v * P + i) added

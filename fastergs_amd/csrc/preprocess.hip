@@ -371,7 +371,7 @@ __device__ __forceinline__ void sh_color_group(const PreArgs& a, float4* rows4, 
       float J[3][3];
       sh_jacobian(ox / len, oy / len, oz / len, ssh + 3 * M * lane, a.cam.deg, J);
       float4* jo = (float4*)(a.shjac + 12 * (size_t)i);
-      jo[0] = make_float4(J[0][0], J[0][1], J[0][2], 0.0f);
+      jo[0] = make_float4(J[0][0], J[0][1], J[0][2], u2f((uint32_t)clamp));
       jo[1] = make_float4(J[1][0], J[1][1], J[1][2], 0.0f);
       jo[2] = make_float4(J[2][0], J[2][1], J[2][2], 0.0f);
     }

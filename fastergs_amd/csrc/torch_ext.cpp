@@ -207,6 +207,8 @@ py::tuple forward(const at::Tensor& means3D, const at::Tensor& opacities, const 
 
 // Returns (dmean2D, dconic, dopacity, dcolor, dmean3D, dcov3D, ddc, dsh,
 // dscale, drot, grad_ws); dconic, dcolor, dcov3D are undefined unless wanted.
+// stats: (xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom),
+// each updated in place when defined and non-empty (else left alone).
 py::tuple backward(const at::Tensor& means3D, const at::Tensor& opacities, const at::Tensor& scales,
                    const at::Tensor& rotations, const at::Tensor& dc, const at::Tensor& shs,
                    const at::Tensor& colors, const at::Tensor& culling, const at::Tensor& view,
@@ -217,7 +219,7 @@ py::tuple backward(const at::Tensor& means3D, const at::Tensor& opacities, const
                    const at::Tensor& pws, int64_t R_cap, int64_t NB_cap, const at::Tensor& radii,
                    const at::Tensor& tfinal, const at::Tensor& contrib,
                    const at::Tensor& grad_out, bool want_conic, bool want_colors,
-                   bool want_cov3D) {
+                   bool want_cov3D, const std::vector<at::Tensor>& stats) {
   Frame f = make_frame(means3D, opacities, scales, rotations, dc, shs, colors, culling, view, proj,
                        campos, bg, width, height, tanfovx, tanfovy, scale_modifier, mult,
                        sh_degree, opts, mlp);
@@ -233,10 +235,22 @@ py::tuple backward(const at::Tensor& means3D, const at::Tensor& opacities, const
   at::Tensor dcolor = want_colors ? e({P, 3}) : at::Tensor();
   at::Tensor dcov = want_cov3D ? e({P, 6}) : at::Tensor();
   at::Tensor grad_ws = at::empty({(int64_t)fgs_grad_ws_size((int)P)}, bytes(f.dev));
+  if (stats.size() != 4) throw std::invalid_argument("stats must have 4 entries");
+  float* st[4];
+  for (int k = 0; k < 4; ++k) {
+    const at::Tensor& t = stats[k];
+    st[k] = nullptr;
+    if (!t.defined() || t.numel() == 0) continue;
+    if (!t.is_cuda() || t.scalar_type() != at::kFloat || !t.is_contiguous() || t.numel() != P)
+      throw std::invalid_argument("densification statistics must be contiguous float32 GPU "
+                                  "tensors of P entries");
+    st[k] = t.data_ptr<float>();
+  }
   const fgs_grads out{(float*)ptr(dmean2D), (float*)ptr(dconic), (float*)ptr(dopac),
                       (float*)ptr(dcolor),  (float*)ptr(dmean3D), (float*)ptr(dcov),
                       (float*)ptr(ddc),     (float*)ptr(dsh),     (float*)ptr(dscale),
-                      (float*)ptr(drot)};
+                      (float*)ptr(drot),    st[0],                st[1],
+                      st[2],                st[3]};
   check(fgs_backward(&f.cam, &f.g, &f.opt, ptr(gws), ptr(pws), R_cap, NB_cap,
                      (const int32_t*)ptr(radii), (const float*)ptr(tfinal),
                      (const int32_t*)ptr(contrib), g.data_ptr<float>(), grad_ws.data_ptr(), &out,

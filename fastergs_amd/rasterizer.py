@@ -278,16 +278,20 @@ class Frame:
         return self
 
     # -- backward -----------------------------------------------------------
-    def backward(self, grad_out_color: torch.Tensor, skip=()) -> dict:
+    def backward(self, grad_out_color: torch.Tensor, skip=(), stats=None) -> dict:
         """Gradients of the frame; `skip` may name "conic", "colors" and
-        "cov3D", which are then neither computed into memory nor returned."""
+        "cov3D", which are then neither computed into memory nor returned.
+        `stats`: (xyz_gradient_accum, xyz_gradient_accum_abs, max_radii,
+        denom) (entries may be None) updated in place as densify_stats
+        would, in the same pass."""
         if not set(skip) <= {"conic", "colors", "cov3D"}:
             raise ValueError("only conic, colors and cov3D gradients can be skipped")
         if self.forward_only:
             raise ValueError("the frame was rendered forward-only")
         r = _C.ops.backward(*self._ext_args(), self.gws, self.pws, self.R_cap, self.NB_cap,
                             self.radii, self.T_final, self.last_contributor, grad_out_color,
-                            "conic" not in skip, "colors" not in skip, "cov3D" not in skip)
+                            "conic" not in skip, "colors" not in skip, "cov3D" not in skip,
+                            [_NONE if t is None else t for t in (stats or (None,) * 4)])
         names = ("mean2D", "conic", "opacities", "colors", "means3D", "cov3D", "dc", "shs",
                  "scales", "rotations")
         out = {k: t for k, t in zip(names, r[:10]) if k not in skip}
@@ -452,10 +456,11 @@ class _RasterizeGaussians(torch.autograd.Function):
         skip = ["conic", "cov3D"]
         if s[3] is None or s[3].numel() == 0:
             skip.append("colors")
-        g = frame.backward(grad_out_color, skip=skip)
         accum, accum_abs, max_radii, denom = ctx.stats
-        if accum is not None and accum.numel() != 0:  # pybind.cpp:324
-            frame.densify_stats(g["mean2D"], accum, accum_abs, max_radii, denom)
+        # the densification statistics (pybind.cpp:324) in the adjoint's pass
+        stats = (accum, accum_abs, max_radii, denom) if accum is not None and \
+            accum.numel() != 0 else None
+        g = frame.backward(grad_out_color, skip=skip, stats=stats)
         ctx.frame = None
 
         def shaped(t, shape):

@@ -205,6 +205,12 @@ int fgs_render(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options*
  * colours are precomputed). grad_ws is fgs_grad_ws_size(P) bytes. */
 typedef struct fgs_grads {
   float *dmean2D, *dconic, *dopacity, *dcolor, *dmean3D, *dcov3D, *ddc, *dsh, *dscale, *drot;
+  /* Optional (NULL: left alone): the frame's densification statistics are
+     added to these P-sized arrays in the same call, exactly as a following
+     fgs_densify_stats(P, radii, dmean2D, ...) would add them (one view: in
+     the per-Gaussian adjoint pass itself, which then needs no second read
+     of dmean2D and radii). */
+  float *xyz_grad_accum, *xyz_grad_accum_abs, *max_radii, *denom;
 } fgs_grads;
 int fgs_backward(const fgs_camera* cam, const fgs_gaussians* g, const fgs_options* opt,
                  void* gauss_ws, void* pair_ws, int64_t R, int64_t NB, const int32_t* radii,

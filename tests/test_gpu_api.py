@@ -305,6 +305,19 @@ def test_densify_stats_closed_form(schedule):
                                atol=1e-12)
     np.testing.assert_array_equal(mr.cpu().numpy(), np.where(vis, np.maximum(i2, r.astype(np.float32)), i2))
     np.testing.assert_array_equal(den.cpu().numpy(), i3 + vis.astype(np.float32))
+    # the same statistics added by the backward itself (in one pass with the
+    # adjoint on FAST frames): bit for bit what fgs_densify_stats adds from
+    # that backward's mean2D gradients
+    fr2 = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+                dc=d["dc"], shs=d["shs"],
+                options=RasterOptions(schedule=schedule, bucket_size=32)).forward()
+    st = [t.clone() for t in init]
+    g2 = fr2.backward(torch.from_numpy(scenes.target_image(320, 240, seed=5)).cuda(), stats=st)
+    sep = [t.clone() for t in init]
+    fr2.densify_stats(g2["mean2D"], *sep)
+    torch.cuda.synchronize()
+    for a_, b_ in zip(st, sep):
+        np.testing.assert_array_equal(a_.cpu().numpy(), b_.cpu().numpy())
 
 
 @pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "fast"])
