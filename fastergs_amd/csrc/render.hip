@@ -103,24 +103,42 @@ hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, cons
   return hipGetLastError();
 }
 
-// Ranges from the sorted tile ids (16- or 32-bit) by binary search, one
-// thread per tile: ranges[t] = [lower_bound(t), lower_bound(t + 1)).
+// Ranges from the sorted tile ids (16- or 32-bit):
+// ranges[t] = [lower_bound(t), lower_bound(t + 1)).
+// Lower bound of v in the sorted a[0, n) by a group of 8 lanes (lane j of
+// the group = j, group bit offset g8 in the wave's ballots): each step the 8
+// lanes probe the 8 points that cut [lo, hi) into 9 parts and the interval
+// shrinks to the part where "a[i] < v" turns false, so 21M ids take 8
+// dependent loads instead of the 25 of a binary search. Invariant: a[i] < v
+// for i < lo and a[i] >= v for i >= hi.
 template <typename TK>
-__device__ __forceinline__ uint32_t lower_bound_tile(const TK* a, uint32_t n, uint32_t v) {
+__device__ __forceinline__ uint32_t lower_bound_group8(const TK* a, uint32_t n, uint32_t v,
+                                                       int j, int g8) {
   uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if ((uint32_t)a[mid] < v) lo = mid + 1; else hi = mid;
+  while (__ballot(hi - lo > 8u)) {
+    const uint32_t len = hi - lo;
+    const bool open = len > 8u;
+    const uint32_t q = lo + (uint32_t)(((uint64_t)(j + 1) * len) / 9u);
+    const bool pred = open && (uint32_t)a[q] < v;
+    const uint32_t m = (uint32_t)__popcll((__ballot(pred) >> g8) & 0xffull);
+    if (open) {
+      const uint32_t nlo = m > 0 ? lo + (uint32_t)(((uint64_t)m * len) / 9u) + 1u : lo;
+      const uint32_t nhi = m < 8 ? lo + (uint32_t)(((uint64_t)(m + 1) * len) / 9u) : hi;
+      lo = nlo;
+      hi = nhi;
+    }
   }
-  return lo;
+  const uint32_t pos = lo + (uint32_t)j;
+  const bool pred = pos < hi && (uint32_t)a[pos] < v;
+  return lo + (uint32_t)__popcll((__ballot(pred) >> g8) & 0xffull);
 }
 // Also writes each tile's bucket count ceil(len / bucket) and clears its
 // furthest-contributor slot, which compositing combines with atomicMax when
 // a tile is split into row bands; the last workgroup to finish scans the
-// bucket counts into boff (T + 1 entries, total at [T]). Workgroup b looks
-// up the lower bounds of tiles 255 b .. 255 b + 255 (one search per
-// thread) and writes the ranges of the first 255 of them.
-constexpr int RANGES_PER_BLOCK = 255;
+// bucket counts into boff (T + 1 entries, total at [T]). Workgroup b's 32
+// groups of 8 lanes look up the lower bounds of tiles 31 b .. 31 b + 31 and
+// it writes the ranges of the first 31 of them.
+constexpr int RANGES_PER_BLOCK = 31;
 template <typename TK>
 __global__ __launch_bounds__(256) void k_tile_ranges(int64_t R_cap,
                                                      const uint32_t* __restrict__ n_dev,
@@ -130,18 +148,25 @@ __global__ __launch_bounds__(256) void k_tile_ranges(int64_t R_cap,
                                                      uint32_t* __restrict__ maxc,
                                                      uint32_t* __restrict__ boff,
                                                      uint32_t* __restrict__ arrive) {
-  __shared__ uint32_t s_lo[256];
-  const int t = blockIdx.x * RANGES_PER_BLOCK + threadIdx.x;
+  __shared__ uint32_t s_lo[32];
+  const int grp = threadIdx.x >> 3, j = threadIdx.x & 7, g8 = (threadIdx.x & 56);
+  const int t = blockIdx.x * RANGES_PER_BLOCK + grp;
   const uint32_t R = (uint32_t)dev_count(n_dev, R_cap);
-  if (t <= T) s_lo[threadIdx.x] = lower_bound_tile(tiles, R, (uint32_t)t);
+  // every lane of the wave takes part in the ballots; groups past T search
+  // for T as well (their result is not used)
+  const uint32_t lb = lower_bound_group8(tiles, R, (uint32_t)min(t, T), j, g8);
+  if (j == 0) s_lo[grp] = lb;
   __syncthreads();
-  if (threadIdx.x < RANGES_PER_BLOCK && t < T) {
-    const uint32_t lo = s_lo[threadIdx.x], hi = s_lo[threadIdx.x + 1];
-    // empty tiles keep [0, 0) like identifyTileRanges (render.cu:406-411)
-    ranges[2 * t] = lo < hi ? lo : 0u;
-    ranges[2 * t + 1] = lo < hi ? hi : 0u;
-    publish_u32(&bcount[t], (hi - lo + bucket - 1) / bucket);
-    maxc[t] = 0u;
+  if (threadIdx.x < RANGES_PER_BLOCK) {
+    const int tt = blockIdx.x * RANGES_PER_BLOCK + threadIdx.x;
+    if (tt < T) {
+      const uint32_t lo = s_lo[threadIdx.x], hi = s_lo[threadIdx.x + 1];
+      // empty tiles keep [0, 0) like identifyTileRanges (render.cu:406-411)
+      ranges[2 * tt] = lo < hi ? lo : 0u;
+      ranges[2 * tt + 1] = lo < hi ? hi : 0u;
+      publish_u32(&bcount[tt], (hi - lo + bucket - 1) / bucket);
+      maxc[tt] = 0u;
+    }
   }
   if (last_workgroup(arrive))
     block_scan_into<256>(T, [&](int i) { return read_published_u32(&bcount[i]); }, boff);
