@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 COMPOSITE_FLOPS = 21  # per (pixel, splat) evaluation
+# untimed steps before a frame shape's first timed run, over which the
+# prefix-list divisor of the drop-in API settles (it doubles after every 4
+# frames whose prefixes sufficed)
+PREFIX_SETTLE_STEPS = 40
 
 
 def parse():
@@ -35,6 +39,8 @@ def parse():
     ap.add_argument("--workload", default="heavy")
     ap.add_argument("--schedule", default="fast", choices=["plain", "fast"])
     ap.add_argument("--bucket", type=int, default=32)
+    ap.add_argument("--list-prefix-div", type=int, default=16,
+                    help="build the lists of the nearest P / div Gaussians first (0: whole lists)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--compare-plain", type=int, default=1)
     ap.add_argument("--small", type=int, default=1,
@@ -67,7 +73,8 @@ def main():
     from fastergs_amd import scenes
     from fastergs_amd._lib import SCHEDULE_FAST, SCHEDULE_PLAIN
     from fastergs_amd.rasterizer import (GaussianRasterizationSettings, GaussianRasterizer,
-                                         RasterOptions, set_default_options)
+                                         RasterOptions, get_default_options,
+                                         set_default_options)
 
     dev = torch.device("cuda:0")
     scene, cam = scenes.workload(args.workload, seed=0)
@@ -109,11 +116,25 @@ def main():
         print("traffic probe done", flush=True)
         return
 
+    settled = set()
+
+    def settle_steps(schedule):
+        o = get_default_options()
+        key = (schedule, o.bucket_size, o.tile_w, o.list_prefix_div)
+        if schedule != SCHEDULE_FAST or o.list_prefix_div < 2 or key in settled:
+            return 0
+        settled.add(key)
+        return PREFIX_SETTLE_STEPS
+
     def timed(schedule: int, steps: int, warmup: int, fwd_only=False, bucket=None, tile_w=16,
-              queue=False):
+              queue=False, prefix_div=None):
         set_default_options(schedule=schedule, bucket_size=args.bucket if bucket is None else bucket,
-                            tile_w=tile_w, composite_queue=queue)
-        for _ in range(warmup):
+                            tile_w=tile_w, composite_queue=queue,
+                            list_prefix_div=args.list_prefix_div if prefix_div is None
+                            else prefix_div)
+        # untimed: the warm-up steps, and before them the steps over which the
+        # prefix-list policy of this frame shape settles (once per shape)
+        for _ in range(warmup + settle_steps(schedule)):
             step()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -134,6 +155,15 @@ def main():
 
     sched = SCHEDULE_FAST if args.schedule == "fast" else SCHEDULE_PLAIN
     ms, wall = timed(sched, args.steps, args.warmup)
+    prefix = None
+    if sched == SCHEDULE_FAST and args.list_prefix_div > 1:
+        from fastergs_amd import rasterizer as rz
+        key = (P, cam.width, cam.height, sched, args.bucket, dev, 16, args.list_prefix_div)
+        pol = rz._prefix_policies.get(key)
+        if pol is not None:
+            prefix = {"start_div": args.list_prefix_div, "div": pol.good,
+                      "records": -(-P // pol.good) if pol.good > 1 else P,
+                      "settle_steps": PREFIX_SETTLE_STEPS}
     fwd_ms, _ = timed(sched, args.steps, 2, fwd_only=True)
     stages, roofline = kernel_roofline(step, params, settings, sched, args, P)
     plain_ms = None
@@ -153,7 +183,10 @@ def main():
         variants = {f"tile_{tw}x{256 // tw}": round(timed(sched, args.steps, 3, tile_w=tw)[0], 4)
                     for tw in (8, 16, 32)}
         variants["composite_queue"] = round(timed(sched, args.steps, 3, queue=True)[0], 4)
-    set_default_options(schedule=sched, bucket_size=args.bucket, tile_w=16, composite_queue=False)
+        # whole lists from the start (no prefix lists of the nearest Gaussians)
+        variants["whole_lists"] = round(timed(sched, args.steps, 3, prefix_div=0)[0], 4)
+    set_default_options(schedule=sched, bucket_size=args.bucket, tile_w=16, composite_queue=False,
+                        list_prefix_div=args.list_prefix_div)
 
     graph_fps = graph_render_fps(settings, params, sched, args)
     step_graph_ms = None
@@ -204,6 +237,7 @@ def main():
         "plain_ms_per_step": None if plain_ms is None else round(plain_ms, 4),
         "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms, 3),
         "dropin_default": dropin,
+        "list_prefix": prefix,
         "variants_ms_per_step": variants,
         "small_config": small,
         "roofline": roofline,

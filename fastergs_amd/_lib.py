@@ -56,7 +56,8 @@ class Options(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int), ("bucket_size", ctypes.c_int),
                 ("split_tiles", ctypes.c_int), ("abs_grad", ctypes.c_int),
                 ("tile_sched", ctypes.POINTER(TileSched)), ("forward_only", ctypes.c_int),
-                ("composite_queue", ctypes.c_int), ("tile_w", ctypes.c_int)]
+                ("composite_queue", ctypes.c_int), ("tile_w", ctypes.c_int),
+                ("list_prefix", ctypes.c_int), ("list_status", vp)]
 
 
 class AdamTensor(ctypes.Structure):

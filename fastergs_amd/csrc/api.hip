@@ -135,6 +135,7 @@ int check_opts(const fgs_options* o) {
     return fail(FGS_ERR_ARG, "split_tiles must be in [1, 2^24]");
   if (o->tile_w != 0 && !tile_w_ok(o->tile_w))
     return fail(FGS_ERR_ARG, "tile_w must be 8, 16 or 32 (0: 16)");
+  if (o->list_prefix < 0) return fail(FGS_ERR_ARG, "list_prefix must be >= 0");
   if (const fgs_tile_sched* ts = o->tile_sched) {
     if (ts->mode < FGS_TILES_RASTER || ts->mode > FGS_TILES_LEARNED)
       return fail(FGS_ERR_ARG, "unknown tile schedule mode");
@@ -666,43 +667,76 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   c.queue = fast && opt->composite_queue ? gw.counts + COMPOSITE_QUEUE_OFF : nullptr;
   c.views = (const ViewRec*)gw.views;
   c.V = b.V;
+  c.redo = nullptr;
+  c.status = nullptr;
+  c.gate = nullptr;
   if (fast) {
     const ViewRec* views = (const ViewRec*)gw.views;
-    // emission in depth order, tile-only stable sort, lists, ranges, buckets.
-    // R is the pair capacity; the pair count itself (counts[0]) stays on
-    // the device, so nothing here waits for the host.
-    // small and large rects write disjoint pair blocks: side by side
+    const int64_t NR = (int64_t)g->P * b.V;
+    // prefix lists: the pairs of the list_prefix nearest records first
+    const bool prefix = b.V == 1 && !opt->tile_sched && opt->list_prefix > 0 &&
+                        (int64_t)opt->list_prefix < NR;
     const Fork f(s);
-    FGS_TRY(f.fork(0, 4), "fork");
-    FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.dsort_k0, gw.poff,
-                              (uint32_t)R,
-                              pw.t0, pw.r0, pw.tile16, f.side(0)),
-            "emit");
-    FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts, nullptr,
-                               gw.poff, gw.lrank, gw.lband, (uint32_t)R, pw.t0, pw.r0,
-                               pw.tile16, s),
-            "emit large");
-    FGS_TRY(f.join(0, 5), "join");
-    // stable sort of (tile id, record) by tile id: the last pass writes the
-    // per-tile lists; ranges by binary search over the sorted tile ids
-    {
-      const int tb = tile_bits(T - 1) > 0 ? tile_bits(T - 1) : 1;
+    // emission in depth order, tile-only stable sort, lists, ranges, buckets.
+    // R is the pair capacity; the pair count itself (counts[0], or the
+    // prefix's counts[COUNT_PREFIX_N]) stays on the device, so nothing here
+    // waits for the host. redo: the second, whole-list build of a prefix
+    // frame, whose launches return at once unless compositing asked for it.
+    const int tb = tile_bits(T - 1) > 0 ? tile_bits(T - 1) : 1;
+    auto build_lists = [&](int mode) -> int {
+      const bool redo = mode == EMIT_REDO;
+      const uint32_t* gate = redo ? gw.counts + COUNT_REDO : nullptr;
+      const uint32_t* n_sort = mode == EMIT_PREFIX ? gw.counts + COUNT_PREFIX_N
+                               : redo              ? gw.counts + COUNT_REDO_N
+                                                   : gw.counts;
+      const int64_t limit = mode == EMIT_PREFIX ? (int64_t)opt->list_prefix : NR;
+      if (mode == EMIT_WHOLE) {
+        // small and large rects write disjoint pair blocks: side by side
+        FGS_TRY(f.fork(0, 4), "fork");
+        FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.dsort_k0,
+                                  gw.poff, (uint32_t)R, pw.t0, pw.r0, pw.tile16, f.side(0), NR,
+                                  gw.counts, EMIT_WHOLE),
+                "emit");
+        FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts,
+                                   nullptr, gw.poff, gw.lrank, gw.lband, (uint32_t)R, pw.t0,
+                                   pw.r0, pw.tile16, s),
+                "emit large");
+        FGS_TRY(f.join(0, 5), "join");
+      } else {
+        // a prefix is short and the redo rarely runs: one stream (a
+        // cross-stream wait costs more than the overlap gains)
+        FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.dsort_k0,
+                                  gw.poff, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s, limit,
+                                  gw.counts, (EmitMode)mode),
+                "emit");
+        FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts,
+                                   nullptr, gw.poff, gw.lrank, gw.lband, (uint32_t)R, pw.t0,
+                                   pw.r0, pw.tile16, s,
+                                   redo ? 0xffffffffu : (uint32_t)limit, gate),
+                "emit large");
+      }
+      // stable sort of (tile id, record) by tile id: the last pass writes the
+      // per-tile lists; ranges by binary search over the sorted tile ids
       bool alt = false;
+      const SortKind kind = redo ? SORT_TILES_REDO : prefix ? SORT_TILES_PREFIX : SORT_TILES;
+      uint32_t* sync = redo ? gw.counts + REDO_SYNC_OFF : nullptr;
       if (pw.tile16)
         FGS_TRY(sort_kv<uint16_t>((uint16_t*)pw.t0, pw.r0, (uint16_t*)pw.t1, pw.r1, R, 0, tb,
-                                  pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES, nullptr,
+                                  pw.scratch, s, alt, n_sort, pw.v0, kind, sync,
                                   gw.counts + COUNT_LIST_FAULT),
                 "tile sort");
       else
         FGS_TRY(sort_kv<uint32_t>((uint32_t*)pw.t0, pw.r0, (uint32_t*)pw.t1, pw.r1, R, 0, tb,
-                                  pw.scratch, s, alt, gw.counts, pw.v0, SORT_TILES, nullptr,
+                                  pw.scratch, s, alt, n_sort, pw.v0, kind, sync,
                                   gw.counts + COUNT_LIST_FAULT),
                 "tile sort");
-      FGS_TRY(launch_tile_ranges(R, gw.counts, alt ? pw.t1 : pw.t0, pw.tile16, gw.ranges, T,
+      FGS_TRY(launch_tile_ranges(R, n_sort, alt ? pw.t1 : pw.t0, pw.tile16, gw.ranges, T,
                                  opt->bucket_size, gw.bcount, gw.maxc, gw.boff,
-                                 gw.counts + COUNT_ARRIVE_RANGES, s),
+                                 gw.counts + COUNT_ARRIVE_RANGES, s, gate),
               "ranges and bucket offsets");
-    }
+      return FGS_OK;
+    };
+    if ((rc = build_lists(prefix ? EMIT_PREFIX : EMIT_WHOLE))) return rc;
     // the bucket -> (tile, list range) tables are not built here: every pass
     // indexes buckets through boff; fgs_batch_bucket_tables builds them on
     // demand
@@ -744,8 +778,21 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
       }
       if (ts->log_cycles) c.cycles = gw.cycles;
     }
+    if (prefix) {
+      c.redo = gw.counts + COUNT_REDO;
+      c.status = opt->list_status;
+    }
     FGS_TRY(launch_composite_fast(c, s), "composite");
     if (feat_side) FGS_TRY(f.join(0, 9), "join");
+    if (prefix) {
+      // the whole lists and a second compositing of every tile, when some
+      // pixel needed more than its prefix list (gated on the device)
+      if ((rc = build_lists(EMIT_REDO))) return rc;
+      c.redo = nullptr;
+      c.status = nullptr;
+      c.gate = gw.counts + COUNT_REDO;
+      FGS_TRY(launch_composite_fast(c, s), "composite again");
+    }
     return FGS_OK;
   }
 

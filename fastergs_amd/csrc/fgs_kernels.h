@@ -122,7 +122,13 @@ hipError_t expand_flags(const uint32_t* flag, int64_t n, uint32_t* ids, uint32_t
 // order); a wait that does sets a fault word to 1: fault_word when given
 // (the caller clears it), else the last word of the sort's own state
 // (sort_fault_word).
-enum SortKind { SORT_PAIRS = 0, SORT_DEPTH = 1, SORT_TILES = 2 };
+enum SortKind {
+  SORT_PAIRS = 0,
+  SORT_DEPTH = 1,
+  SORT_TILES = 2,
+  SORT_TILES_REDO = 3,  // the redo build of prefix-list frames
+  SORT_TILES_PREFIX = 4  // the prefix lists
+};
 template <typename K>
 hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
                    int begin_bit, int end_bit, void* ws, hipStream_t s, bool& in_alt,
@@ -181,13 +187,23 @@ hipError_t launch_duplicate_plain(int P, const float* geom, const uint32_t* dkey
 // poff: pair offsets by depth rank; emission reads order (the sorted depth
 // keys, record in the low 32 bits) for the small rects and lrank (the rank
 // of each large record) for the large ones.
+// Emission of the records of depth rank < rank_limit only (prefix lists);
+// gate: when given, the launch does nothing unless gate[0] != 0.
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
                               const uint32_t* poff, const uint32_t* lrank, uint32_t* lband,
-                              uint32_t cap, void* tkeys, uint32_t* recs, bool t16, hipStream_t s);
+                              uint32_t cap, void* tkeys, uint32_t* recs, bool t16, hipStream_t s,
+                              uint32_t rank_limit = 0xffffffffu, const uint32_t* gate = nullptr);
+// Small-rect emission of the ranks [0, limit) (limit <= P * V). mode:
+// EMIT_WHOLE (every rank; sets counts[COUNT_PREFIX_N], [COUNT_REDO],
+// [COUNT_REDO_N] to 0), EMIT_PREFIX (limit < P * V: counts[COUNT_PREFIX_N] =
+// poff[limit], the others 0) or EMIT_REDO (the launch does nothing unless
+// counts[COUNT_REDO] is set; writes no counts).
+enum EmitMode { EMIT_WHOLE = 0, EMIT_PREFIX = 1, EMIT_REDO = 2 };
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
                              const uint64_t* order, const uint32_t* poff, uint32_t cap,
-                             void* tkeys, uint32_t* recs, bool t16, hipStream_t s);
+                             void* tkeys, uint32_t* recs, bool t16, hipStream_t s,
+                             int64_t limit, uint32_t* counts, EmitMode mode);
 
 // render.hip
 // Range kernels take the pair count as (capacity R, device count n_dev).
@@ -200,9 +216,11 @@ hipError_t launch_bucket_tables(int T, int64_t cap, const uint32_t* ranges, cons
 // furthest-contributor slots (the FAST schedule's bucket_counts).
 // Ranges, bucket counts and (scanned by the last workgroup) the bucket
 // offsets boff; arrive: a zero counter word the launch leaves at zero.
+// gate: when given, the launch does nothing unless gate[0] != 0.
 hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tiles, bool t16,
                               uint32_t* ranges, int T, int bucket, uint32_t* bcount,
-                              uint32_t* maxc, uint32_t* boff, uint32_t* arrive, hipStream_t s);
+                              uint32_t* maxc, uint32_t* boff, uint32_t* arrive, hipStream_t s,
+                              const uint32_t* gate = nullptr);
 hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
                                 hipStream_t s);
 struct CompArgs {
@@ -239,6 +257,13 @@ struct CompArgs {
   uint32_t* queue;
   const ViewRec* views;  // fast: tiles / pixels of V views (W, H, grid_x, bg per view)
   int V;
+  // fast, prefix lists: redo = counts + COUNT_REDO; a workgroup with a pixel
+  // that its list leaves open (not stopped at the end of a whole bucket)
+  // sets redo[0] = 1 and redo[1] = counts[0], and status[0] = 1 when given
+  uint32_t* redo;
+  uint32_t* status;
+  // fast: when given, the launch does nothing unless gate[0] != 0
+  const uint32_t* gate;
 };
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);

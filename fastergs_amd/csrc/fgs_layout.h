@@ -114,12 +114,23 @@ constexpr int LARGE_BAND_TILES = 512;
 constexpr int COMPOSITE_QUEUE_GROUPS = 8;
 constexpr int COMPOSITE_QUEUE_STRIDE = 64;
 constexpr int COMPOSITE_QUEUE_OFF = (16 + SORT_SYNC_WORDS + 63) / 64 * 64;
-constexpr int GAUSS_SYNC_WORDS = COMPOSITE_QUEUE_OFF + COMPOSITE_QUEUE_GROUPS * COMPOSITE_QUEUE_STRIDE;
+// State of the redo list build's sort (prefix lists), zeroed by the prefix
+// emission of every render.
+constexpr int REDO_SYNC_OFF = COMPOSITE_QUEUE_OFF + COMPOSITE_QUEUE_GROUPS * COMPOSITE_QUEUE_STRIDE;
+constexpr int GAUSS_SYNC_WORDS = REDO_SYNC_OFF + SORT_SYNC_WORDS;
 // counts words the last-workgroup launches count arrivals in
 constexpr int COUNT_ARRIVE_RANGES = 6;
 // counts word set when a look-back wait of the render's list sort (tile sort
 // or PLAIN pair sort) exceeded its bound
 constexpr int COUNT_LIST_FAULT = 5;
+// Prefix lists (FAST, one view): [7] pair count of the lists of the nearest
+// records (0 when the frame's lists are whole), [8] set by compositing when
+// some tile's prefix list did not finish it (the whole lists are then built
+// and every tile composited again), [9] the pair count of that second list
+// build: counts[0] when [8] is set, else 0.
+constexpr int COUNT_PREFIX_N = 7;
+constexpr int COUNT_REDO = 8;
+constexpr int COUNT_REDO_N = 9;
 // The depth sort's fault flag (last word of its state, counted from
 // counts), set when a look-back wait exceeds its bound.
 constexpr int DSORT_FAULT = 16 + SORT_SYNC_WORDS - 1;

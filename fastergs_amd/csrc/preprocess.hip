@@ -454,7 +454,9 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
                                                      const uint32_t* __restrict__ lrank,
                                                      uint32_t* lband,
                                                      uint32_t cap, void* tkeys, uint32_t* recs,
-                                                     int t16) {
+                                                     int t16, uint32_t rank_limit,
+                                                     const uint32_t* __restrict__ gate) {
+  if (gate && gate[0] == 0u) return;
   const uint32_t NT = counts[3];
   const int lane = lane_id();
   const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
@@ -474,7 +476,9 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
     const uint32_t toff = views[v].tile_off;
     uint32_t base = 0u;
     if (EMIT) {
-      base = poff[lrank[g]];
+      const uint32_t rank = lrank[g];
+      if (rank >= rank_limit) continue;  // past the prefix (wave-uniform)
+      base = poff[rank];
       for (int k = 1; k <= band; ++k) base += lband[q - k];
     }
     uint32_t run = 0;
@@ -510,22 +514,23 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
 hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, const float* geom,
                               const uint32_t* large_ids, const uint32_t* counts, uint32_t* pcount,
                               const uint32_t* poff, const uint32_t* lrank, uint32_t* lband,
-                              uint32_t cap, void* tkeys, uint32_t* recs, bool t16, hipStream_t s) {
+                              uint32_t cap, void* tkeys, uint32_t* recs, bool t16, hipStream_t s,
+                              uint32_t rank_limit, const uint32_t* gate) {
   if (P <= 0) return hipSuccess;
   // 16384 waves, grid-stride; each exits past the device-side count (fewer
   // for the count, which runs beside the depth sort, measured slower: the
   // pair offsets wait for it)
   const int blocks = 4096;
   if (emit)
-    FGS_TIMED("large_tiles_emit", s,
+    FGS_TIMED(gate ? "large_tiles_emit_redo" : "large_tiles_emit", s,
               (k_large_tiles<true><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
                                                           pcount, poff, lrank, lband, cap, tkeys,
-                                                          recs, t16 ? 1 : 0)));
+                                                          recs, t16 ? 1 : 0, rank_limit, gate)));
   else
     FGS_TIMED("large_tiles_count", s,
               (k_large_tiles<false><<<blocks, 256, 0, s>>>(P, V, views, geom, large_ids, counts,
                                                            pcount, poff, lrank, lband, cap, tkeys,
-                                                           recs, t16 ? 1 : 0)));
+                                                           recs, t16 ? 1 : 0, rank_limit, gate)));
   return hipGetLastError();
 }
 
@@ -533,15 +538,29 @@ hipError_t launch_large_tiles(bool emit, int P, int V, const ViewRec* views, con
 // of ranks [64 w, 64 w + 64) (order = records by depth rank). Pair slots are
 // assigned in that order, so a wave's pairs fill one contiguous stretch of
 // the pair array (apart from the slots of large rects among them, which
-// k_large_tiles fills) and its stores combine in L2.
+// k_large_tiles fills) and its stores combine in L2. Ranks [0, limit) only
+// (a prefix of the depth order: their pairs are the slots [0, poff[limit])).
 __global__ __launch_bounds__(256) void k_emit_small(int P, int NR, const ViewRec* __restrict__ views,
                                                     const float* __restrict__ geom, int split,
                                                     const uint64_t* __restrict__ order,
                                                     const uint32_t* __restrict__ poff, uint32_t cap,
                                                     void* __restrict__ tkeys,
-                                                    uint32_t* __restrict__ recs, int t16) {
+                                                    uint32_t* __restrict__ recs, int t16,
+                                                    int limit, uint32_t* __restrict__ counts,
+                                                    int mode) {
   __shared__ WaveTiles lds[4];
   __shared__ WaveEmit lde[4];
+  if (mode == EMIT_REDO) {
+    if (counts[COUNT_REDO] == 0u) return;  // the prefix lists sufficed
+  } else if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      counts[COUNT_PREFIX_N] = mode == EMIT_PREFIX ? poff[limit] : 0u;
+      counts[COUNT_REDO] = 0u;
+      counts[COUNT_REDO_N] = 0u;
+    }
+    if (mode == EMIT_PREFIX)
+      for (int i = threadIdx.x; i < SORT_SYNC_WORDS; i += 256) counts[REDO_SYNC_OFF + i] = 0u;
+  }
   const int rank = blockIdx.x * 256 + threadIdx.x;
   const int lane = lane_id();
   WaveTiles& L = lds[threadIdx.x / WAVE];
@@ -550,7 +569,7 @@ __global__ __launch_bounds__(256) void k_emit_small(int P, int NR, const ViewRec
   bool small = false;
   uint32_t g = 0u, pos = 0u, toff = 0u;
   int gx = 1;
-  if (rank < NR) {
+  if (rank < limit) {
     g = (uint32_t)order[rank];
     r = load_geom(geom, g);
     const int area = (r.r2 - r.r0) * (r.r3 - r.r1);
@@ -575,12 +594,17 @@ __global__ __launch_bounds__(256) void k_emit_small(int P, int NR, const ViewRec
 
 hipError_t launch_emit_small(int P, int V, const ViewRec* views, const float* geom, int split,
                              const uint64_t* order, const uint32_t* poff, uint32_t cap,
-                             void* tkeys, uint32_t* recs, bool t16, hipStream_t s) {
+                             void* tkeys, uint32_t* recs, bool t16, hipStream_t s,
+                             int64_t limit, uint32_t* counts, EmitMode mode) {
   const int64_t NR = (int64_t)P * V;
+  if (limit < 0 || limit > NR || (mode == EMIT_PREFIX && limit >= NR))
+    return hipErrorInvalidValue;
+  if (mode != EMIT_PREFIX) limit = NR;
   if (NR > 0)
-    FGS_TIMED("emit_small", s,
-              (k_emit_small<<<(unsigned)((NR + 255) / 256), 256, 0, s>>>(
-                  P, (int)NR, views, geom, split, order, poff, cap, tkeys, recs, t16 ? 1 : 0)));
+    FGS_TIMED(mode == EMIT_REDO ? "emit_small_redo" : "emit_small", s,
+              (k_emit_small<<<(unsigned)((limit + 255) / 256 > 0 ? (limit + 255) / 256 : 1), 256,
+                              0, s>>>(P, (int)NR, views, geom, split, order, poff, cap, tkeys,
+                                      recs, t16 ? 1 : 0, (int)limit, counts, (int)mode)));
   return hipGetLastError();
 }
 

@@ -147,7 +147,9 @@ __global__ __launch_bounds__(256) void k_tile_ranges(int64_t R_cap,
                                                      int bucket, uint32_t* __restrict__ bcount,
                                                      uint32_t* __restrict__ maxc,
                                                      uint32_t* __restrict__ boff,
-                                                     uint32_t* __restrict__ arrive) {
+                                                     uint32_t* __restrict__ arrive,
+                                                     const uint32_t* __restrict__ gate) {
+  if (gate && gate[0] == 0u) return;
   __shared__ uint32_t s_lo[32];
   const int grp = threadIdx.x >> 3, j = threadIdx.x & 7, g8 = (threadIdx.x & 56);
   const int t = blockIdx.x * RANGES_PER_BLOCK + grp;
@@ -173,19 +175,21 @@ __global__ __launch_bounds__(256) void k_tile_ranges(int64_t R_cap,
 }
 hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tiles, bool t16,
                               uint32_t* ranges, int T, int bucket, uint32_t* bcount,
-                              uint32_t* maxc, uint32_t* boff, uint32_t* arrive, hipStream_t s) {
+                              uint32_t* maxc, uint32_t* boff, uint32_t* arrive, hipStream_t s,
+                              const uint32_t* gate) {
   if (T <= 0) return hipSuccess;
   const unsigned grid = (unsigned)((T + RANGES_PER_BLOCK - 1) / RANGES_PER_BLOCK);
+  const char* name = gate ? "tile_ranges_redo" : "tile_ranges";
   if (t16)
-    FGS_TIMED("tile_ranges", s,
+    FGS_TIMED(name, s,
               (k_tile_ranges<uint16_t><<<grid, 256, 0, s>>>(R, n_dev, (const uint16_t*)tiles,
                                                             ranges, T, bucket, bcount, maxc,
-                                                            boff, arrive)));
+                                                            boff, arrive, gate)));
   else
-    FGS_TIMED("tile_ranges", s,
+    FGS_TIMED(name, s,
               (k_tile_ranges<uint32_t><<<grid, 256, 0, s>>>(R, n_dev, (const uint32_t*)tiles,
                                                             ranges, T, bucket, bcount, maxc,
-                                                            boff, arrive)));
+                                                            boff, arrive, gate)));
   return hipGetLastError();
 }
 
@@ -396,6 +400,18 @@ __device__ __forceinline__ void composite_item(const CompArgs& a, CompShared& sh
       }
     }
   }
+  if (a.redo) {
+    // Prefix lists: the list holds the front of the tile's whole list. A
+    // pixel that stopped at the end of a whole bucket has its final values;
+    // one still open at the end of the list, or stopped after a bucket the
+    // list cut short, needs the rest of the whole list.
+    const bool open = inside && (!done || cnt % B != 0);
+    if (__syncthreads_or(open) && t == 0) {
+      a.redo[0] = 1u;
+      a.redo[1] = a.counts[0];
+      if (a.status) *a.status = 1u;
+    }
+  }
   if (lists_faulted(a.counts)) T = C0 = __builtin_nanf("");
   if (inside) {
     const size_t HW = (size_t)W * H;
@@ -437,6 +453,7 @@ __device__ __forceinline__ void composite_item(const CompArgs& a, CompShared& sh
 template <bool QUEUE>
 __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   __shared__ CompShared sh;
+  if (a.gate && a.gate[0] == 0u) return;  // whole grid: the prefix lists sufficed
   if (!QUEUE) {
     const uint64_t t_start = a.cycles ? wall_clock64() : 0;
     uint32_t item;
@@ -500,9 +517,11 @@ hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s) {
     const int resident = composite_queue_grid();
     if (resident <= 0) return hipErrorInvalidValue;
     const unsigned g = grid < (unsigned)resident ? grid : (unsigned)resident;
-    FGS_TIMED("composite_fast", s, (k_composite_fast<true><<<g, 256, 0, s>>>(a)));
+    FGS_TIMED(a.gate ? "composite_fast_redo" : "composite_fast", s,
+              (k_composite_fast<true><<<g, 256, 0, s>>>(a)));
   } else {
-    FGS_TIMED("composite_fast", s, (k_composite_fast<false><<<grid, 256, 0, s>>>(a)));
+    FGS_TIMED(a.gate ? "composite_fast_redo" : "composite_fast", s,
+              (k_composite_fast<false><<<grid, 256, 0, s>>>(a)));
   }
   return hipGetLastError();
 }

@@ -585,10 +585,14 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
   __shared__ uint32_t bstart[256];
   __shared__ uint32_t gofs[256];
   const int t = threadIdx.x;
-  const int64_t tile = xcd_tile(blockIdx.x, nb);
-  const int64_t base = tile * SortTiling<K>::TILE;
   const int64_t n = dev_count(n_dev, n_cap);
-  if (tile >= nb || base >= n) return;  // whole workgroup: past the tiles / device-side count
+  // the tiles holding keys (the device-side count may be far below the
+  // capacity the grid is sized for), spread over all eight XCDs
+  const int64_t nbe = min((int64_t)nb, (n + SortTiling<K>::TILE - 1) / SortTiling<K>::TILE);
+  if ((int64_t)blockIdx.x >= 8 * ((nbe + 7) / 8)) return;  // whole workgroup
+  const int64_t tile = xcd_tile(blockIdx.x, nbe);
+  const int64_t base = tile * SortTiling<K>::TILE;
+  if (tile >= nbe) return;  // whole workgroup
   const int nvalid = (int)min((int64_t)SortTiling<K>::TILE, n - base);
 #pragma unroll
   for (int q = 0; q < NW; ++q) wcnt[q][t] = 0u;
@@ -607,9 +611,11 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
 struct SortNames {
   const char *hist, *scatter, *hist_all, *onesweep;
 };
-static const SortNames k_sort_names[3] = {
+static const SortNames k_sort_names[5] = {
     {"radix_hist_kv", "radix_scatter_kv", "sort_hist_kv", "onesweep_kv"},
     {"depth_hist", "depth_scatter", "depth_hist_all", "depth_onesweep"},
+    {"tile_hist", "tile_scatter", "tile_hist_all", "tile_onesweep"},
+    {"redo_hist", "redo_scatter", "redo_hist_all", "redo_onesweep"},
     {"tile_hist", "tile_scatter", "tile_hist_all", "tile_onesweep"},
 };
 
@@ -757,12 +763,17 @@ __global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
   constexpr uint32_t mask = (1u << BITS) - 1u;
+  const int64_t n = dev_count(n_dev, n_cap);
+  // workgroups past the tiles that hold keys leave before taking one (the
+  // device-side count may be far below the capacity the grid is sized for,
+  // and every take is an atomic on one word); the others take exactly the
+  // tiles that hold keys
+  if ((int64_t)blockIdx.x * SortTiling<K>::TILE >= n) return;  // whole workgroup
   if (t == 0) s_tile = atomicAdd(counter, 1u);
 #pragma unroll
   for (int q = 0; q < NW; ++q) wcnt[q][t] = 0u;
   __syncthreads();
   const int64_t tile = s_tile;
-  const int64_t n = dev_count(n_dev, n_cap);
   const int64_t base = tile * SortTiling<K>::TILE;
   if (base >= n) return;  // whole workgroup; no later tile waits on this one
   if (status_next) status_next[tile * 256 + t] = 0u;
@@ -888,7 +899,8 @@ hipError_t sort_kv(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int
       end_bit > (int)(8 * sizeof(K)))
     return hipErrorInvalidValue;
   const SortNames& nm = k_sort_names[kind];
-  if (n <= ONESWEEP_MAX_KEYS)
+  // the redo list build usually has no keys at all: the fewest launches
+  if (n <= ONESWEEP_MAX_KEYS || kind == SORT_TILES_REDO)
     return sort_kv_onesweep<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s,
                                in_alt, n_dev, last_vals, nm, sync, fault_word);
   return sort_kv_classic<K>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, ws, s, in_alt,

@@ -86,10 +86,11 @@ struct Frame {
 // opts = (schedule, bucket_size, split_tiles, abs_grad, forward_only,
 //         tile_mode (-1: none), split_len, features, log_cycles, order,
 //         items, n_items, prev_features, features_out, composite_queue,
-//         tile_w) --
+//         tile_w, list_prefix, list_status) --
 //         items .. features_out are device addresses of the tile schedule's
 //         work list and features (0: none), owned by the caller's schedule
-//         object
+//         object; list_status is the address of a device-accessible word
+//         (0: none)
 Frame make_frame(const at::Tensor& means3D, const at::Tensor& opacities, const at::Tensor& scales,
                  const at::Tensor& rotations, const at::Tensor& dc, const at::Tensor& shs,
                  const at::Tensor& colors, const at::Tensor& culling, const at::Tensor& view,
@@ -98,7 +99,7 @@ Frame make_frame(const at::Tensor& means3D, const at::Tensor& opacities, const a
                  double scale_modifier, double mult, int64_t sh_degree,
                  const std::vector<int64_t>& opts, const at::Tensor& mlp) {
   if (!means3D.is_cuda()) throw std::invalid_argument("means3D must be a GPU tensor");
-  if (opts.size() != 16) throw std::invalid_argument("options must have 16 entries");
+  if (opts.size() != 18) throw std::invalid_argument("options must have 18 entries");
   Frame f;
   f.dev = means3D.device();
   const int64_t P = means3D.size(0);
@@ -125,8 +126,10 @@ Frame make_frame(const at::Tensor& means3D, const at::Tensor& opacities, const a
                       (const float*)ptr(f.scales), (const float*)ptr(f.rotations),
                       (const float*)ptr(f.dc), (const float*)ptr(f.shs),
                       (const float*)ptr(f.colors), (const uint8_t*)ptr(f.culling)};
-  f.opt = fgs_options{(int)opts[0], (int)opts[1], (int)opts[2], (int)opts[3], nullptr,
-                      (int)opts[4], (int)opts[14], (int)opts[15]};
+  f.opt = fgs_options{(int)opts[0],  (int)opts[1],  (int)opts[2],
+                      (int)opts[3],  nullptr,        (int)opts[4],
+                      (int)opts[14], (int)opts[15],  (int)opts[16],
+                      reinterpret_cast<uint32_t*>(static_cast<uintptr_t>(opts[17]))};
   if (opts[5] >= 0) {
     auto addr = [](int64_t a) { return reinterpret_cast<void*>(static_cast<uintptr_t>(a)); };
     f.ts = fgs_tile_sched{(int)opts[5],

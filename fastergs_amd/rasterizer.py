@@ -25,6 +25,7 @@ and the current stream.
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 from dataclasses import dataclass
 from typing import NamedTuple
@@ -62,6 +63,15 @@ class RasterOptions:
     # 16 is the reference's 16 x 16. Semantic in FAST like bucket_size: the
     # per-tile lists and bucket boundaries are defined on the tile
     tile_w: int = 16
+    # FAST single views through the drop-in API: build the per-tile lists of
+    # the nearest ceil(P / div) Gaussians first, starting at div =
+    # list_prefix_div (0 or 1: never). Every pixel that stops at the end of a
+    # whole bucket within them has its final values; if some pixel does not,
+    # the whole lists are built and the frame composited again on the device.
+    # Outputs never depend on it. Per frame shape the divisor then follows
+    # the outcomes (see _PrefixPolicy): larger while prefixes suffice, smaller
+    # when they do not.
+    list_prefix_div: int = 16
     # render with pair / bucket capacities remembered from earlier frames of
     # the same shape instead of waiting for this frame's counts (the counts
     # are checked while the render runs; an overflow renders again exactly)
@@ -124,6 +134,82 @@ def _stream(device) -> int:
 # Largest pair count / bucket bound seen per frame shape (plus headroom):
 # the capacity a speculative forward renders with.
 _capacity_hints: dict = {}
+
+
+class _PrefixPolicy:
+    """Prefix size of the frames of one shape, adapted from their outcomes.
+
+    Each prefix frame's status word (pinned host memory that compositing
+    sets to 1 on the device when the prefix lists did not suffice) is read
+    once an event after the frame has completed, so the host never waits.
+    After STREAK frames that sufficed, one frame tries a divisor twice as
+    large (a prefix half as long); the frames in between keep the last one
+    that sufficed. A tried divisor that did not suffice caps later tries
+    below it; a frame at the kept divisor that did not suffice halves it.
+    Below divisor 2, or when no prefix of the shape has sufficed yet, frames
+    render whole lists for RETRY frames before starting over."""
+    STREAK = 4
+    MAX_DIV = 256
+    RETRY = 256
+
+    def __init__(self, div: int):
+        self.start = self.good = div
+        self.cap = self.MAX_DIV
+        self.streak = 0
+        self.probe = 0      # divisor of the try in flight (0: none)
+        self.off = 0        # frames left with whole lists
+        self.ever_ok = False
+        self.pending = collections.deque()  # (status word, event, divisor) in frame order
+        self.free = []
+
+    def _outcome(self, div: int, failed: bool):
+        self.ever_ok |= not failed
+        if div == self.probe:
+            self.probe = 0
+            if failed:
+                self.cap = div // 2
+            else:
+                self.good, self.streak = div, 0
+        elif failed:
+            if div == self.good:
+                self.good //= 2
+                self.cap = max(self.good, 1)
+                self.streak = 0
+                if self.good < 2 or not self.ever_ok:
+                    self.off = self.RETRY
+        elif div == self.good:
+            self.streak += 1
+
+    def divisor(self) -> int:
+        """Divisor for the next frame (0: whole lists)."""
+        while self.pending and self.pending[0][1].query():
+            status, _, div = self.pending.popleft()
+            failed = int(status[0]) != 0
+            status.zero_()
+            self.free.append(status)
+            self._outcome(div, failed)
+        if self.off > 0:
+            self.off -= 1
+            if self.off == 0:
+                self.good, self.cap, self.streak = self.start, self.MAX_DIV, 0
+                self.ever_ok = False
+            return 0
+        if not self.probe and self.streak >= self.STREAK and 2 * self.good <= self.cap:
+            self.probe = 2 * self.good
+            return self.probe
+        return self.good
+
+    def status_word(self) -> torch.Tensor:
+        return self.free.pop() if self.free else torch.zeros(1, dtype=torch.int32,
+                                                             pin_memory=True)
+
+    def track(self, status: torch.Tensor, div: int):
+        ev = torch.cuda.Event()
+        ev.record()
+        self.pending.append((status, ev, div))
+
+
+_prefix_policies: dict = {}
 
 
 class Frame:
@@ -209,8 +295,10 @@ class Frame:
                    int(bool(ts.log_cycles)), int(bool(ts.order)), *(int(a) for a in addr)]
             if getattr(ts, "model", None) is not None:
                 mlp = ts.model.params
+        status = getattr(self, "list_status", None)
         opts = [int(o.schedule), int(o.bucket_size), int(o.split_tiles), int(o.abs_grad),
-                int(o.forward_only), *tsl, int(o.composite_queue), int(o.tile_w)]
+                int(o.forward_only), *tsl, int(o.composite_queue), int(o.tile_w),
+                int(getattr(self, "list_prefix", 0)), 0 if status is None else status.data_ptr()]
         n = lambda t: _NONE if t is None else t
         return (self.means3D, self.opacities, self.scales, self.rotations, n(self.dc), n(self.shs),
                 n(self.colors), n(self.culling), self.view, self.proj, self.campos, self.bg,
@@ -265,8 +353,21 @@ class Frame:
         key = (self.P, self.W, self.H, self.opts.schedule, self.opts.bucket_size, self.device)
         hint = _capacity_hints.get(key) if self.opts.speculative else None
         R_hint, NB_hint = hint if hint is not None else (0, 0)
+        pol, div = None, int(self.opts.list_prefix_div)
+        if div > 1 and self.opts.schedule == _lib.SCHEDULE_FAST and self.opts.tile_sched is None:
+            pol = _prefix_policies.setdefault(key + (self.opts.tile_w, div), _PrefixPolicy(div))
+            div = pol.divisor()
+            if div > 1 and self.P > div:
+                self.list_prefix = -(-self.P // div)
+                self.list_status = pol.status_word()
+            else:
+                pol = None
         (self.radii, self.gws, self.pws, self.out_color, self.T_final, self.last_contributor,
          c) = _C.ops.forward(*self._ext_args(lists=True), int(R_hint), int(NB_hint))
+        if pol is not None:
+            pol.track(self.list_status, div)
+            self.list_status = None  # later calls on the frame do not report
+            self.prefix_lists = True
         ts = self.opts.tile_sched
         if ts is not None and hasattr(ts, "advance"):
             ts.advance()
@@ -275,6 +376,23 @@ class Frame:
         old = _capacity_hints.get(key, (0, 0))
         _capacity_hints[key] = (max(old[0], self.R + self.R // 8 + 1024),
                                 max(old[1], self.NB + self.NB // 8 + 64))
+        return self
+
+    def complete_lists(self):
+        """Whole per-tile lists for a frame rendered from prefix lists (the
+        list passes and inspection views read whole lists): emission, sort,
+        ranges and compositing again with whole lists into the frame's own
+        workspaces and outputs, which keep their values."""
+        if not getattr(self, "prefix_lists", False):
+            return self
+        self.list_prefix = 0
+        check(lib().fgs_render(ctypes.byref(self.cam), ctypes.byref(self.gauss),
+                               ctypes.byref(self.copts), _ptr(self.gws), self.gws.numel(),
+                               _ptr(self.pws), self.pws.numel(), self.R_cap, self.NB_cap,
+                               _ptr(self.out_color), _ptr(self.T_final),
+                               _ptr(self.last_contributor), ctypes.c_void_p(_stream(self.device))),
+              "fgs_render")
+        self.prefix_lists = False
         return self
 
     # -- backward -----------------------------------------------------------
@@ -304,6 +422,7 @@ class Frame:
     def render_simp(self):
         """Importance statistics over this frame's lists (render_simp):
         (accum_weights (P,) float, accum_count (P,) int32, accum_max_count (P,) float)."""
+        self.complete_lists()
         dev, P = self.device, self.P
         w = torch.empty(P, dtype=torch.float32, device=dev)
         c = torch.empty(P, dtype=torch.int32, device=dev)
@@ -317,6 +436,7 @@ class Frame:
         """Per-pixel depth over this frame's lists (render_depth): out_pts
         (3,H,W), rendered_depth (1,H,W), accum_alpha (1,H,W), gidx (1,H,W)
         int32 and discriminants (1,H,W)."""
+        self.complete_lists()
         dev, H, W = self.device, self.H, self.W
         e = lambda c, dt=torch.float32: torch.empty((c, H, W), dtype=dt, device=dev)
         out = {"out_pts": e(3), "rendered_depth": e(1), "accum_alpha": e(1),
@@ -333,6 +453,7 @@ class Frame:
     # -- inspection -----------------------------------------------------------
     def view_arrays(self) -> dict:
         """Torch views of the intermediate arrays (valid until the frame dies)."""
+        self.complete_lists()
         if self.opts.schedule == _lib.SCHEDULE_FAST and getattr(self, "pws", None) is not None:
             # the bucket tables are built on demand (no pass of the step reads them)
             check(lib().fgs_bucket_tables(_ptr(self.gws), _ptr(self.pws), self.P, self.W, self.H,

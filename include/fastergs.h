@@ -134,6 +134,23 @@ typedef struct fgs_options {
                        the FAST bucket boundaries (where the T < 1e-4 cut-off is tested) and
                        render_simp's per-tile stop, are defined on the tile; the PLAIN images
                        do not depend on it. T (tiles) = ceil(W / tile_w) * ceil(H / tile_h). */
+  int list_prefix;  /* FAST, one view, no tile_sched: > 0 renders the frame first from prefix
+                       lists, the per-tile lists of its list_prefix nearest records (depth ranks
+                       below list_prefix, < P): emission, the list sort and the ranges handle
+                       their pairs only. The lists are in depth order, so a prefix list is the
+                       front of the tile's whole list, and a pixel that stops at the end of a
+                       whole bucket within it has its final values. When some pixel of some tile
+                       is left open, the whole lists are built and every tile is composited
+                       again, on the device (no host wait; launches that find nothing to do
+                       return at once). The images, T_final, last contributors, checkpoints
+                       and gradients never depend on it; the frame's lists and ranges (views,
+                       fgs_render_simp, fgs_render_depth) are the prefix lists when they
+                       sufficed, so pass 0 to fgs_render again on the frame to rebuild whole
+                       lists before those. 0 = whole lists. */
+  uint32_t* list_status; /* with list_prefix: device-accessible word (device memory, or mapped
+                            pinned host memory read after the frame completes) that compositing
+                            sets to 1 when the prefix lists did not suffice; never cleared by
+                            the library. NULL: none. */
 } fgs_options;
 
 /* Device pointers into the two workspaces, for inspection by tests and
