@@ -188,13 +188,16 @@ def main():
     set_default_options(schedule=sched, bucket_size=args.bucket, tile_w=16, composite_queue=False,
                         list_prefix_div=args.list_prefix_div)
 
-    graph_fps = graph_render_fps(settings, params, sched, args)
+    graph_fps = graph_render_fps(settings, params, sched, args, prefix["div"] if prefix else 0)
     step_graph_ms = None
     if sched == SCHEDULE_FAST:
         for t in params.values():
             t.grad = None
+        # the prefix size the eager frames settled on, fixed in the graph
+        div = prefix["div"] if prefix else 0
         step_graph_ms = captured_step_ms([settings], params, [target],
-                                         RasterOptions(schedule=sched, bucket_size=args.bucket),
+                                         RasterOptions(schedule=sched, bucket_size=args.bucket,
+                                                       list_prefix_div=div),
                                          steps=args.steps, stats=True)
     fma_tflops = fma_peak()
     adam = sparse_adam_step(rast, params)
@@ -351,6 +354,10 @@ def kernel_roofline(step, params, settings, sched, args, P):
                dc=params["dc"].detach(), shs=params["shs"].detach(),
                options=RasterOptions(schedule=sched, bucket_size=args.bucket)).forward()
     R, T = fr.R, ((fr.W + 15) // 16) * ((fr.H + 15) // 16)
+    # the tile sort and the ranges handle the prefix lists' pairs when they
+    # sufficed (the step's frames use the same prefix size)
+    lc = fr.list_counts()
+    R_sorted = lc["prefix_pairs"] if lc["prefix_pairs"] and not lc["redo"] else R
     V = int((fr.radii > 0).sum().item())
     E = int(fr.last_contributor.sum().item())
     M = int(params["shs"].shape[1])
@@ -370,7 +377,8 @@ def kernel_roofline(step, params, settings, sched, args, P):
         avg_s = tot_ms / launches / 1e3
         ent = {"ms_per_iter": round(tot_ms / steps, 4), "launches_per_iter": launches / steps,
                "avg_us": round(avg_s * 1e6, 2)}
-        units = kernel_units(name, P, M, V, R, T, E, sched == 1)
+        units = kernel_units(name, P, M, V, R_sorted if name.startswith("tile_") else R, T, E,
+                             sched == 1)
         if name == "composite_fast" and executed is not None:
             # the reference evaluates every pair for all 256 pixels of its
             # tile (E_f = 256 R); this pass stops at the round in which the
@@ -566,15 +574,17 @@ def captured_step_ms(settings_list, params, targets, opts, steps=10, stats=False
     return ms
 
 
-def graph_render_fps(settings, params, sched, args):
-    """Forward-only FPS of the heavy workload with the frame captured in one HIP graph."""
+def graph_render_fps(settings, params, sched, args, div=16):
+    """Forward-only FPS of the heavy workload with the frame captured in one
+    HIP graph (prefix lists of P / div Gaussians)."""
     import torch
     from fastergs_amd.graph import CapturedRenderer
     from fastergs_amd.rasterizer import RasterOptions
     p = {k: v.detach() for k, v in params.items()}
     rend = CapturedRenderer(settings, p["means3D"], p["opacities"], p["scales"], p["rotations"],
                             dc=p["dc"], shs=p["shs"],
-                            options=RasterOptions(schedule=sched, bucket_size=args.bucket))
+                            options=RasterOptions(schedule=sched, bucket_size=args.bucket,
+                                                  list_prefix_div=div))
     camt = rend.cambuf.clone().reshape(1, 35)
     _time_replays(rend, camt, 5)
     ms = _time_replays(rend, camt, 50)
@@ -619,13 +629,20 @@ def render_loop(args, frames=120):
         _time_replays(rend, camt, frames)
         if rend.check(grow=True):
             break
+    rend.prefix_redone()
     ms = _time_replays(rend, camt, frames)
     rend.check()
+    redone = rend.prefix_redone()  # some timed replay needed the whole lists
     R_cap, NB_cap = rend.capacity()
     del rend
     torch.cuda.empty_cache()
     # sort bandwidth and pair counts from eager frames at four orbit positions
-    sort_ms, sort_bytes, Rs = 0.0, 0, []
+    sort_ms, sort_bytes, Rs, sorted_pairs = 0.0, 0, [], []
+
+    def sorted_count(fr):
+        # the tile sort's keys: the prefix lists' pairs when they sufficed
+        lc = fr.list_counts()
+        return lc["prefix_pairs"] if lc["prefix_pairs"] and not lc["redo"] else fr.R
     for k in range(0, frames, frames // 4):
         _lib.profile_enable(True)
         fr = Frame(settings_of(cams[k]), p["means3D"], p["opacities"], p["scales"],
@@ -638,7 +655,7 @@ def render_loop(args, frames=120):
             passes = sum(prof[n][1] for n in ("tile_scatter", "tile_onesweep") if n in prof)
             hists = sum(prof[n][1] for n in ("tile_hist", "tile_hist_all") if n in prof)
             sort_ms += sum(prof[n][0] for n in tile_sort if n in prof)
-            sort_bytes += (2 * hists + 12 * passes) * fr.R
+            sort_bytes += (2 * hists + 12 * passes) * sorted_count(fr)
         else:
             radix = ("sort_hist_k", "onesweep_k", "onesweep_k_last", "radix_hist_k",
                      "radix_scatter_k", "radix_scatter_k_last")
@@ -646,12 +663,14 @@ def render_loop(args, frames=120):
             sort_ms += sum(prof[n][0] for n in radix if n in prof)
             sort_bytes += (8 + passes * 16) * fr.R
         Rs.append(fr.R)
+        sorted_pairs.append(sorted_count(fr))
         del fr
     torch.cuda.empty_cache()
     return {"workload": f"dense: {P} Gaussians, {cams[0].width}x{cams[0].height}, SH3, "
                         f"{frames}-frame orbit r=3.5, one HIP graph per frame",
             "fps": round(frames * 1000.0 / ms, 2), "ms_per_frame": round(ms / frames, 4),
-            "pairs_sampled": Rs, "pair_capacity": R_cap, "bucket_capacity": NB_cap,
+            "pairs_sampled": Rs, "pairs_sorted": sorted_pairs, "pair_capacity": R_cap,
+            "bucket_capacity": NB_cap, "prefix_redone": redone,
             "sort_gbs": round(sort_bytes / (sort_ms / 1e3) / 1e9, 1),
             "sort_frac_hbm": round(sort_bytes / (sort_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
 
@@ -980,8 +999,9 @@ def small_config(args, steps=20, warmup=5):
     res = {"workload": f"small: {P} Gaussians uniform in a ball, {cam.width}x{cam.height}, SH3, "
                        "bucket 32"}
     for name, sch in (("fast", SCHEDULE_FAST), ("plain", SCHEDULE_PLAIN)):
-        set_default_options(schedule=sch, bucket_size=32)
-        for _ in range(warmup):
+        set_default_options(schedule=sch, bucket_size=32, list_prefix_div=args.list_prefix_div)
+        # warm-up, and the steps over which the prefix-list policy settles
+        for _ in range(warmup + (PREFIX_SETTLE_STEPS if sch == SCHEDULE_FAST else 0)):
             step()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

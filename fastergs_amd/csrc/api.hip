@@ -149,6 +149,16 @@ int check_opts(const fgs_options* o) {
   return FGS_OK;
 }
 
+// The list passes read whole lists: a frame rendered with prefix lists is
+// rendered again with list_prefix 0 first (the caller passes the frame's
+// options to every call on it).
+int check_whole_lists(const fgs_options* o) {
+  if (o->list_prefix > 0)
+    return fail(FGS_ERR_ARG,
+                "the frame's lists may be prefix lists: render it again with list_prefix 0 first");
+  return FGS_OK;
+}
+
 int check_cam(const fgs_camera* c, int tw) {
   if (!c) return fail(FGS_ERR_ARG, "camera is NULL");
   if (c->width <= 0 || c->height <= 0) return fail(FGS_ERR_ARG, "image size must be positive");
@@ -1076,7 +1086,7 @@ int fgs_render_simp(void* gauss_ws, void* pair_ws, int P, int width, int height,
                     int64_t NB, const fgs_options* opt, float* accum_weights,
                     int32_t* accum_count, float* accum_max_count, void* stream) {
   int rc;
-  if ((rc = check_opts(opt))) return rc;
+  if ((rc = check_opts(opt)) || (rc = check_whole_lists(opt))) return rc;
   if (P < 0 || width <= 0 || height <= 0 || R < 0) return fail(FGS_ERR_ARG, "bad frame size");
   if (P == 0) return FGS_OK;
   if (!gauss_ws || !pair_ws || !accum_weights || !accum_count || !accum_max_count)
@@ -1102,7 +1112,7 @@ int fgs_render_depth(void* gauss_ws, void* pair_ws, int P, int width, int height
                      const float* campos, float* out_pts, float* out_depth, float* accum_alpha,
                      int32_t* gidx, float* discriminants, void* stream) {
   int rc;
-  if ((rc = check_opts(opt))) return rc;
+  if ((rc = check_opts(opt)) || (rc = check_whole_lists(opt))) return rc;
   if (P < 0 || width <= 0 || height <= 0 || R < 0) return fail(FGS_ERR_ARG, "bad frame size");
   if (!gauss_ws || !pair_ws || !projmatrix || !campos || !out_pts || !out_depth ||
       !accum_alpha || !gidx || !discriminants)

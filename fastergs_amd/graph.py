@@ -56,6 +56,27 @@ def _in_place(name: str, t: torch.Tensor, dev, per: int) -> torch.Tensor:
     return t
 
 
+def _prefix_lists(copts, opts: RasterOptions, P: int, V: int, dev):
+    """Prefix lists in a captured frame (FAST, one view): the lists of the
+    nearest ceil(P / list_prefix_div) Gaussians first, a size fixed at
+    capture. A replay whose prefixes did not suffice builds the whole lists
+    and composites again on the device (same outputs) and sets the returned
+    device word to 1 (read and cleared by ``prefix_redone()``)."""
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    div = int(opts.list_prefix_div)
+    if (opts.schedule == _lib.SCHEDULE_FAST and V == 1 and opts.tile_sched is None and div > 1
+            and P > div):
+        copts.list_prefix = -(-P // div)
+        copts.list_status = status.data_ptr()
+    return status
+
+
+def _redone(status: torch.Tensor) -> bool:
+    r = bool(int(status.item()))
+    status.zero_()
+    return r
+
+
 class CapturedRenderer:
     """Forward renderer whose whole frame is one graph replay.
 
@@ -74,6 +95,7 @@ class CapturedRenderer:
         self.copts = self.opts.to_c(False)
         self.copts.forward_only = 1  # rendering only: no bucket checkpoints
         self.P = P = int(means3D.shape[0])
+        self.prefix_status = _prefix_lists(self.copts, self.opts, P, 1, dev)
         self.W, self.H = int(settings.image_width), int(settings.image_height)
         self.T = _lib.tiles_of(self.W, self.H, self.opts.tile_w)
         self.headroom = float(headroom)
@@ -189,6 +211,12 @@ class CapturedRenderer:
     def capacity(self) -> tuple[int, int]:
         return self.R_cap, self.NB_cap
 
+    def prefix_redone(self) -> bool:
+        """Whether some replay since the last call needed the whole lists
+        (its prefixes did not suffice; its outputs are exact either way):
+        then a capture with a smaller list_prefix_div saves that work."""
+        return _redone(self.prefix_status)
+
     def check(self, grow: bool = False) -> bool:
         """True when every frame since the last check fitted the capacity.
         Otherwise raise CapacityError, or with grow=True re-capture with the
@@ -246,6 +274,7 @@ class CapturedTrainingStep:
         self.copts = self.opts.to_c(want_abs)
         self.P = P = int(means3D.shape[0])
         self.V = V = len(settings_list)
+        self.prefix_status = _prefix_lists(self.copts, self.opts, P, V, dev)
         self.headroom = float(headroom)
         # the Gaussian tensors are read in place by every replay
         for name, t, per in (("means3D", means3D, 3), ("opacities", opacities, 1),
@@ -418,6 +447,12 @@ class CapturedTrainingStep:
 
     def capacity(self) -> tuple[int, int]:
         return self.R_cap, self.NB_cap
+
+    def prefix_redone(self) -> bool:
+        """Whether some replay since the last call needed the whole lists
+        (its prefixes did not suffice; its outputs are exact either way):
+        then a capture with a smaller list_prefix_div saves that work."""
+        return _redone(self.prefix_status)
 
     def check(self, grow: bool = False) -> bool:
         """True when every replay since the last check fitted the capacity.

@@ -378,6 +378,18 @@ class Frame:
                                 max(old[1], self.NB + self.NB // 8 + 64))
         return self
 
+    def list_counts(self) -> dict:
+        """Pair counts of the frame's list build (waits for the frame):
+        ``pairs`` of the whole lists, ``prefix_pairs`` of the prefix lists
+        (0 when the frame was built with whole lists) and ``redo`` whether
+        the prefixes did not suffice and the whole lists were built too."""
+        v = FrameView()
+        check(lib().fgs_frame_view_get(_ptr(self.gws), None, self.P, self.W, self.H, 0, 0,
+                                       ctypes.byref(self.copts), ctypes.byref(v)),
+              "fgs_frame_view_get")
+        c = _tensor_at(v.counts, 16, torch.int32, self).cpu().numpy().view("uint32")
+        return {"pairs": int(c[0]), "prefix_pairs": int(c[7]), "redo": bool(c[8])}
+
     def complete_lists(self):
         """Whole per-tile lists for a frame rendered from prefix lists (the
         list passes and inspection views read whole lists): emission, sort,

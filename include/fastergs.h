@@ -143,10 +143,12 @@ typedef struct fgs_options {
                        is left open, the whole lists are built and every tile is composited
                        again, on the device (no host wait; launches that find nothing to do
                        return at once). The images, T_final, last contributors, checkpoints
-                       and gradients never depend on it; the frame's lists and ranges (views,
-                       fgs_render_simp, fgs_render_depth) are the prefix lists when they
-                       sufficed, so pass 0 to fgs_render again on the frame to rebuild whole
-                       lists before those. 0 = whole lists. */
+                       and gradients never depend on it. The frame's lists and ranges
+                       (fgs_frame_view_get) are the prefix lists when they sufficed: to rebuild
+                       whole lists, call fgs_render again on the frame with list_prefix 0 (the
+                       outputs keep their values). fgs_render_simp and fgs_render_depth, which
+                       read whole lists, return FGS_ERR_ARG for options with list_prefix > 0.
+                       0 = whole lists. */
   uint32_t* list_status; /* with list_prefix: device-accessible word (device memory, or mapped
                             pinned host memory read after the frame completes) that compositing
                             sets to 1 when the prefix lists did not suffice; never cleared by

@@ -144,3 +144,49 @@ def test_status_word_turns_prefixes_off():
     (pol,) = rasterizer._prefix_policies.values()
     assert pol.off == pol.RETRY - 2
     rasterizer._prefix_policies.clear()
+
+
+def test_captured_frames_with_prefix_lists():
+    """Graph-captured frames take a prefix size fixed at capture: replays
+    match the oracle whether the prefixes sufficed (opaque scene) or the
+    replay built the whole lists (sparse scene), which prefix_redone()
+    reports."""
+    from fastergs_amd.graph import CapturedRenderer
+    for (scene, cam), redone in ((_opaque(), False), (_sparse(), True)):
+        d = to_dev(scene)
+        rend = CapturedRenderer(settings_for(cam), d["means3D"], d["opacities"], d["scales"],
+                                d["rotations"], dc=d["dc"], shs=d["shs"],
+                                options=RasterOptions(bucket_size=32, list_prefix_div=16),
+                                headroom=1.5)
+        for _ in range(2):
+            img = rend.render().cpu().numpy()
+            orc = Oracle(scene, cam, bucket=32, mode=0).forward()
+            np.testing.assert_array_equal(img.view(np.uint32), orc.image().view(np.uint32))
+        assert rend.prefix_redone() is redone
+        assert rend.prefix_redone() is False
+
+
+def test_captured_training_step_with_prefix_lists():
+    """One-view captured training step on prefix lists against the eager
+    whole-list step: image bit for bit, gradients within 1e-4."""
+    from fastergs_amd.graph import CapturedTrainingStep
+    from fastergs_amd.multiview import BatchFrame
+    scene, cam = _opaque()
+    d = to_dev(scene)
+    step = CapturedTrainingStep([settings_for(cam)], d["means3D"], d["opacities"], d["scales"],
+                                d["rotations"], dc=d["dc"], shs=d["shs"],
+                                options=RasterOptions(bucket_size=32, list_prefix_div=16),
+                                headroom=1.5)
+    tg = torch.from_numpy(scenes.target_image(cam.width, cam.height, seed=4)).cuda()
+    step.targets()[0].copy_(tg)
+    g = {n: t.clone() for n, t in step.step().items()}
+    img = step.out_color.clone()
+    assert not step.prefix_redone()
+    fr = BatchFrame([settings_for(cam)], d["means3D"], d["opacities"], d["scales"],
+                    d["rotations"], dc=d["dc"], shs=d["shs"],
+                    options=RasterOptions(bucket_size=32, list_prefix_div=0)).forward()
+    n = 3 * cam.height * cam.width
+    ref = fr.backward((fr.out_color - tg.reshape(-1)) * (2.0 / n))
+    np.testing.assert_array_equal(img.cpu().numpy(), fr.out_color.cpu().numpy())
+    for k in ("opacities", "dc", "shs", "means3D", "scales", "rotations"):
+        grad_close(g[k].cpu().numpy(), ref[k].cpu().numpy(), name=k)
