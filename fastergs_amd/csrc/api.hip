@@ -683,8 +683,11 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   if (fast) {
     const ViewRec* views = (const ViewRec*)gw.views;
     const int64_t NR = (int64_t)g->P * b.V;
-    // prefix lists: the pairs of the list_prefix nearest records first
-    const bool prefix = b.V == 1 && !opt->tile_sched && opt->list_prefix > 0 &&
+    // prefix lists: the pairs of the list_prefix nearest records first. The
+    // depth order interleaves the views' records; restricted to one view it
+    // is that view's depth order, so a prefix of it is a prefix of every
+    // view's order.
+    const bool prefix = !opt->tile_sched && opt->list_prefix > 0 &&
                         (int64_t)opt->list_prefix < NR;
     const Fork f(s);
     // emission in depth order, tile-only stable sort, lists, ranges, buckets.

@@ -57,16 +57,15 @@ def _in_place(name: str, t: torch.Tensor, dev, per: int) -> torch.Tensor:
 
 
 def _prefix_lists(copts, opts: RasterOptions, P: int, V: int, dev):
-    """Prefix lists in a captured frame (FAST, one view): the lists of the
-    nearest ceil(P / list_prefix_div) Gaussians first, a size fixed at
+    """Prefix lists in a captured frame (FAST): the lists of the nearest
+    ceil(P V / list_prefix_div) records of the V views first, a size fixed at
     capture. A replay whose prefixes did not suffice builds the whole lists
     and composites again on the device (same outputs) and sets the returned
     device word to 1 (read and cleared by ``prefix_redone()``)."""
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     div = int(opts.list_prefix_div)
-    if (opts.schedule == _lib.SCHEDULE_FAST and V == 1 and opts.tile_sched is None and div > 1
-            and P > div):
-        copts.list_prefix = -(-P // div)
+    if opts.schedule == _lib.SCHEDULE_FAST and opts.tile_sched is None and div > 1 and P * V > div:
+        copts.list_prefix = -(-(P * V) // div)
         copts.list_status = status.data_ptr()
     return status
 

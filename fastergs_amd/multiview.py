@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from ._lib import Batch, Camera, FrameView, Gaussians, Grads, check, lib
-from .rasterizer import RasterOptions, _Grads, _f32, _ptr, _stream
+from .rasterizer import RasterOptions, _Grads, _PrefixPolicy, _f32, _prefix_policies, _ptr, _stream
 
 
 class BatchFrame:
@@ -105,6 +105,20 @@ class BatchFrame:
 
     def render(self):
         L, dev = lib(), self.device
+        # prefix lists of the batch's nearest records first, sized per batch
+        # shape by the drop-in API's policy (RasterOptions.list_prefix_div)
+        pol, div = None, int(self.opts.list_prefix_div)
+        NR = self.P * self.V
+        if div > 1 and self.opts.tile_sched is None:
+            key = (self.P, tuple(self.sizes), self.opts.bucket_size, self.opts.tile_w, dev, div)
+            pol = _prefix_policies.setdefault(key, _PrefixPolicy(div))
+            div = pol.divisor()
+            if div > 1 and NR > div:
+                status = pol.status_word()
+                self.copts.list_prefix = -(-NR // div)
+                self.copts.list_status = status.data_ptr()
+            else:
+                pol = None
         n = L.fgs_batch_pair_ws_size(self.P, ctypes.byref(self.batch), self.R, self.NB,
                                      ctypes.byref(self.copts))
         self.pws = torch.empty(n, dtype=torch.uint8, device=dev)
@@ -117,6 +131,24 @@ class BatchFrame:
                                  _ptr(self.pws), n, self.R, self.NB, _ptr(self.out_color),
                                  _ptr(self.T_final), _ptr(self.last_contributor), self._s()),
               "fgs_batch_render")
+        if pol is not None:
+            pol.track(status, div)
+            self.copts.list_status = None
+            self.prefix_lists = True
+        return self
+
+    def complete_lists(self):
+        """Whole per-tile lists for a batch rendered from prefix lists (the
+        inspection views read whole lists); the outputs keep their values."""
+        if not getattr(self, "prefix_lists", False):
+            return self
+        self.copts.list_prefix = 0
+        check(lib().fgs_batch_render(ctypes.byref(self.batch), ctypes.byref(self.gauss),
+                                     ctypes.byref(self.copts), _ptr(self.gws), self.gws.numel(),
+                                     _ptr(self.pws), self.pws.numel(), self.R, self.NB,
+                                     _ptr(self.out_color), _ptr(self.T_final),
+                                     _ptr(self.last_contributor), self._s()), "fgs_batch_render")
+        self.prefix_lists = False
         return self
 
     def forward(self):
@@ -164,6 +196,7 @@ class BatchFrame:
 
     def view_arrays(self) -> dict:
         """Copies of the batch's intermediate arrays (lists, ranges, buckets)."""
+        self.complete_lists()
         if getattr(self, "pws", None) is not None:
             # the bucket tables are built on demand (no pass of the step reads them)
             check(lib().fgs_batch_bucket_tables(_ptr(self.gws), _ptr(self.pws), self.P,

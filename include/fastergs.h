@@ -134,10 +134,11 @@ typedef struct fgs_options {
                        the FAST bucket boundaries (where the T < 1e-4 cut-off is tested) and
                        render_simp's per-tile stop, are defined on the tile; the PLAIN images
                        do not depend on it. T (tiles) = ceil(W / tile_w) * ceil(H / tile_h). */
-  int list_prefix;  /* FAST, one view, no tile_sched: > 0 renders the frame first from prefix
-                       lists, the per-tile lists of its list_prefix nearest records (depth ranks
-                       below list_prefix, < P): emission, the list sort and the ranges handle
-                       their pairs only. The lists are in depth order, so a prefix list is the
+  int list_prefix;  /* FAST, no tile_sched: > 0 renders the frame first from prefix lists, the
+                       per-tile lists of its list_prefix nearest records (depth ranks below
+                       list_prefix, < P x views; a batch's depth order interleaves its views'
+                       records and keeps each view's own order): emission, the list sort and
+                       the ranges handle their pairs only. The lists are in depth order, so a prefix list is the
                        front of the tile's whole list, and a pixel that stops at the end of a
                        whole bucket within it has its final values. When some pixel of some tile
                        is left open, the whole lists are built and every tile is composited

@@ -190,3 +190,42 @@ def test_captured_training_step_with_prefix_lists():
     np.testing.assert_array_equal(img.cpu().numpy(), fr.out_color.cpu().numpy())
     for k in ("opacities", "dc", "shs", "means3D", "scales", "rotations"):
         grad_close(g[k].cpu().numpy(), ref[k].cpu().numpy(), name=k)
+
+
+def test_batch_prefix_lists():
+    """A batch's depth order interleaves its views' records and keeps each
+    view's own order, so its prefix is a prefix of every view's order: the
+    batch rendered from prefix lists equals the whole-list batch (images,
+    T_final, contributors bit for bit, gradients within 1e-4). The prefix
+    is the batch's nearest records, not a share of each view's."""
+    from fastergs_amd.multiview import BatchFrame
+    scene, cam = _opaque()
+    d = to_dev(scene)
+    # views at one distance from the scene: the batch's nearest records
+    # cover every view (views at unequal distances may need the redo)
+    cams = [cam, scenes.look_at_camera((0.3, 0.2, -4.0), 480, 320),
+            scenes.look_at_camera((-0.2, 0.0, -4.0), 320, 240)]
+    sts = [settings_for(c) for c in cams]
+    rasterizer._prefix_policies.clear()
+
+    def batch(div):
+        return BatchFrame(sts, d["means3D"], d["opacities"], d["scales"], d["rotations"],
+                          dc=d["dc"], shs=d["shs"],
+                          options=RasterOptions(bucket_size=32, list_prefix_div=div)).forward()
+
+    a, b = batch(16), batch(0)
+    v = FrameView()
+    check(lib().fgs_batch_frame_view_get(ctypes.c_void_p(a.gws.data_ptr()), None, a.P,
+                                         ctypes.byref(a.batch), 0, 0, ctypes.byref(a.copts),
+                                         ctypes.byref(v)), "view")
+    off = v.counts - a.gws.data_ptr()
+    c = a.gws[off:off + 64].cpu().numpy().view(np.uint32)
+    assert 0 < c[COUNT_PREFIX_N] < c[0] and c[COUNT_REDO] == 0, (c[COUNT_PREFIX_N], c[0])
+    for x, y in ((a.out_color, b.out_color), (a.T_final, b.T_final),
+                 (a.last_contributor, b.last_contributor)):
+        assert torch.equal(x, y)
+    dimg = torch.randn_like(a.out_color) * 1e-3
+    ga, gb = a.backward(dimg), b.backward(dimg)
+    for k in ("opacities", "dc", "shs", "means3D", "scales", "rotations"):
+        grad_close(ga[k].cpu().numpy(), gb[k].cpu().numpy(), name=k)
+    rasterizer._prefix_policies.clear()
