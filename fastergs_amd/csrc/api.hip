@@ -885,8 +885,8 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   bw.V = b.V;
   if (fast) {
     bw.vals = pw.v0;
-    FGS_TRY(launch_active_counts(T, gw.maxc, opt->bucket_size, gw.bcount, s), "active buckets");
-    FGS_TRY(scan_u32(gw.bcount, gw.actoff, T, gw.actoff + T, gw.scratch, s), "active scan");
+    FGS_TRY(active_bucket_scan(gw.maxc, T, opt->bucket_size, gw.bcount, gw.actoff, gw.scratch, s),
+            "active buckets");
     FGS_TRY(f.join(0, 7), "join");
     FGS_TRY(launch_backward_fast(bw, s), "raster backward");
   } else {

@@ -221,6 +221,11 @@ hipError_t launch_tile_ranges(int64_t R, const uint32_t* n_dev, const void* tile
                               uint32_t* ranges, int T, int bucket, uint32_t* bcount,
                               uint32_t* maxc, uint32_t* boff, uint32_t* arrive, hipStream_t s,
                               const uint32_t* gate = nullptr);
+// Exclusive prefix of the backward's active buckets ceil(maxc[t] / bucket)
+// into out[0..T), the total into out[T] (one launch up to 8192 tiles, else
+// the counts into `counts` and a scan).
+hipError_t active_bucket_scan(const uint32_t* maxc, int T, int bucket, uint32_t* counts,
+                              uint32_t* out, void* ws, hipStream_t s);
 hipError_t launch_active_counts(int T, const uint32_t* maxc, int bucket, uint32_t* out,
                                 hipStream_t s);
 struct CompArgs {

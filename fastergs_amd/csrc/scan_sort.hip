@@ -126,6 +126,41 @@ __global__ __launch_bounds__(SCAN1_THREADS) void k_scan_single(const uint32_t* i
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
+// Active buckets of the backward in one launch (up to SCAN1_MAX tiles):
+// out[t] = exclusive prefix of ceil(maxc[t] / bucket), out[T] = the total.
+__global__ __launch_bounds__(SCAN1_THREADS) void k_active_scan_single(const uint32_t* maxc, int T,
+                                                                      int bucket,
+                                                                      uint32_t* out) {
+  const int b = threadIdx.x * SCAN1_ITEMS;
+  uint32_t v[SCAN1_ITEMS];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN1_ITEMS; ++k) {
+    v[k] = b + k < T ? (maxc[b + k] + (uint32_t)bucket - 1u) / (uint32_t)bucket : 0u;
+    sum += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan<SCAN1_THREADS>(sum, tot);
+#pragma unroll
+  for (int k = 0; k < SCAN1_ITEMS; ++k) {
+    if (b + k < T) out[b + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) out[T] = tot;
+}
+
+hipError_t active_bucket_scan(const uint32_t* maxc, int T, int bucket, uint32_t* counts,
+                              uint32_t* out, void* ws, hipStream_t s) {
+  if (T <= 0) return hipSuccess;
+  if (T <= SCAN1_MAX) {
+    FGS_TIMED("active_scan", s,
+              (k_active_scan_single<<<1, SCAN1_THREADS, 0, s>>>(maxc, T, bucket, out)));
+    return hipGetLastError();
+  }
+  hipError_t e = launch_active_counts(T, maxc, bucket, counts, s);
+  return e != hipSuccess ? e : scan_u32(counts, out, T, out + T, ws, s);
+}
+
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* ws,
                     hipStream_t s) {
   if (n <= SCAN1_MAX) {
