@@ -605,8 +605,14 @@ __global__ __launch_bounds__(256) void k_gauss_bwd_geom(GaussBwdArgs a) {
 // Jacobian). One wave64 per workgroup, one Gaussian per lane; each lane
 // writes its coefficient-gradient row into LDS and the rows leave with
 // coalesced float4 stores.
+// The coefficient-gradient rows go out through LDS 32 rows at a time (6 KB
+// per wave instead of 12 KB for all 64: the LDS a wave holds bounds the
+// waves per CU, and this pass is HBM-bound).
+constexpr int SHJ_ROWS = 32;
+constexpr int SHJ_QMAX = (3 * 15 * SHJ_ROWS / 4 + SHB_THREADS - 1) / SHB_THREADS;  // 6
+constexpr size_t SHJ_SHMEM = (size_t)SHJ_QMAX * SHB_THREADS * sizeof(float4);
 __global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
-  extern __shared__ float4 rows4[];  // SHB_THREADS x 3M floats (padded)
+  extern __shared__ float4 rows4[];  // SHJ_ROWS x 3M floats (padded)
   float* rows = (float*)rows4;
   const int lane = threadIdx.x;
   const int g0 = blockIdx.x * SHB_THREADS;
@@ -615,7 +621,8 @@ __global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
   const int RM = 3 * M;
   const bool sh = a.dsh != nullptr && M > 0;
   const bool colour_sh = a.G.colors == nullptr && a.G.dc != nullptr;
-  const int nb = min(SHB_THREADS, a.P - g0);
+  float dRGB[3] = {0.0f, 0.0f, 0.0f};  // the colour adjoint, kept for the SH rows
+  bool rowv = false;                   // this lane's SH row is non-zero
   if (i < a.P) {
     float raw[RAW_STRIDE];
     load_raw(a.raw_lin, a.raw_con, (size_t)i, raw);
@@ -631,9 +638,7 @@ __global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
       a.dcolor[3 * (size_t)i + 1] = g2.color[1];
       a.dcolor[3 * (size_t)i + 2] = g2.color[2];
     }
-    float* row = rows + RM * lane;
     float tm[3], tc[6], ts[3], tr[4], ddc[3] = {0.0f, 0.0f, 0.0f};
-    int written = 0;  // leading coefficient-gradient entries of the row
     const int radius = a.radii[i];
     if (radius > 0) {
       // densification statistics (backward.cu:892-918, as k_densify_stats)
@@ -649,26 +654,21 @@ __global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
         const float* pm = a.G.means + 3 * (size_t)i;
         const float ox = pm[0] - a.cam.campos[0], oy = pm[1] - a.cam.campos[1],
                     oz = pm[2] - a.cam.campos[2];
-        const float len = sqrt_(ox * ox + oy * oy + oz * oz);
         const float4* jr = (const float4*)(a.shjac + 12 * (size_t)i);
         const float4 j0 = jr[0], j1 = jr[1], j2 = jr[2];
         clamp = (int)f2u(j0.w);
-        float bas[16], dRGB[3], J[3][3], ddir[3];
-        sh_basis(ox / len, oy / len, oz / len, bas);
+        float J[3][3], ddir[3];
         sh_drgb(clamp, g2.color, dRGB);
         J[0][0] = j0.x; J[0][1] = j0.y; J[0][2] = j0.z;
         J[1][0] = j1.x; J[1][1] = j1.y; J[1][2] = j1.z;
         J[2][0] = j2.x; J[2][1] = j2.y; J[2][2] = j2.z;
         sh_ddir(J, dRGB, ddir);
+        // degree-0 basis: a constant
+        float bas[16];
+        const float len = sqrt_(ox * ox + oy * oy + oz * oz);
+        sh_basis(ox / len, oy / len, oz / len, bas);
         for (int ch = 0; ch < 3; ++ch) ddc[ch] = bas[0] * dRGB[ch];
-        if (sh) {
-          const int n = sh_count(a.cam.deg);
-#pragma unroll
-          for (int k = 1; k < 16; ++k)
-            if (k < n)
-              for (int ch = 0; ch < 3; ++ch) row[3 * (k - 1) + ch] = bas[k] * dRGB[ch];
-          written = 3 * (n - 1);
-        }
+        rowv = sh;
         sh_dmean(ox, oy, oz, ddir, dms);
       }
       GaussGrad<float> out;
@@ -684,8 +684,6 @@ __global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
       for (int k = 0; k < 6; ++k) tc[k] = 0.0f;
       for (int k = 0; k < 4; ++k) tr[k] = 0.0f;
     }
-    if (sh)
-      for (int k = written; k < RM; ++k) row[k] = 0.0f;
     for (int k = 0; k < 3; ++k) {
       a.dmean3D[3 * (size_t)i + k] = tm[k];
       a.dscale[3 * (size_t)i + k] = ts[k];
@@ -696,20 +694,45 @@ __global__ __launch_bounds__(SHB_THREADS) void k_gauss_bwd_jac(GaussBwdArgs a) {
     ((float4*)a.drot)[i] = make_float4(tr[0], tr[1], tr[2], tr[3]);
   }
   if (!sh) return;
-  __syncthreads();
-  float* dst = a.dsh + (size_t)RM * g0;
-  const int nq = RM * SHB_THREADS / 4;
-  const bool vec = M <= 15 && nb == SHB_THREADS && RM * SHB_THREADS % 4 == 0 &&
-                   ((uintptr_t)a.dsh & 15) == 0;
-  if (vec) {
-    float4 ov[SHB_QMAX];
+  // coefficient gradients bas_k * dRGB (k >= 1), the basis recomputed from
+  // the direction (the geometric adjoint's registers are free by now)
+  float bas[16];
+  int n = 0;
+  if (rowv) {
+    const float* pm = a.G.means + 3 * (size_t)i;
+    const float ox = pm[0] - a.cam.campos[0], oy = pm[1] - a.cam.campos[1],
+                oz = pm[2] - a.cam.campos[2];
+    const float len = sqrt_(ox * ox + oy * oy + oz * oz);
+    sh_basis(ox / len, oy / len, oz / len, bas);
+    n = sh_count(a.cam.deg);
+  }
+  const bool al16 = ((uintptr_t)a.dsh & 15) == 0 && M <= 15;
+  for (int h = 0; h < SHB_THREADS / SHJ_ROWS; ++h) {
+    const int gb = g0 + h * SHJ_ROWS;
+    if (gb >= a.P) break;  // uniform
+    if (h) __syncthreads();  // the previous rows are copied out
+    if (lane / SHJ_ROWS == h) {
+      float* row = rows + RM * (lane % SHJ_ROWS);
 #pragma unroll
-    for (int u = 0; u < SHB_QMAX; ++u) ov[u] = rows4[lane + u * SHB_THREADS];
+      for (int k = 1; k < 16; ++k)
 #pragma unroll
-    for (int u = 0; u < SHB_QMAX; ++u)
-      if (lane + u * SHB_THREADS < nq) ((float4*)dst)[lane + u * SHB_THREADS] = ov[u];
-  } else {
-    for (int j = lane; j < RM * nb; j += SHB_THREADS) dst[j] = rows[j];
+        for (int ch = 0; ch < 3; ++ch)
+          if (3 * (k - 1) + ch < RM) row[3 * (k - 1) + ch] = k < n ? bas[k] * dRGB[ch] : 0.0f;
+    }
+    __syncthreads();
+    const int nr = min(SHJ_ROWS, a.P - gb);
+    float* dst = a.dsh + (size_t)RM * gb;
+    if (al16 && nr == SHJ_ROWS) {
+      const int nq = RM * SHJ_ROWS / 4;
+      float4 ov[SHJ_QMAX];
+#pragma unroll
+      for (int u = 0; u < SHJ_QMAX; ++u) ov[u] = rows4[lane + u * SHB_THREADS];
+#pragma unroll
+      for (int u = 0; u < SHJ_QMAX; ++u)
+        if (lane + u * SHB_THREADS < nq) ((float4*)dst)[lane + u * SHB_THREADS] = ov[u];
+    } else {
+      for (int j = lane; j < RM * nr; j += SHB_THREADS) dst[j] = rows[j];
+    }
   }
 }
 
@@ -721,8 +744,7 @@ static hipError_t launch_gauss_adjoint(const GaussBwdArgs& a, hipStream_t s, boo
   const bool sh = a.dsh != nullptr && M > 0;
   const bool colour_sh = a.G.colors == nullptr && a.G.dc != nullptr;
   if (a.shjac != nullptr && (a.views == nullptr || a.V <= 1) && M <= 15) {
-    const size_t need = (size_t)SHB_THREADS * 3 * M * sizeof(float);
-    const size_t shmem = sh ? (need > SHB_SHMEM ? need : SHB_SHMEM) : 0;
+    const size_t shmem = sh ? SHJ_SHMEM : 0;
     FGS_TIMED("gauss_bwd_jac", s,
               (k_gauss_bwd_jac<<<(unsigned)((a.P + SHB_THREADS - 1) / SHB_THREADS), SHB_THREADS,
                                  shmem, s>>>(a)));
