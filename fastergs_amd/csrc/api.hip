@@ -715,17 +715,28 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
                                    pw.r0, pw.tile16, s),
                 "emit large");
         FGS_TRY(f.join(0, 5), "join");
-      } else {
-        // a prefix is short and the redo rarely runs: one stream (a
-        // cross-stream wait costs more than the overlap gains)
+      } else if (mode == EMIT_PREFIX) {
+        // the prefix's small and large rects side by side too
+        FGS_TRY(f.fork(0, 4), "fork");
         FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.dsort_k0,
-                                  gw.poff, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s, limit,
-                                  gw.counts, (EmitMode)mode),
+                                  gw.poff, (uint32_t)R, pw.t0, pw.r0, pw.tile16, f.side(0), limit,
+                                  gw.counts, EMIT_PREFIX),
                 "emit");
         FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts,
                                    nullptr, gw.poff, gw.lrank, gw.lband, (uint32_t)R, pw.t0,
-                                   pw.r0, pw.tile16, s,
-                                   redo ? 0xffffffffu : (uint32_t)limit, gate),
+                                   pw.r0, pw.tile16, s, (uint32_t)limit, nullptr),
+                "emit large");
+        FGS_TRY(f.join(0, 5), "join");
+      } else {
+        // the redo rarely runs: one stream (a cross-stream wait would cost
+        // every frame)
+        FGS_TRY(launch_emit_small(g->P, b.V, views, gw.geom, opt->split_tiles, gw.dsort_k0,
+                                  gw.poff, (uint32_t)R, pw.t0, pw.r0, pw.tile16, s, limit,
+                                  gw.counts, EMIT_REDO),
+                "emit");
+        FGS_TRY(launch_large_tiles(true, g->P, b.V, views, gw.geom, gw.large_ids, gw.counts,
+                                   nullptr, gw.poff, gw.lrank, gw.lband, (uint32_t)R, pw.t0,
+                                   pw.r0, pw.tile16, s, 0xffffffffu, gate),
                 "emit large");
       }
       // stable sort of (tile id, record) by tile id: the last pass writes the
