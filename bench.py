@@ -158,7 +158,7 @@ def main():
     prefix = None
     if sched == SCHEDULE_FAST and args.list_prefix_div > 1:
         from fastergs_amd import rasterizer as rz
-        key = (P, cam.width, cam.height, sched, args.bucket, dev, 16, args.list_prefix_div)
+        key = (cam.width, cam.height, args.bucket, 16, dev, args.list_prefix_div)
         pol = rz._prefix_policies.get(key)
         if pol is not None:
             prefix = {"start_div": args.list_prefix_div, "div": pol.good,

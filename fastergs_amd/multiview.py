@@ -110,7 +110,7 @@ class BatchFrame:
         pol, div = None, int(self.opts.list_prefix_div)
         NR = self.P * self.V
         if div > 1 and self.opts.tile_sched is None:
-            key = (self.P, tuple(self.sizes), self.opts.bucket_size, self.opts.tile_w, dev, div)
+            key = (tuple(self.sizes), self.opts.bucket_size, self.opts.tile_w, dev, div)
             pol = _prefix_policies.setdefault(key, _PrefixPolicy(div))
             div = pol.divisor()
             if div > 1 and NR > div:

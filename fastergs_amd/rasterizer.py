@@ -68,7 +68,7 @@ class RasterOptions:
     # list_prefix_div (0 or 1: never). Every pixel that stops at the end of a
     # whole bucket within them has its final values; if some pixel does not,
     # the whole lists are built and the frame composited again on the device.
-    # Outputs never depend on it. Per frame shape the divisor then follows
+    # Outputs never depend on it. Per image shape the divisor then follows
     # the outcomes (see _PrefixPolicy): larger while prefixes suffice, smaller
     # when they do not.
     list_prefix_div: int = 16
@@ -355,7 +355,11 @@ class Frame:
         R_hint, NB_hint = hint if hint is not None else (0, 0)
         pol, div = None, int(self.opts.list_prefix_div)
         if div > 1 and self.opts.schedule == _lib.SCHEDULE_FAST and self.opts.tile_sched is None:
-            pol = _prefix_policies.setdefault(key + (self.opts.tile_w, div), _PrefixPolicy(div))
+            # per image shape, not per Gaussian count: the prefix is a share
+            # of the Gaussians, and its size carries over when densification
+            # changes their number
+            pkey = (self.W, self.H, self.opts.bucket_size, self.opts.tile_w, self.device, div)
+            pol = _prefix_policies.setdefault(pkey, _PrefixPolicy(div))
             div = pol.divisor()
             if div > 1 and self.P > div:
                 self.list_prefix = -(-self.P // div)
