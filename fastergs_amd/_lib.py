@@ -57,7 +57,8 @@ class Options(ctypes.Structure):
                 ("split_tiles", ctypes.c_int), ("abs_grad", ctypes.c_int),
                 ("tile_sched", ctypes.POINTER(TileSched)), ("forward_only", ctypes.c_int),
                 ("composite_queue", ctypes.c_int), ("tile_w", ctypes.c_int),
-                ("list_prefix", ctypes.c_int), ("list_status", vp)]
+                ("list_prefix", ctypes.c_int), ("zero_grad_ws", vp), ("grad_ws_zeroed", ctypes.c_int),
+                ("list_status", vp)]
 
 
 class AdamTensor(ctypes.Structure):

@@ -680,6 +680,8 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
   c.redo = nullptr;
   c.status = nullptr;
   c.gate = nullptr;
+  c.zero = nullptr;
+  c.zero_n = 0;
   if (fast) {
     const ViewRec* views = (const ViewRec*)gw.views;
     const int64_t NR = (int64_t)g->P * b.V;
@@ -690,6 +692,12 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
     const bool prefix = !opt->tile_sched && opt->list_prefix > 0 &&
                         (int64_t)opt->list_prefix < NR;
     const Fork f(s);
+    // the backward's raw sums (all-zero bits) when the caller hands its
+    // gradient workspace over now: stored by compositing, whose time is VALU
+    // issue and which leaves HBM bandwidth free (a fill kernel beside the
+    // list build, joined back, cost the step as much as one in the backward)
+    c.zero = (float*)opt->zero_grad_ws;
+    c.zero_n = opt->zero_grad_ws ? NR * (2 * RAW_CON_STRIDE + RAW_LIN_STRIDE) : 0;
     // emission in depth order, tile-only stable sort, lists, ranges, buckets.
     // R is the pair capacity; the pair count itself (counts[0], or the
     // prefix's counts[COUNT_PREFIX_N]) stays on the device, so nothing here
@@ -814,6 +822,8 @@ int fgs_batch_render(const fgs_batch* batch, const fgs_gaussians* g, const fgs_o
       if ((rc = build_lists(EMIT_REDO))) return rc;
       c.redo = nullptr;
       c.status = nullptr;
+      c.zero = nullptr;  // done by the first compositing
+      c.zero_n = 0;
       c.gate = gw.counts + COUNT_REDO;
       FGS_TRY(launch_composite_fast(c, s), "composite again");
     }
@@ -865,9 +875,13 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
   double* raw_con = (double*)grad_ws;
   float* raw_lin = (float*)(raw_con + (size_t)PV * RAW_CON_STRIDE);
   const Fork f(s);
-  FGS_TRY(f.fork(0, 6), "fork");
-  FGS_TRY(fill_f32((float*)raw_con, 0.0f, PV * (2 * RAW_CON_STRIDE + RAW_LIN_STRIDE), f.side(0)),
-          "zero raw");
+  const bool zero = !opt->grad_ws_zeroed;  // else zeroed by the frame's preprocess
+  if (zero) {
+    FGS_TRY(f.fork(0, 6), "fork");
+    FGS_TRY(fill_f32((float*)raw_con, 0.0f, PV * (2 * RAW_CON_STRIDE + RAW_LIN_STRIDE),
+                     f.side(0)),
+            "zero raw");
+  }
   BwdArgs bw;
   bw.W = cam->width;
   bw.H = cam->height;
@@ -898,11 +912,11 @@ int fgs_batch_backward(const fgs_batch* batch, const fgs_gaussians* g, const fgs
     bw.vals = pw.v0;
     FGS_TRY(active_bucket_scan(gw.maxc, T, opt->bucket_size, gw.bcount, gw.actoff, gw.scratch, s),
             "active buckets");
-    FGS_TRY(f.join(0, 7), "join");
+    if (zero) FGS_TRY(f.join(0, 7), "join");
     FGS_TRY(launch_backward_fast(bw, s), "raster backward");
   } else {
     bw.vals = plain_sorted_in_alt(T) ? pw.v1 : pw.v0;
-    FGS_TRY(f.join(0, 7), "join");
+    if (zero) FGS_TRY(f.join(0, 7), "join");
     FGS_TRY(launch_backward_plain(bw, s), "raster backward");
   }
   GaussBwdArgs q;

@@ -269,6 +269,10 @@ struct CompArgs {
   uint32_t* status;
   // fast: when given, the launch does nothing unless gate[0] != 0
   const uint32_t* gate;
+  // fast: zero_n floats (a multiple of 4, 16-byte aligned) to zero, spread
+  // over the grid's threads (the backward's raw sums), or NULL
+  float* zero;
+  int64_t zero_n;
 };
 hipError_t launch_composite_plain(const CompArgs& a, hipStream_t s);
 hipError_t launch_composite_fast(const CompArgs& a, hipStream_t s);

@@ -454,6 +454,11 @@ template <bool QUEUE>
 __global__ __launch_bounds__(256) void k_composite_fast(CompArgs a) {
   __shared__ CompShared sh;
   if (a.gate && a.gate[0] == 0u) return;  // whole grid: the prefix lists sufficed
+  if (a.zero) {
+    const int64_t n4 = a.zero_n / 4, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n4; j += stride)
+      ((float4*)a.zero)[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
   if (!QUEUE) {
     const uint64_t t_start = a.cycles ? wall_clock64() : 0;
     uint32_t item;

@@ -15,6 +15,8 @@
 //   frame that needed more is rendered again with its exact counts (every
 //   kernel reads the true counts on the device, so outputs never depend on
 //   the capacity). Without a hint the counts are read first.
+//   For a FAST frame that will be differentiated it also returns the
+//   backward's gradient workspace, zeroed by the render's compositing.
 // ops.backward: the rasterizer adjoint and the per-Gaussian adjoint.
 // ops.densify_stats: the in-place densification statistics.
 #include <torch/extension.h>
@@ -126,10 +128,16 @@ Frame make_frame(const at::Tensor& means3D, const at::Tensor& opacities, const a
                       (const float*)ptr(f.scales), (const float*)ptr(f.rotations),
                       (const float*)ptr(f.dc), (const float*)ptr(f.shs),
                       (const float*)ptr(f.colors), (const uint8_t*)ptr(f.culling)};
-  f.opt = fgs_options{(int)opts[0],  (int)opts[1],  (int)opts[2],
-                      (int)opts[3],  nullptr,        (int)opts[4],
-                      (int)opts[14], (int)opts[15],  (int)opts[16],
-                      reinterpret_cast<uint32_t*>(static_cast<uintptr_t>(opts[17]))};
+  f.opt = fgs_options{};
+  f.opt.schedule = (int)opts[0];
+  f.opt.bucket_size = (int)opts[1];
+  f.opt.split_tiles = (int)opts[2];
+  f.opt.abs_grad = (int)opts[3];
+  f.opt.forward_only = (int)opts[4];
+  f.opt.composite_queue = (int)opts[14];
+  f.opt.tile_w = (int)opts[15];
+  f.opt.list_prefix = (int)opts[16];
+  f.opt.list_status = reinterpret_cast<uint32_t*>(static_cast<uintptr_t>(opts[17]));
   if (opts[5] >= 0) {
     auto addr = [](int64_t a) { return reinterpret_cast<void*>(static_cast<uintptr_t>(a)); };
     f.ts = fgs_tile_sched{(int)opts[5],
@@ -171,6 +179,13 @@ py::tuple forward(const at::Tensor& means3D, const at::Tensor& opacities, const 
   at::Tensor gws = at::empty({(int64_t)gn}, bytes(f.dev));
   check(fgs_preprocess(&f.cam, &f.g, &f.opt, gws.data_ptr(), gn, radii.data_ptr<int32_t>(), s),
         "fgs_preprocess");
+  // a FAST frame that will be differentiated: its backward's gradient
+  // workspace, zeroed by the render's compositing
+  at::Tensor grad_ws;
+  if (!f.opt.forward_only && f.opt.schedule == FGS_SCHEDULE_FAST && P > 0) {
+    grad_ws = at::empty({(int64_t)fgs_grad_ws_size(P)}, bytes(f.dev));
+    f.opt.zero_grad_ws = grad_ws.data_ptr();
+  }
   at::Tensor out = at::empty({3, H, W}, floats(f.dev));
   at::Tensor tfinal = at::empty({H, W, 1}, floats(f.dev));
   at::Tensor contrib = at::empty({H, W, 1}, ints(f.dev));
@@ -205,11 +220,13 @@ py::tuple forward(const at::Tensor& means3D, const at::Tensor& opacities, const 
     render();
   }
   return py::make_tuple(radii, gws, pws, out, tfinal, contrib,
-                        std::vector<int64_t>{cnt[0], cnt[1], cap[0], cap[1]});
+                        std::vector<int64_t>{cnt[0], cnt[1], cap[0], cap[1]}, grad_ws);
 }
 
 // Returns (dmean2D, dconic, dopacity, dcolor, dmean3D, dcov3D, ddc, dsh,
 // dscale, drot, grad_ws); dconic, dcolor, dcov3D are undefined unless wanted.
+// zeroed_grad_ws: the gradient workspace ops.forward returned (zeroed by the
+// frame's render), passed to one backward of the frame, or empty.
 // stats: (xyz_gradient_accum, xyz_gradient_accum_abs, max_radii, denom),
 // each updated in place when defined and non-empty (else left alone).
 py::tuple backward(const at::Tensor& means3D, const at::Tensor& opacities, const at::Tensor& scales,
@@ -222,7 +239,8 @@ py::tuple backward(const at::Tensor& means3D, const at::Tensor& opacities, const
                    const at::Tensor& pws, int64_t R_cap, int64_t NB_cap, const at::Tensor& radii,
                    const at::Tensor& tfinal, const at::Tensor& contrib,
                    const at::Tensor& grad_out, bool want_conic, bool want_colors,
-                   bool want_cov3D, const std::vector<at::Tensor>& stats) {
+                   bool want_cov3D, const std::vector<at::Tensor>& stats,
+                   const at::Tensor& zeroed_grad_ws) {
   Frame f = make_frame(means3D, opacities, scales, rotations, dc, shs, colors, culling, view, proj,
                        campos, bg, width, height, tanfovx, tanfovy, scale_modifier, mult,
                        sh_degree, opts, mlp);
@@ -237,7 +255,18 @@ py::tuple backward(const at::Tensor& means3D, const at::Tensor& opacities, const
   at::Tensor dconic = want_conic ? e({P, 2, 2}) : at::Tensor();
   at::Tensor dcolor = want_colors ? e({P, 3}) : at::Tensor();
   at::Tensor dcov = want_cov3D ? e({P, 6}) : at::Tensor();
-  at::Tensor grad_ws = at::empty({(int64_t)fgs_grad_ws_size((int)P)}, bytes(f.dev));
+  // the forward's gradient workspace, zeroed by its preprocess and not used
+  // since (the caller passes it once), else a new one the backward zeroes
+  const size_t gw_bytes = fgs_grad_ws_size((int)P);
+  at::Tensor grad_ws;
+  if (zeroed_grad_ws.defined() && zeroed_grad_ws.is_cuda() &&
+      zeroed_grad_ws.device() == f.dev && zeroed_grad_ws.is_contiguous() &&
+      (size_t)zeroed_grad_ws.numel() * zeroed_grad_ws.element_size() >= gw_bytes) {
+    grad_ws = zeroed_grad_ws;
+    f.opt.grad_ws_zeroed = 1;
+  } else {
+    grad_ws = at::empty({(int64_t)gw_bytes}, bytes(f.dev));
+  }
   if (stats.size() != 4) throw std::invalid_argument("stats must have 4 entries");
   float* st[4];
   for (int k = 0; k < 4; ++k) {

@@ -335,6 +335,10 @@ class CapturedTrainingStep:
                       "shs": e(P, self.M, 3), "scales": e(P, 3), "rotations": e(P, 4)}
         self.cgrads = Grads(*[_ptr(self.grads[k]) for k in self.GRAD_NAMES])
         self.grad_ws = torch.empty(L.fgs_batch_grad_ws_size(P, V), dtype=torch.uint8, device=dev)
+        # every replay's compositing zeroes the backward's raw sums; the
+        # backward then skips its own zeroing
+        self.copts.zero_grad_ws = self.grad_ws.data_ptr()
+        self.copts.grad_ws_zeroed = 1
         v = FrameView()
         check(L.fgs_batch_frame_view_get(_ptr(self.gws), None, P, ctypes.byref(self.batch), 0, 0,
                                          ctypes.byref(self.copts), ctypes.byref(v)),

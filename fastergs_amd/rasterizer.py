@@ -367,7 +367,7 @@ class Frame:
             else:
                 pol = None
         (self.radii, self.gws, self.pws, self.out_color, self.T_final, self.last_contributor,
-         c) = _C.ops.forward(*self._ext_args(lists=True), int(R_hint), int(NB_hint))
+         c, self.grad_ws) = _C.ops.forward(*self._ext_args(lists=True), int(R_hint), int(NB_hint))
         if pol is not None:
             pol.track(self.list_status, div)
             self.list_status = None  # later calls on the frame do not report
@@ -422,10 +422,13 @@ class Frame:
             raise ValueError("only conic, colors and cov3D gradients can be skipped")
         if self.forward_only:
             raise ValueError("the frame was rendered forward-only")
+        # the workspace the forward's preprocess zeroed serves one backward
+        zeroed, self.grad_ws = getattr(self, "grad_ws", None), None
         r = _C.ops.backward(*self._ext_args(), self.gws, self.pws, self.R_cap, self.NB_cap,
                             self.radii, self.T_final, self.last_contributor, grad_out_color,
                             "conic" not in skip, "colors" not in skip, "cov3D" not in skip,
-                            [_NONE if t is None else t for t in (stats or (None,) * 4)])
+                            [_NONE if t is None else t for t in (stats or (None,) * 4)],
+                            _NONE if zeroed is None else zeroed)
         names = ("mean2D", "conic", "opacities", "colors", "means3D", "cov3D", "dc", "shs",
                  "scales", "rotations")
         out = {k: t for k, t in zip(names, r[:10]) if k not in skip}

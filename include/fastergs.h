@@ -150,6 +150,15 @@ typedef struct fgs_options {
                        outputs keep their values). fgs_render_simp and fgs_render_depth, which
                        read whole lists, return FGS_ERR_ARG for options with list_prefix > 0.
                        0 = whole lists. */
+  void* zero_grad_ws; /* fgs_render (FAST): when non-NULL, the raw sums of the gradient
+                         workspace the frame's fgs_backward will take
+                         (fgs_batch_grad_ws_size(P, views) bytes) are zeroed by compositing,
+                         whose time is VALU issue, instead of by a fill at the start of the
+                         backward. No counterpart in the reference (which clears its
+                         gradient buffers in the backward). */
+  int grad_ws_zeroed; /* fgs_backward: the grad_ws it takes was zeroed by this frame's
+                         fgs_render (zero_grad_ws) and not used since; it then skips its own
+                         zeroing */
   uint32_t* list_status; /* with list_prefix: device-accessible word (device memory, or mapped
                             pinned host memory read after the frame completes) that compositing
                             sets to 1 when the prefix lists did not suffice; never cleared by
