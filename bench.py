@@ -865,10 +865,14 @@ def learned_schedule_eval(args, views=200, held_out=40, n=None, check_outputs=Tr
             viewmatrix=torch.from_numpy(c.viewmatrix).to(dev),
             projmatrix=torch.from_numpy(c.projmatrix).to(dev), sh_degree=3,
             campos=torch.from_numpy(c.campos).to(dev), mult=1.0, prefiltered=False, debug=False)
+        # forward-only frames (no gradient workspace for compositing to zero)
+        # with whole lists for every variant: the tile schedules work on them
+        # (a frame with a tile schedule never builds prefix lists), so the
+        # raster order must too
         return Frame(st, p["means3D"], p["opacities"], p["scales"], p["rotations"], dc=p["dc"],
-                     shs=p["shs"], options=RasterOptions(schedule=SCHEDULE_FAST,
-                                                         bucket_size=args.bucket,
-                                                         tile_sched=sched)).forward()
+                     shs=p["shs"], forward_only=True,
+                     options=RasterOptions(schedule=SCHEDULE_FAST, bucket_size=args.bucket,
+                                           tile_sched=sched, list_prefix_div=0)).forward()
 
     sched_kernels = ("sched_items", "tile_features", "mlp_infer", "tile_predict")
 
