@@ -369,3 +369,38 @@ def test_forward_only_frames(schedule):
         rz.Frame = orig
     assert seen == [True, False]
     assert torch.equal(img, full.out_color) and torch.equal(img2.detach(), full.out_color)
+
+
+def test_compositing_zeroes_the_gradient_workspace():
+    """fgs_options.zero_grad_ws: the render's compositing zeroes the raw sums
+    of a gradient workspace (here filled with NaN bytes first) and the
+    backward that takes it with grad_ws_zeroed skips its own zeroing; the
+    gradients equal those of a backward that zeroes its own workspace."""
+    import ctypes
+    from fastergs_amd._lib import lib
+    scene, cam = scenes.uniform_scene(20000, seed=21), scenes.look_at_camera((0.2, 0.0, -4.0), 320, 240)
+    d = to_dev(scene)
+
+    def frame():
+        fr = Frame(settings_for(cam), d["means3D"], d["opacities"], d["scales"], d["rotations"],
+                   dc=d["dc"], shs=d["shs"], options=RasterOptions(bucket_size=32))
+        fr.preprocess(sync=True)
+        return fr
+
+    dimg = torch.from_numpy(scenes.target_image(cam.width, cam.height, seed=3)).cuda()
+    ref = frame().render()
+    ref.grad_ws = None  # the backward zeroes a workspace of its own
+    g_ref = ref.backward(dimg)
+    fr = frame()
+    n = lib().fgs_grad_ws_size(fr.P)
+    ws = torch.full((n,), 0xFF, dtype=torch.uint8, device="cuda")
+    fr.copts.zero_grad_ws = ctypes.c_void_p(ws.data_ptr())
+    fr.render()
+    fr.copts.zero_grad_ws = None
+    torch.cuda.synchronize()
+    raw_bytes = fr.P * 64  # conic rows (4 doubles) + the other fields (8 floats)
+    assert int(torch.count_nonzero(ws[:raw_bytes])) == 0
+    fr.grad_ws = ws
+    g = fr.backward(dimg)
+    for k in ("opacities", "dc", "shs", "means3D", "scales", "rotations"):
+        grad_close(g[k].cpu().numpy(), g_ref[k].cpu().numpy(), name=k)
