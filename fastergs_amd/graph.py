@@ -59,7 +59,7 @@ def _in_place(name: str, t: torch.Tensor, dev, per: int) -> torch.Tensor:
 def _prefix_lists(copts, opts: RasterOptions, P: int, V: int, dev):
     """Prefix lists in a captured frame (FAST): the lists of the nearest
     ceil(P V / list_prefix_div) records of the V views first, a size fixed at
-    capture. A replay whose prefixes did not suffice builds the whole lists
+    capture (whole lists when the capture's warm-up frame needed them). A replay whose prefixes did not suffice builds the whole lists
     and composites again on the device (same outputs) and sets the returned
     device word to 1 (read and cleared by ``prefix_redone()``)."""
     status = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -76,7 +76,16 @@ def _redone(status: torch.Tensor) -> bool:
     return r
 
 
-class CapturedRenderer:
+class _PrefixCapture:
+    def _prefix_check(self):
+        """After the eager warm-up frame of a capture: a frame whose prefix
+        lists did not suffice is captured with whole lists instead (every
+        replay would otherwise build both)."""
+        if self.copts.list_prefix and _redone(self.prefix_status):
+            self.copts.list_prefix = 0
+
+
+class CapturedRenderer(_PrefixCapture):
     """Forward renderer whose whole frame is one graph replay.
 
     ``settings`` fixes the image size, field of view, background, SH degree
@@ -192,6 +201,7 @@ class CapturedRenderer:
         with torch.cuda.stream(side):
             self._frame()
         torch.cuda.current_stream(self.device).wait_stream(side)
+        self._prefix_check()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._frame()
@@ -232,7 +242,7 @@ class CapturedRenderer:
         return False
 
 
-class CapturedTrainingStep:
+class CapturedTrainingStep(_PrefixCapture):
     """One training step of the rasterizer captured into one HIP graph.
 
     The step renders a batch of views (one view is a batch of one; uneven
@@ -430,6 +440,7 @@ class CapturedTrainingStep:
         with torch.cuda.stream(side):
             self._step()
         torch.cuda.current_stream(self.device).wait_stream(side)
+        self._prefix_check()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._step()

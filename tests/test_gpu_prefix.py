@@ -147,23 +147,37 @@ def test_status_word_turns_prefixes_off():
 
 
 def test_captured_frames_with_prefix_lists():
-    """Graph-captured frames take a prefix size fixed at capture: replays
-    match the oracle whether the prefixes sufficed (opaque scene) or the
-    replay built the whole lists (sparse scene), which prefix_redone()
-    reports."""
+    """Graph-captured frames take a prefix size fixed at capture (whole lists
+    when the capture's warm-up frame needed them): replays match the oracle
+    whether the prefixes sufficed or a replay for another camera built the
+    whole lists, which prefix_redone() reports."""
     from fastergs_amd.graph import CapturedRenderer
-    for (scene, cam), redone in ((_opaque(), False), (_sparse(), True)):
+
+    def renderer(scene, cam):
         d = to_dev(scene)
-        rend = CapturedRenderer(settings_for(cam), d["means3D"], d["opacities"], d["scales"],
+        return CapturedRenderer(settings_for(cam), d["means3D"], d["opacities"], d["scales"],
                                 d["rotations"], dc=d["dc"], shs=d["shs"],
                                 options=RasterOptions(bucket_size=32, list_prefix_div=16),
                                 headroom=1.5)
-        for _ in range(2):
-            img = rend.render().cpu().numpy()
-            orc = Oracle(scene, cam, bucket=32, mode=0).forward()
-            np.testing.assert_array_equal(img.view(np.uint32), orc.image().view(np.uint32))
+
+    def cam_tensors(c):
+        return (torch.from_numpy(c.viewmatrix).cuda(), torch.from_numpy(c.projmatrix).cuda(),
+                torch.from_numpy(c.campos).cuda())
+
+    (scene, cam), (sparse, scam) = _opaque(), _sparse()
+    rend = renderer(scene, cam)
+    assert rend.copts.list_prefix > 0
+    far = scenes.look_at_camera((0.0, 0.0, -12.0), cam.width, cam.height)
+    for c, redone in ((cam, False), (far, True), (cam, False)):
+        img = rend.render(*cam_tensors(c)).cpu().numpy()
+        orc = Oracle(scene, c, bucket=32, mode=0).forward()
+        np.testing.assert_array_equal(img.view(np.uint32), orc.image().view(np.uint32))
         assert rend.prefix_redone() is redone
-        assert rend.prefix_redone() is False
+    rend = renderer(sparse, scam)
+    assert rend.copts.list_prefix == 0  # the warm-up frame needed the whole lists
+    img = rend.render().cpu().numpy()
+    orc = Oracle(sparse, scam, bucket=32, mode=0).forward()
+    np.testing.assert_array_equal(img.view(np.uint32), orc.image().view(np.uint32))
 
 
 def test_captured_training_step_with_prefix_lists():
