@@ -622,7 +622,27 @@ def render_loop(args, frames=120):
     camt = torch.stack([torch.from_numpy(np.concatenate(
         [c.viewmatrix.reshape(16), c.projmatrix.reshape(16), c.campos.reshape(3)]))
         for c in cams]).to(dev)
-    opts = RasterOptions(schedule=SCHEDULE_FAST, bucket_size=args.bucket)
+    # the captured frame's prefix size: the largest divisor (shortest
+    # prefix) for which no frame of the orbit needs the whole lists, found
+    # on an untimed pass over the orbit per divisor
+    best = None
+    for div in (16, 32, 64, 128):
+        opts = RasterOptions(schedule=SCHEDULE_FAST, bucket_size=args.bucket, list_prefix_div=div)
+        rend = CapturedRenderer(settings_of(cams[0]), p["means3D"], p["opacities"],
+                                p["scales"], p["rotations"], dc=p["dc"], shs=p["shs"],
+                                options=opts)
+        for _ in range(4):  # size the capacity for the whole orbit
+            _time_replays(rend, camt, frames)
+            if rend.check(grow=True):
+                break
+        ok = not rend.prefix_redone() and rend.copts.list_prefix > 0
+        del rend
+        torch.cuda.empty_cache()
+        if not ok:
+            break
+        best = div
+    opts = RasterOptions(schedule=SCHEDULE_FAST, bucket_size=args.bucket,
+                         list_prefix_div=best or 0)
     rend = CapturedRenderer(settings_of(cams[0]), p["means3D"], p["opacities"], p["scales"],
                             p["rotations"], dc=p["dc"], shs=p["shs"], options=opts)
     for _ in range(4):  # size the capacity for the whole orbit
@@ -670,7 +690,7 @@ def render_loop(args, frames=120):
                         f"{frames}-frame orbit r=3.5, one HIP graph per frame",
             "fps": round(frames * 1000.0 / ms, 2), "ms_per_frame": round(ms / frames, 4),
             "pairs_sampled": Rs, "pairs_sorted": sorted_pairs, "pair_capacity": R_cap,
-            "bucket_capacity": NB_cap, "prefix_redone": redone,
+            "bucket_capacity": NB_cap, "prefix_div": best, "prefix_redone": redone,
             "sort_gbs": round(sort_bytes / (sort_ms / 1e3) / 1e9, 1),
             "sort_frac_hbm": round(sort_bytes / (sort_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
 
