@@ -464,6 +464,12 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
     const uint32_t id = large_ids[q];
     const uint32_t g = id / LARGE_BANDS;  // record index v * P + i
     const int band = (int)(id % LARGE_BANDS);
+    // a band past the prefix leaves before its record's geometry is read
+    uint32_t rank = 0u;
+    if (EMIT) {
+      rank = lrank[g];
+      if (rank >= rank_limit) continue;  // wave-uniform
+    }
     const GeomRow r = load_geom(geom, g);
     const int wr = r.r2 - r.r0;
     int rows, nbands;
@@ -476,8 +482,6 @@ __global__ __launch_bounds__(256) void k_large_tiles(int P, int V, const ViewRec
     const uint32_t toff = views[v].tile_off;
     uint32_t base = 0u;
     if (EMIT) {
-      const uint32_t rank = lrank[g];
-      if (rank >= rank_limit) continue;  // past the prefix (wave-uniform)
       base = poff[rank];
       for (int k = 1; k <= band; ++k) base += lband[q - k];
     }
